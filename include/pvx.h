@@ -1,0 +1,267 @@
+/*
+ * pvx.h — C-ABI of the MI355X ray-intersection engine for POV-Ray's hot path.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int status codes (0 = ok),
+ * no exceptions across the boundary, explicit stream argument (a hipStream_t
+ * passed as void*; NULL = the default stream).  Ray/hit/point buffers passed
+ * to the trace entry points live in DEVICE memory and are owned by the caller.
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the POV-Ray 3.8 source tree).
+ */
+#ifndef PVX_H
+#define PVX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+    PVX_OK = 0,
+    PVX_EINVAL = 1,       /* bad argument (null pointer, bad index, bad size)   */
+    PVX_ENOMEM = 2,       /* host or device allocation failed                   */
+    PVX_EDEVICE = 3,      /* a HIP call failed                                  */
+    PVX_EUNSUPPORTED = 4, /* feature not available in this build                */
+    PVX_ESTATE = 5        /* handle in the wrong state (e.g. no scene uploaded) */
+};
+
+/* Opaque scene handle: device-resident geometry + acceleration structure. */
+typedef struct pvx_scene pvx_scene;
+
+/* ---- object model ------------------------------------------------------ */
+/* Shape kinds: the subset of core/shape on the hot path
+ * (sphere.cpp, box.cpp, torus.cpp, plane.cpp, mesh.cpp, csg.cpp). */
+enum {
+    PVX_SPHERE = 1,
+    PVX_BOX = 2,
+    PVX_TORUS = 3,
+    PVX_PLANE = 4,
+    PVX_MESH = 5,
+    PVX_CSG = 6
+};
+
+/* Object flags: the bits of ObjectBase::Flags that the hot path reads
+ * (core/scene/object.h: NO_SHADOW_FLAG, NO_IMAGE_FLAG, NO_REFLECTION_FLAG,
+ * NO_RADIOSITY_FLAG, INVERTED_FLAG). */
+enum {
+    PVX_FLAG_NO_SHADOW = 1u << 0,
+    PVX_FLAG_NO_IMAGE = 1u << 1,
+    PVX_FLAG_NO_REFLECTION = 1u << 2,
+    PVX_FLAG_NO_RADIOSITY = 1u << 3,
+    PVX_FLAG_INVERTED = 1u << 4
+};
+
+/* Ray kinds: the Ray::IsImageRay/IsReflectionRay/IsRadiosityRay tests read
+ * by NoSomethingFlagRayObjectCondition (core/render/trace.cpp:84-95), plus
+ * SHADOW for the NoShadowFlagRayObjectCondition of shadow rays. */
+enum {
+    PVX_RAY_PRIMARY = 0,
+    PVX_RAY_IMAGE = 1,
+    PVX_RAY_REFLECTION = 2,
+    PVX_RAY_RADIOSITY = 4,
+    PVX_RAY_SHADOW = 8
+};
+
+/* CSG operators (core/shape/csg.h).  A difference is expressed as an
+ * intersection whose children 2..n carry PVX_FLAG_INVERTED, as the parser
+ * builds it (parser/parser.cpp:2223-2225). */
+enum {
+    PVX_CSG_UNION = 1,
+    PVX_CSG_INTERSECTION = 2,
+    PVX_CSG_MERGE = 3
+};
+
+/* One entry of the object table: shape kind + index into that kind's array.
+ * Top-level objects are listed in scene order (SceneData::objects); the
+ * position in the table is the object id reported in hits. */
+typedef struct pvx_object {
+    uint32_t type;  /* PVX_SPHERE ... PVX_CSG                     */
+    uint32_t index; /* index into the per-kind arrays below       */
+    uint32_t flags; /* PVX_FLAG_* bits                            */
+    uint32_t pad;
+} pvx_object;
+
+/* CSG node: operator + a run of children in pvx_scene_desc.csg_children.
+ * A child of type PVX_CSG refers to another node (nested CSG). */
+typedef struct pvx_csg_node {
+    uint32_t op;          /* PVX_CSG_*                          */
+    uint32_t first_child; /* index into csg_children           */
+    uint32_t n_children;
+    uint32_t pad;
+} pvx_csg_node;
+
+/* mesh2 data as Parse_Mesh2 receives it (parser/parser.cpp:4048-4600):
+ * FP32 vertex_vectors, face_indices, optional per-vertex normal_vectors
+ * (normal indices = face indices, the "number_of_normals == number_of_vertices"
+ * rule at parser.cpp:4494-4509).  Vertices are in world space. */
+typedef struct pvx_mesh_desc {
+    uint32_t n_vertices;
+    uint32_t n_triangles;
+    uint32_t n_normals;      /* 0 or n_vertices                          */
+    uint32_t pad;
+    const float *vertices;   /* 3 * n_vertices                           */
+    const int32_t *indices;  /* 3 * n_triangles                          */
+    const float *normals;    /* 3 * n_normals, or NULL                   */
+} pvx_mesh_desc;
+
+/* Scene description.  All arrays are HOST memory and are copied.
+ * Transforms follow POV-Ray's TRANSFORM (core/math/matrix.h): two 4x4 FP64
+ * matrices, `matrix` then `inverse`, row-vector convention
+ * (p' = p.x*m[0][i] + p.y*m[1][i] + p.z*m[2][i] + m[3][i]); 32 doubles each. */
+typedef struct pvx_scene_desc {
+    uint32_t n_objects;
+    const pvx_object *objects;
+
+    uint32_t n_spheres;
+    const double *sphere_center; /* 3n */
+    const double *sphere_radius; /* n  */
+
+    uint32_t n_planes;
+    const double *plane_normal;   /* 3n, unit length                        */
+    const double *plane_distance; /* n: N.P + D = 0 (Plane::Distance)       */
+
+    uint32_t n_boxes;
+    const double *box_corner1;     /* 3n (Corner1 <= Corner2 per axis)       */
+    const double *box_corner2;     /* 3n                                     */
+    const int32_t *box_transform;  /* n, index into transforms or -1         */
+
+    uint32_t n_tori;
+    const double *torus_major;     /* n (MajorRadius)                        */
+    const double *torus_minor;     /* n (MinorRadius)                        */
+    const int32_t *torus_transform;/* n, index into transforms or -1         */
+    const uint8_t *torus_sturm;    /* n, 1 = `sturm` keyword (STURM_FLAG)    */
+
+    uint32_t n_transforms;
+    const double *transforms;      /* 32 * n_transforms                      */
+
+    uint32_t n_meshes;
+    const pvx_mesh_desc *meshes;
+
+    uint32_t n_csg;
+    const pvx_csg_node *csg;
+    uint32_t n_csg_children;
+    const pvx_object *csg_children;
+} pvx_scene_desc;
+
+/* ---- rays and hits ----------------------------------------------------- */
+/* FP64 ray as core/render/ray.h BasicRay (Origin, Direction); 48 bytes. */
+typedef struct pvx_ray {
+    double o[3];
+    double d[3];
+} pvx_ray;
+
+/* Closest hit: Intersection::Depth, object id, primitive id; 16 bytes.
+ * obj = -1 on a miss (t is then the ray's max depth, BOUND_HUGE = 2e10 if
+ * none: the default Intersection::Depth, core/coretypes.h:384-387).
+ * prim = triangle index for meshes (face order of the mesh2), the leaf's
+ * position in csg_children for CSG hits, 0 otherwise. */
+typedef struct pvx_hit {
+    double t;
+    int32_t obj;
+    int32_t prim;
+} pvx_hit;
+
+typedef struct pvx_build_opts {
+    uint32_t max_leaf_size; /* triangles per BVH leaf (0 = default 4) */
+    uint32_t sah_bins;      /* binned-SAH bins (0 = default 16)        */
+    uint32_t reserved[6];
+} pvx_build_opts;
+
+typedef struct pvx_scene_stats {
+    uint64_t n_nodes;        /* BVH nodes                               */
+    uint64_t n_prims;        /* BVH primitive references                */
+    uint64_t node_bytes;     /* bytes of one BVH node record            */
+    uint64_t tri_bytes;      /* bytes of one triangle record            */
+    uint64_t device_bytes;   /* total device bytes held by the scene    */
+    uint32_t bvh_depth;      /* max BVH depth                           */
+    uint32_t n_infinite;     /* objects tested outside the BVH (planes) */
+    double build_ms;         /* host BVH build time                     */
+} pvx_scene_stats;
+
+/* ---- library ----------------------------------------------------------- */
+/* Last error message of the calling thread (empty if none). */
+const char *pvx_last_error(void);
+
+/* Device count / device properties check; returns PVX_EDEVICE without a GPU. */
+int pvx_init(int device);
+
+/* ---- scene ------------------------------------------------------------- */
+/* Upload a scene and build its acceleration structure.
+ * Replaces parse-time Compute_Mesh_Triangle (core/shape/mesh.cpp:838-945)
+ * and BoundingTask::Run -> Build_Bounding_Slabs / Build_Mesh_BBox_Tree
+ * (backend/bounding/boundingtask.cpp:168-210,
+ *  core/bounding/boundingbox.cpp:325-430, core/shape/mesh.cpp:1376-1409). */
+int pvx_scene_create(const pvx_scene_desc *desc, const pvx_build_opts *opts,
+                     pvx_scene **out);
+int pvx_scene_destroy(pvx_scene *scene);
+int pvx_scene_get_stats(const pvx_scene *scene, pvx_scene_stats *out);
+
+/* ---- hot path ---------------------------------------------------------- */
+/* Closest hit for n rays (device buffers).
+ * Replaces bool Trace::FindIntersection(Intersection&, const Ray&,
+ *   const RayObjectCondition& pre, const RayObjectCondition& post)
+ *   (core/render/trace.h:255-258, trace.cpp:285-344) with
+ *   pre = NoSomethingFlagRayObjectCondition for `ray_kind`
+ *   (NoShadowFlag for PVX_RAY_SHADOW), post = TrueRayObjectCondition, and
+ *   bestisect.Depth seeded from max_depth (trace.cpp:157-158); max_depth may
+ *   be NULL (HUGE_VAL for every ray). */
+int pvx_trace_closest(pvx_scene *scene, const pvx_ray *rays,
+                      const double *max_depth, uint32_t ray_kind, size_t n,
+                      pvx_hit *hits, void *stream);
+
+/* pvx_trace_closest plus per-ray work counters: node_visits[i] = BVH nodes
+ * fetched, prim_tests[i] = primitive records tested.  These are the device
+ * counterparts of the reference's RenderStatistics counters nChecked /
+ * Ray_<Shape>_Tests (core/support/statisticids.h:120-268); the bench turns
+ * them into algorithmic bytes per ray. */
+int pvx_trace_closest_counted(pvx_scene *scene, const pvx_ray *rays,
+                              const double *max_depth, uint32_t ray_kind, size_t n,
+                              pvx_hit *hits, uint32_t *node_visits,
+                              uint32_t *prim_tests, void *stream);
+
+/* Per-object intersection list for n (ray, object) pairs.
+ * Replaces virtual bool ObjectBase::All_Intersections(const Ray&, IStack&,
+ *   TraceThreadData*) (core/scene/object.h:238) for the kinds above.
+ * Writes up to max_hits depths per ray into depths[i*max_hits + k]
+ * (sorted ascending) and prims likewise; counts[i] = number pushed
+ * (may exceed max_hits: the list is then truncated). */
+int pvx_all_intersections(pvx_scene *scene, const pvx_ray *rays,
+                          const uint32_t *obj, size_t n, uint32_t max_hits,
+                          double *depths, int32_t *prims, uint32_t *counts,
+                          void *stream);
+
+/* Point membership for n (point, object) pairs.
+ * Replaces virtual bool ObjectBase::Inside(const Vector3d&, TraceThreadData*)
+ *   (core/scene/object.h:240) as called through Inside_Object
+ *   (core/scene/object.cpp:346-355). points: 3n doubles; out: n bytes 0/1. */
+int pvx_inside(pvx_scene *scene, const double *points, const uint32_t *obj,
+               size_t n, uint8_t *out, void *stream);
+
+/* ---- camera ------------------------------------------------------------ */
+/* Perspective camera as TracePixel::CreateCameraRay builds it
+ * (core/render/tracepixel.cpp:357-372, 918-926): for pixel (x, y) sampled at
+ * (x + jx, y + jy), dir = direction + ((x+jx)/w - 0.5)*right
+ *                            + (0.5 - (y+jy)/h)*up, normalised. */
+typedef struct pvx_camera {
+    double location[3];
+    double direction[3];
+    double right[3];
+    double up[3];
+    uint32_t width;
+    uint32_t height;
+    double jitter_x; /* sample offset inside the pixel (0.5 = centre) */
+    double jitter_y;
+} pvx_camera;
+
+/* Writes width*height rays (row-major, y down) into device buffer `rays`. */
+int pvx_camera_rays(const pvx_camera *cam, pvx_ray *rays, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PVX_H */

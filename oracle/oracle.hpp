@@ -1,0 +1,214 @@
+// FP64 CPU ORACLE — TEST INFRASTRUCTURE ONLY.
+//
+// A restatement, in this project's own code, of the POV-Ray 3.8 algorithms on
+// the closest-hit / per-shape / radiosity hot path.  It is the checker for the
+// HIP engine: only tests, the smoke check and the bench's CPU-baseline leg may
+// load it.  The product library never links or calls it.
+//
+// Every function cites the reference file:line it follows (paths relative to
+// the POV-Ray source tree).  Floating-point types follow the reference: DBL
+// (double) for geometry, SNGL (float) where the reference stores FP32
+// (BoundingBox, mesh vertices/normals/Perp/Distance).  Build with
+// -ffp-contract=off so that no FMA contraction changes rounding.
+#ifndef PVX_ORACLE_HPP
+#define PVX_ORACLE_HPP
+
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+#include "../include/pvx.h"
+
+namespace orc {
+
+// core/configcore.h:153-201 (HUGE_VAL is <cmath>'s +inf: <cmath> is included
+// before configcore.h's #ifndef HUGE_VAL fallback).
+constexpr double kEpsilon = 1.0e-10;
+constexpr double kBoundHuge = 2.0e10;
+constexpr double kSmallTolerance = 1.0e-3;
+constexpr double kMaxDistance = 1.0e7;
+constexpr double kMinIsectDepth = 1.0e-4;
+
+struct V3 {
+    double x = 0, y = 0, z = 0;
+    V3() = default;
+    V3(double a, double b, double c) : x(a), y(b), z(c) {}
+    double &operator[](int i) { return i == 0 ? x : (i == 1 ? y : z); }
+    double operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+};
+inline V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V3 operator*(V3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+inline V3 operator*(double s, V3 a) { return {a.x * s, a.y * s, a.z * s}; }
+inline V3 operator/(V3 a, double s) { return {a.x / s, a.y / s, a.z / s}; }
+inline double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V3 cross(V3 a, V3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+inline double length_sqr(V3 a) { return dot(a, a); }
+inline double length(V3 a) { return std::sqrt(dot(a, a)); }
+inline V3 normalized(V3 a) { return a / length(a); }
+
+struct Ray {
+    V3 o, d;
+    V3 eval(double t) const { return o + d * t; }
+};
+
+// core/math/matrix.h TRANSFORM: matrix + inverse, row-vector convention.
+struct Transform {
+    double m[4][4];
+    double inv[4][4];
+};
+V3 mtrans_point(const double (*m)[4], V3 p);     // matrix.cpp:415-427
+V3 mtrans_direction(const double (*m)[4], V3 p); // matrix.cpp:455-466
+V3 minv_trans_normal(const double (*m)[4], V3 p);// matrix.cpp MInvTransNormal
+
+// core/bounding/boundingbox.h:83-108 BoundingBox (lowerLeft/size, FP32).
+struct FBox {
+    float ll[3] = {0, 0, 0};
+    float size[3] = {0, 0, 0};
+};
+FBox make_bbox_from_min_max(V3 mins, V3 maxs); // boundingbox.h:114-118 (Vector3d)
+
+// One entry of an IStack (core/coretypes.h Intersection, reduced to what the
+// hot path reports).
+struct Isect {
+    double depth;
+    int32_t prim;   // triangle index / CSG leaf slot
+    int32_t side;   // box side (Intersection::i1)
+    int32_t leaf;   // CSG: leaf shape (type<<24 | index), else -1
+};
+
+// ---------------------------------------------------------------------------
+// BBox tree (core/bounding/boundingbox.cpp)
+// ---------------------------------------------------------------------------
+struct BNode {
+    FBox bbox;
+    bool infinite = false;
+    std::vector<int32_t> kids; // empty => leaf
+    int32_t element = -1;      // leaf payload (object id / triangle index)
+};
+
+class BBoxTree {
+  public:
+    std::vector<BNode> nodes;
+    int32_t root = -1;
+    // Build_BBox_Tree (boundingbox.cpp:262-323) over leaf boxes; infinite
+    // elements go into the root's first child.
+    void build(const std::vector<FBox> &finite_boxes, const std::vector<int32_t> &finite_ids,
+               const std::vector<FBox> &infinite_boxes, const std::vector<int32_t> &infinite_ids);
+    int depth() const;
+};
+
+// Rayinfo (boundingbox.h:182-216)
+struct RayInfo {
+    float origin[3];
+    float inv_dir[3];
+    bool nonzero[3];
+    bool positive[3];
+    explicit RayInfo(const Ray &r);
+};
+
+// BBoxPriorityQueue (boundingbox.cpp:80-147): binary min-heap, element 0 unused.
+class PQueue {
+  public:
+    struct Elem {
+        double depth;
+        int32_t node;
+    };
+    std::vector<Elem> q{Elem{0, -1}};
+    void clear() { q.resize(1); }
+    bool empty() const { return q.size() == 1; }
+    void insert(double depth, int32_t node);
+    bool remove_min(double &depth, int32_t &node);
+};
+
+// Check_And_Enqueue (boundingbox.cpp:541-648)
+void check_and_enqueue(PQueue &q, const BNode &node, int32_t node_id, const RayInfo &ri);
+
+// Intersect_BBox_Dir (object.cpp:1074-1108) on an object's FP32 box.
+bool intersect_bbox_dir(const FBox &b, const Ray &r, float mind, float maxd);
+
+// ---------------------------------------------------------------------------
+// shapes
+// ---------------------------------------------------------------------------
+// Sphere::Intersect (sphere.cpp:214-241)
+bool sphere_intersect(const Ray &r, V3 centre, double radius2, double *d1, double *d2);
+
+struct MeshTri {                       // Mesh_Triangle_Struct (mesh.h:88-107)
+    float perp[3];
+    float distance;
+    int32_t normal_ind;
+    int32_t p1, p2, p3;
+    int32_t n1, n2, n3;
+    uint8_t smooth, dominant_axis, v_axis;
+};
+
+struct Mesh {
+    std::vector<float> vertices;  // 3 * nv (MeshVector, FP32)
+    std::vector<float> normals;   // 3 * (n_normals + n_triangles)
+    std::vector<MeshTri> tris;
+    std::vector<uint8_t> valid;   // Compute_Mesh_Triangle result (degenerate => 0)
+    BBoxTree tree;
+    FBox bbox;
+    V3 vert(int i) const {
+        return {vertices[3 * i], vertices[3 * i + 1], vertices[3 * i + 2]};
+    }
+};
+void mesh_prepare(Mesh &m, const pvx_mesh_desc &d);                // Parse_Mesh2 tail + Build_Mesh_BBox_Tree
+bool mesh_triangle_intersect(const Mesh &m, const MeshTri &t, const Ray &r, double *depth); // mesh.cpp:1040-1136
+void mesh_all_intersections(const Mesh &m, const Ray &r, std::vector<Isect> &stack, bool use_tree);
+bool mesh_inside(const Mesh &m, V3 p);
+V3 mesh_normal(const Mesh &m, int tri, V3 ipoint);                   // Mesh::Normal + Smooth_Mesh_Normal
+
+// Box::Intersect (box.cpp:172-505)
+bool box_intersect(const Ray &r, const Transform *tr, V3 c1, V3 c2, double *d1, double *d2,
+                   int *s1, int *s2);
+bool box_inside(V3 p, const Transform *tr, V3 c1, V3 c2, bool inverted);
+// Torus (torus.cpp)
+bool torus_all_intersections(const Ray &r, const Transform *tr, double R, double rr, bool sturm,
+                             std::vector<Isect> &stack, int32_t slot);
+FBox box_bbox(V3 c1, V3 c2, const Transform *tr);
+FBox torus_bbox(double R, double rr, const Transform *tr);
+bool torus_inside(V3 p, const Transform *tr, double R, double rr, bool inverted);
+
+// polynomialsolver.cpp:1585-1732
+int solve_polynomial(int n, const double *c0, double *r, bool sturm, double epsilon);
+
+// ---------------------------------------------------------------------------
+// scene (SceneData::objects + shape storage + bounding slabs)
+// ---------------------------------------------------------------------------
+struct Object {
+    uint32_t type, index, flags;
+    FBox bbox;
+    bool infinite = false;
+};
+
+struct Scene {
+    std::vector<Object> objects;
+    std::vector<V3> sph_c;
+    std::vector<double> sph_r;
+    std::vector<V3> pl_n;
+    std::vector<double> pl_d;
+    std::vector<V3> box_c1, box_c2;
+    std::vector<int32_t> box_tr;
+    std::vector<double> tor_R, tor_r;
+    std::vector<int32_t> tor_tr;
+    std::vector<uint8_t> tor_sturm;
+    std::vector<Transform> trs;
+    std::vector<Mesh> meshes;
+    std::vector<pvx_csg_node> csg;
+    std::vector<pvx_object> csg_children;
+    BBoxTree tree;
+
+    const Transform *tr(int32_t i) const { return i < 0 ? nullptr : &trs[i]; }
+    FBox shape_bbox(uint32_t type, uint32_t index, uint32_t flags, bool *infinite) const;
+    bool all_intersections(uint32_t type, uint32_t index, uint32_t flags, const Ray &r,
+                           std::vector<Isect> &st, int32_t slot) const;
+    bool inside(uint32_t type, uint32_t index, uint32_t flags, V3 p) const;
+    bool find_object(uint32_t obj, const Ray &r, bool shadow, Isect &out) const;
+};
+
+} // namespace orc
+
+#endif

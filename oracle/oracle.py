@@ -1,0 +1,122 @@
+"""ctypes wrapper of the FP64 CPU oracle — TEST INFRASTRUCTURE ONLY.
+
+Only tests/, the smoke check and bench.py's CPU-baseline leg load this module;
+the product (povray_amd) never imports it.  The library is built from the C++
+restatement in this directory (`make -C oracle`).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from povray_amd import _abi as A
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+BOUND_HUGE = 2.0e10
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle library not built: {path} (run make -C oracle)")
+        L = C.CDLL(path)
+        L.orc_scene_create.restype = C.c_void_p
+        L.orc_scene_create.argtypes = [C.POINTER(A.PvxSceneDesc)]
+        L.orc_scene_destroy.argtypes = [C.c_void_p]
+        L.orc_find_intersection.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
+                                            C.c_size_t, C.c_int, C.c_void_p, C.c_int]
+        L.orc_all_intersections.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
+                                            C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_inside.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]
+        L.orc_tree_info.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int64),
+                                    C.POINTER(C.c_int32)]
+        L.orc_mesh_triangle.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p]
+        L.orc_mesh_normal.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_size_t,
+                                      C.c_void_p]
+        L.orc_solve_polynomial.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_double, C.c_void_p]
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+HIT_DTYPE = np.dtype([("t", np.float64), ("obj", np.int32), ("prim", np.int32)])
+
+
+class OracleScene:
+    """The oracle's copy of a scene (its own mesh preparation and BBox trees)."""
+
+    def __init__(self, scene):
+        L = lib()
+        self._desc, self._keep = scene.desc()
+        self.h = L.orc_scene_create(C.byref(self._desc))
+        if not self.h:
+            raise RuntimeError("orc_scene_create failed")
+
+    def close(self):
+        if self.h:
+            lib().orc_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def find_intersection(self, rays, max_depth=None, ray_kind=A.PVX_RAY_IMAGE, method=1,
+                          threads=os.cpu_count() or 1):
+        """Trace::FindIntersection for (n, 6) float64 rays -> structured hits."""
+        rays = np.ascontiguousarray(rays, np.float64).reshape(-1, 6)
+        n = rays.shape[0]
+        hits = np.zeros(n, HIT_DTYPE)
+        md = None if max_depth is None else np.ascontiguousarray(max_depth, np.float64)
+        lib().orc_find_intersection(self.h, _p(rays), None if md is None else _p(md),
+                                    ray_kind, n, method, _p(hits), threads)
+        return hits
+
+    def all_intersections(self, rays, obj, max_hits=8):
+        rays = np.ascontiguousarray(rays, np.float64).reshape(-1, 6)
+        obj = np.ascontiguousarray(obj, np.uint32)
+        n = rays.shape[0]
+        d = np.zeros((n, max_hits))
+        p = np.zeros((n, max_hits), np.int32)
+        c = np.zeros(n, np.uint32)
+        lib().orc_all_intersections(self.h, _p(rays), _p(obj), n, max_hits, _p(d), _p(p), _p(c))
+        return d, p, c
+
+    def inside(self, pts, obj):
+        pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+        obj = np.ascontiguousarray(obj, np.uint32)
+        out = np.zeros(pts.shape[0], np.uint8)
+        lib().orc_inside(self.h, _p(pts), _p(obj), pts.shape[0], _p(out))
+        return out.astype(bool)
+
+    def tree_info(self, mesh=-1):
+        n = C.c_int64()
+        d = C.c_int32()
+        lib().orc_tree_info(self.h, mesh, C.byref(n), C.byref(d))
+        return n.value, d.value
+
+    def mesh_triangle(self, mesh, tri):
+        out = np.zeros(10)
+        lib().orc_mesh_triangle(self.h, mesh, tri, _p(out))
+        return out
+
+    def mesh_normal(self, mesh, tri, pts):
+        tri = np.ascontiguousarray(tri, np.int32)
+        pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+        out = np.zeros_like(pts)
+        lib().orc_mesh_normal(self.h, mesh, _p(tri), _p(pts), pts.shape[0], _p(out))
+        return out
+
+
+def solve_polynomial(coeffs, sturm=True, epsilon=1e-4):
+    """Solve_Polynomial: coeffs in descending powers (order = len-1)."""
+    c = np.ascontiguousarray(coeffs, np.float64)
+    r = np.zeros(max(4, c.shape[0]))
+    n = lib().orc_solve_polynomial(c.shape[0] - 1, _p(c), 1 if sturm else 0, epsilon, _p(r))
+    return np.sort(r[:n])

@@ -1,0 +1,96 @@
+"""ctypes mirror of include/pvx.h (struct layouts and enums).
+
+Used by the Python host side (scene upload, tests, bench) to call the C-ABI
+library.  Layouts must match pvx.h byte for byte; the size checks at the end
+of this module fail loudly if they drift.
+"""
+import ctypes as C
+
+PVX_OK, PVX_EINVAL, PVX_ENOMEM, PVX_EDEVICE, PVX_EUNSUPPORTED, PVX_ESTATE = range(6)
+
+PVX_SPHERE, PVX_BOX, PVX_TORUS, PVX_PLANE, PVX_MESH, PVX_CSG = 1, 2, 3, 4, 5, 6
+
+PVX_FLAG_NO_SHADOW = 1 << 0
+PVX_FLAG_NO_IMAGE = 1 << 1
+PVX_FLAG_NO_REFLECTION = 1 << 2
+PVX_FLAG_NO_RADIOSITY = 1 << 3
+PVX_FLAG_INVERTED = 1 << 4
+
+PVX_RAY_PRIMARY = 0
+PVX_RAY_IMAGE = 1
+PVX_RAY_REFLECTION = 2
+PVX_RAY_RADIOSITY = 4
+PVX_RAY_SHADOW = 8
+
+PVX_CSG_UNION, PVX_CSG_INTERSECTION, PVX_CSG_MERGE = 1, 2, 3
+
+c_dp = C.POINTER(C.c_double)
+c_fp = C.POINTER(C.c_float)
+c_i32p = C.POINTER(C.c_int32)
+c_u8p = C.POINTER(C.c_uint8)
+
+
+class PvxObject(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("index", C.c_uint32),
+                ("flags", C.c_uint32), ("pad", C.c_uint32)]
+
+
+class PvxCsgNode(C.Structure):
+    _fields_ = [("op", C.c_uint32), ("first_child", C.c_uint32),
+                ("n_children", C.c_uint32), ("pad", C.c_uint32)]
+
+
+class PvxMeshDesc(C.Structure):
+    _fields_ = [("n_vertices", C.c_uint32), ("n_triangles", C.c_uint32),
+                ("n_normals", C.c_uint32), ("pad", C.c_uint32),
+                ("vertices", c_fp), ("indices", c_i32p), ("normals", c_fp)]
+
+
+class PvxSceneDesc(C.Structure):
+    _fields_ = [
+        ("n_objects", C.c_uint32), ("objects", C.POINTER(PvxObject)),
+        ("n_spheres", C.c_uint32), ("sphere_center", c_dp), ("sphere_radius", c_dp),
+        ("n_planes", C.c_uint32), ("plane_normal", c_dp), ("plane_distance", c_dp),
+        ("n_boxes", C.c_uint32), ("box_corner1", c_dp), ("box_corner2", c_dp),
+        ("box_transform", c_i32p),
+        ("n_tori", C.c_uint32), ("torus_major", c_dp), ("torus_minor", c_dp),
+        ("torus_transform", c_i32p), ("torus_sturm", c_u8p),
+        ("n_transforms", C.c_uint32), ("transforms", c_dp),
+        ("n_meshes", C.c_uint32), ("meshes", C.POINTER(PvxMeshDesc)),
+        ("n_csg", C.c_uint32), ("csg", C.POINTER(PvxCsgNode)),
+        ("n_csg_children", C.c_uint32), ("csg_children", C.POINTER(PvxObject)),
+    ]
+
+
+class PvxRay(C.Structure):
+    _fields_ = [("o", C.c_double * 3), ("d", C.c_double * 3)]
+
+
+class PvxHit(C.Structure):
+    _fields_ = [("t", C.c_double), ("obj", C.c_int32), ("prim", C.c_int32)]
+
+
+class PvxBuildOpts(C.Structure):
+    _fields_ = [("max_leaf_size", C.c_uint32), ("sah_bins", C.c_uint32),
+                ("reserved", C.c_uint32 * 6)]
+
+
+class PvxSceneStats(C.Structure):
+    _fields_ = [("n_nodes", C.c_uint64), ("n_prims", C.c_uint64),
+                ("node_bytes", C.c_uint64), ("tri_bytes", C.c_uint64),
+                ("device_bytes", C.c_uint64), ("bvh_depth", C.c_uint32),
+                ("n_infinite", C.c_uint32), ("build_ms", C.c_double)]
+
+
+class PvxCamera(C.Structure):
+    _fields_ = [("location", C.c_double * 3), ("direction", C.c_double * 3),
+                ("right", C.c_double * 3), ("up", C.c_double * 3),
+                ("width", C.c_uint32), ("height", C.c_uint32),
+                ("jitter_x", C.c_double), ("jitter_y", C.c_double)]
+
+
+assert C.sizeof(PvxObject) == 16
+assert C.sizeof(PvxRay) == 48
+assert C.sizeof(PvxHit) == 16
+assert C.sizeof(PvxMeshDesc) == 40
+assert C.sizeof(PvxCamera) == 4 * 24 + 8 + 16

@@ -1,0 +1,652 @@
+// pvx: MI355X (gfx950) closest-hit / per-shape / camera kernels and the C-ABI.
+//
+// Closest hit (replaces Trace::FindIntersection, core/render/trace.cpp:285-344):
+//   * one lane per ray, persistent grid sized to the device (wave64);
+//   * FP32 traversal of a BVH2 (64 B nodes): slab test in FMA form against
+//     conservative boxes, near child first, far child pushed on a per-lane
+//     LDS stack (32 entries, [depth][lane] so lanes never share a bank) that
+//     overflows into a per-lane global spill area;
+//   * leaves hold 64 B primitive records: mesh triangles and spheres inline,
+//     boxes/tori/CSG by reference.  Each candidate is decided with the
+//     reference's own FP64 arithmetic, so hit ids and depths follow it.
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/pvx.h"
+#include "scene_prep.h"
+#include "shapes_device.h"
+
+using namespace pvx;
+
+// ----------------------------------------------------------------------------
+// errors
+// ----------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(PVX_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));    \
+    } while (0)
+
+extern "C" const char *pvx_last_error(void) { return g_err.c_str(); }
+
+// ----------------------------------------------------------------------------
+// kernels
+// ----------------------------------------------------------------------------
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kStackLds = 32;
+constexpr int kStackSpill = 32;
+constexpr int kStackMax = kStackLds + kStackSpill;
+constexpr int32_t kEmpty = INT32_MIN;
+
+struct RayF {
+    float ix, iy, iz;    // 1 / direction (FP32, zero components clamped)
+    float oix, oiy, oiz; // origin * inverse direction
+};
+
+__device__ inline float safe_inv(double d) {
+    float f = (float)d;
+    if (fabsf(f) < 1.0e-20f) f = f < 0.0f ? -1.0e-20f : 1.0e-20f;
+    return 1.0f / f;
+}
+
+__device__ inline bool better(double dep, int32_t obj, int32_t prim, double bt, int32_t bo,
+                              int32_t bp) {
+    // strictly closer wins; an exact tie between two hits goes to the lower
+    // (object, primitive) key so results do not depend on traversal order.
+    if (dep < bt) return true;
+    return dep == bt && bo >= 0 && (obj < bo || (obj == bo && prim < bp));
+}
+
+// Find_Intersection's depth filter (object.cpp:149-161) with the shadow-ray
+// postcondition dist > SMALL_TOLERANCE (trace.cpp:1946-1949).
+__device__ inline bool depth_ok(double dep, bool shadow) {
+    return dep >= kMinIsectDepth && (!shadow || dep > kSmallTolerance);
+}
+
+// NoSomethingFlagRayObjectCondition (trace.cpp:84-95) /
+// NoShadowFlagRayObjectCondition (trace.cpp:1941-1944).
+__device__ inline bool precondition(uint32_t flags, uint32_t kind) {
+    if (kind == PVX_RAY_SHADOW) return !(flags & PVX_FLAG_NO_SHADOW);
+    if ((kind & PVX_RAY_IMAGE) && (flags & PVX_FLAG_NO_IMAGE)) return false;
+    if ((kind & PVX_RAY_REFLECTION) && (flags & PVX_FLAG_NO_REFLECTION)) return false;
+    if ((kind & PVX_RAY_RADIOSITY) && (flags & PVX_FLAG_NO_RADIOSITY)) return false;
+    return true;
+}
+
+// Closest valid hit of one CSG object (its leaves' hits that survive every
+// CSG filter up to the root); returns false when none beats `limit`.
+__device__ inline bool csg_closest(const SceneView &sv, uint32_t root, D3 o, D3 d, bool shadow,
+                                   double limit, double &dep_out, int32_t &slot_out) {
+    uint32_t b = sv.csg_leaf_range[2 * root], e = sv.csg_leaf_range[2 * root + 1];
+    bool found = false;
+    double best = limit;
+    for (uint32_t k = b; k < e; k++) {
+        uint32_t slot = sv.csg_leaves[k];
+        const CsgChild &c = sv.csg_children[slot];
+        double hs[kMaxLeafHits];
+        int nh = leaf_hits(sv, c.type, c.index, o, d, hs);
+        for (int j = 0; j < nh; j++) {
+            double dep = hs[j];
+            if (!depth_ok(dep, shadow)) continue;
+            if (!(dep < best || (found && dep == best && (int32_t)slot < slot_out))) continue;
+            D3 ip = add3(o, mul3(d, dep));
+            if (csg_survives(sv, slot, ip)) {
+                best = dep;
+                slot_out = (int32_t)slot;
+                found = true;
+            }
+        }
+    }
+    dep_out = best;
+    return found;
+}
+
+// Closest valid hit of a box / torus object (Find_Intersection over its IStack).
+__device__ inline bool simple_closest(const SceneView &sv, uint32_t kind, uint32_t index, D3 o,
+                                      D3 d, bool shadow, double &dep_out) {
+    double hs[kMaxLeafHits];
+    int side[2];
+    int nh = kind == kPrimBox ? box_hits(sv.boxes[index], sv.xforms, o, d, hs, side)
+                              : torus_hits(sv.tori[index], sv.xforms, o, d, hs);
+    bool found = false;
+    double best = 0.0;
+    for (int j = 0; j < nh; j++) {
+        if (depth_ok(hs[j], shadow) && (!found || hs[j] < best)) {
+            best = hs[j];
+            found = true;
+        }
+    }
+    dep_out = best;
+    return found;
+}
+
+template <bool kComplex, bool kFlags, bool kCount>
+__global__ __launch_bounds__(kBlock) void k_trace_closest(SceneView sv, const pvx_ray *__restrict__ rays,
+                                                          const double *__restrict__ max_depth,
+                                                          uint32_t kind, uint32_t n,
+                                                          pvx_hit *__restrict__ hits,
+                                                          int32_t *__restrict__ spill,
+                                                          uint32_t *__restrict__ node_visits,
+                                                          uint32_t *__restrict__ prim_tests) {
+    __shared__ int32_t s_stack[kStackLds * kBlock];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t gtid = blockIdx.x * kBlock + lane;
+    int32_t *my_spill = spill + (size_t)gtid * kStackSpill;
+    const bool shadow = (kind == PVX_RAY_SHADOW);
+
+    for (uint32_t i = gtid; i < n; i += gridDim.x * kBlock) {
+        const pvx_ray r = rays[i];
+        D3 o = d3(r.o[0], r.o[1], r.o[2]);
+        D3 d = d3(r.d[0], r.d[1], r.d[2]);
+        double bt = kBoundHuge;
+        if (max_depth) {
+            double md = max_depth[i];
+            if (md >= kEpsilon) bt = md; // trace.cpp:157-158
+        }
+        int32_t bo = -1, bp = 0;
+
+        // objects outside the BVH (planes, unbounded CSG)
+        for (uint32_t k = 0; k < sv.n_inf; k++) {
+            const InfRec inf = sv.inf[k];
+            if (kFlags && !precondition(sv.obj_flags[inf.obj], kind)) continue;
+            double dep;
+            int32_t prim = 0;
+            bool ok;
+            if (inf.kind == kPrimPlane) {
+                ok = plane_hit(sv.planes[inf.index], o, d, dep) && depth_ok(dep, shadow);
+            } else if (kComplex) {
+                ok = csg_closest(sv, inf.index, o, d, shadow, bt, dep, prim);
+            } else {
+                ok = false;
+            }
+            if (ok && better(dep, inf.obj, prim, bt, bo, bp)) {
+                bt = dep;
+                bo = inf.obj;
+                bp = prim;
+            }
+        }
+
+        RayF rf;
+        rf.ix = safe_inv(d.x);
+        rf.iy = safe_inv(d.y);
+        rf.iz = safe_inv(d.z);
+        rf.oix = (float)o.x * rf.ix;
+        rf.oiy = (float)o.y * rf.iy;
+        rf.oiz = (float)o.z * rf.iz;
+        float tb = (float)bt * 1.000001f;
+
+        int32_t node = sv.root;
+        int sp = 0;
+        uint32_t cnt_nodes = 0, cnt_prims = 0;
+        while (node != kEmpty) {
+            if (node >= 0) {
+                if (kCount) cnt_nodes++;
+                const float4 *np = reinterpret_cast<const float4 *>(sv.nodes + node);
+                const float4 a = np[0], b = np[1], c = np[2];
+                const int4 l = reinterpret_cast<const int4 *>(np)[3];
+                float x0 = fmaf(a.x, rf.ix, -rf.oix), x1 = fmaf(a.y, rf.ix, -rf.oix);
+                float y0 = fmaf(a.z, rf.iy, -rf.oiy), y1 = fmaf(a.w, rf.iy, -rf.oiy);
+                float z0 = fmaf(c.x, rf.iz, -rf.oiz), z1 = fmaf(c.y, rf.iz, -rf.oiz);
+                float tn0 = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
+                float tf0 = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tb));
+                x0 = fmaf(b.x, rf.ix, -rf.oix);
+                x1 = fmaf(b.y, rf.ix, -rf.oix);
+                y0 = fmaf(b.z, rf.iy, -rf.oiy);
+                y1 = fmaf(b.w, rf.iy, -rf.oiy);
+                z0 = fmaf(c.z, rf.iz, -rf.oiz);
+                z1 = fmaf(c.w, rf.iz, -rf.oiz);
+                float tn1 = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
+                float tf1 = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tb));
+                bool h0 = tn0 <= tf0 * 1.000001f;
+                bool h1 = tn1 <= tf1 * 1.000001f;
+                if (h0 && h1) {
+                    int32_t nearc = l.x, farc = l.y;
+                    if (tn1 < tn0) {
+                        nearc = l.y;
+                        farc = l.x;
+                    }
+                    if (sp < kStackLds)
+                        s_stack[sp * kBlock + lane] = farc;
+                    else
+                        my_spill[sp - kStackLds] = farc;
+                    sp++;
+                    node = nearc;
+                    continue;
+                }
+                if (h0) {
+                    node = l.x;
+                    continue;
+                }
+                if (h1) {
+                    node = l.y;
+                    continue;
+                }
+            } else {
+                // leaf: ~node = first << 3 | (count - 1)
+                const uint32_t leaf = (uint32_t)(~node);
+                const uint32_t first = leaf >> 3, count = (leaf & 7u) + 1u;
+                if (kCount) cnt_prims += count;
+                for (uint32_t k = 0; k < count; k++) {
+                    const PrimRec &p = sv.prims[first + k];
+                    const uint32_t ka = p.kind_axis;
+                    const uint32_t pk = ka & 15u;
+                    const int32_t obj = p.obj;
+                    if (kFlags && !precondition(sv.obj_flags[obj], kind)) continue;
+                    if (pk == kPrimTriangle) {
+                        double dep;
+                        if (tri_hit64(p.tri, ka >> 4, o, d, dep) && depth_ok(dep, shadow) &&
+                            better(dep, obj, p.tri.prim, bt, bo, bp)) {
+                            bt = dep;
+                            bo = obj;
+                            bp = p.tri.prim;
+                        }
+                    } else if (pk == kPrimSphere) {
+                        double hs[2];
+                        int nh = sphere_hits(d3(p.sph.c[0], p.sph.c[1], p.sph.c[2]), p.sph.r, o, d, hs);
+                        for (int j = 0; j < nh; j++)
+                            if (depth_ok(hs[j], shadow) && better(hs[j], obj, 0, bt, bo, bp)) {
+                                bt = hs[j];
+                                bo = obj;
+                                bp = 0;
+                            }
+                    } else if (kComplex) {
+                        double dep;
+                        int32_t prim = 0;
+                        bool ok;
+                        if (pk == kPrimCsg)
+                            ok = csg_closest(sv, p.ref.index, o, d, shadow, bt, dep, prim);
+                        else
+                            ok = simple_closest(sv, pk, p.ref.index, o, d, shadow, dep);
+                        if (ok && better(dep, obj, prim, bt, bo, bp)) {
+                            bt = dep;
+                            bo = obj;
+                            bp = prim;
+                        }
+                    }
+                }
+                tb = (float)bt * 1.000001f;
+            }
+            // pop
+            if (sp == 0) {
+                node = kEmpty;
+            } else {
+                sp--;
+                node = sp < kStackLds ? s_stack[sp * kBlock + lane] : my_spill[sp - kStackLds];
+            }
+        }
+        hits[i] = pvx_hit{bt, bo, bp};
+        if (kCount) {
+            node_visits[i] = cnt_nodes;
+            prim_tests[i] = cnt_prims;
+        }
+    }
+}
+
+// Perspective camera rays (tracepixel.cpp:357-372, 918-926).
+__global__ void k_camera_rays(pvx_camera cam, pvx_ray *rays) {
+    uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t y = blockIdx.y;
+    if (x >= cam.width || y >= cam.height) return;
+    double x0 = ((double)x + cam.jitter_x) / (double)cam.width - 0.5;
+    double y0 = 0.5 - ((double)y + cam.jitter_y) / (double)cam.height;
+    double dx = cam.direction[0] + x0 * cam.right[0] + y0 * cam.up[0];
+    double dy = cam.direction[1] + x0 * cam.right[1] + y0 * cam.up[1];
+    double dz = cam.direction[2] + x0 * cam.right[2] + y0 * cam.up[2];
+    double len = sqrt(dx * dx + dy * dy + dz * dz);
+    pvx_ray r;
+    r.o[0] = cam.location[0];
+    r.o[1] = cam.location[1];
+    r.o[2] = cam.location[2];
+    r.d[0] = dx / len;
+    r.d[1] = dy / len;
+    r.d[2] = dz / len;
+    rays[(size_t)y * cam.width + x] = r;
+}
+
+// All intersections of one object per ray, sorted ascending
+// (ObjectBase::All_Intersections, object.h:238).
+__global__ void k_all_intersections(SceneView sv, const pvx_ray *rays, const uint32_t *objs,
+                                    uint32_t n, uint32_t max_hits, double *depths, int32_t *prims,
+                                    uint32_t *counts, const uint32_t *obj_type,
+                                    const uint32_t *obj_index) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const pvx_ray r = rays[i];
+    D3 o = d3(r.o[0], r.o[1], r.o[2]);
+    D3 d = d3(r.d[0], r.d[1], r.d[2]);
+    uint32_t ob = objs[i];
+    uint32_t type = obj_type[ob], index = obj_index[ob];
+    double buf[16];
+    int32_t pbuf[16];
+    uint32_t cnt = 0;
+    auto push = [&](double dep, int32_t prim) {
+        if (cnt < 16) {
+            // insertion sort keeps the list ascending
+            uint32_t j = cnt;
+            while (j > 0 && buf[j - 1] > dep) {
+                buf[j] = buf[j - 1];
+                pbuf[j] = pbuf[j - 1];
+                j--;
+            }
+            buf[j] = dep;
+            pbuf[j] = prim;
+        }
+        cnt++;
+    };
+    if (type == PVX_PLANE) {
+        double dep;
+        if (plane_hit(sv.planes[index], o, d, dep)) push(dep, 0);
+    } else if (type == PVX_CSG) {
+        uint32_t b = sv.csg_leaf_range[2 * index], e = sv.csg_leaf_range[2 * index + 1];
+        for (uint32_t k = b; k < e; k++) {
+            uint32_t slot = sv.csg_leaves[k];
+            const CsgChild &c = sv.csg_children[slot];
+            double hs[kMaxLeafHits];
+            int nh = leaf_hits(sv, c.type, c.index, o, d, hs);
+            for (int j = 0; j < nh; j++)
+                if (csg_survives(sv, slot, add3(o, mul3(d, hs[j])))) push(hs[j], (int32_t)slot);
+        }
+    } else if (type == PVX_MESH) {
+        // every triangle of the mesh: walk the whole BVH without culling by depth
+        int32_t stack[kStackMax];
+        int sp = 0;
+        int32_t node = sv.root;
+        float ix = safe_inv(d.x), iy = safe_inv(d.y), iz = safe_inv(d.z);
+        float oix = (float)o.x * ix, oiy = (float)o.y * iy, oiz = (float)o.z * iz;
+        while (node != kEmpty) {
+            if (node >= 0) {
+                const Node &nd = sv.nodes[node];
+                const float lo0[3] = {nd.lo0x, nd.lo0y, nd.lo0z}, hi0[3] = {nd.hi0x, nd.hi0y, nd.hi0z};
+                const float lo1[3] = {nd.lo1x, nd.lo1y, nd.lo1z}, hi1[3] = {nd.hi1x, nd.hi1y, nd.hi1z};
+                const float iv[3] = {ix, iy, iz}, oi[3] = {oix, oiy, oiz};
+                float tn0 = 0.0f, tf0 = 3.0e38f, tn1 = 0.0f, tf1 = 3.0e38f;
+                for (int k = 0; k < 3; k++) {
+                    float a0 = fmaf(lo0[k], iv[k], -oi[k]), b0 = fmaf(hi0[k], iv[k], -oi[k]);
+                    float a1 = fmaf(lo1[k], iv[k], -oi[k]), b1 = fmaf(hi1[k], iv[k], -oi[k]);
+                    tn0 = fmaxf(tn0, fminf(a0, b0));
+                    tf0 = fminf(tf0, fmaxf(a0, b0));
+                    tn1 = fmaxf(tn1, fminf(a1, b1));
+                    tf1 = fminf(tf1, fmaxf(a1, b1));
+                }
+                bool h0 = tn0 <= tf0 * 1.000001f, h1 = tn1 <= tf1 * 1.000001f;
+                if (h0 && h1 && sp < kStackMax) {
+                    stack[sp++] = nd.c1;
+                    node = nd.c0;
+                    continue;
+                }
+                if (h0) { node = nd.c0; continue; }
+                if (h1) { node = nd.c1; continue; }
+            } else {
+                uint32_t leaf = (uint32_t)(~node), first = leaf >> 3, count = (leaf & 7u) + 1u;
+                for (uint32_t k = 0; k < count; k++) {
+                    const PrimRec &p = sv.prims[first + k];
+                    if ((p.kind_axis & 15u) != kPrimTriangle || p.obj != (int32_t)ob) continue;
+                    double dep;
+                    if (tri_hit64(p.tri, p.kind_axis >> 4, o, d, dep)) push(dep, p.tri.prim);
+                }
+            }
+            node = sp == 0 ? kEmpty : stack[--sp];
+        }
+    } else {
+        double hs[kMaxLeafHits];
+        int nh = leaf_hits(sv, type, index, o, d, hs);
+        for (int j = 0; j < nh; j++) push(hs[j], 0);
+    }
+    counts[i] = cnt;
+    for (uint32_t k = 0; k < max_hits; k++) {
+        bool have = k < cnt && k < 16;
+        depths[(size_t)i * max_hits + k] = have ? buf[k] : HUGE_VAL;
+        prims[(size_t)i * max_hits + k] = have ? pbuf[k] : -1;
+    }
+}
+
+// Inside_Object per (point, object) (object.cpp:346-355).
+__global__ void k_inside(SceneView sv, const double *pts, const uint32_t *objs, uint32_t n,
+                         uint8_t *out, const uint32_t *obj_type, const uint32_t *obj_index,
+                         const uint32_t *obj_flags) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    D3 p = d3(pts[3 * (size_t)i], pts[3 * (size_t)i + 1], pts[3 * (size_t)i + 2]);
+    uint32_t ob = objs[i];
+    out[i] = shape_inside<kCsgMaxDepth>(sv, obj_type[ob], obj_index[ob], obj_flags[ob], p) ? 1 : 0;
+}
+
+} // namespace
+
+// ----------------------------------------------------------------------------
+// scene object
+// ----------------------------------------------------------------------------
+struct pvx_scene {
+    int device = 0;
+    HostScene host;
+    SceneView view{};
+    std::vector<void *> allocs;
+    uint32_t *d_obj_type = nullptr, *d_obj_index = nullptr;
+    int32_t *d_spill = nullptr;
+    uint32_t grid = 0;
+    uint64_t device_bytes = 0;
+    uint32_t n_objects = 0;
+};
+
+template <class T>
+static int upload(pvx_scene *s, const std::vector<T> &v, const T **dst) {
+    *dst = nullptr;
+    if (v.empty()) return PVX_OK;
+    void *p = nullptr;
+    HIP_TRY(hipMalloc(&p, v.size() * sizeof(T)));
+    s->allocs.push_back(p);
+    HIP_TRY(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    s->device_bytes += v.size() * sizeof(T);
+    *dst = static_cast<const T *>(p);
+    return PVX_OK;
+}
+
+static void scene_free(pvx_scene *s) {
+    for (void *p : s->allocs) (void)hipFree(p);
+    s->allocs.clear();
+}
+
+extern "C" int pvx_init(int device) {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) return fail(PVX_EDEVICE, "no HIP device available");
+    if (device < 0 || device >= n) return fail(PVX_EINVAL, "device index out of range");
+    HIP_TRY(hipSetDevice(device));
+    return PVX_OK;
+}
+
+extern "C" int pvx_scene_create(const pvx_scene_desc *desc, const pvx_build_opts *opts,
+                                pvx_scene **out) {
+    if (!desc || !out) return fail(PVX_EINVAL, "null argument");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(PVX_EDEVICE, "no HIP device available");
+    pvx_scene *s = new pvx_scene();
+    HIP_TRY(hipGetDevice(&s->device));
+    std::string err;
+    int rc = prepare_scene(*desc, opts, s->host, err);
+    if (rc != PVX_OK) {
+        delete s;
+        return fail(rc, err);
+    }
+    const HostScene &h = s->host;
+    if (h.depth + 1 >= (uint32_t)kStackMax) {
+        delete s;
+        return fail(PVX_EUNSUPPORTED, "BVH deeper than the traversal stack");
+    }
+    SceneView &v = s->view;
+    int r = PVX_OK;
+#define UP(vec, field)                                          \
+    if ((r = upload(s, vec, &v.field)) != PVX_OK) {             \
+        scene_free(s);                                          \
+        delete s;                                               \
+        return r;                                               \
+    }
+    UP(h.nodes, nodes);
+    UP(h.prims, prims);
+    UP(h.inf, inf);
+    UP(h.obj_flags, obj_flags);
+    UP(h.planes, planes);
+    UP(h.boxes, boxes);
+    UP(h.tori, tori);
+    UP(h.xforms, xforms);
+    UP(h.sph, sph);
+    UP(h.csg, csg);
+    UP(h.csg_children, csg_children);
+    UP(h.csg_leaves, csg_leaves);
+    UP(h.csg_leaf_range, csg_leaf_range);
+#undef UP
+    const uint32_t *tmp = nullptr;
+    if ((r = upload(s, h.obj_type, &tmp)) != PVX_OK) { scene_free(s); delete s; return r; }
+    s->d_obj_type = const_cast<uint32_t *>(tmp);
+    std::vector<uint32_t> obj_index(desc->n_objects);
+    for (uint32_t i = 0; i < desc->n_objects; i++) obj_index[i] = desc->objects[i].index;
+    if ((r = upload(s, obj_index, &tmp)) != PVX_OK) { scene_free(s); delete s; return r; }
+    s->d_obj_index = const_cast<uint32_t *>(tmp);
+    s->n_objects = desc->n_objects;
+    v.root = h.root;
+    v.n_inf = (uint32_t)h.inf.size();
+    v.any_flags = h.any_flags;
+    v.has_complex = h.has_complex;
+
+    // persistent grid: enough resident blocks to fill every CU
+    int cus = 0;
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device));
+    int per_cu = 0;
+    auto kfn = h.has_complex ? (h.any_flags ? (const void *)k_trace_closest<true, true, false>
+                                            : (const void *)k_trace_closest<true, false, false>)
+                             : (h.any_flags ? (const void *)k_trace_closest<false, true, false>
+                                            : (const void *)k_trace_closest<false, false, false>);
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, kBlock, 0));
+    if (per_cu < 1) per_cu = 1;
+    s->grid = (uint32_t)(cus * per_cu);
+    void *spill = nullptr;
+    size_t spill_bytes = (size_t)s->grid * kBlock * kStackSpill * sizeof(int32_t);
+    HIP_TRY(hipMalloc(&spill, spill_bytes));
+    s->allocs.push_back(spill);
+    s->d_spill = static_cast<int32_t *>(spill);
+    s->device_bytes += spill_bytes;
+    *out = s;
+    return PVX_OK;
+}
+
+extern "C" int pvx_scene_destroy(pvx_scene *s) {
+    if (!s) return PVX_OK;
+    (void)hipDeviceSynchronize();
+    scene_free(s);
+    delete s;
+    return PVX_OK;
+}
+
+extern "C" int pvx_scene_get_stats(const pvx_scene *s, pvx_scene_stats *out) {
+    if (!s || !out) return fail(PVX_EINVAL, "null argument");
+    out->n_nodes = s->host.nodes.size();
+    out->n_prims = s->host.prims.size();
+    out->node_bytes = sizeof(Node);
+    out->tri_bytes = sizeof(PrimRec);
+    out->device_bytes = s->device_bytes;
+    out->bvh_depth = s->host.depth;
+    out->n_infinite = (uint32_t)s->host.inf.size();
+    out->build_ms = s->host.build_ms;
+    return PVX_OK;
+}
+
+template <bool kCount>
+static int launch_closest(pvx_scene *s, const pvx_ray *rays, const double *max_depth,
+                          uint32_t ray_kind, size_t n, pvx_hit *hits, uint32_t *nv, uint32_t *pt,
+                          void *stream) {
+    if (!s) return fail(PVX_EINVAL, "null scene");
+    if (n == 0) return PVX_OK;
+    if (!rays || !hits) return fail(PVX_EINVAL, "null ray or hit buffer");
+    if (n > 0xFFFFFFFFull) return fail(PVX_EINVAL, "too many rays in one call");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    uint32_t nblocks = (uint32_t)std::min<size_t>(s->grid, (n + kBlock - 1) / kBlock);
+    bool flags = s->view.any_flags != 0 && ray_kind != PVX_RAY_PRIMARY;
+    uint32_t nn = (uint32_t)n;
+    if (s->view.has_complex) {
+        if (flags)
+            k_trace_closest<true, true, kCount><<<nblocks, kBlock, 0, st>>>(s->view, rays, max_depth, ray_kind, nn, hits, s->d_spill, nv, pt);
+        else
+            k_trace_closest<true, false, kCount><<<nblocks, kBlock, 0, st>>>(s->view, rays, max_depth, ray_kind, nn, hits, s->d_spill, nv, pt);
+    } else {
+        if (flags)
+            k_trace_closest<false, true, kCount><<<nblocks, kBlock, 0, st>>>(s->view, rays, max_depth, ray_kind, nn, hits, s->d_spill, nv, pt);
+        else
+            k_trace_closest<false, false, kCount><<<nblocks, kBlock, 0, st>>>(s->view, rays, max_depth, ray_kind, nn, hits, s->d_spill, nv, pt);
+    }
+    HIP_TRY(hipGetLastError());
+    return PVX_OK;
+}
+
+extern "C" int pvx_trace_closest(pvx_scene *s, const pvx_ray *rays, const double *max_depth,
+                                 uint32_t ray_kind, size_t n, pvx_hit *hits, void *stream) {
+    return launch_closest<false>(s, rays, max_depth, ray_kind, n, hits, nullptr, nullptr, stream);
+}
+
+extern "C" int pvx_trace_closest_counted(pvx_scene *s, const pvx_ray *rays,
+                                         const double *max_depth, uint32_t ray_kind, size_t n,
+                                         pvx_hit *hits, uint32_t *node_visits,
+                                         uint32_t *prim_tests, void *stream) {
+    if (!node_visits || !prim_tests) return fail(PVX_EINVAL, "null counter buffer");
+    return launch_closest<true>(s, rays, max_depth, ray_kind, n, hits, node_visits, prim_tests,
+                                stream);
+}
+
+extern "C" int pvx_all_intersections(pvx_scene *s, const pvx_ray *rays, const uint32_t *obj,
+                                     size_t n, uint32_t max_hits, double *depths, int32_t *prims,
+                                     uint32_t *counts, void *stream) {
+    if (!s) return fail(PVX_EINVAL, "null scene");
+    if (n == 0) return PVX_OK;
+    if (!rays || !obj || !depths || !prims || !counts || max_hits == 0)
+        return fail(PVX_EINVAL, "null buffer or max_hits == 0");
+    if (n > 0xFFFFFFFFull) return fail(PVX_EINVAL, "too many rays in one call");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    uint32_t nb = (uint32_t)((n + 127) / 128);
+    k_all_intersections<<<nb, 128, 0, st>>>(s->view, rays, obj, (uint32_t)n, max_hits, depths,
+                                            prims, counts, s->d_obj_type, s->d_obj_index);
+    HIP_TRY(hipGetLastError());
+    return PVX_OK;
+}
+
+extern "C" int pvx_inside(pvx_scene *s, const double *points, const uint32_t *obj, size_t n,
+                          uint8_t *out, void *stream) {
+    if (!s) return fail(PVX_EINVAL, "null scene");
+    if (n == 0) return PVX_OK;
+    if (!points || !obj || !out) return fail(PVX_EINVAL, "null buffer");
+    if (n > 0xFFFFFFFFull) return fail(PVX_EINVAL, "too many points in one call");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    uint32_t nb = (uint32_t)((n + 255) / 256);
+    k_inside<<<nb, 256, 0, st>>>(s->view, points, obj, (uint32_t)n, out, s->d_obj_type,
+                                 s->d_obj_index, s->view.obj_flags);
+    HIP_TRY(hipGetLastError());
+    return PVX_OK;
+}
+
+extern "C" int pvx_camera_rays(const pvx_camera *cam, pvx_ray *rays, void *stream) {
+    if (!cam || !rays) return fail(PVX_EINVAL, "null argument");
+    if (cam->width == 0 || cam->height == 0 || cam->height > 65535)
+        return fail(PVX_EINVAL, "bad image size");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    dim3 grid((cam->width + 255) / 256, cam->height);
+    k_camera_rays<<<grid, 256, 0, st>>>(*cam, rays);
+    HIP_TRY(hipGetLastError());
+    return PVX_OK;
+}

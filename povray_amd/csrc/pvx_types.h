@@ -1,0 +1,143 @@
+// Shared host/device data layout of the pvx engine (gfx950).
+//
+// HBM layout of a scene:
+//   nodes[]   BVH2 node, 64 B: two FP32 child boxes + two child links.
+//   prims[]   primitive record, 64 B, in BVH leaf order.  Triangles and
+//             spheres are stored inline (no indirection on the hot path);
+//             boxes, tori and CSG roots refer to their shape arrays.
+//   infinite[] objects tested outside the BVH (planes, unbounded CSG).
+// Child link encoding: >= 0 inner node index; < 0 leaf, ~link = first prim
+// << 3 | (count - 1), count <= 8.
+#ifndef PVX_TYPES_H
+#define PVX_TYPES_H
+
+#include <stdint.h>
+
+#include "../../include/pvx.h"
+
+namespace pvx {
+
+// core/configcore.h constants used on the path.
+constexpr double kEpsilon = 1.0e-10;
+constexpr double kBoundHuge = 2.0e10;
+constexpr double kSmallTolerance = 1.0e-3;
+constexpr double kMaxDistance = 1.0e7;
+constexpr double kMinIsectDepth = 1.0e-4;
+
+enum PrimKind : uint32_t {
+    kPrimTriangle = 0,
+    kPrimSphere = 1,
+    kPrimBox = 2,
+    kPrimTorus = 3,
+    kPrimCsg = 4,
+    kPrimPlane = 5,
+};
+
+struct alignas(16) Node {
+    float lo0x, hi0x, lo0y, hi0y; // child 0 x/y slabs
+    float lo1x, hi1x, lo1y, hi1y; // child 1 x/y slabs
+    float lo0z, hi0z, lo1z, hi1z; // z slabs of both children
+    int32_t c0, c1, pad0, pad1;
+};
+static_assert(sizeof(Node) == 64, "node is one 64 B record");
+
+// Mesh triangle after Compute_Mesh_Triangle (core/shape/mesh.cpp:838-945):
+// FP32 face normal (Normals[Normal_Ind]), FP32 Distance, vertices P1..P3 in
+// post-swap order, dominant axis.
+struct TriRec {
+    float n[3];
+    float dist;
+    float p1[3];
+    float p2[3];
+    float p3[3];
+    int32_t prim; // triangle index in the mesh2 face list
+};
+struct SphRec {
+    double c[3];
+    double r;
+    double pad[3];
+};
+struct RefRec {
+    uint32_t index; // index into the kind's shape array
+    uint32_t pad[13];
+};
+
+struct alignas(16) PrimRec {
+    union {
+        TriRec tri;
+        SphRec sph;
+        RefRec ref;
+    };
+    uint32_t kind_axis; // kind | dominant_axis << 4
+    int32_t obj;        // object id
+};
+static_assert(sizeof(PrimRec) == 64, "prim record is 64 B");
+
+// Transformed shapes (FP64): TRANSFORM matrix + inverse, row-vector convention.
+struct Xform {
+    double m[16];
+    double inv[16];
+};
+
+struct BoxRec {
+    double c1[3], c2[3];
+    int32_t xf;    // -1 = none
+    uint32_t flags;
+};
+
+struct TorusRec {
+    double R, r;
+    int32_t xf;    // always valid (identity if the scene gave none)
+    uint32_t sturm;
+    uint32_t flags;
+    uint32_t pad;
+};
+
+struct PlaneRec {
+    double n[3];
+    double d;
+};
+
+// CSG node / child tables (device copies of pvx_csg_node / pvx_object) plus
+// parent links so a leaf hit can be filtered bottom-up.
+struct CsgNode {
+    uint32_t op, first, count;
+    int32_t parent_slot; // slot of this node in its parent's children, -1 at root
+};
+struct CsgChild {
+    uint32_t type, index, flags;
+    uint32_t node; // CSG node that owns this slot
+};
+
+// An object tested outside the BVH.
+struct InfRec {
+    uint32_t kind; // kPrimPlane / kPrimCsg
+    uint32_t index;
+    int32_t obj;
+    uint32_t pad;
+};
+
+// Everything a kernel needs, passed by value.
+struct SceneView {
+    const Node *nodes;
+    const PrimRec *prims;
+    const InfRec *inf;
+    const uint32_t *obj_flags;
+    const PlaneRec *planes;
+    const BoxRec *boxes;
+    const TorusRec *tori;
+    const Xform *xforms;
+    const double *sph; // sphere array (cx, cy, cz, r) per sphere, for CSG leaves
+    const CsgNode *csg;
+    const CsgChild *csg_children;
+    const uint32_t *csg_leaves;      // leaf slots of each CSG root, contiguous
+    const uint32_t *csg_leaf_range;  // 2 per CSG node: [begin, end) into csg_leaves
+    int32_t root;      // root link (node index or leaf link); INT32_MIN = empty
+    uint32_t n_inf;
+    uint32_t any_flags;
+    uint32_t has_complex; // boxes/tori/CSG present in the BVH
+};
+
+} // namespace pvx
+
+#endif

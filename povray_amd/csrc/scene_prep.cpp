@@ -1,0 +1,472 @@
+// Host-side scene preparation for the pvx engine.
+//
+// * mesh2 triangles are prepared exactly as the reference's parser leaves
+//   them (Parse_Mesh2 -> Mesh::Compute_Mesh_Triangle, parser/parser.cpp:
+//   4512-4565, core/shape/mesh.cpp:838-945): FP64 face normal from the FP32
+//   vertices, stored as FP32 together with an FP32 Distance; vertex order
+//   swapped by the dominant-axis orientation test.  Degenerate triangles keep
+//   a zero normal in the reference and can never be hit, so they are dropped.
+// * every bounded primitive gets a conservative FP32 box (FP64 bounds
+//   rounded outward, then padded) and goes into one BVH.
+// * planes and unbounded CSG objects are tested outside the BVH, like the
+//   reference's infinite objects (boundingbox.cpp:288-306).
+#include "scene_prep.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <functional>
+
+#include "bvh_build.h"
+
+namespace pvx {
+
+namespace {
+
+struct DBox {
+    double lo[3], hi[3];
+    bool bounded;
+    static DBox empty() {
+        DBox b;
+        for (int k = 0; k < 3; k++) {
+            b.lo[k] = INFINITY;
+            b.hi[k] = -INFINITY;
+        }
+        b.bounded = true;
+        return b;
+    }
+    static DBox unbounded() {
+        DBox b;
+        for (int k = 0; k < 3; k++) {
+            b.lo[k] = -INFINITY;
+            b.hi[k] = INFINITY;
+        }
+        b.bounded = false;
+        return b;
+    }
+    void grow(const double *p) {
+        for (int k = 0; k < 3; k++) {
+            lo[k] = std::min(lo[k], p[k]);
+            hi[k] = std::max(hi[k], p[k]);
+        }
+    }
+};
+
+inline float round_down(double v) {
+    float f = (float)v;
+    if ((double)f > v) f = std::nextafter(f, -INFINITY);
+    return f;
+}
+inline float round_up(double v) {
+    float f = (float)v;
+    if ((double)f < v) f = std::nextafter(f, INFINITY);
+    return f;
+}
+
+// Conservative FP32 box with a small relative pad (absorbs FP32 slab-test
+// rounding in traversal).
+void to_fbox(const DBox &b, float *lo, float *hi) {
+    for (int k = 0; k < 3; k++) {
+        double ext = b.hi[k] - b.lo[k];
+        double mag = std::max(std::fabs(b.lo[k]), std::fabs(b.hi[k]));
+        double pad = ext * 1.0e-6 + mag * 4.8e-7 + 1.0e-30;
+        lo[k] = round_down(b.lo[k] - pad);
+        hi[k] = round_up(b.hi[k] + pad);
+    }
+}
+
+void xf_point(const double *m, const double *p, double *r) {
+    for (int i = 0; i < 3; i++) r[i] = p[0] * m[i] + p[1] * m[4 + i] + p[2] * m[8 + i] + m[12 + i];
+}
+
+DBox transformed_box(const double *lo, const double *hi, const double *m) {
+    DBox b = DBox::empty();
+    for (int i = 0; i < 8; i++) {
+        double c[3] = {(i & 1) ? hi[0] : lo[0], (i & 2) ? hi[1] : lo[1], (i & 4) ? hi[2] : lo[2]};
+        double r[3];
+        if (m)
+            xf_point(m, c, r);
+        else
+            std::memcpy(r, c, sizeof r);
+        b.grow(r);
+    }
+    return b;
+}
+
+inline int max3_axis(double x, double y, double z) { // mesh.cpp:94 max3_coordinate
+    return (x > y) ? ((x > z) ? 0 : 2) : ((y > z) ? 1 : 2);
+}
+
+} // namespace
+
+int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene &out,
+                  std::string &err) {
+    auto t0 = std::chrono::steady_clock::now();
+    if (d.n_objects == 0 || !d.objects) {
+        err = "scene has no objects";
+        return PVX_EINVAL;
+    }
+    // --- validate arrays -----------------------------------------------------
+    auto need = [&](bool ok, const char *what) {
+        if (!ok) err = std::string("missing or invalid array: ") + what;
+        return ok;
+    };
+    if (!need(d.n_spheres == 0 || (d.sphere_center && d.sphere_radius), "spheres")) return PVX_EINVAL;
+    if (!need(d.n_planes == 0 || (d.plane_normal && d.plane_distance), "planes")) return PVX_EINVAL;
+    if (!need(d.n_boxes == 0 || (d.box_corner1 && d.box_corner2), "boxes")) return PVX_EINVAL;
+    if (!need(d.n_tori == 0 || (d.torus_major && d.torus_minor), "tori")) return PVX_EINVAL;
+    if (!need(d.n_transforms == 0 || d.transforms, "transforms")) return PVX_EINVAL;
+    if (!need(d.n_meshes == 0 || d.meshes, "meshes")) return PVX_EINVAL;
+    if (!need(d.n_csg == 0 || (d.csg && d.csg_children), "csg")) return PVX_EINVAL;
+
+    auto check_ref = [&](uint32_t type, uint32_t index) -> bool {
+        uint32_t lim = 0;
+        switch (type) {
+        case PVX_SPHERE: lim = d.n_spheres; break;
+        case PVX_BOX: lim = d.n_boxes; break;
+        case PVX_TORUS: lim = d.n_tori; break;
+        case PVX_PLANE: lim = d.n_planes; break;
+        case PVX_MESH: lim = d.n_meshes; break;
+        case PVX_CSG: lim = d.n_csg; break;
+        default:
+            err = "unknown object type " + std::to_string(type);
+            return false;
+        }
+        if (index >= lim) {
+            err = "object index out of range (type " + std::to_string(type) + ", index " +
+                  std::to_string(index) + ")";
+            return false;
+        }
+        return true;
+    };
+
+    // --- shared shape arrays -------------------------------------------------
+    out.xforms.resize(d.n_transforms);
+    for (uint32_t i = 0; i < d.n_transforms; i++) {
+        std::memcpy(out.xforms[i].m, d.transforms + 32 * (size_t)i, 16 * sizeof(double));
+        std::memcpy(out.xforms[i].inv, d.transforms + 32 * (size_t)i + 16, 16 * sizeof(double));
+    }
+    int32_t identity = -1;
+    auto get_identity = [&]() {
+        if (identity < 0) {
+            Xform x{};
+            for (int k = 0; k < 4; k++) x.m[5 * k] = x.inv[5 * k] = 1.0;
+            identity = (int32_t)out.xforms.size();
+            out.xforms.push_back(x);
+        }
+        return identity;
+    };
+    out.sph.resize(4 * (size_t)d.n_spheres);
+    for (uint32_t i = 0; i < d.n_spheres; i++) {
+        for (int k = 0; k < 3; k++) out.sph[4 * i + k] = d.sphere_center[3 * i + k];
+        out.sph[4 * i + 3] = d.sphere_radius[i];
+    }
+    out.planes.resize(d.n_planes);
+    for (uint32_t i = 0; i < d.n_planes; i++) {
+        for (int k = 0; k < 3; k++) out.planes[i].n[k] = d.plane_normal[3 * i + k];
+        out.planes[i].d = d.plane_distance[i];
+    }
+    out.boxes.resize(d.n_boxes);
+    for (uint32_t i = 0; i < d.n_boxes; i++) {
+        BoxRec &b = out.boxes[i];
+        for (int k = 0; k < 3; k++) {
+            b.c1[k] = d.box_corner1[3 * i + k];
+            b.c2[k] = d.box_corner2[3 * i + k];
+        }
+        b.xf = d.box_transform ? d.box_transform[i] : -1;
+        if (b.xf >= (int32_t)d.n_transforms) {
+            err = "box transform index out of range";
+            return PVX_EINVAL;
+        }
+        b.flags = 0;
+    }
+    out.tori.resize(d.n_tori);
+    for (uint32_t i = 0; i < d.n_tori; i++) {
+        TorusRec &t = out.tori[i];
+        t.R = d.torus_major[i];
+        t.r = d.torus_minor[i];
+        t.xf = d.torus_transform ? d.torus_transform[i] : -1;
+        if (t.xf >= (int32_t)d.n_transforms) {
+            err = "torus transform index out of range";
+            return PVX_EINVAL;
+        }
+        if (t.xf < 0) t.xf = get_identity(); // the reference always gives a torus a TRANSFORM
+        t.sturm = d.torus_sturm ? d.torus_sturm[i] : 0;
+        t.flags = 0;
+        t.pad = 0;
+    }
+
+    // --- CSG tables ------------------------------------------------------------
+    out.csg.resize(d.n_csg);
+    out.csg_children.resize(d.n_csg_children);
+    std::vector<int32_t> owner_slot(d.n_csg, -1);
+    for (uint32_t i = 0; i < d.n_csg; i++) {
+        const pvx_csg_node &n = d.csg[i];
+        if (n.op < PVX_CSG_UNION || n.op > PVX_CSG_MERGE) {
+            err = "bad CSG operator";
+            return PVX_EINVAL;
+        }
+        if ((uint64_t)n.first_child + n.n_children > d.n_csg_children) {
+            err = "CSG child range out of bounds";
+            return PVX_EINVAL;
+        }
+        out.csg[i] = CsgNode{n.op, n.first_child, n.n_children, -1};
+        for (uint32_t k = 0; k < n.n_children; k++) {
+            uint32_t slot = n.first_child + k;
+            const pvx_object &c = d.csg_children[slot];
+            if (!check_ref(c.type, c.index)) return PVX_EINVAL;
+            if (c.type == PVX_MESH) {
+                err = "meshes inside CSG are not supported";
+                return PVX_EUNSUPPORTED;
+            }
+            out.csg_children[slot] = CsgChild{c.type, c.index, c.flags, i};
+            if (c.type == PVX_CSG) {
+                if (owner_slot[c.index] >= 0) {
+                    err = "CSG node shared by two parents";
+                    return PVX_EUNSUPPORTED;
+                }
+                owner_slot[c.index] = (int32_t)slot;
+            }
+        }
+    }
+    for (uint32_t i = 0; i < d.n_csg; i++) out.csg[i].parent_slot = owner_slot[i];
+    // depth check + leaf lists (tree order) per node
+    out.csg_leaf_range.assign(2 * (size_t)d.n_csg, 0);
+    std::function<int(uint32_t, int)> depth_of = [&](uint32_t node, int depth) -> int {
+        if (depth > 8) return depth;
+        int m = depth;
+        const CsgNode &n = out.csg[node];
+        for (uint32_t k = 0; k < n.count; k++) {
+            const CsgChild &c = out.csg_children[n.first + k];
+            if (c.type == PVX_CSG) m = std::max(m, depth_of(c.index, depth + 1));
+        }
+        return m;
+    };
+    std::function<void(uint32_t)> collect = [&](uint32_t node) {
+        const CsgNode &n = out.csg[node];
+        for (uint32_t k = 0; k < n.count; k++) {
+            const CsgChild &c = out.csg_children[n.first + k];
+            if (c.type == PVX_CSG)
+                collect(c.index);
+            else
+                out.csg_leaves.push_back(n.first + k);
+        }
+    };
+    for (uint32_t i = 0; i < d.n_csg; i++) {
+        if (owner_slot[i] >= 0) continue; // not a root
+        if (depth_of(i, 1) > 8) {
+            err = "CSG nesting deeper than 8 levels";
+            return PVX_EUNSUPPORTED;
+        }
+        out.csg_leaf_range[2 * i] = (uint32_t)out.csg_leaves.size();
+        collect(i);
+        out.csg_leaf_range[2 * i + 1] = (uint32_t)out.csg_leaves.size();
+    }
+
+    // --- bounds of shapes (FP64, conservative) ------------------------------
+    // Two bounds per shape: S bounds its surfaces (where hits can lie), V
+    // bounds the region where Inside can be true.  A CSG hit lies on a child
+    // surface and inside every other child of an intersection, so
+    //   union/merge:  S = U S(c),            V = U V(c)
+    //   intersection: S = (U S(c)) n V,      V = n V(c)
+    // Inversion changes V (unbounded) but not S.  Planes are unbounded.
+    struct Bnd {
+        DBox S, V;
+    };
+    auto leaf_box = [&](uint32_t type, uint32_t index) -> DBox {
+        switch (type) {
+        case PVX_SPHERE: {
+            DBox b;
+            b.bounded = true;
+            for (int k = 0; k < 3; k++) {
+                b.lo[k] = out.sph[4 * index + k] - out.sph[4 * index + 3];
+                b.hi[k] = out.sph[4 * index + k] + out.sph[4 * index + 3];
+            }
+            return b;
+        }
+        case PVX_BOX: {
+            const BoxRec &bx = out.boxes[index];
+            return transformed_box(bx.c1, bx.c2, bx.xf >= 0 ? out.xforms[bx.xf].m : nullptr);
+        }
+        case PVX_TORUS: {
+            const TorusRec &t = out.tori[index];
+            double a = t.R + t.r;
+            double lo[3] = {-a, -t.r, -a}, hi[3] = {a, t.r, a};
+            return transformed_box(lo, hi, out.xforms[t.xf].m);
+        }
+        default:
+            return DBox::unbounded();
+        }
+    };
+    auto join = [](DBox a, const DBox &b) {
+        if (!a.bounded || !b.bounded) return DBox::unbounded();
+        for (int k = 0; k < 3; k++) {
+            a.lo[k] = std::min(a.lo[k], b.lo[k]);
+            a.hi[k] = std::max(a.hi[k], b.hi[k]);
+        }
+        return a;
+    };
+    auto meet = [](DBox a, const DBox &b) {
+        if (!b.bounded) return a;
+        if (!a.bounded) return b;
+        for (int k = 0; k < 3; k++) {
+            a.lo[k] = std::max(a.lo[k], b.lo[k]);
+            a.hi[k] = std::min(a.hi[k], b.hi[k]);
+        }
+        return a;
+    };
+    std::function<Bnd(uint32_t, uint32_t, uint32_t)> shape_bnd = [&](uint32_t type, uint32_t index,
+                                                                       uint32_t flags) -> Bnd {
+        if (type != PVX_CSG) {
+            DBox b = leaf_box(type, index);
+            return Bnd{b, (flags & PVX_FLAG_INVERTED) ? DBox::unbounded() : b};
+        }
+        const CsgNode &n = out.csg[index];
+        DBox S = DBox::empty(), V = (n.op == PVX_CSG_INTERSECTION) ? DBox::unbounded() : DBox::empty();
+        for (uint32_t k = 0; k < n.count; k++) {
+            const CsgChild &c = out.csg_children[n.first + k];
+            Bnd cb = shape_bnd(c.type, c.index, c.flags);
+            S = join(S, cb.S);
+            V = (n.op == PVX_CSG_INTERSECTION) ? meet(V, cb.V) : join(V, cb.V);
+        }
+        if (n.op == PVX_CSG_INTERSECTION) S = meet(S, V);
+        return Bnd{S, V};
+    };
+
+    // --- primitives ------------------------------------------------------------
+    std::vector<PrimRec> prims;
+    std::vector<float> plo, phi;
+    auto add_prim = [&](const PrimRec &p, const float *lo, const float *hi) {
+        prims.push_back(p);
+        plo.insert(plo.end(), lo, lo + 3);
+        phi.insert(phi.end(), hi, hi + 3);
+    };
+    out.obj_flags.resize(d.n_objects);
+    out.obj_type.resize(d.n_objects);
+    for (uint32_t o = 0; o < d.n_objects; o++) {
+        const pvx_object &ob = d.objects[o];
+        if (!check_ref(ob.type, ob.index)) return PVX_EINVAL;
+        if (ob.type == PVX_CSG && owner_slot[ob.index] >= 0) {
+            err = "a nested CSG node is listed as a top-level object";
+            return PVX_EINVAL;
+        }
+        out.obj_flags[o] = ob.flags;
+        out.obj_type[o] = ob.type;
+        if (ob.flags & (PVX_FLAG_NO_SHADOW | PVX_FLAG_NO_IMAGE | PVX_FLAG_NO_REFLECTION |
+                        PVX_FLAG_NO_RADIOSITY))
+            out.any_flags = 1;
+        PrimRec pr;
+        std::memset(&pr, 0, sizeof pr);
+        pr.obj = (int32_t)o;
+        float lo[3], hi[3];
+        switch (ob.type) {
+        case PVX_SPHERE: {
+            pr.kind_axis = kPrimSphere;
+            for (int k = 0; k < 3; k++) pr.sph.c[k] = out.sph[4 * ob.index + k];
+            pr.sph.r = out.sph[4 * ob.index + 3];
+            to_fbox(shape_bnd(ob.type, ob.index, 0).S, lo, hi);
+            add_prim(pr, lo, hi);
+            break;
+        }
+        case PVX_BOX:
+        case PVX_TORUS: {
+            pr.kind_axis = ob.type == PVX_BOX ? kPrimBox : kPrimTorus;
+            pr.ref.index = ob.index;
+            to_fbox(shape_bnd(ob.type, ob.index, 0).S, lo, hi);
+            add_prim(pr, lo, hi);
+            out.has_complex = 1;
+            break;
+        }
+        case PVX_CSG: {
+            DBox b = shape_bnd(ob.type, ob.index, 0).S;
+            if (!b.bounded || !(b.lo[0] <= b.hi[0] && b.lo[1] <= b.hi[1] && b.lo[2] <= b.hi[2])) {
+                if (!b.bounded) {
+                    out.inf.push_back(InfRec{kPrimCsg, ob.index, (int32_t)o, 0});
+                    out.has_complex = 1;
+                }
+                // an empty surface bound: the object can never be hit
+                break;
+            }
+            pr.kind_axis = kPrimCsg;
+            pr.ref.index = ob.index;
+            to_fbox(b, lo, hi);
+            add_prim(pr, lo, hi);
+            out.has_complex = 1;
+            break;
+        }
+        case PVX_PLANE:
+            out.inf.push_back(InfRec{kPrimPlane, ob.index, (int32_t)o, 0});
+            break;
+        case PVX_MESH: {
+            const pvx_mesh_desc &m = d.meshes[ob.index];
+            if (!m.vertices || !m.indices) {
+                err = "mesh without vertices or indices";
+                return PVX_EINVAL;
+            }
+            for (uint32_t t = 0; t < m.n_triangles; t++) {
+                int32_t ia = m.indices[3 * t], ib = m.indices[3 * t + 1], ic = m.indices[3 * t + 2];
+                if (ia < 0 || ib < 0 || ic < 0 || (uint32_t)ia >= m.n_vertices ||
+                    (uint32_t)ib >= m.n_vertices || (uint32_t)ic >= m.n_vertices) {
+                    err = "mesh face index out of range";
+                    return PVX_EINVAL;
+                }
+                const float *fa = m.vertices + 3 * (size_t)ia, *fb = m.vertices + 3 * (size_t)ib,
+                            *fc = m.vertices + 3 * (size_t)ic;
+                double P1[3] = {fa[0], fa[1], fa[2]}, P2[3] = {fb[0], fb[1], fb[2]},
+                       P3[3] = {fc[0], fc[1], fc[2]};
+                double V1[3] = {P2[0] - P1[0], P2[1] - P1[1], P2[2] - P1[2]};
+                double V2[3] = {P3[0] - P1[0], P3[1] - P1[1], P3[2] - P1[2]};
+                // S_Normal = cross(V2, V1)
+                double S[3] = {(V2[1] * V1[2]) - (V2[2] * V1[1]), (V2[2] * V1[0]) - (V2[0] * V1[2]),
+                               (V2[0] * V1[1]) - (V2[1] * V1[0])};
+                double L = std::sqrt(S[0] * S[0] + S[1] * S[1] + S[2] * S[2]);
+                if (L == 0.0) continue; // zero normal: never hit
+                for (int k = 0; k < 3; k++) S[k] = S[k] / L;
+                float dist = (float)((S[0] * P1[0]) + (S[1] * P1[1]) + (S[2] * P1[2]));
+                dist = (float)(dist * -1.0);
+                int axis = max3_axis(std::fabs(S[0]), std::fabs(S[1]), std::fabs(S[2]));
+                int a = axis == 0 ? 1 : 0, b = axis == 2 ? 1 : 2;
+                bool swap = (P2[a] - P3[a]) * (P2[b] - P1[b]) < (P2[b] - P3[b]) * (P2[a] - P1[a]);
+                const float *q1 = swap ? fb : fa, *q2 = swap ? fa : fb;
+                pr.kind_axis = kPrimTriangle | (uint32_t)(axis << 4);
+                for (int k = 0; k < 3; k++) {
+                    pr.tri.n[k] = (float)S[k];
+                    pr.tri.p1[k] = q1[k];
+                    pr.tri.p2[k] = q2[k];
+                    pr.tri.p3[k] = fc[k];
+                }
+                pr.tri.dist = dist;
+                pr.tri.prim = (int32_t)t;
+                DBox bb = DBox::empty();
+                bb.grow(P1);
+                bb.grow(P2);
+                bb.grow(P3);
+                to_fbox(bb, lo, hi);
+                add_prim(pr, lo, hi);
+                out.n_triangles++;
+            }
+            break;
+        }
+        }
+    }
+
+    // --- BVH ---------------------------------------------------------------------
+    int leaf = opts && opts->max_leaf_size ? (int)opts->max_leaf_size : 4;
+    int bins = opts && opts->sah_bins ? (int)opts->sah_bins : 16;
+    if (prims.size() >= (1u << 28)) {
+        err = "too many primitives for the 28-bit leaf encoding";
+        return PVX_EUNSUPPORTED;
+    }
+    BvhResult bvh = build_bvh(plo.data(), phi.data(), (uint32_t)prims.size(), leaf, bins);
+    out.nodes = std::move(bvh.nodes);
+    out.root = bvh.root;
+    out.depth = bvh.depth;
+    out.prims.resize(prims.size());
+    for (size_t k = 0; k < prims.size(); k++) out.prims[k] = prims[bvh.order[k]];
+    out.build_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return PVX_OK;
+}
+
+} // namespace pvx

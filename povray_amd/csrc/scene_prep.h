@@ -1,0 +1,42 @@
+// Host-side scene preparation: pvx_scene_desc -> device-ready arrays.
+#ifndef PVX_SCENE_PREP_H
+#define PVX_SCENE_PREP_H
+
+#include <string>
+#include <vector>
+
+#include "pvx_types.h"
+
+namespace pvx {
+
+struct HostScene {
+    std::vector<PrimRec> prims; // BVH leaf order
+    std::vector<Node> nodes;
+    int32_t root = INT32_MIN;
+    uint32_t depth = 0;
+    std::vector<InfRec> inf;
+    std::vector<uint32_t> obj_flags;
+    std::vector<uint32_t> obj_type;
+    std::vector<PlaneRec> planes;
+    std::vector<BoxRec> boxes;
+    std::vector<TorusRec> tori;
+    std::vector<Xform> xforms;
+    std::vector<double> sph; // 4 per sphere: centre, radius
+    std::vector<CsgNode> csg;
+    std::vector<CsgChild> csg_children;
+    std::vector<uint32_t> csg_leaves;
+    std::vector<uint32_t> csg_leaf_range;
+    uint32_t any_flags = 0;
+    uint32_t has_complex = 0;
+    uint64_t n_triangles = 0;
+    double build_ms = 0.0;
+};
+
+// Validates the description and builds every host array plus the BVH.
+// Returns a PVX_* status; `err` explains a failure.
+int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene &out,
+                  std::string &err);
+
+} // namespace pvx
+
+#endif

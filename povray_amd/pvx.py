@@ -1,0 +1,177 @@
+"""Python host side of the pvx C-ABI (include/pvx.h).
+
+Mirrors the reference's entry points for the hot path in batch form:
+`GpuScene.find_intersection` is Trace::FindIntersection
+(core/render/trace.cpp:285-344), `all_intersections` is
+ObjectBase::All_Intersections and `inside` is ObjectBase::Inside
+(core/scene/object.h:238-240).  Every call runs the HIP kernels in
+libpvx.so; there is no CPU fallback — a missing library or device raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _abi as A
+from . import hip
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpvx.so")
+_L = None
+
+HIT_DTYPE = np.dtype([("t", np.float64), ("obj", np.int32), ("prim", np.int32)])
+BOUND_HUGE = 2.0e10
+
+
+class PvxError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libpvx.so (built in-tree by __graft_entry__.build / make)."""
+    global _L
+    if _L is None:
+        if not os.path.exists(LIB_PATH):
+            raise PvxError(f"libpvx.so is not built ({LIB_PATH}); run __graft_entry__.build()")
+        hip._lib()  # bind the same HIP runtime first
+        L = C.CDLL(LIB_PATH)
+        vp, sz, u32 = C.c_void_p, C.c_size_t, C.c_uint32
+        L.pvx_last_error.restype = C.c_char_p
+        L.pvx_init.argtypes = [C.c_int]
+        L.pvx_scene_create.argtypes = [C.POINTER(A.PvxSceneDesc), C.POINTER(A.PvxBuildOpts),
+                                       C.POINTER(vp)]
+        L.pvx_scene_destroy.argtypes = [vp]
+        L.pvx_scene_get_stats.argtypes = [vp, C.POINTER(A.PvxSceneStats)]
+        L.pvx_trace_closest.argtypes = [vp, vp, vp, u32, sz, vp, vp]
+        L.pvx_trace_closest_counted.argtypes = [vp, vp, vp, u32, sz, vp, vp, vp, vp]
+        L.pvx_all_intersections.argtypes = [vp, vp, vp, sz, u32, vp, vp, vp, vp]
+        L.pvx_inside.argtypes = [vp, vp, vp, sz, vp, vp]
+        L.pvx_camera_rays.argtypes = [C.POINTER(A.PvxCamera), vp, vp]
+        _L = L
+    return _L
+
+
+EXPORTED_SYMBOLS = (
+    "pvx_last_error", "pvx_init", "pvx_scene_create", "pvx_scene_destroy",
+    "pvx_scene_get_stats", "pvx_trace_closest", "pvx_trace_closest_counted",
+    "pvx_all_intersections", "pvx_inside",
+    "pvx_camera_rays",
+)
+
+
+def _check(rc, what):
+    if rc != 0:
+        msg = lib().pvx_last_error().decode()
+        raise PvxError(f"{what} failed ({rc}): {msg}")
+
+
+def init(device: int = 0):
+    _check(lib().pvx_init(device), "pvx_init")
+
+
+def _ptr(buf):
+    if buf is None:
+        return None
+    return buf.value if isinstance(buf, hip.DeviceBuffer) else buf
+
+
+class GpuScene:
+    """A scene uploaded to the GPU with its BVH."""
+
+    def __init__(self, scene, max_leaf_size: int = 0, sah_bins: int = 0):
+        L = lib()
+        self.scene = scene
+        desc, keep = scene.desc()
+        opts = A.PvxBuildOpts()
+        opts.max_leaf_size = max_leaf_size
+        opts.sah_bins = sah_bins
+        h = C.c_void_p()
+        _check(L.pvx_scene_create(C.byref(desc), C.byref(opts), C.byref(h)), "pvx_scene_create")
+        self.h = h
+        del keep
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().pvx_scene_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def stats(self) -> dict:
+        s = A.PvxSceneStats()
+        _check(lib().pvx_scene_get_stats(self.h, C.byref(s)), "pvx_scene_get_stats")
+        return {k: getattr(s, k) for k, _ in A.PvxSceneStats._fields_}
+
+    # -- device-buffer API (what a C caller does) ---------------------------------
+    def trace_closest(self, rays, n, hits, max_depth=None, ray_kind=A.PVX_RAY_IMAGE,
+                      stream=None):
+        _check(lib().pvx_trace_closest(self.h, _ptr(rays), _ptr(max_depth), ray_kind, n,
+                                       _ptr(hits), stream.h if stream else None),
+               "pvx_trace_closest")
+
+    def trace_closest_counted(self, rays, n, hits, node_visits, prim_tests, max_depth=None,
+                              ray_kind=A.PVX_RAY_IMAGE, stream=None):
+        _check(lib().pvx_trace_closest_counted(self.h, _ptr(rays), _ptr(max_depth), ray_kind, n,
+                                               _ptr(hits), _ptr(node_visits), _ptr(prim_tests),
+                                               stream.h if stream else None),
+               "pvx_trace_closest_counted")
+
+    # -- numpy convenience (upload, run, download) -------------------------------
+    def find_intersection(self, rays: np.ndarray, max_depth=None, ray_kind=A.PVX_RAY_IMAGE):
+        rays = np.ascontiguousarray(rays, np.float64).reshape(-1, 6)
+        n = rays.shape[0]
+        out = np.zeros(n, HIT_DTYPE)
+        if n == 0:
+            return out
+        d_rays = hip.DeviceBuffer.from_array(rays)
+        d_hits = hip.DeviceBuffer(n * 16)
+        d_md = None
+        if max_depth is not None:
+            d_md = hip.DeviceBuffer.from_array(np.ascontiguousarray(max_depth, np.float64))
+        self.trace_closest(d_rays, n, d_hits, d_md, ray_kind)
+        hip.synchronize()
+        raw = d_hits.download(np.uint8, n * 16)
+        return raw.view(HIT_DTYPE).copy()
+
+    def all_intersections(self, rays: np.ndarray, obj, max_hits: int = 8):
+        rays = np.ascontiguousarray(rays, np.float64).reshape(-1, 6)
+        n = rays.shape[0]
+        obj = np.ascontiguousarray(obj, np.uint32)
+        d_r = hip.DeviceBuffer.from_array(rays)
+        d_o = hip.DeviceBuffer.from_array(obj)
+        d_d = hip.DeviceBuffer(n * max_hits * 8)
+        d_p = hip.DeviceBuffer(n * max_hits * 4)
+        d_c = hip.DeviceBuffer(n * 4)
+        _check(lib().pvx_all_intersections(self.h, d_r.value, d_o.value, n, max_hits, d_d.value,
+                                           d_p.value, d_c.value, None), "pvx_all_intersections")
+        hip.synchronize()
+        return (d_d.download(np.float64).reshape(n, max_hits),
+                d_p.download(np.int32).reshape(n, max_hits), d_c.download(np.uint32))
+
+    def inside(self, pts: np.ndarray, obj):
+        pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+        n = pts.shape[0]
+        obj = np.ascontiguousarray(obj, np.uint32)
+        d_p = hip.DeviceBuffer.from_array(pts)
+        d_o = hip.DeviceBuffer.from_array(obj)
+        d_out = hip.DeviceBuffer(max(n, 1))
+        _check(lib().pvx_inside(self.h, d_p.value, d_o.value, n, d_out.value, None), "pvx_inside")
+        hip.synchronize()
+        return d_out.download(np.uint8, n).astype(bool)
+
+
+def camera_rays(scene, width=None, height=None, jitter=(0.5, 0.5), out=None, stream=None):
+    """Primary rays for the scene's camera, generated on the GPU."""
+    cam = scene.camera_struct(jitter=jitter, width=width, height=height)
+    n = cam.width * cam.height
+    if out is None:
+        out = hip.DeviceBuffer(n * 48)
+    _check(lib().pvx_camera_rays(C.byref(cam), out.value, stream.h if stream else None),
+           "pvx_camera_rays")
+    return out, n

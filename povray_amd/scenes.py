@@ -1,0 +1,503 @@
+"""Seeded synthetic scenes for the hot-path workloads.
+
+Every scene is generated from a splitmix64 stream so that the GPU engine, the
+CPU oracle and the `.pov` emitter see bit-identical inputs.  Geometry follows
+POV-Ray's conventions: left-handed camera basis built like the `look_at`
+keyword (parser/parser.cpp:1926-1969), mesh2 vertices stored as FP32
+(core/shape/mesh.h: MeshVector = SnglVector3d), planes as N.P + D = 0
+(core/shape/plane.cpp:92-170).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _abi as A
+
+_GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def _mix64(z: np.ndarray) -> np.ndarray:
+    z = (z ^ (z >> np.uint64(30))) * _M1
+    z = (z ^ (z >> np.uint64(27))) * _M2
+    return z ^ (z >> np.uint64(31))
+
+
+class Rng:
+    """splitmix64 counter stream; uniform() returns doubles in [0, 1)."""
+
+    def __init__(self, seed: int):
+        self.state = np.uint64(seed & 0xFFFFFFFFFFFFFFFF)
+        self.count = 0
+
+    def bits(self, n: int) -> np.ndarray:
+        idx = np.arange(self.count + 1, self.count + n + 1, dtype=np.uint64)
+        self.count += n
+        with np.errstate(over="ignore"):
+            return _mix64(self.state + idx * _GOLDEN)
+
+    def uniform(self, n: int, lo=0.0, hi=1.0) -> np.ndarray:
+        u = (self.bits(n) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+        return lo + (hi - lo) * u
+
+
+# --------------------------------------------------------------------------
+# camera
+# --------------------------------------------------------------------------
+def look_at_camera(location, look_at, width, height, sky=(0.0, 1.0, 0.0)):
+    """Camera vectors as the `look_at` keyword leaves them, with the default
+    direction <0,0,1>, up <0,1,0>, right <w/h,0,0> (parser/parser.cpp:1926-1969,
+    core/scene/camera.cpp:228-231)."""
+    loc = np.asarray(location, np.float64)
+    d_len, up_len, right_len = 1.0, 1.0, width / height
+    direction = np.asarray(look_at, np.float64) - loc
+    direction /= np.linalg.norm(direction)
+    right = np.cross(np.asarray(sky, np.float64), direction)
+    right /= np.linalg.norm(right)
+    up = np.cross(direction, right)
+    # handedness of the default basis is positive: cross(up, dir) . right > 0
+    return dict(location=loc, direction=direction * d_len, right=right * right_len,
+                up=up * up_len, width=int(width), height=int(height))
+
+
+# --------------------------------------------------------------------------
+# scene container
+# --------------------------------------------------------------------------
+def _f64(a, shape):
+    return np.ascontiguousarray(np.asarray(a, np.float64).reshape(shape))
+
+
+@dataclass
+class Scene:
+    name: str
+    objects: np.ndarray                      # (n, 4) uint32: type, index, flags, pad
+    sphere_center: np.ndarray = None
+    sphere_radius: np.ndarray = None
+    plane_normal: np.ndarray = None
+    plane_distance: np.ndarray = None
+    box_corner1: np.ndarray = None
+    box_corner2: np.ndarray = None
+    box_transform: np.ndarray = None
+    torus_major: np.ndarray = None
+    torus_minor: np.ndarray = None
+    torus_transform: np.ndarray = None
+    torus_sturm: np.ndarray = None
+    transforms: np.ndarray = None            # (n, 32) float64
+    meshes: list = field(default_factory=list)  # (verts f32 (n,3), idx i32 (m,3), normals f32|None)
+    csg: np.ndarray = None                   # (n, 4) uint32: op, first, count, pad
+    csg_children: np.ndarray = None          # (n, 4) uint32 like objects
+    camera: dict = None
+    lights: list = field(default_factory=list)
+    materials: dict = field(default_factory=dict)  # per top-level object arrays
+    settings: dict = field(default_factory=dict)
+
+    @property
+    def n_objects(self):
+        return int(self.objects.shape[0])
+
+    @property
+    def n_triangles(self):
+        return int(sum(m[1].shape[0] for m in self.meshes))
+
+    def desc(self):
+        """Build a pvx_scene_desc that points into this scene's arrays.
+        Returns (desc, keepalive); keep both alive while the desc is used."""
+        keep = []
+
+        def arr(a, dtype, ctype):
+            if a is None or len(a) == 0:
+                return None, 0
+            a = np.ascontiguousarray(a, dtype=dtype)
+            keep.append(a)
+            return a.ctypes.data_as(C.POINTER(ctype)), a.shape[0]
+
+        d = A.PvxSceneDesc()
+        objs = np.ascontiguousarray(self.objects, np.uint32).reshape(-1, 4)
+        keep.append(objs)
+        d.n_objects = objs.shape[0]
+        d.objects = objs.ctypes.data_as(C.POINTER(A.PvxObject))
+        if self.sphere_center is not None and len(self.sphere_center):
+            d.sphere_center, d.n_spheres = arr(self.sphere_center, np.float64, C.c_double)
+            d.sphere_radius, _ = arr(self.sphere_radius, np.float64, C.c_double)
+        if self.plane_normal is not None and len(self.plane_normal):
+            d.plane_normal, d.n_planes = arr(self.plane_normal, np.float64, C.c_double)
+            d.plane_distance, _ = arr(self.plane_distance, np.float64, C.c_double)
+        if self.box_corner1 is not None and len(self.box_corner1):
+            d.box_corner1, d.n_boxes = arr(self.box_corner1, np.float64, C.c_double)
+            d.box_corner2, _ = arr(self.box_corner2, np.float64, C.c_double)
+            d.box_transform, _ = arr(self.box_transform, np.int32, C.c_int32)
+        if self.torus_major is not None and len(self.torus_major):
+            d.torus_major, d.n_tori = arr(self.torus_major, np.float64, C.c_double)
+            d.torus_minor, _ = arr(self.torus_minor, np.float64, C.c_double)
+            d.torus_transform, _ = arr(self.torus_transform, np.int32, C.c_int32)
+            d.torus_sturm, _ = arr(self.torus_sturm, np.uint8, C.c_uint8)
+        if self.transforms is not None and len(self.transforms):
+            d.transforms, d.n_transforms = arr(self.transforms, np.float64, C.c_double)
+        if self.meshes:
+            md = (A.PvxMeshDesc * len(self.meshes))()
+            for i, (v, ix, nrm) in enumerate(self.meshes):
+                v = np.ascontiguousarray(v, np.float32)
+                ix = np.ascontiguousarray(ix, np.int32)
+                keep += [v, ix]
+                md[i].n_vertices = v.shape[0]
+                md[i].n_triangles = ix.shape[0]
+                md[i].vertices = v.ctypes.data_as(A.c_fp)
+                md[i].indices = ix.ctypes.data_as(A.c_i32p)
+                if nrm is not None:
+                    nrm = np.ascontiguousarray(nrm, np.float32)
+                    keep.append(nrm)
+                    md[i].n_normals = nrm.shape[0]
+                    md[i].normals = nrm.ctypes.data_as(A.c_fp)
+            keep.append(md)
+            d.n_meshes = len(self.meshes)
+            d.meshes = C.cast(md, C.POINTER(A.PvxMeshDesc))
+        if self.csg is not None and len(self.csg):
+            c = np.ascontiguousarray(self.csg, np.uint32).reshape(-1, 4)
+            ch = np.ascontiguousarray(self.csg_children, np.uint32).reshape(-1, 4)
+            keep += [c, ch]
+            d.n_csg = c.shape[0]
+            d.csg = c.ctypes.data_as(C.POINTER(A.PvxCsgNode))
+            d.n_csg_children = ch.shape[0]
+            d.csg_children = ch.ctypes.data_as(C.POINTER(A.PvxObject))
+        return d, keep
+
+    def camera_struct(self, jitter=(0.5, 0.5), width=None, height=None):
+        c = self.camera
+        cam = A.PvxCamera()
+        for k in ("location", "direction", "right", "up"):
+            getattr(cam, k)[:] = [float(x) for x in c[k]]
+        cam.width = int(width or c["width"])
+        cam.height = int(height or c["height"])
+        cam.jitter_x, cam.jitter_y = jitter
+        return cam
+
+
+def camera_rays_np(cam: dict, width=None, height=None, jitter=(0.5, 0.5)):
+    """Camera rays on the host, same formula as pvx_camera_rays
+    (core/render/tracepixel.cpp:357-372, 924): (n, 6) float64 [o, d]."""
+    w = int(width or cam["width"])
+    h = int(height or cam["height"])
+    ys, xs = np.meshgrid(np.arange(h, dtype=np.float64), np.arange(w, dtype=np.float64),
+                         indexing="ij")
+    x0 = (xs + jitter[0]) / w - 0.5
+    y0 = 0.5 - (ys + jitter[1]) / h
+    d = (cam["direction"][None, None, :] + x0[..., None] * cam["right"][None, None, :]
+         + y0[..., None] * cam["up"][None, None, :])
+    d /= np.sqrt((d * d).sum(-1))[..., None]
+    o = np.broadcast_to(cam["location"], d.shape)
+    return np.ascontiguousarray(np.concatenate([o, d], -1).reshape(-1, 6))
+
+
+# --------------------------------------------------------------------------
+# 1,000 spheres (the CPU-runnable workload)
+# --------------------------------------------------------------------------
+def sphere_scene(seed: int = 0xC0FFEE + 0, n: int = 1000, width=640, height=480) -> Scene:
+    """1,000 spheres, centres U[-10,10]^3, radii U[0.2,0.8], diffuse+phong
+    finish, 5% reflective (0.3); point light (20,30,-20); camera (0,0,-30)
+    looking at the origin; max_trace_level 5."""
+    r = Rng(seed)
+    centre = r.uniform(3 * n, -10.0, 10.0).reshape(n, 3)
+    radius = r.uniform(n, 0.2, 0.8)
+    albedo = r.uniform(3 * n).reshape(n, 3)
+    reflect = np.where(r.uniform(n) < 0.05, 0.3, 0.0)
+    objects = np.zeros((n, 4), np.uint32)
+    objects[:, 0] = A.PVX_SPHERE
+    objects[:, 1] = np.arange(n)
+    return Scene(
+        name="spheres1000", objects=objects, sphere_center=centre, sphere_radius=radius,
+        camera=look_at_camera((0.0, 0.0, -30.0), (0.0, 0.0, 0.0), width, height),
+        lights=[dict(kind="point", position=np.array([20.0, 30.0, -20.0]),
+                     colour=np.ones(3))],
+        materials=dict(pigment=albedo, diffuse=np.full(n, 0.6), ambient=np.full(n, 0.1),
+                       phong=np.full(n, 0.5), phong_size=np.full(n, 40.0),
+                       reflection=reflect, ior=np.ones(n), transmit=np.zeros(n)),
+        settings=dict(max_trace_level=5, adc_bailout=1.0 / 255.0),
+    )
+
+
+# --------------------------------------------------------------------------
+# procedural icosphere mesh
+# --------------------------------------------------------------------------
+def icosphere(level: int):
+    """Unit icosphere: 20 * 4**level faces, outward (counter-clockwise seen
+    from outside) winding.  Returns (verts float64 (n,3), faces int32 (m,3))."""
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    v = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0],
+                  [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+                  [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], np.float64)
+    v /= np.linalg.norm(v, axis=1)[:, None]
+    f = np.array([[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11],
+                  [1, 5, 9], [5, 11, 4], [11, 10, 2], [10, 7, 6], [7, 1, 8],
+                  [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9],
+                  [4, 9, 5], [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1]], np.int64)
+    for _ in range(level):
+        nv = v.shape[0]
+        e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]], 0)
+        lo = e.min(1)
+        hi = e.max(1)
+        key = lo * nv + hi
+        uk, inv = np.unique(key, return_inverse=True)
+        a = uk // nv
+        b = uk % nv
+        mid = v[a] + v[b]
+        mid /= np.linalg.norm(mid, axis=1)[:, None]
+        m = inv.reshape(3, -1) + nv  # midpoint ids for edges 01, 12, 20
+        v = np.concatenate([v, mid], 0)
+        m01, m12, m20 = m[0], m[1], m[2]
+        f = np.concatenate([
+            np.stack([f[:, 0], m01, m20], 1),
+            np.stack([f[:, 1], m12, m01], 1),
+            np.stack([f[:, 2], m20, m12], 1),
+            np.stack([m01, m12, m20], 1)], 0)
+    return v, f.astype(np.int32)
+
+
+def _lattice_hash(ix, iy, iz, seed):
+    with np.errstate(over="ignore"):
+        k = (ix.astype(np.int64).astype(np.uint64) * np.uint64(0x8DA6B343)
+             ^ iy.astype(np.int64).astype(np.uint64) * np.uint64(0xD8163841)
+             ^ iz.astype(np.int64).astype(np.uint64) * np.uint64(0xCB1AB31F)
+             ^ np.uint64(seed & 0xFFFFFFFFFFFFFFFF))
+        h = _mix64(k)
+    return (h >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def value_noise(p: np.ndarray, seed: int, octaves: int = 4, base_freq: float = 2.0):
+    """Seeded multi-octave value noise in [-1, 1]: trilinear interpolation of
+    hashed lattice values with a smoothstep fade; octave k has frequency
+    base_freq * 2**k and weight 0.5**k (normalised by the weight sum)."""
+    total = np.zeros(p.shape[0])
+    wsum = 0.0
+    for k in range(octaves):
+        q = p * (base_freq * 2.0 ** k)
+        i0 = np.floor(q)
+        fr = q - i0
+        s = fr * fr * (3.0 - 2.0 * fr)
+        ix, iy, iz = (i0[:, 0].astype(np.int64), i0[:, 1].astype(np.int64),
+                      i0[:, 2].astype(np.int64))
+        acc = np.zeros(p.shape[0])
+        for dx in (0, 1):
+            wx = s[:, 0] if dx else 1.0 - s[:, 0]
+            for dy in (0, 1):
+                wy = s[:, 1] if dy else 1.0 - s[:, 1]
+                for dz in (0, 1):
+                    wz = s[:, 2] if dz else 1.0 - s[:, 2]
+                    acc += wx * wy * wz * _lattice_hash(ix + dx, iy + dy, iz + dz, seed + k)
+        w = 0.5 ** k
+        total += w * (2.0 * acc - 1.0)
+        wsum += w
+    return total / wsum
+
+
+def vertex_normals(v: np.ndarray, f: np.ndarray) -> np.ndarray:
+    """Smoothed per-vertex normals: area-weighted sum of face normals."""
+    fn = np.cross(v[f[:, 1]] - v[f[:, 0]], v[f[:, 2]] - v[f[:, 0]])
+    n = np.zeros_like(v)
+    for k in range(3):
+        np.add.at(n, f[:, k], fn)
+    n /= np.linalg.norm(n, axis=1)[:, None]
+    return n
+
+
+def displaced_icosphere(level: int, seed: int, amplitude: float = 0.15, radius: float = 1.0):
+    v, f = icosphere(level)
+    disp = value_noise(v, seed) * amplitude
+    v = v * (radius * (1.0 + disp))[:, None]
+    v32 = v.astype(np.float32)
+    n32 = vertex_normals(v32.astype(np.float64), f).astype(np.float32)
+    return v32, f, n32
+
+
+def mesh_scene(seed: int = 0xC0FFEE + 1, level: int = 8, width=1920, height=1080) -> Scene:
+    """Icosphere subdivided `level` times (20*4**level triangles; 1,310,720
+    at level 8) displaced along the normal by 4-octave value noise
+    (amplitude 0.15) with smoothed vertex normals, as mesh2 with an index
+    buffer; ground plane y = -1.25; two 4x4 area lights; 1920x1080."""
+    v, f, n = displaced_icosphere(level, seed)
+    objects = np.array([[A.PVX_MESH, 0, 0, 0], [A.PVX_PLANE, 0, 0, 0]], np.uint32)
+    return Scene(
+        name=f"icosphere_L{level}", objects=objects, meshes=[(v, f, n)],
+        plane_normal=np.array([[0.0, 1.0, 0.0]]), plane_distance=np.array([1.25]),
+        camera=look_at_camera((0.0, 0.6, -3.2), (0.0, -0.1, 0.0), width, height),
+        lights=[dict(kind="area", position=np.array([-6.0, 8.0, -6.0]),
+                     axis1=np.array([2.0, 0.0, 0.0]), axis2=np.array([0.0, 0.0, 2.0]),
+                     size=(4, 4), colour=np.full(3, 0.6)),
+                dict(kind="area", position=np.array([7.0, 6.0, -3.0]),
+                     axis1=np.array([0.0, 2.0, 0.0]), axis2=np.array([0.0, 0.0, 2.0]),
+                     size=(4, 4), colour=np.full(3, 0.5))],
+        materials=dict(pigment=np.array([[0.8, 0.6, 0.4], [0.7, 0.7, 0.7]]),
+                       diffuse=np.array([0.7, 0.7]), ambient=np.array([0.1, 0.1]),
+                       phong=np.array([0.4, 0.0]), phong_size=np.array([60.0, 40.0]),
+                       reflection=np.array([0.25, 0.1]), ior=np.ones(2),
+                       transmit=np.zeros(2)),
+        settings=dict(max_trace_level=5, adc_bailout=1.0 / 255.0),
+    )
+
+
+# --------------------------------------------------------------------------
+# transforms (core/math/matrix.cpp conventions: row vectors, matrix+inverse)
+# --------------------------------------------------------------------------
+def rotation_transform(deg):
+    """Compute_Rotation_Transform (core/math/matrix.cpp): X, then Y, then Z;
+    the inverse is the product of the transposes.  Returns (matrix, inverse)."""
+    rx, ry, rz = np.radians(np.asarray(deg, np.float64))
+    cx, sx, cy, sy, cz, sz = np.cos(rx), np.sin(rx), np.cos(ry), np.sin(ry), np.cos(rz), np.sin(rz)
+    mx = np.eye(4)
+    mx[1, 1] = mx[2, 2] = cx
+    mx[1, 2] = sx
+    mx[2, 1] = -sx
+    my = np.eye(4)
+    my[0, 0] = my[2, 2] = cy
+    my[0, 2] = -sy
+    my[2, 0] = sy
+    mz = np.eye(4)
+    mz[0, 0] = mz[1, 1] = cz
+    mz[0, 1] = sz
+    mz[1, 0] = -sz
+    m = mx @ my @ mz
+    inv = mz.T @ my.T @ mx.T
+    return m, inv
+
+
+def translation_transform(v):
+    m = np.eye(4)
+    m[3, :3] = v
+    inv = np.eye(4)
+    inv[3, :3] = -np.asarray(v, np.float64)
+    return m, inv
+
+
+def compose(a, b):
+    """Compose_Transforms(a, b): apply a, then b."""
+    return a[0] @ b[0], b[1] @ a[1]
+
+
+def _xf_row(t):
+    return np.concatenate([t[0].reshape(-1), t[1].reshape(-1)])
+
+
+# --------------------------------------------------------------------------
+# CSG and quartic mix
+# --------------------------------------------------------------------------
+_OPS = ("union", "intersection", "difference", "merge")
+
+
+def csg_scene(seed: int = 0xC0FFEE + 2, n_objects: int = 10000, width=1280, height=720,
+              extent: float | None = None, sturm: bool = True) -> Scene:
+    """`n_objects` leaves in equal thirds boxes (extents U[0.1,1]), spheres
+    and tori (R U[0.5,1.5], r = U[0.1,0.3]*R), grouped into full binary CSG
+    trees of depth 3 (8 leaves each) with random union / intersection /
+    difference / merge operators and random rotations; half of the trees are
+    glass (ior 1.5); max_trace_level 10.  Differences are stored the way the
+    parser builds them: an intersection whose children 2..n are inverted
+    (parser/parser.cpp:2223-2225, csg.cpp:629-645)."""
+    r = Rng(seed)
+    n_trees = max(1, n_objects // 8)
+    if extent is None:  # keep the object density of the 1,250-tree field
+        extent = 24.0 * (n_trees / 1250.0) ** (1.0 / 3.0)
+    kinds = np.array([0, 1, 2] * ((n_trees * 8 + 2) // 3))[: n_trees * 8]
+    perm = np.argsort(r.uniform(kinds.shape[0]))
+    kinds = kinds[perm]
+    centres = r.uniform(3 * n_trees, -extent, extent).reshape(n_trees, 3)
+    centres[:, 2] = r.uniform(n_trees, 0.0, 2.0 * extent)
+
+    sph_c, sph_r = [], []
+    box_c1, box_c2, box_tr = [], [], []
+    tor_R, tor_r, tor_tr = [], [], []
+    xforms = []
+    csg_nodes, csg_children = [], []
+
+    def leaf(kind, pos):
+        if kind == 1:
+            sph_c.append(pos)
+            sph_r.append(float(r.uniform(1, 0.2, 0.6)[0]))
+            return [A.PVX_SPHERE, len(sph_c) - 1, 0, 0]
+        rot = rotation_transform(r.uniform(3, 0.0, 360.0))
+        t = compose(rot, translation_transform(pos))
+        xforms.append(_xf_row(t))
+        if kind == 0:
+            e = r.uniform(3, 0.1, 1.0)
+            box_c1.append(-0.5 * e)
+            box_c2.append(0.5 * e)
+            box_tr.append(len(xforms) - 1)
+            return [A.PVX_BOX, len(box_c1) - 1, 0, 0]
+        R = float(r.uniform(1, 0.5, 1.5)[0])
+        tor_R.append(R)
+        tor_r.append(R * float(r.uniform(1, 0.1, 0.3)[0]))
+        tor_tr.append(len(xforms) - 1)
+        return [A.PVX_TORUS, len(tor_R) - 1, 0, 0]
+
+    # build nested trees as python structures first, then flatten
+    def invert(node):
+        """Object::Invert: every node toggles INVERTED; CSG operators change
+        union -> intersection, intersection -> merge, merge -> intersection
+        (CompoundObject::Invert, csg.cpp:629-645), children inverted too."""
+        if node[0] == "leaf":
+            node[1][2] ^= A.PVX_FLAG_INVERTED
+            return node
+        op, kids, flags = node[1], node[2], node[3]
+        kids = [invert(k) for k in kids]
+        new_op = {"union": "intersection", "intersection": "merge", "merge": "intersection"}[op]
+        return ("csg", new_op, kids, flags ^ A.PVX_FLAG_INVERTED)
+
+    def make(level, base, pos):
+        if level == 3:
+            return ("leaf", leaf(int(kinds[base]), pos + r.uniform(3, -0.6, 0.6)))
+        op = _OPS[int(r.uniform(1)[0] * 4) % 4]
+        span = 1 << (3 - level - 1)
+        kids = [make(level + 1, base, pos), make(level + 1, base + span, pos)]
+        if op == "difference":
+            kids = [kids[0]] + [invert(k) for k in kids[1:]]
+            op = "intersection"
+        return ("csg", op, kids, 0)
+
+    op_code = {"union": A.PVX_CSG_UNION, "intersection": A.PVX_CSG_INTERSECTION,
+               "merge": A.PVX_CSG_MERGE}
+
+    def flatten(node):
+        idx = len(csg_nodes)
+        csg_nodes.append(None)
+        first = len(csg_children)
+        kids = node[2]
+        csg_children.extend([None] * len(kids))
+        for k, kid in enumerate(kids):
+            if kid[0] == "leaf":
+                csg_children[first + k] = kid[1]
+            else:
+                csg_children[first + k] = [A.PVX_CSG, flatten(kid), kid[3], 0]
+        csg_nodes[idx] = [op_code[node[1]], first, len(kids), 0]
+        return idx
+
+    roots = []
+    for t in range(n_trees):
+        tree = make(0, t * 8, centres[t])
+        roots.append(flatten(tree))
+    objects = np.zeros((n_trees, 4), np.uint32)
+    objects[:, 0] = A.PVX_CSG
+    objects[:, 1] = roots
+    glass = r.uniform(n_trees) < 0.5
+    alb = r.uniform(3 * n_trees).reshape(n_trees, 3)
+    return Scene(
+        name=f"csg{n_objects}", objects=objects,
+        sphere_center=np.array(sph_c).reshape(-1, 3), sphere_radius=np.array(sph_r),
+        box_corner1=np.array(box_c1).reshape(-1, 3), box_corner2=np.array(box_c2).reshape(-1, 3),
+        box_transform=np.array(box_tr, np.int32),
+        torus_major=np.array(tor_R), torus_minor=np.array(tor_r),
+        torus_transform=np.array(tor_tr, np.int32),
+        torus_sturm=np.full(len(tor_R), 1 if sturm else 0, np.uint8),
+        transforms=np.array(xforms).reshape(-1, 32),
+        csg=np.array(csg_nodes, np.uint32).reshape(-1, 4),
+        csg_children=np.array(csg_children, np.uint32).reshape(-1, 4),
+        camera=look_at_camera((0.0, 0.0, -extent * 1.2), (0.0, 0.0, extent * 0.5), width, height),
+        lights=[dict(kind="point", position=np.array([30.0, 40.0, -30.0]), colour=np.ones(3))],
+        materials=dict(pigment=alb, diffuse=np.full(n_trees, 0.6), ambient=np.full(n_trees, 0.1),
+                       phong=np.full(n_trees, 0.3), phong_size=np.full(n_trees, 40.0),
+                       reflection=np.where(glass, 0.1, 0.0), ior=np.where(glass, 1.5, 1.0),
+                       transmit=np.where(glass, 0.9, 0.0)),
+        settings=dict(max_trace_level=10, adc_bailout=1.0 / 255.0),
+    )
