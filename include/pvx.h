@@ -241,6 +241,13 @@ int pvx_all_intersections(pvx_scene *scene, const pvx_ray *rays,
 int pvx_inside(pvx_scene *scene, const double *points, const uint32_t *obj,
                size_t n, uint8_t *out, void *stream);
 
+/* ---- utility ----------------------------------------------------------- */
+/* Plain device-to-device copy kernel (16 B per lane per step).  No reference
+ * counterpart: it is the known-size workload used to calibrate hardware byte
+ * counters (FETCH_SIZE / WRITE_SIZE) against the trace kernels.  nbytes must
+ * be a multiple of 16 and both pointers 16 B aligned. */
+int pvx_copy(void *dst, const void *src, size_t nbytes, void *stream);
+
 /* ---- camera ------------------------------------------------------------ */
 /* Perspective camera as TracePixel::CreateCameraRay builds it
  * (core/render/tracepixel.cpp:357-372, 918-926): for pixel (x, y) sampled at

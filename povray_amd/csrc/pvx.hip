@@ -428,6 +428,12 @@ __global__ void k_inside(SceneView sv, const double *pts, const uint32_t *objs, 
     out[i] = shape_inside<kCsgMaxDepth>(sv, obj_type[ob], obj_index[ob], obj_flags[ob], p) ? 1 : 0;
 }
 
+__global__ void k_copy16(uint4 *__restrict__ dst, const uint4 *__restrict__ src, size_t n16) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16;
+         i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 } // namespace
 
 // ----------------------------------------------------------------------------
@@ -647,6 +653,19 @@ extern "C" int pvx_camera_rays(const pvx_camera *cam, pvx_ray *rays, void *strea
     hipStream_t st = static_cast<hipStream_t>(stream);
     dim3 grid((cam->width + 255) / 256, cam->height);
     k_camera_rays<<<grid, 256, 0, st>>>(*cam, rays);
+    HIP_TRY(hipGetLastError());
+    return PVX_OK;
+}
+
+extern "C" int pvx_copy(void *dst, const void *src, size_t nbytes, void *stream) {
+    if (nbytes == 0) return PVX_OK;
+    if (!dst || !src) return fail(PVX_EINVAL, "null pointer");
+    if ((nbytes & 15) || ((uintptr_t)dst & 15) || ((uintptr_t)src & 15))
+        return fail(PVX_EINVAL, "pvx_copy needs 16 B multiples and alignment");
+    size_t n16 = nbytes / 16;
+    uint32_t blocks = (uint32_t)std::min<size_t>((n16 + 255) / 256, 65536);
+    k_copy16<<<blocks, 256, 0, static_cast<hipStream_t>(stream)>>>(
+        static_cast<uint4 *>(dst), static_cast<const uint4 *>(src), n16);
     HIP_TRY(hipGetLastError());
     return PVX_OK;
 }

@@ -49,6 +49,7 @@ def lib():
         L.pvx_all_intersections.argtypes = [vp, vp, vp, sz, u32, vp, vp, vp, vp]
         L.pvx_inside.argtypes = [vp, vp, vp, sz, vp, vp]
         L.pvx_camera_rays.argtypes = [C.POINTER(A.PvxCamera), vp, vp]
+        L.pvx_copy.argtypes = [vp, vp, sz, vp]
         _L = L
     return _L
 
@@ -57,7 +58,7 @@ EXPORTED_SYMBOLS = (
     "pvx_last_error", "pvx_init", "pvx_scene_create", "pvx_scene_destroy",
     "pvx_scene_get_stats", "pvx_trace_closest", "pvx_trace_closest_counted",
     "pvx_all_intersections", "pvx_inside",
-    "pvx_camera_rays",
+    "pvx_camera_rays", "pvx_copy",
 )
 
 
@@ -175,3 +176,8 @@ def camera_rays(scene, width=None, height=None, jitter=(0.5, 0.5), out=None, str
     _check(lib().pvx_camera_rays(C.byref(cam), out.value, stream.h if stream else None),
            "pvx_camera_rays")
     return out, n
+
+
+def copy(dst, src, nbytes, stream=None):
+    """Device copy kernel (byte-counter calibration workload)."""
+    _check(lib().pvx_copy(_ptr(dst), _ptr(src), nbytes, stream.h if stream else None), "pvx_copy")

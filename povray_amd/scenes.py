@@ -323,7 +323,7 @@ def mesh_scene(seed: int = 0xC0FFEE + 1, level: int = 8, width=1920, height=1080
     return Scene(
         name=f"icosphere_L{level}", objects=objects, meshes=[(v, f, n)],
         plane_normal=np.array([[0.0, 1.0, 0.0]]), plane_distance=np.array([1.25]),
-        camera=look_at_camera((0.0, 0.6, -3.2), (0.0, -0.1, 0.0), width, height),
+        camera=look_at_camera((0.0, 0.3, -1.95), (0.0, -0.05, 0.0), width, height),
         lights=[dict(kind="area", position=np.array([-6.0, 8.0, -6.0]),
                      axis1=np.array([2.0, 0.0, 0.0]), axis2=np.array([0.0, 0.0, 2.0]),
                      size=(4, 4), colour=np.full(3, 0.6)),
