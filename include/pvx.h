@@ -241,13 +241,6 @@ int pvx_all_intersections(pvx_scene *scene, const pvx_ray *rays,
 int pvx_inside(pvx_scene *scene, const double *points, const uint32_t *obj,
                size_t n, uint8_t *out, void *stream);
 
-/* ---- utility ----------------------------------------------------------- */
-/* Plain device-to-device copy kernel (16 B per lane per step).  No reference
- * counterpart: it is the known-size workload used to calibrate hardware byte
- * counters (FETCH_SIZE / WRITE_SIZE) against the trace kernels.  nbytes must
- * be a multiple of 16 and both pointers 16 B aligned. */
-int pvx_copy(void *dst, const void *src, size_t nbytes, void *stream);
-
 /* ---- camera ------------------------------------------------------------ */
 /* Perspective camera as TracePixel::CreateCameraRay builds it
  * (core/render/tracepixel.cpp:357-372, 918-926): for pixel (x, y) sampled at
@@ -266,6 +259,86 @@ typedef struct pvx_camera {
 
 /* Writes width*height rays (row-major, y down) into device buffer `rays`. */
 int pvx_camera_rays(const pvx_camera *cam, pvx_ray *rays, void *stream);
+
+/* ---- shading (plain finishes) ------------------------------------------ */
+/* Surface description of one top-level object: pigment { rgb filter
+ * transmit }, finish { ambient diffuse brilliance phong phong_size
+ * reflection }, interior { ior } (core/material/texture.h FINISH, pigment,
+ * interior; defaults as Create_Finish, core/material/texture.cpp:236-270). */
+typedef struct pvx_material {
+    float pigment[3];
+    float filter;            /* pigment filter (0 = opaque)                   */
+    float transmit;          /* pigment transmit (0 = opaque)                 */
+    float ambient[3];        /* finish ambient, times the global ambient_light */
+    float diffuse;           /* default 0.6                                  */
+    float brilliance;        /* default 1.0                                  */
+    float phong;             /* default 0.0                                  */
+    float phong_size;        /* default 40.0                                 */
+    float reflection_min[3]; /* `reflection r` sets min = max = r            */
+    float reflection_max[3];
+    float reflection_falloff;/* default 1.0                                  */
+    float ior;               /* interior ior; 0 = no interior                */
+} pvx_material;
+
+enum { PVX_LIGHT_POINT = 1, PVX_LIGHT_AREA = 2 };
+
+/* light_source { position, color [area_light axis1, axis2, size1, size2] },
+ * no jitter, default adaptive (core/lighting/lightsource.cpp:375-400). */
+typedef struct pvx_light {
+    uint32_t kind;
+    uint32_t size1, size2;   /* area light grid (>= 1)                       */
+    uint32_t pad;
+    double position[3];
+    double axis1[3];
+    double axis2[3];
+    float colour[3];
+    float pad2;
+} pvx_light;
+
+typedef struct pvx_shading_desc {
+    uint32_t n_materials;          /* == scene objects                      */
+    const pvx_material *materials; /* host memory, copied                   */
+    uint32_t n_lights;
+    const pvx_light *lights;
+    float ambient_light[3];        /* global_settings ambient_light (1,1,1)  */
+    float background[3];           /* background colour (black)              */
+    uint32_t max_trace_level;      /* default 5                              */
+    float adc_bailout;             /* default 1/255                          */
+} pvx_shading_desc;
+
+typedef struct pvx_render_stats {
+    uint64_t closest_rays;   /* camera + reflection + refraction rays traced  */
+    uint64_t shadow_rays;    /* light-visibility rays traced                  */
+    uint32_t bounces;        /* wavefront iterations run                      */
+    uint32_t pad;
+    double trace_ms;         /* GPU time in closest-hit launches              */
+    double shadow_ms;        /* GPU time in shadow-ray launches               */
+    double total_ms;         /* GPU time of the whole frame                   */
+} pvx_render_stats;
+
+/* Attach surface and light descriptions to a scene (replaces the textures,
+ * interiors and light list the parser attaches to SceneData). */
+int pvx_scene_set_shading(pvx_scene *scene, const pvx_shading_desc *desc);
+
+/* Render one frame, 1 sample per pixel at the pixel centre, as
+ * TracePixel -> Trace::TraceRay -> ComputeTextureColour ->
+ * ComputeLightedTexture (core/render/trace.cpp:135-228, 457-586, 739-1179)
+ * for opaque plain finishes: classic ambient, diffuse and Phong per light
+ * (ComputeOneDiffuseLight, trace.cpp:1637-1729) with shadow rays
+ * (TracePointLightShadowRay / TraceAreaLightShadowRay, trace.cpp:1951-2272),
+ * reflection rays (ComputeReflection, trace.cpp:1264-1321) up to
+ * max_trace_level, ADC bailout.  Runs as a wavefront: camera rays, then per
+ * bounce one closest-hit launch, one shade launch, one shadow-ray launch.
+ * rgb: device buffer of 3*width*height floats (linear RGB, row-major). */
+int pvx_render(pvx_scene *scene, const pvx_camera *cam, float *rgb, pvx_render_stats *stats,
+               void *stream);
+
+/* ---- utility ----------------------------------------------------------- */
+/* Plain device-to-device copy kernel (16 B per lane per step).  No reference
+ * counterpart: it is the known-size workload used to calibrate hardware byte
+ * counters (FETCH_SIZE / WRITE_SIZE) against the trace kernels.  nbytes must
+ * be a multiple of 16 and both pointers 16 B aligned. */
+int pvx_copy(void *dst, const void *src, size_t nbytes, void *stream);
 
 #ifdef __cplusplus
 }

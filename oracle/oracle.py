@@ -37,6 +37,8 @@ def lib():
         L.orc_mesh_triangle.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p]
         L.orc_mesh_normal.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_size_t,
                                       C.c_void_p]
+        L.orc_render.argtypes = [C.c_void_p, C.POINTER(A.PvxShadingDesc), C.POINTER(A.PvxCamera),
+                                 C.c_void_p, C.c_int, C.c_void_p, C.c_int]
         L.orc_solve_polynomial.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_double, C.c_void_p]
         _LIB = L
     return _LIB
@@ -105,6 +107,17 @@ class OracleScene:
         out = np.zeros(10)
         lib().orc_mesh_triangle(self.h, mesh, tri, _p(out))
         return out
+
+    def render(self, scene, width=None, height=None, threads=os.cpu_count() or 1, method=1):
+        """Plain-finish TraceRay over the camera; returns (h, w, 3) float32
+        linear RGB and (closest rays, shadow rays) counters."""
+        sd, keep = scene.shading()
+        cam = scene.camera_struct(width=width, height=height)
+        out = np.zeros((cam.height, cam.width, 3), np.float32)
+        cnt = np.zeros(2, np.uint64)
+        lib().orc_render(self.h, C.byref(sd), C.byref(cam), _p(out), threads, _p(cnt), method)
+        del keep
+        return out, (int(cnt[0]), int(cnt[1]))
 
     def mesh_normal(self, mesh, tri, pts):
         tri = np.ascontiguousarray(tri, np.int32)

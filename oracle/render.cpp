@@ -1,0 +1,251 @@
+// FP64 CPU ORACLE — TEST INFRASTRUCTURE ONLY (see oracle.hpp).
+// Restatement of the plain-finish shading path for opaque surfaces:
+// Trace::TraceRay (core/render/trace.cpp:135-228), ComputeTextureColour
+// (457-586), ComputeLightedTexture (739-1179, single layer, no radiosity,
+// no media, no photons), ComputeOneDiffuseLight (1637-1729),
+// ComputeOneWhiteLightRay (2710-2767), TracePointLightShadowRay /
+// TraceAreaLightSubsetShadowRay (1951-2272, without the order-dependent
+// shadow cache), ComputeDiffuseColour (2441-2484), ComputePhongColour
+// (2520-2558), ComputeReflectivity (2627-2656), ComputeReflection (1264-1321).
+// Colours are FP32 (MathColour), geometry FP64.
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <thread>
+
+#include "oracle.hpp"
+
+namespace orc {
+
+bool scene_find_intersection(const Scene &s, const Ray &r, double max_depth, uint32_t kind,
+                             Isect &best, int32_t &best_obj, int method);
+
+namespace {
+
+const double kShadowTolerance = 1.0e-3; // trace.cpp:81
+
+struct RGB {
+    float r = 0, g = 0, b = 0;
+};
+inline RGB operator+(RGB a, RGB b) { return {a.r + b.r, a.g + b.g, a.b + b.b}; }
+inline RGB operator*(RGB a, RGB b) { return {a.r * b.r, a.g * b.g, a.b * b.b}; }
+inline RGB scale(RGB a, double s) {
+    return {(float)(s * a.r), (float)(s * a.g), (float)(s * a.b)};
+}
+inline RGB rgb3(const float *p) { return {p[0], p[1], p[2]}; }
+inline bool near_zero(RGB c) {
+    return std::fabs(c.r) < kEpsilon && std::fabs(c.g) < kEpsilon && std::fabs(c.b) < kEpsilon;
+}
+inline float max3f(const float *p) { return std::max(p[0], std::max(p[1], p[2])); }
+
+struct Shading {
+    int method = 1;
+    std::vector<pvx_material> mats;
+    std::vector<pvx_light> lights;
+    float amb[3], bg[3];
+    uint32_t max_level;
+    float bailout;
+};
+
+struct Counters {
+    std::atomic<uint64_t> closest{0}, shadow{0};
+};
+
+V3 xf_normal_out(const Transform *tr, V3 n) { // MTransNormal + normalize
+    if (!tr) return n;
+    return normalized(minv_trans_normal(tr->inv, n));
+}
+
+// Object::Normal for a hit (raw, not flipped).
+V3 shape_normal(const Scene &s, uint32_t type, uint32_t index, const Isect &h, V3 ip) {
+    switch (type) {
+    case PVX_SPHERE: // Sphere::Normal (sphere.cpp), spherical mode
+        return (ip - s.sph_c[index]) / s.sph_r[index];
+    case PVX_PLANE:
+        return s.pl_n[index];
+    case PVX_MESH:
+        return mesh_normal(s.meshes[index], h.prim, ip);
+    case PVX_BOX: { // Box::Normal (box.cpp:600-625)
+        static const double tbl[7][3] = {{0, 0, 0}, {-1, 0, 0}, {1, 0, 0}, {0, -1, 0},
+                                         {0, 1, 0},  {0, 0, -1}, {0, 0, 1}};
+        int side = h.side >= 1 && h.side <= 6 ? h.side : 0;
+        V3 n(tbl[side][0], tbl[side][1], tbl[side][2]);
+        return xf_normal_out(s.tr(s.box_tr[index]), n);
+    }
+    case PVX_TORUS: { // Torus::Normal (torus.cpp:458-490)
+        const Transform *tr = s.tr(s.tor_tr[index]);
+        static const Transform id = {{{1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}, {0, 0, 0, 1}},
+                                     {{1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}, {0, 0, 0, 1}}};
+        if (!tr) tr = &id;
+        V3 P = mtrans_point(tr->inv, ip);
+        double dist = std::sqrt(P.x * P.x + P.z * P.z);
+        V3 M(0, 0, 0);
+        if (dist > kEpsilon) M = V3(s.tor_R[index] * P.x / dist, 0.0, s.tor_R[index] * P.z / dist);
+        return normalized(minv_trans_normal(tr->inv, P - M));
+    }
+    case PVX_CSG: { // the leaf that produced the hit
+        const pvx_object &c = s.csg_children[h.leaf];
+        Isect leaf = h;
+        return shape_normal(s, c.type, c.index, leaf, ip);
+    }
+    }
+    return V3(0, 1, 0);
+}
+
+bool blocked(const Scene &s, const Shading &sh, Counters &cnt, V3 ip, V3 target) {
+    Ray lr;
+    lr.o = ip;
+    V3 dir = target - ip;
+    double lsd = length(dir);
+    lr.d = dir / lsd;
+    Isect h;
+    int32_t obj;
+    cnt.shadow++;
+    bool found = scene_find_intersection(s, lr, lsd, PVX_RAY_SHADOW, h, obj, sh.method);
+    return found && h.depth < lsd - kShadowTolerance && h.depth > kShadowTolerance;
+}
+
+RGB trace_ray(const Scene &s, const Shading &sh, Counters &cnt, const Ray &ray, float weight,
+              uint32_t level, uint32_t kind) {
+    if (level >= sh.max_level || weight < sh.bailout) return RGB{};
+    Isect best;
+    int32_t obj;
+    cnt.closest++;
+    if (!scene_find_intersection(s, ray, 0.0, kind, best, obj, sh.method)) return rgb3(sh.bg);
+    level++;
+    const pvx_material &m = sh.mats[obj];
+    V3 ip = ray.eval(best.depth);
+    const Object &o = s.objects[obj];
+    V3 n = shape_normal(s, o.type, o.index, best, ip);
+    if (dot(n, ray.d) > 0.0) n = n * -1.0;
+    RGB pig = rgb3(m.pigment);
+    double att = 1.0 - m.filter - m.transmit;
+    // classic ambient: Ambient * ambient_light (emission 0), times opacity
+    RGB emission = rgb3(m.ambient) * rgb3(sh.amb);
+    RGB col = scale(pig * emission, att);
+    for (const pvx_light &L : sh.lights) {
+        V3 C(L.position[0], L.position[1], L.position[2]);
+        V3 dir = C - ip;
+        double lsd = length(dir);
+        dir = dir / lsd;
+        RGB lc = rgb3(L.colour); // Attenuate_Light = 1 without fade
+        if (dot(n, dir) < kEpsilon) continue;
+        if (L.kind == PVX_LIGHT_AREA && (L.size1 > 1 || L.size2 > 1)) {
+            V3 a1(L.axis1[0], L.axis1[1], L.axis1[2]), a2(L.axis2[0], L.axis2[1], L.axis2[2]);
+            // default adaptive 100: every grid point is sampled; the
+            // quadtree average equals the mean over the grid for 2^k+1 and
+            // 4x4 grids alike, computed here as nested corner averages.
+            int n1 = (int)L.size1, n2 = (int)L.size2;
+            std::vector<float> vis((size_t)n1 * n2);
+            for (int u = 0; u < n1; u++)
+                for (int v = 0; v < n2; v++) {
+                    V3 j1 = n1 > 1 ? a1 * ((double)u / (double)(n1 - 1) - 0.5) : V3(0, 0, 0);
+                    V3 j2 = n2 > 1 ? a2 * ((double)v / (double)(n2 - 1) - 0.5) : V3(0, 0, 0);
+                    vis[(size_t)u * n2 + v] = blocked(s, sh, cnt, ip, C + (j1 + j2)) ? 0.0f : 1.0f;
+                }
+            // TraceAreaLightSubsetShadowRay recursion (trace.cpp:2131-2272)
+            struct R {
+                static float sub(const std::vector<float> &g, int n2, int u1, int v1, int u2, int v2) {
+                    float c[4] = {g[(size_t)u1 * n2 + v1], g[(size_t)u2 * n2 + v1],
+                                  g[(size_t)u1 * n2 + v2], g[(size_t)u2 * n2 + v2]};
+                    if ((u2 - u1 > 1) || (v2 - v1 > 1)) {
+                        int mu0 = (int)std::floor((u1 + u2) / 2.0), mu1 = (int)std::ceil((u1 + u2) / 2.0);
+                        int mv0 = (int)std::floor((v1 + v2) / 2.0), mv1 = (int)std::ceil((v1 + v2) / 2.0);
+                        c[0] = sub(g, n2, u1, v1, mu0, mv0);
+                        c[1] = sub(g, n2, mu1, v1, u2, mv0);
+                        c[2] = sub(g, n2, u1, mv1, mu0, v2);
+                        c[3] = sub(g, n2, mu1, mv1, u2, v2);
+                    }
+                    return (c[0] + c[1] + c[2] + c[3]) * 0.25f;
+                }
+            };
+            float f = R::sub(vis, n2, 0, 0, n1 - 1, n2 - 1);
+            lc = RGB{lc.r * f, lc.g * f, lc.b * f};
+        } else if (blocked(s, sh, cnt, ip, C)) {
+            lc = RGB{};
+        }
+        if (near_zero(lc)) continue;
+        double cosi = dot(n, dir);
+        double intensity = (m.brilliance != 1.0f) ? std::pow(std::fabs(cosi), (double)m.brilliance)
+                                                  : std::fabs(cosi);
+        intensity *= (double)m.diffuse * att;
+        if (m.diffuse > 0.0f) col = col + scale(pig * lc, intensity);
+        if (m.phong > 0.0f && kind != PVX_RAY_RADIOSITY) {
+            double c2 = -2.0 * dot(ray.d, n);
+            V3 rd = ray.d + n * c2;
+            double ca = dot(rd, dir);
+            if (ca > 0.0 && (m.phong_size < 60 || ca > 0.0008))
+                col = col + scale(lc, (double)m.phong * std::pow(ca, (double)m.phong_size));
+        }
+    }
+    // reflection: ComputeReflectivity (no fresnel) + ComputeReflection
+    double cos_inc = -dot(ray.d, n);
+    double w = (double)weight * std::max(max3f(m.reflection_max), max3f(m.reflection_min));
+    double frac = std::fabs((double)m.reflection_falloff - 1.0) > kEpsilon
+                      ? std::pow(1.0 - cos_inc, (double)m.reflection_falloff)
+                      : 1.0 - cos_inc;
+    RGB refl;
+    if (std::fabs(frac) < kEpsilon)
+        refl = rgb3(m.reflection_min);
+    else if (std::fabs(frac - 1.0) < kEpsilon)
+        refl = rgb3(m.reflection_max);
+    else
+        refl = scale(rgb3(m.reflection_max), frac) + scale(rgb3(m.reflection_min), 1.0 - frac);
+    if (!(refl.r == 0.0f && refl.g == 0.0f && refl.b == 0.0f)) {
+        Ray nr;
+        double k = -2.0 * dot(ray.d, n);
+        nr.d = normalized(ray.d + n * k);
+        nr.o = ip;
+        RGB rc = trace_ray(s, sh, cnt, nr, (float)w, level, PVX_RAY_REFLECTION);
+        col = col + refl * rc;
+    }
+    return col;
+}
+
+} // namespace
+} // namespace orc
+
+using namespace orc;
+
+extern "C" int orc_render(void *sp, const pvx_shading_desc *sd, const pvx_camera *cam, float *rgb,
+                          int threads, uint64_t *counters, int method) {
+    const Scene &s = *static_cast<Scene *>(sp);
+    Shading sh;
+    sh.method = method;
+    sh.mats.assign(sd->materials, sd->materials + sd->n_materials);
+    sh.lights.assign(sd->lights, sd->lights + sd->n_lights);
+    std::memcpy(sh.amb, sd->ambient_light, sizeof sh.amb);
+    std::memcpy(sh.bg, sd->background, sizeof sh.bg);
+    sh.max_level = sd->max_trace_level;
+    sh.bailout = sd->adc_bailout;
+    Counters cnt;
+    size_t npix = (size_t)cam->width * cam->height;
+    std::atomic<size_t> next(0);
+    auto work = [&]() {
+        for (;;) {
+            size_t b = next.fetch_add(64);
+            if (b >= npix) break;
+            for (size_t i = b; i < std::min(npix, b + 64); i++) {
+                uint32_t x = (uint32_t)(i % cam->width), y = (uint32_t)(i / cam->width);
+                double x0 = ((double)x + cam->jitter_x) / (double)cam->width - 0.5;
+                double y0 = 0.5 - ((double)y + cam->jitter_y) / (double)cam->height;
+                V3 d;
+                for (int k = 0; k < 3; k++)
+                    d[k] = cam->direction[k] + x0 * cam->right[k] + y0 * cam->up[k];
+                Ray r{V3(cam->location[0], cam->location[1], cam->location[2]), normalized(d)};
+                RGB c = trace_ray(s, sh, cnt, r, 1.0f, 0, PVX_RAY_IMAGE);
+                rgb[3 * i] = c.r;
+                rgb[3 * i + 1] = c.g;
+                rgb[3 * i + 2] = c.b;
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 0; t < std::max(1, threads); t++) pool.emplace_back(work);
+    for (auto &t : pool) t.join();
+    if (counters) {
+        counters[0] = cnt.closest;
+        counters[1] = cnt.shadow;
+    }
+    return 0;
+}

@@ -383,6 +383,11 @@ static bool find_intersection(const Scene &s, const Ray &r, double max_depth, ui
     return found;
 }
 
+bool scene_find_intersection(const Scene &s, const Ray &r, double max_depth, uint32_t kind,
+                             Isect &best, int32_t &best_obj, int method) {
+    return find_intersection(s, r, max_depth, kind, method, best, best_obj);
+}
+
 template <class F>
 static void parallel_for(size_t n, int threads, F f) {
     if (threads <= 1 || n < 64) {

@@ -89,6 +89,36 @@ class PvxCamera(C.Structure):
                 ("jitter_x", C.c_double), ("jitter_y", C.c_double)]
 
 
+PVX_LIGHT_POINT, PVX_LIGHT_AREA = 1, 2
+
+
+class PvxMaterial(C.Structure):
+    _fields_ = [("pigment", C.c_float * 3), ("filter", C.c_float), ("transmit", C.c_float),
+                ("ambient", C.c_float * 3), ("diffuse", C.c_float), ("brilliance", C.c_float),
+                ("phong", C.c_float), ("phong_size", C.c_float),
+                ("reflection_min", C.c_float * 3), ("reflection_max", C.c_float * 3),
+                ("reflection_falloff", C.c_float), ("ior", C.c_float)]
+
+
+class PvxLight(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("size1", C.c_uint32), ("size2", C.c_uint32),
+                ("pad", C.c_uint32), ("position", C.c_double * 3), ("axis1", C.c_double * 3),
+                ("axis2", C.c_double * 3), ("colour", C.c_float * 3), ("pad2", C.c_float)]
+
+
+class PvxShadingDesc(C.Structure):
+    _fields_ = [("n_materials", C.c_uint32), ("materials", C.POINTER(PvxMaterial)),
+                ("n_lights", C.c_uint32), ("lights", C.POINTER(PvxLight)),
+                ("ambient_light", C.c_float * 3), ("background", C.c_float * 3),
+                ("max_trace_level", C.c_uint32), ("adc_bailout", C.c_float)]
+
+
+class PvxRenderStats(C.Structure):
+    _fields_ = [("closest_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
+                ("bounces", C.c_uint32), ("pad", C.c_uint32), ("trace_ms", C.c_double),
+                ("shadow_ms", C.c_double), ("total_ms", C.c_double)]
+
+
 assert C.sizeof(PvxObject) == 16
 assert C.sizeof(PvxRay) == 48
 assert C.sizeof(PvxHit) == 16

@@ -20,29 +20,21 @@
 #include <vector>
 
 #include "../../include/pvx.h"
+#include "internal.h"
 #include "scene_prep.h"
 #include "shapes_device.h"
 
 using namespace pvx;
 
-// ----------------------------------------------------------------------------
-// errors
-// ----------------------------------------------------------------------------
-static thread_local std::string g_err;
-
-static int fail(int code, const std::string &msg) {
+namespace pvx {
+thread_local std::string g_err;
+int fail(int code, const std::string &msg) {
     g_err = msg;
     return code;
 }
+} // namespace pvx
 
-#define HIP_TRY(expr)                                                                       \
-    do {                                                                                    \
-        hipError_t e_ = (expr);                                                             \
-        if (e_ != hipSuccess)                                                               \
-            return fail(PVX_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));    \
-    } while (0)
-
-extern "C" const char *pvx_last_error(void) { return g_err.c_str(); }
+extern "C" const char *pvx_last_error(void) { return pvx::g_err.c_str(); }
 
 // ----------------------------------------------------------------------------
 // kernels
@@ -436,21 +428,6 @@ __global__ void k_copy16(uint4 *__restrict__ dst, const uint4 *__restrict__ src,
 
 } // namespace
 
-// ----------------------------------------------------------------------------
-// scene object
-// ----------------------------------------------------------------------------
-struct pvx_scene {
-    int device = 0;
-    HostScene host;
-    SceneView view{};
-    std::vector<void *> allocs;
-    uint32_t *d_obj_type = nullptr, *d_obj_index = nullptr;
-    int32_t *d_spill = nullptr;
-    uint32_t grid = 0;
-    uint64_t device_bytes = 0;
-    uint32_t n_objects = 0;
-};
-
 template <class T>
 static int upload(pvx_scene *s, const std::vector<T> &v, const T **dst) {
     *dst = nullptr;
@@ -467,6 +444,12 @@ static int upload(pvx_scene *s, const std::vector<T> &v, const T **dst) {
 static void scene_free(pvx_scene *s) {
     for (void *p : s->allocs) (void)hipFree(p);
     s->allocs.clear();
+    for (void *p : s->render_allocs) (void)hipFree(p);
+    s->render_allocs.clear();
+    if (s->shading.d_mats) (void)hipFree(s->shading.d_mats);
+    if (s->shading.d_lights) (void)hipFree(s->shading.d_lights);
+    s->shading.d_mats = nullptr;
+    s->shading.d_lights = nullptr;
 }
 
 extern "C" int pvx_init(int device) {
@@ -519,6 +502,10 @@ extern "C" int pvx_scene_create(const pvx_scene_desc *desc, const pvx_build_opts
     UP(h.csg_children, csg_children);
     UP(h.csg_leaves, csg_leaves);
     UP(h.csg_leaf_range, csg_leaf_range);
+    UP(h.shade_tris, shade_tris);
+    UP(h.obj_tri_base, obj_tri_base);
+    UP(h.mesh_vtx, mesh_vtx);
+    UP(h.mesh_nrm, mesh_nrm);
 #undef UP
     const uint32_t *tmp = nullptr;
     if ((r = upload(s, h.obj_type, &tmp)) != PVX_OK) { scene_free(s); delete s; return r; }
@@ -600,6 +587,11 @@ static int launch_closest(pvx_scene *s, const pvx_ray *rays, const double *max_d
     }
     HIP_TRY(hipGetLastError());
     return PVX_OK;
+}
+
+int pvx::trace_closest_internal(pvx_scene *s, const pvx_ray *rays, const double *max_depth,
+                                uint32_t ray_kind, size_t n, pvx_hit *hits, hipStream_t stream) {
+    return launch_closest<false>(s, rays, max_depth, ray_kind, n, hits, nullptr, nullptr, stream);
 }
 
 extern "C" int pvx_trace_closest(pvx_scene *s, const pvx_ray *rays, const double *max_depth,

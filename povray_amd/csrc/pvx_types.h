@@ -109,6 +109,37 @@ struct CsgChild {
     uint32_t node; // CSG node that owns this slot
 };
 
+// Smooth-normal data of a mesh triangle (Mesh_Triangle_Struct fields used by
+// Mesh::Normal / Smooth_Mesh_Normal, core/shape/mesh.cpp:380-420, 2366-2394),
+// indices into the scene-wide vertex / normal arrays.
+struct ShadeTri {
+    float perp[3];
+    int32_t p1, p2, p3;   // post-swap vertex indices
+    int32_t n1, n2, n3;   // post-swap smooth normal indices
+    int32_t fn;           // face normal index
+    uint32_t vaxis_smooth; // vAxis | smooth << 4
+};
+
+// Device copies of the shading description.
+struct MatRec {
+    float pigment[3];
+    float att;            // opacity 1 - filter - transmit
+    float emission[3];    // ambient * ambient_light
+    float diffuse, brilliance, phong, phong_size;
+    float rmin[3], rmax[3];
+    float falloff;
+    float rweight;        // max(WeightMax(rmax), WeightMax(rmin))
+    uint32_t reflective;  // any reflection component non-zero
+};
+struct LightRec {
+    double pos[3];
+    double axis1[3];
+    double axis2[3];
+    float colour[3];
+    uint32_t n1, n2;      // grid (1x1 for point lights)
+    uint32_t first;       // first shadow-sample slot of this light within a hit
+};
+
 // An object tested outside the BVH.
 struct InfRec {
     uint32_t kind; // kPrimPlane / kPrimCsg
@@ -132,6 +163,10 @@ struct SceneView {
     const CsgChild *csg_children;
     const uint32_t *csg_leaves;      // leaf slots of each CSG root, contiguous
     const uint32_t *csg_leaf_range;  // 2 per CSG node: [begin, end) into csg_leaves
+    const ShadeTri *shade_tris;      // per mesh triangle (global index)
+    const uint32_t *obj_tri_base;    // per object: first ShadeTri of a mesh object
+    const float *mesh_vtx;           // 3 per vertex (all meshes)
+    const float *mesh_nrm;           // 3 per normal (all meshes: vertex + face normals)
     int32_t root;      // root link (node index or leaf link); INT32_MIN = empty
     uint32_t n_inf;
     uint32_t any_flags;

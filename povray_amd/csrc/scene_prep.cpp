@@ -98,6 +98,27 @@ inline int max3_axis(double x, double y, double z) { // mesh.cpp:94 max3_coordin
     return (x > y) ? ((x > z) ? 0 : 2) : ((y > z) ? 1 : 2);
 }
 
+// Mesh::compute_smooth_triangle (mesh.cpp:983-1008) on post-swap P1, P2, P3.
+void smooth_triangle(const double *P1, const double *P2, const double *P3, ShadeTri &t) {
+    double d32[3] = {P3[0] - P2[0], P3[1] - P2[1], P3[2] - P2[2]};
+    int vaxis = max3_axis(std::fabs(d32[0]), std::fabs(d32[1]), std::fabs(d32[2]));
+    double v1[3] = {P2[0] - P3[0], P2[1] - P3[1], P2[2] - P3[2]};
+    double l1 = std::sqrt(v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2]);
+    if (l1 != 0.0)
+        for (int k = 0; k < 3; k++) v1[k] = v1[k] / l1;
+    double v2[3] = {P1[0] - P3[0], P1[1] - P3[1], P1[2] - P3[2]};
+    double proj = (v2[0] * v1[0]) + (v2[1] * v1[1]) + (v2[2] * v1[2]);
+    double pp[3];
+    for (int k = 0; k < 3; k++) pp[k] = v1[k] * proj - v2[k];
+    double lp = std::sqrt(pp[0] * pp[0] + pp[1] * pp[1] + pp[2] * pp[2]);
+    float pf[3];
+    for (int k = 0; k < 3; k++) pf[k] = (float)(lp != 0.0 ? pp[k] / lp : pp[k]);
+    double uden = -((v2[0] * (double)pf[0]) + (v2[1] * (double)pf[1]) + (v2[2] * (double)pf[2]));
+    float udf = (float)uden;
+    for (int k = 0; k < 3; k++) t.perp[k] = pf[k] / udf;
+    t.vaxis_smooth = (uint32_t)vaxis;
+}
+
 } // namespace
 
 int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene &out,
@@ -344,6 +365,7 @@ int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene
     };
     out.obj_flags.resize(d.n_objects);
     out.obj_type.resize(d.n_objects);
+    out.obj_tri_base.assign(d.n_objects, 0);
     for (uint32_t o = 0; o < d.n_objects; o++) {
         const pvx_object &ob = d.objects[o];
         if (!check_ref(ob.type, ob.index)) return PVX_EINVAL;
@@ -404,6 +426,16 @@ int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene
                 err = "mesh without vertices or indices";
                 return PVX_EINVAL;
             }
+            // shading data: vertices, vertex normals then face normals
+            // (Normals[] layout of Parse_Mesh2, parser.cpp:4512-4565)
+            int32_t vbase = (int32_t)(out.mesh_vtx.size() / 3);
+            int32_t nbase = (int32_t)(out.mesh_nrm.size() / 3);
+            uint32_t nn = (m.normals && m.n_normals == m.n_vertices) ? m.n_normals : 0;
+            out.mesh_vtx.insert(out.mesh_vtx.end(), m.vertices, m.vertices + 3 * (size_t)m.n_vertices);
+            if (nn) out.mesh_nrm.insert(out.mesh_nrm.end(), m.normals, m.normals + 3 * (size_t)nn);
+            out.mesh_nrm.resize(out.mesh_nrm.size() + 3 * (size_t)m.n_triangles, 0.0f);
+            out.obj_tri_base[o] = (uint32_t)out.shade_tris.size();
+            out.shade_tris.resize(out.shade_tris.size() + m.n_triangles);
             for (uint32_t t = 0; t < m.n_triangles; t++) {
                 int32_t ia = m.indices[3 * t], ib = m.indices[3 * t + 1], ic = m.indices[3 * t + 2];
                 if (ia < 0 || ib < 0 || ic < 0 || (uint32_t)ia >= m.n_vertices ||
@@ -421,6 +453,9 @@ int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene
                 double S[3] = {(V2[1] * V1[2]) - (V2[2] * V1[1]), (V2[2] * V1[0]) - (V2[0] * V1[2]),
                                (V2[0] * V1[1]) - (V2[1] * V1[0])};
                 double L = std::sqrt(S[0] * S[0] + S[1] * S[1] + S[2] * S[2]);
+                ShadeTri &st = out.shade_tris[out.obj_tri_base[o] + t];
+                std::memset(&st, 0, sizeof st);
+                st.fn = nbase + (int32_t)nn + (int32_t)t;
                 if (L == 0.0) continue; // zero normal: never hit
                 for (int k = 0; k < 3; k++) S[k] = S[k] / L;
                 float dist = (float)((S[0] * P1[0]) + (S[1] * P1[1]) + (S[2] * P1[2]));
@@ -429,6 +464,30 @@ int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene
                 int a = axis == 0 ? 1 : 0, b = axis == 2 ? 1 : 2;
                 bool swap = (P2[a] - P3[a]) * (P2[b] - P1[b]) < (P2[b] - P3[b]) * (P2[a] - P1[a]);
                 const float *q1 = swap ? fb : fa, *q2 = swap ? fa : fb;
+                for (int k = 0; k < 3; k++) out.mesh_nrm[3 * (size_t)st.fn + k] = (float)S[k];
+                st.p1 = vbase + (swap ? ib : ia);
+                st.p2 = vbase + (swap ? ia : ib);
+                st.p3 = vbase + ic;
+                bool smooth = false;
+                if (nn) {
+                    const float *na = m.normals + 3 * (size_t)ia, *nb = m.normals + 3 * (size_t)ib,
+                                *nc = m.normals + 3 * (size_t)ic;
+                    double l1 = 0, l2 = 0;
+                    for (int k = 0; k < 3; k++) {
+                        double e1 = (double)na[k] - (double)nb[k], e2 = (double)na[k] - (double)nc[k];
+                        l1 += e1 * e1;
+                        l2 += e2 * e2;
+                    }
+                    smooth = std::fabs(l1) > 1.0e-10 || std::fabs(l2) > 1.0e-10;
+                }
+                st.n1 = nbase + (smooth && swap ? ib : ia);
+                st.n2 = nbase + (smooth && swap ? ia : ib);
+                st.n3 = nbase + ic;
+                {
+                    double Q1[3] = {q1[0], q1[1], q1[2]}, Q2[3] = {q2[0], q2[1], q2[2]};
+                    smooth_triangle(Q1, Q2, P3, st);
+                    if (smooth) st.vaxis_smooth |= 1u << 4;
+                }
                 pr.kind_axis = kPrimTriangle | (uint32_t)(axis << 4);
                 for (int k = 0; k < 3; k++) {
                     pr.tri.n[k] = (float)S[k];

@@ -26,6 +26,9 @@ struct HostScene {
     std::vector<CsgChild> csg_children;
     std::vector<uint32_t> csg_leaves;
     std::vector<uint32_t> csg_leaf_range;
+    std::vector<ShadeTri> shade_tris;
+    std::vector<uint32_t> obj_tri_base;
+    std::vector<float> mesh_vtx, mesh_nrm;
     uint32_t any_flags = 0;
     uint32_t has_complex = 0;
     uint64_t n_triangles = 0;

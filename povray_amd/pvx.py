@@ -50,6 +50,8 @@ def lib():
         L.pvx_inside.argtypes = [vp, vp, vp, sz, vp, vp]
         L.pvx_camera_rays.argtypes = [C.POINTER(A.PvxCamera), vp, vp]
         L.pvx_copy.argtypes = [vp, vp, sz, vp]
+        L.pvx_scene_set_shading.argtypes = [vp, C.POINTER(A.PvxShadingDesc)]
+        L.pvx_render.argtypes = [vp, C.POINTER(A.PvxCamera), vp, C.POINTER(A.PvxRenderStats), vp]
         _L = L
     return _L
 
@@ -58,7 +60,7 @@ EXPORTED_SYMBOLS = (
     "pvx_last_error", "pvx_init", "pvx_scene_create", "pvx_scene_destroy",
     "pvx_scene_get_stats", "pvx_trace_closest", "pvx_trace_closest_counted",
     "pvx_all_intersections", "pvx_inside",
-    "pvx_camera_rays", "pvx_copy",
+    "pvx_camera_rays", "pvx_copy", "pvx_scene_set_shading", "pvx_render",
 )
 
 
@@ -154,6 +156,29 @@ class GpuScene:
         hip.synchronize()
         return (d_d.download(np.float64).reshape(n, max_hits),
                 d_p.download(np.int32).reshape(n, max_hits), d_c.download(np.uint32))
+
+    def set_shading(self, scene=None):
+        """Attach the scene's materials and lights (pvx_scene_set_shading)."""
+        sd, keep = (scene or self.scene).shading()
+        _check(lib().pvx_scene_set_shading(self.h, C.byref(sd)), "pvx_scene_set_shading")
+        del keep
+
+    def render_device(self, out, width=None, height=None, stream=None):
+        """Render into a device buffer of 3*w*h floats; returns the stats dict."""
+        cam = self.scene.camera_struct(width=width, height=height)
+        st = A.PvxRenderStats()
+        _check(lib().pvx_render(self.h, C.byref(cam), _ptr(out), C.byref(st),
+                                stream.h if stream else None), "pvx_render")
+        return {k: getattr(st, k) for k, _ in A.PvxRenderStats._fields_ if k != "pad"}
+
+    def render(self, width=None, height=None):
+        """Render one frame; returns ((h, w, 3) float32 linear RGB, stats)."""
+        cam = self.scene.camera_struct(width=width, height=height)
+        out = hip.DeviceBuffer(cam.width * cam.height * 12)
+        stats = self.render_device(out, width, height)
+        hip.synchronize()
+        img = out.download(np.float32, cam.width * cam.height * 3)
+        return img.reshape(cam.height, cam.width, 3), stats
 
     def inside(self, pts: np.ndarray, obj):
         pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)

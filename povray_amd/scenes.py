@@ -166,6 +166,51 @@ class Scene:
             d.csg_children = ch.ctypes.data_as(C.POINTER(A.PvxObject))
         return d, keep
 
+    def shading(self):
+        """pvx_shading_desc for this scene's materials and lights.
+        Returns (desc, keepalive)."""
+        m = self.materials
+        n = self.n_objects
+        mats = (A.PvxMaterial * n)()
+        for i in range(n):
+            t = mats[i]
+            t.pigment[:] = [float(x) for x in m["pigment"][i]]
+            t.filter = 0.0
+            t.transmit = float(m.get("transmit", np.zeros(n))[i])
+            amb = float(m.get("ambient", np.full(n, 0.1))[i])
+            t.ambient[:] = [amb, amb, amb]
+            t.diffuse = float(m.get("diffuse", np.full(n, 0.6))[i])
+            t.brilliance = 1.0
+            t.phong = float(m.get("phong", np.zeros(n))[i])
+            t.phong_size = float(m.get("phong_size", np.full(n, 40.0))[i])
+            r = float(m.get("reflection", np.zeros(n))[i])
+            t.reflection_min[:] = [r, r, r]
+            t.reflection_max[:] = [r, r, r]
+            t.reflection_falloff = 1.0
+            ior = float(m.get("ior", np.ones(n))[i])
+            t.ior = ior if t.transmit > 0.0 else 0.0
+        lights = (A.PvxLight * max(1, len(self.lights)))()
+        for i, L in enumerate(self.lights):
+            lights[i].kind = A.PVX_LIGHT_AREA if L["kind"] == "area" else A.PVX_LIGHT_POINT
+            lights[i].position[:] = [float(x) for x in L["position"]]
+            if L["kind"] == "area":
+                lights[i].axis1[:] = [float(x) for x in L["axis1"]]
+                lights[i].axis2[:] = [float(x) for x in L["axis2"]]
+                lights[i].size1, lights[i].size2 = L["size"]
+            else:
+                lights[i].size1 = lights[i].size2 = 1
+            lights[i].colour[:] = [float(x) for x in L["colour"]]
+        d = A.PvxShadingDesc()
+        d.n_materials = n
+        d.materials = C.cast(mats, C.POINTER(A.PvxMaterial))
+        d.n_lights = len(self.lights)
+        d.lights = C.cast(lights, C.POINTER(A.PvxLight))
+        d.ambient_light[:] = [1.0, 1.0, 1.0]
+        d.background[:] = [0.0, 0.0, 0.0]
+        d.max_trace_level = int(self.settings.get("max_trace_level", 5))
+        d.adc_bailout = float(self.settings.get("adc_bailout", 1.0 / 255.0))
+        return d, [mats, lights]
+
     def camera_struct(self, jitter=(0.5, 0.5), width=None, height=None):
         c = self.camera
         cam = A.PvxCamera()
