@@ -269,7 +269,9 @@ typedef struct pvx_material {
     float pigment[3];
     float filter;            /* pigment filter (0 = opaque)                   */
     float transmit;          /* pigment transmit (0 = opaque)                 */
-    float ambient[3];        /* finish ambient, times the global ambient_light */
+    float ambient[3];        /* finish ambient, times the global ambient_light
+                              * (classic ambient: ignored when radiosity is on) */
+    float emission[3];       /* finish emission (default 0)                  */
     float diffuse;           /* default 0.6                                  */
     float brilliance;        /* default 1.0                                  */
     float phong;             /* default 0.0                                  */
@@ -332,6 +334,58 @@ int pvx_scene_set_shading(pvx_scene *scene, const pvx_shading_desc *desc);
  * rgb: device buffer of 3*width*height floats (linear RGB, row-major). */
 int pvx_render(pvx_scene *scene, const pvx_camera *cam, float *rgb, pvx_render_stats *stats,
                void *stream);
+
+/* ---- radiosity ---------------------------------------------------------- */
+enum { PVX_RADIOSITY_PLAIN = 0, PVX_RADIOSITY_LEARNED = 1 };
+
+/* global_settings { radiosity { ... } } (SceneRadiositySettings defaults,
+ * core/lighting/radiosity.h:141-166) plus the surrogate controls. */
+typedef struct pvx_radiosity_settings {
+    uint32_t count;            /* rays per gather at depth 0 (default 35)       */
+    uint32_t recursion_limit;  /* default 2                                     */
+    uint32_t nearest_count;    /* default 5                                     */
+    uint32_t mode;             /* PVX_RADIOSITY_PLAIN / PVX_RADIOSITY_LEARNED   */
+    float error_bound;         /* default 1.8                                   */
+    float minimum_reuse;       /* default 0.015                                 */
+    float maximum_reuse;       /* default 0.2                                   */
+    float pretrace_start;      /* default 0.08                                  */
+    float pretrace_end;        /* default 0.04                                  */
+    float brightness;          /* default 1.0                                   */
+    float adc_bailout;         /* default 0.01                                  */
+    float probe_threshold;     /* learned: max mean probe error per tile (0.02) */
+    uint32_t train_samples;    /* learned: plain gathers taken as training data */
+    uint32_t train_steps;      /* learned: optimiser steps                      */
+    uint32_t probe_tile;       /* learned: probe tile edge in pixels (32)       */
+    uint32_t probes_per_tile;  /* learned: plain gathers probed per tile (2)    */
+} pvx_radiosity_settings;
+
+typedef struct pvx_radiosity_stats {
+    double frame_ms;           /* whole frame (GPU events)                      */
+    double pretrace_ms;
+    double final_ms;
+    double train_ms;
+    uint64_t gathers[2];       /* plain gathers at recursion depth 0 / 1        */
+    uint64_t gather_rays;      /* rays traced by plain gathers                  */
+    uint64_t records[2];       /* irradiance-cache records at depth 0 / 1       */
+    uint64_t final_misses;     /* final-pass points with no reusable record     */
+    uint64_t surrogate_queries;/* misses answered by the surrogate              */
+    uint64_t fallbacks;        /* misses sent back to the plain gather          */
+    uint64_t probes;           /* plain gathers spent on probing                */
+    float probe_error;         /* mean abs probe error (surrogate vs gather)    */
+    float train_loss;          /* final training loss                           */
+} pvx_radiosity_stats;
+
+/* Render one frame lit by radiosity: pretrace passes fill the irradiance
+ * cache, the final pass reuses it and gathers where nothing is reusable
+ * (RadiosityFunction::ComputeAmbient / GatherLight, RadiosityCache
+ * FindReusableBlock / AverageNearBlock / AddBlock,
+ * core/lighting/radiosity.cpp:385-743, 1194-1769).  Surfaces:
+ * pigment * (emission + diffuse * irradiance).  In LEARNED mode the
+ * final-pass misses are answered by the MLP irradiance surrogate, trained on
+ * this frame's plain gathers, except in tiles whose probes disagree. */
+int pvx_radiosity_render(pvx_scene *scene, const pvx_camera *cam,
+                         const pvx_radiosity_settings *settings, float *rgb,
+                         pvx_radiosity_stats *stats, void *stream);
 
 /* ---- utility ----------------------------------------------------------- */
 /* Plain device-to-device copy kernel (16 B per lane per step).  No reference

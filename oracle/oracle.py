@@ -39,6 +39,10 @@ def lib():
                                       C.c_void_p]
         L.orc_render.argtypes = [C.c_void_p, C.POINTER(A.PvxShadingDesc), C.POINTER(A.PvxCamera),
                                  C.c_void_p, C.c_int, C.c_void_p, C.c_int]
+        L.orc_radiosity_render.argtypes = [C.c_void_p, C.POINTER(A.PvxShadingDesc),
+                                           C.POINTER(A.PvxCamera),
+                                           C.POINTER(A.PvxRadiositySettings), C.c_void_p,
+                                           C.c_void_p]
         L.orc_solve_polynomial.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_double, C.c_void_p]
         _LIB = L
     return _LIB
@@ -118,6 +122,19 @@ class OracleScene:
         lib().orc_render(self.h, C.byref(sd), C.byref(cam), _p(out), threads, _p(cnt), method)
         del keep
         return out, (int(cnt[0]), int(cnt[1]))
+
+    def radiosity_render(self, scene, width=None, height=None, **over):
+        """Sequential FP64 radiosity (plain gathers, Ward cache) over the
+        camera; returns ((h, w, 3) float32 RGB, counters dict)."""
+        sd, keep = scene.shading()
+        cam = scene.camera_struct(width=width, height=height)
+        rs = A.radiosity_settings(scene, **over)
+        out = np.zeros((cam.height, cam.width, 3), np.float32)
+        cnt = np.zeros(5, np.uint64)
+        lib().orc_radiosity_render(self.h, C.byref(sd), C.byref(cam), C.byref(rs), _p(out), _p(cnt))
+        del keep
+        return out, dict(gathers0=int(cnt[0]), gathers1=int(cnt[1]), rays=int(cnt[2]),
+                         records0=int(cnt[3]), records1=int(cnt[4]))
 
     def mesh_normal(self, mesh, tri, pts):
         tri = np.ascontiguousarray(tri, np.int32)

@@ -120,8 +120,8 @@ RGB trace_ray(const Scene &s, const Shading &sh, Counters &cnt, const Ray &ray, 
     if (dot(n, ray.d) > 0.0) n = n * -1.0;
     RGB pig = rgb3(m.pigment);
     double att = 1.0 - m.filter - m.transmit;
-    // classic ambient: Ambient * ambient_light (emission 0), times opacity
-    RGB emission = rgb3(m.ambient) * rgb3(sh.amb);
+    // emission + classic ambient (Ambient * ambient_light), times opacity
+    RGB emission = rgb3(m.emission) + rgb3(m.ambient) * rgb3(sh.amb);
     RGB col = scale(pig * emission, att);
     for (const pvx_light &L : sh.lights) {
         V3 C(L.position[0], L.position[1], L.position[2]);
@@ -203,6 +203,12 @@ RGB trace_ray(const Scene &s, const Shading &sh, Counters &cnt, const Ray &ray, 
 }
 
 } // namespace
+
+V3 oracle_shape_normal(const Scene &s, uint32_t obj, const Isect &h, V3 ip) {
+    const Object &o = s.objects[obj];
+    return shape_normal(s, o.type, o.index, h, ip);
+}
+
 } // namespace orc
 
 using namespace orc;

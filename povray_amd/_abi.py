@@ -94,7 +94,8 @@ PVX_LIGHT_POINT, PVX_LIGHT_AREA = 1, 2
 
 class PvxMaterial(C.Structure):
     _fields_ = [("pigment", C.c_float * 3), ("filter", C.c_float), ("transmit", C.c_float),
-                ("ambient", C.c_float * 3), ("diffuse", C.c_float), ("brilliance", C.c_float),
+                ("ambient", C.c_float * 3), ("emission", C.c_float * 3),
+                ("diffuse", C.c_float), ("brilliance", C.c_float),
                 ("phong", C.c_float), ("phong_size", C.c_float),
                 ("reflection_min", C.c_float * 3), ("reflection_max", C.c_float * 3),
                 ("reflection_falloff", C.c_float), ("ior", C.c_float)]
@@ -117,6 +118,47 @@ class PvxRenderStats(C.Structure):
     _fields_ = [("closest_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
                 ("bounces", C.c_uint32), ("pad", C.c_uint32), ("trace_ms", C.c_double),
                 ("shadow_ms", C.c_double), ("total_ms", C.c_double)]
+
+
+PVX_RADIOSITY_PLAIN, PVX_RADIOSITY_LEARNED = 0, 1
+
+
+class PvxRadiositySettings(C.Structure):
+    _fields_ = [("count", C.c_uint32), ("recursion_limit", C.c_uint32),
+                ("nearest_count", C.c_uint32), ("mode", C.c_uint32),
+                ("error_bound", C.c_float), ("minimum_reuse", C.c_float),
+                ("maximum_reuse", C.c_float), ("pretrace_start", C.c_float),
+                ("pretrace_end", C.c_float), ("brightness", C.c_float),
+                ("adc_bailout", C.c_float), ("probe_threshold", C.c_float),
+                ("train_samples", C.c_uint32), ("train_steps", C.c_uint32),
+                ("probe_tile", C.c_uint32), ("probes_per_tile", C.c_uint32)]
+
+
+class PvxRadiosityStats(C.Structure):
+    _fields_ = [("frame_ms", C.c_double), ("pretrace_ms", C.c_double), ("final_ms", C.c_double),
+                ("train_ms", C.c_double), ("gathers", C.c_uint64 * 2),
+                ("gather_rays", C.c_uint64), ("records", C.c_uint64 * 2),
+                ("final_misses", C.c_uint64), ("surrogate_queries", C.c_uint64),
+                ("fallbacks", C.c_uint64), ("probes", C.c_uint64), ("probe_error", C.c_float),
+                ("train_loss", C.c_float)]
+
+
+def radiosity_settings(scene, mode=PVX_RADIOSITY_PLAIN, **over):
+    """pvx_radiosity_settings from scene.settings['radiosity'] (defaults of
+    SceneRadiositySettings otherwise) with keyword overrides."""
+    cfg = dict(count=35, recursion_limit=2, nearest_count=5, error_bound=1.8,
+               minimum_reuse=0.015, maximum_reuse=0.2, pretrace_start=0.08, pretrace_end=0.04,
+               brightness=1.0, adc_bailout=0.01, probe_threshold=0.02, train_samples=8192,
+               train_steps=1500, probe_tile=32, probes_per_tile=2)
+    cfg.update(scene.settings.get("radiosity", {}))
+    cfg.update(over)
+    st = PvxRadiositySettings()
+    for k, _ in PvxRadiositySettings._fields_:
+        if k == "mode":
+            st.mode = mode
+        else:
+            setattr(st, k, cfg[k])
+    return st
 
 
 assert C.sizeof(PvxObject) == 16

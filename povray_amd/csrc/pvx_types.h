@@ -124,7 +124,8 @@ struct ShadeTri {
 struct MatRec {
     float pigment[3];
     float att;            // opacity 1 - filter - transmit
-    float emission[3];    // ambient * ambient_light
+    float emission[3];    // emission + ambient * ambient_light (classic lighting)
+    float ambient_only[3];// emission alone (radiosity lighting)
     float diffuse, brilliance, phong, phong_size;
     float rmin[3], rmax[3];
     float falloff;

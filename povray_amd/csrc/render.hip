@@ -374,7 +374,8 @@ extern "C" int pvx_scene_set_shading(pvx_scene *s, const pvx_shading_desc *d) {
         MatRec &m = sh.mats[i];
         for (int k = 0; k < 3; k++) {
             m.pigment[k] = a.pigment[k];
-            m.emission[k] = a.ambient[k] * d->ambient_light[k];
+            m.emission[k] = a.emission[k] + a.ambient[k] * d->ambient_light[k];
+            m.ambient_only[k] = a.emission[k];
             m.rmin[k] = a.reflection_min[k];
             m.rmax[k] = a.reflection_max[k];
         }

@@ -179,6 +179,8 @@ class Scene:
             t.transmit = float(m.get("transmit", np.zeros(n))[i])
             amb = float(m.get("ambient", np.full(n, 0.1))[i])
             t.ambient[:] = [amb, amb, amb]
+            em = m.get("emission", None)
+            t.emission[:] = [float(x) for x in em[i]] if em is not None else [0.0, 0.0, 0.0]
             t.diffuse = float(m.get("diffuse", np.full(n, 0.6))[i])
             t.brilliance = 1.0
             t.phong = float(m.get("phong", np.zeros(n))[i])
@@ -545,4 +547,64 @@ def csg_scene(seed: int = 0xC0FFEE + 2, n_objects: int = 10000, width=1280, heig
                        reflection=np.where(glass, 0.1, 0.0), ior=np.where(glass, 1.5, 1.0),
                        transmit=np.where(glass, 0.9, 0.0)),
         settings=dict(max_trace_level=10, adc_bailout=1.0 / 255.0),
+    )
+
+
+# --------------------------------------------------------------------------
+# radiosity room
+# --------------------------------------------------------------------------
+def radiosity_scene(seed: int = 0xC0FFEE + 3, n_boxes: int = 5000, width=1280, height=720) -> Scene:
+    """A closed 10x4x10 room (six planes) with `n_boxes` seeded boxes
+    (extents U[0.05,0.8], albedo U[0.2,0.9]^3, random rotation about y) and
+    one emissive ceiling panel; lit only by radiosity
+    (count 200, recursion_limit 2, error_bound 0.5)."""
+    r = Rng(seed)
+    ext = r.uniform(3 * n_boxes, 0.05, 0.8).reshape(n_boxes, 3)
+    # boxes fill the room except a 1.5 m slab in front of the camera (z < -3.2)
+    pos = np.stack([r.uniform(n_boxes, -4.5, 4.5), r.uniform(n_boxes, 0.0, 3.3),
+                    r.uniform(n_boxes, -2.7, 4.5)], 1)
+    rot = r.uniform(n_boxes, 0.0, 360.0)
+    alb = r.uniform(3 * n_boxes, 0.2, 0.9).reshape(n_boxes, 3)
+    xforms = []
+    c1, c2, tr = [], [], []
+    for i in range(n_boxes):
+        t = compose(rotation_transform((0.0, rot[i], 0.0)), translation_transform(pos[i]))
+        xforms.append(_xf_row(t))
+        c1.append(-0.5 * ext[i])
+        c2.append(0.5 * ext[i])
+        tr.append(i)
+    # the emissive ceiling panel (untransformed)
+    c1.append([-1.5, 3.9, -1.5])
+    c2.append([1.5, 3.99, 1.5])
+    tr.append(-1)
+    planes_n = np.array([[0, 1, 0], [0, -1, 0], [1, 0, 0], [-1, 0, 0], [0, 0, 1], [0, 0, -1]],
+                        np.float64)
+    planes_d = np.array([0.0, 4.0, 5.0, 5.0, 5.0, 5.0])
+    nb = n_boxes + 1
+    objects = np.zeros((nb + 6, 4), np.uint32)
+    objects[:nb, 0] = A.PVX_BOX
+    objects[:nb, 1] = np.arange(nb)
+    objects[nb:, 0] = A.PVX_PLANE
+    objects[nb:, 1] = np.arange(6)
+    n = nb + 6
+    pig = np.concatenate([alb, np.ones((1, 3)), np.full((6, 3), 0.75)], 0)
+    emission = np.zeros((n, 3))
+    emission[n_boxes] = [3.0, 3.0, 3.0]
+    diffuse = np.full(n, 0.75)
+    diffuse[n_boxes] = 0.0
+    return Scene(
+        name=f"radiosity_room_{n_boxes}", objects=objects,
+        box_corner1=np.array(c1), box_corner2=np.array(c2), box_transform=np.array(tr, np.int32),
+        transforms=np.array(xforms).reshape(-1, 32),
+        plane_normal=planes_n, plane_distance=planes_d,
+        camera=look_at_camera((0.0, 2.0, -4.7), (0.0, 1.4, 0.0), width, height),
+        lights=[],
+        materials=dict(pigment=pig, diffuse=diffuse, ambient=np.zeros(n), phong=np.zeros(n),
+                       phong_size=np.full(n, 40.0), reflection=np.zeros(n), ior=np.ones(n),
+                       transmit=np.zeros(n), emission=emission),
+        settings=dict(max_trace_level=5, adc_bailout=1.0 / 255.0,
+                      radiosity=dict(count=200, recursion_limit=2, error_bound=0.5,
+                                     minimum_reuse=0.015, maximum_reuse=0.2, nearest_count=5,
+                                     pretrace_start=0.08, pretrace_end=0.04, brightness=1.0,
+                                     adc_bailout=0.01)),
     )
