@@ -258,8 +258,8 @@ extern "C" int orc_radiosity_render(void *sp, const pvx_shading_desc *sd, const 
         r = Ray{V3(cam->location[0], cam->location[1], cam->location[2]), normalized(d)};
     };
     // pretrace: pixel step from pretrace_start to pretrace_end of the width
-    double step = std::max(1.0, std::floor(rs->pretrace_start * std::max(W, H)));
-    double end = std::max(1.0, std::floor(rs->pretrace_end * std::max(W, H)));
+    double step = std::max(1.0, std::floor((double)rs->pretrace_start * std::max(W, H)));
+    double end = std::max(1.0, std::floor((double)rs->pretrace_end * std::max(W, H)));
     for (;;) {
         for (double y = 0; y < H; y += step)
             for (double x = 0; x < W; x += step) {

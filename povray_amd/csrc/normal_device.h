@@ -1,0 +1,101 @@
+// Device-side surface normals (Object::Normal for the shapes on the path).
+#ifndef PVX_NORMAL_DEVICE_H
+#define PVX_NORMAL_DEVICE_H
+
+#include "shapes_device.h"
+
+namespace pvx {
+
+struct NormalParams {
+    SceneView sv;
+    const uint32_t *obj_type;
+    const uint32_t *obj_index;
+};
+
+__device__ inline D3 mtrans_normal(const double *inv, D3 n) { // MTransNormal via inverse
+    return D3{n.x * inv[0] + n.y * inv[1] + n.z * inv[2], n.x * inv[4] + n.y * inv[5] + n.z * inv[6],
+              n.x * inv[8] + n.y * inv[9] + n.z * inv[10]};
+}
+__device__ inline D3 normalize3(D3 a) {
+    double l = sqrt(a.x * a.x + a.y * a.y + a.z * a.z);
+    return l != 0.0 ? D3{a.x / l, a.y / l, a.z / l} : a;
+}
+
+// Object::Normal of the surface that was hit (raw, not yet flipped).
+__device__ inline D3 leaf_normal(const NormalParams &p, uint32_t type, uint32_t index, double t, D3 o, D3 d,
+                          D3 ip) {
+    const SceneView &sv = p.sv;
+    switch (type) {
+    case PVX_SPHERE: { // Sphere::Normal (sphere.cpp), spherical mode
+        const double *s = sv.sph + 4 * (size_t)index;
+        return D3{(ip.x - s[0]) / s[3], (ip.y - s[1]) / s[3], (ip.z - s[2]) / s[3]};
+    }
+    case PVX_PLANE: {
+        const PlaneRec &pl = sv.planes[index];
+        return D3{pl.n[0], pl.n[1], pl.n[2]};
+    }
+    case PVX_BOX: { // Box::Normal (box.cpp:600-625): side of the hit depth
+        const BoxRec &b = sv.boxes[index];
+        double hs[2];
+        int side[2] = {0, 0};
+        int nh = box_hits(b, sv.xforms, o, d, hs, side);
+        int sd = 0;
+        for (int k = 0; k < nh; k++)
+            if (hs[k] == t) sd = side[k];
+        D3 n{0, 0, 0};
+        if (sd == 1) n.x = -1; else if (sd == 2) n.x = 1;
+        else if (sd == 3) n.y = -1; else if (sd == 4) n.y = 1;
+        else if (sd == 5) n.z = -1; else if (sd == 6) n.z = 1;
+        if (b.xf >= 0) n = normalize3(mtrans_normal(sv.xforms[b.xf].inv, n));
+        return n;
+    }
+    case PVX_TORUS: { // Torus::Normal (torus.cpp:458-490)
+        const TorusRec &tr = sv.tori[index];
+        const Xform &x = sv.xforms[tr.xf];
+        D3 P = xf_point(x.inv, ip);
+        double dist = sqrt(P.x * P.x + P.z * P.z);
+        D3 M{0, 0, 0};
+        if (dist > kEpsilon) M = D3{tr.R * P.x / dist, 0.0, tr.R * P.z / dist};
+        return normalize3(mtrans_normal(x.inv, sub3(P, M)));
+    }
+    default:
+        return D3{0, 1, 0};
+    }
+}
+
+__device__ inline D3 hit_normal(const NormalParams &p, const pvx_hit &h, D3 o, D3 d, D3 ip) {
+    const SceneView &sv = p.sv;
+    uint32_t type = p.obj_type[h.obj], index = p.obj_index[h.obj];
+    if (type == PVX_MESH) { // Mesh::Normal + Smooth_Mesh_Normal (mesh.cpp:380-420, 2366-2394)
+        const ShadeTri &st = sv.shade_tris[sv.obj_tri_base[h.obj] + (uint32_t)h.prim];
+        if (!(st.vaxis_smooth >> 4)) {
+            const float *fn = sv.mesh_nrm + 3 * (size_t)st.fn;
+            return D3{fn[0], fn[1], fn[2]};
+        }
+        const float *n1 = sv.mesh_nrm + 3 * (size_t)st.n1, *n2 = sv.mesh_nrm + 3 * (size_t)st.n2,
+                    *n3 = sv.mesh_nrm + 3 * (size_t)st.n3;
+        const float *p1 = sv.mesh_vtx + 3 * (size_t)st.p1, *p2 = sv.mesh_vtx + 3 * (size_t)st.p2,
+                    *p3 = sv.mesh_vtx + 3 * (size_t)st.p3;
+        D3 pip1{ip.x - (double)p1[0], ip.y - (double)p1[1], ip.z - (double)p1[2]};
+        double u = pip1.x * (double)st.perp[0] + pip1.y * (double)st.perp[1] + pip1.z * (double)st.perp[2];
+        D3 N1{n1[0], n1[1], n1[2]};
+        if (u < kEpsilon) return normalize3(N1);
+        int axis = (int)(st.vaxis_smooth & 3u);
+        double k1 = p1[axis], k2 = p2[axis], k3 = p3[axis];
+        double v = (comp(pip1, axis) / u + k1 - k2) / (k3 - k2);
+        D3 N2{n2[0], n2[1], n2[2]}, N3{n3[0], n3[1], n3[2]};
+        D3 t1{N2.x - N1.x + v * (N3.x - N2.x), N2.y - N1.y + v * (N3.y - N2.y),
+              N2.z - N1.z + v * (N3.z - N2.z)};
+        return normalize3(D3{N1.x + u * t1.x, N1.y + u * t1.y, N1.z + u * t1.z});
+    }
+    if (type == PVX_CSG) {
+        const CsgChild &c = sv.csg_children[(uint32_t)h.prim];
+        return leaf_normal(p, c.type, c.index, h.t, o, d, ip);
+    }
+    return leaf_normal(p, type, index, h.t, o, d, ip);
+}
+
+
+} // namespace pvx
+
+#endif

@@ -1,0 +1,843 @@
+// Radiosity on the GPU (gfx950): irradiance cache + batched gathers.
+//
+// The reference fills its Ward irradiance cache one gather at a time while
+// pixels are traced in order (RadiosityFunction::ComputeAmbient /
+// GatherLight, core/lighting/radiosity.cpp:385-743).  Here every query of a
+// pass is looked up at once; the points the cache cannot serve are gathered
+// in rounds — a spread-out subset first, its records inserted, the rest
+// looked up again — so later points reuse earlier gathers as in the
+// sequential algorithm, while each round is one wide launch:
+//   gather rays (Halton(2,3) cosine-weighted pool, SampleDirectionGenerator
+//   frame, radiosity.cpp:758-868) -> closest hit -> depth+1 irradiance
+//   (recursively resolved the same way) -> per-query reduction (mean
+//   radiance, harmonic mean distance, nearest hit) -> new cache records.
+// Lookups follow AverageNearBlock (radiosity.cpp:1551-1769): quick-check
+// radius, nearest-surface radius blend, error_reuse = dist/r + 2 sqrt(1-n.n'),
+// in-front test, saw weight (pseudo-sigmoid for depth-0 records in the final
+// pass).  The cache lives in a uniform grid rebuilt after each round.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <unordered_set>
+#include <vector>
+
+#include "internal.h"
+#include "normal_device.h"
+#include "radiosity_internal.h"
+#include "shapes_device.h"
+
+using namespace pvx;
+
+namespace {
+
+constexpr double kInFrontLimit = -0.05;   // radiosity.cpp:100
+constexpr double kAvgNearEps = 0.000001;  // radiosity.cpp:109
+constexpr double kRadEps = 0.001;         // radiosity.cpp:110
+constexpr float kWeightErrOffset = 0.25f; // radiosity.cpp:111
+constexpr int kPoolSize = 1600;           // RADIOSITY_MAX_SAMPLE_DIRECTIONS
+
+struct CacheView {
+    const RadRec *recs;
+    const uint32_t *cell_start; // ncells + 1
+    const uint32_t *cell_items;
+    float lo[3];
+    float cell;
+    int dims[3];
+    uint32_t n;
+};
+
+__device__ inline float grey3(const float *c) { return 0.297f * c[0] + 0.589f * c[1] + 0.114f * c[2]; }
+
+// FindReusableBlock + AverageNearBlock for a batch of queries.
+__global__ void k_lookup(const RadQuery *__restrict__ q, uint32_t n, CacheView c, float eb,
+                         float factor, int sigmoid, float *__restrict__ E, float *__restrict__ reuse) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const RadQuery qq = q[i];
+    double bound = (double)eb;
+    if (qq.w < kWeightErrOffset) bound += (double)(kWeightErrOffset - qq.w);
+    bound *= (double)factor;
+    double W = 0.0, we[3] = {0, 0, 0};
+    if (c.n > 0) {
+        int cx = (int)floorf(((float)qq.p[0] - c.lo[0]) / c.cell);
+        int cy = (int)floorf(((float)qq.p[1] - c.lo[1]) / c.cell);
+        int cz = (int)floorf(((float)qq.p[2] - c.lo[2]) / c.cell);
+        if (cx >= 0 && cy >= 0 && cz >= 0 && cx < c.dims[0] && cy < c.dims[1] && cz < c.dims[2]) {
+            uint32_t cid = ((uint32_t)cz * c.dims[1] + (uint32_t)cy) * c.dims[0] + (uint32_t)cx;
+            for (uint32_t k = c.cell_start[cid]; k < c.cell_start[cid + 1]; k++) {
+                const RadRec b = c.recs[c.cell_items[k]];
+                double dx = qq.p[0] - b.p[0], dy = qq.p[1] - b.p[1], dz = qq.p[2] - b.p[2];
+                double sq = dx * dx + dy * dy + dz * dz;
+                double quick = (double)b.r * bound;
+                if (!(sq < quick * quick)) continue;
+                double dist = sqrt(sq);
+                double ri = b.r;
+                bool far = dist > kAvgNearEps;
+                double ux = 0, uy = 0, uz = 0;
+                if (far) {
+                    ux = dx / dist;
+                    uy = dy / dist;
+                    uz = dz / dist;
+                    double cs = b.tn[0] * ux + b.tn[1] * uy + b.tn[2] * uz;
+                    if (cs > 0.0) ri = cs * b.nearest + (1.0 - cs) * ri;
+                }
+                if (!(dist < ri * bound)) continue;
+                double dd = (double)qq.n[0] * b.n[0] + (double)qq.n[1] * b.n[1] + (double)qq.n[2] * b.n[2];
+                double err = dist / ri + 2.0 * sqrt(fabs(1.0 - dd));
+                if (!(err < bound)) continue;
+                double in_front = 1.0;
+                if (far) {
+                    double hx = qq.n[0] + b.n[0], hy = qq.n[1] + b.n[1], hz = qq.n[2] + b.n[2];
+                    in_front = (ux * hx + uy * hy + uz * hz) / sqrt(hx * hx + hy * hy + hz * hz);
+                }
+                if (!(in_front > kInFrontLimit)) continue;
+                double w;
+                if (!sigmoid) {
+                    w = 1.0 - err / bound;
+                } else { // PSEUDO_SIGMOID_METHOD (final pass, bounce depth 0)
+                    w = err / bound;
+                    w = w < 0.5 ? 1.0 - (w * 2.0) * (w * 2.0) / 2.0 : ((1.0 - w) * 2.0) * ((1.0 - w) * 2.0) / 2.0;
+                }
+                if (in_front <= 0) w = w * (1 - in_front / kInFrontLimit);
+                if (w > kRadEps) {
+                    we[0] += b.E[0] * w;
+                    we[1] += b.E[1] * w;
+                    we[2] += b.E[2] * w;
+                    W += w;
+                }
+            }
+        }
+    }
+    float e[3] = {0.0f, 0.0f, 0.0f};
+    if (W > 0) {
+        e[0] = (float)(we[0] / W);
+        e[1] = (float)(we[1] / W);
+        e[2] = (float)(we[2] / W);
+    }
+    E[3 * (size_t)i] = e[0];
+    E[3 * (size_t)i + 1] = e[1];
+    E[3 * (size_t)i + 2] = e[2];
+    reuse[i] = (float)W;
+}
+
+// Gather rays: m queries x N directions from the cosine-weighted pool,
+// continuing a running cursor as SampleDirectionGenerator does.
+__global__ void k_gather_rays(const RadQuery *__restrict__ q, const uint32_t *__restrict__ idx,
+                              uint32_t m, uint32_t N, const double *__restrict__ pool,
+                              uint32_t cursor0, pvx_ray *__restrict__ rays) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m * N) return;
+    uint32_t s = k / N, j = k % N;
+    const RadQuery qq = q[idx[s]];
+    uint32_t pi = (uint32_t)(((uint64_t)cursor0 + (uint64_t)s * N + j) % kPoolSize);
+    double vx = pool[3 * pi], vy = fabs(pool[3 * pi + 1]), vz = pool[3 * pi + 2];
+    double fy[3] = {qq.n[0], qq.n[1], qq.n[2]};
+    double off[3] = {0, 0, 1};
+    if (fabs(fy[2]) > 0.9) {
+        off[1] = 1;
+        off[2] = 0;
+    }
+    // frameX = normalize(cross(frameY, offY)), frameZ = normalize(cross(frameX, frameY))
+    double fx[3] = {fy[1] * off[2] - fy[2] * off[1], fy[2] * off[0] - fy[0] * off[2],
+                    fy[0] * off[1] - fy[1] * off[0]};
+    double lx = sqrt(fx[0] * fx[0] + fx[1] * fx[1] + fx[2] * fx[2]);
+    for (int a = 0; a < 3; a++) fx[a] /= lx;
+    double fz[3] = {fx[1] * fy[2] - fx[2] * fy[1], fx[2] * fy[0] - fx[0] * fy[2],
+                    fx[0] * fy[1] - fx[1] * fy[0]};
+    double lz = sqrt(fz[0] * fz[0] + fz[1] * fz[1] + fz[2] * fz[2]);
+    for (int a = 0; a < 3; a++) fz[a] /= lz;
+    double d[3];
+    if (fy[1] > 1.0 - kRadEps) {
+        d[0] = vx; d[1] = vy; d[2] = vz;
+    } else if (fy[1] < -1.0 + kRadEps) {
+        d[0] = -vx; d[1] = -vy; d[2] = -vz;
+    } else {
+        for (int a = 0; a < 3; a++) d[a] = fx[a] * vx + fy[a] * vy + fz[a] * vz;
+    }
+    pvx_ray r;
+    for (int a = 0; a < 3; a++) {
+        r.o[a] = qq.p[a];
+        r.d[a] = d[a];
+    }
+    rays[k] = r;
+}
+
+struct ShadeParams : NormalParams {
+    const MatRec *mats;
+    float adc;
+};
+
+// Surface response at each hit: base = pigment*emission*opacity and
+// coef = pigment*opacity*diffuse, plus a depth+1 irradiance query when
+// radiosity is needed there (ComputeLightedTexture, trace.cpp:826-939).
+__global__ void k_hit_queries(ShadeParams p, const pvx_ray *__restrict__ rays,
+                              const pvx_hit *__restrict__ hits, uint32_t n, float wnext,
+                              int want_ambient, float *__restrict__ base, float *__restrict__ coef,
+                              RadQuery *__restrict__ q, uint8_t *__restrict__ need) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const pvx_hit h = hits[i];
+    need[i] = 0;
+    if (h.obj < 0) {
+        for (int a = 0; a < 3; a++) {
+            base[3 * (size_t)i + a] = 0.0f;
+            coef[3 * (size_t)i + a] = 0.0f;
+        }
+        return;
+    }
+    const pvx_ray r = rays[i];
+    D3 o{r.o[0], r.o[1], r.o[2]}, d{r.d[0], r.d[1], r.d[2]};
+    D3 ip = add3(o, mul3(d, h.t));
+    D3 nrm = hit_normal(p, h, o, d, ip);
+    if (dot3(nrm, d) > 0.0) nrm = D3{-nrm.x, -nrm.y, -nrm.z};
+    const MatRec &m = p.mats[h.obj];
+    float cf[3];
+    for (int a = 0; a < 3; a++) {
+        base[3 * (size_t)i + a] = m.pigment[a] * m.ambient_only[a] * m.att;
+        cf[a] = m.pigment[a] * (m.att * m.diffuse);
+    }
+    float maxc = grey3(m.pigment) * m.att * m.diffuse;
+    bool amb = want_ambient && m.diffuse > 0.0f && maxc > p.adc;
+    for (int a = 0; a < 3; a++) coef[3 * (size_t)i + a] = amb ? cf[a] : 0.0f;
+    if (amb) {
+        RadQuery qq;
+        qq.p[0] = ip.x; qq.p[1] = ip.y; qq.p[2] = ip.z;
+        qq.n[0] = (float)nrm.x; qq.n[1] = (float)nrm.y; qq.n[2] = (float)nrm.z;
+        qq.w = wnext * maxc;
+        q[i] = qq;
+        need[i] = 1;
+    }
+}
+
+// GatherLight's reduction for m queries of N rays each.
+__global__ void k_gather_reduce(const RadQuery *__restrict__ q, const uint32_t *__restrict__ idx,
+                                uint32_t m, uint32_t N, const pvx_ray *__restrict__ rays,
+                                const pvx_hit *__restrict__ hits, const float *__restrict__ base,
+                                const float *__restrict__ coef, const float *__restrict__ amb,
+                                double cx, double cy, double cz, float min_reuse, float max_reuse,
+                                float factor, float *__restrict__ Eg, RadRec *__restrict__ rec,
+                                uint8_t *__restrict__ rec_ok) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= m) return;
+    const RadQuery qq = q[idx[s]];
+    double sum[3] = {0, 0, 0}, inv = 0.0, smallest = kBoundHuge;
+    double md[3] = {0, 0, 0};
+    for (uint32_t j = 0; j < N; j++) {
+        size_t k = (size_t)s * N + j;
+        const pvx_hit h = hits[k];
+        double depth = HUGE_VAL;
+        if (h.obj >= 0) {
+            depth = h.t;
+            for (int a = 0; a < 3; a++)
+                sum[a] += (double)(base[3 * k + a] + coef[3 * k + a] * amb[3 * k + a]);
+        }
+        if (depth < smallest) {
+            smallest = depth;
+            md[0] = rays[k].d[0];
+            md[1] = rays[k].d[1];
+            md[2] = rays[k].d[2];
+        }
+        inv += 1.0 / depth;
+    }
+    float e[3] = {(float)(sum[0] / N), (float)(sum[1] / N), (float)(sum[2] / N)};
+    Eg[3 * (size_t)s] = e[0];
+    Eg[3 * (size_t)s + 1] = e[1];
+    Eg[3 * (size_t)s + 2] = e[2];
+    double ex = cx - qq.p[0], ey = cy - qq.p[1], ez = cz - qq.p[2];
+    double to_eye = sqrt(ex * ex + ey * ey + ez * ez);
+    double rmin = to_eye * (double)min_reuse * factor, rmax = to_eye * (double)max_reuse * factor;
+    double mean = (double)N / inv;
+    bool ok = smallest > rmax * 0.0001;
+    if (mean < rmin) mean = rmin;
+    if (mean > rmax) mean = rmax;
+    RadRec r;
+    for (int a = 0; a < 3; a++) {
+        r.p[a] = (float)qq.p[a];
+        r.n[a] = qq.n[a];
+        r.E[a] = e[a];
+        r.tn[a] = (float)md[a];
+    }
+    r.r = (float)mean;
+    r.nearest = (float)smallest;
+    r.pad[0] = r.pad[1] = 0.0f;
+    rec[s] = r;
+    rec_ok[s] = ok ? 1 : 0;
+}
+
+// ComputeAmbient's combination of a partial reuse with the new gather.
+__global__ void k_combine(const uint32_t *__restrict__ idx, uint32_t m, const float *__restrict__ Eg,
+                          float *__restrict__ E, const float *__restrict__ reuse) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= m) return;
+    uint32_t i = idx[s];
+    float w = reuse[i];
+    for (int a = 0; a < 3; a++) {
+        float g = Eg[3 * (size_t)s + a];
+        E[3 * (size_t)i + a] = w > 0.0f ? (E[3 * (size_t)i + a] * w + g * 1.0f) / (w + 1.0f) : g;
+    }
+}
+
+// Needs-a-gather test of ComputeAmbient (radiosity.cpp:404-409).
+__global__ void k_need(const float *__restrict__ reuse, uint32_t n, float reuse_count, int final,
+                       uint8_t *__restrict__ need) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float w = reuse[i];
+    need[i] = (w * 4.0f >= reuse_count || (final && w > 0.0f)) ? 0 : 1;
+}
+
+__global__ void k_scale(float *E, uint32_t n, float k) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 3 * n) E[i] *= k;
+}
+
+// pixel colour = base + coef * E (the primary hit's ComputeLightedTexture)
+__global__ void k_final_colour(const float *__restrict__ base, const float *__restrict__ coef,
+                               const float *__restrict__ E, const uint32_t *__restrict__ qpix,
+                               uint32_t n, const uint8_t *__restrict__ has_q, float *__restrict__ rgb) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int a = 0; a < 3; a++) {
+        float c = base[3 * (size_t)i + a];
+        if (has_q[i]) c = c + coef[3 * (size_t)i + a] * E[3 * (size_t)qpix[i] + a];
+        rgb[3 * (size_t)i + a] = c;
+    }
+}
+
+__global__ void k_pixel_rays(pvx_camera cam, const uint32_t *__restrict__ px, uint32_t n,
+                             pvx_ray *__restrict__ rays) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t x = px[i] % cam.width, y = px[i] / cam.width;
+    double x0 = ((double)x + cam.jitter_x) / (double)cam.width - 0.5;
+    double y0 = 0.5 - ((double)y + cam.jitter_y) / (double)cam.height;
+    double dx = cam.direction[0] + x0 * cam.right[0] + y0 * cam.up[0];
+    double dy = cam.direction[1] + x0 * cam.right[1] + y0 * cam.up[1];
+    double dz = cam.direction[2] + x0 * cam.right[2] + y0 * cam.up[2];
+    double len = sqrt(dx * dx + dy * dy + dz * dz);
+    pvx_ray r;
+    r.o[0] = cam.location[0]; r.o[1] = cam.location[1]; r.o[2] = cam.location[2];
+    r.d[0] = dx / len; r.d[1] = dy / len; r.d[2] = dz / len;
+    rays[i] = r;
+}
+
+__global__ void k_gather_subq(const RadQuery *__restrict__ q, const uint32_t *__restrict__ idx, uint32_t m,
+                              RadQuery *__restrict__ out) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < m) out[s] = q[idx[s]];
+}
+
+__global__ void k_scatter3(const float *__restrict__ src, const uint32_t *__restrict__ idx, uint32_t m,
+                           float *__restrict__ dst) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= m) return;
+    uint32_t i = idx[s];
+    for (int a = 0; a < 3; a++) dst[3 * (size_t)i + a] = src[3 * (size_t)s + a];
+}
+
+__global__ void k_scatter_lookup(const float *__restrict__ subE, const float *__restrict__ subreuse,
+                                 const uint32_t *__restrict__ idx, uint32_t m, float *__restrict__ E,
+                                 float *__restrict__ reuse) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= m) return;
+    uint32_t i = idx[s];
+    for (int a = 0; a < 3; a++) E[3 * (size_t)i + a] = subE[3 * (size_t)s + a];
+    reuse[i] = subreuse[s];
+}
+
+inline uint32_t blocks(size_t n, uint32_t b = 256) { return (uint32_t)((n + b - 1) / b); }
+
+} // namespace
+
+// ----------------------------------------------------------------------------
+// host side
+// ----------------------------------------------------------------------------
+namespace pvx {
+
+DevBuf::~DevBuf() {
+    if (p) (void)hipFree(p);
+}
+int DevBuf::reserve(size_t bytes) {
+    if (bytes <= cap) return PVX_OK;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    HIP_TRY(hipMalloc(&p, bytes));
+    cap = bytes;
+    return PVX_OK;
+}
+
+int RadCache::rebuild(hipStream_t st) {
+    if (recs.empty()) return PVX_OK;
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    std::vector<float> rad(recs.size());
+    std::vector<float> rs(recs.size());
+    for (size_t i = 0; i < recs.size(); i++) {
+        rad[i] = recs[i].r * bound_max * 1.001f + 1e-6f; // >= quick-check radius
+        rs[i] = rad[i];
+        for (int a = 0; a < 3; a++) {
+            lo[a] = std::min(lo[a], recs[i].p[a] - rad[i]);
+            hi[a] = std::max(hi[a], recs[i].p[a] + rad[i]);
+        }
+    }
+    std::nth_element(rs.begin(), rs.begin() + rs.size() / 2, rs.end());
+    float ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+    // cells about the median sphere size, at most 64 per axis, grown until
+    // the record->cell lists stay within a few entries per record
+    cell = std::max(std::max(rs[rs.size() / 2], ext / 64.0f), 1e-6f);
+    const double budget = 8.0 * (double)recs.size() + 65536.0;
+    for (int tries = 0; tries < 64; tries++) {
+        double total = 0;
+        for (size_t i = 0; i < recs.size() && total <= budget; i++) {
+            double k = 2.0 * rad[i] / cell + 2.0;
+            total += k * k * k;
+        }
+        if (total <= budget) break;
+        cell *= 1.5f;
+    }
+    for (int a = 0; a < 3; a++) {
+        this->lo[a] = lo[a];
+        dims[a] = std::max(1, (int)std::ceil((hi[a] - lo[a]) / cell) + 1);
+    }
+    size_t ncell = (size_t)dims[0] * dims[1] * dims[2];
+    std::vector<uint32_t> count(ncell + 1, 0);
+    auto range = [&](size_t i, int a, int &c0, int &c1) {
+        c0 = std::max(0, (int)std::floor((recs[i].p[a] - rad[i] - lo[a]) / cell));
+        c1 = std::min(dims[a] - 1, (int)std::floor((recs[i].p[a] + rad[i] - lo[a]) / cell));
+    };
+    for (int pass = 0; pass < 2; pass++) {
+        if (pass == 1) {
+            uint32_t acc = 0;
+            for (size_t c = 0; c <= ncell; c++) {
+                uint32_t v = count[c];
+                count[c] = acc;
+                acc += v;
+            }
+            items.assign(count[ncell], 0);
+            start = count;
+        }
+        for (size_t i = 0; i < recs.size(); i++) {
+            int x0, x1, y0, y1, z0, z1;
+            range(i, 0, x0, x1);
+            range(i, 1, y0, y1);
+            range(i, 2, z0, z1);
+            for (int z = z0; z <= z1; z++)
+                for (int y = y0; y <= y1; y++)
+                    for (int x = x0; x <= x1; x++) {
+                        size_t c = ((size_t)z * dims[1] + y) * dims[0] + x;
+                        if (pass == 0)
+                            count[c]++;
+                        else
+                            items[count[c]++] = (uint32_t)i;
+                    }
+        }
+    }
+    int r;
+    if ((r = d_recs.reserve(recs.size() * sizeof(RadRec))) || (r = d_start.reserve(start.size() * 4)) ||
+        (r = d_items.reserve(std::max<size_t>(1, items.size()) * 4)))
+        return r;
+    HIP_TRY(hipMemcpyAsync(d_recs.p, recs.data(), recs.size() * sizeof(RadRec), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d_start.p, start.data(), start.size() * 4, hipMemcpyHostToDevice, st));
+    if (!items.empty())
+        HIP_TRY(hipMemcpyAsync(d_items.p, items.data(), items.size() * 4, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return PVX_OK;
+}
+
+RadiosityEngine::RadiosityEngine(pvx_scene *s, const pvx_radiosity_settings &set, const pvx_camera &c,
+                                 hipStream_t st)
+    : scene(s), rs(set), cam(c), stream(st) {
+    for (int d = 0; d < kMaxRadDepth; d++) {
+        caches[d].bound_max = (float)(rs.error_bound + kWeightErrOffset) * std::pow(2.0f, (float)d);
+        cursor[d] = 0;
+    }
+    pool.resize(3 * kPoolSize);
+    for (int i = 0; i < kPoolSize; i++) { // HaltonCosWeightedDirectionGenerator (bases 2, 3)
+        auto halton = [](size_t index, unsigned base) {
+            size_t k = 1 + index;
+            double h = 0, q = 1.0 / base;
+            while (k > 0) {
+                h = h + (unsigned)(k % base) * q;
+                k /= base;
+                q /= base;
+            }
+            return h;
+        };
+        double r = std::sqrt(halton(i, 2));
+        double theta = halton(i, 3) * (2 * M_PI);
+        double x = r * std::cos(theta), z = r * std::sin(theta);
+        pool[3 * i] = x;
+        pool[3 * i + 1] = std::sqrt(1 - x * x - z * z);
+        pool[3 * i + 2] = z;
+    }
+}
+
+int RadiosityEngine::init() {
+    int r;
+    if ((r = d_pool.reserve(pool.size() * 8))) return r;
+    HIP_TRY(hipMemcpyAsync(d_pool.p, pool.data(), pool.size() * 8, hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    return PVX_OK;
+}
+
+int RadiosityEngine::rays_per_sample(int d) const {
+    return std::max(5, (int)(rs.count * std::pow(0.5, (double)d)));
+}
+float RadiosityEngine::reuse_count(int d) const {
+    return (float)(d == 0 ? (int)rs.nearest_count : (d == 1 ? std::min(2, (int)rs.nearest_count) : 1));
+}
+
+int RadiosityEngine::lookup(int depth, const RadQuery *q, uint32_t n, bool final, float *E, float *reuse) {
+    if (n == 0) return PVX_OK;
+    RadCache &c = caches[depth];
+    CacheView v;
+    v.recs = static_cast<const RadRec *>(c.d_recs.p);
+    v.cell_start = static_cast<const uint32_t *>(c.d_start.p);
+    v.cell_items = static_cast<const uint32_t *>(c.d_items.p);
+    for (int a = 0; a < 3; a++) {
+        v.lo[a] = c.lo[a];
+        v.dims[a] = c.dims[a];
+    }
+    v.cell = c.cell;
+    v.n = (uint32_t)c.recs.size();
+    k_lookup<<<blocks(n), 256, 0, stream>>>(q, n, v, rs.error_bound, std::pow(2.0f, (float)depth),
+                                            (final && depth == 0) ? 1 : 0, E, reuse);
+    HIP_TRY(hipGetLastError());
+    return PVX_OK;
+}
+
+// Gather at `depth` for the m queries q[idx[*]]; writes their irradiance to Eg
+// (3m floats, device) and appends the new records to the depth cache.
+int RadiosityEngine::gather(int depth, const RadQuery *q, const uint32_t *idx, uint32_t m, bool final,
+                            float *Eg) {
+    if (m == 0) return PVX_OK;
+    const uint32_t N = (uint32_t)rays_per_sample(depth);
+    const size_t nr = (size_t)m * N;
+    Scratch &S = scratch[depth];
+    int r;
+    if ((r = S.rays.reserve(nr * sizeof(pvx_ray))) || (r = S.hits.reserve(nr * sizeof(pvx_hit))) ||
+        (r = S.base.reserve(nr * 12)) || (r = S.coef.reserve(nr * 12)) || (r = S.amb.reserve(nr * 12)) ||
+        (r = S.q.reserve(nr * sizeof(RadQuery))) || (r = S.need.reserve(nr)) ||
+        (r = S.rec.reserve(m * sizeof(RadRec))) || (r = S.rec_ok.reserve(m)))
+        return r;
+    k_gather_rays<<<blocks(nr), 256, 0, stream>>>(q, idx, m, N, static_cast<const double *>(d_pool.p),
+                                                  cursor[depth], static_cast<pvx_ray *>(S.rays.p));
+    HIP_TRY(hipGetLastError());
+    cursor[depth] = (uint32_t)((cursor[depth] + (uint64_t)m * N) % kPoolSize);
+    if ((r = trace_closest_internal(scene, static_cast<const pvx_ray *>(S.rays.p), nullptr,
+                                    PVX_RAY_RADIOSITY, nr, static_cast<pvx_hit *>(S.hits.p), stream)))
+        return r;
+    stats.gathers[std::min(depth, 1)] += m;
+    stats.gather_rays += nr;
+    bool deeper = depth + 1 < (int)rs.recursion_limit;
+    float wnext = std::max(rs.adc_bailout + 1e-10f, (float)std::pow(0.5 * rs.brightness, depth + 1.0));
+    ShadeParams sp;
+    sp.sv = scene->view;
+    sp.obj_type = scene->d_obj_type;
+    sp.obj_index = scene->d_obj_index;
+    sp.mats = scene->shading.d_mats;
+    sp.adc = rs.adc_bailout;
+    k_hit_queries<<<blocks(nr), 256, 0, stream>>>(sp, static_cast<const pvx_ray *>(S.rays.p),
+                                                  static_cast<const pvx_hit *>(S.hits.p), (uint32_t)nr,
+                                                  wnext, deeper ? 1 : 0, static_cast<float *>(S.base.p),
+                                                  static_cast<float *>(S.coef.p),
+                                                  static_cast<RadQuery *>(S.q.p),
+                                                  static_cast<uint8_t *>(S.need.p));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemsetAsync(S.amb.p, 0, nr * 12, stream));
+    if (deeper) {
+        // compact the hits that need irradiance at depth + 1 (deterministic order)
+        std::vector<uint8_t> need(nr);
+        HIP_TRY(hipMemcpyAsync(need.data(), S.need.p, nr, hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        std::vector<uint32_t> sel;
+        sel.reserve(nr);
+        for (size_t k = 0; k < nr; k++)
+            if (need[k]) sel.push_back((uint32_t)k);
+        if (!sel.empty()) {
+            DevBuf &di = S.sel;
+            if ((r = di.reserve(sel.size() * 4)) || (r = S.subq.reserve(sel.size() * sizeof(RadQuery))) ||
+                (r = S.subE.reserve(sel.size() * 12)))
+                return r;
+            HIP_TRY(hipMemcpyAsync(di.p, sel.data(), sel.size() * 4, hipMemcpyHostToDevice, stream));
+            k_gather_subq<<<blocks(sel.size()), 256, 0, stream>>>(
+                static_cast<const RadQuery *>(S.q.p), static_cast<const uint32_t *>(di.p),
+                (uint32_t)sel.size(), static_cast<RadQuery *>(S.subq.p));
+            if ((r = resolve(depth + 1, static_cast<const RadQuery *>(S.subq.p), (uint32_t)sel.size(), final,
+                             static_cast<float *>(S.subE.p))))
+                return r;
+            k_scatter3<<<blocks(sel.size()), 256, 0, stream>>>(
+                static_cast<const float *>(S.subE.p), static_cast<const uint32_t *>(di.p),
+                (uint32_t)sel.size(), static_cast<float *>(S.amb.p));
+            HIP_TRY(hipGetLastError());
+        }
+    }
+    float factor = std::pow(2.0f, (float)depth);
+    k_gather_reduce<<<blocks(m, 128), 128, 0, stream>>>(
+        q, idx, m, N, static_cast<const pvx_ray *>(S.rays.p), static_cast<const pvx_hit *>(S.hits.p),
+        static_cast<const float *>(S.base.p), static_cast<const float *>(S.coef.p),
+        static_cast<const float *>(S.amb.p), cam.location[0], cam.location[1], cam.location[2],
+        rs.minimum_reuse, rs.maximum_reuse, factor, Eg, static_cast<RadRec *>(S.rec.p),
+        static_cast<uint8_t *>(S.rec_ok.p));
+    HIP_TRY(hipGetLastError());
+    // append the records to the cache (AddBlock)
+    std::vector<RadRec> rec(m);
+    std::vector<uint8_t> ok(m);
+    HIP_TRY(hipMemcpyAsync(rec.data(), S.rec.p, m * sizeof(RadRec), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipMemcpyAsync(ok.data(), S.rec_ok.p, m, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    RadCache &c = caches[depth];
+    for (uint32_t k = 0; k < m; k++)
+        if (ok[k]) c.recs.push_back(rec[k]);
+    return c.rebuild(stream);
+}
+
+// ComputeAmbient for n queries at `depth`: reuse the cache where it has
+// enough weight, gather the rest in rounds (spread-out subsets first).
+int RadiosityEngine::resolve(int depth, const RadQuery *q, uint32_t n, bool final, float *E) {
+    if (n == 0) return PVX_OK;
+    Resolve &R = rstate[depth];
+    int r;
+    if ((r = R.reuse.reserve(n * 4)) || (r = R.need.reserve(n)) || (r = R.idx.reserve(n * 4)) ||
+        (r = R.Eg.reserve(n * 12)))
+        return r;
+    float *reuse = static_cast<float *>(R.reuse.p);
+    if ((r = lookup(depth, q, n, final, E, reuse))) return r;
+    k_need<<<blocks(n), 256, 0, stream>>>(reuse, n, reuse_count(depth), final ? 1 : 0,
+                                          static_cast<uint8_t *>(R.need.p));
+    std::vector<uint8_t> need(n);
+    HIP_TRY(hipMemcpyAsync(need.data(), R.need.p, n, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    std::vector<uint32_t> todo;
+    for (uint32_t i = 0; i < n; i++)
+        if (need[i]) todo.push_back(i);
+    if (depth == 0 && final) stats.final_misses += todo.size();
+    std::vector<RadQuery> hq;
+    if (!todo.empty()) {
+        hq.resize(n);
+        HIP_TRY(hipMemcpyAsync(hq.data(), q, (size_t)n * sizeof(RadQuery), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+    }
+    const double factor = std::pow(2.0, (double)depth);
+    for (int round = 0; !todo.empty(); round++) {
+        // Spread-out subset: the first pending point of every grid cell whose
+        // edge is the typical reach of a record (median harmonic-mean radius
+        // of the records so far times the error bound), so one round's
+        // gathers do not crowd each other and the rest of each cell can reuse
+        // them, as with the sequential order.
+        std::vector<uint32_t> sel, rest;
+        if (round < kMaxResolveRounds - 1 && todo.size() > 64) {
+            std::vector<double> rad;
+            const RadCache &c = caches[depth];
+            if (!c.recs.empty()) {
+                size_t stride = std::max<size_t>(1, c.recs.size() / 4096);
+                for (size_t k = 0; k < c.recs.size(); k += stride) rad.push_back(c.recs[k].r);
+            } else { // nothing gathered yet at this depth: half the largest reach
+                size_t stride = std::max<size_t>(1, todo.size() / 4096);
+                for (size_t k = 0; k < todo.size(); k += stride) {
+                    const RadQuery &a = hq[todo[k]];
+                    double ex = cam.location[0] - a.p[0], ey = cam.location[1] - a.p[1],
+                           ez = cam.location[2] - a.p[2];
+                    rad.push_back(0.5 * std::sqrt(ex * ex + ey * ey + ez * ez) * rs.maximum_reuse * factor);
+                }
+            }
+            std::nth_element(rad.begin(), rad.begin() + rad.size() / 2, rad.end());
+            double D = std::max(rad[rad.size() / 2] * rs.error_bound * factor, 1e-9);
+            std::unordered_set<uint64_t> taken;
+            taken.reserve(todo.size());
+            for (uint32_t i : todo) {
+                const RadQuery &a = hq[i];
+                uint64_t key = 0;
+                for (int c = 0; c < 3; c++)
+                    key = key * 2097143u + (uint64_t)(int64_t)std::floor(a.p[c] / D);
+                (taken.insert(key).second ? sel : rest).push_back(i);
+            }
+        } else {
+            sel.swap(todo);
+        }
+        stats_rounds[depth]++;
+        HIP_TRY(hipMemcpyAsync(R.idx.p, sel.data(), sel.size() * 4, hipMemcpyHostToDevice, stream));
+        if ((r = gather(depth, q, static_cast<const uint32_t *>(R.idx.p), (uint32_t)sel.size(), final,
+                        static_cast<float *>(R.Eg.p))))
+            return r;
+        k_combine<<<blocks(sel.size()), 256, 0, stream>>>(static_cast<const uint32_t *>(R.idx.p),
+                                                          (uint32_t)sel.size(),
+                                                          static_cast<const float *>(R.Eg.p), E, reuse);
+        HIP_TRY(hipGetLastError());
+        if (rest.empty()) break;
+        // look the rest up again against the grown cache
+        if ((r = R.subq.reserve(rest.size() * sizeof(RadQuery))) || (r = R.subE.reserve(rest.size() * 12)) ||
+            (r = R.subreuse.reserve(rest.size() * 4)) || (r = R.idx2.reserve(rest.size() * 4)))
+            return r;
+        HIP_TRY(hipMemcpyAsync(R.idx2.p, rest.data(), rest.size() * 4, hipMemcpyHostToDevice, stream));
+        k_gather_subq<<<blocks(rest.size()), 256, 0, stream>>>(q, static_cast<const uint32_t *>(R.idx2.p),
+                                                               (uint32_t)rest.size(),
+                                                               static_cast<RadQuery *>(R.subq.p));
+        if ((r = lookup(depth, static_cast<const RadQuery *>(R.subq.p), (uint32_t)rest.size(), final,
+                        static_cast<float *>(R.subE.p), static_cast<float *>(R.subreuse.p))))
+            return r;
+        k_scatter_lookup<<<blocks(rest.size()), 256, 0, stream>>>(
+            static_cast<const float *>(R.subE.p), static_cast<const float *>(R.subreuse.p),
+            static_cast<const uint32_t *>(R.idx2.p), (uint32_t)rest.size(), E, reuse);
+        k_need<<<blocks(n), 256, 0, stream>>>(reuse, n, reuse_count(depth), final ? 1 : 0,
+                                              static_cast<uint8_t *>(R.need.p));
+        HIP_TRY(hipMemcpyAsync(need.data(), R.need.p, n, hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        todo.clear();
+        for (uint32_t i : rest)
+            if (need[i]) todo.push_back(i);
+    }
+    if (rs.brightness != 1.0f) k_scale<<<blocks(3 * (size_t)n), 256, 0, stream>>>(E, n, rs.brightness);
+    return PVX_OK;
+}
+
+} // namespace pvx
+
+// ----------------------------------------------------------------------------
+// entry point
+// ----------------------------------------------------------------------------
+namespace pvx {
+
+// One pass over a pixel set: primary rays, depth-0 irradiance for every hit
+// that wants it, and (final pass) the pixel colours.
+int RadiosityEngine::pixel_pass(const std::vector<uint32_t> &px, bool final, float *rgb) {
+    const uint32_t n = (uint32_t)px.size();
+    if (n == 0) return PVX_OK;
+    Scratch &S = prim;
+    int r;
+    if ((r = S.sel.reserve(n * 4)) || (r = S.rays.reserve(n * sizeof(pvx_ray))) ||
+        (r = S.hits.reserve(n * sizeof(pvx_hit))) || (r = S.base.reserve(n * 12)) ||
+        (r = S.coef.reserve(n * 12)) || (r = S.q.reserve(n * sizeof(RadQuery))) || (r = S.need.reserve(n)) ||
+        (r = S.subq.reserve(n * sizeof(RadQuery))) || (r = S.subE.reserve(n * 12)) ||
+        (r = S.rec.reserve(n * 4)))
+        return r;
+    HIP_TRY(hipMemcpyAsync(S.sel.p, px.data(), n * 4, hipMemcpyHostToDevice, stream));
+    k_pixel_rays<<<blocks(n), 256, 0, stream>>>(cam, static_cast<const uint32_t *>(S.sel.p), n,
+                                                static_cast<pvx_ray *>(S.rays.p));
+    HIP_TRY(hipGetLastError());
+    if ((r = trace_closest_internal(scene, static_cast<const pvx_ray *>(S.rays.p), nullptr, PVX_RAY_IMAGE, n,
+                                    static_cast<pvx_hit *>(S.hits.p), stream)))
+        return r;
+    ShadeParams sp;
+    sp.sv = scene->view;
+    sp.obj_type = scene->d_obj_type;
+    sp.obj_index = scene->d_obj_index;
+    sp.mats = scene->shading.d_mats;
+    sp.adc = rs.adc_bailout;
+    k_hit_queries<<<blocks(n), 256, 0, stream>>>(sp, static_cast<const pvx_ray *>(S.rays.p),
+                                                 static_cast<const pvx_hit *>(S.hits.p), n, 1.0f,
+                                                 rs.recursion_limit > 0 ? 1 : 0, static_cast<float *>(S.base.p),
+                                                 static_cast<float *>(S.coef.p), static_cast<RadQuery *>(S.q.p),
+                                                 static_cast<uint8_t *>(S.need.p));
+    HIP_TRY(hipGetLastError());
+    std::vector<uint8_t> need(n);
+    HIP_TRY(hipMemcpyAsync(need.data(), S.need.p, n, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    std::vector<uint32_t> sel, qpix(n, 0);
+    for (uint32_t i = 0; i < n; i++)
+        if (need[i]) {
+            qpix[i] = (uint32_t)sel.size();
+            sel.push_back(i);
+        }
+    if (!sel.empty()) {
+        const uint32_t m = (uint32_t)sel.size();
+        HIP_TRY(hipMemcpyAsync(S.sel.p, sel.data(), m * 4, hipMemcpyHostToDevice, stream));
+        k_gather_subq<<<blocks(m), 256, 0, stream>>>(static_cast<const RadQuery *>(S.q.p),
+                                                     static_cast<const uint32_t *>(S.sel.p), m,
+                                                     static_cast<RadQuery *>(S.subq.p));
+        HIP_TRY(hipGetLastError());
+        if ((r = resolve(0, static_cast<const RadQuery *>(S.subq.p), m, final, static_cast<float *>(S.subE.p))))
+            return r;
+    }
+    if (final) {
+        HIP_TRY(hipMemcpyAsync(S.rec.p, qpix.data(), n * 4, hipMemcpyHostToDevice, stream));
+        k_final_colour<<<blocks(n), 256, 0, stream>>>(
+            static_cast<const float *>(S.base.p), static_cast<const float *>(S.coef.p),
+            static_cast<const float *>(S.subE.p), static_cast<const uint32_t *>(S.rec.p), n,
+            static_cast<const uint8_t *>(S.need.p), rgb);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(stream));
+    }
+    return PVX_OK;
+}
+
+} // namespace pvx
+
+extern "C" int pvx_radiosity_render(pvx_scene *s, const pvx_camera *cam, const pvx_radiosity_settings *set,
+                                    float *rgb, pvx_radiosity_stats *stats, void *stream) {
+    if (!s || !cam || !set || !rgb) return fail(PVX_EINVAL, "null argument");
+    if (!s->shading.set) return fail(PVX_ESTATE, "pvx_scene_set_shading was not called");
+    if (cam->width == 0 || cam->height == 0) return fail(PVX_EINVAL, "bad image size");
+    size_t npix = (size_t)cam->width * cam->height;
+    if (npix > 0x3FFFFFFFu) return fail(PVX_EINVAL, "image too large");
+    if (set->count == 0 || set->count > 1600) return fail(PVX_EINVAL, "count must be in [1, 1600]");
+    if (set->nearest_count == 0 || set->nearest_count > 20)
+        return fail(PVX_EINVAL, "nearest_count must be in [1, 20]");
+    if (set->recursion_limit > (uint32_t)kMaxRadDepth)
+        return fail(PVX_EUNSUPPORTED, "recursion_limit above " + std::to_string(kMaxRadDepth));
+    if (!(set->error_bound > 0.0f) || !(set->minimum_reuse > 0.0f) || !(set->maximum_reuse >= set->minimum_reuse) ||
+        !(set->pretrace_start > 0.0f) || !(set->pretrace_end > 0.0f) || !(set->pretrace_end <= set->pretrace_start))
+        return fail(PVX_EINVAL, "bad radiosity settings");
+    if (set->mode != PVX_RADIOSITY_PLAIN && set->mode != PVX_RADIOSITY_LEARNED)
+        return fail(PVX_EINVAL, "unknown radiosity mode");
+    if (set->mode == PVX_RADIOSITY_LEARNED) return fail(PVX_EUNSUPPORTED, "learned radiosity is not built yet");
+    for (const MatRec &m : s->shading.mats)
+        if (m.reflective)
+            return fail(PVX_EUNSUPPORTED, "radiosity frames take plain diffuse / emissive finishes only");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    RadiosityEngine eng(s, *set, *cam, st);
+    int r;
+    if ((r = eng.init())) return r;
+    hipEvent_t e0, e1, e2;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventCreate(&e2));
+    HIP_TRY(hipEventRecord(e0, st));
+    const uint32_t W = cam->width, H = cam->height;
+    // pretrace: pixel step from pretrace_start to pretrace_end of the larger
+    // image side, halving (View::StartRender pretrace steps,
+    // backend/scene/view.cpp:1056-1106; integer pixel steps here)
+    double step = std::max(1.0, std::floor((double)set->pretrace_start * std::max(W, H)));
+    double end = std::max(1.0, std::floor((double)set->pretrace_end * std::max(W, H)));
+    for (;;) {
+        std::vector<uint32_t> px;
+        uint32_t k = (uint32_t)step;
+        for (uint32_t y = 0; y < H; y += k)
+            for (uint32_t x = 0; x < W; x += k) px.push_back(y * W + x);
+        if ((r = eng.pixel_pass(px, false, nullptr))) return r;
+        if (step <= end) break;
+        step = std::max(end, std::floor(step / 2));
+    }
+    HIP_TRY(hipEventRecord(e1, st));
+    std::vector<uint32_t> px(npix);
+    for (size_t i = 0; i < npix; i++) px[i] = (uint32_t)i;
+    if ((r = eng.pixel_pass(px, true, rgb))) return r;
+    HIP_TRY(hipEventRecord(e2, st));
+    HIP_TRY(hipEventSynchronize(e2));
+    if (stats) {
+        float a = 0, b = 0;
+        HIP_TRY(hipEventElapsedTime(&a, e0, e1));
+        HIP_TRY(hipEventElapsedTime(&b, e1, e2));
+        *stats = eng.stats;
+        stats->pretrace_ms = a;
+        stats->final_ms = b;
+        stats->frame_ms = (double)a + b;
+        stats->records[0] = eng.caches[0].recs.size();
+        stats->records[1] = eng.caches[1].recs.size();
+    }
+    if (std::getenv("PVX_RADIOSITY_VERBOSE"))
+        std::fprintf(stderr, "[pvx radiosity] rounds d0=%llu d1=%llu gathers d0=%llu d1=%llu records d0=%zu d1=%zu\n",
+                     (unsigned long long)eng.stats_rounds[0], (unsigned long long)eng.stats_rounds[1],
+                     (unsigned long long)eng.stats.gathers[0], (unsigned long long)eng.stats.gathers[1],
+                     eng.caches[0].recs.size(), eng.caches[1].recs.size());
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e2);
+    return PVX_OK;
+}
+

@@ -52,6 +52,8 @@ def lib():
         L.pvx_copy.argtypes = [vp, vp, sz, vp]
         L.pvx_scene_set_shading.argtypes = [vp, C.POINTER(A.PvxShadingDesc)]
         L.pvx_render.argtypes = [vp, C.POINTER(A.PvxCamera), vp, C.POINTER(A.PvxRenderStats), vp]
+        L.pvx_radiosity_render.argtypes = [vp, C.POINTER(A.PvxCamera), C.POINTER(A.PvxRadiositySettings),
+                                           vp, C.POINTER(A.PvxRadiosityStats), vp]
         _L = L
     return _L
 
@@ -61,6 +63,7 @@ EXPORTED_SYMBOLS = (
     "pvx_scene_get_stats", "pvx_trace_closest", "pvx_trace_closest_counted",
     "pvx_all_intersections", "pvx_inside",
     "pvx_camera_rays", "pvx_copy", "pvx_scene_set_shading", "pvx_render",
+    "pvx_radiosity_render",
 )
 
 
@@ -176,6 +179,30 @@ class GpuScene:
         cam = self.scene.camera_struct(width=width, height=height)
         out = hip.DeviceBuffer(cam.width * cam.height * 12)
         stats = self.render_device(out, width, height)
+        hip.synchronize()
+        img = out.download(np.float32, cam.width * cam.height * 3)
+        return img.reshape(cam.height, cam.width, 3), stats
+
+    def radiosity_render_device(self, out, width=None, height=None, mode=A.PVX_RADIOSITY_PLAIN,
+                                stream=None, **over):
+        """Radiosity frame into a device buffer of 3*w*h floats (pvx_radiosity_render);
+        returns the stats dict."""
+        cam = self.scene.camera_struct(width=width, height=height)
+        rs = A.radiosity_settings(self.scene, mode=mode, **over)
+        st = A.PvxRadiosityStats()
+        _check(lib().pvx_radiosity_render(self.h, C.byref(cam), C.byref(rs), _ptr(out), C.byref(st),
+                                          stream.h if stream else None), "pvx_radiosity_render")
+        d = {}
+        for k, _ in A.PvxRadiosityStats._fields_:
+            v = getattr(st, k)
+            d[k] = list(v) if hasattr(v, "__len__") else v
+        return d
+
+    def radiosity_render(self, width=None, height=None, mode=A.PVX_RADIOSITY_PLAIN, **over):
+        """Render one radiosity frame; returns ((h, w, 3) float32 RGB, stats)."""
+        cam = self.scene.camera_struct(width=width, height=height)
+        out = hip.DeviceBuffer(cam.width * cam.height * 12)
+        stats = self.radiosity_render_device(out, width, height, mode=mode, **over)
         hip.synchronize()
         img = out.download(np.float32, cam.width * cam.height * 3)
         return img.reshape(cam.height, cam.width, 3), stats

@@ -560,9 +560,10 @@ def radiosity_scene(seed: int = 0xC0FFEE + 3, n_boxes: int = 5000, width=1280, h
     (count 200, recursion_limit 2, error_bound 0.5)."""
     r = Rng(seed)
     ext = r.uniform(3 * n_boxes, 0.05, 0.8).reshape(n_boxes, 3)
-    # boxes fill the room except a 1.5 m slab in front of the camera (z < -3.2)
-    pos = np.stack([r.uniform(n_boxes, -4.5, 4.5), r.uniform(n_boxes, 0.0, 3.3),
-                    r.uniform(n_boxes, -2.7, 4.5)], 1)
+    # boxes clutter the floor (resting on it or slightly raised), leaving a
+    # 1.5 m strip in front of the camera free
+    pos = np.stack([r.uniform(n_boxes, -4.6, 4.6), 0.5 * ext[:, 1] + r.uniform(n_boxes, 0.0, 0.5),
+                    r.uniform(n_boxes, -3.0, 4.6)], 1)
     rot = r.uniform(n_boxes, 0.0, 360.0)
     alb = r.uniform(3 * n_boxes, 0.2, 0.9).reshape(n_boxes, 3)
     xforms = []
@@ -589,7 +590,7 @@ def radiosity_scene(seed: int = 0xC0FFEE + 3, n_boxes: int = 5000, width=1280, h
     n = nb + 6
     pig = np.concatenate([alb, np.ones((1, 3)), np.full((6, 3), 0.75)], 0)
     emission = np.zeros((n, 3))
-    emission[n_boxes] = [3.0, 3.0, 3.0]
+    emission[n_boxes] = [6.0, 6.0, 6.0]
     diffuse = np.full(n, 0.75)
     diffuse[n_boxes] = 0.0
     return Scene(
@@ -597,7 +598,7 @@ def radiosity_scene(seed: int = 0xC0FFEE + 3, n_boxes: int = 5000, width=1280, h
         box_corner1=np.array(c1), box_corner2=np.array(c2), box_transform=np.array(tr, np.int32),
         transforms=np.array(xforms).reshape(-1, 32),
         plane_normal=planes_n, plane_distance=planes_d,
-        camera=look_at_camera((0.0, 2.0, -4.7), (0.0, 1.4, 0.0), width, height),
+        camera=look_at_camera((0.0, 2.4, -4.7), (0.0, 0.5, 0.5), width, height),
         lights=[],
         materials=dict(pigment=pig, diffuse=diffuse, ambient=np.zeros(n), phong=np.zeros(n),
                        phong_size=np.full(n, 40.0), reflection=np.zeros(n), ior=np.ones(n),
