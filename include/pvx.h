@@ -387,6 +387,34 @@ int pvx_radiosity_render(pvx_scene *scene, const pvx_camera *cam,
                          const pvx_radiosity_settings *settings, float *rgb,
                          pvx_radiosity_stats *stats, void *stream);
 
+/* ---- irradiance surrogate ------------------------------------------------ */
+/* No reference counterpart: a small MLP that stands in for
+ * RadiosityFunction::GatherLight (core/lighting/radiosity.cpp:509-743) at
+ * cache misses.  Input per point: 6 floats (position normalised to [0,1]^3,
+ * unit normal); 16-frequency sin/cos features per input (pi 2^(k/3) x,
+ * k = 0..15) -> 3 ReLU layers of width 64 -> RGB irradiance.  BF16 MFMA with
+ * FP32 accumulation; FP32 master weights and Adam (beta 0.9 / 0.99).
+ * Parameter vector (FP32, PVX_SURROGATE_PARAMS): W1[192][64], b1[64],
+ * W2[64][64], b2[64], W3[64][64], b3[64], W4[64][3], b4[3] (row = input). */
+enum { PVX_SURROGATE_PARAMS = 20867 };
+typedef struct pvx_surrogate pvx_surrogate;
+
+/* params: host array of PVX_SURROGATE_PARAMS floats, or NULL for a seeded
+ * He-uniform initialisation. */
+int pvx_surrogate_create(const float *params, uint64_t seed, pvx_surrogate **out);
+void pvx_surrogate_destroy(pvx_surrogate *s);
+int pvx_surrogate_get_params(pvx_surrogate *s, float *params /* host */);
+/* Loss L = (1/n) sum_p |y_p - target_p|^2 and dL/dparams (host outputs) over
+ * the n points (device x: n*6, target: n*3). */
+int pvx_surrogate_grad(pvx_surrogate *s, const float *x, const float *target, size_t n, float *loss,
+                       float *grad, void *stream);
+/* `steps` Adam steps on seeded mini-batches of `batch` points (multiple of
+ * 256) drawn from the n samples; learning rate decays from lr to lr/10. */
+int pvx_surrogate_train(pvx_surrogate *s, const float *x, const float *target, size_t n, uint32_t steps,
+                        uint32_t batch, float lr, float *final_loss, void *stream);
+/* out (device, n*3) = network(x). */
+int pvx_surrogate_infer(pvx_surrogate *s, const float *x, size_t n, float *out, void *stream);
+
 /* ---- utility ----------------------------------------------------------- */
 /* Plain device-to-device copy kernel (16 B per lane per step).  No reference
  * counterpart: it is the known-size workload used to calibrate hardware byte

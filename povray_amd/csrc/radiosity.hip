@@ -14,7 +14,8 @@
 // Lookups follow AverageNearBlock (radiosity.cpp:1551-1769): quick-check
 // radius, nearest-surface radius blend, error_reuse = dist/r + 2 sqrt(1-n.n'),
 // in-front test, saw weight (pseudo-sigmoid for depth-0 records in the final
-// pass).  The cache lives in a uniform grid rebuilt after each round.
+// pass).  A query keeps running sums (sum w*E, sum w), so after a round only
+// the round's new records (their own small grid) are visited again.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -53,24 +54,35 @@ struct CacheView {
 
 __device__ inline float grey3(const float *c) { return 0.297f * c[0] + 0.589f * c[1] + 0.114f * c[2]; }
 
-// FindReusableBlock + AverageNearBlock for a batch of queries.
-__global__ void k_lookup(const RadQuery *__restrict__ q, uint32_t n, CacheView c, float eb,
-                         float factor, int sigmoid, float *__restrict__ E, float *__restrict__ reuse) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+// FindReusableBlock + AverageNearBlock for a batch of queries: adds the
+// weighted illuminance of every record of the grid that qualifies to the
+// query's running sums (or replaces them when `accumulate` is 0).
+__global__ void k_lookup(const RadQuery *__restrict__ q, const uint32_t *__restrict__ idx, uint32_t n,
+                         CacheView c, float eb, float factor, int sigmoid, int accumulate,
+                         RadAcc *__restrict__ acc) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t i = idx ? idx[t] : t;
     const RadQuery qq = q[i];
     double bound = (double)eb;
     if (qq.w < kWeightErrOffset) bound += (double)(kWeightErrOffset - qq.w);
     bound *= (double)factor;
     double W = 0.0, we[3] = {0, 0, 0};
     if (c.n > 0) {
-        int cx = (int)floorf(((float)qq.p[0] - c.lo[0]) / c.cell);
-        int cy = (int)floorf(((float)qq.p[1] - c.lo[1]) / c.cell);
-        int cz = (int)floorf(((float)qq.p[2] - c.lo[2]) / c.cell);
+        const float qx = (float)qq.p[0], qy = (float)qq.p[1], qz = (float)qq.p[2];
+        int cx = (int)floorf((qx - c.lo[0]) / c.cell);
+        int cy = (int)floorf((qy - c.lo[1]) / c.cell);
+        int cz = (int)floorf((qz - c.lo[2]) / c.cell);
         if (cx >= 0 && cy >= 0 && cz >= 0 && cx < c.dims[0] && cy < c.dims[1] && cz < c.dims[2]) {
+            const float fb = (float)bound * 1.0001f;
             uint32_t cid = ((uint32_t)cz * c.dims[1] + (uint32_t)cy) * c.dims[0] + (uint32_t)cx;
             for (uint32_t k = c.cell_start[cid]; k < c.cell_start[cid + 1]; k++) {
-                const RadRec b = c.recs[c.cell_items[k]];
+                const RadRec *bp = c.recs + c.cell_items[k];
+                // FP32 pre-test of the quick-check sphere (conservative)
+                float fx = qx - bp->p[0], fy = qy - bp->p[1], fz = qz - bp->p[2];
+                float fq = bp->r * fb;
+                if (fx * fx + fy * fy + fz * fz > fq * fq + 1e-12f) continue;
+                const RadRec b = *bp;
                 double dx = qq.p[0] - b.p[0], dy = qq.p[1] - b.p[1], dz = qq.p[2] - b.p[2];
                 double sq = dx * dx + dy * dy + dz * dz;
                 double quick = (double)b.r * bound;
@@ -113,16 +125,20 @@ __global__ void k_lookup(const RadQuery *__restrict__ q, uint32_t n, CacheView c
             }
         }
     }
-    float e[3] = {0.0f, 0.0f, 0.0f};
-    if (W > 0) {
-        e[0] = (float)(we[0] / W);
-        e[1] = (float)(we[1] / W);
-        e[2] = (float)(we[2] / W);
+    RadAcc a;
+    if (accumulate) {
+        a = acc[i];
+        a.e[0] += we[0];
+        a.e[1] += we[1];
+        a.e[2] += we[2];
+        a.w += W;
+    } else {
+        a.e[0] = we[0];
+        a.e[1] = we[1];
+        a.e[2] = we[2];
+        a.w = W;
     }
-    E[3 * (size_t)i] = e[0];
-    E[3 * (size_t)i + 1] = e[1];
-    E[3 * (size_t)i + 2] = e[2];
-    reuse[i] = (float)W;
+    acc[i] = a;
 }
 
 // Gather rays: m queries x N directions from the cosine-weighted pool,
@@ -214,20 +230,20 @@ __global__ void k_hit_queries(ShadeParams p, const pvx_ray *__restrict__ rays,
     }
 }
 
-// GatherLight's reduction for m queries of N rays each.
-__global__ void k_gather_reduce(const RadQuery *__restrict__ q, const uint32_t *__restrict__ idx,
-                                uint32_t m, uint32_t N, const pvx_ray *__restrict__ rays,
-                                const pvx_hit *__restrict__ hits, const float *__restrict__ base,
-                                const float *__restrict__ coef, const float *__restrict__ amb,
-                                double cx, double cy, double cz, float min_reuse, float max_reuse,
-                                float factor, float *__restrict__ Eg, RadRec *__restrict__ rec,
-                                uint8_t *__restrict__ rec_ok) {
-    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= m) return;
-    const RadQuery qq = q[idx[s]];
+// GatherLight's reduction (radiosity.cpp:585-700) for m queries of N rays
+// each: one wavefront per query, lanes striding over the rays.
+__global__ __launch_bounds__(256) void k_gather_reduce(
+    const RadQuery *__restrict__ q, const uint32_t *__restrict__ idx, uint32_t m, uint32_t N,
+    const pvx_ray *__restrict__ rays, const pvx_hit *__restrict__ hits, const float *__restrict__ base,
+    const float *__restrict__ coef, const float *__restrict__ amb, double cx, double cy, double cz,
+    float min_reuse, float max_reuse, float factor, float *__restrict__ Eg, RadRec *__restrict__ rec,
+    uint8_t *__restrict__ rec_ok) {
+    const uint32_t s = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t lane = threadIdx.x & 63;
+    if (s >= m) return; // whole wavefronts leave together
     double sum[3] = {0, 0, 0}, inv = 0.0, smallest = kBoundHuge;
-    double md[3] = {0, 0, 0};
-    for (uint32_t j = 0; j < N; j++) {
+    uint32_t sj = 0xFFFFFFFFu;
+    for (uint32_t j = lane; j < N; j += 64) {
         size_t k = (size_t)s * N + j;
         const pvx_hit h = hits[k];
         double depth = HUGE_VAL;
@@ -236,18 +252,35 @@ __global__ void k_gather_reduce(const RadQuery *__restrict__ q, const uint32_t *
             for (int a = 0; a < 3; a++)
                 sum[a] += (double)(base[3 * k + a] + coef[3 * k + a] * amb[3 * k + a]);
         }
-        if (depth < smallest) {
+        if (depth < smallest) { // first (lowest j) of the smallest depths
             smallest = depth;
-            md[0] = rays[k].d[0];
-            md[1] = rays[k].d[1];
-            md[2] = rays[k].d[2];
+            sj = j;
         }
         inv += 1.0 / depth;
     }
+    for (int off = 32; off > 0; off >>= 1) {
+        for (int a = 0; a < 3; a++) sum[a] += __shfl_xor(sum[a], off, 64);
+        inv += __shfl_xor(inv, off, 64);
+        double os = __shfl_xor(smallest, off, 64);
+        uint32_t oj = __shfl_xor(sj, off, 64);
+        if (os < smallest || (os == smallest && oj < sj)) {
+            smallest = os;
+            sj = oj;
+        }
+    }
+    if (lane != 0) return;
+    const RadQuery qq = q[idx[s]];
     float e[3] = {(float)(sum[0] / N), (float)(sum[1] / N), (float)(sum[2] / N)};
     Eg[3 * (size_t)s] = e[0];
     Eg[3 * (size_t)s + 1] = e[1];
     Eg[3 * (size_t)s + 2] = e[2];
+    double md[3] = {0, 0, 0};
+    if (sj != 0xFFFFFFFFu) {
+        const pvx_ray &r = rays[(size_t)s * N + sj];
+        md[0] = r.d[0];
+        md[1] = r.d[1];
+        md[2] = r.d[2];
+    }
     double ex = cx - qq.p[0], ey = cy - qq.p[1], ez = cz - qq.p[2];
     double to_eye = sqrt(ex * ex + ey * ey + ez * ez);
     double rmin = to_eye * (double)min_reuse * factor, rmax = to_eye * (double)max_reuse * factor;
@@ -269,31 +302,41 @@ __global__ void k_gather_reduce(const RadQuery *__restrict__ q, const uint32_t *
     rec_ok[s] = ok ? 1 : 0;
 }
 
-// ComputeAmbient's combination of a partial reuse with the new gather.
+// ComputeAmbient's combination of the partial reuse with the new gather
+// (radiosity.cpp:430-440); marks the query as gathered.
 __global__ void k_combine(const uint32_t *__restrict__ idx, uint32_t m, const float *__restrict__ Eg,
-                          float *__restrict__ E, const float *__restrict__ reuse) {
+                          RadAcc *__restrict__ acc, float *__restrict__ E) {
     uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= m) return;
     uint32_t i = idx[s];
-    float w = reuse[i];
-    for (int a = 0; a < 3; a++) {
-        float g = Eg[3 * (size_t)s + a];
-        E[3 * (size_t)i + a] = w > 0.0f ? (E[3 * (size_t)i + a] * w + g * 1.0f) / (w + 1.0f) : g;
+    RadAcc a = acc[i];
+    for (int c = 0; c < 3; c++) {
+        double g = Eg[3 * (size_t)s + c];
+        E[3 * (size_t)i + c] = (float)(a.w > 0.0 ? (a.e[c] + g * 1.0) / (a.w + 1.0) : g);
+    }
+    a.w = -1.0;
+    acc[i] = a;
+}
+
+// Reused irradiance of the queries that were not gathered.
+__global__ void k_finish(const RadAcc *__restrict__ acc, uint32_t n, float brightness, float *__restrict__ E) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const RadAcc a = acc[i];
+    for (int c = 0; c < 3; c++) {
+        float v = E[3 * (size_t)i + c];
+        if (a.w >= 0.0) v = a.w > 0.0 ? (float)(a.e[c] / a.w) : 0.0f;
+        E[3 * (size_t)i + c] = v * brightness;
     }
 }
 
-// Needs-a-gather test of ComputeAmbient (radiosity.cpp:404-409).
-__global__ void k_need(const float *__restrict__ reuse, uint32_t n, float reuse_count, int final,
-                       uint8_t *__restrict__ need) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    float w = reuse[i];
-    need[i] = (w * 4.0f >= reuse_count || (final && w > 0.0f)) ? 0 : 1;
-}
-
-__global__ void k_scale(float *E, uint32_t n, float k) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < 3 * n) E[i] *= k;
+// Needs-a-gather test of ComputeAmbient (radiosity.cpp:410-411; gray_threshold 0).
+__global__ void k_need(const RadAcc *__restrict__ acc, const uint32_t *__restrict__ idx, uint32_t n,
+                       float reuse_count, int final, uint8_t *__restrict__ need) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    double w = acc[idx ? idx[t] : t].w;
+    need[t] = (w * 4.0 >= (double)reuse_count || (final && w > 0.0)) ? 0 : 1;
 }
 
 // pixel colour = base + coef * E (the primary hit's ComputeLightedTexture)
@@ -340,16 +383,6 @@ __global__ void k_scatter3(const float *__restrict__ src, const uint32_t *__rest
     for (int a = 0; a < 3; a++) dst[3 * (size_t)i + a] = src[3 * (size_t)s + a];
 }
 
-__global__ void k_scatter_lookup(const float *__restrict__ subE, const float *__restrict__ subreuse,
-                                 const uint32_t *__restrict__ idx, uint32_t m, float *__restrict__ E,
-                                 float *__restrict__ reuse) {
-    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= m) return;
-    uint32_t i = idx[s];
-    for (int a = 0; a < 3; a++) E[3 * (size_t)i + a] = subE[3 * (size_t)s + a];
-    reuse[i] = subreuse[s];
-}
-
 inline uint32_t blocks(size_t n, uint32_t b = 256) { return (uint32_t)((n + b - 1) / b); }
 
 } // namespace
@@ -372,28 +405,29 @@ int DevBuf::reserve(size_t bytes) {
     return PVX_OK;
 }
 
-int RadCache::rebuild(hipStream_t st) {
-    if (recs.empty()) return PVX_OK;
-    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    std::vector<float> rad(recs.size());
-    std::vector<float> rs(recs.size());
-    for (size_t i = 0; i < recs.size(); i++) {
-        rad[i] = recs[i].r * bound_max * 1.001f + 1e-6f; // >= quick-check radius
+int RadGrid::build(const std::vector<RadRec> &recs, size_t first, size_t last, float reach, hipStream_t st) {
+    n = (uint32_t)(last - first);
+    if (n == 0) return PVX_OK;
+    float lo_[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    std::vector<float> rad(n), rs(n);
+    for (size_t i = 0; i < n; i++) {
+        const RadRec &b = recs[first + i];
+        rad[i] = b.r * reach * 1.001f + 1e-6f; // >= any quick-check radius of the record
         rs[i] = rad[i];
         for (int a = 0; a < 3; a++) {
-            lo[a] = std::min(lo[a], recs[i].p[a] - rad[i]);
-            hi[a] = std::max(hi[a], recs[i].p[a] + rad[i]);
+            lo_[a] = std::min(lo_[a], b.p[a] - rad[i]);
+            hi[a] = std::max(hi[a], b.p[a] + rad[i]);
         }
     }
     std::nth_element(rs.begin(), rs.begin() + rs.size() / 2, rs.end());
-    float ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+    float ext = std::max(hi[0] - lo_[0], std::max(hi[1] - lo_[1], hi[2] - lo_[2]));
     // cells about the median sphere size, at most 64 per axis, grown until
     // the record->cell lists stay within a few entries per record
     cell = std::max(std::max(rs[rs.size() / 2], ext / 64.0f), 1e-6f);
-    const double budget = 8.0 * (double)recs.size() + 65536.0;
+    const double budget = 8.0 * (double)n + 65536.0;
     for (int tries = 0; tries < 64; tries++) {
         double total = 0;
-        for (size_t i = 0; i < recs.size() && total <= budget; i++) {
+        for (size_t i = 0; i < n && total <= budget; i++) {
             double k = 2.0 * rad[i] / cell + 2.0;
             total += k * k * k;
         }
@@ -401,14 +435,15 @@ int RadCache::rebuild(hipStream_t st) {
         cell *= 1.5f;
     }
     for (int a = 0; a < 3; a++) {
-        this->lo[a] = lo[a];
-        dims[a] = std::max(1, (int)std::ceil((hi[a] - lo[a]) / cell) + 1);
+        lo[a] = lo_[a];
+        dims[a] = std::max(1, (int)std::ceil((hi[a] - lo_[a]) / cell) + 1);
     }
     size_t ncell = (size_t)dims[0] * dims[1] * dims[2];
     std::vector<uint32_t> count(ncell + 1, 0);
     auto range = [&](size_t i, int a, int &c0, int &c1) {
-        c0 = std::max(0, (int)std::floor((recs[i].p[a] - rad[i] - lo[a]) / cell));
-        c1 = std::min(dims[a] - 1, (int)std::floor((recs[i].p[a] + rad[i] - lo[a]) / cell));
+        const RadRec &b = recs[first + i];
+        c0 = std::max(0, (int)std::floor((b.p[a] - rad[i] - lo[a]) / cell));
+        c1 = std::min(dims[a] - 1, (int)std::floor((b.p[a] + rad[i] - lo[a]) / cell));
     };
     for (int pass = 0; pass < 2; pass++) {
         if (pass == 1) {
@@ -421,7 +456,7 @@ int RadCache::rebuild(hipStream_t st) {
             items.assign(count[ncell], 0);
             start = count;
         }
-        for (size_t i = 0; i < recs.size(); i++) {
+        for (size_t i = 0; i < n; i++) {
             int x0, x1, y0, y1, z0, z1;
             range(i, 0, x0, x1);
             range(i, 1, y0, y1);
@@ -433,19 +468,39 @@ int RadCache::rebuild(hipStream_t st) {
                         if (pass == 0)
                             count[c]++;
                         else
-                            items[count[c]++] = (uint32_t)i;
+                            items[count[c]++] = (uint32_t)(first + i);
                     }
         }
     }
+    // count[] now holds the cell ends; start[] the cell starts
     int r;
-    if ((r = d_recs.reserve(recs.size() * sizeof(RadRec))) || (r = d_start.reserve(start.size() * 4)) ||
-        (r = d_items.reserve(std::max<size_t>(1, items.size()) * 4)))
+    if ((r = d_start.reserve(start.size() * 4)) || (r = d_items.reserve(std::max<size_t>(1, items.size()) * 4)))
         return r;
-    HIP_TRY(hipMemcpyAsync(d_recs.p, recs.data(), recs.size() * sizeof(RadRec), hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(d_start.p, start.data(), start.size() * 4, hipMemcpyHostToDevice, st));
     if (!items.empty())
         HIP_TRY(hipMemcpyAsync(d_items.p, items.data(), items.size() * 4, hipMemcpyHostToDevice, st));
-    HIP_TRY(hipStreamSynchronize(st));
+    return PVX_OK;
+}
+
+int RadCache::upload(hipStream_t st) {
+    if (uploaded == recs.size()) return PVX_OK;
+    size_t bytes = recs.size() * sizeof(RadRec);
+    if (bytes > d_recs.cap) { // grow (doubling) and send everything
+        int r;
+        if ((r = d_recs.reserve(std::max(bytes, 2 * d_recs.cap)))) return r;
+        uploaded = 0;
+    }
+    HIP_TRY(hipMemcpyAsync(static_cast<RadRec *>(d_recs.p) + uploaded, recs.data() + uploaded,
+                           (recs.size() - uploaded) * sizeof(RadRec), hipMemcpyHostToDevice, st));
+    uploaded = recs.size();
+    return PVX_OK;
+}
+
+int RadCache::index_all(hipStream_t st) {
+    if (indexed == recs.size()) return PVX_OK;
+    int r;
+    if ((r = upload(st)) || (r = all.build(recs, 0, recs.size(), reach, st))) return r;
+    indexed = recs.size();
     return PVX_OK;
 }
 
@@ -453,7 +508,7 @@ RadiosityEngine::RadiosityEngine(pvx_scene *s, const pvx_radiosity_settings &set
                                  hipStream_t st)
     : scene(s), rs(set), cam(c), stream(st) {
     for (int d = 0; d < kMaxRadDepth; d++) {
-        caches[d].bound_max = (float)(rs.error_bound + kWeightErrOffset) * std::pow(2.0f, (float)d);
+        caches[d].reach = (float)(rs.error_bound + kWeightErrOffset) * std::pow(2.0f, (float)d);
         cursor[d] = 0;
     }
     pool.resize(3 * kPoolSize);
@@ -492,21 +547,23 @@ float RadiosityEngine::reuse_count(int d) const {
     return (float)(d == 0 ? (int)rs.nearest_count : (d == 1 ? std::min(2, (int)rs.nearest_count) : 1));
 }
 
-int RadiosityEngine::lookup(int depth, const RadQuery *q, uint32_t n, bool final, float *E, float *reuse) {
+int RadiosityEngine::lookup(int depth, const RadGrid &g, const RadQuery *q, const uint32_t *idx, uint32_t n,
+                            bool final, bool accumulate, RadAcc *acc) {
     if (n == 0) return PVX_OK;
+    if (accumulate && g.n == 0) return PVX_OK;
     RadCache &c = caches[depth];
     CacheView v;
     v.recs = static_cast<const RadRec *>(c.d_recs.p);
-    v.cell_start = static_cast<const uint32_t *>(c.d_start.p);
-    v.cell_items = static_cast<const uint32_t *>(c.d_items.p);
+    v.cell_start = static_cast<const uint32_t *>(g.d_start.p);
+    v.cell_items = static_cast<const uint32_t *>(g.d_items.p);
     for (int a = 0; a < 3; a++) {
-        v.lo[a] = c.lo[a];
-        v.dims[a] = c.dims[a];
+        v.lo[a] = g.lo[a];
+        v.dims[a] = g.dims[a];
     }
-    v.cell = c.cell;
-    v.n = (uint32_t)c.recs.size();
-    k_lookup<<<blocks(n), 256, 0, stream>>>(q, n, v, rs.error_bound, std::pow(2.0f, (float)depth),
-                                            (final && depth == 0) ? 1 : 0, E, reuse);
+    v.cell = g.cell;
+    v.n = g.n;
+    k_lookup<<<blocks(n), 256, 0, stream>>>(q, idx, n, v, rs.error_bound, std::pow(2.0f, (float)depth),
+                                            (final && depth == 0) ? 1 : 0, accumulate ? 1 : 0, acc);
     HIP_TRY(hipGetLastError());
     return PVX_OK;
 }
@@ -578,7 +635,7 @@ int RadiosityEngine::gather(int depth, const RadQuery *q, const uint32_t *idx, u
         }
     }
     float factor = std::pow(2.0f, (float)depth);
-    k_gather_reduce<<<blocks(m, 128), 128, 0, stream>>>(
+    k_gather_reduce<<<blocks((size_t)m * 64), 256, 0, stream>>>(
         q, idx, m, N, static_cast<const pvx_ray *>(S.rays.p), static_cast<const pvx_hit *>(S.hits.p),
         static_cast<const float *>(S.base.p), static_cast<const float *>(S.coef.p),
         static_cast<const float *>(S.amb.p), cam.location[0], cam.location[1], cam.location[2],
@@ -594,7 +651,7 @@ int RadiosityEngine::gather(int depth, const RadQuery *q, const uint32_t *idx, u
     RadCache &c = caches[depth];
     for (uint32_t k = 0; k < m; k++)
         if (ok[k]) c.recs.push_back(rec[k]);
-    return c.rebuild(stream);
+    return c.upload(stream);
 }
 
 // ComputeAmbient for n queries at `depth`: reuse the cache where it has
@@ -602,27 +659,36 @@ int RadiosityEngine::gather(int depth, const RadQuery *q, const uint32_t *idx, u
 int RadiosityEngine::resolve(int depth, const RadQuery *q, uint32_t n, bool final, float *E) {
     if (n == 0) return PVX_OK;
     Resolve &R = rstate[depth];
+    RadCache &c = caches[depth];
     int r;
-    if ((r = R.reuse.reserve(n * 4)) || (r = R.need.reserve(n)) || (r = R.idx.reserve(n * 4)) ||
-        (r = R.Eg.reserve(n * 12)))
+    if ((r = R.acc.reserve((size_t)n * sizeof(RadAcc))) || (r = R.need.reserve(n)) ||
+        (r = R.idx.reserve((size_t)n * 4)) || (r = R.Eg.reserve((size_t)n * 12)) ||
+        (r = R.tq.reserve((size_t)n * sizeof(RadQuery))))
         return r;
-    float *reuse = static_cast<float *>(R.reuse.p);
-    if ((r = lookup(depth, q, n, final, E, reuse))) return r;
-    k_need<<<blocks(n), 256, 0, stream>>>(reuse, n, reuse_count(depth), final ? 1 : 0,
-                                          static_cast<uint8_t *>(R.need.p));
+    RadAcc *acc = static_cast<RadAcc *>(R.acc.p);
+    uint32_t *d_idx = static_cast<uint32_t *>(R.idx.p);
+    uint8_t *d_need = static_cast<uint8_t *>(R.need.p);
+    if ((r = c.index_all(stream)) || (r = lookup(depth, c.all, q, nullptr, n, final, false, acc))) return r;
+    k_need<<<blocks(n), 256, 0, stream>>>(acc, nullptr, n, reuse_count(depth), final ? 1 : 0, d_need);
     std::vector<uint8_t> need(n);
-    HIP_TRY(hipMemcpyAsync(need.data(), R.need.p, n, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipMemcpyAsync(need.data(), d_need, n, hipMemcpyDeviceToHost, stream));
     HIP_TRY(hipStreamSynchronize(stream));
     std::vector<uint32_t> todo;
     for (uint32_t i = 0; i < n; i++)
         if (need[i]) todo.push_back(i);
     if (depth == 0 && final) stats.final_misses += todo.size();
+    // positions of the pending queries, for the subset choice
     std::vector<RadQuery> hq;
     if (!todo.empty()) {
-        hq.resize(n);
-        HIP_TRY(hipMemcpyAsync(hq.data(), q, (size_t)n * sizeof(RadQuery), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipMemcpyAsync(d_idx, todo.data(), todo.size() * 4, hipMemcpyHostToDevice, stream));
+        k_gather_subq<<<blocks(todo.size()), 256, 0, stream>>>(q, d_idx, (uint32_t)todo.size(),
+                                                               static_cast<RadQuery *>(R.tq.p));
+        hq.resize(todo.size());
+        HIP_TRY(hipMemcpyAsync(hq.data(), R.tq.p, todo.size() * sizeof(RadQuery), hipMemcpyDeviceToHost, stream));
         HIP_TRY(hipStreamSynchronize(stream));
     }
+    std::vector<uint32_t> slot(todo.size()); // todo position -> hq entry
+    for (size_t k = 0; k < todo.size(); k++) slot[k] = (uint32_t)k;
     const double factor = std::pow(2.0, (double)depth);
     for (int round = 0; !todo.empty(); round++) {
         // Spread-out subset: the first pending point of every grid cell whose
@@ -630,17 +696,16 @@ int RadiosityEngine::resolve(int depth, const RadQuery *q, uint32_t n, bool fina
         // of the records so far times the error bound), so one round's
         // gathers do not crowd each other and the rest of each cell can reuse
         // them, as with the sequential order.
-        std::vector<uint32_t> sel, rest;
+        std::vector<uint32_t> sel, rest, rest_slot;
         if (round < kMaxResolveRounds - 1 && todo.size() > 64) {
             std::vector<double> rad;
-            const RadCache &c = caches[depth];
             if (!c.recs.empty()) {
                 size_t stride = std::max<size_t>(1, c.recs.size() / 4096);
                 for (size_t k = 0; k < c.recs.size(); k += stride) rad.push_back(c.recs[k].r);
             } else { // nothing gathered yet at this depth: half the largest reach
                 size_t stride = std::max<size_t>(1, todo.size() / 4096);
                 for (size_t k = 0; k < todo.size(); k += stride) {
-                    const RadQuery &a = hq[todo[k]];
+                    const RadQuery &a = hq[slot[k]];
                     double ex = cam.location[0] - a.p[0], ey = cam.location[1] - a.p[1],
                            ez = cam.location[2] - a.p[2];
                     rad.push_back(0.5 * std::sqrt(ex * ex + ey * ey + ez * ez) * rs.maximum_reuse * factor);
@@ -650,49 +715,46 @@ int RadiosityEngine::resolve(int depth, const RadQuery *q, uint32_t n, bool fina
             double D = std::max(rad[rad.size() / 2] * rs.error_bound * factor, 1e-9);
             std::unordered_set<uint64_t> taken;
             taken.reserve(todo.size());
-            for (uint32_t i : todo) {
-                const RadQuery &a = hq[i];
+            for (size_t k = 0; k < todo.size(); k++) {
+                const RadQuery &a = hq[slot[k]];
                 uint64_t key = 0;
-                for (int c = 0; c < 3; c++)
-                    key = key * 2097143u + (uint64_t)(int64_t)std::floor(a.p[c] / D);
-                (taken.insert(key).second ? sel : rest).push_back(i);
+                for (int d = 0; d < 3; d++) key = key * 2097143u + (uint64_t)(int64_t)std::floor(a.p[d] / D);
+                if (taken.insert(key).second) {
+                    sel.push_back(todo[k]);
+                } else {
+                    rest.push_back(todo[k]);
+                    rest_slot.push_back(slot[k]);
+                }
             }
         } else {
             sel.swap(todo);
         }
         stats_rounds[depth]++;
-        HIP_TRY(hipMemcpyAsync(R.idx.p, sel.data(), sel.size() * 4, hipMemcpyHostToDevice, stream));
-        if ((r = gather(depth, q, static_cast<const uint32_t *>(R.idx.p), (uint32_t)sel.size(), final,
-                        static_cast<float *>(R.Eg.p))))
-            return r;
-        k_combine<<<blocks(sel.size()), 256, 0, stream>>>(static_cast<const uint32_t *>(R.idx.p),
-                                                          (uint32_t)sel.size(),
-                                                          static_cast<const float *>(R.Eg.p), E, reuse);
+        const size_t first_new = c.recs.size();
+        HIP_TRY(hipMemcpyAsync(d_idx, sel.data(), sel.size() * 4, hipMemcpyHostToDevice, stream));
+        if ((r = gather(depth, q, d_idx, (uint32_t)sel.size(), final, static_cast<float *>(R.Eg.p)))) return r;
+        k_combine<<<blocks(sel.size()), 256, 0, stream>>>(d_idx, (uint32_t)sel.size(),
+                                                          static_cast<const float *>(R.Eg.p), acc, E);
         HIP_TRY(hipGetLastError());
         if (rest.empty()) break;
-        // look the rest up again against the grown cache
-        if ((r = R.subq.reserve(rest.size() * sizeof(RadQuery))) || (r = R.subE.reserve(rest.size() * 12)) ||
-            (r = R.subreuse.reserve(rest.size() * 4)) || (r = R.idx2.reserve(rest.size() * 4)))
-            return r;
-        HIP_TRY(hipMemcpyAsync(R.idx2.p, rest.data(), rest.size() * 4, hipMemcpyHostToDevice, stream));
-        k_gather_subq<<<blocks(rest.size()), 256, 0, stream>>>(q, static_cast<const uint32_t *>(R.idx2.p),
-                                                               (uint32_t)rest.size(),
-                                                               static_cast<RadQuery *>(R.subq.p));
-        if ((r = lookup(depth, static_cast<const RadQuery *>(R.subq.p), (uint32_t)rest.size(), final,
-                        static_cast<float *>(R.subE.p), static_cast<float *>(R.subreuse.p))))
-            return r;
-        k_scatter_lookup<<<blocks(rest.size()), 256, 0, stream>>>(
-            static_cast<const float *>(R.subE.p), static_cast<const float *>(R.subreuse.p),
-            static_cast<const uint32_t *>(R.idx2.p), (uint32_t)rest.size(), E, reuse);
-        k_need<<<blocks(n), 256, 0, stream>>>(reuse, n, reuse_count(depth), final ? 1 : 0,
-                                              static_cast<uint8_t *>(R.need.p));
-        HIP_TRY(hipMemcpyAsync(need.data(), R.need.p, n, hipMemcpyDeviceToHost, stream));
+        // the rest meets only this round's records again
+        if ((r = c.fresh.build(c.recs, first_new, c.recs.size(), c.reach, stream))) return r;
+        HIP_TRY(hipMemcpyAsync(d_idx, rest.data(), rest.size() * 4, hipMemcpyHostToDevice, stream));
+        if ((r = lookup(depth, c.fresh, q, d_idx, (uint32_t)rest.size(), final, true, acc))) return r;
+        k_need<<<blocks(rest.size()), 256, 0, stream>>>(acc, d_idx, (uint32_t)rest.size(), reuse_count(depth),
+                                                        final ? 1 : 0, d_need);
+        HIP_TRY(hipMemcpyAsync(need.data(), d_need, rest.size(), hipMemcpyDeviceToHost, stream));
         HIP_TRY(hipStreamSynchronize(stream));
         todo.clear();
-        for (uint32_t i : rest)
-            if (need[i]) todo.push_back(i);
+        slot.clear();
+        for (size_t k = 0; k < rest.size(); k++)
+            if (need[k]) {
+                todo.push_back(rest[k]);
+                slot.push_back(rest_slot[k]);
+            }
     }
-    if (rs.brightness != 1.0f) k_scale<<<blocks(3 * (size_t)n), 256, 0, stream>>>(E, n, rs.brightness);
+    k_finish<<<blocks(n), 256, 0, stream>>>(acc, n, rs.brightness, E);
+    HIP_TRY(hipGetLastError());
     return PVX_OK;
 }
 

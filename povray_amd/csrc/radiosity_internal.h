@@ -14,8 +14,10 @@ namespace pvx {
 constexpr int kMaxRadDepth = 4;
 constexpr int kMaxResolveRounds = 32; // gather rounds per resolve before the rest goes at once
 
-// One irradiance-cache record (ot_block fields that AverageNearBlock reads,
-// core/lighting/radiosity.h:93-105): 64 bytes.
+// One irradiance-cache record (the ot_block fields that AverageNearBlock
+// reads, core/support/octree.h:101-117): 64 bytes.  The reference keeps the
+// point in FP64; FP32 here moves a point by < 1e-6 of the scene size, far
+// below the smallest reuse radius.
 struct RadRec {
     float p[3];      // Point
     float n[3];      // S_Normal
@@ -36,6 +38,13 @@ struct RadQuery {
 };
 static_assert(sizeof(RadQuery) == 40, "RadQuery layout");
 
+// Running reuse state of one query: sum of weight * illuminance and sum of
+// weights over the records seen so far (w < 0 marks a gathered query).
+struct RadAcc {
+    double e[3];
+    double w;
+};
+
 // Device buffer grown on demand (contents are not preserved).
 struct DevBuf {
     void *p = nullptr;
@@ -47,17 +56,29 @@ struct DevBuf {
     int reserve(size_t bytes);
 };
 
-// Records of one recursion depth, with a uniform grid over their
-// quick-check spheres (cell -> record ids, CSR).
-struct RadCache {
-    std::vector<RadRec> recs;
-    std::vector<uint32_t> start, items;
+// Uniform grid over the quick-check spheres of a range of records
+// (cell -> record ids, CSR), host copy plus device copy.
+struct RadGrid {
     float lo[3] = {0, 0, 0};
     float cell = 1.0f;
     int dims[3] = {1, 1, 1};
-    float bound_max = 1.0f; // largest error bound a lookup at this depth uses
-    DevBuf d_recs, d_start, d_items;
-    int rebuild(hipStream_t st);
+    uint32_t n = 0; // records indexed
+    std::vector<uint32_t> start, items;
+    DevBuf d_start, d_items;
+    int build(const std::vector<RadRec> &recs, size_t first, size_t last, float reach, hipStream_t st);
+};
+
+// Records of one recursion depth.  `all` indexes every record; `fresh`
+// indexes the records of the latest gather round only.
+struct RadCache {
+    std::vector<RadRec> recs;
+    size_t uploaded = 0;  // records present in d_recs
+    size_t indexed = 0;   // records covered by `all`
+    float reach = 1.0f;   // largest quick-check radius / harmonic mean radius
+    DevBuf d_recs;
+    RadGrid all, fresh;
+    int upload(hipStream_t st);
+    int index_all(hipStream_t st);
 };
 
 struct RadiosityEngine {
@@ -76,7 +97,7 @@ struct RadiosityEngine {
         DevBuf rays, hits, base, coef, amb, q, need, rec, rec_ok, sel, subq, subE;
     } scratch[kMaxRadDepth];
     struct Resolve { // per resolve depth
-        DevBuf reuse, need, idx, Eg, subq, subE, subreuse, idx2;
+        DevBuf acc, need, idx, Eg, tq;
     } rstate[kMaxRadDepth];
     Scratch prim; // primary rays of a pixel pass
 
@@ -84,7 +105,8 @@ struct RadiosityEngine {
     int init();
     int rays_per_sample(int depth) const;
     float reuse_count(int depth) const;
-    int lookup(int depth, const RadQuery *q, uint32_t n, bool final, float *E, float *reuse);
+    int lookup(int depth, const RadGrid &g, const RadQuery *q, const uint32_t *idx, uint32_t n, bool final,
+               bool accumulate, RadAcc *acc);
     int gather(int depth, const RadQuery *q, const uint32_t *idx, uint32_t m, bool final, float *Eg);
     int resolve(int depth, const RadQuery *q, uint32_t n, bool final, float *E);
     int pixel_pass(const std::vector<uint32_t> &px, bool final, float *rgb);

@@ -54,6 +54,14 @@ def lib():
         L.pvx_render.argtypes = [vp, C.POINTER(A.PvxCamera), vp, C.POINTER(A.PvxRenderStats), vp]
         L.pvx_radiosity_render.argtypes = [vp, C.POINTER(A.PvxCamera), C.POINTER(A.PvxRadiositySettings),
                                            vp, C.POINTER(A.PvxRadiosityStats), vp]
+        fp = C.POINTER(C.c_float)
+        L.pvx_surrogate_create.argtypes = [fp, C.c_uint64, C.POINTER(vp)]
+        L.pvx_surrogate_destroy.argtypes = [vp]
+        L.pvx_surrogate_destroy.restype = None
+        L.pvx_surrogate_get_params.argtypes = [vp, fp]
+        L.pvx_surrogate_grad.argtypes = [vp, vp, vp, sz, fp, fp, vp]
+        L.pvx_surrogate_train.argtypes = [vp, vp, vp, sz, u32, u32, C.c_float, fp, vp]
+        L.pvx_surrogate_infer.argtypes = [vp, vp, sz, vp, vp]
         _L = L
     return _L
 
@@ -63,7 +71,8 @@ EXPORTED_SYMBOLS = (
     "pvx_scene_get_stats", "pvx_trace_closest", "pvx_trace_closest_counted",
     "pvx_all_intersections", "pvx_inside",
     "pvx_camera_rays", "pvx_copy", "pvx_scene_set_shading", "pvx_render",
-    "pvx_radiosity_render",
+    "pvx_radiosity_render", "pvx_surrogate_create", "pvx_surrogate_destroy",
+    "pvx_surrogate_get_params", "pvx_surrogate_grad", "pvx_surrogate_train", "pvx_surrogate_infer",
 )
 
 
@@ -217,6 +226,71 @@ class GpuScene:
         _check(lib().pvx_inside(self.h, d_p.value, d_o.value, n, d_out.value, None), "pvx_inside")
         hip.synchronize()
         return d_out.download(np.uint8, n).astype(bool)
+
+
+class Surrogate:
+    """The irradiance MLP (pvx_surrogate_*): 6 inputs -> RGB."""
+
+    N_PARAMS = 20867
+
+    def __init__(self, params=None, seed: int = 1):
+        h = C.c_void_p()
+        fp = C.POINTER(C.c_float)
+        if params is not None:
+            params = np.ascontiguousarray(params, np.float32)
+            assert params.size == self.N_PARAMS
+            ptr = params.ctypes.data_as(fp)
+        else:
+            ptr = None
+        _check(lib().pvx_surrogate_create(ptr, seed, C.byref(h)), "pvx_surrogate_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().pvx_surrogate_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self) -> np.ndarray:
+        out = np.zeros(self.N_PARAMS, np.float32)
+        _check(lib().pvx_surrogate_get_params(self.h, out.ctypes.data_as(C.POINTER(C.c_float))),
+               "pvx_surrogate_get_params")
+        return out
+
+    def grad(self, x: np.ndarray, t: np.ndarray):
+        """(loss, gradient) over the points (x: (n,6), t: (n,3))."""
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 6)
+        t = np.ascontiguousarray(t, np.float32).reshape(-1, 3)
+        dx, dt = hip.DeviceBuffer.from_array(x), hip.DeviceBuffer.from_array(t)
+        loss = C.c_float()
+        g = np.zeros(self.N_PARAMS, np.float32)
+        _check(lib().pvx_surrogate_grad(self.h, dx.value, dt.value, x.shape[0], C.byref(loss),
+                                        g.ctypes.data_as(C.POINTER(C.c_float)), None), "pvx_surrogate_grad")
+        return float(loss.value), g
+
+    def train(self, x, t, steps: int, batch: int = 4096, lr: float = 5e-3) -> float:
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 6)
+        t = np.ascontiguousarray(t, np.float32).reshape(-1, 3)
+        dx, dt = hip.DeviceBuffer.from_array(x), hip.DeviceBuffer.from_array(t)
+        loss = C.c_float()
+        _check(lib().pvx_surrogate_train(self.h, dx.value, dt.value, x.shape[0], steps, batch, lr,
+                                         C.byref(loss), None), "pvx_surrogate_train")
+        hip.synchronize()
+        return float(loss.value)
+
+    def infer(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 6)
+        n = x.shape[0]
+        dx = hip.DeviceBuffer.from_array(x)
+        dy = hip.DeviceBuffer(max(1, n) * 12)
+        _check(lib().pvx_surrogate_infer(self.h, dx.value, n, dy.value, None), "pvx_surrogate_infer")
+        hip.synchronize()
+        return dy.download(np.float32, n * 3).reshape(n, 3)
 
 
 def camera_rays(scene, width=None, height=None, jitter=(0.5, 0.5), out=None, stream=None):
