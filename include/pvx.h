@@ -352,7 +352,9 @@ typedef struct pvx_radiosity_settings {
     float pretrace_end;        /* default 0.04                                  */
     float brightness;          /* default 1.0                                   */
     float adc_bailout;         /* default 0.01                                  */
-    float probe_threshold;     /* learned: max mean probe error per tile (0.02) */
+    float probe_threshold;     /* learned: max mean probe error per tile in     
+                                * displayed colour, diffuse reflectance x
+                                * irradiance error (0.02)                      */
     uint32_t train_samples;    /* learned: plain gathers taken as training data */
     uint32_t train_steps;      /* learned: optimiser steps                      */
     uint32_t probe_tile;       /* learned: probe tile edge in pixels (32)       */

@@ -22,6 +22,17 @@ int fail(int code, const std::string &msg);
             return ::pvx::fail(PVX_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+// Device buffer grown on demand (contents are not preserved).
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf();
+    int reserve(size_t bytes);
+};
+
 // Shading state attached by pvx_scene_set_shading.
 struct ShadingState {
     bool set = false;

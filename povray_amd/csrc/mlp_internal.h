@@ -9,7 +9,7 @@
 
 #include <cstdint>
 
-#include "radiosity_internal.h"
+#include "internal.h"
 
 namespace pvx {
 

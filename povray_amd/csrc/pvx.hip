@@ -129,7 +129,8 @@ __device__ inline bool simple_closest(const SceneView &sv, uint32_t kind, uint32
     return found;
 }
 
-template <bool kComplex, bool kFlags, bool kCount>
+// kLevel 0: triangles and spheres; 1: also boxes; 2: also tori and CSG.
+template <int kLevel, bool kFlags, bool kCount>
 __global__ __launch_bounds__(kBlock) void k_trace_closest(SceneView sv, const pvx_ray *__restrict__ rays,
                                                           const double *__restrict__ max_depth,
                                                           uint32_t kind, uint32_t n,
@@ -163,7 +164,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_closest(SceneView sv, const pv
             bool ok;
             if (inf.kind == kPrimPlane) {
                 ok = plane_hit(sv.planes[inf.index], o, d, dep) && depth_ok(dep, shadow);
-            } else if (kComplex) {
+            } else if (kLevel >= 2) {
                 ok = csg_closest(sv, inf.index, o, d, shadow, bt, dep, prim);
             } else {
                 ok = false;
@@ -258,7 +259,17 @@ __global__ __launch_bounds__(kBlock) void k_trace_closest(SceneView sv, const pv
                                 bo = obj;
                                 bp = 0;
                             }
-                    } else if (kComplex) {
+                    } else if (kLevel == 1) { // boxes only
+                        double hs[2];
+                        int side[2];
+                        int nh = box_hits(sv.boxes[p.ref.index], sv.xforms, o, d, hs, side);
+                        for (int j = 0; j < nh; j++)
+                            if (depth_ok(hs[j], shadow) && better(hs[j], obj, 0, bt, bo, bp)) {
+                                bt = hs[j];
+                                bo = obj;
+                                bp = 0;
+                            }
+                    } else if (kLevel >= 2) {
                         double dep;
                         int32_t prim = 0;
                         bool ok;
@@ -524,10 +535,12 @@ extern "C" int pvx_scene_create(const pvx_scene_desc *desc, const pvx_build_opts
     int cus = 0;
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device));
     int per_cu = 0;
-    auto kfn = h.has_complex ? (h.any_flags ? (const void *)k_trace_closest<true, true, false>
-                                            : (const void *)k_trace_closest<true, false, false>)
-                             : (h.any_flags ? (const void *)k_trace_closest<false, true, false>
-                                            : (const void *)k_trace_closest<false, false, false>);
+    auto pick = [](uint32_t level, bool flags) -> const void * {
+        if (level >= 2) return flags ? (const void *)k_trace_closest<2, true, false> : (const void *)k_trace_closest<2, false, false>;
+        if (level == 1) return flags ? (const void *)k_trace_closest<1, true, false> : (const void *)k_trace_closest<1, false, false>;
+        return flags ? (const void *)k_trace_closest<0, true, false> : (const void *)k_trace_closest<0, false, false>;
+    };
+    auto kfn = pick(h.has_complex, h.any_flags != 0);
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, kBlock, 0));
     if (per_cu < 1) per_cu = 1;
     s->grid = (uint32_t)(cus * per_cu);
@@ -574,17 +587,17 @@ static int launch_closest(pvx_scene *s, const pvx_ray *rays, const double *max_d
     uint32_t nblocks = (uint32_t)std::min<size_t>(s->grid, (n + kBlock - 1) / kBlock);
     bool flags = s->view.any_flags != 0 && ray_kind != PVX_RAY_PRIMARY;
     uint32_t nn = (uint32_t)n;
-    if (s->view.has_complex) {
-        if (flags)
-            k_trace_closest<true, true, kCount><<<nblocks, kBlock, 0, st>>>(s->view, rays, max_depth, ray_kind, nn, hits, s->d_spill, nv, pt);
-        else
-            k_trace_closest<true, false, kCount><<<nblocks, kBlock, 0, st>>>(s->view, rays, max_depth, ray_kind, nn, hits, s->d_spill, nv, pt);
+#define PVX_LAUNCH(L, F) \
+    k_trace_closest<L, F, kCount><<<nblocks, kBlock, 0, st>>>(s->view, rays, max_depth, ray_kind, nn, hits, s->d_spill, nv, pt)
+    const uint32_t level = s->view.has_complex;
+    if (level >= 2) {
+        if (flags) PVX_LAUNCH(2, true); else PVX_LAUNCH(2, false);
+    } else if (level == 1) {
+        if (flags) PVX_LAUNCH(1, true); else PVX_LAUNCH(1, false);
     } else {
-        if (flags)
-            k_trace_closest<false, true, kCount><<<nblocks, kBlock, 0, st>>>(s->view, rays, max_depth, ray_kind, nn, hits, s->d_spill, nv, pt);
-        else
-            k_trace_closest<false, false, kCount><<<nblocks, kBlock, 0, st>>>(s->view, rays, max_depth, ray_kind, nn, hits, s->d_spill, nv, pt);
+        if (flags) PVX_LAUNCH(0, true); else PVX_LAUNCH(0, false);
     }
+#undef PVX_LAUNCH
     HIP_TRY(hipGetLastError());
     return PVX_OK;
 }

@@ -171,7 +171,7 @@ struct SceneView {
     int32_t root;      // root link (node index or leaf link); INT32_MIN = empty
     uint32_t n_inf;
     uint32_t any_flags;
-    uint32_t has_complex; // boxes/tori/CSG present in the BVH
+    uint32_t has_complex; // 0: triangles/spheres only, 1: + boxes, 2: + tori or CSG
 };
 
 } // namespace pvx

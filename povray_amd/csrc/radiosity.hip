@@ -241,7 +241,7 @@ __global__ __launch_bounds__(256) void k_gather_reduce(
     const uint32_t s = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t lane = threadIdx.x & 63;
     if (s >= m) return; // whole wavefronts leave together
-    double sum[3] = {0, 0, 0}, inv = 0.0, smallest = kBoundHuge;
+    double sum[3] = {0, 0, 0}, inv = 0.0, smallest = kBoundHuge, sq = 0.0;
     uint32_t sj = 0xFFFFFFFFu;
     for (uint32_t j = lane; j < N; j += 64) {
         size_t k = (size_t)s * N + j;
@@ -249,8 +249,13 @@ __global__ __launch_bounds__(256) void k_gather_reduce(
         double depth = HUGE_VAL;
         if (h.obj >= 0) {
             depth = h.t;
-            for (int a = 0; a < 3; a++)
-                sum[a] += (double)(base[3 * k + a] + coef[3 * k + a] * amb[3 * k + a]);
+            double c[3];
+            for (int a = 0; a < 3; a++) {
+                c[a] = (double)(base[3 * k + a] + coef[3 * k + a] * amb[3 * k + a]);
+                sum[a] += c[a];
+            }
+            double gr = 0.297 * c[0] + 0.589 * c[1] + 0.114 * c[2];
+            sq += gr * gr;
         }
         if (depth < smallest) { // first (lowest j) of the smallest depths
             smallest = depth;
@@ -261,6 +266,7 @@ __global__ __launch_bounds__(256) void k_gather_reduce(
     for (int off = 32; off > 0; off >>= 1) {
         for (int a = 0; a < 3; a++) sum[a] += __shfl_xor(sum[a], off, 64);
         inv += __shfl_xor(inv, off, 64);
+        sq += __shfl_xor(sq, off, 64);
         double os = __shfl_xor(smallest, off, 64);
         uint32_t oj = __shfl_xor(sj, off, 64);
         if (os < smallest || (os == smallest && oj < sj)) {
@@ -297,7 +303,12 @@ __global__ __launch_bounds__(256) void k_gather_reduce(
     }
     r.r = (float)mean;
     r.nearest = (float)smallest;
-    r.pad[0] = r.pad[1] = 0.0f;
+    // standard error of the grey mean over the N rays (the probe test's
+    // allowance for the gather's own noise)
+    double gm = (0.297 * sum[0] + 0.589 * sum[1] + 0.114 * sum[2]) / N;
+    double var = N > 1 ? fmax(0.0, (sq - N * gm * gm) / (N - 1)) : 0.0;
+    r.pad[0] = (float)sqrt(var / N);
+    r.pad[1] = 0.0f;
     rec[s] = r;
     rec_ok[s] = ok ? 1 : 0;
 }
@@ -381,6 +392,42 @@ __global__ void k_scatter3(const float *__restrict__ src, const uint32_t *__rest
     if (s >= m) return;
     uint32_t i = idx[s];
     for (int a = 0; a < 3; a++) dst[3 * (size_t)i + a] = src[3 * (size_t)s + a];
+}
+
+// Surrogate inputs: position scaled into the unit cube, eye-facing normal.
+__global__ void k_rec_to_sample(const RadRec *__restrict__ recs, uint32_t n, float lx, float ly, float lz,
+                                float inv, float *__restrict__ x6, float *__restrict__ t) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const RadRec b = recs[i];
+    x6[6 * (size_t)i + 0] = fminf(fmaxf((b.p[0] - lx) * inv, 0.0f), 1.0f);
+    x6[6 * (size_t)i + 1] = fminf(fmaxf((b.p[1] - ly) * inv, 0.0f), 1.0f);
+    x6[6 * (size_t)i + 2] = fminf(fmaxf((b.p[2] - lz) * inv, 0.0f), 1.0f);
+    for (int a = 0; a < 3; a++) {
+        x6[6 * (size_t)i + 3 + a] = b.n[a];
+        t[3 * (size_t)i + a] = b.E[a];
+    }
+}
+
+__global__ void k_query_to_x6(const RadQuery *__restrict__ q, const uint32_t *__restrict__ idx, uint32_t m,
+                              float lx, float ly, float lz, float inv, float *__restrict__ x6) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= m) return;
+    const RadQuery a = q[idx[s]];
+    x6[6 * (size_t)s + 0] = fminf(fmaxf(((float)a.p[0] - lx) * inv, 0.0f), 1.0f);
+    x6[6 * (size_t)s + 1] = fminf(fmaxf(((float)a.p[1] - ly) * inv, 0.0f), 1.0f);
+    x6[6 * (size_t)s + 2] = fminf(fmaxf(((float)a.p[2] - lz) * inv, 0.0f), 1.0f);
+    for (int c = 0; c < 3; c++) x6[6 * (size_t)s + 3 + c] = a.n[c];
+}
+
+// Surrogate answer for the queries idx[*]: irradiance max(y, 0); marks them done.
+__global__ void k_apply_pred(const uint32_t *__restrict__ idx, uint32_t m, const float *__restrict__ y,
+                             RadAcc *__restrict__ acc, float *__restrict__ E) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= m) return;
+    uint32_t i = idx[s];
+    for (int c = 0; c < 3; c++) E[3 * (size_t)i + c] = fmaxf(y[3 * (size_t)s + c], 0.0f);
+    acc[i].w = -1.0;
 }
 
 inline uint32_t blocks(size_t n, uint32_t b = 256) { return (uint32_t)((n + b - 1) / b); }
@@ -654,10 +701,117 @@ int RadiosityEngine::gather(int depth, const RadQuery *q, const uint32_t *idx, u
     return c.upload(stream);
 }
 
-// ComputeAmbient for n queries at `depth`: reuse the cache where it has
-// enough weight, gather the rest in rounds (spread-out subsets first).
-int RadiosityEngine::resolve(int depth, const RadQuery *q, uint32_t n, bool final, float *E) {
-    if (n == 0) return PVX_OK;
+// Spread-out subset of the pending queries: the first pending point of
+// every grid cell whose edge is the typical reach of a record (median
+// harmonic-mean radius of the records so far times the error bound), so one
+// round's gathers do not crowd each other and the rest of each cell can
+// reuse them, as with the sequential order.
+void RadiosityEngine::spread_select(int depth, const std::vector<uint32_t> &todo, const std::vector<uint32_t> &slot,
+                                    const std::vector<RadQuery> &hq, std::vector<uint32_t> &sel,
+                                    std::vector<uint32_t> &sel_slot, std::vector<uint32_t> &rest,
+                                    std::vector<uint32_t> &rest_slot) const {
+    const RadCache &c = caches[depth];
+    const double factor = std::pow(2.0, (double)depth);
+    std::vector<double> rad;
+    if (!c.recs.empty()) {
+        size_t stride = std::max<size_t>(1, c.recs.size() / 4096);
+        for (size_t k = 0; k < c.recs.size(); k += stride) rad.push_back(c.recs[k].r);
+    } else { // nothing gathered yet at this depth: half the largest reach
+        size_t stride = std::max<size_t>(1, todo.size() / 4096);
+        for (size_t k = 0; k < todo.size(); k += stride) {
+            const RadQuery &a = hq[slot[k]];
+            double ex = cam.location[0] - a.p[0], ey = cam.location[1] - a.p[1], ez = cam.location[2] - a.p[2];
+            rad.push_back(0.5 * std::sqrt(ex * ex + ey * ey + ez * ez) * rs.maximum_reuse * factor);
+        }
+    }
+    std::nth_element(rad.begin(), rad.begin() + rad.size() / 2, rad.end());
+    const double D = std::max(rad[rad.size() / 2] * rs.error_bound * factor, 1e-9);
+    std::unordered_set<uint64_t> taken;
+    taken.reserve(todo.size());
+    for (size_t k = 0; k < todo.size(); k++) {
+        const RadQuery &a = hq[slot[k]];
+        uint64_t key = 0;
+        for (int d = 0; d < 3; d++) key = key * 2097143u + (uint64_t)(int64_t)std::floor(a.p[d] / D);
+        if (taken.insert(key).second) {
+            sel.push_back(todo[k]);
+            sel_slot.push_back(slot[k]);
+        } else {
+            rest.push_back(todo[k]);
+            rest_slot.push_back(slot[k]);
+        }
+    }
+}
+
+// One round: gather at the `sel` queries, combine with their partial reuse
+// (ComputeAmbient, radiosity.cpp:430-440) and file the new records.
+int RadiosityEngine::gather_round(int depth, const RadQuery *q, const std::vector<uint32_t> &sel, bool final,
+                                  RadAcc *acc, float *E) {
+    if (sel.empty()) return PVX_OK;
+    Resolve &R = rstate[depth];
+    uint32_t *d_idx = static_cast<uint32_t *>(R.idx.p);
+    stats_rounds[depth]++;
+    HIP_TRY(hipMemcpyAsync(d_idx, sel.data(), sel.size() * 4, hipMemcpyHostToDevice, stream));
+    int r;
+    if ((r = gather(depth, q, d_idx, (uint32_t)sel.size(), final, static_cast<float *>(R.Eg.p)))) return r;
+    k_combine<<<blocks(sel.size()), 256, 0, stream>>>(d_idx, (uint32_t)sel.size(),
+                                                      static_cast<const float *>(R.Eg.p), acc, E);
+    HIP_TRY(hipGetLastError());
+    return PVX_OK;
+}
+
+// The `rest` queries meet the records filed since `first_new`; those that
+// still need a gather go back to todo (with their hq slots).
+int RadiosityEngine::relookup(int depth, const RadQuery *q, size_t first_new, bool final, RadAcc *acc,
+                              const std::vector<uint32_t> &rest, const std::vector<uint32_t> &rest_slot,
+                              std::vector<uint32_t> &todo, std::vector<uint32_t> &slot) {
+    todo.clear();
+    slot.clear();
+    if (rest.empty()) return PVX_OK;
+    Resolve &R = rstate[depth];
+    RadCache &c = caches[depth];
+    uint32_t *d_idx = static_cast<uint32_t *>(R.idx.p);
+    uint8_t *d_need = static_cast<uint8_t *>(R.need.p);
+    int r;
+    if ((r = c.fresh.build(c.recs, first_new, c.recs.size(), c.reach, stream))) return r;
+    HIP_TRY(hipMemcpyAsync(d_idx, rest.data(), rest.size() * 4, hipMemcpyHostToDevice, stream));
+    if ((r = lookup(depth, c.fresh, q, d_idx, (uint32_t)rest.size(), final, true, acc))) return r;
+    k_need<<<blocks(rest.size()), 256, 0, stream>>>(acc, d_idx, (uint32_t)rest.size(), reuse_count(depth),
+                                                    final ? 1 : 0, d_need);
+    std::vector<uint8_t> need(rest.size());
+    HIP_TRY(hipMemcpyAsync(need.data(), d_need, rest.size(), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    for (size_t k = 0; k < rest.size(); k++)
+        if (need[k]) {
+            todo.push_back(rest[k]);
+            slot.push_back(rest_slot[k]);
+        }
+    return PVX_OK;
+}
+
+// Gathers rounds until every pending query has its irradiance.
+int RadiosityEngine::gather_rounds(int depth, const RadQuery *q, bool final, std::vector<uint32_t> todo,
+                                   std::vector<uint32_t> slot, const std::vector<RadQuery> &hq, RadAcc *acc,
+                                   float *E) {
+    RadCache &c = caches[depth];
+    int r;
+    for (int round = 0; !todo.empty(); round++) {
+        std::vector<uint32_t> sel, sel_slot, rest, rest_slot;
+        if (round < kMaxResolveRounds - 1 && todo.size() > 64)
+            spread_select(depth, todo, slot, hq, sel, sel_slot, rest, rest_slot);
+        else
+            sel.swap(todo);
+        const size_t first_new = c.recs.size();
+        if ((r = gather_round(depth, q, sel, final, acc, E))) return r;
+        if ((r = relookup(depth, q, first_new, final, acc, rest, rest_slot, todo, slot))) return r;
+    }
+    return PVX_OK;
+}
+
+// Initial lookup of n queries against the whole depth cache; returns the
+// queries that need a gather and their positions (hq, indexed by slot).
+int RadiosityEngine::start_resolve(int depth, const RadQuery *q, uint32_t n, bool final, RadAcc *&acc,
+                                   std::vector<uint32_t> &todo, std::vector<uint32_t> &slot,
+                                   std::vector<RadQuery> &hq) {
     Resolve &R = rstate[depth];
     RadCache &c = caches[depth];
     int r;
@@ -665,7 +819,7 @@ int RadiosityEngine::resolve(int depth, const RadQuery *q, uint32_t n, bool fina
         (r = R.idx.reserve((size_t)n * 4)) || (r = R.Eg.reserve((size_t)n * 12)) ||
         (r = R.tq.reserve((size_t)n * sizeof(RadQuery))))
         return r;
-    RadAcc *acc = static_cast<RadAcc *>(R.acc.p);
+    acc = static_cast<RadAcc *>(R.acc.p);
     uint32_t *d_idx = static_cast<uint32_t *>(R.idx.p);
     uint8_t *d_need = static_cast<uint8_t *>(R.need.p);
     if ((r = c.index_all(stream)) || (r = lookup(depth, c.all, q, nullptr, n, final, false, acc))) return r;
@@ -673,12 +827,13 @@ int RadiosityEngine::resolve(int depth, const RadQuery *q, uint32_t n, bool fina
     std::vector<uint8_t> need(n);
     HIP_TRY(hipMemcpyAsync(need.data(), d_need, n, hipMemcpyDeviceToHost, stream));
     HIP_TRY(hipStreamSynchronize(stream));
-    std::vector<uint32_t> todo;
+    todo.clear();
     for (uint32_t i = 0; i < n; i++)
         if (need[i]) todo.push_back(i);
     if (depth == 0 && final) stats.final_misses += todo.size();
-    // positions of the pending queries, for the subset choice
-    std::vector<RadQuery> hq;
+    slot.resize(todo.size());
+    for (size_t k = 0; k < todo.size(); k++) slot[k] = (uint32_t)k;
+    hq.clear();
     if (!todo.empty()) {
         HIP_TRY(hipMemcpyAsync(d_idx, todo.data(), todo.size() * 4, hipMemcpyHostToDevice, stream));
         k_gather_subq<<<blocks(todo.size()), 256, 0, stream>>>(q, d_idx, (uint32_t)todo.size(),
@@ -687,71 +842,226 @@ int RadiosityEngine::resolve(int depth, const RadQuery *q, uint32_t n, bool fina
         HIP_TRY(hipMemcpyAsync(hq.data(), R.tq.p, todo.size() * sizeof(RadQuery), hipMemcpyDeviceToHost, stream));
         HIP_TRY(hipStreamSynchronize(stream));
     }
-    std::vector<uint32_t> slot(todo.size()); // todo position -> hq entry
-    for (size_t k = 0; k < todo.size(); k++) slot[k] = (uint32_t)k;
-    const double factor = std::pow(2.0, (double)depth);
-    for (int round = 0; !todo.empty(); round++) {
-        // Spread-out subset: the first pending point of every grid cell whose
-        // edge is the typical reach of a record (median harmonic-mean radius
-        // of the records so far times the error bound), so one round's
-        // gathers do not crowd each other and the rest of each cell can reuse
-        // them, as with the sequential order.
-        std::vector<uint32_t> sel, rest, rest_slot;
-        if (round < kMaxResolveRounds - 1 && todo.size() > 64) {
-            std::vector<double> rad;
-            if (!c.recs.empty()) {
-                size_t stride = std::max<size_t>(1, c.recs.size() / 4096);
-                for (size_t k = 0; k < c.recs.size(); k += stride) rad.push_back(c.recs[k].r);
-            } else { // nothing gathered yet at this depth: half the largest reach
-                size_t stride = std::max<size_t>(1, todo.size() / 4096);
-                for (size_t k = 0; k < todo.size(); k += stride) {
-                    const RadQuery &a = hq[slot[k]];
-                    double ex = cam.location[0] - a.p[0], ey = cam.location[1] - a.p[1],
-                           ez = cam.location[2] - a.p[2];
-                    rad.push_back(0.5 * std::sqrt(ex * ex + ey * ey + ez * ez) * rs.maximum_reuse * factor);
+    return PVX_OK;
+}
+
+// ComputeAmbient for n queries at `depth`: reuse the cache where it has
+// enough weight, gather the rest in rounds (spread-out subsets first).
+int RadiosityEngine::resolve(int depth, const RadQuery *q, uint32_t n, bool final, float *E) {
+    if (n == 0) return PVX_OK;
+    RadAcc *acc = nullptr;
+    std::vector<uint32_t> todo, slot;
+    std::vector<RadQuery> hq;
+    int r;
+    if ((r = start_resolve(depth, q, n, final, acc, todo, slot, hq)) ||
+        (r = gather_rounds(depth, q, final, std::move(todo), std::move(slot), hq, acc, E)))
+        return r;
+    k_finish<<<blocks(n), 256, 0, stream>>>(acc, n, rs.brightness, E);
+    HIP_TRY(hipGetLastError());
+    return PVX_OK;
+}
+
+// Final-pass irradiance with the surrogate (depth 0): the points no record
+// reaches are split into (1) a spread-out set of plain gathers, which become
+// training samples together with the pretrace records, (2) probe gathers,
+// a few per image tile, that check the trained network, and (3) the rest,
+// answered by the network except in tiles whose probes disagree by more than
+// the threshold, which fall back to plain gathers.
+int RadiosityEngine::resolve_learned(const RadQuery *q, uint32_t n, const std::vector<uint32_t> &pix, float *E) {
+    if (n == 0) return PVX_OK;
+    RadCache &c = caches[0];
+    RadAcc *acc = nullptr;
+    std::vector<uint32_t> todo, slot;
+    std::vector<RadQuery> hq;
+    int r;
+    if ((r = start_resolve(0, q, n, true, acc, todo, slot, hq))) return r;
+    const std::vector<uint32_t> todo_all = todo; // hq[k] belongs to todo_all[k]
+    if (!todo.empty()) {
+        // (1) training gathers
+        std::vector<uint32_t> sel, sel_slot, rest, rest_slot;
+        spread_select(0, todo, slot, hq, sel, sel_slot, rest, rest_slot);
+        const size_t T = std::max<uint32_t>(1, rs.train_samples);
+        if (sel.size() > T) { // keep an even subset
+            std::vector<uint32_t> keep, keep_slot;
+            const double step = (double)sel.size() / (double)T;
+            size_t next = 0;
+            for (size_t k = 0; k < sel.size(); k++) {
+                if (k == (size_t)(next * step) && keep.size() < T) {
+                    keep.push_back(sel[k]);
+                    keep_slot.push_back(sel_slot[k]);
+                    next++;
+                } else {
+                    rest.push_back(sel[k]);
+                    rest_slot.push_back(sel_slot[k]);
                 }
             }
-            std::nth_element(rad.begin(), rad.begin() + rad.size() / 2, rad.end());
-            double D = std::max(rad[rad.size() / 2] * rs.error_bound * factor, 1e-9);
-            std::unordered_set<uint64_t> taken;
-            taken.reserve(todo.size());
+            sel.swap(keep);
+        }
+        size_t first_new = c.recs.size();
+        if ((r = gather_round(0, q, sel, true, acc, E)) ||
+            (r = relookup(0, q, first_new, true, acc, rest, rest_slot, todo, slot)))
+            return r;
+    }
+    if (!todo.empty()) {
+        // (2) train on every depth-0 record
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (const RadRec &b : c.recs)
+            for (int a = 0; a < 3; a++) {
+                lo[a] = std::min(lo[a], b.p[a]);
+                hi[a] = std::max(hi[a], b.p[a]);
+            }
+        for (const RadQuery &a : hq)
+            for (int d = 0; d < 3; d++) {
+                lo[d] = std::min(lo[d], (float)a.p[d]);
+                hi[d] = std::max(hi[d], (float)a.p[d]);
+            }
+        float ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+        ext = std::max(ext * 1.02f, 1e-6f);
+        for (int a = 0; a < 3; a++) lo[a] -= 0.01f * ext;
+        const float inv = 1.0f / ext;
+        const uint32_t R0 = (uint32_t)c.recs.size();
+        if ((r = c.upload(stream)) || (r = lx.reserve((size_t)std::max(R0, n) * 24)) ||
+            (r = lt.reserve((size_t)R0 * 12)) || (r = ly.reserve((size_t)n * 12)) ||
+            (r = lidx.reserve((size_t)n * 4)))
+            return r;
+        k_rec_to_sample<<<blocks(R0), 256, 0, stream>>>(static_cast<const RadRec *>(c.d_recs.p), R0, lo[0], lo[1],
+                                                        lo[2], inv, static_cast<float *>(lx.p),
+                                                        static_cast<float *>(lt.p));
+        HIP_TRY(hipGetLastError());
+        hipEvent_t t0, t1;
+        HIP_TRY(hipEventCreate(&t0));
+        HIP_TRY(hipEventCreate(&t1));
+        HIP_TRY(hipEventRecord(t0, stream));
+        float loss = 0.0f;
+        const uint32_t batch = std::min<uint32_t>(2048, (R0 + kMlpChunk - 1) / kMlpChunk * kMlpChunk);
+        if ((r = net.init(nullptr, 0xC0FFEEull, stream)) ||
+            (r = net.train(static_cast<const float *>(lx.p), static_cast<const float *>(lt.p), R0, rs.train_steps,
+                           batch, 5e-3f, &loss, stream)))
+            return r;
+        HIP_TRY(hipEventRecord(t1, stream));
+        HIP_TRY(hipEventSynchronize(t1));
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, t0, t1));
+        (void)hipEventDestroy(t0);
+        (void)hipEventDestroy(t1);
+        train_ms += ms;
+        stats.train_loss = loss;
+
+        // (3) probes: up to probes_per_tile pending points per image tile
+        const uint32_t tsz = std::max<uint32_t>(1, rs.probe_tile);
+        const uint32_t tx = (cam.width + tsz - 1) / tsz;
+        auto tile_of = [&](uint32_t i) {
+            uint32_t p = pix[i];
+            return (p % cam.width) / tsz + ((p / cam.width) / tsz) * tx;
+        };
+        std::vector<std::vector<uint32_t>> by_tile; // positions in todo
+        std::vector<uint32_t> tiles_used;
+        {
+            std::vector<int32_t> tmap((size_t)tx * ((cam.height + tsz - 1) / tsz), -1);
             for (size_t k = 0; k < todo.size(); k++) {
-                const RadQuery &a = hq[slot[k]];
-                uint64_t key = 0;
-                for (int d = 0; d < 3; d++) key = key * 2097143u + (uint64_t)(int64_t)std::floor(a.p[d] / D);
-                if (taken.insert(key).second) {
-                    sel.push_back(todo[k]);
-                } else {
+                uint32_t t = tile_of(todo[k]);
+                if (tmap[t] < 0) {
+                    tmap[t] = (int32_t)by_tile.size();
+                    by_tile.emplace_back();
+                    tiles_used.push_back(t);
+                }
+                by_tile[tmap[t]].push_back((uint32_t)k);
+            }
+        }
+        std::vector<uint8_t> is_probe(todo.size(), 0);
+        std::vector<uint32_t> probes, probe_group;
+        const uint32_t P = rs.probes_per_tile;
+        for (size_t gi = 0; gi < by_tile.size(); gi++) {
+            const auto &v = by_tile[gi];
+            uint32_t cnt = std::min<uint32_t>(P, (uint32_t)v.size());
+            for (uint32_t j = 0; j < cnt; j++) {
+                uint32_t k = v[(size_t)((j + 0.5) * v.size() / cnt)];
+                if (is_probe[k]) continue;
+                is_probe[k] = 1;
+                probes.push_back(todo[k]);
+                probe_group.push_back((uint32_t)gi);
+            }
+        }
+        std::vector<uint8_t> fallback(by_tile.size(), 0);
+        if (!probes.empty()) {
+            size_t first_new = c.recs.size();
+            if ((r = gather_round(0, q, probes, true, acc, E))) return r;
+            stats.probes += probes.size();
+            const uint32_t m = (uint32_t)probes.size();
+            HIP_TRY(hipMemcpyAsync(lidx.p, probes.data(), m * 4, hipMemcpyHostToDevice, stream));
+            k_query_to_x6<<<blocks(m), 256, 0, stream>>>(q, static_cast<const uint32_t *>(lidx.p), m, lo[0], lo[1],
+                                                         lo[2], inv, static_cast<float *>(lx.p));
+            if ((r = net.infer(static_cast<const float *>(lx.p), m, static_cast<float *>(ly.p), stream))) return r;
+            std::vector<float> y(3 * (size_t)m), Eall(3 * (size_t)n);
+            std::vector<RadRec> prec(m); // the probes' gathers, in probe order
+            HIP_TRY(hipMemcpyAsync(prec.data(), scratch[0].rec.p, m * sizeof(RadRec), hipMemcpyDeviceToHost, stream));
+            HIP_TRY(hipMemcpyAsync(y.data(), ly.p, y.size() * 4, hipMemcpyDeviceToHost, stream));
+            HIP_TRY(hipMemcpyAsync(Eall.data(), E, Eall.size() * 4, hipMemcpyDeviceToHost, stream));
+            HIP_TRY(hipStreamSynchronize(stream));
+            // probe error in displayed colour: the query weight of a primary
+            // hit is its diffuse reflectance (grey(pigment) * opacity * diffuse),
+            // so w * |irradiance error| is what the surrogate would change there
+            std::vector<float> wq(n, 0.0f);
+            for (size_t k = 0; k < todo_all.size(); k++) wq[todo_all[k]] = hq[k].w;
+            std::vector<double> err(by_tile.size(), 0.0);
+            std::vector<uint32_t> cnt(by_tile.size(), 0);
+            double tot = 0;
+            for (uint32_t k = 0; k < m; k++) {
+                // grey irradiance difference beyond twice the gather's
+                // standard error, in displayed colour
+                double dg = 0;
+                const double gw[3] = {0.297, 0.589, 0.114};
+                for (int a = 0; a < 3; a++)
+                    dg += gw[a] * ((double)std::max(y[3 * (size_t)k + a], 0.0f) -
+                                   Eall[3 * (size_t)probes[k] + a]);
+                double e = std::max(0.0, std::fabs(dg) - 2.0 * prec[k].pad[0]) * wq[probes[k]] * rs.brightness;
+                err[probe_group[k]] += e;
+                cnt[probe_group[k]]++;
+                tot += e;
+            }
+            stats.probe_error = (float)(tot / m);
+            for (size_t gi = 0; gi < by_tile.size(); gi++)
+                fallback[gi] = cnt[gi] && err[gi] / cnt[gi] > rs.probe_threshold;
+            // the probes' records serve their neighbours too
+            std::vector<uint32_t> rest, rest_slot;
+            for (size_t k = 0; k < todo.size(); k++)
+                if (!is_probe[k]) {
                     rest.push_back(todo[k]);
                     rest_slot.push_back(slot[k]);
                 }
-            }
-        } else {
-            sel.swap(todo);
+            if ((r = relookup(0, q, first_new, true, acc, rest, rest_slot, todo, slot))) return r;
         }
-        stats_rounds[depth]++;
-        const size_t first_new = c.recs.size();
-        HIP_TRY(hipMemcpyAsync(d_idx, sel.data(), sel.size() * 4, hipMemcpyHostToDevice, stream));
-        if ((r = gather(depth, q, d_idx, (uint32_t)sel.size(), final, static_cast<float *>(R.Eg.p)))) return r;
-        k_combine<<<blocks(sel.size()), 256, 0, stream>>>(d_idx, (uint32_t)sel.size(),
-                                                          static_cast<const float *>(R.Eg.p), acc, E);
-        HIP_TRY(hipGetLastError());
-        if (rest.empty()) break;
-        // the rest meets only this round's records again
-        if ((r = c.fresh.build(c.recs, first_new, c.recs.size(), c.reach, stream))) return r;
-        HIP_TRY(hipMemcpyAsync(d_idx, rest.data(), rest.size() * 4, hipMemcpyHostToDevice, stream));
-        if ((r = lookup(depth, c.fresh, q, d_idx, (uint32_t)rest.size(), final, true, acc))) return r;
-        k_need<<<blocks(rest.size()), 256, 0, stream>>>(acc, d_idx, (uint32_t)rest.size(), reuse_count(depth),
-                                                        final ? 1 : 0, d_need);
-        HIP_TRY(hipMemcpyAsync(need.data(), d_need, rest.size(), hipMemcpyDeviceToHost, stream));
-        HIP_TRY(hipStreamSynchronize(stream));
-        todo.clear();
-        slot.clear();
-        for (size_t k = 0; k < rest.size(); k++)
-            if (need[k]) {
-                todo.push_back(rest[k]);
-                slot.push_back(rest_slot[k]);
+        // (4) fallback tiles: plain gathers; the rest: the network
+        std::vector<uint8_t> fb_tile((size_t)tx * ((cam.height + tsz - 1) / tsz), 0);
+        for (size_t gi = 0; gi < by_tile.size(); gi++) fb_tile[tiles_used[gi]] = fallback[gi];
+        std::vector<uint32_t> fb, fb_slot, ml, ml_slot;
+        for (size_t k = 0; k < todo.size(); k++) {
+            if (fb_tile[tile_of(todo[k])]) {
+                fb.push_back(todo[k]);
+                fb_slot.push_back(slot[k]);
+            } else {
+                ml.push_back(todo[k]);
+                ml_slot.push_back(slot[k]);
             }
+        }
+        stats.fallbacks += fb.size();
+        if (!fb.empty()) {
+            size_t first_new = c.recs.size();
+            if ((r = gather_rounds(0, q, true, fb, fb_slot, hq, acc, E))) return r;
+            if ((r = relookup(0, q, first_new, true, acc, ml, ml_slot, todo, slot))) return r;
+            ml.swap(todo);
+        }
+        stats.surrogate_queries += ml.size();
+        if (!ml.empty()) {
+            const uint32_t m = (uint32_t)ml.size();
+            HIP_TRY(hipMemcpyAsync(lidx.p, ml.data(), m * 4, hipMemcpyHostToDevice, stream));
+            k_query_to_x6<<<blocks(m), 256, 0, stream>>>(q, static_cast<const uint32_t *>(lidx.p), m, lo[0], lo[1],
+                                                         lo[2], inv, static_cast<float *>(lx.p));
+            if ((r = net.infer(static_cast<const float *>(lx.p), m, static_cast<float *>(ly.p), stream))) return r;
+            k_apply_pred<<<blocks(m), 256, 0, stream>>>(static_cast<const uint32_t *>(lidx.p), m,
+                                                        static_cast<const float *>(ly.p), acc, E);
+            HIP_TRY(hipGetLastError());
+        }
     }
     k_finish<<<blocks(n), 256, 0, stream>>>(acc, n, rs.brightness, E);
     HIP_TRY(hipGetLastError());
@@ -813,8 +1123,16 @@ int RadiosityEngine::pixel_pass(const std::vector<uint32_t> &px, bool final, flo
                                                      static_cast<const uint32_t *>(S.sel.p), m,
                                                      static_cast<RadQuery *>(S.subq.p));
         HIP_TRY(hipGetLastError());
-        if ((r = resolve(0, static_cast<const RadQuery *>(S.subq.p), m, final, static_cast<float *>(S.subE.p))))
+        if (final && rs.mode == PVX_RADIOSITY_LEARNED) {
+            std::vector<uint32_t> pix(m);
+            for (uint32_t k = 0; k < m; k++) pix[k] = px[sel[k]];
+            if ((r = resolve_learned(static_cast<const RadQuery *>(S.subq.p), m, pix,
+                                     static_cast<float *>(S.subE.p))))
+                return r;
+        } else if ((r = resolve(0, static_cast<const RadQuery *>(S.subq.p), m, final,
+                                static_cast<float *>(S.subE.p)))) {
             return r;
+        }
     }
     if (final) {
         HIP_TRY(hipMemcpyAsync(S.rec.p, qpix.data(), n * 4, hipMemcpyHostToDevice, stream));
@@ -847,7 +1165,9 @@ extern "C" int pvx_radiosity_render(pvx_scene *s, const pvx_camera *cam, const p
         return fail(PVX_EINVAL, "bad radiosity settings");
     if (set->mode != PVX_RADIOSITY_PLAIN && set->mode != PVX_RADIOSITY_LEARNED)
         return fail(PVX_EINVAL, "unknown radiosity mode");
-    if (set->mode == PVX_RADIOSITY_LEARNED) return fail(PVX_EUNSUPPORTED, "learned radiosity is not built yet");
+    if (set->mode == PVX_RADIOSITY_LEARNED &&
+        (set->probe_tile == 0 || set->train_steps == 0 || set->train_samples == 0 || !(set->probe_threshold >= 0.0f)))
+        return fail(PVX_EINVAL, "learned radiosity needs probe_tile, train_steps, train_samples > 0");
     for (const MatRec &m : s->shading.mats)
         if (m.reflective)
             return fail(PVX_EUNSUPPORTED, "radiosity frames take plain diffuse / emissive finishes only");
@@ -891,6 +1211,7 @@ extern "C" int pvx_radiosity_render(pvx_scene *s, const pvx_camera *cam, const p
         stats->frame_ms = (double)a + b;
         stats->records[0] = eng.caches[0].recs.size();
         stats->records[1] = eng.caches[1].recs.size();
+        stats->train_ms = eng.train_ms;
     }
     if (std::getenv("PVX_RADIOSITY_VERBOSE"))
         std::fprintf(stderr, "[pvx radiosity] rounds d0=%llu d1=%llu gathers d0=%llu d1=%llu records d0=%zu d1=%zu\n",

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "internal.h"
+#include "mlp_internal.h"
 
 namespace pvx {
 
@@ -43,17 +44,6 @@ static_assert(sizeof(RadQuery) == 40, "RadQuery layout");
 struct RadAcc {
     double e[3];
     double w;
-};
-
-// Device buffer grown on demand (contents are not preserved).
-struct DevBuf {
-    void *p = nullptr;
-    size_t cap = 0;
-    DevBuf() = default;
-    DevBuf(const DevBuf &) = delete;
-    DevBuf &operator=(const DevBuf &) = delete;
-    ~DevBuf();
-    int reserve(size_t bytes);
 };
 
 // Uniform grid over the quick-check spheres of a range of records
@@ -100,6 +90,9 @@ struct RadiosityEngine {
         DevBuf acc, need, idx, Eg, tq;
     } rstate[kMaxRadDepth];
     Scratch prim; // primary rays of a pixel pass
+    Surrogate net; // learned mode
+    DevBuf lx, lt, ly, lidx;
+    double train_ms = 0.0;
 
     RadiosityEngine(pvx_scene *s, const pvx_radiosity_settings &set, const pvx_camera &c, hipStream_t st);
     int init();
@@ -109,6 +102,19 @@ struct RadiosityEngine {
                bool accumulate, RadAcc *acc);
     int gather(int depth, const RadQuery *q, const uint32_t *idx, uint32_t m, bool final, float *Eg);
     int resolve(int depth, const RadQuery *q, uint32_t n, bool final, float *E);
+    int start_resolve(int depth, const RadQuery *q, uint32_t n, bool final, RadAcc *&acc,
+                      std::vector<uint32_t> &todo, std::vector<uint32_t> &slot, std::vector<RadQuery> &hq);
+    void spread_select(int depth, const std::vector<uint32_t> &todo, const std::vector<uint32_t> &slot,
+                       const std::vector<RadQuery> &hq, std::vector<uint32_t> &sel, std::vector<uint32_t> &sel_slot,
+                       std::vector<uint32_t> &rest, std::vector<uint32_t> &rest_slot) const;
+    int gather_round(int depth, const RadQuery *q, const std::vector<uint32_t> &sel, bool final, RadAcc *acc,
+                     float *E);
+    int relookup(int depth, const RadQuery *q, size_t first_new, bool final, RadAcc *acc,
+                 const std::vector<uint32_t> &rest, const std::vector<uint32_t> &rest_slot,
+                 std::vector<uint32_t> &todo, std::vector<uint32_t> &slot);
+    int gather_rounds(int depth, const RadQuery *q, bool final, std::vector<uint32_t> todo,
+                      std::vector<uint32_t> slot, const std::vector<RadQuery> &hq, RadAcc *acc, float *E);
+    int resolve_learned(const RadQuery *q, uint32_t n, const std::vector<uint32_t> &pix, float *E);
     int pixel_pass(const std::vector<uint32_t> &px, bool final, float *rgb);
 };
 

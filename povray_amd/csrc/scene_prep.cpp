@@ -397,7 +397,7 @@ int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene
             pr.ref.index = ob.index;
             to_fbox(shape_bnd(ob.type, ob.index, 0).S, lo, hi);
             add_prim(pr, lo, hi);
-            out.has_complex = 1;
+            out.has_complex = std::max<uint32_t>(out.has_complex, ob.type == PVX_BOX ? 1u : 2u);
             break;
         }
         case PVX_CSG: {
@@ -405,7 +405,7 @@ int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene
             if (!b.bounded || !(b.lo[0] <= b.hi[0] && b.lo[1] <= b.hi[1] && b.lo[2] <= b.hi[2])) {
                 if (!b.bounded) {
                     out.inf.push_back(InfRec{kPrimCsg, ob.index, (int32_t)o, 0});
-                    out.has_complex = 1;
+                    out.has_complex = 2;
                 }
                 // an empty surface bound: the object can never be hit
                 break;
@@ -414,7 +414,7 @@ int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene
             pr.ref.index = ob.index;
             to_fbox(b, lo, hi);
             add_prim(pr, lo, hi);
-            out.has_complex = 1;
+            out.has_complex = 2;
             break;
         }
         case PVX_PLANE:

@@ -30,7 +30,7 @@ struct HostScene {
     std::vector<uint32_t> obj_tri_base;
     std::vector<float> mesh_vtx, mesh_nrm;
     uint32_t any_flags = 0;
-    uint32_t has_complex = 0;
+    uint32_t has_complex = 0; // 0: triangles/spheres, 1: + boxes, 2: + tori or CSG
     uint64_t n_triangles = 0;
     double build_ms = 0.0;
 };
