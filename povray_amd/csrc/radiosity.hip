@@ -583,8 +583,16 @@ int RadiosityEngine::init() {
     int r;
     if ((r = d_pool.reserve(pool.size() * 8))) return r;
     HIP_TRY(hipMemcpyAsync(d_pool.p, pool.data(), pool.size() * 8, hipMemcpyHostToDevice, stream));
-    HIP_TRY(hipStreamSynchronize(stream));
+    HIP_TRY(sync());
     return PVX_OK;
+}
+
+hipError_t RadiosityEngine::sync() {
+    auto t0 = std::chrono::steady_clock::now();
+    hipError_t e = hipStreamSynchronize(stream);
+    t_sync_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    n_sync++;
+    return e;
 }
 
 int RadiosityEngine::rays_per_sample(int d) const {
@@ -658,7 +666,7 @@ int RadiosityEngine::gather(int depth, const RadQuery *q, const uint32_t *idx, u
         // compact the hits that need irradiance at depth + 1 (deterministic order)
         std::vector<uint8_t> need(nr);
         HIP_TRY(hipMemcpyAsync(need.data(), S.need.p, nr, hipMemcpyDeviceToHost, stream));
-        HIP_TRY(hipStreamSynchronize(stream));
+        HIP_TRY(sync());
         std::vector<uint32_t> sel;
         sel.reserve(nr);
         for (size_t k = 0; k < nr; k++)
@@ -694,7 +702,7 @@ int RadiosityEngine::gather(int depth, const RadQuery *q, const uint32_t *idx, u
     std::vector<uint8_t> ok(m);
     HIP_TRY(hipMemcpyAsync(rec.data(), S.rec.p, m * sizeof(RadRec), hipMemcpyDeviceToHost, stream));
     HIP_TRY(hipMemcpyAsync(ok.data(), S.rec_ok.p, m, hipMemcpyDeviceToHost, stream));
-    HIP_TRY(hipStreamSynchronize(stream));
+    HIP_TRY(sync());
     RadCache &c = caches[depth];
     for (uint32_t k = 0; k < m; k++)
         if (ok[k]) c.recs.push_back(rec[k]);
@@ -772,14 +780,16 @@ int RadiosityEngine::relookup(int depth, const RadQuery *q, size_t first_new, bo
     uint32_t *d_idx = static_cast<uint32_t *>(R.idx.p);
     uint8_t *d_need = static_cast<uint8_t *>(R.need.p);
     int r;
+    auto tg = std::chrono::steady_clock::now();
     if ((r = c.fresh.build(c.recs, first_new, c.recs.size(), c.reach, stream))) return r;
+    t_grid_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tg).count();
     HIP_TRY(hipMemcpyAsync(d_idx, rest.data(), rest.size() * 4, hipMemcpyHostToDevice, stream));
     if ((r = lookup(depth, c.fresh, q, d_idx, (uint32_t)rest.size(), final, true, acc))) return r;
     k_need<<<blocks(rest.size()), 256, 0, stream>>>(acc, d_idx, (uint32_t)rest.size(), reuse_count(depth),
                                                     final ? 1 : 0, d_need);
     std::vector<uint8_t> need(rest.size());
     HIP_TRY(hipMemcpyAsync(need.data(), d_need, rest.size(), hipMemcpyDeviceToHost, stream));
-    HIP_TRY(hipStreamSynchronize(stream));
+    HIP_TRY(sync());
     for (size_t k = 0; k < rest.size(); k++)
         if (need[k]) {
             todo.push_back(rest[k]);
@@ -822,11 +832,14 @@ int RadiosityEngine::start_resolve(int depth, const RadQuery *q, uint32_t n, boo
     acc = static_cast<RadAcc *>(R.acc.p);
     uint32_t *d_idx = static_cast<uint32_t *>(R.idx.p);
     uint8_t *d_need = static_cast<uint8_t *>(R.need.p);
-    if ((r = c.index_all(stream)) || (r = lookup(depth, c.all, q, nullptr, n, final, false, acc))) return r;
+    auto tg = std::chrono::steady_clock::now();
+    if ((r = c.index_all(stream))) return r;
+    t_grid_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tg).count();
+    if ((r = lookup(depth, c.all, q, nullptr, n, final, false, acc))) return r;
     k_need<<<blocks(n), 256, 0, stream>>>(acc, nullptr, n, reuse_count(depth), final ? 1 : 0, d_need);
     std::vector<uint8_t> need(n);
     HIP_TRY(hipMemcpyAsync(need.data(), d_need, n, hipMemcpyDeviceToHost, stream));
-    HIP_TRY(hipStreamSynchronize(stream));
+    HIP_TRY(sync());
     todo.clear();
     for (uint32_t i = 0; i < n; i++)
         if (need[i]) todo.push_back(i);
@@ -840,7 +853,7 @@ int RadiosityEngine::start_resolve(int depth, const RadQuery *q, uint32_t n, boo
                                                                static_cast<RadQuery *>(R.tq.p));
         hq.resize(todo.size());
         HIP_TRY(hipMemcpyAsync(hq.data(), R.tq.p, todo.size() * sizeof(RadQuery), hipMemcpyDeviceToHost, stream));
-        HIP_TRY(hipStreamSynchronize(stream));
+        HIP_TRY(sync());
     }
     return PVX_OK;
 }
@@ -933,7 +946,7 @@ int RadiosityEngine::resolve_learned(const RadQuery *q, uint32_t n, const std::v
         HIP_TRY(hipEventCreate(&t1));
         HIP_TRY(hipEventRecord(t0, stream));
         float loss = 0.0f;
-        const uint32_t batch = std::min<uint32_t>(2048, (R0 + kMlpChunk - 1) / kMlpChunk * kMlpChunk);
+        const uint32_t batch = 2048; // samples drawn with replacement from the R0 records
         if ((r = net.init(nullptr, 0xC0FFEEull, stream)) ||
             (r = net.train(static_cast<const float *>(lx.p), static_cast<const float *>(lt.p), R0, rs.train_steps,
                            batch, 5e-3f, &loss, stream)))
@@ -997,7 +1010,7 @@ int RadiosityEngine::resolve_learned(const RadQuery *q, uint32_t n, const std::v
             HIP_TRY(hipMemcpyAsync(prec.data(), scratch[0].rec.p, m * sizeof(RadRec), hipMemcpyDeviceToHost, stream));
             HIP_TRY(hipMemcpyAsync(y.data(), ly.p, y.size() * 4, hipMemcpyDeviceToHost, stream));
             HIP_TRY(hipMemcpyAsync(Eall.data(), E, Eall.size() * 4, hipMemcpyDeviceToHost, stream));
-            HIP_TRY(hipStreamSynchronize(stream));
+            HIP_TRY(sync());
             // probe error in displayed colour: the query weight of a primary
             // hit is its diffuse reflectance (grey(pigment) * opacity * diffuse),
             // so w * |irradiance error| is what the surrogate would change there
@@ -1109,7 +1122,7 @@ int RadiosityEngine::pixel_pass(const std::vector<uint32_t> &px, bool final, flo
     HIP_TRY(hipGetLastError());
     std::vector<uint8_t> need(n);
     HIP_TRY(hipMemcpyAsync(need.data(), S.need.p, n, hipMemcpyDeviceToHost, stream));
-    HIP_TRY(hipStreamSynchronize(stream));
+    HIP_TRY(sync());
     std::vector<uint32_t> sel, qpix(n, 0);
     for (uint32_t i = 0; i < n; i++)
         if (need[i]) {
@@ -1141,7 +1154,7 @@ int RadiosityEngine::pixel_pass(const std::vector<uint32_t> &px, bool final, flo
             static_cast<const float *>(S.subE.p), static_cast<const uint32_t *>(S.rec.p), n,
             static_cast<const uint8_t *>(S.need.p), rgb);
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipStreamSynchronize(stream));
+        HIP_TRY(sync());
     }
     return PVX_OK;
 }
@@ -1214,10 +1227,13 @@ extern "C" int pvx_radiosity_render(pvx_scene *s, const pvx_camera *cam, const p
         stats->train_ms = eng.train_ms;
     }
     if (std::getenv("PVX_RADIOSITY_VERBOSE"))
-        std::fprintf(stderr, "[pvx radiosity] rounds d0=%llu d1=%llu gathers d0=%llu d1=%llu records d0=%zu d1=%zu\n",
+        std::fprintf(stderr,
+                     "[pvx radiosity] rounds d0=%llu d1=%llu gathers d0=%llu d1=%llu records d0=%zu d1=%zu "
+                     "syncs=%llu wait=%.1fms grids=%.1fms\n",
                      (unsigned long long)eng.stats_rounds[0], (unsigned long long)eng.stats_rounds[1],
                      (unsigned long long)eng.stats.gathers[0], (unsigned long long)eng.stats.gathers[1],
-                     eng.caches[0].recs.size(), eng.caches[1].recs.size());
+                     eng.caches[0].recs.size(), eng.caches[1].recs.size(), (unsigned long long)eng.n_sync,
+                     eng.t_sync_ms, eng.t_grid_ms);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     (void)hipEventDestroy(e2);

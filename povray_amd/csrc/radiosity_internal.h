@@ -93,6 +93,9 @@ struct RadiosityEngine {
     Surrogate net; // learned mode
     DevBuf lx, lt, ly, lidx;
     double train_ms = 0.0;
+    double t_sync_ms = 0.0, t_grid_ms = 0.0; // host-side waits and grid builds
+    uint64_t n_sync = 0;
+    hipError_t sync();
 
     RadiosityEngine(pvx_scene *s, const pvx_radiosity_settings &set, const pvx_camera &c, hipStream_t st);
     int init();
