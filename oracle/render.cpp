@@ -1,12 +1,16 @@
 // FP64 CPU ORACLE — TEST INFRASTRUCTURE ONLY (see oracle.hpp).
-// Restatement of the plain-finish shading path for opaque surfaces:
+// Restatement of the plain-finish shading path:
 // Trace::TraceRay (core/render/trace.cpp:135-228), ComputeTextureColour
 // (457-586), ComputeLightedTexture (739-1179, single layer, no radiosity,
 // no media, no photons), ComputeOneDiffuseLight (1637-1729),
 // ComputeOneWhiteLightRay (2710-2767), TracePointLightShadowRay /
 // TraceAreaLightSubsetShadowRay (1951-2272, without the order-dependent
-// shadow cache), ComputeDiffuseColour (2441-2484), ComputePhongColour
-// (2520-2558), ComputeReflectivity (2627-2656), ComputeReflection (1264-1321).
+// shadow cache), ComputeShadowColour / ComputeShadowTexture (2274-2420,
+// 1181-1262: filter * pigment + transmit, no caustics or fade),
+// ComputeDiffuseColour (2441-2484), ComputePhongColour (2520-2558),
+// ComputeReflectivity (2627-2656), ComputeReflection (1264-1321),
+// ComputeRefraction / TraceRefractionRay (1323-1483: interior list,
+// Heckbert refraction, total internal reflection; no dispersion).
 // Colours are FP32 (MathColour), geometry FP64.
 #include <algorithm>
 #include <atomic>
@@ -92,27 +96,65 @@ V3 shape_normal(const Scene &s, uint32_t type, uint32_t index, const Isect &h, V
     return V3(0, 1, 0);
 }
 
-bool blocked(const Scene &s, const Shading &sh, Counters &cnt, V3 ip, V3 target) {
+// TransColour::TransmittedColour (base/colour.h:1912-1915): filter * pigment + transmit.
+inline RGB transmitted(const pvx_material &m) {
+    return {m.pigment[0] * m.filter + m.transmit, m.pigment[1] * m.filter + m.transmit,
+            m.pigment[2] * m.filter + m.transmit};
+}
+inline bool opaque(const pvx_material &m) { return m.filter == 0.0f && m.transmit == 0.0f; }
+inline double interior_ior(const pvx_material &m) { return m.ior > 0.0f ? (double)m.ior : 1.0; }
+
+// Light reaching ip from target: TracePointLightShadowRay's loop, which
+// filters the light through every transparent object in between and stops
+// at the first opaque one.
+RGB light_through(const Scene &s, const Shading &sh, Counters &cnt, V3 ip, V3 target, RGB lc) {
     Ray lr;
     lr.o = ip;
     V3 dir = target - ip;
     double lsd = length(dir);
     lr.d = dir / lsd;
-    Isect h;
-    int32_t obj;
-    cnt.shadow++;
-    bool found = scene_find_intersection(s, lr, lsd, PVX_RAY_SHADOW, h, obj, sh.method);
-    return found && h.depth < lsd - kShadowTolerance && h.depth > kShadowTolerance;
+    for (int guard = 0; guard < 64; guard++) {
+        Isect h;
+        int32_t obj;
+        cnt.shadow++;
+        bool found = scene_find_intersection(s, lr, lsd, PVX_RAY_SHADOW, h, obj, sh.method);
+        if (!(found && h.depth < lsd - kShadowTolerance && h.depth > kShadowTolerance)) return lc;
+        const pvx_material &m = sh.mats[obj];
+        if (opaque(m)) return RGB{};
+        lc = lc * transmitted(m);
+        lsd -= h.depth;
+        lr.o = lr.eval(h.depth);
+    }
+    return lc;
+}
+
+// Interiors a ray is inside (object ids), as Ray::GetInteriors.
+using Interiors = std::vector<int32_t>;
+
+// Ray type bits as Ray::SetFlags(type, parent) sets them (ray.cpp:155-166).
+struct RayFlags {
+    bool primary = false, refl = false, refr = false;
+};
+inline RayFlags reflected(RayFlags p) { return RayFlags{false, true, p.refr}; }
+inline RayFlags refracted(RayFlags p) { return RayFlags{false, p.refl, true}; }
+// The ray kind NoSomethingFlagRayObjectCondition tests (trace.cpp:84-95,
+// Ray::IsImageRay ray.h:100).
+inline uint32_t kind_of(RayFlags f) {
+    uint32_t k = 0;
+    if (f.primary || (f.refr && !f.refl)) k |= PVX_RAY_IMAGE;
+    if (f.refl) k |= PVX_RAY_REFLECTION;
+    return k;
 }
 
 RGB trace_ray(const Scene &s, const Shading &sh, Counters &cnt, const Ray &ray, float weight,
-              uint32_t level, uint32_t kind) {
+              uint32_t level, RayFlags rf, const Interiors &ins, bool continued) {
+    const uint32_t kind = kind_of(rf);
     if (level >= sh.max_level || weight < sh.bailout) return RGB{};
     Isect best;
     int32_t obj;
     cnt.closest++;
     if (!scene_find_intersection(s, ray, 0.0, kind, best, obj, sh.method)) return rgb3(sh.bg);
-    level++;
+    if (!continued) level++;
     const pvx_material &m = sh.mats[obj];
     V3 ip = ray.eval(best.depth);
     const Object &o = s.objects[obj];
@@ -136,18 +178,18 @@ RGB trace_ray(const Scene &s, const Shading &sh, Counters &cnt, const Ray &ray, 
             // quadtree average equals the mean over the grid for 2^k+1 and
             // 4x4 grids alike, computed here as nested corner averages.
             int n1 = (int)L.size1, n2 = (int)L.size2;
-            std::vector<float> vis((size_t)n1 * n2);
+            std::vector<RGB> vis((size_t)n1 * n2);
             for (int u = 0; u < n1; u++)
                 for (int v = 0; v < n2; v++) {
                     V3 j1 = n1 > 1 ? a1 * ((double)u / (double)(n1 - 1) - 0.5) : V3(0, 0, 0);
                     V3 j2 = n2 > 1 ? a2 * ((double)v / (double)(n2 - 1) - 0.5) : V3(0, 0, 0);
-                    vis[(size_t)u * n2 + v] = blocked(s, sh, cnt, ip, C + (j1 + j2)) ? 0.0f : 1.0f;
+                    vis[(size_t)u * n2 + v] = light_through(s, sh, cnt, ip, C + (j1 + j2), lc);
                 }
             // TraceAreaLightSubsetShadowRay recursion (trace.cpp:2131-2272)
             struct R {
-                static float sub(const std::vector<float> &g, int n2, int u1, int v1, int u2, int v2) {
-                    float c[4] = {g[(size_t)u1 * n2 + v1], g[(size_t)u2 * n2 + v1],
-                                  g[(size_t)u1 * n2 + v2], g[(size_t)u2 * n2 + v2]};
+                static RGB sub(const std::vector<RGB> &g, int n2, int u1, int v1, int u2, int v2) {
+                    RGB c[4] = {g[(size_t)u1 * n2 + v1], g[(size_t)u2 * n2 + v1], g[(size_t)u1 * n2 + v2],
+                                g[(size_t)u2 * n2 + v2]};
                     if ((u2 - u1 > 1) || (v2 - v1 > 1)) {
                         int mu0 = (int)std::floor((u1 + u2) / 2.0), mu1 = (int)std::ceil((u1 + u2) / 2.0);
                         int mv0 = (int)std::floor((v1 + v2) / 2.0), mv1 = (int)std::ceil((v1 + v2) / 2.0);
@@ -156,13 +198,13 @@ RGB trace_ray(const Scene &s, const Shading &sh, Counters &cnt, const Ray &ray, 
                         c[2] = sub(g, n2, u1, mv1, mu0, v2);
                         c[3] = sub(g, n2, mu1, mv1, u2, v2);
                     }
-                    return (c[0] + c[1] + c[2] + c[3]) * 0.25f;
+                    RGB a = c[0] + c[1] + c[2] + c[3];
+                    return RGB{a.r * 0.25f, a.g * 0.25f, a.b * 0.25f};
                 }
             };
-            float f = R::sub(vis, n2, 0, 0, n1 - 1, n2 - 1);
-            lc = RGB{lc.r * f, lc.g * f, lc.b * f};
-        } else if (blocked(s, sh, cnt, ip, C)) {
-            lc = RGB{};
+            lc = R::sub(vis, n2, 0, 0, n1 - 1, n2 - 1);
+        } else {
+            lc = light_through(s, sh, cnt, ip, C, lc);
         }
         if (near_zero(lc)) continue;
         double cosi = dot(n, dir);
@@ -178,7 +220,59 @@ RGB trace_ray(const Scene &s, const Shading &sh, Counters &cnt, const Ray &ray, 
                 col = col + scale(lc, (double)m.phong * std::pow(ca, (double)m.phong_size));
         }
     }
-    // reflection: ComputeReflectivity (no fresnel) + ComputeReflection
+    // transmitted component (ComputeLightedTexture, trace.cpp:1075-1131)
+    const RGB fil = transmitted(m);
+    const double trans = std::min(1.0, (double)std::fabs(0.297f * fil.r + 0.589f * fil.g + 0.114f * fil.b));
+    bool tir = false;
+    if (trans > sh.bailout) {
+        const float nw = (float)((double)weight *
+                                 std::max(std::fabs(fil.r), std::max(std::fabs(fil.g), std::fabs(fil.b))));
+        // ComputeRefraction: ior ratio from the interior list
+        Interiors ni = ins;
+        const double own = interior_ior(m);
+        double ior;
+        if (ni.empty()) {
+            ni.push_back(obj);
+            ior = 1.0 / own;
+        } else if (ni.back() == obj) {
+            ni.pop_back();
+            ior = ni.empty() ? own : own / interior_ior(sh.mats[ni.back()]);
+        } else {
+            auto it = std::find(ni.begin(), ni.end(), obj);
+            if (it != ni.end()) {
+                ni.erase(it);
+                ior = 1.0;
+            } else {
+                ior = interior_ior(sh.mats[ni.back()]) / own;
+                ni.push_back(obj);
+            }
+        }
+        RGB rfr;
+        if (std::fabs(ior - 1.0) < kEpsilon) {
+            rfr = trace_ray(s, sh, cnt, Ray{ip, ray.d}, nw, level, refracted(rf), ni, true);
+        } else {
+            double nn = dot(ray.d, n);
+            V3 ln = n;
+            if (nn <= 0.0)
+                nn = -nn;
+            else
+                ln = n * -1.0;
+            double t = 1.0 + ior * ior * (nn * nn - 1.0);
+            if (t < 0.0) { // total internal reflection: a reflected ray, unfiltered
+                tir = true;
+                double k = -2.0 * dot(ray.d, n);
+                Ray nr{ip, normalized(ray.d + n * k)};
+                rfr = trace_ray(s, sh, cnt, nr, nw, level, reflected(rf), ins, false);
+            } else {
+                t = ior * nn - std::sqrt(t);
+                Ray nr{ip, ray.d * ior + ln * t};
+                rfr = trace_ray(s, sh, cnt, nr, nw, level, refracted(rf), ni, false);
+            }
+        }
+        col = col + (tir ? rfr : rfr * fil);
+    }
+    // reflection: ComputeReflectivity (no fresnel) + ComputeReflection;
+    // skipped after total internal reflection (trace.cpp:1141-1147)
     double cos_inc = -dot(ray.d, n);
     double w = (double)weight * std::max(max3f(m.reflection_max), max3f(m.reflection_min));
     double frac = std::fabs((double)m.reflection_falloff - 1.0) > kEpsilon
@@ -191,12 +285,12 @@ RGB trace_ray(const Scene &s, const Shading &sh, Counters &cnt, const Ray &ray, 
         refl = rgb3(m.reflection_max);
     else
         refl = scale(rgb3(m.reflection_max), frac) + scale(rgb3(m.reflection_min), 1.0 - frac);
-    if (!(refl.r == 0.0f && refl.g == 0.0f && refl.b == 0.0f)) {
+    if (!tir && !(refl.r == 0.0f && refl.g == 0.0f && refl.b == 0.0f)) {
         Ray nr;
         double k = -2.0 * dot(ray.d, n);
         nr.d = normalized(ray.d + n * k);
         nr.o = ip;
-        RGB rc = trace_ray(s, sh, cnt, nr, (float)w, level, PVX_RAY_REFLECTION);
+        RGB rc = trace_ray(s, sh, cnt, nr, (float)w, level, reflected(rf), ins, false);
         col = col + refl * rc;
     }
     return col;
@@ -239,7 +333,7 @@ extern "C" int orc_render(void *sp, const pvx_shading_desc *sd, const pvx_camera
                 for (int k = 0; k < 3; k++)
                     d[k] = cam->direction[k] + x0 * cam->right[k] + y0 * cam->up[k];
                 Ray r{V3(cam->location[0], cam->location[1], cam->location[2]), normalized(d)};
-                RGB c = trace_ray(s, sh, cnt, r, 1.0f, 0, PVX_RAY_IMAGE);
+                RGB c = trace_ray(s, sh, cnt, r, 1.0f, 0, RayFlags{true, false, false}, Interiors{}, false);
                 rgb[3 * i] = c.r;
                 rgb[3 * i + 1] = c.g;
                 rgb[3 * i + 2] = c.b;

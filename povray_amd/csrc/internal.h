@@ -33,6 +33,17 @@ struct DevBuf {
     int reserve(size_t bytes);
 };
 
+// Pinned host buffer grown on demand (contents are not preserved).
+struct HostBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    HostBuf() = default;
+    HostBuf(const HostBuf &) = delete;
+    HostBuf &operator=(const HostBuf &) = delete;
+    ~HostBuf();
+    int reserve(size_t bytes);
+};
+
 // Shading state attached by pvx_scene_set_shading.
 struct ShadingState {
     bool set = false;
@@ -46,15 +57,13 @@ struct ShadingState {
     LightRec *d_lights = nullptr;
 };
 
-// Render work buffers, grown on demand.
+// Render work buffers, grown on demand and kept between frames.
 struct RenderBuffers {
-    size_t cap_rays = 0, cap_shadow = 0, cap_shade = 0;
-    void *rays[2] = {nullptr, nullptr};
-    void *state[2] = {nullptr, nullptr};
-    void *hits = nullptr;
-    void *shade = nullptr;
-    void *srays = nullptr, *sdepth = nullptr, *shits = nullptr;
-    uint32_t *counters = nullptr; // [next rays, shadow rays]
+    DevBuf rays[2], state[2], kinds[2]; // ray queues (ping-pong)
+    DevBuf hits, contrib, shade, lslot, lterm;
+    DevBuf crays, cstate, nchild, offs, sums; // children of a bounce and their placement
+    DevBuf srays[2], sdepth[2], sowner[2], shits, scol; // light rays
+    DevBuf counters;
 };
 
 } // namespace pvx
@@ -64,7 +73,6 @@ struct pvx_scene {
     pvx::HostScene host;
     pvx::SceneView view{};
     std::vector<void *> allocs;
-    std::vector<void *> render_allocs;
     uint32_t *d_obj_type = nullptr, *d_obj_index = nullptr;
     int32_t *d_spill = nullptr;
     uint32_t grid = 0;
@@ -72,12 +80,17 @@ struct pvx_scene {
     uint32_t n_objects = 0;
     pvx::ShadingState shading;
     pvx::RenderBuffers rb;
+    void *rad_engine = nullptr;               // radiosity engine kept between frames
+    void (*rad_engine_free)(void *) = nullptr;
 };
 
 namespace pvx {
 // Closest-hit launch shared with the renderer (no counters).
+// `kinds` (optional): one PVX_RAY_* kind per ray for the object-flag
+// precondition of rays of mixed kinds (reflected and refracted).
 int trace_closest_internal(pvx_scene *s, const pvx_ray *rays, const double *max_depth,
-                           uint32_t ray_kind, size_t n, pvx_hit *hits, hipStream_t stream);
+                           uint32_t ray_kind, size_t n, pvx_hit *hits, hipStream_t stream,
+                           const uint8_t *kinds = nullptr);
 } // namespace pvx
 
 #endif

@@ -133,8 +133,8 @@ __device__ inline bool simple_closest(const SceneView &sv, uint32_t kind, uint32
 template <int kLevel, bool kFlags, bool kCount>
 __global__ __launch_bounds__(kBlock) void k_trace_closest(SceneView sv, const pvx_ray *__restrict__ rays,
                                                           const double *__restrict__ max_depth,
-                                                          uint32_t kind, uint32_t n,
-                                                          pvx_hit *__restrict__ hits,
+                                                          uint32_t kind, const uint8_t *__restrict__ kinds,
+                                                          uint32_t n, pvx_hit *__restrict__ hits,
                                                           int32_t *__restrict__ spill,
                                                           uint32_t *__restrict__ node_visits,
                                                           uint32_t *__restrict__ prim_tests) {
@@ -146,6 +146,8 @@ __global__ __launch_bounds__(kBlock) void k_trace_closest(SceneView sv, const pv
 
     for (uint32_t i = gtid; i < n; i += gridDim.x * kBlock) {
         const pvx_ray r = rays[i];
+        // per-ray kind (precondition only; shadow rays come in launches of their own)
+        const uint32_t rk = (kFlags && kinds) ? (uint32_t)kinds[i] : kind;
         D3 o = d3(r.o[0], r.o[1], r.o[2]);
         D3 d = d3(r.d[0], r.d[1], r.d[2]);
         double bt = kBoundHuge;
@@ -158,7 +160,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_closest(SceneView sv, const pv
         // objects outside the BVH (planes, unbounded CSG)
         for (uint32_t k = 0; k < sv.n_inf; k++) {
             const InfRec inf = sv.inf[k];
-            if (kFlags && !precondition(sv.obj_flags[inf.obj], kind)) continue;
+            if (kFlags && !precondition(sv.obj_flags[inf.obj], rk)) continue;
             double dep;
             int32_t prim = 0;
             bool ok;
@@ -241,7 +243,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_closest(SceneView sv, const pv
                     const uint32_t ka = p.kind_axis;
                     const uint32_t pk = ka & 15u;
                     const int32_t obj = p.obj;
-                    if (kFlags && !precondition(sv.obj_flags[obj], kind)) continue;
+                    if (kFlags && !precondition(sv.obj_flags[obj], rk)) continue;
                     if (pk == kPrimTriangle) {
                         double dep;
                         if (tri_hit64(p.tri, ka >> 4, o, d, dep) && depth_ok(dep, shadow) &&
@@ -453,10 +455,10 @@ static int upload(pvx_scene *s, const std::vector<T> &v, const T **dst) {
 }
 
 static void scene_free(pvx_scene *s) {
+    if (s->rad_engine_free && s->rad_engine) s->rad_engine_free(s->rad_engine);
+    s->rad_engine = nullptr;
     for (void *p : s->allocs) (void)hipFree(p);
     s->allocs.clear();
-    for (void *p : s->render_allocs) (void)hipFree(p);
-    s->render_allocs.clear();
     if (s->shading.d_mats) (void)hipFree(s->shading.d_mats);
     if (s->shading.d_lights) (void)hipFree(s->shading.d_lights);
     s->shading.d_mats = nullptr;
@@ -577,18 +579,18 @@ extern "C" int pvx_scene_get_stats(const pvx_scene *s, pvx_scene_stats *out) {
 
 template <bool kCount>
 static int launch_closest(pvx_scene *s, const pvx_ray *rays, const double *max_depth,
-                          uint32_t ray_kind, size_t n, pvx_hit *hits, uint32_t *nv, uint32_t *pt,
-                          void *stream) {
+                          uint32_t ray_kind, const uint8_t *kinds, size_t n, pvx_hit *hits, uint32_t *nv,
+                          uint32_t *pt, void *stream) {
     if (!s) return fail(PVX_EINVAL, "null scene");
     if (n == 0) return PVX_OK;
     if (!rays || !hits) return fail(PVX_EINVAL, "null ray or hit buffer");
     if (n > 0xFFFFFFFFull) return fail(PVX_EINVAL, "too many rays in one call");
     hipStream_t st = static_cast<hipStream_t>(stream);
     uint32_t nblocks = (uint32_t)std::min<size_t>(s->grid, (n + kBlock - 1) / kBlock);
-    bool flags = s->view.any_flags != 0 && ray_kind != PVX_RAY_PRIMARY;
+    bool flags = s->view.any_flags != 0 && (ray_kind != PVX_RAY_PRIMARY || kinds);
     uint32_t nn = (uint32_t)n;
 #define PVX_LAUNCH(L, F) \
-    k_trace_closest<L, F, kCount><<<nblocks, kBlock, 0, st>>>(s->view, rays, max_depth, ray_kind, nn, hits, s->d_spill, nv, pt)
+    k_trace_closest<L, F, kCount><<<nblocks, kBlock, 0, st>>>(s->view, rays, max_depth, ray_kind, kinds, nn, hits, s->d_spill, nv, pt)
     const uint32_t level = s->view.has_complex;
     if (level >= 2) {
         if (flags) PVX_LAUNCH(2, true); else PVX_LAUNCH(2, false);
@@ -603,13 +605,14 @@ static int launch_closest(pvx_scene *s, const pvx_ray *rays, const double *max_d
 }
 
 int pvx::trace_closest_internal(pvx_scene *s, const pvx_ray *rays, const double *max_depth,
-                                uint32_t ray_kind, size_t n, pvx_hit *hits, hipStream_t stream) {
-    return launch_closest<false>(s, rays, max_depth, ray_kind, n, hits, nullptr, nullptr, stream);
+                                uint32_t ray_kind, size_t n, pvx_hit *hits, hipStream_t stream,
+                                const uint8_t *kinds) {
+    return launch_closest<false>(s, rays, max_depth, ray_kind, kinds, n, hits, nullptr, nullptr, stream);
 }
 
 extern "C" int pvx_trace_closest(pvx_scene *s, const pvx_ray *rays, const double *max_depth,
                                  uint32_t ray_kind, size_t n, pvx_hit *hits, void *stream) {
-    return launch_closest<false>(s, rays, max_depth, ray_kind, n, hits, nullptr, nullptr, stream);
+    return launch_closest<false>(s, rays, max_depth, ray_kind, nullptr, n, hits, nullptr, nullptr, stream);
 }
 
 extern "C" int pvx_trace_closest_counted(pvx_scene *s, const pvx_ray *rays,
@@ -617,7 +620,7 @@ extern "C" int pvx_trace_closest_counted(pvx_scene *s, const pvx_ray *rays,
                                          pvx_hit *hits, uint32_t *node_visits,
                                          uint32_t *prim_tests, void *stream) {
     if (!node_visits || !prim_tests) return fail(PVX_EINVAL, "null counter buffer");
-    return launch_closest<true>(s, rays, max_depth, ray_kind, n, hits, node_visits, prim_tests,
+    return launch_closest<true>(s, rays, max_depth, ray_kind, nullptr, n, hits, node_visits, prim_tests,
                                 stream);
 }
 

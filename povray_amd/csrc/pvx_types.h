@@ -131,6 +131,11 @@ struct MatRec {
     float falloff;
     float rweight;        // max(WeightMax(rmax), WeightMax(rmin))
     uint32_t reflective;  // any reflection component non-zero
+    float fil[3];         // TransmittedColour: pigment * filter + transmit
+    float trans;          // min(1, |grey(fil)|): remaining translucency
+    float fmax;           // WeightMaxAbs(fil)
+    float ior;            // interior IOR (every object has an interior; default 1)
+    uint32_t opaque;      // filter == transmit == 0 (OPAQUE_FLAG)
 };
 struct LightRec {
     double pos[3];

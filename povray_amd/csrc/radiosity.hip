@@ -24,7 +24,6 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
-#include <unordered_set>
 #include <vector>
 
 #include "internal.h"
@@ -236,8 +235,7 @@ __global__ __launch_bounds__(256) void k_gather_reduce(
     const RadQuery *__restrict__ q, const uint32_t *__restrict__ idx, uint32_t m, uint32_t N,
     const pvx_ray *__restrict__ rays, const pvx_hit *__restrict__ hits, const float *__restrict__ base,
     const float *__restrict__ coef, const float *__restrict__ amb, double cx, double cy, double cz,
-    float min_reuse, float max_reuse, float factor, float *__restrict__ Eg, RadRec *__restrict__ rec,
-    uint8_t *__restrict__ rec_ok) {
+    float min_reuse, float max_reuse, float factor, float *__restrict__ Eg, RadRec *__restrict__ rec) {
     const uint32_t s = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t lane = threadIdx.x & 63;
     if (s >= m) return; // whole wavefronts leave together
@@ -308,9 +306,8 @@ __global__ __launch_bounds__(256) void k_gather_reduce(
     double gm = (0.297 * sum[0] + 0.589 * sum[1] + 0.114 * sum[2]) / N;
     double var = N > 1 ? fmax(0.0, (sq - N * gm * gm) / (N - 1)) : 0.0;
     r.pad[0] = (float)sqrt(var / N);
-    r.pad[1] = 0.0f;
+    r.pad[1] = ok ? 1.0f : 0.0f; // filed in the cache (AddBlock condition)
     rec[s] = r;
-    rec_ok[s] = ok ? 1 : 0;
 }
 
 // ComputeAmbient's combination of the partial reuse with the new gather
@@ -395,11 +392,12 @@ __global__ void k_scatter3(const float *__restrict__ src, const uint32_t *__rest
 }
 
 // Surrogate inputs: position scaled into the unit cube, eye-facing normal.
-__global__ void k_rec_to_sample(const RadRec *__restrict__ recs, uint32_t n, float lx, float ly, float lz,
-                                float inv, float *__restrict__ x6, float *__restrict__ t) {
+__global__ void k_rec_to_sample(const RadRec *__restrict__ recs, const uint32_t *__restrict__ ridx, uint32_t n,
+                                float lx, float ly, float lz, float inv, float *__restrict__ x6,
+                                float *__restrict__ t) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const RadRec b = recs[i];
+    const RadRec b = recs[ridx[i]];
     x6[6 * (size_t)i + 0] = fminf(fmaxf((b.p[0] - lx) * inv, 0.0f), 1.0f);
     x6[6 * (size_t)i + 1] = fminf(fmaxf((b.p[1] - ly) * inv, 0.0f), 1.0f);
     x6[6 * (size_t)i + 2] = fminf(fmaxf((b.p[2] - lz) * inv, 0.0f), 1.0f);
@@ -447,12 +445,13 @@ int DevBuf::reserve(size_t bytes) {
     if (p) (void)hipFree(p);
     p = nullptr;
     cap = 0;
-    HIP_TRY(hipMalloc(&p, bytes));
-    cap = bytes;
+    size_t want = std::max(bytes + bytes / 4, (size_t)4096); // slack: fewer regrowths
+    HIP_TRY(hipMalloc(&p, want));
+    cap = want;
     return PVX_OK;
 }
 
-int RadGrid::build(const std::vector<RadRec> &recs, size_t first, size_t last, float reach, hipStream_t st) {
+int RadGrid::build(const std::vector<RadRec> &recs, size_t first, size_t last, float reach, RadiosityEngine &eng) {
     n = (uint32_t)(last - first);
     if (n == 0) return PVX_OK;
     float lo_[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -468,9 +467,13 @@ int RadGrid::build(const std::vector<RadRec> &recs, size_t first, size_t last, f
     }
     std::nth_element(rs.begin(), rs.begin() + rs.size() / 2, rs.end());
     float ext = std::max(hi[0] - lo_[0], std::max(hi[1] - lo_[1], hi[2] - lo_[2]));
-    // cells about the median sphere size, at most 64 per axis, grown until
-    // the record->cell lists stay within a few entries per record
-    cell = std::max(std::max(rs[rs.size() / 2], ext / 64.0f), 1e-6f);
+    // cells about the median sphere size, at most 64 per axis and about four
+    // cells per record, grown until the record->cell lists stay within a few
+    // entries per record
+    const float vol = std::max(hi[0] - lo_[0], 1e-6f) * std::max(hi[1] - lo_[1], 1e-6f) *
+                      std::max(hi[2] - lo_[2], 1e-6f);
+    cell = std::max(std::max(rs[rs.size() / 2], ext / 64.0f), std::cbrt(vol / (4.0f * n + 64.0f)));
+    cell = std::max(cell, 1e-6f);
     const double budget = 8.0 * (double)n + 65536.0;
     for (int tries = 0; tries < 64; tries++) {
         double total = 0;
@@ -523,13 +526,13 @@ int RadGrid::build(const std::vector<RadRec> &recs, size_t first, size_t last, f
     int r;
     if ((r = d_start.reserve(start.size() * 4)) || (r = d_items.reserve(std::max<size_t>(1, items.size()) * 4)))
         return r;
-    HIP_TRY(hipMemcpyAsync(d_start.p, start.data(), start.size() * 4, hipMemcpyHostToDevice, st));
-    if (!items.empty())
-        HIP_TRY(hipMemcpyAsync(d_items.p, items.data(), items.size() * 4, hipMemcpyHostToDevice, st));
+    if ((r = eng.send(d_start.p, start.data(), start.size() * 4)) ||
+        (r = eng.send(d_items.p, items.data(), items.size() * 4)))
+        return r;
     return PVX_OK;
 }
 
-int RadCache::upload(hipStream_t st) {
+int RadCache::upload(RadiosityEngine &eng) {
     if (uploaded == recs.size()) return PVX_OK;
     size_t bytes = recs.size() * sizeof(RadRec);
     if (bytes > d_recs.cap) { // grow (doubling) and send everything
@@ -537,27 +540,23 @@ int RadCache::upload(hipStream_t st) {
         if ((r = d_recs.reserve(std::max(bytes, 2 * d_recs.cap)))) return r;
         uploaded = 0;
     }
-    HIP_TRY(hipMemcpyAsync(static_cast<RadRec *>(d_recs.p) + uploaded, recs.data() + uploaded,
-                           (recs.size() - uploaded) * sizeof(RadRec), hipMemcpyHostToDevice, st));
+    int r2;
+    if ((r2 = eng.send(static_cast<RadRec *>(d_recs.p) + uploaded, recs.data() + uploaded,
+                       (recs.size() - uploaded) * sizeof(RadRec))))
+        return r2;
     uploaded = recs.size();
     return PVX_OK;
 }
 
-int RadCache::index_all(hipStream_t st) {
+int RadCache::index_all(RadiosityEngine &eng) {
     if (indexed == recs.size()) return PVX_OK;
     int r;
-    if ((r = upload(st)) || (r = all.build(recs, 0, recs.size(), reach, st))) return r;
+    if ((r = upload(eng)) || (r = all.build(recs, 0, recs.size(), reach, eng))) return r;
     indexed = recs.size();
     return PVX_OK;
 }
 
-RadiosityEngine::RadiosityEngine(pvx_scene *s, const pvx_radiosity_settings &set, const pvx_camera &c,
-                                 hipStream_t st)
-    : scene(s), rs(set), cam(c), stream(st) {
-    for (int d = 0; d < kMaxRadDepth; d++) {
-        caches[d].reach = (float)(rs.error_bound + kWeightErrOffset) * std::pow(2.0f, (float)d);
-        cursor[d] = 0;
-    }
+RadiosityEngine::RadiosityEngine(pvx_scene *s) : scene(s) {
     pool.resize(3 * kPoolSize);
     for (int i = 0; i < kPoolSize; i++) { // HaltonCosWeightedDirectionGenerator (bases 2, 3)
         auto halton = [](size_t index, unsigned base) {
@@ -579,17 +578,81 @@ RadiosityEngine::RadiosityEngine(pvx_scene *s, const pvx_radiosity_settings &set
     }
 }
 
-int RadiosityEngine::init() {
+int RadiosityEngine::begin(const pvx_radiosity_settings &set, const pvx_camera &c, hipStream_t st) {
+    rs = set;
+    cam = c;
+    stream = st;
+    for (int d = 0; d < kMaxRadDepth; d++) {
+        RadCache &cc = caches[d];
+        cc.recs.clear();
+        cc.uploaded = cc.indexed = 0;
+        cc.all.n = cc.fresh.n = 0;
+        cc.reach = (float)(rs.error_bound + kWeightErrOffset) * std::pow(2.0f, (float)d);
+        cursor[d] = 0;
+        stats_rounds[d] = 0;
+    }
+    stats = pvx_radiosity_stats{};
+    train_ms = t_sync_ms = t_grid_ms = 0.0;
+    n_sync = 0;
+    up_off = 0;
+    if (!d_pool.p) {
+        int r;
+        if ((r = d_pool.reserve(pool.size() * 8))) return r;
+        HIP_TRY(hipMemcpy(d_pool.p, pool.data(), pool.size() * 8, hipMemcpyHostToDevice));
+    }
+    return PVX_OK;
+}
+
+// Host -> device through the pinned staging buffer; regions are reused only
+// after the next wait on the stream, when every earlier copy has finished.
+int RadiosityEngine::send(void *dev, const void *host, size_t bytes) {
+    if (bytes == 0) return PVX_OK;
+    const size_t need = (bytes + 255) & ~(size_t)255;
+    if (up_off + need > up.cap) {
+        HIP_TRY(sync()); // every copy from the buffer has finished
+        if (need > up.cap) {
+            int r;
+            if ((r = up.reserve(std::max(need, 2 * up.cap)))) return r;
+        }
+    }
+    char *p = static_cast<char *>(up.p) + up_off;
+    std::memcpy(p, host, bytes);
+    HIP_TRY(hipMemcpyAsync(dev, p, bytes, hipMemcpyHostToDevice, stream));
+    up_off += need;
+    return PVX_OK;
+}
+
+
+// Device -> host through a pinned bounce buffer (an asynchronous copy and
+// one wait, instead of a pageable copy that blocks inside the runtime).
+int RadiosityEngine::fetch(void *host, const void *dev, size_t bytes) {
+    if (bytes == 0) return PVX_OK;
     int r;
-    if ((r = d_pool.reserve(pool.size() * 8))) return r;
-    HIP_TRY(hipMemcpyAsync(d_pool.p, pool.data(), pool.size() * 8, hipMemcpyHostToDevice, stream));
+    if ((r = bounce.reserve(bytes))) return r;
+    HIP_TRY(hipMemcpyAsync(bounce.p, dev, bytes, hipMemcpyDeviceToHost, stream));
     HIP_TRY(sync());
+    std::memcpy(host, bounce.p, bytes);
+    return PVX_OK;
+}
+
+HostBuf::~HostBuf() {
+    if (p) (void)hipHostFree(p);
+}
+int HostBuf::reserve(size_t bytes) {
+    if (bytes <= cap) return PVX_OK;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t want = std::max(bytes, (size_t)1 << 20);
+    HIP_TRY(hipHostMalloc(&p, want, hipHostMallocDefault));
+    cap = want;
     return PVX_OK;
 }
 
 hipError_t RadiosityEngine::sync() {
     auto t0 = std::chrono::steady_clock::now();
     hipError_t e = hipStreamSynchronize(stream);
+    if (e == hipSuccess) up_off = 0;
     t_sync_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     n_sync++;
     return e;
@@ -635,7 +698,7 @@ int RadiosityEngine::gather(int depth, const RadQuery *q, const uint32_t *idx, u
     if ((r = S.rays.reserve(nr * sizeof(pvx_ray))) || (r = S.hits.reserve(nr * sizeof(pvx_hit))) ||
         (r = S.base.reserve(nr * 12)) || (r = S.coef.reserve(nr * 12)) || (r = S.amb.reserve(nr * 12)) ||
         (r = S.q.reserve(nr * sizeof(RadQuery))) || (r = S.need.reserve(nr)) ||
-        (r = S.rec.reserve(m * sizeof(RadRec))) || (r = S.rec_ok.reserve(m)))
+        (r = S.rec.reserve(m * sizeof(RadRec))))
         return r;
     k_gather_rays<<<blocks(nr), 256, 0, stream>>>(q, idx, m, N, static_cast<const double *>(d_pool.p),
                                                   cursor[depth], static_cast<pvx_ray *>(S.rays.p));
@@ -665,8 +728,7 @@ int RadiosityEngine::gather(int depth, const RadQuery *q, const uint32_t *idx, u
     if (deeper) {
         // compact the hits that need irradiance at depth + 1 (deterministic order)
         std::vector<uint8_t> need(nr);
-        HIP_TRY(hipMemcpyAsync(need.data(), S.need.p, nr, hipMemcpyDeviceToHost, stream));
-        HIP_TRY(sync());
+        if ((r = fetch(need.data(), S.need.p, nr))) return r;
         std::vector<uint32_t> sel;
         sel.reserve(nr);
         for (size_t k = 0; k < nr; k++)
@@ -676,7 +738,7 @@ int RadiosityEngine::gather(int depth, const RadQuery *q, const uint32_t *idx, u
             if ((r = di.reserve(sel.size() * 4)) || (r = S.subq.reserve(sel.size() * sizeof(RadQuery))) ||
                 (r = S.subE.reserve(sel.size() * 12)))
                 return r;
-            HIP_TRY(hipMemcpyAsync(di.p, sel.data(), sel.size() * 4, hipMemcpyHostToDevice, stream));
+            if ((r = send(di.p, sel.data(), sel.size() * 4))) return r;
             k_gather_subq<<<blocks(sel.size()), 256, 0, stream>>>(
                 static_cast<const RadQuery *>(S.q.p), static_cast<const uint32_t *>(di.p),
                 (uint32_t)sel.size(), static_cast<RadQuery *>(S.subq.p));
@@ -694,19 +756,20 @@ int RadiosityEngine::gather(int depth, const RadQuery *q, const uint32_t *idx, u
         q, idx, m, N, static_cast<const pvx_ray *>(S.rays.p), static_cast<const pvx_hit *>(S.hits.p),
         static_cast<const float *>(S.base.p), static_cast<const float *>(S.coef.p),
         static_cast<const float *>(S.amb.p), cam.location[0], cam.location[1], cam.location[2],
-        rs.minimum_reuse, rs.maximum_reuse, factor, Eg, static_cast<RadRec *>(S.rec.p),
-        static_cast<uint8_t *>(S.rec_ok.p));
+        rs.minimum_reuse, rs.maximum_reuse, factor, Eg, static_cast<RadRec *>(S.rec.p));
     HIP_TRY(hipGetLastError());
     // append the records to the cache (AddBlock)
     std::vector<RadRec> rec(m);
-    std::vector<uint8_t> ok(m);
-    HIP_TRY(hipMemcpyAsync(rec.data(), S.rec.p, m * sizeof(RadRec), hipMemcpyDeviceToHost, stream));
-    HIP_TRY(hipMemcpyAsync(ok.data(), S.rec_ok.p, m, hipMemcpyDeviceToHost, stream));
-    HIP_TRY(sync());
+    if ((r = fetch(rec.data(), S.rec.p, m * sizeof(RadRec)))) return r;
     RadCache &c = caches[depth];
+    std::vector<int32_t> &lr = last_rec[depth];
+    lr.assign(m, -1);
     for (uint32_t k = 0; k < m; k++)
-        if (ok[k]) c.recs.push_back(rec[k]);
-    return c.upload(stream);
+        if (rec[k].pad[1] != 0.0f) {
+            lr[k] = (int32_t)c.recs.size();
+            c.recs.push_back(rec[k]);
+        }
+    return c.upload(*this);
 }
 
 // Spread-out subset of the pending queries: the first pending point of
@@ -734,13 +797,33 @@ void RadiosityEngine::spread_select(int depth, const std::vector<uint32_t> &todo
     }
     std::nth_element(rad.begin(), rad.begin() + rad.size() / 2, rad.end());
     const double D = std::max(rad[rad.size() / 2] * rs.error_bound * factor, 1e-9);
-    std::unordered_set<uint64_t> taken;
-    taken.reserve(todo.size());
+    // open-addressing set of occupied cells (21-bit cell coordinates)
+    size_t cap = 16;
+    while (cap < 2 * todo.size()) cap <<= 1;
+    std::vector<uint64_t> table(cap, 0);
+    double pmax = 0;
+    for (size_t k = 0; k < todo.size(); k += std::max<size_t>(1, todo.size() / 256))
+        for (int d = 0; d < 3; d++) pmax = std::max(pmax, std::fabs(hq[slot[k]].p[d]));
+    const double Dc = std::max(D, (pmax + 1.0) * 2e-6);
     for (size_t k = 0; k < todo.size(); k++) {
         const RadQuery &a = hq[slot[k]];
-        uint64_t key = 0;
-        for (int d = 0; d < 3; d++) key = key * 2097143u + (uint64_t)(int64_t)std::floor(a.p[d] / D);
-        if (taken.insert(key).second) {
+        uint64_t key = 1; // never 0: 0 marks an empty slot
+        for (int d = 0; d < 3; d++) {
+            int64_t cidx = (int64_t)std::floor(a.p[d] / Dc) + (1 << 20);
+            key = (key << 21) | ((uint64_t)cidx & 0x1FFFFFu);
+        }
+        uint64_t h = key * 0x9E3779B97F4A7C15ull;
+        size_t pos = (size_t)(h >> 20) & (cap - 1);
+        bool fresh = true;
+        while (table[pos]) {
+            if (table[pos] == key) {
+                fresh = false;
+                break;
+            }
+            pos = (pos + 1) & (cap - 1);
+        }
+        if (fresh) table[pos] = key;
+        if (fresh) {
             sel.push_back(todo[k]);
             sel_slot.push_back(slot[k]);
         } else {
@@ -758,8 +841,8 @@ int RadiosityEngine::gather_round(int depth, const RadQuery *q, const std::vecto
     Resolve &R = rstate[depth];
     uint32_t *d_idx = static_cast<uint32_t *>(R.idx.p);
     stats_rounds[depth]++;
-    HIP_TRY(hipMemcpyAsync(d_idx, sel.data(), sel.size() * 4, hipMemcpyHostToDevice, stream));
     int r;
+    if ((r = send(d_idx, sel.data(), sel.size() * 4))) return r;
     if ((r = gather(depth, q, d_idx, (uint32_t)sel.size(), final, static_cast<float *>(R.Eg.p)))) return r;
     k_combine<<<blocks(sel.size()), 256, 0, stream>>>(d_idx, (uint32_t)sel.size(),
                                                       static_cast<const float *>(R.Eg.p), acc, E);
@@ -781,15 +864,14 @@ int RadiosityEngine::relookup(int depth, const RadQuery *q, size_t first_new, bo
     uint8_t *d_need = static_cast<uint8_t *>(R.need.p);
     int r;
     auto tg = std::chrono::steady_clock::now();
-    if ((r = c.fresh.build(c.recs, first_new, c.recs.size(), c.reach, stream))) return r;
+    if ((r = c.fresh.build(c.recs, first_new, c.recs.size(), c.reach, *this))) return r;
     t_grid_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tg).count();
-    HIP_TRY(hipMemcpyAsync(d_idx, rest.data(), rest.size() * 4, hipMemcpyHostToDevice, stream));
+    if ((r = send(d_idx, rest.data(), rest.size() * 4))) return r;
     if ((r = lookup(depth, c.fresh, q, d_idx, (uint32_t)rest.size(), final, true, acc))) return r;
     k_need<<<blocks(rest.size()), 256, 0, stream>>>(acc, d_idx, (uint32_t)rest.size(), reuse_count(depth),
                                                     final ? 1 : 0, d_need);
     std::vector<uint8_t> need(rest.size());
-    HIP_TRY(hipMemcpyAsync(need.data(), d_need, rest.size(), hipMemcpyDeviceToHost, stream));
-    HIP_TRY(sync());
+    if ((r = fetch(need.data(), d_need, rest.size()))) return r;
     for (size_t k = 0; k < rest.size(); k++)
         if (need[k]) {
             todo.push_back(rest[k]);
@@ -801,12 +883,12 @@ int RadiosityEngine::relookup(int depth, const RadQuery *q, size_t first_new, bo
 // Gathers rounds until every pending query has its irradiance.
 int RadiosityEngine::gather_rounds(int depth, const RadQuery *q, bool final, std::vector<uint32_t> todo,
                                    std::vector<uint32_t> slot, const std::vector<RadQuery> &hq, RadAcc *acc,
-                                   float *E) {
+                                   float *E, int max_rounds) {
     RadCache &c = caches[depth];
     int r;
     for (int round = 0; !todo.empty(); round++) {
         std::vector<uint32_t> sel, sel_slot, rest, rest_slot;
-        if (round < kMaxResolveRounds - 1 && todo.size() > 64)
+        if (round < max_rounds - 1 && todo.size() > 64)
             spread_select(depth, todo, slot, hq, sel, sel_slot, rest, rest_slot);
         else
             sel.swap(todo);
@@ -833,13 +915,12 @@ int RadiosityEngine::start_resolve(int depth, const RadQuery *q, uint32_t n, boo
     uint32_t *d_idx = static_cast<uint32_t *>(R.idx.p);
     uint8_t *d_need = static_cast<uint8_t *>(R.need.p);
     auto tg = std::chrono::steady_clock::now();
-    if ((r = c.index_all(stream))) return r;
+    if ((r = c.index_all(*this))) return r;
     t_grid_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tg).count();
     if ((r = lookup(depth, c.all, q, nullptr, n, final, false, acc))) return r;
     k_need<<<blocks(n), 256, 0, stream>>>(acc, nullptr, n, reuse_count(depth), final ? 1 : 0, d_need);
     std::vector<uint8_t> need(n);
-    HIP_TRY(hipMemcpyAsync(need.data(), d_need, n, hipMemcpyDeviceToHost, stream));
-    HIP_TRY(sync());
+    if ((r = fetch(need.data(), d_need, n))) return r;
     todo.clear();
     for (uint32_t i = 0; i < n; i++)
         if (need[i]) todo.push_back(i);
@@ -848,12 +929,11 @@ int RadiosityEngine::start_resolve(int depth, const RadQuery *q, uint32_t n, boo
     for (size_t k = 0; k < todo.size(); k++) slot[k] = (uint32_t)k;
     hq.clear();
     if (!todo.empty()) {
-        HIP_TRY(hipMemcpyAsync(d_idx, todo.data(), todo.size() * 4, hipMemcpyHostToDevice, stream));
+        if ((r = send(d_idx, todo.data(), todo.size() * 4))) return r;
         k_gather_subq<<<blocks(todo.size()), 256, 0, stream>>>(q, d_idx, (uint32_t)todo.size(),
                                                                static_cast<RadQuery *>(R.tq.p));
         hq.resize(todo.size());
-        HIP_TRY(hipMemcpyAsync(hq.data(), R.tq.p, todo.size() * sizeof(RadQuery), hipMemcpyDeviceToHost, stream));
-        HIP_TRY(sync());
+        if ((r = fetch(hq.data(), R.tq.p, todo.size() * sizeof(RadQuery)))) return r;
     }
     return PVX_OK;
 }
@@ -874,12 +954,13 @@ int RadiosityEngine::resolve(int depth, const RadQuery *q, uint32_t n, bool fina
     return PVX_OK;
 }
 
-// Final-pass irradiance with the surrogate (depth 0): the points no record
-// reaches are split into (1) a spread-out set of plain gathers, which become
-// training samples together with the pretrace records, (2) probe gathers,
-// a few per image tile, that check the trained network, and (3) the rest,
-// answered by the network except in tiles whose probes disagree by more than
-// the threshold, which fall back to plain gathers.
+// Final-pass irradiance with the surrogate (depth 0).  The points no record
+// reaches get one round of plain gathers at (1) a spread-out subset, whose
+// records join the pretrace records as training samples, and (2) probe
+// points, a few per image tile, held out to check the trained network.
+// What the round's records still do not reach is answered by the network,
+// except in tiles whose probes disagree by more than the threshold; those
+// fall back to plain gathers (at most two more rounds).
 int RadiosityEngine::resolve_learned(const RadQuery *q, uint32_t n, const std::vector<uint32_t> &pix, float *E) {
     if (n == 0) return PVX_OK;
     RadCache &c = caches[0];
@@ -888,193 +969,183 @@ int RadiosityEngine::resolve_learned(const RadQuery *q, uint32_t n, const std::v
     std::vector<RadQuery> hq;
     int r;
     if ((r = start_resolve(0, q, n, true, acc, todo, slot, hq))) return r;
-    const std::vector<uint32_t> todo_all = todo; // hq[k] belongs to todo_all[k]
-    if (!todo.empty()) {
-        // (1) training gathers
-        std::vector<uint32_t> sel, sel_slot, rest, rest_slot;
-        spread_select(0, todo, slot, hq, sel, sel_slot, rest, rest_slot);
-        const size_t T = std::max<uint32_t>(1, rs.train_samples);
-        if (sel.size() > T) { // keep an even subset
-            std::vector<uint32_t> keep, keep_slot;
-            const double step = (double)sel.size() / (double)T;
-            size_t next = 0;
-            for (size_t k = 0; k < sel.size(); k++) {
-                if (k == (size_t)(next * step) && keep.size() < T) {
-                    keep.push_back(sel[k]);
-                    keep_slot.push_back(sel_slot[k]);
-                    next++;
-                } else {
-                    rest.push_back(sel[k]);
-                    rest_slot.push_back(sel_slot[k]);
-                }
-            }
-            sel.swap(keep);
-        }
-        size_t first_new = c.recs.size();
-        if ((r = gather_round(0, q, sel, true, acc, E)) ||
-            (r = relookup(0, q, first_new, true, acc, rest, rest_slot, todo, slot)))
-            return r;
+    if (todo.empty()) {
+        k_finish<<<blocks(n), 256, 0, stream>>>(acc, n, rs.brightness, E);
+        HIP_TRY(hipGetLastError());
+        return PVX_OK;
     }
-    if (!todo.empty()) {
-        // (2) train on every depth-0 record
-        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-        for (const RadRec &b : c.recs)
-            for (int a = 0; a < 3; a++) {
-                lo[a] = std::min(lo[a], b.p[a]);
-                hi[a] = std::max(hi[a], b.p[a]);
+    // image tiles of the pending points
+    const uint32_t tsz = std::max<uint32_t>(1, rs.probe_tile);
+    const uint32_t tx = (cam.width + tsz - 1) / tsz, ty = (cam.height + tsz - 1) / tsz;
+    auto tile_of = [&](uint32_t i) {
+        uint32_t p = pix[i];
+        return (p % cam.width) / tsz + ((p / cam.width) / tsz) * tx;
+    };
+    // (1) training subset: spread out, at most train_samples
+    std::vector<uint32_t> sel, sel_slot, rest, rest_slot;
+    spread_select(0, todo, slot, hq, sel, sel_slot, rest, rest_slot);
+    const size_t T = std::max<uint32_t>(1, rs.train_samples);
+    if (sel.size() > T) { // keep an even subset
+        std::vector<uint32_t> keep, keep_slot;
+        const double step = (double)sel.size() / (double)T;
+        size_t next = 0;
+        for (size_t k = 0; k < sel.size(); k++) {
+            if (k == (size_t)(next * step) && keep.size() < T) {
+                keep.push_back(sel[k]);
+                keep_slot.push_back(sel_slot[k]);
+                next++;
+            } else {
+                rest.push_back(sel[k]);
+                rest_slot.push_back(sel_slot[k]);
             }
-        for (const RadQuery &a : hq)
-            for (int d = 0; d < 3; d++) {
-                lo[d] = std::min(lo[d], (float)a.p[d]);
-                hi[d] = std::max(hi[d], (float)a.p[d]);
-            }
-        float ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
-        ext = std::max(ext * 1.02f, 1e-6f);
-        for (int a = 0; a < 3; a++) lo[a] -= 0.01f * ext;
-        const float inv = 1.0f / ext;
-        const uint32_t R0 = (uint32_t)c.recs.size();
-        if ((r = c.upload(stream)) || (r = lx.reserve((size_t)std::max(R0, n) * 24)) ||
-            (r = lt.reserve((size_t)R0 * 12)) || (r = ly.reserve((size_t)n * 12)) ||
-            (r = lidx.reserve((size_t)n * 4)))
-            return r;
-        k_rec_to_sample<<<blocks(R0), 256, 0, stream>>>(static_cast<const RadRec *>(c.d_recs.p), R0, lo[0], lo[1],
+        }
+        sel.swap(keep);
+    }
+    // (2) probes: up to probes_per_tile of the other pending points per tile
+    std::vector<std::vector<uint32_t>> by_tile((size_t)tx * ty); // positions in rest
+    for (size_t k = 0; k < rest.size(); k++) by_tile[tile_of(rest[k])].push_back((uint32_t)k);
+    std::vector<uint8_t> is_probe(rest.size(), 0);
+    std::vector<uint32_t> probes, probe_tile;
+    for (uint32_t t = 0; t < by_tile.size(); t++) {
+        const auto &v = by_tile[t];
+        const uint32_t cnt = std::min<uint32_t>(rs.probes_per_tile, (uint32_t)v.size());
+        for (uint32_t j = 0; j < cnt; j++) {
+            uint32_t k = v[(size_t)((j + 0.5) * v.size() / cnt)];
+            if (is_probe[k]) continue;
+            is_probe[k] = 1;
+            probes.push_back(rest[k]);
+            probe_tile.push_back(t);
+        }
+    }
+    std::vector<uint32_t> rest2, rest2_slot;
+    for (size_t k = 0; k < rest.size(); k++)
+        if (!is_probe[k]) {
+            rest2.push_back(rest[k]);
+            rest2_slot.push_back(rest_slot[k]);
+        }
+    // one round of plain gathers for both
+    const size_t first_new = c.recs.size();
+    std::vector<uint32_t> round = sel;
+    round.insert(round.end(), probes.begin(), probes.end());
+    if ((r = gather_round(0, q, round, true, acc, E))) return r;
+    const std::vector<int32_t> filed = last_rec[0];
+    const uint32_t m = (uint32_t)probes.size();
+    std::vector<RadRec> prec(m); // the probes' gathers (standard error in pad[0])
+    if (m && (r = fetch(prec.data(), static_cast<RadRec *>(scratch[0].rec.p) + sel.size(), m * sizeof(RadRec))))
+        return r;
+    stats.probes += m;
+
+    // (3) train on the pretrace records and the training gathers
+    std::vector<uint32_t> train_idx;
+    for (size_t i = 0; i < first_new; i++) train_idx.push_back((uint32_t)i);
+    for (size_t k = 0; k < sel.size(); k++)
+        if (filed[k] >= 0) train_idx.push_back((uint32_t)filed[k]);
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (const RadRec &b : c.recs)
+        for (int a = 0; a < 3; a++) {
+            lo[a] = std::min(lo[a], b.p[a]);
+            hi[a] = std::max(hi[a], b.p[a]);
+        }
+    for (const RadQuery &a : hq)
+        for (int d = 0; d < 3; d++) {
+            lo[d] = std::min(lo[d], (float)a.p[d]);
+            hi[d] = std::max(hi[d], (float)a.p[d]);
+        }
+    float ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+    ext = std::max(ext * 1.02f, 1e-6f);
+    for (int a = 0; a < 3; a++) lo[a] -= 0.01f * ext;
+    const float inv = 1.0f / ext;
+    const uint32_t R0 = (uint32_t)train_idx.size();
+    if ((r = lx.reserve((size_t)std::max(R0, n) * 24)) || (r = lt.reserve((size_t)std::max<uint32_t>(R0, 1) * 12)) ||
+        (r = ly.reserve((size_t)n * 12)) || (r = lidx.reserve((size_t)std::max(R0, n) * 4)))
+        return r;
+    if (R0 > 0) {
+        if ((r = send(lidx.p, train_idx.data(), R0 * 4))) return r;
+        k_rec_to_sample<<<blocks(R0), 256, 0, stream>>>(static_cast<const RadRec *>(c.d_recs.p),
+                                                        static_cast<const uint32_t *>(lidx.p), R0, lo[0], lo[1],
                                                         lo[2], inv, static_cast<float *>(lx.p),
                                                         static_cast<float *>(lt.p));
         HIP_TRY(hipGetLastError());
-        hipEvent_t t0, t1;
-        HIP_TRY(hipEventCreate(&t0));
-        HIP_TRY(hipEventCreate(&t1));
-        HIP_TRY(hipEventRecord(t0, stream));
-        float loss = 0.0f;
-        const uint32_t batch = 2048; // samples drawn with replacement from the R0 records
-        if ((r = net.init(nullptr, 0xC0FFEEull, stream)) ||
-            (r = net.train(static_cast<const float *>(lx.p), static_cast<const float *>(lt.p), R0, rs.train_steps,
-                           batch, 5e-3f, &loss, stream)))
-            return r;
-        HIP_TRY(hipEventRecord(t1, stream));
-        HIP_TRY(hipEventSynchronize(t1));
-        float ms = 0;
-        HIP_TRY(hipEventElapsedTime(&ms, t0, t1));
-        (void)hipEventDestroy(t0);
-        (void)hipEventDestroy(t1);
-        train_ms += ms;
-        stats.train_loss = loss;
+    }
+    hipEvent_t t0, t1;
+    HIP_TRY(hipEventCreate(&t0));
+    HIP_TRY(hipEventCreate(&t1));
+    HIP_TRY(hipEventRecord(t0, stream));
+    float loss = 0.0f;
+    const uint32_t batch = 2048; // samples drawn with replacement from the R0 records
+    if ((r = net.init(nullptr, 0xC0FFEEull, stream))) return r;
+    if (R0 > 0 && (r = net.train(static_cast<const float *>(lx.p), static_cast<const float *>(lt.p), R0,
+                                 rs.train_steps, batch, 5e-3f, &loss, stream)))
+        return r;
+    HIP_TRY(hipEventRecord(t1, stream));
+    HIP_TRY(hipEventSynchronize(t1));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, t0, t1));
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    train_ms += ms;
+    stats.train_loss = loss;
 
-        // (3) probes: up to probes_per_tile pending points per image tile
-        const uint32_t tsz = std::max<uint32_t>(1, rs.probe_tile);
-        const uint32_t tx = (cam.width + tsz - 1) / tsz;
-        auto tile_of = [&](uint32_t i) {
-            uint32_t p = pix[i];
-            return (p % cam.width) / tsz + ((p / cam.width) / tsz) * tx;
-        };
-        std::vector<std::vector<uint32_t>> by_tile; // positions in todo
-        std::vector<uint32_t> tiles_used;
-        {
-            std::vector<int32_t> tmap((size_t)tx * ((cam.height + tsz - 1) / tsz), -1);
-            for (size_t k = 0; k < todo.size(); k++) {
-                uint32_t t = tile_of(todo[k]);
-                if (tmap[t] < 0) {
-                    tmap[t] = (int32_t)by_tile.size();
-                    by_tile.emplace_back();
-                    tiles_used.push_back(t);
-                }
-                by_tile[tmap[t]].push_back((uint32_t)k);
-            }
+    // (4) probe test per tile: grey irradiance difference beyond twice the
+    // probe gather's standard error, in displayed colour (the query weight
+    // of a primary hit is its diffuse reflectance, grey(pigment) * opacity
+    // * diffuse, so w * |error| is what the surrogate would change there)
+    std::vector<uint8_t> fb_tile(by_tile.size(), 0);
+    if (m) {
+        if ((r = send(lidx.p, probes.data(), m * 4))) return r;
+        k_query_to_x6<<<blocks(m), 256, 0, stream>>>(q, static_cast<const uint32_t *>(lidx.p), m, lo[0], lo[1], lo[2],
+                                                     inv, static_cast<float *>(lx.p));
+        if ((r = net.infer(static_cast<const float *>(lx.p), m, static_cast<float *>(ly.p), stream))) return r;
+        std::vector<float> y(3 * (size_t)m);
+        if ((r = fetch(y.data(), ly.p, y.size() * 4))) return r;
+        std::vector<double> err(by_tile.size(), 0.0);
+        std::vector<uint32_t> cnt(by_tile.size(), 0);
+        std::vector<float> wq(n, 0.0f);
+        for (size_t k = 0; k < slot.size(); k++) wq[todo[k]] = hq[slot[k]].w;
+        const double gw[3] = {0.297, 0.589, 0.114};
+        double tot = 0;
+        for (uint32_t k = 0; k < m; k++) {
+            double dg = 0;
+            for (int a = 0; a < 3; a++) dg += gw[a] * ((double)std::max(y[3 * (size_t)k + a], 0.0f) - prec[k].E[a]);
+            double e = std::max(0.0, std::fabs(dg) - 2.0 * prec[k].pad[0]) * wq[probes[k]] * rs.brightness;
+            err[probe_tile[k]] += e;
+            cnt[probe_tile[k]]++;
+            tot += e;
         }
-        std::vector<uint8_t> is_probe(todo.size(), 0);
-        std::vector<uint32_t> probes, probe_group;
-        const uint32_t P = rs.probes_per_tile;
-        for (size_t gi = 0; gi < by_tile.size(); gi++) {
-            const auto &v = by_tile[gi];
-            uint32_t cnt = std::min<uint32_t>(P, (uint32_t)v.size());
-            for (uint32_t j = 0; j < cnt; j++) {
-                uint32_t k = v[(size_t)((j + 0.5) * v.size() / cnt)];
-                if (is_probe[k]) continue;
-                is_probe[k] = 1;
-                probes.push_back(todo[k]);
-                probe_group.push_back((uint32_t)gi);
-            }
+        stats.probe_error = (float)(tot / m);
+        for (size_t t = 0; t < by_tile.size(); t++) fb_tile[t] = cnt[t] && err[t] / cnt[t] > rs.probe_threshold;
+    }
+    // the round's records serve the other pending points
+    if ((r = relookup(0, q, first_new, true, acc, rest2, rest2_slot, todo, slot))) return r;
+
+    // (5) fallback tiles: plain gathers; the rest: the network
+    std::vector<uint32_t> fb, fb_slot, ml, ml_slot;
+    for (size_t k = 0; k < todo.size(); k++) {
+        if (fb_tile[tile_of(todo[k])]) {
+            fb.push_back(todo[k]);
+            fb_slot.push_back(slot[k]);
+        } else {
+            ml.push_back(todo[k]);
+            ml_slot.push_back(slot[k]);
         }
-        std::vector<uint8_t> fallback(by_tile.size(), 0);
-        if (!probes.empty()) {
-            size_t first_new = c.recs.size();
-            if ((r = gather_round(0, q, probes, true, acc, E))) return r;
-            stats.probes += probes.size();
-            const uint32_t m = (uint32_t)probes.size();
-            HIP_TRY(hipMemcpyAsync(lidx.p, probes.data(), m * 4, hipMemcpyHostToDevice, stream));
-            k_query_to_x6<<<blocks(m), 256, 0, stream>>>(q, static_cast<const uint32_t *>(lidx.p), m, lo[0], lo[1],
-                                                         lo[2], inv, static_cast<float *>(lx.p));
-            if ((r = net.infer(static_cast<const float *>(lx.p), m, static_cast<float *>(ly.p), stream))) return r;
-            std::vector<float> y(3 * (size_t)m), Eall(3 * (size_t)n);
-            std::vector<RadRec> prec(m); // the probes' gathers, in probe order
-            HIP_TRY(hipMemcpyAsync(prec.data(), scratch[0].rec.p, m * sizeof(RadRec), hipMemcpyDeviceToHost, stream));
-            HIP_TRY(hipMemcpyAsync(y.data(), ly.p, y.size() * 4, hipMemcpyDeviceToHost, stream));
-            HIP_TRY(hipMemcpyAsync(Eall.data(), E, Eall.size() * 4, hipMemcpyDeviceToHost, stream));
-            HIP_TRY(sync());
-            // probe error in displayed colour: the query weight of a primary
-            // hit is its diffuse reflectance (grey(pigment) * opacity * diffuse),
-            // so w * |irradiance error| is what the surrogate would change there
-            std::vector<float> wq(n, 0.0f);
-            for (size_t k = 0; k < todo_all.size(); k++) wq[todo_all[k]] = hq[k].w;
-            std::vector<double> err(by_tile.size(), 0.0);
-            std::vector<uint32_t> cnt(by_tile.size(), 0);
-            double tot = 0;
-            for (uint32_t k = 0; k < m; k++) {
-                // grey irradiance difference beyond twice the gather's
-                // standard error, in displayed colour
-                double dg = 0;
-                const double gw[3] = {0.297, 0.589, 0.114};
-                for (int a = 0; a < 3; a++)
-                    dg += gw[a] * ((double)std::max(y[3 * (size_t)k + a], 0.0f) -
-                                   Eall[3 * (size_t)probes[k] + a]);
-                double e = std::max(0.0, std::fabs(dg) - 2.0 * prec[k].pad[0]) * wq[probes[k]] * rs.brightness;
-                err[probe_group[k]] += e;
-                cnt[probe_group[k]]++;
-                tot += e;
-            }
-            stats.probe_error = (float)(tot / m);
-            for (size_t gi = 0; gi < by_tile.size(); gi++)
-                fallback[gi] = cnt[gi] && err[gi] / cnt[gi] > rs.probe_threshold;
-            // the probes' records serve their neighbours too
-            std::vector<uint32_t> rest, rest_slot;
-            for (size_t k = 0; k < todo.size(); k++)
-                if (!is_probe[k]) {
-                    rest.push_back(todo[k]);
-                    rest_slot.push_back(slot[k]);
-                }
-            if ((r = relookup(0, q, first_new, true, acc, rest, rest_slot, todo, slot))) return r;
-        }
-        // (4) fallback tiles: plain gathers; the rest: the network
-        std::vector<uint8_t> fb_tile((size_t)tx * ((cam.height + tsz - 1) / tsz), 0);
-        for (size_t gi = 0; gi < by_tile.size(); gi++) fb_tile[tiles_used[gi]] = fallback[gi];
-        std::vector<uint32_t> fb, fb_slot, ml, ml_slot;
-        for (size_t k = 0; k < todo.size(); k++) {
-            if (fb_tile[tile_of(todo[k])]) {
-                fb.push_back(todo[k]);
-                fb_slot.push_back(slot[k]);
-            } else {
-                ml.push_back(todo[k]);
-                ml_slot.push_back(slot[k]);
-            }
-        }
-        stats.fallbacks += fb.size();
-        if (!fb.empty()) {
-            size_t first_new = c.recs.size();
-            if ((r = gather_rounds(0, q, true, fb, fb_slot, hq, acc, E))) return r;
-            if ((r = relookup(0, q, first_new, true, acc, ml, ml_slot, todo, slot))) return r;
-            ml.swap(todo);
-        }
-        stats.surrogate_queries += ml.size();
-        if (!ml.empty()) {
-            const uint32_t m = (uint32_t)ml.size();
-            HIP_TRY(hipMemcpyAsync(lidx.p, ml.data(), m * 4, hipMemcpyHostToDevice, stream));
-            k_query_to_x6<<<blocks(m), 256, 0, stream>>>(q, static_cast<const uint32_t *>(lidx.p), m, lo[0], lo[1],
-                                                         lo[2], inv, static_cast<float *>(lx.p));
-            if ((r = net.infer(static_cast<const float *>(lx.p), m, static_cast<float *>(ly.p), stream))) return r;
-            k_apply_pred<<<blocks(m), 256, 0, stream>>>(static_cast<const uint32_t *>(lidx.p), m,
-                                                        static_cast<const float *>(ly.p), acc, E);
-            HIP_TRY(hipGetLastError());
-        }
+    }
+    stats.fallbacks += fb.size();
+    if (!fb.empty()) {
+        const size_t first_fb = c.recs.size();
+        if ((r = gather_rounds(0, q, true, fb, fb_slot, hq, acc, E, 2))) return r;
+        if ((r = relookup(0, q, first_fb, true, acc, ml, ml_slot, todo, slot))) return r;
+        ml.swap(todo);
+    }
+    stats.surrogate_queries += ml.size();
+    if (!ml.empty()) {
+        const uint32_t mm = (uint32_t)ml.size();
+        if ((r = send(lidx.p, ml.data(), mm * 4))) return r;
+        k_query_to_x6<<<blocks(mm), 256, 0, stream>>>(q, static_cast<const uint32_t *>(lidx.p), mm, lo[0], lo[1],
+                                                      lo[2], inv, static_cast<float *>(lx.p));
+        if ((r = net.infer(static_cast<const float *>(lx.p), mm, static_cast<float *>(ly.p), stream))) return r;
+        k_apply_pred<<<blocks(mm), 256, 0, stream>>>(static_cast<const uint32_t *>(lidx.p), mm,
+                                                     static_cast<const float *>(ly.p), acc, E);
+        HIP_TRY(hipGetLastError());
     }
     k_finish<<<blocks(n), 256, 0, stream>>>(acc, n, rs.brightness, E);
     HIP_TRY(hipGetLastError());
@@ -1101,7 +1172,7 @@ int RadiosityEngine::pixel_pass(const std::vector<uint32_t> &px, bool final, flo
         (r = S.subq.reserve(n * sizeof(RadQuery))) || (r = S.subE.reserve(n * 12)) ||
         (r = S.rec.reserve(n * 4)))
         return r;
-    HIP_TRY(hipMemcpyAsync(S.sel.p, px.data(), n * 4, hipMemcpyHostToDevice, stream));
+    if ((r = send(S.sel.p, px.data(), n * 4))) return r;
     k_pixel_rays<<<blocks(n), 256, 0, stream>>>(cam, static_cast<const uint32_t *>(S.sel.p), n,
                                                 static_cast<pvx_ray *>(S.rays.p));
     HIP_TRY(hipGetLastError());
@@ -1121,8 +1192,7 @@ int RadiosityEngine::pixel_pass(const std::vector<uint32_t> &px, bool final, flo
                                                  static_cast<uint8_t *>(S.need.p));
     HIP_TRY(hipGetLastError());
     std::vector<uint8_t> need(n);
-    HIP_TRY(hipMemcpyAsync(need.data(), S.need.p, n, hipMemcpyDeviceToHost, stream));
-    HIP_TRY(sync());
+    if ((r = fetch(need.data(), S.need.p, n))) return r;
     std::vector<uint32_t> sel, qpix(n, 0);
     for (uint32_t i = 0; i < n; i++)
         if (need[i]) {
@@ -1131,7 +1201,7 @@ int RadiosityEngine::pixel_pass(const std::vector<uint32_t> &px, bool final, flo
         }
     if (!sel.empty()) {
         const uint32_t m = (uint32_t)sel.size();
-        HIP_TRY(hipMemcpyAsync(S.sel.p, sel.data(), m * 4, hipMemcpyHostToDevice, stream));
+        if ((r = send(S.sel.p, sel.data(), m * 4))) return r;
         k_gather_subq<<<blocks(m), 256, 0, stream>>>(static_cast<const RadQuery *>(S.q.p),
                                                      static_cast<const uint32_t *>(S.sel.p), m,
                                                      static_cast<RadQuery *>(S.subq.p));
@@ -1148,7 +1218,7 @@ int RadiosityEngine::pixel_pass(const std::vector<uint32_t> &px, bool final, flo
         }
     }
     if (final) {
-        HIP_TRY(hipMemcpyAsync(S.rec.p, qpix.data(), n * 4, hipMemcpyHostToDevice, stream));
+        if ((r = send(S.rec.p, qpix.data(), n * 4))) return r;
         k_final_colour<<<blocks(n), 256, 0, stream>>>(
             static_cast<const float *>(S.base.p), static_cast<const float *>(S.coef.p),
             static_cast<const float *>(S.subE.p), static_cast<const uint32_t *>(S.rec.p), n,
@@ -1185,9 +1255,15 @@ extern "C" int pvx_radiosity_render(pvx_scene *s, const pvx_camera *cam, const p
         if (m.reflective)
             return fail(PVX_EUNSUPPORTED, "radiosity frames take plain diffuse / emissive finishes only");
     hipStream_t st = static_cast<hipStream_t>(stream);
-    RadiosityEngine eng(s, *set, *cam, st);
+    if (!s->rad_engine) {
+        RadiosityEngine *e = new (std::nothrow) RadiosityEngine(s);
+        if (!e) return fail(PVX_ENOMEM, "host allocation failed");
+        s->rad_engine = e;
+        s->rad_engine_free = [](void *p) { delete static_cast<RadiosityEngine *>(p); };
+    }
+    RadiosityEngine &eng = *static_cast<RadiosityEngine *>(s->rad_engine);
     int r;
-    if ((r = eng.init())) return r;
+    if ((r = eng.begin(*set, *cam, st))) return r;
     hipEvent_t e0, e1, e2;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));

@@ -12,6 +12,8 @@
 
 namespace pvx {
 
+struct RadiosityEngine;
+
 constexpr int kMaxRadDepth = 4;
 constexpr int kMaxResolveRounds = 32; // gather rounds per resolve before the rest goes at once
 
@@ -55,7 +57,7 @@ struct RadGrid {
     uint32_t n = 0; // records indexed
     std::vector<uint32_t> start, items;
     DevBuf d_start, d_items;
-    int build(const std::vector<RadRec> &recs, size_t first, size_t last, float reach, hipStream_t st);
+    int build(const std::vector<RadRec> &recs, size_t first, size_t last, float reach, RadiosityEngine &eng);
 };
 
 // Records of one recursion depth.  `all` indexes every record; `fresh`
@@ -67,15 +69,15 @@ struct RadCache {
     float reach = 1.0f;   // largest quick-check radius / harmonic mean radius
     DevBuf d_recs;
     RadGrid all, fresh;
-    int upload(hipStream_t st);
-    int index_all(hipStream_t st);
+    int upload(RadiosityEngine &eng);
+    int index_all(RadiosityEngine &eng);
 };
 
 struct RadiosityEngine {
     pvx_scene *scene;
-    pvx_radiosity_settings rs;
-    pvx_camera cam;
-    hipStream_t stream;
+    pvx_radiosity_settings rs{};
+    pvx_camera cam{};
+    hipStream_t stream = nullptr;
     RadCache caches[kMaxRadDepth];
     uint32_t cursor[kMaxRadDepth];
     std::vector<double> pool;
@@ -84,21 +86,30 @@ struct RadiosityEngine {
     uint64_t stats_rounds[kMaxRadDepth] = {0, 0, 0, 0};
 
     struct Scratch { // per gather depth
-        DevBuf rays, hits, base, coef, amb, q, need, rec, rec_ok, sel, subq, subE;
+        DevBuf rays, hits, base, coef, amb, q, need, rec, sel, subq, subE;
     } scratch[kMaxRadDepth];
     struct Resolve { // per resolve depth
         DevBuf acc, need, idx, Eg, tq;
     } rstate[kMaxRadDepth];
     Scratch prim; // primary rays of a pixel pass
+    std::vector<int32_t> last_rec[kMaxRadDepth]; // cache index of each query of the last gather (-1: not filed)
     Surrogate net; // learned mode
     DevBuf lx, lt, ly, lidx;
     double train_ms = 0.0;
     double t_sync_ms = 0.0, t_grid_ms = 0.0; // host-side waits and grid builds
     uint64_t n_sync = 0;
+    HostBuf bounce; // pinned staging for device -> host copies
     hipError_t sync();
+    int fetch(void *host, const void *dev, size_t bytes);
 
-    RadiosityEngine(pvx_scene *s, const pvx_radiosity_settings &set, const pvx_camera &c, hipStream_t st);
-    int init();
+    HostBuf up;            // pinned staging for host -> device copies (reused after each wait)
+    size_t up_off = 0;
+    int send(void *dev, const void *host, size_t bytes);
+
+    // The engine lives with its scene so that device buffers survive from
+    // frame to frame; begin() clears the per-frame state.
+    explicit RadiosityEngine(pvx_scene *s);
+    int begin(const pvx_radiosity_settings &set, const pvx_camera &c, hipStream_t st);
     int rays_per_sample(int depth) const;
     float reuse_count(int depth) const;
     int lookup(int depth, const RadGrid &g, const RadQuery *q, const uint32_t *idx, uint32_t n, bool final,
@@ -116,7 +127,8 @@ struct RadiosityEngine {
                  const std::vector<uint32_t> &rest, const std::vector<uint32_t> &rest_slot,
                  std::vector<uint32_t> &todo, std::vector<uint32_t> &slot);
     int gather_rounds(int depth, const RadQuery *q, bool final, std::vector<uint32_t> todo,
-                      std::vector<uint32_t> slot, const std::vector<RadQuery> &hq, RadAcc *acc, float *E);
+                      std::vector<uint32_t> slot, const std::vector<RadQuery> &hq, RadAcc *acc, float *E,
+                      int max_rounds = kMaxResolveRounds);
     int resolve_learned(const RadQuery *q, uint32_t n, const std::vector<uint32_t> &pix, float *E);
     int pixel_pass(const std::vector<uint32_t> &px, bool final, float *rgb);
 };

@@ -1,17 +1,20 @@
-// Wavefront shading of opaque plain-finish surfaces (gfx950).
+// Wavefront shading of plain-finish surfaces (gfx950).
 //
 // Per bounce: one closest-hit launch over the live rays, one shade launch
-// (normal, classic ambient, per-light diffuse/Phong terms, shadow-ray and
-// reflection-ray emission through atomic queues), one shadow-ray launch and
-// one finish launch (light visibility -> colour).  A path has at most one
-// live ray per pixel (reflection only), so pixels are accumulated without
-// atomics and the frame is deterministic.
+// (normal, classic ambient, per-light diffuse/Phong terms and shadow rays,
+// the refracted / reflected rays of the next bounce), shadow launches that
+// follow each light ray through transparent objects, one finish launch
+// (light colour -> surface colour) and one accumulation launch.  Every ray
+// writes its own contribution; the queue stays ordered by pixel (children
+// are placed by a prefix sum in parent order), so each pixel's rays are
+// contiguous and are summed in a fixed order: the frame is deterministic.
 //
 // Follows Trace::TraceRay / ComputeLightedTexture / ComputeOneDiffuseLight /
 // TracePointLightShadowRay / TraceAreaLightSubsetShadowRay /
-// ComputeDiffuseColour / ComputePhongColour / ComputeReflectivity /
-// ComputeReflection (core/render/trace.cpp:135-228, 739-1179, 1264-1321,
-// 1637-1729, 1951-2272, 2441-2558, 2627-2656).
+// ComputeShadowColour / ComputeDiffuseColour / ComputePhongColour /
+// ComputeReflectivity / ComputeReflection / ComputeRefraction /
+// TraceRefractionRay (core/render/trace.cpp:135-228, 739-1179, 1181-1262,
+// 1264-1483, 1637-1729, 1951-2420, 2441-2558, 2627-2656).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -28,19 +31,33 @@ namespace {
 
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr double kShadowTolerance = 1.0e-3; // trace.cpp:81
+constexpr int kMaxInteriors = 10;           // objects a ray can be inside at once
+constexpr int kMaxShadowSteps = 64;         // transparent layers followed per light ray
+
+// Ray type bits, as Ray::SetFlags(type, parent) sets them (ray.cpp:155-166).
+enum : uint8_t { kFPrimary = 1, kFRefl = 2, kFRefr = 4, kFContinued = 8 };
+
+__host__ __device__ inline uint8_t ray_kind(uint8_t f) { // Ray::IsImageRay etc. (ray.h:99-106)
+    uint8_t k = 0;
+    if ((f & kFPrimary) || ((f & kFRefr) && !(f & kFRefl))) k |= PVX_RAY_IMAGE;
+    if (f & kFRefl) k |= PVX_RAY_REFLECTION;
+    return k;
+}
 
 struct PathState {
     uint32_t pixel;
-    uint32_t level; // trace level before this ray's TraceRay
-    float thr[3];   // product of reflectivities along the path
+    uint16_t level; // trace level when this ray's TraceRay starts
+    uint8_t flags;  // kF* bits
+    uint8_t n_in;   // interiors entered
+    float thr[3];   // product of reflectivities / filter colours along the path
     float weight;   // ADC weight
+    int32_t ins[kMaxInteriors];
 };
+static_assert(sizeof(PathState) == 64, "PathState is one 64-byte line");
 
 struct ShadeOut {
-    uint32_t pixel;
-    uint32_t obj;
+    uint32_t obj; // kNone: nothing for the finish pass
     float thr[3];
-    float pad;
 };
 
 struct RenderParams : NormalParams {
@@ -61,50 +78,73 @@ __global__ void k_zero_counters(uint32_t *c) {
     if (threadIdx.x < 4) c[threadIdx.x] = 0;
 }
 
-__global__ void k_init_paths(const pvx_ray *__restrict__ cam_rays, uint32_t n, PathState *st) {
+__global__ void k_init_paths(uint32_t n, PathState *st, uint8_t *kinds) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    st[i] = PathState{i, 0u, {1.0f, 1.0f, 1.0f}, 1.0f};
+    PathState ps;
+    ps.pixel = i;
+    ps.level = 0;
+    ps.flags = kFPrimary;
+    ps.n_in = 0;
+    ps.thr[0] = ps.thr[1] = ps.thr[2] = 1.0f;
+    ps.weight = 1.0f;
+    for (int k = 0; k < kMaxInteriors; k++) ps.ins[k] = -1;
+    st[i] = ps;
+    kinds[i] = ray_kind(kFPrimary);
+}
+
+__device__ inline void write_child(pvx_ray *crays, PathState *cst, size_t slot, D3 o, D3 d, const PathState &ps,
+                                   const float *thr, float weight, uint8_t flags, uint16_t level) {
+    pvx_ray r;
+    r.o[0] = o.x; r.o[1] = o.y; r.o[2] = o.z;
+    r.d[0] = d.x; r.d[1] = d.y; r.d[2] = d.z;
+    crays[slot] = r;
+    PathState c = ps;
+    c.thr[0] = thr[0];
+    c.thr[1] = thr[1];
+    c.thr[2] = thr[2];
+    c.weight = weight;
+    c.flags = flags;
+    c.level = level;
+    cst[slot] = c;
 }
 
 // Shade the hits of one bounce: ambient now, per-light terms and shadow rays
-// for the finish pass, and the reflected ray of the next bounce.
-__global__ void k_shade(RenderParams p, const pvx_ray *__restrict__ rays,
-                        const PathState *__restrict__ st, const pvx_hit *__restrict__ hits,
-                        uint32_t n, uint32_t ray_kind, float *__restrict__ rgb,
-                        ShadeOut *__restrict__ shade, uint32_t *__restrict__ lslot,
-                        double *__restrict__ lterm, pvx_ray *__restrict__ srays,
-                        double *__restrict__ sdepth, pvx_ray *__restrict__ next_rays,
-                        PathState *__restrict__ next_st, uint32_t *__restrict__ counters) {
+// for the finish pass, and up to two rays of the next bounce (refracted or
+// totally reflected first, reflected second) in slots 2i, 2i+1.
+__global__ void k_shade(RenderParams p, const pvx_ray *__restrict__ rays, const PathState *__restrict__ st,
+                        const pvx_hit *__restrict__ hits, uint32_t n, float *__restrict__ contrib,
+                        ShadeOut *__restrict__ shade, uint32_t *__restrict__ lslot, double *__restrict__ lterm,
+                        pvx_ray *__restrict__ srays, double *__restrict__ sdepth, uint32_t *__restrict__ sowner,
+                        float *__restrict__ scol, pvx_ray *__restrict__ crays, PathState *__restrict__ cst,
+                        uint32_t *__restrict__ nchild, uint32_t *__restrict__ counters) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const pvx_ray r = rays[i];
     const PathState ps = st[i];
     const pvx_hit h = hits[i];
-    float *px = rgb + 3 * (size_t)ps.pixel;
+    float *ct = contrib + 3 * (size_t)i;
     ShadeOut so;
-    so.pixel = kNone;
+    so.obj = kNone;
+    nchild[i] = 0;
     if (h.obj < 0) { // ComputeSky: background colour
-        px[0] += ps.thr[0] * p.bg[0];
-        px[1] += ps.thr[1] * p.bg[1];
-        px[2] += ps.thr[2] * p.bg[2];
+        ct[0] = ps.thr[0] * p.bg[0];
+        ct[1] = ps.thr[1] * p.bg[1];
+        ct[2] = ps.thr[2] * p.bg[2];
         shade[i] = so;
         return;
     }
+    // TraceRay raises the trace level unless the ray continues a transmission
+    const uint16_t lev = (uint16_t)(ps.level + ((ps.flags & kFContinued) ? 0 : 1));
     D3 o{r.o[0], r.o[1], r.o[2]}, d{r.d[0], r.d[1], r.d[2]};
     D3 ip = add3(o, mul3(d, h.t));
     D3 nrm = hit_normal(p, h, o, d, ip);
     if (dot3(nrm, d) > 0.0) nrm = D3{-nrm.x, -nrm.y, -nrm.z};
     const MatRec m = p.mats[h.obj];
     // classic ambient: pigment * (ambient * ambient_light) * opacity
-    {
-        float a0 = (float)((double)m.att * (double)(m.pigment[0] * m.emission[0]));
-        float a1 = (float)((double)m.att * (double)(m.pigment[1] * m.emission[1]));
-        float a2 = (float)((double)m.att * (double)(m.pigment[2] * m.emission[2]));
-        px[0] += ps.thr[0] * a0;
-        px[1] += ps.thr[1] * a1;
-        px[2] += ps.thr[2] * a2;
-    }
+    ct[0] = ps.thr[0] * (float)((double)m.att * (double)(m.pigment[0] * m.emission[0]));
+    ct[1] = ps.thr[1] * (float)((double)m.att * (double)(m.pigment[1] * m.emission[1]));
+    ct[2] = ps.thr[2] * (float)((double)m.att * (double)(m.pigment[2] * m.emission[2]));
     for (uint32_t L = 0; L < p.n_lights; L++) {
         const LightRec &lt = p.lights[L];
         D3 C{lt.pos[0], lt.pos[1], lt.pos[2]};
@@ -120,7 +160,7 @@ __global__ void k_shade(RenderParams p, const pvx_ray *__restrict__ rays,
         double inten = (m.brilliance != 1.0f) ? pow(fabs(cosi), (double)m.brilliance) : fabs(cosi);
         inten *= (double)m.diffuse * (double)m.att;
         double ph = 0.0;
-        if (m.phong > 0.0f && ray_kind != PVX_RAY_RADIOSITY) {
+        if (m.phong > 0.0f) {
             double c2 = -2.0 * dot3(d, nrm);
             D3 rd = add3(d, mul3(nrm, c2));
             double ca = dot3(rd, dir);
@@ -151,16 +191,86 @@ __global__ void k_shade(RenderParams p, const pvx_ray *__restrict__ rays,
                 uint32_t k = base + u * lt.n2 + v;
                 srays[k] = sr;
                 sdepth[k] = sl;
+                sowner[k] = k;
+                scol[3 * (size_t)k] = scol[3 * (size_t)k + 1] = scol[3 * (size_t)k + 2] = 1.0f;
             }
     }
-    so.pixel = ps.pixel;
     so.obj = (uint32_t)h.obj;
     so.thr[0] = ps.thr[0];
     so.thr[1] = ps.thr[1];
     so.thr[2] = ps.thr[2];
     shade[i] = so;
-    // reflection (ComputeReflectivity without fresnel, ComputeReflection)
-    if (m.reflective & 1u) {
+
+    uint32_t nc = 0;
+    bool tir = false;
+    // transmitted component (ComputeLightedTexture, trace.cpp:1075-1131)
+    if (m.trans > p.bailout) {
+        const float nw = (float)((double)ps.weight * (double)m.fmax);
+        // ComputeRefraction (trace.cpp:1323-1452): ior ratio from the interiors
+        PathState c = ps;
+        double ior;
+        const double own = (double)m.ior;
+        if (c.n_in == 0) {
+            c.ins[c.n_in++] = h.obj;
+            ior = 1.0 / own;
+        } else if (c.ins[c.n_in - 1] == h.obj) {
+            c.n_in--;
+            ior = c.n_in == 0 ? own : own / (double)p.mats[c.ins[c.n_in - 1]].ior;
+        } else {
+            int at = -1;
+            for (int k = 0; k < c.n_in; k++)
+                if (c.ins[k] == h.obj) {
+                    at = k;
+                    break;
+                }
+            if (at >= 0) { // leaving an overlap
+                for (int k = at; k + 1 < c.n_in; k++) c.ins[k] = c.ins[k + 1];
+                c.n_in--;
+                ior = 1.0;
+            } else {
+                ior = (double)p.mats[c.ins[c.n_in - 1]].ior / own;
+                if (c.n_in < kMaxInteriors) c.ins[c.n_in++] = h.obj;
+            }
+        }
+        const uint8_t rf = (uint8_t)(kFRefr | (ps.flags & kFRefl));
+        const size_t slot = 2 * (size_t)i;
+        if (fabs(ior - 1.0) < kEpsilon) { // only transmit the ray (a continued ray)
+            float thr[3] = {ps.thr[0] * m.fil[0], ps.thr[1] * m.fil[1], ps.thr[2] * m.fil[2]};
+            if (lev < p.max_level && nw >= p.bailout) {
+                write_child(crays, cst, slot, ip, d, c, thr, nw, (uint8_t)(rf | kFContinued), lev);
+                nc = 1;
+            }
+        } else {
+            double nn = dot3(d, nrm);
+            D3 ln = nrm;
+            if (nn <= 0.0)
+                nn = -nn;
+            else
+                ln = D3{-nrm.x, -nrm.y, -nrm.z};
+            double t = 1.0 + ior * ior * (nn * nn - 1.0);
+            if (t < 0.0) { // total internal reflection: a reflected ray, unfiltered
+                tir = true;
+                double k2 = -2.0 * dot3(d, nrm);
+                D3 nd = normalize3(add3(d, mul3(nrm, k2)));
+                if (lev < p.max_level && nw >= p.bailout) {
+                    write_child(crays, cst, slot, ip, nd, ps, ps.thr, nw,
+                                (uint8_t)(kFRefl | (ps.flags & kFRefr)), lev);
+                    nc = 1;
+                }
+            } else {
+                t = ior * nn - sqrt(t);
+                D3 nd = add3(mul3(d, ior), mul3(ln, t));
+                float thr[3] = {ps.thr[0] * m.fil[0], ps.thr[1] * m.fil[1], ps.thr[2] * m.fil[2]};
+                if (lev < p.max_level && nw >= p.bailout) {
+                    write_child(crays, cst, slot, ip, nd, c, thr, nw, rf, lev);
+                    nc = 1;
+                }
+            }
+        }
+    }
+    // reflection (ComputeReflectivity without fresnel, ComputeReflection);
+    // skipped after total internal reflection (trace.cpp:1141-1147)
+    if ((m.reflective & 1u) && !tir) {
         double cos_inc = -dot3(d, nrm);
         double w = (double)ps.weight * (double)m.rweight;
         double frac = fabs((double)m.falloff - 1.0) > kEpsilon ? pow(1.0 - cos_inc, (double)m.falloff)
@@ -175,24 +285,47 @@ __global__ void k_shade(RenderParams p, const pvx_ray *__restrict__ rays,
                 rf[k] = (float)(frac * m.rmax[k]) + (float)((1.0 - frac) * m.rmin[k]);
         }
         bool nonzero = rf[0] != 0.0f || rf[1] != 0.0f || rf[2] != 0.0f;
-        if (nonzero && ps.level + 1 < p.max_level && (float)w >= p.bailout) {
+        if (nonzero && lev < p.max_level && (float)w >= p.bailout) {
             double k2 = -2.0 * dot3(d, nrm);
             D3 nd = normalize3(add3(d, mul3(nrm, k2)));
-            uint32_t slot = atomicAdd(&counters[0], 1u);
-            pvx_ray nr;
-            nr.o[0] = ip.x; nr.o[1] = ip.y; nr.o[2] = ip.z;
-            nr.d[0] = nd.x; nr.d[1] = nd.y; nr.d[2] = nd.z;
-            next_rays[slot] = nr;
-            PathState ns;
-            ns.pixel = ps.pixel;
-            ns.level = ps.level + 1;
-            ns.thr[0] = ps.thr[0] * rf[0];
-            ns.thr[1] = ps.thr[1] * rf[1];
-            ns.thr[2] = ps.thr[2] * rf[2];
-            ns.weight = (float)w;
-            next_st[slot] = ns;
+            float thr[3] = {ps.thr[0] * rf[0], ps.thr[1] * rf[1], ps.thr[2] * rf[2]};
+            write_child(crays, cst, 2 * (size_t)i + nc, ip, nd, ps, thr, (float)w,
+                        (uint8_t)(kFRefl | (ps.flags & kFRefr)), lev);
+            nc++;
         }
     }
+    nchild[i] = nc;
+}
+
+// One step of TracePointLightShadowRay's loop (trace.cpp:2021-2073): a light
+// ray blocked by a transparent object is filtered by it and continues from
+// the hit point; an opaque blocker ends it.
+__global__ void k_shadow_step(RenderParams p, const pvx_ray *__restrict__ srays, const double *__restrict__ sdepth,
+                              const uint32_t *__restrict__ sowner, const pvx_hit *__restrict__ shits, uint32_t n,
+                              float *__restrict__ scol, pvx_ray *__restrict__ nrays, double *__restrict__ ndepth,
+                              uint32_t *__restrict__ nowner, uint32_t *__restrict__ counter) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const pvx_hit h = shits[k];
+    const double lsd = sdepth[k];
+    if (!(h.obj >= 0 && h.t < lsd - kShadowTolerance && h.t > kShadowTolerance)) return;
+    const uint32_t ow = sowner[k];
+    const MatRec &m = p.mats[h.obj];
+    float *c = scol + 3 * (size_t)ow;
+    if (m.opaque) {
+        c[0] = c[1] = c[2] = 0.0f;
+        return;
+    }
+    c[0] *= m.fil[0];
+    c[1] *= m.fil[1];
+    c[2] *= m.fil[2];
+    const pvx_ray r = srays[k];
+    pvx_ray nr = r;
+    for (int a = 0; a < 3; a++) nr.o[a] = r.o[a] + r.d[a] * h.t;
+    uint32_t slot = atomicAdd(counter, 1u);
+    nrays[slot] = nr;
+    ndepth[slot] = lsd - h.t;
+    nowner[slot] = ow;
 }
 
 // TraceAreaLightSubsetShadowRay's quadtree average with adaptive 100 (every
@@ -215,14 +348,14 @@ __device__ float area_average(const float *vis, int n2, int u1, int v1, int u2, 
 
 constexpr int kMaxAreaGrid = 16; // per axis
 
+// Light colour -> surface colour for each shaded hit, added to its contribution.
 __global__ void k_finish(RenderParams p, const ShadeOut *__restrict__ shade, uint32_t n,
                          const uint32_t *__restrict__ lslot, const double *__restrict__ lterm,
-                         const pvx_hit *__restrict__ shits, const double *__restrict__ sdepth,
-                         float *__restrict__ rgb) {
+                         const float *__restrict__ scol, float *__restrict__ contrib) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const ShadeOut so = shade[i];
-    if (so.pixel == kNone) return;
+    if (so.obj == kNone) return;
     const MatRec &m = p.mats[so.obj];
     float col[3] = {0.0f, 0.0f, 0.0f};
     for (uint32_t L = 0; L < p.n_lights; L++) {
@@ -230,23 +363,17 @@ __global__ void k_finish(RenderParams p, const ShadeOut *__restrict__ shade, uin
         uint32_t base = lslot[li];
         if (base == kNone) continue;
         const LightRec &lt = p.lights[L];
-        float f;
+        float lc[3];
         uint32_t ns = lt.n1 * lt.n2;
         if (ns == 1) {
-            const pvx_hit sh = shits[base];
-            bool blocked = sh.obj >= 0 && sh.t < sdepth[base] - kShadowTolerance && sh.t > kShadowTolerance;
-            f = blocked ? 0.0f : 1.0f;
+            for (int c = 0; c < 3; c++) lc[c] = lt.colour[c] * scol[3 * (size_t)base + c];
         } else {
             float vis[kMaxAreaGrid * kMaxAreaGrid];
-            for (uint32_t k = 0; k < ns; k++) {
-                const pvx_hit sh = shits[base + k];
-                bool blocked = sh.obj >= 0 && sh.t < sdepth[base + k] - kShadowTolerance &&
-                               sh.t > kShadowTolerance;
-                vis[k] = blocked ? 0.0f : 1.0f;
+            for (int c = 0; c < 3; c++) {
+                for (uint32_t k = 0; k < ns; k++) vis[k] = lt.colour[c] * scol[3 * (size_t)(base + k) + c];
+                lc[c] = area_average<5>(vis, (int)lt.n2, 0, 0, (int)lt.n1 - 1, (int)lt.n2 - 1);
             }
-            f = area_average<5>(vis, (int)lt.n2, 0, 0, (int)lt.n1 - 1, (int)lt.n2 - 1);
         }
-        float lc[3] = {lt.colour[0] * f, lt.colour[1] * f, lt.colour[2] * f};
         if (fabsf(lc[0]) < 1e-10f && fabsf(lc[1]) < 1e-10f && fabsf(lc[2]) < 1e-10f) continue;
         double inten = lterm[2 * li], ph = lterm[2 * li + 1];
         for (int k = 0; k < 3; k++) {
@@ -255,23 +382,90 @@ __global__ void k_finish(RenderParams p, const ShadeOut *__restrict__ shade, uin
         if (ph != 0.0)
             for (int k = 0; k < 3; k++) col[k] = col[k] + (float)(ph * (double)lc[k]);
     }
-    float *px = rgb + 3 * (size_t)so.pixel;
-    px[0] += so.thr[0] * col[0];
-    px[1] += so.thr[1] * col[1];
-    px[2] += so.thr[2] * col[2];
+    float *ct = contrib + 3 * (size_t)i;
+    ct[0] += so.thr[0] * col[0];
+    ct[1] += so.thr[1] * col[1];
+    ct[2] += so.thr[2] * col[2];
 }
 
-template <class T>
-int grow(pvx_scene *s, void *&ptr, size_t bytes) {
-    if (ptr) {
-        for (auto &a : s->render_allocs)
-            if (a == ptr) a = nullptr;
-        (void)hipFree(ptr);
-        ptr = nullptr;
+// Adds each pixel's contributions in queue order (the queue is ordered by
+// pixel, so a pixel's rays form one run).
+__global__ void k_accumulate(const PathState *__restrict__ st, const float *__restrict__ contrib, uint32_t n,
+                             float *__restrict__ rgb) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t px = st[i].pixel;
+    if (i > 0 && st[i - 1].pixel == px) return;
+    float c[3] = {0.0f, 0.0f, 0.0f};
+    for (uint32_t j = i; j < n && st[j].pixel == px; j++)
+        for (int a = 0; a < 3; a++) c[a] += contrib[3 * (size_t)j + a];
+    float *o = rgb + 3 * (size_t)px;
+    o[0] += c[0];
+    o[1] += c[1];
+    o[2] += c[2];
+}
+
+// Exclusive prefix sum of child counts: per-block scan (1024 per block) ...
+constexpr int kScanBlock = 1024;
+__global__ __launch_bounds__(kScanBlock) void k_scan_block(const uint32_t *__restrict__ in, uint32_t n,
+                                                           uint32_t *__restrict__ out, uint32_t *__restrict__ sums) {
+    __shared__ uint32_t s[kScanBlock];
+    const uint32_t t = threadIdx.x, i = blockIdx.x * kScanBlock + t;
+    uint32_t v = i < n ? in[i] : 0u;
+    s[t] = v;
+    __syncthreads();
+    for (int off = 1; off < kScanBlock; off <<= 1) {
+        uint32_t a = t >= (uint32_t)off ? s[t - off] : 0u;
+        __syncthreads();
+        s[t] += a;
+        __syncthreads();
     }
-    HIP_TRY(hipMalloc(&ptr, bytes));
-    s->render_allocs.push_back(ptr);
-    return PVX_OK;
+    if (i < n) out[i] = s[t] - v;
+    if (t == kScanBlock - 1) sums[blockIdx.x] = s[t];
+}
+// ... scan of the block sums (one block, up to 1024 x 8 sums) ...
+__global__ __launch_bounds__(kScanBlock) void k_scan_sums(uint32_t *__restrict__ sums, uint32_t nb,
+                                                          uint32_t *__restrict__ total) {
+    __shared__ uint32_t s[kScanBlock];
+    const uint32_t t = threadIdx.x;
+    constexpr int kPer = 8;
+    uint32_t loc[kPer], acc = 0;
+    for (int k = 0; k < kPer; k++) {
+        uint32_t idx = t * kPer + k;
+        loc[k] = acc;
+        acc += idx < nb ? sums[idx] : 0u;
+    }
+    s[t] = acc;
+    __syncthreads();
+    for (int off = 1; off < kScanBlock; off <<= 1) {
+        uint32_t a = t >= (uint32_t)off ? s[t - off] : 0u;
+        __syncthreads();
+        s[t] += a;
+        __syncthreads();
+    }
+    const uint32_t before = s[t] - acc;
+    for (int k = 0; k < kPer; k++) {
+        uint32_t idx = t * kPer + k;
+        if (idx < nb) sums[idx] = before + loc[k];
+    }
+    if (t == kScanBlock - 1) *total = s[t];
+}
+// ... then every child is copied to its place in the next queue.
+__global__ void k_emit(const uint32_t *__restrict__ nchild, const uint32_t *__restrict__ offs,
+                       const uint32_t *__restrict__ sums, uint32_t n, const pvx_ray *__restrict__ crays,
+                       const PathState *__restrict__ cst, pvx_ray *__restrict__ next_rays,
+                       PathState *__restrict__ next_st, uint8_t *__restrict__ next_kinds) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t c = nchild[i];
+    if (!c) return;
+    const uint32_t o = offs[i] + sums[i / kScanBlock];
+    for (uint32_t j = 0; j < c; j++) {
+        next_rays[o + j] = crays[2 * (size_t)i + j];
+        const PathState ps = cst[2 * (size_t)i + j];
+        next_st[o + j] = ps;
+        next_kinds[o + j] = ray_kind(ps.flags);
+    }
 }
 
 } // namespace
@@ -305,7 +499,14 @@ extern "C" int pvx_scene_set_shading(pvx_scene *s, const pvx_shading_desc *d) {
         m.rweight = std::max(wmax, wmin);
         m.reflective = (wmax != 0.0f || wmin != 0.0f ||
                         a.reflection_min[0] != 0.0f || a.reflection_max[0] != 0.0f) ? 1u : 0u;
-        if (a.filter != 0.0f || a.transmit != 0.0f || a.ior != 0.0f) m.reflective |= 2u;
+        // TransColour::TransmittedColour (base/colour.h:1912-1915) and the
+        // remaining translucency of ComputeLightedTexture (trace.cpp:1068)
+        for (int k = 0; k < 3; k++) m.fil[k] = a.pigment[k] * a.filter + a.transmit;
+        m.trans = std::min(1.0f, std::fabs(0.297f * m.fil[0] + 0.589f * m.fil[1] + 0.114f * m.fil[2]));
+        m.fmax = std::max(std::fabs(m.fil[0]), std::max(std::fabs(m.fil[1]), std::fabs(m.fil[2])));
+        m.ior = a.ior > 0.0f ? a.ior : 1.0f; // every object has an interior (parser.cpp:9226-9230)
+        m.opaque = (a.filter == 0.0f && a.transmit == 0.0f) ? 1u : 0u;
+        if (m.trans > 0.0f) m.reflective |= 2u;
     }
     sh.lights.resize(d->n_lights);
     sh.samples_per_hit = 0;
@@ -348,53 +549,40 @@ extern "C" int pvx_render(pvx_scene *s, const pvx_camera *cam, float *rgb, pvx_r
     if (!s || !cam || !rgb) return fail(PVX_EINVAL, "null argument");
     ShadingState &sh = s->shading;
     if (!sh.set) return fail(PVX_ESTATE, "pvx_scene_set_shading was not called");
-    for (const MatRec &m : sh.mats)
-        if (m.reflective & 2u)
-            return fail(PVX_EUNSUPPORTED, "transparent / refractive materials are not rendered yet");
     if (cam->width == 0 || cam->height == 0 || cam->height > 65535)
         return fail(PVX_EINVAL, "bad image size");
     hipStream_t st = static_cast<hipStream_t>(stream);
     size_t npix = (size_t)cam->width * cam->height;
-    if (npix > 0xFFFFFFF0u) return fail(PVX_EINVAL, "image too large");
+    if (npix > 0x3FFFFFFFu) return fail(PVX_EINVAL, "image too large");
     RenderBuffers &b = s->rb;
-    size_t nl = std::max<size_t>(1, sh.lights.size());
-    size_t nshadow = npix * std::max<uint32_t>(1, sh.samples_per_hit);
-    if (b.cap_rays < npix) {
-        int r;
-        if ((r = grow<char>(s, b.rays[0], npix * sizeof(pvx_ray))) ||
-            (r = grow<char>(s, b.rays[1], npix * sizeof(pvx_ray))) ||
-            (r = grow<char>(s, b.state[0], npix * sizeof(PathState))) ||
-            (r = grow<char>(s, b.state[1], npix * sizeof(PathState))) ||
-            (r = grow<char>(s, b.hits, npix * sizeof(pvx_hit))))
-            return r;
-        b.cap_rays = npix;
-    }
-    size_t shade_bytes = npix * (sizeof(ShadeOut) + nl * (sizeof(uint32_t) + 2 * sizeof(double))) + 64;
-    if (b.cap_shade < shade_bytes) {
-        int r;
-        if ((r = grow<char>(s, b.shade, shade_bytes))) return r;
-        b.cap_shade = shade_bytes;
-    }
-    if (b.cap_shadow < nshadow) {
-        int r;
-        if ((r = grow<char>(s, b.srays, nshadow * sizeof(pvx_ray))) ||
-            (r = grow<char>(s, b.sdepth, nshadow * sizeof(double))) ||
-            (r = grow<char>(s, b.shits, nshadow * sizeof(pvx_hit))))
-            return r;
-        b.cap_shadow = nshadow;
-    }
-    if (!b.counters) {
-        void *c = nullptr;
-        HIP_TRY(hipMalloc(&c, 16));
-        s->render_allocs.push_back(c);
-        b.counters = static_cast<uint32_t *>(c);
-    }
-    // the per-light arrays live behind the ShadeOut records (re-derived each
-    // frame because the light count may change between frames)
-    ShadeOut *shade = static_cast<ShadeOut *>(b.shade);
-    uint32_t *lslot = reinterpret_cast<uint32_t *>(shade + npix);
-    double *lterm = reinterpret_cast<double *>(
-        reinterpret_cast<char *>(lslot) + ((npix * nl * sizeof(uint32_t) + 7) & ~size_t(7)));
+    const size_t nl = std::max<size_t>(1, sh.lights.size());
+    const size_t spp = std::max<uint32_t>(1, sh.samples_per_hit);
+    int r;
+    // queue buffers sized for `cap` rays; grown when a bounce needs more
+    auto ensure_queue = [&](size_t cap) -> int {
+        int rr;
+        if ((rr = b.rays[0].reserve(cap * sizeof(pvx_ray))) || (rr = b.rays[1].reserve(cap * sizeof(pvx_ray))) ||
+            (rr = b.state[0].reserve(cap * sizeof(PathState))) || (rr = b.state[1].reserve(cap * sizeof(PathState))) ||
+            (rr = b.kinds[0].reserve(cap)) || (rr = b.kinds[1].reserve(cap)) ||
+            (rr = b.hits.reserve(cap * sizeof(pvx_hit))) || (rr = b.contrib.reserve(cap * 12)) ||
+            (rr = b.shade.reserve(cap * sizeof(ShadeOut))) || (rr = b.lslot.reserve(cap * nl * 4)) ||
+            (rr = b.lterm.reserve(cap * nl * 16)) || (rr = b.crays.reserve(2 * cap * sizeof(pvx_ray))) ||
+            (rr = b.cstate.reserve(2 * cap * sizeof(PathState))) || (rr = b.nchild.reserve(cap * 4)) ||
+            (rr = b.offs.reserve(cap * 4)) || (rr = b.sums.reserve((cap / kScanBlock + 2) * 4)) ||
+            (rr = b.counters.reserve(64)))
+            return rr;
+        return PVX_OK;
+    };
+    auto ensure_shadow = [&](size_t ns) -> int {
+        int rr;
+        for (int k = 0; k < 2; k++)
+            if ((rr = b.srays[k].reserve(ns * sizeof(pvx_ray))) || (rr = b.sdepth[k].reserve(ns * 8)) ||
+                (rr = b.sowner[k].reserve(ns * 4)))
+                return rr;
+        if ((rr = b.shits.reserve(ns * sizeof(pvx_hit))) || (rr = b.scol.reserve(ns * 12))) return rr;
+        return PVX_OK;
+    };
+    if ((r = ensure_queue(npix)) || (r = ensure_shadow(npix * spp))) return r;
 
     RenderParams p;
     p.sv = s->view;
@@ -420,54 +608,126 @@ extern "C" int pvx_render(pvx_scene *s, const pvx_camera *cam, float *rgb, pvx_r
         size_t n3 = 3 * npix;
         k_fill<<<(unsigned)((n3 + 255) / 256), 256, 0, st>>>(rgb, n3, 0.0f);
     }
-    int rc = pvx_camera_rays(cam, static_cast<pvx_ray *>(b.rays[0]), st);
+    int rc = pvx_camera_rays(cam, static_cast<pvx_ray *>(b.rays[0].p), st);
     if (rc) return rc;
     uint32_t n = (uint32_t)npix;
-    k_init_paths<<<(n + 255) / 256, 256, 0, st>>>(static_cast<pvx_ray *>(b.rays[0]), n,
-                                                   static_cast<PathState *>(b.state[0]));
+    k_init_paths<<<(n + 255) / 256, 256, 0, st>>>(n, static_cast<PathState *>(b.state[0].p),
+                                                   static_cast<uint8_t *>(b.kinds[0].p));
     int cur = 0;
-    uint32_t kind = PVX_RAY_IMAGE;
-    while (n > 0 && bounces < sh.max_level) {
-        const pvx_ray *rays = static_cast<const pvx_ray *>(b.rays[cur]);
+    // every bounce raises the trace level of all but continued rays; the cap
+    // only guards against pathological chains of continued rays
+    const uint32_t max_bounces = 4 * std::max<uint32_t>(1, sh.max_level) + 16;
+    while (n > 0 && bounces < max_bounces) {
+        const pvx_ray *rays = static_cast<const pvx_ray *>(b.rays[cur].p);
+        const PathState *pst = static_cast<const PathState *>(b.state[cur].p);
         HIP_TRY(hipEventRecord(t0, st));
-        rc = trace_closest_internal(s, rays, nullptr, kind, n, static_cast<pvx_hit *>(b.hits), st);
+        rc = trace_closest_internal(s, rays, nullptr, bounces == 0 ? (uint32_t)PVX_RAY_IMAGE : (uint32_t)PVX_RAY_REFLECTION,
+                                    n, static_cast<pvx_hit *>(b.hits.p), st,
+                                    bounces == 0 ? nullptr : static_cast<const uint8_t *>(b.kinds[cur].p));
         if (rc) return rc;
         HIP_TRY(hipEventRecord(t1, st));
         n_closest += n;
-        k_zero_counters<<<1, 64, 0, st>>>(b.counters);
+        uint32_t *counters = static_cast<uint32_t *>(b.counters.p);
+        k_zero_counters<<<1, 64, 0, st>>>(counters);
         k_shade<<<(n + 127) / 128, 128, 0, st>>>(
-            p, rays, static_cast<const PathState *>(b.state[cur]), static_cast<const pvx_hit *>(b.hits),
-            n, kind, rgb, shade, lslot, lterm, static_cast<pvx_ray *>(b.srays),
-            static_cast<double *>(b.sdepth), static_cast<pvx_ray *>(b.rays[1 - cur]),
-            static_cast<PathState *>(b.state[1 - cur]), b.counters);
+            p, rays, pst, static_cast<const pvx_hit *>(b.hits.p), n, static_cast<float *>(b.contrib.p),
+            static_cast<ShadeOut *>(b.shade.p), static_cast<uint32_t *>(b.lslot.p), static_cast<double *>(b.lterm.p),
+            static_cast<pvx_ray *>(b.srays[0].p), static_cast<double *>(b.sdepth[0].p),
+            static_cast<uint32_t *>(b.sowner[0].p), static_cast<float *>(b.scol.p),
+            static_cast<pvx_ray *>(b.crays.p), static_cast<PathState *>(b.cstate.p),
+            static_cast<uint32_t *>(b.nchild.p), counters);
         HIP_TRY(hipGetLastError());
-        uint32_t cnt[2];
-        HIP_TRY(hipMemcpyAsync(cnt, b.counters, sizeof cnt, hipMemcpyDeviceToHost, st));
+        // next queue size: exclusive scan of the child counts
+        const uint32_t nb = (n + kScanBlock - 1) / kScanBlock;
+        if (nb > (uint32_t)kScanBlock * 8) return fail(PVX_EUNSUPPORTED, "ray queue above 8M rays");
+        k_scan_block<<<nb, kScanBlock, 0, st>>>(static_cast<const uint32_t *>(b.nchild.p), n,
+                                                static_cast<uint32_t *>(b.offs.p), static_cast<uint32_t *>(b.sums.p));
+        k_scan_sums<<<1, kScanBlock, 0, st>>>(static_cast<uint32_t *>(b.sums.p), nb, counters + 2);
+        HIP_TRY(hipGetLastError());
+        uint32_t cnt[3];
+        HIP_TRY(hipMemcpyAsync(cnt, counters, sizeof cnt, hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
         float ms = 0.0f;
         HIP_TRY(hipEventElapsedTime(&ms, t0, t1));
         trace_ms += ms;
-        if (cnt[1] > 0) {
+        // light rays, followed through transparent objects
+        uint32_t ns = cnt[1];
+        int sq = 0;
+        for (int step = 0; ns > 0 && step < kMaxShadowSteps; step++) {
             HIP_TRY(hipEventRecord(t0, st));
-            rc = trace_closest_internal(s, static_cast<const pvx_ray *>(b.srays),
-                                        static_cast<const double *>(b.sdepth), PVX_RAY_SHADOW, cnt[1],
-                                        static_cast<pvx_hit *>(b.shits), st);
+            rc = trace_closest_internal(s, static_cast<const pvx_ray *>(b.srays[sq].p),
+                                        static_cast<const double *>(b.sdepth[sq].p), PVX_RAY_SHADOW, ns,
+                                        static_cast<pvx_hit *>(b.shits.p), st);
             if (rc) return rc;
             HIP_TRY(hipEventRecord(t1, st));
-            n_shadow += cnt[1];
-        }
-        k_finish<<<(n + 127) / 128, 128, 0, st>>>(p, shade, n, lslot, lterm,
-                                                  static_cast<const pvx_hit *>(b.shits),
-                                                  static_cast<const double *>(b.sdepth), rgb);
-        HIP_TRY(hipGetLastError());
-        if (cnt[1] > 0) {
-            HIP_TRY(hipEventSynchronize(t1));
+            n_shadow += ns;
+            k_zero_counters<<<1, 64, 0, st>>>(counters);
+            k_shadow_step<<<(ns + 255) / 256, 256, 0, st>>>(
+                p, static_cast<const pvx_ray *>(b.srays[sq].p), static_cast<const double *>(b.sdepth[sq].p),
+                static_cast<const uint32_t *>(b.sowner[sq].p), static_cast<const pvx_hit *>(b.shits.p), ns,
+                static_cast<float *>(b.scol.p), static_cast<pvx_ray *>(b.srays[1 - sq].p),
+                static_cast<double *>(b.sdepth[1 - sq].p), static_cast<uint32_t *>(b.sowner[1 - sq].p), counters);
+            HIP_TRY(hipGetLastError());
+            uint32_t c2;
+            HIP_TRY(hipMemcpyAsync(&c2, counters, 4, hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipStreamSynchronize(st));
             HIP_TRY(hipEventElapsedTime(&ms, t0, t1));
             shadow_ms += ms;
+            ns = c2;
+            sq = 1 - sq;
         }
-        n = cnt[0];
+        k_finish<<<(n + 127) / 128, 128, 0, st>>>(p, static_cast<const ShadeOut *>(b.shade.p), n,
+                                                  static_cast<const uint32_t *>(b.lslot.p),
+                                                  static_cast<const double *>(b.lterm.p),
+                                                  static_cast<const float *>(b.scol.p),
+                                                  static_cast<float *>(b.contrib.p));
+        k_accumulate<<<(n + 255) / 256, 256, 0, st>>>(pst, static_cast<const float *>(b.contrib.p), n, rgb);
+        HIP_TRY(hipGetLastError());
+        const uint32_t next = cnt[2];
+        if (next > 0) {
+            if (next > b.rays[1 - cur].cap / sizeof(pvx_ray) || next > b.nchild.cap / 4) {
+                // grow the queue (keeping this bounce's children, which live in crays/cstate)
+                HIP_TRY(hipStreamSynchronize(st));
+                DevBuf keep_r, keep_s;
+                std::swap(keep_r.p, b.crays.p);
+                std::swap(keep_r.cap, b.crays.cap);
+                std::swap(keep_s.p, b.cstate.p);
+                std::swap(keep_s.cap, b.cstate.cap);
+                DevBuf keep_n, keep_o, keep_u;
+                std::swap(keep_n.p, b.nchild.p);
+                std::swap(keep_n.cap, b.nchild.cap);
+                std::swap(keep_o.p, b.offs.p);
+                std::swap(keep_o.cap, b.offs.cap);
+                std::swap(keep_u.p, b.sums.p);
+                std::swap(keep_u.cap, b.sums.cap);
+                // the current queue must survive too
+                DevBuf cur_r, cur_s, cur_k;
+                std::swap(cur_r.p, b.rays[cur].p);
+                std::swap(cur_r.cap, b.rays[cur].cap);
+                std::swap(cur_s.p, b.state[cur].p);
+                std::swap(cur_s.cap, b.state[cur].cap);
+                std::swap(cur_k.p, b.kinds[cur].p);
+                std::swap(cur_k.cap, b.kinds[cur].cap);
+                if ((r = ensure_queue((size_t)next + next / 2)) || (r = ensure_shadow(((size_t)next + next / 2) * spp)))
+                    return r;
+                k_emit<<<(n + 255) / 256, 256, 0, st>>>(
+                    static_cast<const uint32_t *>(keep_n.p), static_cast<const uint32_t *>(keep_o.p),
+                    static_cast<const uint32_t *>(keep_u.p), n, static_cast<const pvx_ray *>(keep_r.p),
+                    static_cast<const PathState *>(keep_s.p), static_cast<pvx_ray *>(b.rays[1 - cur].p),
+                    static_cast<PathState *>(b.state[1 - cur].p), static_cast<uint8_t *>(b.kinds[1 - cur].p));
+                HIP_TRY(hipGetLastError());
+                HIP_TRY(hipStreamSynchronize(st));
+            } else {
+                k_emit<<<(n + 255) / 256, 256, 0, st>>>(
+                    static_cast<const uint32_t *>(b.nchild.p), static_cast<const uint32_t *>(b.offs.p),
+                    static_cast<const uint32_t *>(b.sums.p), n, static_cast<const pvx_ray *>(b.crays.p),
+                    static_cast<const PathState *>(b.cstate.p), static_cast<pvx_ray *>(b.rays[1 - cur].p),
+                    static_cast<PathState *>(b.state[1 - cur].p), static_cast<uint8_t *>(b.kinds[1 - cur].p));
+                HIP_TRY(hipGetLastError());
+            }
+        }
+        n = next;
         cur = 1 - cur;
-        kind = PVX_RAY_REFLECTION;
         bounces++;
     }
     HIP_TRY(hipEventRecord(e1, st));

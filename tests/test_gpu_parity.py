@@ -148,10 +148,27 @@ def test_render_mesh_area_lights(gpu):
     assert abs(st["shadow_rays"] - n_shadow) <= 1e-4 * n_shadow
 
 
-def test_render_is_deterministic(gpu):
-    s = scenes.sphere_scene(width=160, height=120)
+def test_render_csg_glass_refraction(gpu):
+    """Half of the CSG trees are glass (transmit 0.9, ior 1.5, reflection
+    0.1): refraction through interior lists, total internal reflection and
+    light filtered through transparent occluders (max_trace_level 10)."""
+    s = scenes.csg_scene(n_objects=2000, width=480, height=270)
     gs = gpu.GpuScene(s)
     gs.set_shading()
-    a, _ = gs.render()
-    b, _ = gs.render()
-    assert np.array_equal(a, b)
+    img, st = gs.render()
+    ref, (n_closest, n_shadow) = OracleScene(s).render(s)
+    print(f"[csg glass] gpu rays {st['closest_rays']} + {st['shadow_rays']} shadow, bounces "
+          f"{st['bounces']}; oracle {n_closest} + {n_shadow}", flush=True)
+    _image_parity(img, ref, "csg2000 glass 480x270")
+    assert n_closest > 1.5 * 480 * 270  # the frame does branch into refracted rays
+    assert abs(st["closest_rays"] - n_closest) <= 1e-3 * n_closest
+    assert abs(st["shadow_rays"] - n_shadow) <= 1e-3 * n_shadow
+
+
+def test_render_is_deterministic(gpu):
+    for s in (scenes.sphere_scene(width=160, height=120), scenes.csg_scene(n_objects=800, width=160, height=90)):
+        gs = gpu.GpuScene(s)
+        gs.set_shading()
+        a, _ = gs.render()
+        b, _ = gs.render()
+        assert np.array_equal(a, b)
