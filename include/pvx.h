@@ -279,7 +279,7 @@ typedef struct pvx_material {
     float reflection_min[3]; /* `reflection r` sets min = max = r            */
     float reflection_max[3];
     float reflection_falloff;/* default 1.0                                  */
-    float ior;               /* interior ior; 0 = no interior                */
+    float ior;               /* interior ior (default 1.0; <= 0 means 1.0)    */
 } pvx_material;
 
 enum { PVX_LIGHT_POINT = 1, PVX_LIGHT_AREA = 2 };
@@ -325,12 +325,15 @@ int pvx_scene_set_shading(pvx_scene *scene, const pvx_shading_desc *desc);
 /* Render one frame, 1 sample per pixel at the pixel centre, as
  * TracePixel -> Trace::TraceRay -> ComputeTextureColour ->
  * ComputeLightedTexture (core/render/trace.cpp:135-228, 457-586, 739-1179)
- * for opaque plain finishes: classic ambient, diffuse and Phong per light
- * (ComputeOneDiffuseLight, trace.cpp:1637-1729) with shadow rays
- * (TracePointLightShadowRay / TraceAreaLightShadowRay, trace.cpp:1951-2272),
- * reflection rays (ComputeReflection, trace.cpp:1264-1321) up to
- * max_trace_level, ADC bailout.  Runs as a wavefront: camera rays, then per
- * bounce one closest-hit launch, one shade launch, one shadow-ray launch.
+ * for plain finishes: classic ambient, diffuse and Phong per light
+ * (ComputeOneDiffuseLight, trace.cpp:1637-1729) with shadow rays filtered by
+ * transparent occluders (TracePointLightShadowRay / TraceAreaLightShadowRay /
+ * ComputeShadowColour, trace.cpp:1951-2420), reflection (ComputeReflection,
+ * trace.cpp:1264-1321) and refraction through the ray's interior list with
+ * total internal reflection (ComputeRefraction / TraceRefractionRay,
+ * trace.cpp:1323-1483) up to max_trace_level, ADC bailout.  Runs as a
+ * wavefront: camera rays, then per bounce one closest-hit launch, one shade
+ * launch, light-ray launches and the placement of the next bounce's rays.
  * rgb: device buffer of 3*width*height floats (linear RGB, row-major). */
 int pvx_render(pvx_scene *scene, const pvx_camera *cam, float *rgb, pvx_render_stats *stats,
                void *stream);

@@ -628,6 +628,7 @@ extern "C" int pvx_render(pvx_scene *s, const pvx_camera *cam, float *rgb, pvx_r
         HIP_TRY(hipEventRecord(t1, st));
         n_closest += n;
         uint32_t *counters = static_cast<uint32_t *>(b.counters.p);
+        if ((r = ensure_shadow((size_t)n * spp))) return r; // light rays of this bounce
         k_zero_counters<<<1, 64, 0, st>>>(counters);
         k_shade<<<(n + 127) / 128, 128, 0, st>>>(
             p, rays, pst, static_cast<const pvx_hit *>(b.hits.p), n, static_cast<float *>(b.contrib.p),
