@@ -338,6 +338,16 @@ int pvx_scene_set_shading(pvx_scene *scene, const pvx_shading_desc *desc);
 int pvx_render(pvx_scene *scene, const pvx_camera *cam, float *rgb, pvx_render_stats *stats,
                void *stream);
 
+/* Supersampled frame: n_samples rays per pixel at the given offsets from the
+ * pixel corner (host array, 2 doubles each; cam->jitter_* is ignored), the
+ * pixel being the mean of its samples.  Non-adaptive supersampling with
+ * threshold 0 and no jitter (TraceTask::NonAdaptiveSupersamplingM1 /
+ * SupersampleOnePixel, backend/render/tracetask.cpp:521-600, 838-890) is the
+ * centre plus an aa_depth x aa_depth grid: offsets (0.5, 0.5) and
+ * 0.5 + (-range .. range step 1/aa_depth), range = 0.5 - 0.5/aa_depth. */
+int pvx_render_samples(pvx_scene *scene, const pvx_camera *cam, uint32_t n_samples, const double *offsets,
+                       float *rgb, pvx_render_stats *stats, void *stream);
+
 /* ---- radiosity ---------------------------------------------------------- */
 enum { PVX_RADIOSITY_PLAIN = 0, PVX_RADIOSITY_LEARNED = 1 };
 

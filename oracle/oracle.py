@@ -38,7 +38,7 @@ def lib():
         L.orc_mesh_normal.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_size_t,
                                       C.c_void_p]
         L.orc_render.argtypes = [C.c_void_p, C.POINTER(A.PvxShadingDesc), C.POINTER(A.PvxCamera),
-                                 C.c_void_p, C.c_int, C.c_void_p, C.c_int]
+                                 C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p]
         L.orc_radiosity_render.argtypes = [C.c_void_p, C.POINTER(A.PvxShadingDesc),
                                            C.POINTER(A.PvxCamera),
                                            C.POINTER(A.PvxRadiositySettings), C.c_void_p,
@@ -112,14 +112,20 @@ class OracleScene:
         lib().orc_mesh_triangle(self.h, mesh, tri, _p(out))
         return out
 
-    def render(self, scene, width=None, height=None, threads=os.cpu_count() or 1, method=1):
+    def render(self, scene, width=None, height=None, threads=os.cpu_count() or 1, method=1, offsets=None):
         """Plain-finish TraceRay over the camera; returns (h, w, 3) float32
-        linear RGB and (closest rays, shadow rays) counters."""
+        linear RGB and (closest rays, shadow rays) counters.  `offsets`
+        ((S, 2) sub-pixel positions) averages S samples per pixel."""
         sd, keep = scene.shading()
         cam = scene.camera_struct(width=width, height=height)
         out = np.zeros((cam.height, cam.width, 3), np.float32)
         cnt = np.zeros(2, np.uint64)
-        lib().orc_render(self.h, C.byref(sd), C.byref(cam), _p(out), threads, _p(cnt), method)
+        if offsets is None:
+            ns, op = 0, None
+        else:
+            offsets = np.ascontiguousarray(offsets, np.float64).reshape(-1, 2)
+            ns, op = offsets.shape[0], _p(offsets)
+        lib().orc_render(self.h, C.byref(sd), C.byref(cam), _p(out), threads, _p(cnt), method, ns, op)
         del keep
         return out, (int(cnt[0]), int(cnt[1]))
 

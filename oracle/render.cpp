@@ -307,8 +307,10 @@ V3 oracle_shape_normal(const Scene &s, uint32_t obj, const Isect &h, V3 ip) {
 
 using namespace orc;
 
+// n_samples rays per pixel at `offsets` (from the pixel corner); the pixel is
+// their sum divided by the count (SupersampleOnePixel, tracetask.cpp:861-890).
 extern "C" int orc_render(void *sp, const pvx_shading_desc *sd, const pvx_camera *cam, float *rgb,
-                          int threads, uint64_t *counters, int method) {
+                          int threads, uint64_t *counters, int method, int n_samples, const double *offsets) {
     const Scene &s = *static_cast<Scene *>(sp);
     Shading sh;
     sh.method = method;
@@ -327,13 +329,20 @@ extern "C" int orc_render(void *sp, const pvx_shading_desc *sd, const pvx_camera
             if (b >= npix) break;
             for (size_t i = b; i < std::min(npix, b + 64); i++) {
                 uint32_t x = (uint32_t)(i % cam->width), y = (uint32_t)(i / cam->width);
-                double x0 = ((double)x + cam->jitter_x) / (double)cam->width - 0.5;
-                double y0 = 0.5 - ((double)y + cam->jitter_y) / (double)cam->height;
-                V3 d;
-                for (int k = 0; k < 3; k++)
-                    d[k] = cam->direction[k] + x0 * cam->right[k] + y0 * cam->up[k];
-                Ray r{V3(cam->location[0], cam->location[1], cam->location[2]), normalized(d)};
-                RGB c = trace_ray(s, sh, cnt, r, 1.0f, 0, RayFlags{true, false, false}, Interiors{}, false);
+                RGB c;
+                const int S = n_samples > 0 ? n_samples : 1;
+                for (int smp = 0; smp < S; smp++) {
+                    double ox = n_samples > 0 ? offsets[2 * smp] : cam->jitter_x;
+                    double oy = n_samples > 0 ? offsets[2 * smp + 1] : cam->jitter_y;
+                    double x0 = ((double)x + ox) / (double)cam->width - 0.5;
+                    double y0 = 0.5 - ((double)y + oy) / (double)cam->height;
+                    V3 d;
+                    for (int k = 0; k < 3; k++)
+                        d[k] = cam->direction[k] + x0 * cam->right[k] + y0 * cam->up[k];
+                    Ray r{V3(cam->location[0], cam->location[1], cam->location[2]), normalized(d)};
+                    c = c + trace_ray(s, sh, cnt, r, 1.0f, 0, RayFlags{true, false, false}, Interiors{}, false);
+                }
+                if (S > 1) c = RGB{c.r / (float)S, c.g / (float)S, c.b / (float)S};
                 rgb[3 * i] = c.r;
                 rgb[3 * i + 1] = c.g;
                 rgb[3 * i + 2] = c.b;

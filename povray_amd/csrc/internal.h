@@ -63,7 +63,7 @@ struct RenderBuffers {
     DevBuf hits, contrib, shade, lslot, lterm;
     DevBuf crays, cstate, nchild, offs, sums; // children of a bounce and their placement
     DevBuf srays[2], sdepth[2], sowner[2], shits, scol; // light rays
-    DevBuf counters;
+    DevBuf counters, offsets;
 };
 
 } // namespace pvx

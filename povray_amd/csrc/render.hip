@@ -78,11 +78,40 @@ __global__ void k_zero_counters(uint32_t *c) {
     if (threadIdx.x < 4) c[threadIdx.x] = 0;
 }
 
-__global__ void k_init_paths(uint32_t n, PathState *st, uint8_t *kinds) {
+// Camera rays of sample s of pixel p at queue position p * S + s
+// (TracePixel::CreateCameraRay perspective branch, tracepixel.cpp:357-372).
+__global__ void k_sample_rays(pvx_camera cam, const double *__restrict__ off, uint32_t S, uint32_t n,
+                              pvx_ray *__restrict__ rays) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t px = i / S, s = i % S;
+    const uint32_t x = px % cam.width, y = px / cam.width;
+    double x0 = ((double)x + off[2 * s]) / (double)cam.width - 0.5;
+    double y0 = 0.5 - ((double)y + off[2 * s + 1]) / (double)cam.height;
+    double dx = cam.direction[0] + x0 * cam.right[0] + y0 * cam.up[0];
+    double dy = cam.direction[1] + x0 * cam.right[1] + y0 * cam.up[1];
+    double dz = cam.direction[2] + x0 * cam.right[2] + y0 * cam.up[2];
+    double len = sqrt(dx * dx + dy * dy + dz * dz);
+    pvx_ray r;
+    r.o[0] = cam.location[0];
+    r.o[1] = cam.location[1];
+    r.o[2] = cam.location[2];
+    r.d[0] = dx / len;
+    r.d[1] = dy / len;
+    r.d[2] = dz / len;
+    rays[i] = r;
+}
+
+__global__ void k_scale3(float *rgb, size_t n3, float k) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n3) rgb[i] *= k;
+}
+
+__global__ void k_init_paths(uint32_t n, uint32_t S, PathState *st, uint8_t *kinds) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     PathState ps;
-    ps.pixel = i;
+    ps.pixel = i / S;
     ps.level = 0;
     ps.flags = kFPrimary;
     ps.n_in = 0;
@@ -546,14 +575,23 @@ extern "C" int pvx_scene_set_shading(pvx_scene *s, const pvx_shading_desc *d) {
 
 extern "C" int pvx_render(pvx_scene *s, const pvx_camera *cam, float *rgb, pvx_render_stats *stats,
                           void *stream) {
-    if (!s || !cam || !rgb) return fail(PVX_EINVAL, "null argument");
+    if (!cam) return fail(PVX_EINVAL, "null argument");
+    const double off[2] = {cam->jitter_x, cam->jitter_y};
+    return pvx_render_samples(s, cam, 1, off, rgb, stats, stream);
+}
+
+extern "C" int pvx_render_samples(pvx_scene *s, const pvx_camera *cam, uint32_t S, const double *offsets,
+                                  float *rgb, pvx_render_stats *stats, void *stream) {
+    if (!s || !cam || !rgb || !offsets) return fail(PVX_EINVAL, "null argument");
+    if (S == 0 || S > 256) return fail(PVX_EINVAL, "n_samples must be in [1, 256]");
     ShadingState &sh = s->shading;
     if (!sh.set) return fail(PVX_ESTATE, "pvx_scene_set_shading was not called");
     if (cam->width == 0 || cam->height == 0 || cam->height > 65535)
         return fail(PVX_EINVAL, "bad image size");
     hipStream_t st = static_cast<hipStream_t>(stream);
-    size_t npix = (size_t)cam->width * cam->height;
-    if (npix > 0x3FFFFFFFu) return fail(PVX_EINVAL, "image too large");
+    const size_t npix = (size_t)cam->width * cam->height;
+    const size_t nray = npix * S;
+    if (nray > 0x3FFFFFFFu) return fail(PVX_EINVAL, "image too large");
     RenderBuffers &b = s->rb;
     const size_t nl = std::max<size_t>(1, sh.lights.size());
     const size_t spp = std::max<uint32_t>(1, sh.samples_per_hit);
@@ -582,7 +620,8 @@ extern "C" int pvx_render(pvx_scene *s, const pvx_camera *cam, float *rgb, pvx_r
         if ((rr = b.shits.reserve(ns * sizeof(pvx_hit))) || (rr = b.scol.reserve(ns * 12))) return rr;
         return PVX_OK;
     };
-    if ((r = ensure_queue(npix)) || (r = ensure_shadow(npix * spp))) return r;
+    if ((r = ensure_queue(nray)) || (r = ensure_shadow(nray * spp)) || (r = b.offsets.reserve(16 * S))) return r;
+    HIP_TRY(hipMemcpyAsync(b.offsets.p, offsets, 16 * S, hipMemcpyHostToDevice, st));
 
     RenderParams p;
     p.sv = s->view;
@@ -608,11 +647,12 @@ extern "C" int pvx_render(pvx_scene *s, const pvx_camera *cam, float *rgb, pvx_r
         size_t n3 = 3 * npix;
         k_fill<<<(unsigned)((n3 + 255) / 256), 256, 0, st>>>(rgb, n3, 0.0f);
     }
-    int rc = pvx_camera_rays(cam, static_cast<pvx_ray *>(b.rays[0].p), st);
-    if (rc) return rc;
-    uint32_t n = (uint32_t)npix;
-    k_init_paths<<<(n + 255) / 256, 256, 0, st>>>(n, static_cast<PathState *>(b.state[0].p),
+    uint32_t n = (uint32_t)nray;
+    k_sample_rays<<<(n + 255) / 256, 256, 0, st>>>(*cam, static_cast<const double *>(b.offsets.p), S, n,
+                                                   static_cast<pvx_ray *>(b.rays[0].p));
+    k_init_paths<<<(n + 255) / 256, 256, 0, st>>>(n, S, static_cast<PathState *>(b.state[0].p),
                                                    static_cast<uint8_t *>(b.kinds[0].p));
+    int rc = PVX_OK;
     int cur = 0;
     // every bounce raises the trace level of all but continued rays; the cap
     // only guards against pathological chains of continued rays
@@ -731,6 +771,8 @@ extern "C" int pvx_render(pvx_scene *s, const pvx_camera *cam, float *rgb, pvx_r
         cur = 1 - cur;
         bounces++;
     }
+    if (S > 1) // the pixel is the mean of its samples (tracetask.cpp:889)
+        k_scale3<<<(unsigned)((3 * npix + 255) / 256), 256, 0, st>>>(rgb, 3 * npix, 1.0f / (float)S);
     HIP_TRY(hipEventRecord(e1, st));
     HIP_TRY(hipEventSynchronize(e1));
     float total = 0.0f;

@@ -52,6 +52,8 @@ def lib():
         L.pvx_copy.argtypes = [vp, vp, sz, vp]
         L.pvx_scene_set_shading.argtypes = [vp, C.POINTER(A.PvxShadingDesc)]
         L.pvx_render.argtypes = [vp, C.POINTER(A.PvxCamera), vp, C.POINTER(A.PvxRenderStats), vp]
+        L.pvx_render_samples.argtypes = [vp, C.POINTER(A.PvxCamera), u32, vp, vp, C.POINTER(A.PvxRenderStats),
+                                         vp]
         L.pvx_radiosity_render.argtypes = [vp, C.POINTER(A.PvxCamera), C.POINTER(A.PvxRadiositySettings),
                                            vp, C.POINTER(A.PvxRadiosityStats), vp]
         fp = C.POINTER(C.c_float)
@@ -70,7 +72,7 @@ EXPORTED_SYMBOLS = (
     "pvx_last_error", "pvx_init", "pvx_scene_create", "pvx_scene_destroy",
     "pvx_scene_get_stats", "pvx_trace_closest", "pvx_trace_closest_counted",
     "pvx_all_intersections", "pvx_inside",
-    "pvx_camera_rays", "pvx_copy", "pvx_scene_set_shading", "pvx_render",
+    "pvx_camera_rays", "pvx_copy", "pvx_scene_set_shading", "pvx_render", "pvx_render_samples",
     "pvx_radiosity_render", "pvx_surrogate_create", "pvx_surrogate_destroy",
     "pvx_surrogate_get_params", "pvx_surrogate_grad", "pvx_surrogate_train", "pvx_surrogate_infer",
 )
@@ -183,6 +185,20 @@ class GpuScene:
                                 stream.h if stream else None), "pvx_render")
         return {k: getattr(st, k) for k, _ in A.PvxRenderStats._fields_ if k != "pad"}
 
+    def render_samples(self, offsets, width=None, height=None, stream=None):
+        """Supersampled frame (pvx_render_samples); offsets: (S, 2) sub-pixel
+        positions from the pixel corner.  Returns ((h, w, 3) RGB, stats)."""
+        cam = self.scene.camera_struct(width=width, height=height)
+        offsets = np.ascontiguousarray(offsets, np.float64).reshape(-1, 2)
+        out = hip.DeviceBuffer(cam.width * cam.height * 12)
+        st = A.PvxRenderStats()
+        _check(lib().pvx_render_samples(self.h, C.byref(cam), offsets.shape[0], offsets.ctypes.data, out.value,
+                                        C.byref(st), stream.h if stream else None), "pvx_render_samples")
+        hip.synchronize()
+        img = out.download(np.float32, cam.width * cam.height * 3)
+        return (img.reshape(cam.height, cam.width, 3),
+                {k: getattr(st, k) for k, _ in A.PvxRenderStats._fields_ if k != "pad"})
+
     def render(self, width=None, height=None):
         """Render one frame; returns ((h, w, 3) float32 linear RGB, stats)."""
         cam = self.scene.camera_struct(width=width, height=height)
@@ -291,6 +307,16 @@ class Surrogate:
         _check(lib().pvx_surrogate_infer(self.h, dx.value, n, dy.value, None), "pvx_surrogate_infer")
         hip.synchronize()
         return dy.download(np.float32, n * 3).reshape(n, 3)
+
+
+def m1_offsets(aa_depth: int = 2):
+    """Sample positions of non-adaptive supersampling at threshold 0 without
+    jitter (TraceTask::SupersampleOnePixel, tracetask.cpp:861-890): the pixel
+    centre, then an aa_depth x aa_depth grid (rows outer)."""
+    step = 1.0 / aa_depth
+    rng = 0.5 - step * 0.5
+    grid = [rng * -1 + k * step for k in range(aa_depth)]
+    return np.array([[0.5, 0.5]] + [[0.5 + xx, 0.5 + yy] for yy in grid for xx in grid])
 
 
 def camera_rays(scene, width=None, height=None, jitter=(0.5, 0.5), out=None, stream=None):

@@ -165,6 +165,23 @@ def test_render_csg_glass_refraction(gpu):
     assert abs(st["shadow_rays"] - n_shadow) <= 1e-3 * n_shadow
 
 
+def test_render_supersampled_m1(gpu):
+    """Non-adaptive supersampling at threshold 0, no jitter (+AM1 +A0.0 +R2
+    -J): centre + 2x2 sub-samples, the pixel their mean."""
+    offs = gpu.m1_offsets(2)
+    s = scenes.csg_scene(n_objects=800, width=240, height=136)
+    gs = gpu.GpuScene(s)
+    gs.set_shading()
+    img, st = gs.render_samples(offs)
+    ref, (n_closest, n_shadow) = OracleScene(s).render(s, offsets=offs)
+    _image_parity(img, ref, "csg800 glass 240x136, 5 samples")
+    assert abs(st["closest_rays"] - n_closest) <= 1e-3 * n_closest
+    # one sample at the centre is the plain frame
+    one, _ = gs.render_samples(offs[:1])
+    plain, _ = gs.render()
+    assert np.array_equal(one, plain)
+
+
 def test_render_is_deterministic(gpu):
     for s in (scenes.sphere_scene(width=160, height=120), scenes.csg_scene(n_objects=800, width=160, height=90)):
         gs = gpu.GpuScene(s)
