@@ -407,7 +407,7 @@ int pvx_radiosity_render(pvx_scene *scene, const pvx_camera *cam,
  * RadiosityFunction::GatherLight (core/lighting/radiosity.cpp:509-743) at
  * cache misses.  Input per point: 6 floats (position normalised to [0,1]^3,
  * unit normal); 16-frequency sin/cos features per input (pi 2^(k/3) x,
- * k = 0..15) -> 3 ReLU layers of width 64 -> RGB irradiance.  BF16 MFMA with
+ * k = 0..15, a quarter of that for the normal) -> 3 ReLU layers of width 64 -> RGB irradiance.  BF16 MFMA with
  * FP32 accumulation; FP32 master weights and Adam (beta 0.9 / 0.99).
  * Parameter vector (FP32, PVX_SURROGATE_PARAMS): W1[192][64], b1[64],
  * W2[64][64], b2[64], W3[64][64], b3[64], W4[64][3], b4[3] (row = input). */

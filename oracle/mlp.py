@@ -5,8 +5,9 @@ Restatement of the irradiance surrogate (no reference counterpart: the
 network stands in for RadiosityFunction::GatherLight,
 core/lighting/radiosity.cpp:509-743):
 
-    features  sin(pi f_k x_j), cos(pi f_k x_j), f_k = 2^(k/3), k = 0..15,
-              j = 0..5  (feature 32 j + 2 k + {0: sin, 1: cos})
+    features  sin(pi f_jk x_j), cos(pi f_jk x_j), f_jk = 2^(k/3), k = 0..15,
+              for the position (j = 0..2) and a quarter of that for the
+              normal (j = 3..5)  (feature 32 j + 2 k + {0: sin, 1: cos})
     h1 = relu(F W1 + b1), h2 = relu(h1 W2 + b2), h3 = relu(h2 W3 + b3)
     y  = h3 W4 + b4
     L  = (1/n) sum_p |y_p - t_p|^2
@@ -55,10 +56,14 @@ def pack(d):
                            d["b3"], d["W4"].ravel(), d["b4"]])
 
 
+# per input dimension: position frequencies, then a quarter of them (exact) for the normal
+FREQ_DIM = np.stack([FREQ] * 3 + [FREQ * np.float32(0.25)] * 3).astype(np.float32)  # (6, 16)
+
+
 def features(x):
     """(n, 6) float32 inputs -> (n, 192) features (FP32 argument, FP64 sin/cos)."""
     x = np.asarray(x, np.float32).reshape(-1, 6)
-    v = (x[:, :, None] * FREQ[None, None, :]).astype(np.float32).astype(np.float64)  # (n, 6, 16)
+    v = (x[:, :, None] * FREQ_DIM[None, :, :]).astype(np.float32).astype(np.float64)  # (n, 6, 16)
     f = np.stack([np.sin(np.pi * v), np.cos(np.pi * v)], axis=-1)                    # (n, 6, 16, 2)
     return f.reshape(x.shape[0], N_IN)
 

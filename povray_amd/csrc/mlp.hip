@@ -90,12 +90,15 @@ __device__ inline float freq(int k) {
 }
 
 // Features 8s + 4h .. 8s + 4h + 3 of one point: dimension s / 4, frequencies
-// k0 and k0 + 1 (k0 = 4 (s % 4) + 2h), each as (sin, cos) of pi f x.
+// k0 and k0 + 1 (k0 = 4 (s % 4) + 2h), each as (sin, cos) of pi f x.  The
+// normal's dimensions (3..5) use a quarter of the position frequencies:
+// irradiance varies smoothly with the normal.
 __device__ inline bf16x4 feat4(const float *x, int s, int h) {
     const int j = s >> 2, k0 = 4 * (s & 3) + 2 * h;
+    const float sc = j >= 3 ? 0.25f : 1.0f;
     float s0, c0, s1, c1;
-    sincospif(freq(k0) * x[j], &s0, &c0);
-    sincospif(freq(k0 + 1) * x[j], &s1, &c1);
+    sincospif((freq(k0) * sc) * x[j], &s0, &c0);
+    sincospif((freq(k0 + 1) * sc) * x[j], &s1, &c1);
     bf16x4 r;
     r[0] = (short)f2bf(s0);
     r[1] = (short)f2bf(c0);

@@ -29,8 +29,11 @@ def test_features_closed_form():
     assert np.all(f[..., 0] == 0.0) and np.all(f[..., 1] == 1.0)
     x = np.full((1, 6), 0.5, np.float32)
     f = M.features(x).reshape(6, 16, 2)
-    # f_0 = 1: sin(pi/2) = 1; f_3 = 2: sin(pi) = 0, cos(pi) = -1
-    assert np.allclose(f[:, 0, 0], 1.0) and np.allclose(f[:, 3, 1], -1.0) and np.allclose(f[:, 3, 0], 0, atol=1e-12)
+    # position: f_0 = 1: sin(pi/2) = 1; f_3 = 2: sin(pi) = 0, cos(pi) = -1
+    assert np.allclose(f[:3, 0, 0], 1.0) and np.allclose(f[:3, 3, 1], -1.0)
+    assert np.allclose(f[:3, 3, 0], 0, atol=1e-12)
+    # normal: a quarter of the frequency: f_9 = 8 / 4 = 2
+    assert np.allclose(f[3:, 9, 1], -1.0) and np.allclose(f[3:, 0, 0], np.sin(np.pi / 8))
     assert M.FREQ[15] == 32.0 and M.FREQ[9] == 8.0
     assert np.allclose(M.FREQ, 2.0 ** (np.arange(16) / 3.0), rtol=1e-7)
 
