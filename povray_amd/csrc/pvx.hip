@@ -130,7 +130,8 @@ __device__ inline bool simple_closest(const SceneView &sv, uint32_t kind, uint32
 }
 
 // kLevel 0: triangles and spheres; 1: also boxes; 2: also tori and CSG.
-template <int kLevel, bool kFlags, bool kCount>
+// kWW: while-while traversal (triangle scenes, camera and bounce rays).
+template <int kLevel, bool kFlags, bool kCount, bool kWW = false>
 __global__ __launch_bounds__(kBlock) void k_trace_closest(SceneView sv, const pvx_ray *__restrict__ rays,
                                                           const double *__restrict__ max_depth,
                                                           uint32_t kind, const uint8_t *__restrict__ kinds,
@@ -190,112 +191,237 @@ __global__ __launch_bounds__(kBlock) void k_trace_closest(SceneView sv, const pv
         int32_t node = sv.root;
         int sp = 0;
         uint32_t cnt_nodes = 0, cnt_prims = 0;
-        while (node != kEmpty) {
-            if (node >= 0) {
-                if (kCount) cnt_nodes++;
-                const float4 *np = reinterpret_cast<const float4 *>(sv.nodes + node);
-                const float4 a = np[0], b = np[1], c = np[2];
-                const int4 l = reinterpret_cast<const int4 *>(np)[3];
-                float x0 = fmaf(a.x, rf.ix, -rf.oix), x1 = fmaf(a.y, rf.ix, -rf.oix);
-                float y0 = fmaf(a.z, rf.iy, -rf.oiy), y1 = fmaf(a.w, rf.iy, -rf.oiy);
-                float z0 = fmaf(c.x, rf.iz, -rf.oiz), z1 = fmaf(c.y, rf.iz, -rf.oiz);
-                float tn0 = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
-                float tf0 = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tb));
-                x0 = fmaf(b.x, rf.ix, -rf.oix);
-                x1 = fmaf(b.y, rf.ix, -rf.oix);
-                y0 = fmaf(b.z, rf.iy, -rf.oiy);
-                y1 = fmaf(b.w, rf.iy, -rf.oiy);
-                z0 = fmaf(c.z, rf.iz, -rf.oiz);
-                z1 = fmaf(c.w, rf.iz, -rf.oiz);
-                float tn1 = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
-                float tf1 = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tb));
-                bool h0 = tn0 <= tf0 * 1.000001f;
-                bool h1 = tn1 <= tf1 * 1.000001f;
-                if (h0 && h1) {
-                    int32_t nearc = l.x, farc = l.y;
-                    if (tn1 < tn0) {
-                        nearc = l.y;
-                        farc = l.x;
-                    }
-                    if (sp < kStackLds)
-                        s_stack[sp * kBlock + lane] = farc;
-                    else
-                        my_spill[sp - kStackLds] = farc;
-                    sp++;
-                    node = nearc;
-                    continue;
+        // While-while traversal (the lanes of a wavefront visit inner nodes
+        // together, then leaves together) pays for camera and bounce rays in
+        // triangle scenes; for light rays, and with boxes, tori and CSG, whose
+        // leaf tests are long, each pass of the loop takes one node or one
+        // leaf step.  The order each lane visits its nodes in is the same
+        // either way.
+#define PVX_POP()                                                                       \
+    do {                                                                                \
+        if (sp == 0) {                                                                  \
+            node = kEmpty;                                                              \
+        } else {                                                                        \
+            sp--;                                                                       \
+            node = sp < kStackLds ? s_stack[sp * kBlock + lane] : my_spill[sp - kStackLds]; \
+        }                                                                               \
+    } while (0)
+        // one inner-node step (pushes the far child or pops when no child is hit)
+        auto step_node = [&]() {
+            if (kCount) cnt_nodes++;
+            const float4 *np = reinterpret_cast<const float4 *>(sv.nodes + node);
+            const float4 a = np[0], b = np[1], c = np[2];
+            const int4 l = reinterpret_cast<const int4 *>(np)[3];
+            float x0 = fmaf(a.x, rf.ix, -rf.oix), x1 = fmaf(a.y, rf.ix, -rf.oix);
+            float y0 = fmaf(a.z, rf.iy, -rf.oiy), y1 = fmaf(a.w, rf.iy, -rf.oiy);
+            float z0 = fmaf(c.x, rf.iz, -rf.oiz), z1 = fmaf(c.y, rf.iz, -rf.oiz);
+            float tn0 = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
+            float tf0 = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tb));
+            x0 = fmaf(b.x, rf.ix, -rf.oix);
+            x1 = fmaf(b.y, rf.ix, -rf.oix);
+            y0 = fmaf(b.z, rf.iy, -rf.oiy);
+            y1 = fmaf(b.w, rf.iy, -rf.oiy);
+            z0 = fmaf(c.z, rf.iz, -rf.oiz);
+            z1 = fmaf(c.w, rf.iz, -rf.oiz);
+            float tn1 = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
+            float tf1 = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tb));
+            bool h0 = tn0 <= tf0 * 1.000001f;
+            bool h1 = tn1 <= tf1 * 1.000001f;
+            if (h0 && h1) {
+                int32_t nearc = l.x, farc = l.y;
+                if (tn1 < tn0) {
+                    nearc = l.y;
+                    farc = l.x;
                 }
-                if (h0) {
-                    node = l.x;
-                    continue;
-                }
-                if (h1) {
-                    node = l.y;
-                    continue;
-                }
-            } else {
-                // leaf: ~node = first << 3 | (count - 1)
-                const uint32_t leaf = (uint32_t)(~node);
-                const uint32_t first = leaf >> 3, count = (leaf & 7u) + 1u;
-                if (kCount) cnt_prims += count;
-                for (uint32_t k = 0; k < count; k++) {
-                    const PrimRec &p = sv.prims[first + k];
-                    const uint32_t ka = p.kind_axis;
-                    const uint32_t pk = ka & 15u;
-                    const int32_t obj = p.obj;
-                    if (kFlags && !precondition(sv.obj_flags[obj], rk)) continue;
-                    if (pk == kPrimTriangle) {
-                        double dep;
-                        if (tri_hit64(p.tri, ka >> 4, o, d, dep) && depth_ok(dep, shadow) &&
-                            better(dep, obj, p.tri.prim, bt, bo, bp)) {
-                            bt = dep;
-                            bo = obj;
-                            bp = p.tri.prim;
-                        }
-                    } else if (pk == kPrimSphere) {
-                        double hs[2];
-                        int nh = sphere_hits(d3(p.sph.c[0], p.sph.c[1], p.sph.c[2]), p.sph.r, o, d, hs);
-                        for (int j = 0; j < nh; j++)
-                            if (depth_ok(hs[j], shadow) && better(hs[j], obj, 0, bt, bo, bp)) {
-                                bt = hs[j];
-                                bo = obj;
-                                bp = 0;
-                            }
-                    } else if (kLevel == 1) { // boxes only
-                        double hs[2];
-                        int side[2];
-                        int nh = box_hits(sv.boxes[p.ref.index], sv.xforms, o, d, hs, side);
-                        for (int j = 0; j < nh; j++)
-                            if (depth_ok(hs[j], shadow) && better(hs[j], obj, 0, bt, bo, bp)) {
-                                bt = hs[j];
-                                bo = obj;
-                                bp = 0;
-                            }
-                    } else if (kLevel >= 2) {
-                        double dep;
-                        int32_t prim = 0;
-                        bool ok;
-                        if (pk == kPrimCsg)
-                            ok = csg_closest(sv, p.ref.index, o, d, shadow, bt, dep, prim);
-                        else
-                            ok = simple_closest(sv, pk, p.ref.index, o, d, shadow, dep);
-                        if (ok && better(dep, obj, prim, bt, bo, bp)) {
-                            bt = dep;
-                            bo = obj;
-                            bp = prim;
-                        }
-                    }
-                }
-                tb = (float)bt * 1.000001f;
+                if (sp < kStackLds)
+                    s_stack[sp * kBlock + lane] = farc;
+                else
+                    my_spill[sp - kStackLds] = farc;
+                sp++;
+                node = nearc;
+                return;
             }
-            // pop
-            if (sp == 0) {
-                node = kEmpty;
-            } else {
-                sp--;
-                node = sp < kStackLds ? s_stack[sp * kBlock + lane] : my_spill[sp - kStackLds];
+            if (h0) {
+                node = l.x;
+                return;
+            }
+            if (h1) {
+                node = l.y;
+                return;
+            }
+            PVX_POP();
+        };
+        // one leaf step (tests its primitives, then pops)
+        auto step_leaf = [&]() {
+            // leaf: ~node = first << 3 | (count - 1)
+            const uint32_t leaf = (uint32_t)(~node);
+            const uint32_t first = leaf >> 3, count = (leaf & 7u) + 1u;
+            if (kCount) cnt_prims += count;
+            for (uint32_t k = 0; k < count; k++) {
+                const PrimRec &p = sv.prims[first + k];
+                const uint32_t ka = p.kind_axis;
+                const uint32_t pk = ka & 15u;
+                const int32_t obj = p.obj;
+                if (kFlags && !precondition(sv.obj_flags[obj], rk)) continue;
+                if (pk == kPrimTriangle) {
+                    double dep;
+                    if (tri_hit64(p.tri, ka >> 4, o, d, dep) && depth_ok(dep, shadow) &&
+                        better(dep, obj, p.tri.prim, bt, bo, bp)) {
+                        bt = dep;
+                        bo = obj;
+                        bp = p.tri.prim;
+                    }
+                } else if (pk == kPrimSphere) {
+                    double hs[2];
+                    int nh = sphere_hits(d3(p.sph.c[0], p.sph.c[1], p.sph.c[2]), p.sph.r, o, d, hs);
+                    for (int j = 0; j < nh; j++)
+                        if (depth_ok(hs[j], shadow) && better(hs[j], obj, 0, bt, bo, bp)) {
+                            bt = hs[j];
+                            bo = obj;
+                            bp = 0;
+                        }
+                } else if (kLevel == 1) { // boxes only
+                    double hs[2];
+                    int side[2];
+                    int nh = box_hits(sv.boxes[p.ref.index], sv.xforms, o, d, hs, side);
+                    for (int j = 0; j < nh; j++)
+                        if (depth_ok(hs[j], shadow) && better(hs[j], obj, 0, bt, bo, bp)) {
+                            bt = hs[j];
+                            bo = obj;
+                            bp = 0;
+                        }
+                } else if (kLevel >= 2) {
+                    double dep;
+                    int32_t prim = 0;
+                    bool ok;
+                    if (pk == kPrimCsg)
+                        ok = csg_closest(sv, p.ref.index, o, d, shadow, bt, dep, prim);
+                    else
+                        ok = simple_closest(sv, pk, p.ref.index, o, d, shadow, dep);
+                    if (ok && better(dep, obj, prim, bt, bo, bp)) {
+                        bt = dep;
+                        bo = obj;
+                        bp = prim;
+                    }
+                }
+            }
+            tb = (float)bt * 1.000001f;
+            PVX_POP();
+        };
+        if constexpr (kWW) {
+            while (node != kEmpty) {
+                while (node >= 0) step_node();
+                while (node < 0 && node != kEmpty) step_leaf();
+            }
+        } else {
+            while (node != kEmpty) {
+                if (node >= 0) {
+                    if (kCount) cnt_nodes++;
+                    const float4 *np = reinterpret_cast<const float4 *>(sv.nodes + node);
+                    const float4 a = np[0], b = np[1], c = np[2];
+                    const int4 l = reinterpret_cast<const int4 *>(np)[3];
+                    float x0 = fmaf(a.x, rf.ix, -rf.oix), x1 = fmaf(a.y, rf.ix, -rf.oix);
+                    float y0 = fmaf(a.z, rf.iy, -rf.oiy), y1 = fmaf(a.w, rf.iy, -rf.oiy);
+                    float z0 = fmaf(c.x, rf.iz, -rf.oiz), z1 = fmaf(c.y, rf.iz, -rf.oiz);
+                    float tn0 = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
+                    float tf0 = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tb));
+                    x0 = fmaf(b.x, rf.ix, -rf.oix);
+                    x1 = fmaf(b.y, rf.ix, -rf.oix);
+                    y0 = fmaf(b.z, rf.iy, -rf.oiy);
+                    y1 = fmaf(b.w, rf.iy, -rf.oiy);
+                    z0 = fmaf(c.z, rf.iz, -rf.oiz);
+                    z1 = fmaf(c.w, rf.iz, -rf.oiz);
+                    float tn1 = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
+                    float tf1 = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tb));
+                    bool h0 = tn0 <= tf0 * 1.000001f;
+                    bool h1 = tn1 <= tf1 * 1.000001f;
+                    if (h0 && h1) {
+                        int32_t nearc = l.x, farc = l.y;
+                        if (tn1 < tn0) {
+                            nearc = l.y;
+                            farc = l.x;
+                        }
+                        if (sp < kStackLds)
+                            s_stack[sp * kBlock + lane] = farc;
+                        else
+                            my_spill[sp - kStackLds] = farc;
+                        sp++;
+                        node = nearc;
+                        continue;
+                    }
+                    if (h0) {
+                        node = l.x;
+                        continue;
+                    }
+                    if (h1) {
+                        node = l.y;
+                        continue;
+                    }
+                } else {
+                    // leaf: ~node = first << 3 | (count - 1)
+                    const uint32_t leaf = (uint32_t)(~node);
+                    const uint32_t first = leaf >> 3, count = (leaf & 7u) + 1u;
+                    if (kCount) cnt_prims += count;
+                    for (uint32_t k = 0; k < count; k++) {
+                        const PrimRec &p = sv.prims[first + k];
+                        const uint32_t ka = p.kind_axis;
+                        const uint32_t pk = ka & 15u;
+                        const int32_t obj = p.obj;
+                        if (kFlags && !precondition(sv.obj_flags[obj], rk)) continue;
+                        if (pk == kPrimTriangle) {
+                            double dep;
+                            if (tri_hit64(p.tri, ka >> 4, o, d, dep) && depth_ok(dep, shadow) &&
+                                better(dep, obj, p.tri.prim, bt, bo, bp)) {
+                                bt = dep;
+                                bo = obj;
+                                bp = p.tri.prim;
+                            }
+                        } else if (pk == kPrimSphere) {
+                            double hs[2];
+                            int nh = sphere_hits(d3(p.sph.c[0], p.sph.c[1], p.sph.c[2]), p.sph.r, o, d, hs);
+                            for (int j = 0; j < nh; j++)
+                                if (depth_ok(hs[j], shadow) && better(hs[j], obj, 0, bt, bo, bp)) {
+                                    bt = hs[j];
+                                    bo = obj;
+                                    bp = 0;
+                                }
+                        } else if (kLevel == 1) { // boxes only
+                            double hs[2];
+                            int side[2];
+                            int nh = box_hits(sv.boxes[p.ref.index], sv.xforms, o, d, hs, side);
+                            for (int j = 0; j < nh; j++)
+                                if (depth_ok(hs[j], shadow) && better(hs[j], obj, 0, bt, bo, bp)) {
+                                    bt = hs[j];
+                                    bo = obj;
+                                    bp = 0;
+                                }
+                        } else if (kLevel >= 2) {
+                            double dep;
+                            int32_t prim = 0;
+                            bool ok;
+                            if (pk == kPrimCsg)
+                                ok = csg_closest(sv, p.ref.index, o, d, shadow, bt, dep, prim);
+                            else
+                                ok = simple_closest(sv, pk, p.ref.index, o, d, shadow, dep);
+                            if (ok && better(dep, obj, prim, bt, bo, bp)) {
+                                bt = dep;
+                                bo = obj;
+                                bp = prim;
+                            }
+                        }
+                    }
+                    tb = (float)bt * 1.000001f;
+                }
+                // pop
+                if (sp == 0) {
+                    node = kEmpty;
+                } else {
+                    sp--;
+                    node = sp < kStackLds ? s_stack[sp * kBlock + lane] : my_spill[sp - kStackLds];
+                }
             }
         }
+#undef PVX_POP
         hits[i] = pvx_hit{bt, bo, bp};
         if (kCount) {
             node_visits[i] = cnt_nodes;
@@ -591,15 +717,20 @@ static int launch_closest(pvx_scene *s, const pvx_ray *rays, const double *max_d
     uint32_t nn = (uint32_t)n;
 #define PVX_LAUNCH(L, F) \
     k_trace_closest<L, F, kCount><<<nblocks, kBlock, 0, st>>>(s->view, rays, max_depth, ray_kind, kinds, nn, hits, s->d_spill, nv, pt)
+#define PVX_LAUNCH_WW(L, F) \
+    k_trace_closest<L, F, kCount, true><<<nblocks, kBlock, 0, st>>>(s->view, rays, max_depth, ray_kind, kinds, nn, hits, s->d_spill, nv, pt)
     const uint32_t level = s->view.has_complex;
     if (level >= 2) {
         if (flags) PVX_LAUNCH(2, true); else PVX_LAUNCH(2, false);
     } else if (level == 1) {
         if (flags) PVX_LAUNCH(1, true); else PVX_LAUNCH(1, false);
+    } else if (ray_kind != PVX_RAY_SHADOW) {
+        if (flags) PVX_LAUNCH_WW(0, true); else PVX_LAUNCH_WW(0, false);
     } else {
         if (flags) PVX_LAUNCH(0, true); else PVX_LAUNCH(0, false);
     }
 #undef PVX_LAUNCH
+#undef PVX_LAUNCH_WW
     HIP_TRY(hipGetLastError());
     return PVX_OK;
 }
