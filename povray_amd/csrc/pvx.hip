@@ -42,8 +42,8 @@ extern "C" const char *pvx_last_error(void) { return pvx::g_err.c_str(); }
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kStackLds = 32;
-constexpr int kStackSpill = 32;
+constexpr int kStackLds = 24;
+constexpr int kStackSpill = 40;
 constexpr int kStackMax = kStackLds + kStackSpill;
 constexpr int32_t kEmpty = INT32_MIN;
 
@@ -131,15 +131,13 @@ __device__ inline bool simple_closest(const SceneView &sv, uint32_t kind, uint32
 
 // kLevel 0: triangles and spheres; 1: also boxes; 2: also tori and CSG.
 // kWW: while-while traversal (triangle scenes, camera and bounce rays).
-template <int kLevel, bool kFlags, bool kCount, bool kWW = false>
-__global__ __launch_bounds__(kBlock) void k_trace_closest(SceneView sv, const pvx_ray *__restrict__ rays,
-                                                          const double *__restrict__ max_depth,
-                                                          uint32_t kind, const uint8_t *__restrict__ kinds,
-                                                          uint32_t n, pvx_hit *__restrict__ hits,
-                                                          int32_t *__restrict__ spill,
-                                                          uint32_t *__restrict__ node_visits,
-                                                          uint32_t *__restrict__ prim_tests) {
-    __shared__ int32_t s_stack[kStackLds * kBlock];
+template <int kLevel, bool kFlags, bool kCount, bool kWW>
+__device__ __forceinline__ void trace_closest_body(const SceneView &sv, const pvx_ray *__restrict__ rays,
+                                                   const double *__restrict__ max_depth, uint32_t kind,
+                                                   const uint8_t *__restrict__ kinds, uint32_t n,
+                                                   pvx_hit *__restrict__ hits, int32_t *__restrict__ spill,
+                                                   uint32_t *__restrict__ node_visits,
+                                                   uint32_t *__restrict__ prim_tests, int32_t *s_stack) {
     const uint32_t lane = threadIdx.x;
     const uint32_t gtid = blockIdx.x * kBlock + lane;
     int32_t *my_spill = spill + (size_t)gtid * kStackSpill;
@@ -430,6 +428,30 @@ __global__ __launch_bounds__(kBlock) void k_trace_closest(SceneView sv, const pv
     }
 }
 
+#define PVX_TRACE_ARGS                                                                                      \
+    SceneView sv, const pvx_ray *__restrict__ rays, const double *__restrict__ max_depth, uint32_t kind,    \
+        const uint8_t *__restrict__ kinds, uint32_t n, pvx_hit *__restrict__ hits,                          \
+        int32_t *__restrict__ spill, uint32_t *__restrict__ node_visits, uint32_t *__restrict__ prim_tests
+
+template <int kLevel, bool kFlags, bool kCount>
+__global__ __launch_bounds__(kBlock) void k_trace_closest(PVX_TRACE_ARGS) {
+    __shared__ int32_t s_stack[kStackLds * kBlock];
+    trace_closest_body<kLevel, kFlags, kCount, false>(sv, rays, max_depth, kind, kinds, n, hits, spill, node_visits,
+                                                      prim_tests, s_stack);
+}
+
+// Triangle scenes, camera and bounce rays: while-while traversal (78 VGPRs
+// and a 24 KB LDS stack per block: 6 wavefronts per SIMD; forcing 7 or 8
+// costs more in register pressure than it gains).
+template <bool kFlags, bool kCount>
+__global__ __launch_bounds__(kBlock) void k_trace_closest_ww(
+    PVX_TRACE_ARGS) {
+    __shared__ int32_t s_stack[kStackLds * kBlock];
+    trace_closest_body<0, kFlags, kCount, true>(sv, rays, max_depth, kind, kinds, n, hits, spill, node_visits,
+                                                prim_tests, s_stack);
+}
+#undef PVX_TRACE_ARGS
+
 // Perspective camera rays (tracepixel.cpp:357-372, 918-926).
 __global__ void k_camera_rays(pvx_camera cam, pvx_ray *rays) {
     uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -666,7 +688,7 @@ extern "C" int pvx_scene_create(const pvx_scene_desc *desc, const pvx_build_opts
     auto pick = [](uint32_t level, bool flags) -> const void * {
         if (level >= 2) return flags ? (const void *)k_trace_closest<2, true, false> : (const void *)k_trace_closest<2, false, false>;
         if (level == 1) return flags ? (const void *)k_trace_closest<1, true, false> : (const void *)k_trace_closest<1, false, false>;
-        return flags ? (const void *)k_trace_closest<0, true, false> : (const void *)k_trace_closest<0, false, false>;
+        return flags ? (const void *)k_trace_closest_ww<true, false> : (const void *)k_trace_closest_ww<false, false>;
     };
     auto kfn = pick(h.has_complex, h.any_flags != 0);
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, kBlock, 0));
@@ -718,7 +740,7 @@ static int launch_closest(pvx_scene *s, const pvx_ray *rays, const double *max_d
 #define PVX_LAUNCH(L, F) \
     k_trace_closest<L, F, kCount><<<nblocks, kBlock, 0, st>>>(s->view, rays, max_depth, ray_kind, kinds, nn, hits, s->d_spill, nv, pt)
 #define PVX_LAUNCH_WW(L, F) \
-    k_trace_closest<L, F, kCount, true><<<nblocks, kBlock, 0, st>>>(s->view, rays, max_depth, ray_kind, kinds, nn, hits, s->d_spill, nv, pt)
+    k_trace_closest_ww<F, kCount><<<nblocks, kBlock, 0, st>>>(s->view, rays, max_depth, ray_kind, kinds, nn, hits, s->d_spill, nv, pt)
     const uint32_t level = s->view.has_complex;
     if (level >= 2) {
         if (flags) PVX_LAUNCH(2, true); else PVX_LAUNCH(2, false);
