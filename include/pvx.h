@@ -241,6 +241,40 @@ int pvx_all_intersections(pvx_scene *scene, const pvx_ray *rays,
 int pvx_inside(pvx_scene *scene, const double *points, const uint32_t *obj,
                size_t n, uint8_t *out, void *stream);
 
+/* ---- scene batches ----------------------------------------------------- */
+/* Many scenes traced in one launch.  The batch keeps a device table of the
+ * scenes' device-resident BVHs and primitive pools; each wavefront takes up
+ * to 64 rays of one scene, so scenes of uneven size share one persistent
+ * grid.  The scenes stay owned by the caller and must outlive the batch.
+ * No reference counterpart (the reference renders one scene per process):
+ * for each scene the results are exactly those of pvx_trace_closest, i.e.
+ * Trace::FindIntersection (core/render/trace.cpp:285-344). */
+typedef struct pvx_batch pvx_batch;
+int pvx_batch_create(pvx_scene *const *scenes, uint32_t n_scenes, pvx_batch **out);
+int pvx_batch_destroy(pvx_batch *batch);
+
+/* Closest hit for the rays of every scene of the batch.  rays, max_depth
+ * (may be NULL) and hits are device buffers holding the scenes' rays one
+ * after another; ray_first (HOST memory, n_scenes + 1 non-decreasing
+ * entries, ray_first[0] = 0) gives scene s the rays
+ * [ray_first[s], ray_first[s+1]); a scene may have none.  scene_clock
+ * (optional device buffer of n_scenes + 1 entries) receives [0] = the
+ * earliest wavefront start and [1 + s] = when scene s's last wavefront
+ * finished (0 if it had no rays), in ticks of pvx_clock_rate. */
+int pvx_batch_trace_closest(pvx_batch *batch, const pvx_ray *rays, const uint64_t *ray_first,
+                            const double *max_depth, uint32_t ray_kind, pvx_hit *hits,
+                            uint64_t *scene_clock, void *stream);
+
+/* pvx_batch_trace_closest plus the per-ray work counters of
+ * pvx_trace_closest_counted. */
+int pvx_batch_trace_closest_counted(pvx_batch *batch, const pvx_ray *rays,
+                                    const uint64_t *ray_first, const double *max_depth,
+                                    uint32_t ray_kind, pvx_hit *hits, uint32_t *node_visits,
+                                    uint32_t *prim_tests, uint64_t *scene_clock, void *stream);
+
+/* Ticks per second of the device wall clock that scene_clock counts. */
+int pvx_clock_rate(double *hz);
+
 /* ---- camera ------------------------------------------------------------ */
 /* Perspective camera as TracePixel::CreateCameraRay builds it
  * (core/render/tracepixel.cpp:357-372, 918-926): for pixel (x, y) sampled at

@@ -84,6 +84,22 @@ struct pvx_scene {
     void (*rad_engine_free)(void *) = nullptr;
 };
 
+// A batch of caller-owned scenes traced in one launch: a device table of
+// their SceneViews and, per call, a table of 64-ray chunks {scene, first ray,
+// count} that never straddle two scenes.
+struct pvx_batch {
+    int device = 0;
+    std::vector<pvx_scene *> scenes;
+    uint32_t level = 0;     // largest has_complex of the scenes
+    uint32_t any_flags = 0; // union of the scenes' object flags
+    uint32_t grid_ww = 0, grid = 0;
+    pvx::DevBuf views, chunks, spill;
+    pvx::HostBuf hchunks;
+    hipEvent_t uploaded = nullptr; // last chunk-table upload (guards hchunks)
+    std::vector<uint64_t> last_first;
+    uint32_t n_chunks = 0;
+};
+
 namespace pvx {
 // Closest-hit launch shared with the renderer (no counters).
 // `kinds` (optional): one PVX_RAY_* kind per ray for the object-flag

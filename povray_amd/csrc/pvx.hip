@@ -11,6 +11,7 @@
 //     reference's own FP64 arithmetic, so hit ids and depths follow it.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <cstdio>
@@ -131,8 +132,17 @@ __device__ inline bool simple_closest(const SceneView &sv, uint32_t kind, uint32
 
 // kLevel 0: triangles and spheres; 1: also boxes; 2: also tori and CSG.
 // kWW: while-while traversal (triangle scenes, camera and bounce rays).
-template <int kLevel, bool kFlags, bool kCount, bool kWW>
-__device__ __forceinline__ void trace_closest_body(const SceneView &sv, const pvx_ray *__restrict__ rays,
+// kMulti: rays of many scenes in one launch.  `n` is then 64 x the number of
+// chunks; chunk c = {scene, first ray, ray count <= 64} covers the wavefront
+// whose loop index is 64c..64c+63, so the scene (and its SceneView, read from
+// `views`) is uniform across the wavefront.  With `clocks`, lane 0 of each
+// wavefront records the wall clock at its start (minimum into clocks[0]) and
+// at the end of each chunk (maximum into clocks[1 + scene]).
+template <int kLevel, bool kFlags, bool kCount, bool kWW, bool kMulti = false>
+__device__ __forceinline__ void trace_closest_body(const SceneView &sv0, const SceneView *__restrict__ views,
+                                                   const uint4 *__restrict__ chunks,
+                                                   unsigned long long *__restrict__ clocks,
+                                                   const pvx_ray *__restrict__ rays,
                                                    const double *__restrict__ max_depth, uint32_t kind,
                                                    const uint8_t *__restrict__ kinds, uint32_t n,
                                                    pvx_hit *__restrict__ hits, int32_t *__restrict__ spill,
@@ -142,16 +152,27 @@ __device__ __forceinline__ void trace_closest_body(const SceneView &sv, const pv
     const uint32_t gtid = blockIdx.x * kBlock + lane;
     int32_t *my_spill = spill + (size_t)gtid * kStackSpill;
     const bool shadow = (kind == PVX_RAY_SHADOW);
+    if constexpr (kMulti) {
+        if (clocks && (lane & 63u) == 0) atomicMin(&clocks[0], (unsigned long long)wall_clock64());
+    }
 
     for (uint32_t i = gtid; i < n; i += gridDim.x * kBlock) {
-        const pvx_ray r = rays[i];
+        uint32_t ri = i, scene = 0;
+        if constexpr (kMulti) {
+            const uint4 ch = chunks[__builtin_amdgcn_readfirstlane(i >> 6)];
+            scene = ch.x;
+            if ((i & 63u) >= ch.z) continue; // lane 0 always has a ray
+            ri = ch.y + (i & 63u);
+        }
+        const SceneView sv = kMulti ? views[scene] : sv0;
+        const pvx_ray r = rays[ri];
         // per-ray kind (precondition only; shadow rays come in launches of their own)
-        const uint32_t rk = (kFlags && kinds) ? (uint32_t)kinds[i] : kind;
+        const uint32_t rk = (kFlags && kinds) ? (uint32_t)kinds[ri] : kind;
         D3 o = d3(r.o[0], r.o[1], r.o[2]);
         D3 d = d3(r.d[0], r.d[1], r.d[2]);
         double bt = kBoundHuge;
         if (max_depth) {
-            double md = max_depth[i];
+            double md = max_depth[ri];
             if (md >= kEpsilon) bt = md; // trace.cpp:157-158
         }
         int32_t bo = -1, bp = 0;
@@ -420,10 +441,13 @@ __device__ __forceinline__ void trace_closest_body(const SceneView &sv, const pv
             }
         }
 #undef PVX_POP
-        hits[i] = pvx_hit{bt, bo, bp};
+        hits[ri] = pvx_hit{bt, bo, bp};
         if (kCount) {
-            node_visits[i] = cnt_nodes;
-            prim_tests[i] = cnt_prims;
+            node_visits[ri] = cnt_nodes;
+            prim_tests[ri] = cnt_prims;
+        }
+        if constexpr (kMulti) {
+            if (clocks && (lane & 63u) == 0) atomicMax(&clocks[1 + scene], (unsigned long long)wall_clock64());
         }
     }
 }
@@ -436,8 +460,8 @@ __device__ __forceinline__ void trace_closest_body(const SceneView &sv, const pv
 template <int kLevel, bool kFlags, bool kCount>
 __global__ __launch_bounds__(kBlock) void k_trace_closest(PVX_TRACE_ARGS) {
     __shared__ int32_t s_stack[kStackLds * kBlock];
-    trace_closest_body<kLevel, kFlags, kCount, false>(sv, rays, max_depth, kind, kinds, n, hits, spill, node_visits,
-                                                      prim_tests, s_stack);
+    trace_closest_body<kLevel, kFlags, kCount, false>(sv, nullptr, nullptr, nullptr, rays, max_depth, kind, kinds, n,
+                                                      hits, spill, node_visits, prim_tests, s_stack);
 }
 
 // Triangle scenes, camera and bounce rays: while-while traversal (78 VGPRs
@@ -447,10 +471,38 @@ template <bool kFlags, bool kCount>
 __global__ __launch_bounds__(kBlock) void k_trace_closest_ww(
     PVX_TRACE_ARGS) {
     __shared__ int32_t s_stack[kStackLds * kBlock];
-    trace_closest_body<0, kFlags, kCount, true>(sv, rays, max_depth, kind, kinds, n, hits, spill, node_visits,
-                                                prim_tests, s_stack);
+    trace_closest_body<0, kFlags, kCount, true>(sv, nullptr, nullptr, nullptr, rays, max_depth, kind, kinds, n, hits,
+                                                spill, node_visits, prim_tests, s_stack);
 }
 #undef PVX_TRACE_ARGS
+
+// The same kernels over a batch of scenes (chunk table and SceneView table).
+#define PVX_MULTI_ARGS                                                                                      \
+    const SceneView *__restrict__ views, const uint4 *__restrict__ chunks,                                  \
+        unsigned long long *__restrict__ clocks, const pvx_ray *__restrict__ rays,                          \
+        const double *__restrict__ max_depth, uint32_t kind, const uint8_t *__restrict__ kinds, uint32_t n, \
+        pvx_hit *__restrict__ hits, int32_t *__restrict__ spill, uint32_t *__restrict__ node_visits,        \
+        uint32_t *__restrict__ prim_tests
+
+template <int kLevel, bool kFlags, bool kCount>
+__global__ __launch_bounds__(kBlock) void k_trace_closest_multi(PVX_MULTI_ARGS) {
+    __shared__ int32_t s_stack[kStackLds * kBlock];
+    trace_closest_body<kLevel, kFlags, kCount, false, true>(views[0], views, chunks, clocks, rays, max_depth, kind,
+                                                            kinds, n, hits, spill, node_visits, prim_tests, s_stack);
+}
+
+template <bool kFlags, bool kCount>
+__global__ __launch_bounds__(kBlock) void k_trace_closest_ww_multi(PVX_MULTI_ARGS) {
+    __shared__ int32_t s_stack[kStackLds * kBlock];
+    trace_closest_body<0, kFlags, kCount, true, true>(views[0], views, chunks, clocks, rays, max_depth, kind, kinds,
+                                                      n, hits, spill, node_visits, prim_tests, s_stack);
+}
+#undef PVX_MULTI_ARGS
+
+__global__ void k_clock_init(unsigned long long *clocks, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) clocks[i] = i == 0 ? ~0ull : 0ull;
+}
 
 // Perspective camera rays (tracepixel.cpp:357-372, 918-926).
 __global__ void k_camera_rays(pvx_camera cam, pvx_ray *rays) {
@@ -775,6 +827,184 @@ extern "C" int pvx_trace_closest_counted(pvx_scene *s, const pvx_ray *rays,
     if (!node_visits || !prim_tests) return fail(PVX_EINVAL, "null counter buffer");
     return launch_closest<true>(s, rays, max_depth, ray_kind, nullptr, n, hits, node_visits, prim_tests,
                                 stream);
+}
+
+// ----------------------------------------------------------------------------
+// scene batches
+// ----------------------------------------------------------------------------
+extern "C" int pvx_batch_create(pvx_scene *const *scenes, uint32_t n_scenes, pvx_batch **out) {
+    if (!out) return fail(PVX_EINVAL, "null argument");
+    *out = nullptr;
+    if (!scenes || n_scenes == 0) return fail(PVX_EINVAL, "empty scene list");
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    std::vector<SceneView> views(n_scenes);
+    uint32_t level = 0, flags = 0;
+    for (uint32_t k = 0; k < n_scenes; k++) {
+        const pvx_scene *s = scenes[k];
+        if (!s) return fail(PVX_EINVAL, "null scene in the batch");
+        if (s->device != dev) return fail(PVX_EINVAL, "scene " + std::to_string(k) + " lives on another device");
+        views[k] = s->view;
+        level = std::max(level, s->view.has_complex);
+        flags |= s->view.any_flags;
+    }
+    pvx_batch *b = new pvx_batch();
+    b->device = dev;
+    b->scenes.assign(scenes, scenes + n_scenes);
+    b->level = level;
+    b->any_flags = flags;
+    int r;
+    if ((r = b->views.reserve(views.size() * sizeof(SceneView))) != PVX_OK) {
+        delete b;
+        return r;
+    }
+    hipError_t e = hipMemcpy(b->views.p, views.data(), views.size() * sizeof(SceneView), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->uploaded, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        pvx_batch_destroy(b);
+        return fail(PVX_EDEVICE, std::string("batch set-up: ") + hipGetErrorString(e));
+    }
+    // persistent grids of the two kernel families the batch can launch
+    int cus = 0, per_ww = 0, per = 0;
+    const void *kfn = level >= 2   ? (flags ? (const void *)k_trace_closest_multi<2, true, false>
+                                            : (const void *)k_trace_closest_multi<2, false, false>)
+                      : level == 1 ? (flags ? (const void *)k_trace_closest_multi<1, true, false>
+                                            : (const void *)k_trace_closest_multi<1, false, false>)
+                                   : (flags ? (const void *)k_trace_closest_multi<0, true, false>
+                                            : (const void *)k_trace_closest_multi<0, false, false>);
+    const void *kww = flags ? (const void *)k_trace_closest_ww_multi<true, false>
+                            : (const void *)k_trace_closest_ww_multi<false, false>;
+    e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kfn, kBlock, 0);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_ww, kww, kBlock, 0);
+    if (e != hipSuccess) {
+        pvx_batch_destroy(b);
+        return fail(PVX_EDEVICE, std::string("batch occupancy: ") + hipGetErrorString(e));
+    }
+    b->grid = (uint32_t)(cus * std::max(per, 1));
+    b->grid_ww = (uint32_t)(cus * std::max(per_ww, 1));
+    const size_t spill_bytes = (size_t)std::max(b->grid, b->grid_ww) * kBlock * kStackSpill * sizeof(int32_t);
+    if ((r = b->spill.reserve(spill_bytes)) != PVX_OK) {
+        pvx_batch_destroy(b);
+        return r;
+    }
+    *out = b;
+    return PVX_OK;
+}
+
+extern "C" int pvx_batch_destroy(pvx_batch *b) {
+    if (!b) return PVX_OK;
+    (void)hipDeviceSynchronize();
+    if (b->uploaded) (void)hipEventDestroy(b->uploaded);
+    delete b;
+    return PVX_OK;
+}
+
+// Chunk table for the ray ranges `first` (host, n_scenes + 1 entries); kept
+// on the device and rebuilt only when the ranges change.
+static int batch_chunks(pvx_batch *b, const uint64_t *first, hipStream_t st) {
+    const size_t ns = b->scenes.size();
+    if (!first) return fail(PVX_EINVAL, "null ray_first");
+    if (first[0] != 0) return fail(PVX_EINVAL, "ray_first[0] must be 0");
+    for (size_t k = 0; k < ns; k++)
+        if (first[k + 1] < first[k]) return fail(PVX_EINVAL, "ray_first must not decrease");
+    if (first[ns] > 0xFFFFFFFFull) return fail(PVX_EINVAL, "too many rays in one call");
+    if (b->last_first.size() == ns + 1 && std::equal(b->last_first.begin(), b->last_first.end(), first))
+        return PVX_OK;
+    uint64_t nc = 0;
+    for (size_t k = 0; k < ns; k++) nc += (first[k + 1] - first[k] + 63) / 64;
+    if (nc * 64 > 0xFFFFFFFFull) return fail(PVX_EINVAL, "too many rays in one call");
+    b->last_first.clear();
+    b->n_chunks = 0;
+    if (nc == 0) {
+        b->last_first.assign(first, first + ns + 1);
+        return PVX_OK;
+    }
+    // the pinned table may still be feeding the previous upload
+    HIP_TRY(hipEventSynchronize(b->uploaded));
+    int r;
+    if ((r = b->hchunks.reserve(nc * sizeof(uint4))) != PVX_OK) return r;
+    if ((r = b->chunks.reserve(nc * sizeof(uint4))) != PVX_OK) return r;
+    uint4 *h = static_cast<uint4 *>(b->hchunks.p);
+    uint64_t c = 0;
+    for (size_t k = 0; k < ns; k++)
+        for (uint64_t f = first[k]; f < first[k + 1]; f += 64)
+            h[c++] = make_uint4((uint32_t)k, (uint32_t)f, (uint32_t)std::min<uint64_t>(64, first[k + 1] - f), 0u);
+    HIP_TRY(hipMemcpyAsync(b->chunks.p, h, nc * sizeof(uint4), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipEventRecord(b->uploaded, st));
+    b->n_chunks = (uint32_t)nc;
+    b->last_first.assign(first, first + ns + 1);
+    return PVX_OK;
+}
+
+template <bool kCount>
+static int launch_batch(pvx_batch *b, const pvx_ray *rays, const uint64_t *first, const double *max_depth,
+                        uint32_t ray_kind, pvx_hit *hits, uint64_t *clocks, uint32_t *nv, uint32_t *pt,
+                        void *stream) {
+    if (!b) return fail(PVX_EINVAL, "null batch");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    int r;
+    if ((r = batch_chunks(b, first, st)) != PVX_OK) return r;
+    const uint32_t ns = (uint32_t)b->scenes.size();
+    auto *clk = reinterpret_cast<unsigned long long *>(clocks);
+    if (clk) {
+        k_clock_init<<<(ns + 1 + 255) / 256, 256, 0, st>>>(clk, ns + 1);
+        HIP_TRY(hipGetLastError());
+    }
+    if (b->n_chunks == 0) return PVX_OK;
+    if (!rays || !hits) return fail(PVX_EINVAL, "null ray or hit buffer");
+    const uint32_t n = b->n_chunks * 64u;
+    const bool flags = b->any_flags != 0 && ray_kind != PVX_RAY_PRIMARY;
+    const bool ww = b->level == 0 && ray_kind != PVX_RAY_SHADOW;
+    const uint32_t nblocks = std::min<uint32_t>(ww ? b->grid_ww : b->grid, (n + kBlock - 1) / kBlock);
+    const SceneView *views = static_cast<const SceneView *>(b->views.p);
+    const uint4 *chunks = static_cast<const uint4 *>(b->chunks.p);
+    int32_t *spill = static_cast<int32_t *>(b->spill.p);
+#define PVX_LAUNCH(L, F)                                                                                     \
+    k_trace_closest_multi<L, F, kCount><<<nblocks, kBlock, 0, st>>>(views, chunks, clk, rays, max_depth,     \
+                                                                    ray_kind, nullptr, n, hits, spill, nv, pt)
+#define PVX_LAUNCH_WW(F)                                                                                     \
+    k_trace_closest_ww_multi<F, kCount><<<nblocks, kBlock, 0, st>>>(views, chunks, clk, rays, max_depth,     \
+                                                                    ray_kind, nullptr, n, hits, spill, nv, pt)
+    if (b->level >= 2) {
+        if (flags) PVX_LAUNCH(2, true); else PVX_LAUNCH(2, false);
+    } else if (b->level == 1) {
+        if (flags) PVX_LAUNCH(1, true); else PVX_LAUNCH(1, false);
+    } else if (ww) {
+        if (flags) PVX_LAUNCH_WW(true); else PVX_LAUNCH_WW(false);
+    } else {
+        if (flags) PVX_LAUNCH(0, true); else PVX_LAUNCH(0, false);
+    }
+#undef PVX_LAUNCH
+#undef PVX_LAUNCH_WW
+    HIP_TRY(hipGetLastError());
+    return PVX_OK;
+}
+
+extern "C" int pvx_batch_trace_closest(pvx_batch *b, const pvx_ray *rays, const uint64_t *ray_first,
+                                       const double *max_depth, uint32_t ray_kind, pvx_hit *hits,
+                                       uint64_t *scene_clock, void *stream) {
+    return launch_batch<false>(b, rays, ray_first, max_depth, ray_kind, hits, scene_clock, nullptr, nullptr,
+                               stream);
+}
+
+extern "C" int pvx_batch_trace_closest_counted(pvx_batch *b, const pvx_ray *rays, const uint64_t *ray_first,
+                                               const double *max_depth, uint32_t ray_kind, pvx_hit *hits,
+                                               uint32_t *node_visits, uint32_t *prim_tests,
+                                               uint64_t *scene_clock, void *stream) {
+    if (!node_visits || !prim_tests) return fail(PVX_EINVAL, "null counter buffer");
+    return launch_batch<true>(b, rays, ray_first, max_depth, ray_kind, hits, scene_clock, node_visits, prim_tests,
+                              stream);
+}
+
+extern "C" int pvx_clock_rate(double *hz) {
+    if (!hz) return fail(PVX_EINVAL, "null argument");
+    int dev = 0, khz = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    if (khz <= 0) return fail(PVX_EDEVICE, "device reports no wall-clock rate");
+    *hz = 1000.0 * (double)khz;
+    return PVX_OK;
 }
 
 extern "C" int pvx_all_intersections(pvx_scene *s, const pvx_ray *rays, const uint32_t *obj,

@@ -56,6 +56,11 @@ def lib():
                                          vp]
         L.pvx_radiosity_render.argtypes = [vp, C.POINTER(A.PvxCamera), C.POINTER(A.PvxRadiositySettings),
                                            vp, C.POINTER(A.PvxRadiosityStats), vp]
+        L.pvx_batch_create.argtypes = [C.POINTER(vp), u32, C.POINTER(vp)]
+        L.pvx_batch_destroy.argtypes = [vp]
+        L.pvx_batch_trace_closest.argtypes = [vp, vp, vp, vp, u32, vp, vp, vp]
+        L.pvx_batch_trace_closest_counted.argtypes = [vp, vp, vp, vp, u32, vp, vp, vp, vp, vp]
+        L.pvx_clock_rate.argtypes = [C.POINTER(C.c_double)]
         fp = C.POINTER(C.c_float)
         L.pvx_surrogate_create.argtypes = [fp, C.c_uint64, C.POINTER(vp)]
         L.pvx_surrogate_destroy.argtypes = [vp]
@@ -75,6 +80,8 @@ EXPORTED_SYMBOLS = (
     "pvx_camera_rays", "pvx_copy", "pvx_scene_set_shading", "pvx_render", "pvx_render_samples",
     "pvx_radiosity_render", "pvx_surrogate_create", "pvx_surrogate_destroy",
     "pvx_surrogate_get_params", "pvx_surrogate_grad", "pvx_surrogate_train", "pvx_surrogate_infer",
+    "pvx_batch_create", "pvx_batch_destroy", "pvx_batch_trace_closest", "pvx_batch_trace_closest_counted",
+    "pvx_clock_rate",
 )
 
 
@@ -242,6 +249,85 @@ class GpuScene:
         _check(lib().pvx_inside(self.h, d_p.value, d_o.value, n, d_out.value, None), "pvx_inside")
         hip.synchronize()
         return d_out.download(np.uint8, n).astype(bool)
+
+
+class GpuBatch:
+    """Scenes traced together, one launch for the rays of all of them
+    (pvx_batch_*).  Keeps the GpuScenes alive."""
+
+    def __init__(self, scenes):
+        self.scenes = list(scenes)
+        arr = (C.c_void_p * len(self.scenes))(*[s.h.value for s in self.scenes])
+        h = C.c_void_p()
+        _check(lib().pvx_batch_create(arr, len(self.scenes), C.byref(h)), "pvx_batch_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().pvx_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _first(self, ray_first):
+        first = np.ascontiguousarray(ray_first, np.uint64)
+        if first.shape != (len(self.scenes) + 1,):
+            raise ValueError("ray_first needs n_scenes + 1 entries")
+        return first
+
+    # -- device-buffer API ----------------------------------------------------------
+    def trace_closest(self, rays, ray_first, hits, max_depth=None, ray_kind=A.PVX_RAY_IMAGE,
+                      scene_clock=None, stream=None):
+        first = self._first(ray_first)
+        _check(lib().pvx_batch_trace_closest(self.h, _ptr(rays), first.ctypes.data, _ptr(max_depth), ray_kind,
+                                             _ptr(hits), _ptr(scene_clock), stream.h if stream else None),
+               "pvx_batch_trace_closest")
+
+    def trace_closest_counted(self, rays, ray_first, hits, node_visits, prim_tests, max_depth=None,
+                              ray_kind=A.PVX_RAY_IMAGE, scene_clock=None, stream=None):
+        first = self._first(ray_first)
+        _check(lib().pvx_batch_trace_closest_counted(self.h, _ptr(rays), first.ctypes.data, _ptr(max_depth),
+                                                     ray_kind, _ptr(hits), _ptr(node_visits), _ptr(prim_tests),
+                                                     _ptr(scene_clock), stream.h if stream else None),
+               "pvx_batch_trace_closest_counted")
+
+    # -- numpy convenience ------------------------------------------------------------
+    def find_intersection(self, rays_per_scene, ray_kind=A.PVX_RAY_IMAGE, max_depth_per_scene=None,
+                          with_clock=False):
+        """rays_per_scene: one (n_s, 6) array per scene.  Returns the list of
+        hit arrays (and, with_clock, the n_scenes + 1 clock ticks)."""
+        parts = [np.ascontiguousarray(r, np.float64).reshape(-1, 6) for r in rays_per_scene]
+        if len(parts) != len(self.scenes):
+            raise ValueError("one ray array per scene")
+        counts = np.array([p.shape[0] for p in parts], np.uint64)
+        first = np.concatenate([[0], np.cumsum(counts)]).astype(np.uint64)
+        n = int(first[-1])
+        rays = np.concatenate(parts, 0) if n else np.zeros((0, 6))
+        d_rays = hip.DeviceBuffer.from_array(rays) if n else None
+        d_hits = hip.DeviceBuffer(max(n, 1) * 16)
+        d_md = None
+        if max_depth_per_scene is not None:
+            md = np.concatenate([np.asarray(m, np.float64).ravel() for m in max_depth_per_scene])
+            d_md = hip.DeviceBuffer.from_array(md) if n else None
+        d_clk = hip.DeviceBuffer(8 * (len(self.scenes) + 1)) if with_clock else None
+        self.trace_closest(d_rays, first, d_hits, d_md, ray_kind, d_clk)
+        hip.synchronize()
+        out = d_hits.download(np.uint8, n * 16).view(HIT_DTYPE).copy() if n else np.zeros(0, HIT_DTYPE)
+        res = [out[int(first[k]):int(first[k + 1])] for k in range(len(self.scenes))]
+        if with_clock:
+            return res, d_clk.download(np.uint64, len(self.scenes) + 1)
+        return res
+
+
+def clock_rate() -> float:
+    """Ticks per second of the device wall clock (scene_clock units)."""
+    hz = C.c_double()
+    _check(lib().pvx_clock_rate(C.byref(hz)), "pvx_clock_rate")
+    return hz.value
 
 
 class Surrogate:

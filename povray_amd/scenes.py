@@ -610,3 +610,85 @@ def radiosity_scene(seed: int = 0xC0FFEE + 3, n_boxes: int = 5000, width=1280, h
                                      adc_bailout=0.01, probe_threshold=0.02, train_samples=1024,
                                      train_steps=200, probe_tile=64, probes_per_tile=1)),
     )
+
+
+# --------------------------------------------------------------------------
+# multi-scene batch
+# --------------------------------------------------------------------------
+def terrain_mesh(n_tris: int, seed: int, half: float = 10.0, height: float = 1.5):
+    """Heightfield over [-half, half]^2 (y up): a grid of nx x nz cells, two
+    triangles each (nx * nz * 2 ~ n_tris, at least 2), heights from 4-octave
+    value noise, smoothed vertex normals.  Returns (verts f32, idx i32,
+    normals f32)."""
+    cells = max(1, int(round(n_tris / 2)))
+    nx = max(1, int(round(np.sqrt(cells))))
+    nz = max(1, int(round(cells / nx)))
+    xs = np.linspace(-half, half, nx + 1)
+    zs = np.linspace(-half, half, nz + 1)
+    gx, gz = np.meshgrid(xs, zs, indexing="ij")
+    p = np.stack([gx.ravel(), np.zeros(gx.size), gz.ravel()], 1)
+    y = value_noise(p * (1.0 / half), seed, octaves=4, base_freq=1.5) * height
+    v = np.stack([p[:, 0], y, p[:, 2]], 1)
+    i = np.arange(nx)[:, None] * (nz + 1) + np.arange(nz)[None, :]
+    a, b, c, d = i.ravel(), i.ravel() + (nz + 1), i.ravel() + 1, i.ravel() + nz + 2
+    # counter-clockwise seen from above (normals +y)
+    f = np.concatenate([np.stack([a, c, b], 1), np.stack([c, d, b], 1)], 0).astype(np.int32)
+    v32 = v.astype(np.float32)
+    n32 = vertex_normals(v32.astype(np.float64), f).astype(np.float32)
+    return v32, f, n32
+
+
+def batch_scene(seed: int, n_tris: int, n_spheres: int, width: int, height: int, name: str = None) -> Scene:
+    """One scene of the multi-scene batch: a terrain mesh of about `n_tris`
+    triangles plus `n_spheres` spheres above it (centres U[-9,9] x U[0.3,4] x
+    U[-9,9], radii U[0.1,0.6]), its own camera and one point light."""
+    r = Rng(seed)
+    v, f, nrm = terrain_mesh(n_tris, seed ^ 0x5EED)
+    centre = np.stack([r.uniform(n_spheres, -9.0, 9.0), r.uniform(n_spheres, 0.3, 4.0),
+                       r.uniform(n_spheres, -9.0, 9.0)], 1) if n_spheres else np.zeros((0, 3))
+    radius = r.uniform(n_spheres, 0.1, 0.6) if n_spheres else np.zeros(0)
+    n = 1 + n_spheres
+    objects = np.zeros((n, 4), np.uint32)
+    objects[0, 0] = A.PVX_MESH
+    objects[1:, 0] = A.PVX_SPHERE
+    objects[1:, 1] = np.arange(n_spheres)
+    ang = r.uniform(1, 0.0, 2.0 * np.pi)[0]
+    loc = (13.0 * np.sin(ang), r.uniform(1, 5.0, 9.0)[0], -13.0 * np.cos(ang))
+    alb = r.uniform(3 * n, 0.2, 0.9).reshape(n, 3)
+    refl = np.where(r.uniform(n) < 0.1, 0.3, 0.0)
+    refl[0] = 0.0
+    return Scene(
+        name=name or f"batch_{n_tris}t_{n_spheres}s_{width}x{height}", objects=objects,
+        sphere_center=centre if n_spheres else None, sphere_radius=radius if n_spheres else None,
+        meshes=[(v, f, nrm)],
+        camera=look_at_camera(loc, (0.0, 0.0, 0.0), width, height),
+        lights=[dict(kind="point", position=np.array([20.0, 30.0, -20.0]), colour=np.ones(3))],
+        materials=dict(pigment=alb, diffuse=np.full(n, 0.6), ambient=np.full(n, 0.1),
+                       phong=np.full(n, 0.4), phong_size=np.full(n, 40.0), reflection=refl,
+                       ior=np.ones(n), transmit=np.zeros(n)),
+        settings=dict(max_trace_level=5, adc_bailout=1.0 / 255.0),
+    )
+
+
+def batch_plan(seed: int = 0xC0FFEE + 4, n_scenes: int = 64, tris=(1000, 500000), max_spheres: int = 2000,
+               res_lo=(256, 256), res_hi=(1920, 1080)):
+    """Sizes of the multi-scene batch: per scene (seed, triangles drawn
+    log-uniform in `tris`, spheres uniform in 0..max_spheres, width and height
+    uniform in [res_lo, res_hi])."""
+    r = Rng(seed)
+    lt = r.uniform(n_scenes, np.log(tris[0]), np.log(tris[1]))
+    ns = np.minimum((r.uniform(n_scenes) * (max_spheres + 1)).astype(np.int64), max_spheres)
+    w = np.rint(r.uniform(n_scenes, res_lo[0], res_hi[0])).astype(np.int64)
+    h = np.rint(r.uniform(n_scenes, res_lo[1], res_hi[1])).astype(np.int64)
+    seeds = (r.bits(n_scenes) >> np.uint64(1)).astype(np.int64)
+    return [dict(seed=int(seeds[k]), n_tris=int(round(np.exp(lt[k]))), n_spheres=int(ns[k]),
+                 width=int(w[k]), height=int(h[k])) for k in range(n_scenes)]
+
+
+def batch_scenes(plan=None, **kw):
+    """The multi-scene batch: 64 seeded scenes of uneven size (terrain meshes
+    of 1k to 500k triangles drawn log-uniform, 0 to 2k spheres, resolutions
+    from 256x256 to 1920x1080)."""
+    plan = plan if plan is not None else batch_plan(**kw)
+    return [batch_scene(p["seed"], p["n_tris"], p["n_spheres"], p["width"], p["height"],
+                        name=f"batch{k:02d}") for k, p in enumerate(plan)]

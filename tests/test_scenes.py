@@ -74,3 +74,33 @@ def test_scene_desc_points_at_the_arrays():
     assert d.n_objects == 7 and d.n_spheres == 7 and d.n_meshes == 0
     assert d.sphere_radius[3] == s.sphere_radius[3]
     assert d.sphere_center[3 * 2 + 1] == s.sphere_center[2, 1]
+
+
+def test_batch_plan_ranges_and_determinism():
+    plan = scenes.batch_plan()
+    assert len(plan) == 64 and plan == scenes.batch_plan()
+    tris = np.array([p["n_tris"] for p in plan])
+    assert tris.min() >= 1000 and tris.max() <= 500000
+    # log-uniform: about half of the draws below the geometric mean
+    assert 20 <= int(np.sum(tris < np.sqrt(1000 * 500000))) <= 44
+    sph = np.array([p["n_spheres"] for p in plan])
+    assert sph.min() >= 0 and sph.max() <= 2000
+    w = np.array([p["width"] for p in plan])
+    h = np.array([p["height"] for p in plan])
+    assert w.min() >= 256 and w.max() <= 1920 and h.min() >= 256 and h.max() <= 1080
+    assert len({p["seed"] for p in plan}) == 64
+
+
+def test_terrain_mesh_counts_and_winding():
+    for n in (1000, 4321, 20000):
+        v, f, nrm = scenes.terrain_mesh(n, seed=5)
+        assert abs(f.shape[0] - n) <= 0.02 * n + 4
+        assert f.min() == 0 and f.max() == v.shape[0] - 1
+        fn = np.cross(v[f[:, 1]] - v[f[:, 0]], v[f[:, 2]] - v[f[:, 0]])
+        assert np.all(fn[:, 1] > 0)  # faces point up
+        assert np.all(nrm[:, 1] > 0) and np.allclose(np.linalg.norm(nrm, axis=1), 1.0, atol=1e-6)
+    s = scenes.batch_scene(seed=3, n_tris=2000, n_spheres=17, width=300, height=260)
+    assert s.n_objects == 18 and s.objects[0, 0] == A.PVX_MESH
+    assert np.all(s.objects[1:, 0] == A.PVX_SPHERE) and s.camera["width"] == 300
+    s0 = scenes.batch_scene(seed=3, n_tris=2000, n_spheres=0, width=300, height=260)
+    assert s0.n_objects == 1 and s0.sphere_center is None
