@@ -402,8 +402,8 @@ typedef struct pvx_radiosity_settings {
     float probe_threshold;     /* learned: max mean probe error per tile in     
                                 * displayed colour, diffuse reflectance x
                                 * irradiance error (0.02)                      */
-    uint32_t train_samples;    /* learned: plain gathers taken as training data */
-    uint32_t train_steps;      /* learned: optimiser steps                      */
+    uint32_t train_samples;    /* learned: plain training gathers per network   */
+    uint32_t train_steps;      /* learned: optimiser steps per network          */
     uint32_t probe_tile;       /* learned: probe tile edge in pixels (32)       */
     uint32_t probes_per_tile;  /* learned: plain gathers probed per tile (2)    */
 } pvx_radiosity_settings;
@@ -429,9 +429,13 @@ typedef struct pvx_radiosity_stats {
  * (RadiosityFunction::ComputeAmbient / GatherLight, RadiosityCache
  * FindReusableBlock / AverageNearBlock / AddBlock,
  * core/lighting/radiosity.cpp:385-743, 1194-1769).  Surfaces:
- * pigment * (emission + diffuse * irradiance).  In LEARNED mode the
- * final-pass misses are answered by the MLP irradiance surrogate, trained on
- * this frame's plain gathers, except in tiles whose probes disagree. */
+ * pigment * (emission + diffuse * irradiance).  LEARNED mode has no
+ * pretrace: one batch of spread-out plain gathers (train_samples at depth 0
+ * and as many at depth 1) trains a depth-1 and a depth-0 MLP irradiance
+ * surrogate; the final pass reuses this frame's records where they reach and
+ * takes the depth-0 network's answer at the misses, except in tiles whose
+ * held-out probe gathers disagree (probe_threshold), which fall back to the
+ * plain gather. */
 int pvx_radiosity_render(pvx_scene *scene, const pvx_camera *cam,
                          const pvx_radiosity_settings *settings, float *rgb,
                          pvx_radiosity_stats *stats, void *stream);

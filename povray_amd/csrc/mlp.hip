@@ -9,7 +9,7 @@
 // which is exactly the B operand the next layer's K step q needs — so the
 // activations stay in registers from layer to layer, forward and backward.
 // Weight gradients (sums over points) are separate GEMMs over the stored
-// activations, split over chunks of 256 points and reduced in fixed order
+// activations, split over chunks of 128 points and reduced in fixed order
 // (deterministic), followed by Adam on the FP32 master weights, which also
 // rewrites the BF16 copies the kernels read.
 #include <hip/hip_runtime.h>
@@ -167,9 +167,21 @@ __device__ inline void relu_back(const f32x16 &a0, const f32x16 &a1, const bf16x
     }
 }
 
+// Features of n points, [point][192] BF16 (the B-operand order of feat4),
+// computed once per training set instead of once per step.
+__global__ void k_mlp_features(const float *__restrict__ x, uint32_t n, uint16_t *__restrict__ F) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n * 48u) return;
+    const uint32_t pt = i / 48u, sh = i % 48u, s = sh >> 1, h = sh & 1u;
+    float xv[6];
+    for (int c = 0; c < 6; c++) xv[c] = x[6 * (size_t)pt + c];
+    *reinterpret_cast<bf16x4 *>(F + (size_t)pt * kMlpIn + 8 * s + 4 * h) = feat4(xv, (int)s, (int)h);
+}
+
 struct TrainArgs {
     const uint8_t *pack;
     const float *x; // n x 6
+    const uint16_t *F; // n x 192 precomputed features (or null: computed from x)
     const float *t; // n x 3
     uint32_t n;
     uint32_t batch;  // multiple of kMlpChunk
@@ -197,13 +209,22 @@ __global__ __launch_bounds__(256) void k_mlp_fwd_bwd(TrainArgs a) {
         valid = true;
     }
     float x[6];
-    for (int c = 0; c < 6; c++) x[c] = valid ? a.x[6 * (size_t)src + c] : 0.0f;
+    if (!a.F)
+        for (int c = 0; c < 6; c++) x[c] = valid ? a.x[6 * (size_t)src + c] : 0.0f;
+    const uint16_t *Fp = a.F ? a.F + (size_t)(valid ? src : 0) * kMlpIn : nullptr;
     uint16_t *A = a.acts;
     const uint32_t B = a.batch;
-    // layer 1 (features computed in place)
+    // layer 1 (features loaded, or computed in place)
     f32x16 a0 = zero16(), a1 = zero16();
     for (int s = 0; s < 24; s++) {
-        bf16x4 b = feat4(x, s, h);
+        bf16x4 b;
+        if (Fp) {
+            b = *reinterpret_cast<const bf16x4 *>(Fp + 8 * s + 4 * h);
+            if (!valid)
+                for (int t = 0; t < 4; t++) b[t] = 0;
+        } else {
+            b = feat4(x, s, h);
+        }
         for (int t = 0; t < 4; t++) A[(size_t)(kRowX0 + 8 * s + 4 * h + t) * B + slot] = (uint16_t)b[t];
         a0 = mfma(lds4(sw, kOffP_W1T + p * kP1 + 8 * s + 4 * h), b, a0);
         a1 = mfma(lds4(sw, kOffP_W1T + (32 + p) * kP1 + 8 * s + 4 * h), b, a1);
@@ -256,7 +277,7 @@ __global__ __launch_bounds__(256) void k_mlp_fwd_bwd(TrainArgs a) {
 }
 
 // Weight-gradient GEMMs: dW[in][out] = sum_p X[in][p] dZ[out][p] over one
-// chunk of 256 points per wavefront; bias gradients ride along as a
+// chunk of 128 points per wavefront; bias gradients ride along as a
 // ones-row A operand on the m-block-0 tiles.  Partial sums per chunk.
 __global__ __launch_bounds__(256) void k_mlp_wgrad(const uint16_t *__restrict__ acts, uint32_t batch,
                                                    uint32_t nchunks, float *__restrict__ partial) {
@@ -459,13 +480,14 @@ int Surrogate::ensure_acts(size_t batch) {
     return PVX_OK;
 }
 
-int Surrogate::fwd_bwd(const float *x, const float *t, uint32_t n, uint32_t batch, bool sample, uint32_t sseed,
-                       hipStream_t st) {
+int Surrogate::fwd_bwd(const float *x, const uint16_t *F, const float *t, uint32_t n, uint32_t batch, bool sample,
+                       uint32_t sseed, hipStream_t st) {
     int r;
     if ((r = ensure_acts(batch))) return r;
     TrainArgs a;
     a.pack = static_cast<const uint8_t *>(pack.p);
     a.x = x;
+    a.F = F;
     a.t = t;
     a.n = n;
     a.batch = batch;
@@ -486,7 +508,7 @@ int Surrogate::grad_batch(const float *x, const float *t, uint32_t n, float *los
                           hipStream_t st) {
     const uint32_t batch = (uint32_t)(((size_t)n + kMlpChunk - 1) / kMlpChunk * kMlpChunk);
     int r;
-    if ((r = fwd_bwd(x, t, n, batch, false, 0, st))) return r;
+    if ((r = fwd_bwd(x, nullptr, t, n, batch, false, 0, st))) return r;
     k_mlp_update<<<nblocks(kMlpParams), 256, 0, st>>>(kUpdGrad, static_cast<const float *>(partial.p),
                                                       batch / kMlpChunk, static_cast<float *>(w.p), nullptr,
                                                       nullptr, static_cast<float *>(grad.p), nullptr, 0, 1, 1);
@@ -504,9 +526,12 @@ int Surrogate::grad_batch(const float *x, const float *t, uint32_t n, float *los
 int Surrogate::train(const float *x, const float *t, uint32_t n, uint32_t steps, uint32_t batch, float lr,
                      float *final_loss, hipStream_t st) {
     int r;
+    if ((r = feats.reserve((size_t)n * kMlpIn * 2))) return r;
+    k_mlp_features<<<nblocks((size_t)n * 48), 256, 0, st>>>(x, n, static_cast<uint16_t *>(feats.p));
+    HIP_TRY(hipGetLastError());
     for (uint32_t k = 0; k < steps; k++) {
         const uint32_t sseed = (uint32_t)(seed * 0x9E3779B97F4A7C15ull >> 32) + 7919u * (step + 1);
-        if ((r = fwd_bwd(x, t, n, batch, true, sseed, st))) return r;
+        if ((r = fwd_bwd(x, static_cast<const uint16_t *>(feats.p), t, n, batch, true, sseed, st))) return r;
         step++;
         // exponential decay to a tenth of the rate over the run
         float lr_k = lr * std::pow(0.1f, (float)k / (float)std::max<uint32_t>(1, steps));
@@ -584,7 +609,7 @@ extern "C" int pvx_surrogate_train(pvx_surrogate *s, const float *x, const float
                                    uint32_t steps, uint32_t batch, float lr, float *final_loss, void *stream) {
     if (!s || !x || !target) return fail(PVX_EINVAL, "null argument");
     if (n == 0 || n > (1u << 30)) return fail(PVX_EINVAL, "n must be in [1, 2^30]");
-    if (batch == 0 || batch % kMlpChunk || batch > (1u << 20))
+    if (batch == 0 || batch % 256 || batch > (1u << 20))
         return fail(PVX_EINVAL, "batch must be a multiple of 256 up to 2^20");
     if (!(lr > 0.0f)) return fail(PVX_EINVAL, "lr must be positive");
     return s->net.train(x, target, (uint32_t)n, steps, batch, lr, final_loss, static_cast<hipStream_t>(stream));

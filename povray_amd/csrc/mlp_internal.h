@@ -30,14 +30,14 @@ constexpr int kOffW4 = kOffB3 + kMlpHid;
 constexpr int kOffB4 = kOffW4 + kMlpHid * kMlpOut;
 constexpr int kMlpParams = kOffB4 + kMlpOut; // 20867
 
-constexpr int kMlpChunk = 256; // points per weight-gradient partial sum
+constexpr int kMlpChunk = 128; // points per weight-gradient partial sum
 
 struct Surrogate {
     uint64_t seed = 0;
     uint32_t step = 0;        // Adam steps taken
     DevBuf w, m, v;           // FP32 master weights and Adam moments
     DevBuf pack;              // BF16 weight copies laid out for the kernels + FP32 biases
-    DevBuf acts, partial, loss_part, grad;
+    DevBuf acts, partial, loss_part, grad, feats;
     size_t acts_batch = 0;
 
     int init(const float *host_params, uint64_t seed, hipStream_t st);
@@ -50,8 +50,8 @@ struct Surrogate {
     int get_params(float *host, hipStream_t st);
 
   private:
-    int fwd_bwd(const float *x, const float *t, uint32_t n, uint32_t batch, bool sample, uint32_t sample_seed,
-                hipStream_t st);
+    int fwd_bwd(const float *x, const uint16_t *F, const float *t, uint32_t n, uint32_t batch, bool sample,
+                uint32_t sample_seed, hipStream_t st);
     int ensure_acts(size_t batch);
 };
 

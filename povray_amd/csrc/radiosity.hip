@@ -17,6 +17,7 @@
 // pass).  A query keeps running sums (sum w*E, sum w), so after a round only
 // the round's new records (their own small grid) are visited again.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <chrono>
@@ -56,12 +57,20 @@ __device__ inline float grey3(const float *c) { return 0.297f * c[0] + 0.589f * 
 // FindReusableBlock + AverageNearBlock for a batch of queries: adds the
 // weighted illuminance of every record of the grid that qualifies to the
 // query's running sums (or replaces them when `accumulate` is 0).
+// (Queries with mask[i] == 0 are unset: their sums are cleared.)
 __global__ void k_lookup(const RadQuery *__restrict__ q, const uint32_t *__restrict__ idx, uint32_t n,
                          CacheView c, float eb, float factor, int sigmoid, int accumulate,
-                         RadAcc *__restrict__ acc) {
+                         RadAcc *__restrict__ acc, const uint8_t *__restrict__ mask) {
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const uint32_t i = idx ? idx[t] : t;
+    if (mask && !mask[i]) {
+        RadAcc z;
+        z.e[0] = z.e[1] = z.e[2] = 0.0;
+        z.w = 0.0;
+        acc[i] = z;
+        return;
+    }
     const RadQuery qq = q[i];
     double bound = (double)eb;
     if (qq.w < kWeightErrOffset) bound += (double)(kWeightErrOffset - qq.w);
@@ -397,7 +406,7 @@ __global__ void k_rec_to_sample(const RadRec *__restrict__ recs, const uint32_t 
                                 float *__restrict__ t) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const RadRec b = recs[ridx[i]];
+    const RadRec b = recs[ridx ? ridx[i] : i];
     x6[6 * (size_t)i + 0] = fminf(fmaxf((b.p[0] - lx) * inv, 0.0f), 1.0f);
     x6[6 * (size_t)i + 1] = fminf(fmaxf((b.p[1] - ly) * inv, 0.0f), 1.0f);
     x6[6 * (size_t)i + 2] = fminf(fmaxf((b.p[2] - lz) * inv, 0.0f), 1.0f);
@@ -407,11 +416,18 @@ __global__ void k_rec_to_sample(const RadRec *__restrict__ recs, const uint32_t 
     }
 }
 
-__global__ void k_query_to_x6(const RadQuery *__restrict__ q, const uint32_t *__restrict__ idx, uint32_t m,
-                              float lx, float ly, float lz, float inv, float *__restrict__ x6) {
+// (idx may be null: queries 0..m-1; where mask[s] is 0 the query is unset
+// and the inputs are zero.)
+__global__ void k_query_to_x6(const RadQuery *__restrict__ q, const uint32_t *__restrict__ idx,
+                              const uint8_t *__restrict__ mask, uint32_t m, float lx, float ly, float lz,
+                              float inv, float *__restrict__ x6) {
     uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= m) return;
-    const RadQuery a = q[idx[s]];
+    if (mask && !mask[s]) {
+        for (int c = 0; c < 6; c++) x6[6 * (size_t)s + c] = 0.0f;
+        return;
+    }
+    const RadQuery a = q[idx ? idx[s] : s];
     x6[6 * (size_t)s + 0] = fminf(fmaxf(((float)a.p[0] - lx) * inv, 0.0f), 1.0f);
     x6[6 * (size_t)s + 1] = fminf(fmaxf(((float)a.p[1] - ly) * inv, 0.0f), 1.0f);
     x6[6 * (size_t)s + 2] = fminf(fmaxf(((float)a.p[2] - lz) * inv, 0.0f), 1.0f);
@@ -426,6 +442,160 @@ __global__ void k_apply_pred(const uint32_t *__restrict__ idx, uint32_t m, const
     uint32_t i = idx[s];
     for (int c = 0; c < 3; c++) E[3 * (size_t)i + c] = fmaxf(y[3 * (size_t)s + c], 0.0f);
     acc[i].w = -1.0;
+}
+
+__global__ void k_fill_f32(float *__restrict__ p, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0f;
+}
+
+__global__ void k_acc_zero(RadAcc *__restrict__ acc, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    RadAcc a;
+    a.e[0] = a.e[1] = a.e[2] = 0.0;
+    a.w = 0.0;
+    acc[i] = a;
+}
+
+// depth+1 irradiance of the gather hits that want it, from the network
+// (what resolve(depth + 1) would return: max(y, 0) x brightness).
+__global__ void k_amb_from_pred(const uint8_t *__restrict__ need, const float *__restrict__ y, uint32_t n,
+                                float brightness, float *__restrict__ amb) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int c = 0; c < 3; c++)
+        amb[3 * (size_t)i + c] = need[i] ? fmaxf(y[3 * (size_t)i + c], 0.0f) * brightness : 0.0f;
+}
+
+__device__ inline uint32_t dmix32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+__global__ void k_fill_u64(unsigned long long *__restrict__ p, size_t n, unsigned long long v) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_u8(uint8_t *__restrict__ p, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// Training points: per cell of cpx x cpx pixels, the query with the lowest
+// pixel hash (key = hash << 32 | query; the minimum does not depend on the
+// order of the atomics).
+// atomicMin of (slot, key) per lane, combined first across the lanes of the
+// wavefront that share lane 0's slot (consecutive queries mostly share a
+// cell or tile), so a hot slot takes one atomic per wavefront.
+__device__ inline void wave_min_into(unsigned long long *keys, uint32_t slot, unsigned long long key, bool active) {
+    const uint32_t s0 = __shfl(slot, 0, 64);
+    const bool act0 = __shfl((int)active, 0, 64) != 0;
+    const bool same = active && act0 && slot == s0;
+    unsigned long long v = same ? key : ~0ull;
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(v, off, 64);
+        v = o < v ? o : v;
+    }
+    if ((threadIdx.x & 63) == 0 && v != ~0ull) atomicMin(&keys[s0], v);
+    if (active && !same) atomicMin(&keys[slot], key);
+}
+
+__global__ void k_cell_keys(const uint32_t *__restrict__ pix, uint32_t n, uint32_t W, uint32_t cpx, uint32_t ncx,
+                            unsigned long long *__restrict__ keys) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = k < n;
+    uint32_t c = 0;
+    unsigned long long key = ~0ull;
+    if (active) {
+        const uint32_t p = pix[k];
+        c = (p % W) / cpx + ((p / W) / cpx) * ncx;
+        key = ((unsigned long long)dmix32(p) << 32) | k;
+    }
+    wave_min_into(keys, c, key, active);
+}
+
+// Probes: per probe tile, the not-yet-chosen query with the lowest salted hash.
+__global__ void k_tile_keys(const uint32_t *__restrict__ pix, const uint8_t *__restrict__ role, uint32_t n,
+                            uint32_t W, uint32_t tsz, uint32_t tx, uint32_t salt,
+                            unsigned long long *__restrict__ keys) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = k < n && !role[k];
+    uint32_t t = 0;
+    unsigned long long key = ~0ull;
+    if (active) {
+        const uint32_t p = pix[k];
+        t = (p % W) / tsz + ((p / W) / tsz) * tx;
+        key = ((unsigned long long)dmix32(p ^ salt) << 32) | k;
+    }
+    wave_min_into(keys, t, key, active);
+}
+
+__global__ void k_mark_role(const uint32_t *__restrict__ idx, uint32_t m, uint8_t v, uint8_t *__restrict__ role) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < m) role[idx[s]] = v;
+}
+
+__global__ void k_acc_reset(const uint32_t *__restrict__ idx, uint32_t m, RadAcc *__restrict__ acc) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= m) return;
+    RadAcc z;
+    z.e[0] = z.e[1] = z.e[2] = 0.0;
+    z.w = 0.0;
+    acc[idx[s]] = z;
+}
+
+// Final-pass misses (no record reaches them, not gathered): the network's
+// answer, or a fallback flag in tiles whose probes failed.  counters[0] /
+// [1] count the two.
+__global__ void k_learned_apply(RadAcc *__restrict__ acc, const uint32_t *__restrict__ pix, uint32_t n, uint32_t W,
+                                uint32_t tsz, uint32_t tx, const uint8_t *__restrict__ fb_tile,
+                                const float *__restrict__ y, float *__restrict__ E, uint8_t *__restrict__ fb,
+                                uint32_t *__restrict__ counters) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    int kind = 0; // 1: network, 2: fallback
+    if (k < n) {
+        fb[k] = 0;
+        if (acc[k].w == 0.0) { // neither reused (> 0) nor gathered (< 0)
+            const uint32_t p = pix[k];
+            const size_t t = (p % W) / tsz + (size_t)((p / W) / tsz) * tx;
+            if (fb_tile[t]) {
+                fb[k] = 1;
+                kind = 2;
+            } else {
+                for (int c = 0; c < 3; c++) E[3 * (size_t)k + c] = fmaxf(y[3 * (size_t)k + c], 0.0f);
+                acc[k].w = -1.0;
+                kind = 1;
+            }
+        }
+    }
+    const unsigned long long b1 = __ballot(kind == 1), b2 = __ballot(kind == 2);
+    if ((threadIdx.x & 63) == 0) {
+        if (b1) atomicAdd(&counters[0], (uint32_t)__popcll(b1));
+        if (b2) atomicAdd(&counters[1], (uint32_t)__popcll(b2));
+    }
+}
+
+__global__ void k_gather_u32(const uint32_t *__restrict__ src, const uint32_t *__restrict__ idx, uint32_t m,
+                             uint32_t *__restrict__ out) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < m) out[s] = src[idx[s]];
+}
+
+// rank[sel[s]] = s: each pixel's query slot
+__global__ void k_scatter_rank(const uint32_t *__restrict__ sel, uint32_t m, uint32_t *__restrict__ rank) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < m) rank[sel[s]] = s;
+}
+
+__global__ void k_fill_u32(uint32_t *__restrict__ p, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
 }
 
 inline uint32_t blocks(size_t n, uint32_t b = 256) { return (uint32_t)((n + b - 1) / b); }
@@ -592,6 +762,7 @@ int RadiosityEngine::begin(const pvx_radiosity_settings &set, const pvx_camera &
         stats_rounds[d] = 0;
     }
     stats = pvx_radiosity_stats{};
+    amb_net = nullptr;
     train_ms = t_sync_ms = t_grid_ms = 0.0;
     n_sync = 0;
     up_off = 0;
@@ -666,7 +837,7 @@ float RadiosityEngine::reuse_count(int d) const {
 }
 
 int RadiosityEngine::lookup(int depth, const RadGrid &g, const RadQuery *q, const uint32_t *idx, uint32_t n,
-                            bool final, bool accumulate, RadAcc *acc) {
+                            bool final, bool accumulate, RadAcc *acc, const uint8_t *mask) {
     if (n == 0) return PVX_OK;
     if (accumulate && g.n == 0) return PVX_OK;
     RadCache &c = caches[depth];
@@ -681,7 +852,7 @@ int RadiosityEngine::lookup(int depth, const RadGrid &g, const RadQuery *q, cons
     v.cell = g.cell;
     v.n = g.n;
     k_lookup<<<blocks(n), 256, 0, stream>>>(q, idx, n, v, rs.error_bound, std::pow(2.0f, (float)depth),
-                                            (final && depth == 0) ? 1 : 0, accumulate ? 1 : 0, acc);
+                                            (final && depth == 0) ? 1 : 0, accumulate ? 1 : 0, acc, mask);
     HIP_TRY(hipGetLastError());
     return PVX_OK;
 }
@@ -691,6 +862,14 @@ int RadiosityEngine::lookup(int depth, const RadGrid &g, const RadQuery *q, cons
 int RadiosityEngine::gather(int depth, const RadQuery *q, const uint32_t *idx, uint32_t m, bool final,
                             float *Eg) {
     if (m == 0) return PVX_OK;
+    int r;
+    if ((r = gather_trace(depth, q, idx, m))) return r;
+    return gather_finish(depth, q, idx, m, final, Eg);
+}
+
+// First half of a gather: the rays, their closest hits and what each hit
+// contributes (scratch[depth]: base, coef and the depth+1 queries).
+int RadiosityEngine::gather_trace(int depth, const RadQuery *q, const uint32_t *idx, uint32_t m) {
     const uint32_t N = (uint32_t)rays_per_sample(depth);
     const size_t nr = (size_t)m * N;
     Scratch &S = scratch[depth];
@@ -724,8 +903,38 @@ int RadiosityEngine::gather(int depth, const RadQuery *q, const uint32_t *idx, u
                                                   static_cast<RadQuery *>(S.q.p),
                                                   static_cast<uint8_t *>(S.need.p));
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemsetAsync(S.amb.p, 0, nr * 12, stream));
-    if (deeper) {
+    return PVX_OK;
+}
+
+// Second half: the depth+1 irradiance of the hits (resolved with the
+// depth+1 cache, or from `amb_net` at depth 0), GatherLight's reduction,
+// and the new records (AddBlock).
+int RadiosityEngine::gather_finish(int depth, const RadQuery *q, const uint32_t *idx, uint32_t m, bool final,
+                                   float *Eg) {
+    const uint32_t N = (uint32_t)rays_per_sample(depth);
+    const size_t nr = (size_t)m * N;
+    Scratch &S = scratch[depth];
+    int r;
+    bool deeper = depth + 1 < (int)rs.recursion_limit;
+    k_fill_f32<<<blocks(nr * 3), 256, 0, stream>>>(static_cast<float *>(S.amb.p), nr * 3);
+    HIP_TRY(hipGetLastError());
+    if (deeper && depth == 0 && amb_net) {
+        // depth+1 irradiance of every hit from the network (reusing this
+        // frame's sparse depth+1 records as well measured no better, at the
+        // cost of a lookup over scattered hits)
+        if ((r = ax.reserve(nr * 24)) || (r = ay.reserve(nr * 12))) return r;
+        k_query_to_x6<<<blocks(nr), 256, 0, stream>>>(static_cast<const RadQuery *>(S.q.p), nullptr,
+                                                      static_cast<const uint8_t *>(S.need.p), (uint32_t)nr,
+                                                      nlo[0], nlo[1], nlo[2], ninv, static_cast<float *>(ax.p));
+        HIP_TRY(hipGetLastError());
+        if ((r = amb_net->infer(static_cast<const float *>(ax.p), (uint32_t)nr, static_cast<float *>(ay.p),
+                                stream)))
+            return r;
+        k_amb_from_pred<<<blocks(nr), 256, 0, stream>>>(static_cast<const uint8_t *>(S.need.p),
+                                                        static_cast<const float *>(ay.p), (uint32_t)nr,
+                                                        rs.brightness, static_cast<float *>(S.amb.p));
+        HIP_TRY(hipGetLastError());
+    } else if (deeper) {
         // compact the hits that need irradiance at depth + 1 (deterministic order)
         std::vector<uint8_t> need(nr);
         if ((r = fetch(need.data(), S.need.p, nr))) return r;
@@ -759,7 +968,8 @@ int RadiosityEngine::gather(int depth, const RadQuery *q, const uint32_t *idx, u
         rs.minimum_reuse, rs.maximum_reuse, factor, Eg, static_cast<RadRec *>(S.rec.p));
     HIP_TRY(hipGetLastError());
     // append the records to the cache (AddBlock)
-    std::vector<RadRec> rec(m);
+    std::vector<RadRec> &rec = last_fetched[depth];
+    rec.resize(m);
     if ((r = fetch(rec.data(), S.rec.p, m * sizeof(RadRec)))) return r;
     RadCache &c = caches[depth];
     std::vector<int32_t> &lr = last_rec[depth];
@@ -954,129 +1164,27 @@ int RadiosityEngine::resolve(int depth, const RadQuery *q, uint32_t n, bool fina
     return PVX_OK;
 }
 
-// Final-pass irradiance with the surrogate (depth 0).  The points no record
-// reaches get one round of plain gathers at (1) a spread-out subset, whose
-// records join the pretrace records as training samples, and (2) probe
-// points, a few per image tile, held out to check the trained network.
-// What the round's records still do not reach is answered by the network,
-// except in tiles whose probes disagree by more than the threshold; those
-// fall back to plain gathers (at most two more rounds).
-int RadiosityEngine::resolve_learned(const RadQuery *q, uint32_t n, const std::vector<uint32_t> &pix, float *E) {
+// Trains `nt` from scratch on n records (device), inputs normalised by
+// (nlo, ninv); adds the time to train_ms.
+int RadiosityEngine::train_on(Surrogate &nt, const RadRec *recs, uint32_t n, uint64_t seed, float *loss) {
     if (n == 0) return PVX_OK;
-    RadCache &c = caches[0];
-    RadAcc *acc = nullptr;
-    std::vector<uint32_t> todo, slot;
-    std::vector<RadQuery> hq;
     int r;
-    if ((r = start_resolve(0, q, n, true, acc, todo, slot, hq))) return r;
-    if (todo.empty()) {
-        k_finish<<<blocks(n), 256, 0, stream>>>(acc, n, rs.brightness, E);
-        HIP_TRY(hipGetLastError());
-        return PVX_OK;
-    }
-    // image tiles of the pending points
-    const uint32_t tsz = std::max<uint32_t>(1, rs.probe_tile);
-    const uint32_t tx = (cam.width + tsz - 1) / tsz, ty = (cam.height + tsz - 1) / tsz;
-    auto tile_of = [&](uint32_t i) {
-        uint32_t p = pix[i];
-        return (p % cam.width) / tsz + ((p / cam.width) / tsz) * tx;
-    };
-    // (1) training subset: spread out, at most train_samples
-    std::vector<uint32_t> sel, sel_slot, rest, rest_slot;
-    spread_select(0, todo, slot, hq, sel, sel_slot, rest, rest_slot);
-    const size_t T = std::max<uint32_t>(1, rs.train_samples);
-    if (sel.size() > T) { // keep an even subset
-        std::vector<uint32_t> keep, keep_slot;
-        const double step = (double)sel.size() / (double)T;
-        size_t next = 0;
-        for (size_t k = 0; k < sel.size(); k++) {
-            if (k == (size_t)(next * step) && keep.size() < T) {
-                keep.push_back(sel[k]);
-                keep_slot.push_back(sel_slot[k]);
-                next++;
-            } else {
-                rest.push_back(sel[k]);
-                rest_slot.push_back(sel_slot[k]);
-            }
-        }
-        sel.swap(keep);
-    }
-    // (2) probes: up to probes_per_tile of the other pending points per tile
-    std::vector<std::vector<uint32_t>> by_tile((size_t)tx * ty); // positions in rest
-    for (size_t k = 0; k < rest.size(); k++) by_tile[tile_of(rest[k])].push_back((uint32_t)k);
-    std::vector<uint8_t> is_probe(rest.size(), 0);
-    std::vector<uint32_t> probes, probe_tile;
-    for (uint32_t t = 0; t < by_tile.size(); t++) {
-        const auto &v = by_tile[t];
-        const uint32_t cnt = std::min<uint32_t>(rs.probes_per_tile, (uint32_t)v.size());
-        for (uint32_t j = 0; j < cnt; j++) {
-            uint32_t k = v[(size_t)((j + 0.5) * v.size() / cnt)];
-            if (is_probe[k]) continue;
-            is_probe[k] = 1;
-            probes.push_back(rest[k]);
-            probe_tile.push_back(t);
-        }
-    }
-    std::vector<uint32_t> rest2, rest2_slot;
-    for (size_t k = 0; k < rest.size(); k++)
-        if (!is_probe[k]) {
-            rest2.push_back(rest[k]);
-            rest2_slot.push_back(rest_slot[k]);
-        }
-    // one round of plain gathers for both
-    const size_t first_new = c.recs.size();
-    std::vector<uint32_t> round = sel;
-    round.insert(round.end(), probes.begin(), probes.end());
-    if ((r = gather_round(0, q, round, true, acc, E))) return r;
-    const std::vector<int32_t> filed = last_rec[0];
-    const uint32_t m = (uint32_t)probes.size();
-    std::vector<RadRec> prec(m); // the probes' gathers (standard error in pad[0])
-    if (m && (r = fetch(prec.data(), static_cast<RadRec *>(scratch[0].rec.p) + sel.size(), m * sizeof(RadRec))))
-        return r;
-    stats.probes += m;
-
-    // (3) train on the pretrace records and the training gathers
-    std::vector<uint32_t> train_idx;
-    for (size_t i = 0; i < first_new; i++) train_idx.push_back((uint32_t)i);
-    for (size_t k = 0; k < sel.size(); k++)
-        if (filed[k] >= 0) train_idx.push_back((uint32_t)filed[k]);
-    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (const RadRec &b : c.recs)
-        for (int a = 0; a < 3; a++) {
-            lo[a] = std::min(lo[a], b.p[a]);
-            hi[a] = std::max(hi[a], b.p[a]);
-        }
-    for (const RadQuery &a : hq)
-        for (int d = 0; d < 3; d++) {
-            lo[d] = std::min(lo[d], (float)a.p[d]);
-            hi[d] = std::max(hi[d], (float)a.p[d]);
-        }
-    float ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
-    ext = std::max(ext * 1.02f, 1e-6f);
-    for (int a = 0; a < 3; a++) lo[a] -= 0.01f * ext;
-    const float inv = 1.0f / ext;
-    const uint32_t R0 = (uint32_t)train_idx.size();
-    if ((r = lx.reserve((size_t)std::max(R0, n) * 24)) || (r = lt.reserve((size_t)std::max<uint32_t>(R0, 1) * 12)) ||
-        (r = ly.reserve((size_t)n * 12)) || (r = lidx.reserve((size_t)std::max(R0, n) * 4)))
-        return r;
-    if (R0 > 0) {
-        if ((r = send(lidx.p, train_idx.data(), R0 * 4))) return r;
-        k_rec_to_sample<<<blocks(R0), 256, 0, stream>>>(static_cast<const RadRec *>(c.d_recs.p),
-                                                        static_cast<const uint32_t *>(lidx.p), R0, lo[0], lo[1],
-                                                        lo[2], inv, static_cast<float *>(lx.p),
-                                                        static_cast<float *>(lt.p));
-        HIP_TRY(hipGetLastError());
-    }
+    if ((r = lx.reserve((size_t)n * 24)) || (r = lt.reserve((size_t)n * 12))) return r;
+    k_rec_to_sample<<<blocks(n), 256, 0, stream>>>(recs, nullptr, n, nlo[0], nlo[1], nlo[2], ninv,
+                                                    static_cast<float *>(lx.p), static_cast<float *>(lt.p));
+    HIP_TRY(hipGetLastError());
     hipEvent_t t0, t1;
     HIP_TRY(hipEventCreate(&t0));
     HIP_TRY(hipEventCreate(&t1));
     HIP_TRY(hipEventRecord(t0, stream));
-    float loss = 0.0f;
-    const uint32_t batch = 2048; // samples drawn with replacement from the R0 records
-    if ((r = net.init(nullptr, 0xC0FFEEull, stream))) return r;
-    if (R0 > 0 && (r = net.train(static_cast<const float *>(lx.p), static_cast<const float *>(lt.p), R0,
-                                 rs.train_steps, batch, 5e-3f, &loss, stream)))
+    const uint32_t batch = 2048; // samples drawn with replacement from the n records
+    if ((r = nt.init(nullptr, seed, stream)) ||
+        (r = nt.train(static_cast<const float *>(lx.p), static_cast<const float *>(lt.p), n, rs.train_steps, batch,
+                      5e-3f, loss, stream))) {
+        (void)hipEventDestroy(t0);
+        (void)hipEventDestroy(t1);
         return r;
+    }
     HIP_TRY(hipEventRecord(t1, stream));
     HIP_TRY(hipEventSynchronize(t1));
     float ms = 0;
@@ -1084,69 +1192,271 @@ int RadiosityEngine::resolve_learned(const RadQuery *q, uint32_t n, const std::v
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
     train_ms += ms;
-    stats.train_loss = loss;
+    return PVX_OK;
+}
 
-    // (4) probe test per tile: grey irradiance difference beyond twice the
-    // probe gather's standard error, in displayed colour (the query weight
-    // of a primary hit is its diffuse reflectance, grey(pigment) * opacity
-    // * diffuse, so w * |error| is what the surrogate would change there)
-    std::vector<uint8_t> fb_tile(by_tile.size(), 0);
-    if (m) {
-        if ((r = send(lidx.p, probes.data(), m * 4))) return r;
-        k_query_to_x6<<<blocks(m), 256, 0, stream>>>(q, static_cast<const uint32_t *>(lidx.p), m, lo[0], lo[1], lo[2],
-                                                     inv, static_cast<float *>(lx.p));
-        if ((r = net.infer(static_cast<const float *>(lx.p), m, static_cast<float *>(ly.p), stream))) return r;
-        std::vector<float> y(3 * (size_t)m);
-        if ((r = fetch(y.data(), ly.p, y.size() * 4))) return r;
-        std::vector<double> err(by_tile.size(), 0.0);
-        std::vector<uint32_t> cnt(by_tile.size(), 0);
-        std::vector<float> wq(n, 0.0f);
-        for (size_t k = 0; k < slot.size(); k++) wq[todo[k]] = hq[slot[k]].w;
-        const double gw[3] = {0.297, 0.589, 0.114};
-        double tot = 0;
-        for (uint32_t k = 0; k < m; k++) {
-            double dg = 0;
-            for (int a = 0; a < 3; a++) dg += gw[a] * ((double)std::max(y[3 * (size_t)k + a], 0.0f) - prec[k].E[a]);
-            double e = std::max(0.0, std::fabs(dg) - 2.0 * prec[k].pad[0]) * wq[probes[k]] * rs.brightness;
-            err[probe_tile[k]] += e;
-            cnt[probe_tile[k]]++;
-            tot += e;
-        }
-        stats.probe_error = (float)(tot / m);
-        for (size_t t = 0; t < by_tile.size(); t++) fb_tile[t] = cnt[t] && err[t] / cnt[t] > rs.probe_threshold;
+// Probe test: for the m queries q[d_idx[*]] whose plain gathers gave the
+// records pr[*] (standard error of the gather in pad[0]), the grey
+// difference between the network and the gather beyond twice the gather's
+// standard error, times the query weight (the surface's diffuse
+// reflectance, times 1/2 per bounce), i.e. in displayed colour.
+int RadiosityEngine::probe_errors(Surrogate &nt, const RadQuery *q, const uint32_t *d_idx, uint32_t m,
+                                  const RadRec *pr, std::vector<double> &err) {
+    err.assign(m, 0.0);
+    if (m == 0) return PVX_OK;
+    int r;
+    if ((r = px6.reserve((size_t)m * 24)) || (r = py.reserve((size_t)m * 12)) ||
+        (r = pq.reserve((size_t)m * sizeof(RadQuery))))
+        return r;
+    k_query_to_x6<<<blocks(m), 256, 0, stream>>>(q, d_idx, nullptr, m, nlo[0], nlo[1], nlo[2], ninv,
+                                                 static_cast<float *>(px6.p));
+    k_gather_subq<<<blocks(m), 256, 0, stream>>>(q, d_idx, m, static_cast<RadQuery *>(pq.p));
+    HIP_TRY(hipGetLastError());
+    if ((r = nt.infer(static_cast<const float *>(px6.p), m, static_cast<float *>(py.p), stream))) return r;
+    std::vector<float> y(3 * (size_t)m);
+    std::vector<RadQuery> hq(m);
+    if ((r = fetch(y.data(), py.p, y.size() * 4)) || (r = fetch(hq.data(), pq.p, m * sizeof(RadQuery)))) return r;
+    const double gw[3] = {0.297, 0.589, 0.114};
+    for (uint32_t k = 0; k < m; k++) {
+        double dg = 0;
+        for (int a = 0; a < 3; a++) dg += gw[a] * ((double)std::max(y[3 * (size_t)k + a], 0.0f) - pr[k].E[a]);
+        err[k] = std::max(0.0, std::fabs(dg) - 2.0 * pr[k].pad[0]) * hq[k].w * rs.brightness;
     }
-    // the round's records serve the other pending points
-    if ((r = relookup(0, q, first_new, true, acc, rest2, rest2_slot, todo, slot))) return r;
+    return PVX_OK;
+}
 
-    // (5) fallback tiles: plain gathers; the rest: the network
-    std::vector<uint32_t> fb, fb_slot, ml, ml_slot;
-    for (size_t k = 0; k < todo.size(); k++) {
-        if (fb_tile[tile_of(todo[k])]) {
-            fb.push_back(todo[k]);
-            fb_slot.push_back(slot[k]);
-        } else {
-            ml.push_back(todo[k]);
-            ml_slot.push_back(slot[k]);
-        }
-    }
-    stats.fallbacks += fb.size();
-    if (!fb.empty()) {
-        const size_t first_fb = c.recs.size();
-        if ((r = gather_rounds(0, q, true, fb, fb_slot, hq, acc, E, 2))) return r;
-        if ((r = relookup(0, q, first_fb, true, acc, ml, ml_slot, todo, slot))) return r;
-        ml.swap(todo);
-    }
-    stats.surrogate_queries += ml.size();
-    if (!ml.empty()) {
-        const uint32_t mm = (uint32_t)ml.size();
-        if ((r = send(lidx.p, ml.data(), mm * 4))) return r;
-        k_query_to_x6<<<blocks(mm), 256, 0, stream>>>(q, static_cast<const uint32_t *>(lidx.p), mm, lo[0], lo[1],
-                                                      lo[2], inv, static_cast<float *>(lx.p));
-        if ((r = net.infer(static_cast<const float *>(lx.p), mm, static_cast<float *>(ly.p), stream))) return r;
-        k_apply_pred<<<blocks(mm), 256, 0, stream>>>(static_cast<const uint32_t *>(lidx.p), mm,
-                                                     static_cast<const float *>(ly.p), acc, E);
+// Indices i < n with flags[i] != 0, ascending (device `out`); count to the host.
+int RadiosityEngine::compact(const uint8_t *flags, uint32_t n, uint32_t *out, uint32_t &count) {
+    count = 0;
+    if (n == 0) return PVX_OK;
+    int r;
+    if ((r = d_count.reserve(4))) return r;
+    hipcub::CountingInputIterator<uint32_t> it(0);
+    size_t tb = 0;
+    HIP_TRY(hipcub::DeviceSelect::Flagged(nullptr, tb, it, flags, out, static_cast<uint32_t *>(d_count.p), n,
+                                          stream));
+    if ((r = cub_tmp.reserve(std::max<size_t>(tb, 16)))) return r;
+    HIP_TRY(hipcub::DeviceSelect::Flagged(cub_tmp.p, tb, it, flags, out, static_cast<uint32_t *>(d_count.p), n,
+                                          stream));
+    return fetch(&count, d_count.p, 4);
+}
+
+// Final-pass irradiance with the surrogate (learned mode; no pretrace).
+//  1. Plain depth-0 gathers at a spread-out training set (per cell of about
+//     W*H / train_samples pixels, the point with the lowest pixel hash) and
+//     at probes (up to probes_per_tile per probe_tile^2 image tile), all in
+//     one batch.
+//  2. Their rays' hits want depth-1 irradiance: plain depth-1 gathers at a
+//     spread-out subset of those hits (every 17th held out as a probe)
+//     train a depth-1 network, which then answers every hit of the batch —
+//     unless its probes disagree, in which case depth 1 is resolved the
+//     plain way (irradiance cache and gather rounds).
+//  3. The depth-0 training gathers train the depth-0 network.  Every query
+//     first reuses this frame's depth-0 records as the plain final pass
+//     would; the misses take the network's answer, except in tiles whose
+//     probes disagree by more than probe_threshold, which fall back to the
+//     plain method (gather rounds over the same cache).
+// Each network is judged only by plain gathers it was not trained on.
+int RadiosityEngine::resolve_learned(const RadQuery *q, uint32_t n, const uint32_t *d_pix, float *E) {
+    if (n == 0) return PVX_OK;
+    int r;
+    Resolve &R = rstate[0];
+    const uint32_t W = cam.width, H = cam.height;
+    const uint32_t T = std::max<uint32_t>(1, rs.train_samples);
+    const uint32_t cpx = std::max<uint32_t>(1, (uint32_t)std::floor(std::sqrt((double)W * H / T)));
+    const uint32_t ncx = (W + cpx - 1) / cpx, ncy = (H + cpx - 1) / cpx;
+    const size_t ncell = (size_t)ncx * ncy;
+    const uint32_t tsz = std::max<uint32_t>(1, rs.probe_tile);
+    const uint32_t tx = (W + tsz - 1) / tsz, ty = (H + tsz - 1) / tsz;
+    const size_t ntile = (size_t)tx * ty;
+    if ((r = R.acc.reserve((size_t)n * sizeof(RadAcc))) || (r = R.need.reserve(n)) ||
+        (r = R.idx.reserve((size_t)n * 4)) || (r = R.Eg.reserve((size_t)n * 12)) ||
+        (r = R.tq.reserve((size_t)n * sizeof(RadQuery))) || (r = lrole.reserve(n)) ||
+        (r = lkeys.reserve(std::max(ncell, ntile) * 8)) || (r = lfbflag.reserve(n)) ||
+        (r = lfbtile.reserve(ntile)) || (r = lcnt.reserve(8)) || (r = lx.reserve((size_t)n * 24)) ||
+        (r = ly.reserve((size_t)n * 12)))
+        return r;
+    RadAcc *acc = static_cast<RadAcc *>(R.acc.p);
+    uint8_t *role = static_cast<uint8_t *>(lrole.p);
+    auto *keys = static_cast<unsigned long long *>(lkeys.p);
+
+    // (1) training points and probes
+    k_fill_u64<<<blocks(ncell), 256, 0, stream>>>(keys, ncell, ~0ull);
+    k_cell_keys<<<blocks(n), 256, 0, stream>>>(d_pix, n, W, cpx, ncx, keys);
+    k_fill_u8<<<blocks(n), 256, 0, stream>>>(role, n);
+    HIP_TRY(hipGetLastError());
+    std::vector<unsigned long long> hk(std::max(ncell, ntile));
+    if ((r = fetch(hk.data(), keys, ncell * 8))) return r;
+    std::vector<uint32_t> g0;
+    for (size_t c = 0; c < ncell; c++)
+        if (hk[c] != ~0ull) g0.push_back((uint32_t)hk[c]);
+    const uint32_t n_train = (uint32_t)g0.size();
+    if ((r = lg0.reserve(((size_t)n_train + ntile * 16 + 1) * 4))) return r;
+    uint32_t *d_g0 = static_cast<uint32_t *>(lg0.p);
+    if ((r = send(d_g0, g0.data(), (size_t)n_train * 4))) return r;
+    k_mark_role<<<blocks(n_train), 256, 0, stream>>>(d_g0, n_train, 1, role);
+    std::vector<uint32_t> probe_tile;
+    const uint32_t P = std::min<uint32_t>(rs.probes_per_tile, 16);
+    for (uint32_t j = 0; j < P; j++) {
+        k_fill_u64<<<blocks(ntile), 256, 0, stream>>>(keys, ntile, ~0ull);
+        k_tile_keys<<<blocks(n), 256, 0, stream>>>(d_pix, role, n, W, tsz, tx, 0x5bd1e995u * (j + 1), keys);
         HIP_TRY(hipGetLastError());
+        if ((r = fetch(hk.data(), keys, ntile * 8))) return r;
+        const size_t first = g0.size();
+        for (size_t t = 0; t < ntile; t++)
+            if (hk[t] != ~0ull) {
+                g0.push_back((uint32_t)hk[t]);
+                probe_tile.push_back((uint32_t)t);
+            }
+        const uint32_t add = (uint32_t)(g0.size() - first);
+        if (add == 0) break;
+        if ((r = send(d_g0 + first, g0.data() + first, (size_t)add * 4))) return r;
+        k_mark_role<<<blocks(add), 256, 0, stream>>>(d_g0 + first, add, 2, role);
     }
+    const uint32_t m0 = (uint32_t)g0.size();
+    const uint32_t n_probe = m0 - n_train;
+    stats.probes += n_probe;
+
+    // (2) the depth-0 gathers' rays; normalisation from the gathered points
+    if ((r = gather_trace(0, q, d_g0, m0))) return r;
+    std::vector<RadQuery> hq0(m0);
+    k_gather_subq<<<blocks(m0), 256, 0, stream>>>(q, d_g0, m0, static_cast<RadQuery *>(R.tq.p));
+    HIP_TRY(hipGetLastError());
+    if ((r = fetch(hq0.data(), R.tq.p, (size_t)m0 * sizeof(RadQuery)))) return r;
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    auto grow = [&](const float *pp) {
+        for (int a = 0; a < 3; a++) {
+            lo[a] = std::min(lo[a], pp[a]);
+            hi[a] = std::max(hi[a], pp[a]);
+        }
+    };
+    for (const RadQuery &a : hq0) {
+        float pp[3] = {(float)a.p[0], (float)a.p[1], (float)a.p[2]};
+        grow(pp);
+    }
+    auto set_norm = [&]() {
+        float ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+        ext = std::max(ext * 1.04f, 1e-6f);
+        for (int a = 0; a < 3; a++) nlo[a] = lo[a] - 0.02f * ext;
+        ninv = 1.0f / (ext * 1.02f);
+    };
+    set_norm();
+    amb_net = nullptr;
+    Scratch &S0 = scratch[0];
+    const size_t nr0 = (size_t)m0 * rays_per_sample(0);
+    if (rs.recursion_limit > 1) {
+        std::vector<uint8_t> need(nr0);
+        if ((r = fetch(need.data(), S0.need.p, nr0))) return r;
+        size_t cntn = 0;
+        for (size_t k = 0; k < nr0; k++) cntn += need[k];
+        const uint32_t want = T + (P > 0 ? std::max<uint32_t>(1, T / 16) : 0);
+        std::vector<uint32_t> tr1, pr1;
+        if (cntn) {
+            const uint32_t cnt = (uint32_t)std::min<size_t>(want, cntn);
+            const double stride = (double)cntn / cnt;
+            size_t seen = 0, next = 0;
+            uint32_t j = 0;
+            for (size_t k = 0; k < nr0 && j < cnt; k++) {
+                if (!need[k]) continue;
+                if (seen == next) {
+                    (P > 0 && j % 17 == 16 ? pr1 : tr1).push_back((uint32_t)k);
+                    j++;
+                    next = (size_t)(j * stride);
+                }
+                seen++;
+            }
+        }
+        if (!tr1.empty()) {
+            std::vector<uint32_t> g1 = tr1;
+            g1.insert(g1.end(), pr1.begin(), pr1.end());
+            const uint32_t m1 = (uint32_t)g1.size();
+            Resolve &R1 = rstate[1];
+            if ((r = lg1.reserve((size_t)m1 * 4)) || (r = R1.Eg.reserve((size_t)m1 * 12))) return r;
+            uint32_t *d_g1 = static_cast<uint32_t *>(lg1.p);
+            if ((r = send(d_g1, g1.data(), (size_t)m1 * 4))) return r;
+            const RadQuery *q1 = static_cast<const RadQuery *>(S0.q.p);
+            if ((r = gather(1, q1, d_g1, m1, true, static_cast<float *>(R1.Eg.p)))) return r;
+            const std::vector<RadRec> rec1 = last_fetched[1];
+            for (const RadRec &b : rec1) grow(b.p);
+            set_norm();
+            float loss1 = 0.0f;
+            if ((r = train_on(net1, static_cast<const RadRec *>(scratch[1].rec.p), (uint32_t)tr1.size(),
+                              0xC0FFEEull + 1, &loss1)))
+                return r;
+            bool ok = true;
+            if (!pr1.empty()) {
+                std::vector<double> e1;
+                if ((r = probe_errors(net1, q1, d_g1 + tr1.size(), (uint32_t)pr1.size(), rec1.data() + tr1.size(),
+                                      e1)))
+                    return r;
+                double tot = 0;
+                for (double e : e1) tot += e;
+                ok = tot / e1.size() <= rs.probe_threshold;
+                stats.probes += pr1.size();
+            }
+            if (ok) amb_net = &net1;
+        }
+    }
+    if ((r = gather_finish(0, q, d_g0, m0, true, static_cast<float *>(R.Eg.p)))) return r;
+    const std::vector<RadRec> rec0 = last_fetched[0];
+
+    // (3) the depth-0 network and its probes
+    float loss = 0.0f;
+    if ((r = train_on(net, static_cast<const RadRec *>(S0.rec.p), n_train, 0xC0FFEEull, &loss))) return r;
+    stats.train_loss = loss;
+    std::vector<uint8_t> fb_tile(ntile, 0);
+    if (n_probe) {
+        std::vector<double> e0;
+        if ((r = probe_errors(net, q, d_g0 + n_train, n_probe, rec0.data() + n_train, e0))) return r;
+        std::vector<double> err(ntile, 0.0);
+        std::vector<uint32_t> cnt(ntile, 0);
+        double tot = 0;
+        for (uint32_t k = 0; k < n_probe; k++) {
+            err[probe_tile[k]] += e0[k];
+            cnt[probe_tile[k]]++;
+            tot += e0[k];
+        }
+        stats.probe_error = (float)(tot / n_probe);
+        for (size_t t = 0; t < ntile; t++) fb_tile[t] = cnt[t] && err[t] / cnt[t] > rs.probe_threshold;
+    }
+
+    // (4) reuse of this frame's records, the gathers themselves, the network
+    RadCache &c = caches[0];
+    if ((r = c.index_all(*this))) return r;
+    if ((r = lookup(0, c.all, q, nullptr, n, true, false, acc))) return r;
+    k_acc_reset<<<blocks(m0), 256, 0, stream>>>(d_g0, m0, acc);
+    k_combine<<<blocks(m0), 256, 0, stream>>>(d_g0, m0, static_cast<const float *>(R.Eg.p), acc, E);
+    k_query_to_x6<<<blocks(n), 256, 0, stream>>>(q, nullptr, nullptr, n, nlo[0], nlo[1], nlo[2], ninv,
+                                                 static_cast<float *>(lx.p));
+    HIP_TRY(hipGetLastError());
+    if ((r = net.infer(static_cast<const float *>(lx.p), n, static_cast<float *>(ly.p), stream))) return r;
+    if ((r = send(lfbtile.p, fb_tile.data(), ntile))) return r;
+    uint32_t *cnt = static_cast<uint32_t *>(lcnt.p);
+    k_fill_u32<<<1, 64, 0, stream>>>(cnt, 2);
+    k_learned_apply<<<blocks(n), 256, 0, stream>>>(acc, d_pix, n, W, tsz, tx, static_cast<const uint8_t *>(lfbtile.p),
+                                                   static_cast<const float *>(ly.p), E,
+                                                   static_cast<uint8_t *>(lfbflag.p), cnt);
+    HIP_TRY(hipGetLastError());
+    uint32_t hc[2];
+    if ((r = fetch(hc, cnt, 8))) return r;
+    stats.surrogate_queries += hc[0];
+    stats.fallbacks += hc[1];
+    stats.final_misses += (uint64_t)hc[0] + hc[1];
+    if (hc[1]) {
+        uint32_t mf = 0;
+        if ((r = compact(static_cast<const uint8_t *>(lfbflag.p), n, static_cast<uint32_t *>(R.idx.p), mf))) return r;
+        std::vector<uint32_t> todo(mf), slot(mf);
+        std::vector<RadQuery> hq(mf);
+        if ((r = fetch(todo.data(), R.idx.p, (size_t)mf * 4))) return r;
+        k_gather_subq<<<blocks(mf), 256, 0, stream>>>(q, static_cast<const uint32_t *>(R.idx.p), mf,
+                                                      static_cast<RadQuery *>(R.tq.p));
+        HIP_TRY(hipGetLastError());
+        if ((r = fetch(hq.data(), R.tq.p, (size_t)mf * sizeof(RadQuery)))) return r;
+        for (uint32_t k = 0; k < mf; k++) slot[k] = k;
+        // one spread-out round with reuse, then every point still unreached
+        if ((r = gather_rounds(0, q, true, std::move(todo), std::move(slot), hq, acc, E, 2))) return r;
+    }
+    amb_net = nullptr;
     k_finish<<<blocks(n), 256, 0, stream>>>(acc, n, rs.brightness, E);
     HIP_TRY(hipGetLastError());
     return PVX_OK;
@@ -1170,7 +1480,7 @@ int RadiosityEngine::pixel_pass(const std::vector<uint32_t> &px, bool final, flo
         (r = S.hits.reserve(n * sizeof(pvx_hit))) || (r = S.base.reserve(n * 12)) ||
         (r = S.coef.reserve(n * 12)) || (r = S.q.reserve(n * sizeof(RadQuery))) || (r = S.need.reserve(n)) ||
         (r = S.subq.reserve(n * sizeof(RadQuery))) || (r = S.subE.reserve(n * 12)) ||
-        (r = S.rec.reserve(n * 4)))
+        (r = psel.reserve(n * 4)) || (r = ppix.reserve(n * 4)) || (r = pqpix.reserve(n * 4)))
         return r;
     if ((r = send(S.sel.p, px.data(), n * 4))) return r;
     k_pixel_rays<<<blocks(n), 256, 0, stream>>>(cam, static_cast<const uint32_t *>(S.sel.p), n,
@@ -1191,26 +1501,20 @@ int RadiosityEngine::pixel_pass(const std::vector<uint32_t> &px, bool final, flo
                                                  static_cast<float *>(S.coef.p), static_cast<RadQuery *>(S.q.p),
                                                  static_cast<uint8_t *>(S.need.p));
     HIP_TRY(hipGetLastError());
-    std::vector<uint8_t> need(n);
-    if ((r = fetch(need.data(), S.need.p, n))) return r;
-    std::vector<uint32_t> sel, qpix(n, 0);
-    for (uint32_t i = 0; i < n; i++)
-        if (need[i]) {
-            qpix[i] = (uint32_t)sel.size();
-            sel.push_back(i);
-        }
-    if (!sel.empty()) {
-        const uint32_t m = (uint32_t)sel.size();
-        if ((r = send(S.sel.p, sel.data(), m * 4))) return r;
-        k_gather_subq<<<blocks(m), 256, 0, stream>>>(static_cast<const RadQuery *>(S.q.p),
-                                                     static_cast<const uint32_t *>(S.sel.p), m,
+    // the hits that want irradiance, in pixel-list order
+    uint32_t m = 0;
+    uint32_t *sel = static_cast<uint32_t *>(psel.p);
+    if ((r = compact(static_cast<const uint8_t *>(S.need.p), n, sel, m))) return r;
+    if (m) {
+        k_gather_subq<<<blocks(m), 256, 0, stream>>>(static_cast<const RadQuery *>(S.q.p), sel, m,
                                                      static_cast<RadQuery *>(S.subq.p));
         HIP_TRY(hipGetLastError());
         if (final && rs.mode == PVX_RADIOSITY_LEARNED) {
-            std::vector<uint32_t> pix(m);
-            for (uint32_t k = 0; k < m; k++) pix[k] = px[sel[k]];
-            if ((r = resolve_learned(static_cast<const RadQuery *>(S.subq.p), m, pix,
-                                     static_cast<float *>(S.subE.p))))
+            k_gather_u32<<<blocks(m), 256, 0, stream>>>(static_cast<const uint32_t *>(S.sel.p), sel, m,
+                                                        static_cast<uint32_t *>(ppix.p));
+            HIP_TRY(hipGetLastError());
+            if ((r = resolve_learned(static_cast<const RadQuery *>(S.subq.p), m,
+                                     static_cast<const uint32_t *>(ppix.p), static_cast<float *>(S.subE.p))))
                 return r;
         } else if ((r = resolve(0, static_cast<const RadQuery *>(S.subq.p), m, final,
                                 static_cast<float *>(S.subE.p)))) {
@@ -1218,10 +1522,10 @@ int RadiosityEngine::pixel_pass(const std::vector<uint32_t> &px, bool final, flo
         }
     }
     if (final) {
-        if ((r = send(S.rec.p, qpix.data(), n * 4))) return r;
+        if (m) k_scatter_rank<<<blocks(m), 256, 0, stream>>>(sel, m, static_cast<uint32_t *>(pqpix.p));
         k_final_colour<<<blocks(n), 256, 0, stream>>>(
             static_cast<const float *>(S.base.p), static_cast<const float *>(S.coef.p),
-            static_cast<const float *>(S.subE.p), static_cast<const uint32_t *>(S.rec.p), n,
+            static_cast<const float *>(S.subE.p), static_cast<const uint32_t *>(pqpix.p), n,
             static_cast<const uint8_t *>(S.need.p), rgb);
         HIP_TRY(hipGetLastError());
         HIP_TRY(sync());
@@ -1275,7 +1579,8 @@ extern "C" int pvx_radiosity_render(pvx_scene *s, const pvx_camera *cam, const p
     // backend/scene/view.cpp:1056-1106; integer pixel steps here)
     double step = std::max(1.0, std::floor((double)set->pretrace_start * std::max(W, H)));
     double end = std::max(1.0, std::floor((double)set->pretrace_end * std::max(W, H)));
-    for (;;) {
+    // the learned mode has no pretrace: its networks stand in for the cache
+    for (; set->mode != PVX_RADIOSITY_LEARNED;) {
         std::vector<uint32_t> px;
         uint32_t k = (uint32_t)step;
         for (uint32_t y = 0; y < H; y += k)

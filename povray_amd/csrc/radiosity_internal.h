@@ -93,8 +93,16 @@ struct RadiosityEngine {
     } rstate[kMaxRadDepth];
     Scratch prim; // primary rays of a pixel pass
     std::vector<int32_t> last_rec[kMaxRadDepth]; // cache index of each query of the last gather (-1: not filed)
-    Surrogate net; // learned mode
-    DevBuf lx, lt, ly, lidx;
+    Surrogate net;  // learned mode: depth-0 irradiance
+    Surrogate net1; // learned mode: depth-1 irradiance (inside depth-0 gathers)
+    DevBuf lx, lt, ly, lidx, lg0, lg1, ax, ay, px6, py, pq;
+    DevBuf lrole, lkeys, lfbflag, lfbtile, lcnt;      // learned final pass
+    DevBuf psel, ppix, pqpix, cub_tmp, d_count;       // pixel-pass compaction
+    // While set, depth-0 gathers take the irradiance of their hits from this
+    // network instead of resolving depth 1 (inputs normalised by nlo / ninv).
+    Surrogate *amb_net = nullptr;
+    float nlo[3] = {0, 0, 0}, ninv = 1.0f;
+    std::vector<RadRec> last_fetched[kMaxRadDepth]; // records of the last gather per depth
     double train_ms = 0.0;
     double t_sync_ms = 0.0, t_grid_ms = 0.0; // host-side waits and grid builds
     uint64_t n_sync = 0;
@@ -113,8 +121,14 @@ struct RadiosityEngine {
     int rays_per_sample(int depth) const;
     float reuse_count(int depth) const;
     int lookup(int depth, const RadGrid &g, const RadQuery *q, const uint32_t *idx, uint32_t n, bool final,
-               bool accumulate, RadAcc *acc);
+               bool accumulate, RadAcc *acc, const uint8_t *mask = nullptr);
+    int compact(const uint8_t *flags, uint32_t n, uint32_t *out, uint32_t &count);
     int gather(int depth, const RadQuery *q, const uint32_t *idx, uint32_t m, bool final, float *Eg);
+    int gather_trace(int depth, const RadQuery *q, const uint32_t *idx, uint32_t m);
+    int gather_finish(int depth, const RadQuery *q, const uint32_t *idx, uint32_t m, bool final, float *Eg);
+    int train_on(Surrogate &nt, const RadRec *recs, uint32_t n, uint64_t seed, float *loss);
+    int probe_errors(Surrogate &nt, const RadQuery *q, const uint32_t *d_idx, uint32_t m, const RadRec *pr,
+                     std::vector<double> &err);
     int resolve(int depth, const RadQuery *q, uint32_t n, bool final, float *E);
     int start_resolve(int depth, const RadQuery *q, uint32_t n, bool final, RadAcc *&acc,
                       std::vector<uint32_t> &todo, std::vector<uint32_t> &slot, std::vector<RadQuery> &hq);
@@ -129,7 +143,7 @@ struct RadiosityEngine {
     int gather_rounds(int depth, const RadQuery *q, bool final, std::vector<uint32_t> todo,
                       std::vector<uint32_t> slot, const std::vector<RadQuery> &hq, RadAcc *acc, float *E,
                       int max_rounds = kMaxResolveRounds);
-    int resolve_learned(const RadQuery *q, uint32_t n, const std::vector<uint32_t> &pix, float *E);
+    int resolve_learned(const RadQuery *q, uint32_t n, const uint32_t *d_pix, float *E);
     int pixel_pass(const std::vector<uint32_t> &px, bool final, float *rgb);
 };
 

@@ -607,8 +607,8 @@ def radiosity_scene(seed: int = 0xC0FFEE + 3, n_boxes: int = 5000, width=1280, h
                       radiosity=dict(count=200, recursion_limit=2, error_bound=0.5,
                                      minimum_reuse=0.015, maximum_reuse=0.2, nearest_count=5,
                                      pretrace_start=0.08, pretrace_end=0.04, brightness=1.0,
-                                     adc_bailout=0.01, probe_threshold=0.02, train_samples=1024,
-                                     train_steps=200, probe_tile=64, probes_per_tile=1)),
+                                     adc_bailout=0.01, probe_threshold=0.02, train_samples=6144,
+                                     train_steps=100, probe_tile=64, probes_per_tile=1)),
     )
 
 

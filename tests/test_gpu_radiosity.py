@@ -91,3 +91,55 @@ def test_radiosity_rejects_bad_settings(gpu):
         g.radiosity_render(nearest_count=0)
     with pytest.raises(gpu.PvxError, match="recursion_limit"):
         g.radiosity_render(recursion_limit=9)
+
+
+# ---------------------------------------------------------------------------
+# learned mode: what the networks gain over the plain method
+# ---------------------------------------------------------------------------
+def _learned(g, **over):
+    from povray_amd import _abi as A
+
+    return g.radiosity_render(mode=A.PVX_RADIOSITY_LEARNED, **over)
+
+
+def test_learned_is_faster_at_plain_accuracy(gpu):
+    """Against a converged frame (8x the rays per gather, tighter error
+    bound), the learned frame is as accurate as the plain one while taking
+    a fraction of its time: that is what the networks buy."""
+    s = scenes.radiosity_scene(n_boxes=2000, width=640, height=360)
+    g = _gpu_scene(gpu, s)
+    conv, cs = g.radiosity_render(count=1600, error_bound=0.2, nearest_count=10)
+    g.radiosity_render()
+    plain, ps = g.radiosity_render()
+    _learned(g)
+    img, ls = _learned(g)
+    e_plain, e_learned = _rmse(plain, conv), _rmse(img, conv)
+    print(f"[rad-learned] plain {ps['frame_ms']:.1f} ms rmse {e_plain * 255:.2f}/255; learned "
+          f"{ls['frame_ms']:.1f} ms rmse {e_learned * 255:.2f}/255 (train {ls['train_ms']:.1f} ms, "
+          f"network {ls['surrogate_queries']}, fallback {ls['fallbacks']}, gathers {ls['gathers']})", flush=True)
+    assert ls["surrogate_queries"] > 0 and ls["pretrace_ms"] < 0.1  # no pretrace passes
+    assert e_learned <= 1.15 * e_plain
+    # mean brightness: no further from the converged frame than plain's, give or take 2 %
+    bias_plain = abs(float(plain.mean() / conv.mean() - 1.0))
+    bias_learned = abs(float(img.mean() / conv.mean() - 1.0))
+    print(f"[rad-learned] mean bias vs converged: plain {bias_plain:.4f} learned {bias_learned:.4f}", flush=True)
+    assert bias_learned <= bias_plain + 0.02
+    assert ls["frame_ms"] < 0.6 * ps["frame_ms"]
+
+
+def test_learned_is_deterministic_and_probes_guard_a_bad_network(gpu):
+    s = scenes.radiosity_scene(n_boxes=500, width=320, height=180)
+    g = _gpu_scene(gpu, s)
+    a, sa = _learned(g)
+    b, sb = _learned(g)
+    assert np.array_equal(a, b) and sa["gathers"] == sb["gathers"]
+    plain, _ = g.radiosity_render()
+    # a network trained for one step is poor: its probes fail, the tiles
+    # fall back to plain gathers, and the frame stays close to plain
+    bad, sbad = _learned(g, train_steps=1, probe_tile=32)
+    print(f"[rad-guard] trained: fallbacks {sa['fallbacks']} probe err {sa['probe_error']:.4f}; one step: "
+          f"fallbacks {sbad['fallbacks']} probe err {sbad['probe_error']:.4f}; rmse vs plain "
+          f"{_rmse(bad, plain) * 255:.2f}/255 (trained {_rmse(a, plain) * 255:.2f}/255)", flush=True)
+    assert sbad["probe_error"] > sa["probe_error"]
+    assert sbad["fallbacks"] > sa["fallbacks"]
+    assert _rmse(bad, plain) < 2.0 * _rmse(a, plain) + 2.0 / 255
