@@ -171,6 +171,8 @@ typedef struct pvx_build_opts {
     uint32_t reserved[6];
 } pvx_build_opts;
 
+/* Counts describe the tree the closest-hit kernels walk: the 4-wide tree
+ * (128 B nodes) for triangle / sphere scenes, the binary tree (64 B) else. */
 typedef struct pvx_scene_stats {
     uint64_t n_nodes;        /* BVH nodes                               */
     uint64_t n_prims;        /* BVH primitive references                */

@@ -305,4 +305,80 @@ BvhResult build_bvh(const float *lo, const float *hi, uint32_t n, int max_leaf, 
     return b.run();
 }
 
+namespace {
+
+struct Cand {
+    int32_t link;
+    float lo[3], hi[3];
+};
+
+float cand_area(const Cand &c) {
+    float d[3];
+    for (int k = 0; k < 3; k++) d[k] = std::max(0.0f, c.hi[k] - c.lo[k]);
+    return d[0] * d[1] + d[1] * d[2] + d[2] * d[0];
+}
+
+void node_children(const Node &m, Cand &a, Cand &b) {
+    a = Cand{m.c0, {m.lo0x, m.lo0y, m.lo0z}, {m.hi0x, m.hi0y, m.hi0z}};
+    b = Cand{m.c1, {m.lo1x, m.lo1y, m.lo1z}, {m.hi1x, m.hi1y, m.hi1z}};
+}
+
+int32_t collapse_rec(const std::vector<Node> &nodes, int32_t n, uint32_t d, std::vector<Node4> &out,
+                     uint32_t &depth) {
+    depth = std::max(depth, d);
+    Cand c[4];
+    int cnt = 2;
+    node_children(nodes[n], c[0], c[1]);
+    while (cnt < 4) {
+        int best = -1;
+        float ba = -1.0f;
+        for (int k = 0; k < cnt; k++)
+            if (c[k].link >= 0 && cand_area(c[k]) > ba) {
+                ba = cand_area(c[k]);
+                best = k;
+            }
+        if (best < 0) break;
+        Cand a, b;
+        node_children(nodes[c[best].link], a, b);
+        c[best] = a;
+        c[cnt++] = b;
+    }
+    const int32_t idx = (int32_t)out.size();
+    out.emplace_back();
+    int32_t links[4];
+    for (int k = 0; k < 4; k++) {
+        if (k >= cnt)
+            links[k] = INT32_MIN;
+        else
+            links[k] = c[k].link >= 0 ? collapse_rec(nodes, c[k].link, d + 1, out, depth) : c[k].link;
+    }
+    Node4 &w = out[(size_t)idx];
+    for (int k = 0; k < 4; k++) {
+        const bool v = k < cnt;
+        w.lox[k] = v ? c[k].lo[0] : 0.0f;
+        w.loy[k] = v ? c[k].lo[1] : 0.0f;
+        w.loz[k] = v ? c[k].lo[2] : 0.0f;
+        w.hix[k] = v ? c[k].hi[0] : 0.0f;
+        w.hiy[k] = v ? c[k].hi[1] : 0.0f;
+        w.hiz[k] = v ? c[k].hi[2] : 0.0f;
+        w.c[k] = links[k];
+        w.pad[k] = 0;
+    }
+    return idx;
+}
+
+} // namespace
+
+void collapse_bvh4(const std::vector<Node> &nodes, int32_t root, std::vector<Node4> &nodes4, int32_t &root4,
+                   uint32_t &depth4) {
+    nodes4.clear();
+    depth4 = 0;
+    if (root < 0) { // empty, or a single leaf
+        root4 = root;
+        return;
+    }
+    nodes4.reserve(nodes.size() / 2 + 1);
+    root4 = collapse_rec(nodes, root, 0, nodes4, depth4);
+}
+
 } // namespace pvx

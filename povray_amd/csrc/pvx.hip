@@ -207,15 +207,14 @@ __device__ __forceinline__ void trace_closest_body(const SceneView &sv0, const S
         rf.oiz = (float)o.z * rf.iz;
         float tb = (float)bt * 1.000001f;
 
-        int32_t node = sv.root;
+        int32_t node = kLevel == 0 ? sv.root4 : sv.root;
         int sp = 0;
         uint32_t cnt_nodes = 0, cnt_prims = 0;
-        // While-while traversal (the lanes of a wavefront visit inner nodes
-        // together, then leaves together) pays for camera and bounce rays in
-        // triangle scenes; for light rays, and with boxes, tori and CSG, whose
-        // leaf tests are long, each pass of the loop takes one node or one
-        // leaf step.  The order each lane visits its nodes in is the same
-        // either way.
+        // Triangle / sphere scenes traverse the 4-wide tree; camera and
+        // bounce rays with while-while passes (the lanes of a wavefront visit
+        // inner nodes together, then leaves together), light rays one node or
+        // leaf per pass.  Scenes with boxes, tori and CSG, whose leaf tests
+        // are long, take the binary tree one step per pass.
 #define PVX_POP()                                                                       \
     do {                                                                                \
         if (sp == 0) {                                                                  \
@@ -225,50 +224,57 @@ __device__ __forceinline__ void trace_closest_body(const SceneView &sv0, const S
             node = sp < kStackLds ? s_stack[sp * kBlock + lane] : my_spill[sp - kStackLds]; \
         }                                                                               \
     } while (0)
-        // one inner-node step (pushes the far child or pops when no child is hit)
-        auto step_node = [&]() {
+        // one 4-wide inner-node step (triangle / sphere scenes): the hit
+        // children sorted by entry distance, the nearest taken, the others
+        // pushed farthest first; pops when no child is hit
+        auto step_node4 = [&]() {
             if (kCount) cnt_nodes++;
-            const float4 *np = reinterpret_cast<const float4 *>(sv.nodes + node);
-            const float4 a = np[0], b = np[1], c = np[2];
-            const int4 l = reinterpret_cast<const int4 *>(np)[3];
-            float x0 = fmaf(a.x, rf.ix, -rf.oix), x1 = fmaf(a.y, rf.ix, -rf.oix);
-            float y0 = fmaf(a.z, rf.iy, -rf.oiy), y1 = fmaf(a.w, rf.iy, -rf.oiy);
-            float z0 = fmaf(c.x, rf.iz, -rf.oiz), z1 = fmaf(c.y, rf.iz, -rf.oiz);
-            float tn0 = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
-            float tf0 = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tb));
-            x0 = fmaf(b.x, rf.ix, -rf.oix);
-            x1 = fmaf(b.y, rf.ix, -rf.oix);
-            y0 = fmaf(b.z, rf.iy, -rf.oiy);
-            y1 = fmaf(b.w, rf.iy, -rf.oiy);
-            z0 = fmaf(c.z, rf.iz, -rf.oiz);
-            z1 = fmaf(c.w, rf.iz, -rf.oiz);
-            float tn1 = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
-            float tf1 = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tb));
-            bool h0 = tn0 <= tf0 * 1.000001f;
-            bool h1 = tn1 <= tf1 * 1.000001f;
-            if (h0 && h1) {
-                int32_t nearc = l.x, farc = l.y;
-                if (tn1 < tn0) {
-                    nearc = l.y;
-                    farc = l.x;
+            const float4 *np = reinterpret_cast<const float4 *>(sv.nodes4 + node);
+            const float4 xl = np[0], xh = np[1], yl = np[2], yh = np[3], zl = np[4], zh = np[5];
+            const int4 cl = reinterpret_cast<const int4 *>(np)[6];
+            auto slab = [&](float lx, float hx, float ly, float hy, float lz, float hz, int32_t link) {
+                const float x0 = fmaf(lx, rf.ix, -rf.oix), x1 = fmaf(hx, rf.ix, -rf.oix);
+                const float y0 = fmaf(ly, rf.iy, -rf.oiy), y1 = fmaf(hy, rf.iy, -rf.oiy);
+                const float z0 = fmaf(lz, rf.iz, -rf.oiz), z1 = fmaf(hz, rf.iz, -rf.oiz);
+                const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
+                const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tb));
+                return (link != kEmpty && tn <= tf * 1.000001f) ? tn : INFINITY;
+            };
+            float d0 = slab(xl.x, xh.x, yl.x, yh.x, zl.x, zh.x, cl.x);
+            float d1 = slab(xl.y, xh.y, yl.y, yh.y, zl.y, zh.y, cl.y);
+            float d2 = slab(xl.z, xh.z, yl.z, yh.z, zl.z, zh.z, cl.z);
+            float d3 = slab(xl.w, xh.w, yl.w, yh.w, zl.w, zh.w, cl.w);
+            int32_t l0 = cl.x, l1 = cl.y, l2 = cl.z, l3 = cl.w;
+            auto cx = [](float &da, int32_t &la, float &db, int32_t &lb) {
+                if (db < da) {
+                    const float td = da;
+                    da = db;
+                    db = td;
+                    const int32_t tl = la;
+                    la = lb;
+                    lb = tl;
                 }
+            };
+            cx(d0, l0, d1, l1);
+            cx(d2, l2, d3, l3);
+            cx(d0, l0, d2, l2);
+            cx(d1, l1, d3, l3);
+            cx(d1, l1, d2, l2);
+            if (d0 == INFINITY) {
+                PVX_POP();
+                return;
+            }
+            auto push = [&](int32_t x) {
                 if (sp < kStackLds)
-                    s_stack[sp * kBlock + lane] = farc;
+                    s_stack[sp * kBlock + lane] = x;
                 else
-                    my_spill[sp - kStackLds] = farc;
+                    my_spill[sp - kStackLds] = x;
                 sp++;
-                node = nearc;
-                return;
-            }
-            if (h0) {
-                node = l.x;
-                return;
-            }
-            if (h1) {
-                node = l.y;
-                return;
-            }
-            PVX_POP();
+            };
+            if (d3 != INFINITY) push(l3);
+            if (d2 != INFINITY) push(l2);
+            if (d1 != INFINITY) push(l1);
+            node = l0;
         };
         // one leaf step (tests its primitives, then pops)
         auto step_leaf = [&]() {
@@ -327,10 +333,19 @@ __device__ __forceinline__ void trace_closest_body(const SceneView &sv0, const S
             tb = (float)bt * 1.000001f;
             PVX_POP();
         };
-        if constexpr (kWW) {
-            while (node != kEmpty) {
-                while (node >= 0) step_node();
-                while (node < 0 && node != kEmpty) step_leaf();
+        if constexpr (kLevel == 0) {
+            if constexpr (kWW) {
+                while (node != kEmpty) {
+                    while (node >= 0) step_node4();
+                    while (node < 0 && node != kEmpty) step_leaf();
+                }
+            } else {
+                while (node != kEmpty) {
+                    if (node >= 0)
+                        step_node4();
+                    else
+                        step_leaf();
+                }
             }
         } else {
             while (node != kEmpty) {
@@ -665,6 +680,11 @@ static void scene_free(pvx_scene *s) {
     s->shading.d_lights = nullptr;
 }
 
+// Kernel family of a scene: 0 (triangles / spheres through the 4-wide
+// tree), 1 (+ boxes) or 2 (+ tori, CSG).  A triangle scene whose 4-wide tree
+// is too deep for the traversal stack takes the binary tree (family 1).
+static uint32_t kernel_level(const SceneView &v) { return v.has_complex == 0 && !v.wide ? 1u : v.has_complex; }
+
 extern "C" int pvx_init(int device) {
     int n = 0;
     hipError_t e = hipGetDeviceCount(&n);
@@ -719,6 +739,7 @@ extern "C" int pvx_scene_create(const pvx_scene_desc *desc, const pvx_build_opts
     UP(h.obj_tri_base, obj_tri_base);
     UP(h.mesh_vtx, mesh_vtx);
     UP(h.mesh_nrm, mesh_nrm);
+    UP(h.nodes4, nodes4);
 #undef UP
     const uint32_t *tmp = nullptr;
     if ((r = upload(s, h.obj_type, &tmp)) != PVX_OK) { scene_free(s); delete s; return r; }
@@ -729,6 +750,9 @@ extern "C" int pvx_scene_create(const pvx_scene_desc *desc, const pvx_build_opts
     s->d_obj_index = const_cast<uint32_t *>(tmp);
     s->n_objects = desc->n_objects;
     v.root = h.root;
+    v.root4 = h.root4;
+    // the 4-wide walk pushes up to three links per level
+    v.wide = (h.has_complex == 0 && 3 * h.depth4 + 1 < (uint32_t)kStackMax) ? 1u : 0u;
     v.n_inf = (uint32_t)h.inf.size();
     v.any_flags = h.any_flags;
     v.has_complex = h.has_complex;
@@ -742,7 +766,7 @@ extern "C" int pvx_scene_create(const pvx_scene_desc *desc, const pvx_build_opts
         if (level == 1) return flags ? (const void *)k_trace_closest<1, true, false> : (const void *)k_trace_closest<1, false, false>;
         return flags ? (const void *)k_trace_closest_ww<true, false> : (const void *)k_trace_closest_ww<false, false>;
     };
-    auto kfn = pick(h.has_complex, h.any_flags != 0);
+    auto kfn = pick(kernel_level(v), h.any_flags != 0);
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, kBlock, 0));
     if (per_cu < 1) per_cu = 1;
     s->grid = (uint32_t)(cus * per_cu);
@@ -766,12 +790,13 @@ extern "C" int pvx_scene_destroy(pvx_scene *s) {
 
 extern "C" int pvx_scene_get_stats(const pvx_scene *s, pvx_scene_stats *out) {
     if (!s || !out) return fail(PVX_EINVAL, "null argument");
-    out->n_nodes = s->host.nodes.size();
+    const bool wide = s->view.wide != 0; // the tree the closest-hit kernels walk
+    out->n_nodes = wide ? s->host.nodes4.size() : s->host.nodes.size();
     out->n_prims = s->host.prims.size();
-    out->node_bytes = sizeof(Node);
+    out->node_bytes = wide ? sizeof(Node4) : sizeof(Node);
     out->tri_bytes = sizeof(PrimRec);
     out->device_bytes = s->device_bytes;
-    out->bvh_depth = s->host.depth;
+    out->bvh_depth = wide ? s->host.depth4 : s->host.depth;
     out->n_infinite = (uint32_t)s->host.inf.size();
     out->build_ms = s->host.build_ms;
     return PVX_OK;
@@ -793,7 +818,7 @@ static int launch_closest(pvx_scene *s, const pvx_ray *rays, const double *max_d
     k_trace_closest<L, F, kCount><<<nblocks, kBlock, 0, st>>>(s->view, rays, max_depth, ray_kind, kinds, nn, hits, s->d_spill, nv, pt)
 #define PVX_LAUNCH_WW(L, F) \
     k_trace_closest_ww<F, kCount><<<nblocks, kBlock, 0, st>>>(s->view, rays, max_depth, ray_kind, kinds, nn, hits, s->d_spill, nv, pt)
-    const uint32_t level = s->view.has_complex;
+    const uint32_t level = kernel_level(s->view);
     if (level >= 2) {
         if (flags) PVX_LAUNCH(2, true); else PVX_LAUNCH(2, false);
     } else if (level == 1) {
@@ -845,7 +870,7 @@ extern "C" int pvx_batch_create(pvx_scene *const *scenes, uint32_t n_scenes, pvx
         if (!s) return fail(PVX_EINVAL, "null scene in the batch");
         if (s->device != dev) return fail(PVX_EINVAL, "scene " + std::to_string(k) + " lives on another device");
         views[k] = s->view;
-        level = std::max(level, s->view.has_complex);
+        level = std::max(level, kernel_level(s->view));
         flags |= s->view.any_flags;
     }
     pvx_batch *b = new pvx_batch();

@@ -41,6 +41,16 @@ struct alignas(16) Node {
 };
 static_assert(sizeof(Node) == 64, "node is one 64 B record");
 
+// 4-wide node, 128 B (triangle / sphere scenes): the boxes of up to four
+// children as six float4 rows (x lo, x hi, y lo, y hi, z lo, z hi), their
+// links (same encoding as Node; INT32_MIN = no child in this slot).
+struct alignas(16) Node4 {
+    float lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
+    int32_t c[4];
+    int32_t pad[4];
+};
+static_assert(sizeof(Node4) == 128, "wide node is 128 B");
+
 // Mesh triangle after Compute_Mesh_Triangle (core/shape/mesh.cpp:838-945):
 // FP32 face normal (Normals[Normal_Ind]), FP32 Distance, vertices P1..P3 in
 // post-swap order, dominant axis.
@@ -173,6 +183,9 @@ struct SceneView {
     const uint32_t *obj_tri_base;    // per object: first ShadeTri of a mesh object
     const float *mesh_vtx;           // 3 per vertex (all meshes)
     const float *mesh_nrm;           // 3 per normal (all meshes: vertex + face normals)
+    const Node4 *nodes4;             // 4-wide tree over the same leaves (wide scenes)
+    int32_t root4;     // root link of the 4-wide tree
+    uint32_t wide;     // 1: triangle/sphere scene traversed through nodes4
     int32_t root;      // root link (node index or leaf link); INT32_MIN = empty
     uint32_t n_inf;
     uint32_t any_flags;

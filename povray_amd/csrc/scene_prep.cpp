@@ -523,6 +523,7 @@ int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene
     out.depth = bvh.depth;
     out.prims.resize(prims.size());
     for (size_t k = 0; k < prims.size(); k++) out.prims[k] = prims[bvh.order[k]];
+    if (out.has_complex == 0) collapse_bvh4(out.nodes, out.root, out.nodes4, out.root4, out.depth4);
     out.build_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return PVX_OK;

@@ -14,6 +14,9 @@ struct HostScene {
     std::vector<Node> nodes;
     int32_t root = INT32_MIN;
     uint32_t depth = 0;
+    std::vector<Node4> nodes4; // 4-wide tree (triangle / sphere scenes)
+    int32_t root4 = INT32_MIN;
+    uint32_t depth4 = 0;
     std::vector<InfRec> inf;
     std::vector<uint32_t> obj_flags;
     std::vector<uint32_t> obj_type;
