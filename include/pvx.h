@@ -41,7 +41,9 @@ enum {
     PVX_TORUS = 3,
     PVX_PLANE = 4,
     PVX_MESH = 5,
-    PVX_CSG = 6
+    PVX_CSG = 6,
+    PVX_TRIANGLE = 7,       /* stand-alone triangle (triangle.cpp, Triangle)              */
+    PVX_SMOOTH_TRIANGLE = 8 /* stand-alone smooth triangle (triangle.cpp, SmoothTriangle) */
 };
 
 /* Object flags: the bits of ObjectBase::Flags that the hot path reads
@@ -145,6 +147,14 @@ typedef struct pvx_scene_desc {
     const pvx_csg_node *csg;
     uint32_t n_csg_children;
     const pvx_object *csg_children;
+
+    /* stand-alone triangles (FP64, world space; not CSG children): an object
+     * of type PVX_TRIANGLE or PVX_SMOOTH_TRIANGLE refers to triangle `index`;
+     * a triangle is referred to as one of the two kinds only */
+    uint32_t n_triangles;
+    const double *triangle_vertices; /* 9n: P1, P2, P3                          */
+    const double *triangle_normals;  /* 9n: N1, N2, N3 (smooth ones; may be NULL
+                                        when no smooth triangle is listed)      */
 } pvx_scene_desc;
 
 /* ---- rays and hits ----------------------------------------------------- */

@@ -184,8 +184,22 @@ struct Object {
     bool infinite = false;
 };
 
+// A stand-alone triangle after Compute_Triangle (triangle.cpp:134-420).
+struct Tri {
+    V3 p1, p2, p3, n1, n2, n3, normal, perp;
+    double distance = 0;
+    int axis = 0, vaxis = 0;
+    bool smooth = false, degenerate = false;
+};
+Tri make_tri(const double *P, const double *N, bool smooth);
+bool tri_intersect(const Tri &t, const Ray &r, double *depth);
+V3 tri_normal(const Tri &t, V3 ip);
+FBox tri_bbox(const double *P);
+
 struct Scene {
     std::vector<Object> objects;
+    std::vector<Tri> tris;
+    std::vector<double> tri_p; // 9 per triangle, as given
     std::vector<V3> sph_c;
     std::vector<double> sph_r;
     std::vector<V3> pl_n;

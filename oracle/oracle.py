@@ -44,6 +44,7 @@ def lib():
                                            C.POINTER(A.PvxRadiositySettings), C.c_void_p,
                                            C.c_void_p]
         L.orc_solve_polynomial.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_double, C.c_void_p]
+        L.orc_triangle_normal.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t, C.c_void_p]
         _LIB = L
     return _LIB
 
@@ -147,6 +148,13 @@ class OracleScene:
         pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
         out = np.zeros_like(pts)
         lib().orc_mesh_normal(self.h, mesh, _p(tri), _p(pts), pts.shape[0], _p(out))
+        return out
+
+    def triangle_normal(self, tri, pts):
+        """Triangle / SmoothTriangle::Normal of stand-alone triangle `tri`."""
+        pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+        out = np.zeros_like(pts)
+        lib().orc_triangle_normal(self.h, tri, _p(pts), pts.shape[0], _p(out))
         return out
 
 

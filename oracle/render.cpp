@@ -69,6 +69,9 @@ V3 shape_normal(const Scene &s, uint32_t type, uint32_t index, const Isect &h, V
         return s.pl_n[index];
     case PVX_MESH:
         return mesh_normal(s.meshes[index], h.prim, ip);
+    case PVX_TRIANGLE:
+    case PVX_SMOOTH_TRIANGLE:
+        return tri_normal(s.tris[index], ip);
     case PVX_BOX: { // Box::Normal (box.cpp:600-625)
         static const double tbl[7][3] = {{0, 0, 0}, {-1, 0, 0}, {1, 0, 0}, {0, -1, 0},
                                          {0, 1, 0},  {0, 0, -1}, {0, 0, 1}};

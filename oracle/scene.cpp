@@ -51,6 +51,10 @@ FBox Scene::shape_bbox(uint32_t type, uint32_t index, uint32_t flags, bool *infi
     case PVX_TORUS:
         b = torus_bbox(tor_R[index], tor_r[index], tr(tor_tr[index]));
         break;
+    case PVX_TRIANGLE:
+    case PVX_SMOOTH_TRIANGLE:
+        b = tri_bbox(&tri_p[9 * (size_t)index]);
+        break;
     case PVX_CSG: {
         // CSG::Compute_BBox runs when the CSG is parsed, before any later
         // Invert() (which keeps the box): undo this node's inversion to get
@@ -164,6 +168,13 @@ bool Scene::all_intersections(uint32_t type, uint32_t index, uint32_t flags, con
     case PVX_TORUS:
         return torus_all_intersections(r, tr(tor_tr[index]), tor_R[index], tor_r[index],
                                        tor_sturm[index] != 0, st, slot);
+    case PVX_TRIANGLE:
+    case PVX_SMOOTH_TRIANGLE: { // Triangle::All_Intersections (triangle.cpp:426-445)
+        double depth;
+        if (!tri_intersect(tris[index], r, &depth)) return false;
+        st.push_back(Isect{depth, slot, 0, slot});
+        return true;
+    }
     case PVX_CSG: {
         const pvx_csg_node &n = csg[index];
         bool found = false;
@@ -222,6 +233,8 @@ bool Scene::inside(uint32_t type, uint32_t index, uint32_t flags, V3 p) const {
         return inv ? !((t + pl_d[index]) < kEpsilon) : ((t + pl_d[index]) < kEpsilon);
     }
     case PVX_MESH:
+    case PVX_TRIANGLE: // Triangle::Inside (triangle.cpp:608-611): never inside
+    case PVX_SMOOTH_TRIANGLE:
         return inv;
     case PVX_BOX:
         return box_inside(p, tr(box_tr[index]), box_c1[index], box_c2[index], inv);
@@ -311,6 +324,16 @@ static Scene *scene_create(const pvx_scene_desc &d) {
         s->tor_r.push_back(d.torus_minor[i]);
         s->tor_tr.push_back(d.torus_transform ? d.torus_transform[i] : -1);
         s->tor_sturm.push_back(d.torus_sturm ? d.torus_sturm[i] : 0);
+    }
+    if (d.n_triangles) s->tri_p.assign(d.triangle_vertices, d.triangle_vertices + 9 * (size_t)d.n_triangles);
+    s->tris.resize(d.n_triangles);
+    for (uint32_t i = 0; i < d.n_objects; i++) {
+        const pvx_object &o = d.objects[i];
+        if ((o.type == PVX_TRIANGLE || o.type == PVX_SMOOTH_TRIANGLE) && o.index < d.n_triangles) {
+            bool sm = o.type == PVX_SMOOTH_TRIANGLE;
+            s->tris[o.index] = make_tri(d.triangle_vertices + 9 * (size_t)o.index,
+                                        sm ? d.triangle_normals + 9 * (size_t)o.index : nullptr, sm);
+        }
     }
     s->csg.assign(d.csg, d.csg + d.n_csg);
     s->csg_children.assign(d.csg_children, d.csg_children + d.n_csg_children);
@@ -501,6 +524,19 @@ int orc_mesh_normal(void *sp, int mesh, const int32_t *tri, const double *pts, s
     const Scene &s = *static_cast<Scene *>(sp);
     for (size_t i = 0; i < n; i++) {
         V3 r = mesh_normal(s.meshes[mesh], tri[i], V3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]));
+        out[3 * i] = r.x;
+        out[3 * i + 1] = r.y;
+        out[3 * i + 2] = r.z;
+    }
+    return 0;
+}
+
+// Triangle::Normal / SmoothTriangle::Normal of stand-alone triangle `tri`
+// at n points.
+int orc_triangle_normal(void *sp, int tri, const double *pts, size_t n, double *out) {
+    const Scene &s = *static_cast<Scene *>(sp);
+    for (size_t i = 0; i < n; i++) {
+        V3 r = tri_normal(s.tris[tri], V3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]));
         out[3 * i] = r.x;
         out[3 * i + 1] = r.y;
         out[3 * i + 2] = r.z;

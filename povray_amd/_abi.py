@@ -9,6 +9,7 @@ import ctypes as C
 PVX_OK, PVX_EINVAL, PVX_ENOMEM, PVX_EDEVICE, PVX_EUNSUPPORTED, PVX_ESTATE = range(6)
 
 PVX_SPHERE, PVX_BOX, PVX_TORUS, PVX_PLANE, PVX_MESH, PVX_CSG = 1, 2, 3, 4, 5, 6
+PVX_TRIANGLE, PVX_SMOOTH_TRIANGLE = 7, 8
 
 PVX_FLAG_NO_SHADOW = 1 << 0
 PVX_FLAG_NO_IMAGE = 1 << 1
@@ -59,6 +60,7 @@ class PvxSceneDesc(C.Structure):
         ("n_meshes", C.c_uint32), ("meshes", C.POINTER(PvxMeshDesc)),
         ("n_csg", C.c_uint32), ("csg", C.POINTER(PvxCsgNode)),
         ("n_csg_children", C.c_uint32), ("csg_children", C.POINTER(PvxObject)),
+        ("n_triangles", C.c_uint32), ("triangle_vertices", c_dp), ("triangle_normals", c_dp),
     ]
 
 

@@ -58,6 +58,23 @@ __device__ inline D3 leaf_normal(const NormalParams &p, uint32_t type, uint32_t 
         if (dist > kEpsilon) M = D3{tr.R * P.x / dist, 0.0, tr.R * P.z / dist};
         return normalize3(mtrans_normal(x.inv, sub3(P, M)));
     }
+    case PVX_TRIANGLE: { // Triangle::Normal (triangle.cpp:641-644)
+        const TriangleRec &t = sv.triangles[index];
+        return D3{t.n[0], t.n[1], t.n[2]};
+    }
+    case PVX_SMOOTH_TRIANGLE: { // SmoothTriangle::Normal (triangle.cpp:712-738)
+        const TriangleRec &t = sv.triangles[index];
+        const D3 pi{ip.x - t.p1[0], ip.y - t.p1[1], ip.z - t.p1[2]};
+        const double u = ((pi.x * t.perp[0]) + (pi.y * t.perp[1])) + (pi.z * t.perp[2]);
+        if (u < kEpsilon) return D3{t.n1[0], t.n1[1], t.n1[2]};
+        const int a = t.vaxis;
+        const double v = (comp(pi, a) / u + t.p1[a] - t.p2[a]) / (t.p3[a] - t.p2[a]);
+        D3 r;
+        r.x = t.n1[0] + u * (t.n2[0] - t.n1[0] + v * (t.n3[0] - t.n2[0]));
+        r.y = t.n1[1] + u * (t.n2[1] - t.n1[1] + v * (t.n3[1] - t.n2[1]));
+        r.z = t.n1[2] + u * (t.n2[2] - t.n1[2] + v * (t.n3[2] - t.n2[2]));
+        return normalize3(r);
+    }
     default:
         return D3{0, 1, 0};
     }

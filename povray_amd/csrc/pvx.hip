@@ -111,9 +111,16 @@ __device__ inline bool csg_closest(const SceneView &sv, uint32_t root, D3 o, D3 
     return found;
 }
 
-// Closest valid hit of a box / torus object (Find_Intersection over its IStack).
+// Closest valid hit of a box / torus / stand-alone triangle object
+// (Find_Intersection over its IStack).
 __device__ inline bool simple_closest(const SceneView &sv, uint32_t kind, uint32_t index, D3 o,
                                       D3 d, bool shadow, double &dep_out) {
+    if (kind == kPrimTriangle64) {
+        double dep;
+        if (!tri_rec_hit(sv.triangles[index], o, d, dep) || !depth_ok(dep, shadow)) return false;
+        dep_out = dep;
+        return true;
+    }
     double hs[kMaxLeafHits];
     int side[2];
     int nh = kind == kPrimBox ? box_hits(sv.boxes[index], sv.xforms, o, d, hs, side)
@@ -276,7 +283,7 @@ __device__ __forceinline__ void trace_closest_body(const SceneView &sv0, const S
             if (d1 != INFINITY) push(l1);
             node = l0;
         };
-        // one leaf step (tests its primitives, then pops)
+        // one leaf step of a triangle / sphere scene (tests its primitives, then pops)
         auto step_leaf = [&]() {
             // leaf: ~node = first << 3 | (count - 1)
             const uint32_t leaf = (uint32_t)(~node);
@@ -305,29 +312,6 @@ __device__ __forceinline__ void trace_closest_body(const SceneView &sv0, const S
                             bo = obj;
                             bp = 0;
                         }
-                } else if (kLevel == 1) { // boxes only
-                    double hs[2];
-                    int side[2];
-                    int nh = box_hits(sv.boxes[p.ref.index], sv.xforms, o, d, hs, side);
-                    for (int j = 0; j < nh; j++)
-                        if (depth_ok(hs[j], shadow) && better(hs[j], obj, 0, bt, bo, bp)) {
-                            bt = hs[j];
-                            bo = obj;
-                            bp = 0;
-                        }
-                } else if (kLevel >= 2) {
-                    double dep;
-                    int32_t prim = 0;
-                    bool ok;
-                    if (pk == kPrimCsg)
-                        ok = csg_closest(sv, p.ref.index, o, d, shadow, bt, dep, prim);
-                    else
-                        ok = simple_closest(sv, pk, p.ref.index, o, d, shadow, dep);
-                    if (ok && better(dep, obj, prim, bt, bo, bp)) {
-                        bt = dep;
-                        bo = obj;
-                        bp = prim;
-                    }
                 }
             }
             tb = (float)bt * 1.000001f;
@@ -419,16 +403,26 @@ __device__ __forceinline__ void trace_closest_body(const SceneView &sv0, const S
                                     bo = obj;
                                     bp = 0;
                                 }
-                        } else if (kLevel == 1) { // boxes only
-                            double hs[2];
-                            int side[2];
-                            int nh = box_hits(sv.boxes[p.ref.index], sv.xforms, o, d, hs, side);
-                            for (int j = 0; j < nh; j++)
-                                if (depth_ok(hs[j], shadow) && better(hs[j], obj, 0, bt, bo, bp)) {
-                                    bt = hs[j];
+                        } else if (kLevel == 1) { // boxes and stand-alone triangles
+                            if (pk == kPrimTriangle64) {
+                                double dep;
+                                if (tri_rec_hit(sv.triangles[p.ref.index], o, d, dep) && depth_ok(dep, shadow) &&
+                                    better(dep, obj, 0, bt, bo, bp)) {
+                                    bt = dep;
                                     bo = obj;
                                     bp = 0;
                                 }
+                            } else {
+                                double hs[2];
+                                int side[2];
+                                int nh = box_hits(sv.boxes[p.ref.index], sv.xforms, o, d, hs, side);
+                                for (int j = 0; j < nh; j++)
+                                    if (depth_ok(hs[j], shadow) && better(hs[j], obj, 0, bt, bo, bp)) {
+                                        bt = hs[j];
+                                        bo = obj;
+                                        bp = 0;
+                                    }
+                            }
                         } else if (kLevel >= 2) {
                             double dep;
                             int32_t prim = 0;
@@ -740,6 +734,7 @@ extern "C" int pvx_scene_create(const pvx_scene_desc *desc, const pvx_build_opts
     UP(h.mesh_vtx, mesh_vtx);
     UP(h.mesh_nrm, mesh_nrm);
     UP(h.nodes4, nodes4);
+    UP(h.triangles, triangles);
 #undef UP
     const uint32_t *tmp = nullptr;
     if ((r = upload(s, h.obj_type, &tmp)) != PVX_OK) { scene_free(s); delete s; return r; }

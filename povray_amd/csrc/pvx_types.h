@@ -31,6 +31,7 @@ enum PrimKind : uint32_t {
     kPrimTorus = 3,
     kPrimCsg = 4,
     kPrimPlane = 5,
+    kPrimTriangle64 = 6, // stand-alone triangle, by reference (FP64 record)
 };
 
 struct alignas(16) Node {
@@ -93,6 +94,19 @@ struct BoxRec {
     double c1[3], c2[3];
     int32_t xf;    // -1 = none
     uint32_t flags;
+};
+
+// Triangle / SmoothTriangle after Compute_Triangle and
+// Compute_Smooth_Triangle (core/shape/triangle.cpp:134-420): FP64 unit
+// normal and Distance, P1..P3 in post-swap order, dominant axis; smooth
+// ones also N1..N3 (post-swap), Perp and vAxis.  degenerate: never hit.
+struct TriangleRec {
+    double n[3];
+    double dist;
+    double p1[3], p2[3], p3[3];
+    double n1[3], n2[3], n3[3];
+    double perp[3];
+    int32_t axis, vaxis, smooth, degenerate;
 };
 
 struct TorusRec {
@@ -183,13 +197,14 @@ struct SceneView {
     const uint32_t *obj_tri_base;    // per object: first ShadeTri of a mesh object
     const float *mesh_vtx;           // 3 per vertex (all meshes)
     const float *mesh_nrm;           // 3 per normal (all meshes: vertex + face normals)
+    const TriangleRec *triangles;    // stand-alone triangles (by triangle index)
     const Node4 *nodes4;             // 4-wide tree over the same leaves (wide scenes)
     int32_t root4;     // root link of the 4-wide tree
     uint32_t wide;     // 1: triangle/sphere scene traversed through nodes4
     int32_t root;      // root link (node index or leaf link); INT32_MIN = empty
     uint32_t n_inf;
     uint32_t any_flags;
-    uint32_t has_complex; // 0: triangles/spheres only, 1: + boxes, 2: + tori or CSG
+    uint32_t has_complex; // 0: mesh triangles/spheres only, 1: + boxes, stand-alone triangles, 2: + tori or CSG
 };
 
 } // namespace pvx

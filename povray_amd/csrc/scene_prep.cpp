@@ -119,6 +119,112 @@ void smooth_triangle(const double *P1, const double *P2, const double *P3, Shade
     t.vaxis_smooth = (uint32_t)vaxis;
 }
 
+// Triangle::Compute_Triangle / SmoothTriangle::Compute_Triangle and
+// Compute_Smooth_Triangle (core/shape/triangle.cpp:95-420) on the parsed
+// points; smooth normals are first normalised as Parse_Smooth_Triangle
+// does (parser/parser.cpp:5555-5583).  Same operations in the same order,
+// so every stored value matches the reference's bits.
+TriangleRec make_triangle(const double *P, const double *N, bool smooth) {
+    TriangleRec t;
+    std::memset(&t, 0, sizeof t);
+    double P1[3], P2[3], P3[3], N1[3] = {0, 0, 0}, N2[3] = {0, 0, 0}, N3[3] = {0, 0, 0};
+    for (int k = 0; k < 3; k++) {
+        P1[k] = P[k];
+        P2[k] = P[3 + k];
+        P3[k] = P[6 + k];
+    }
+    auto length = [](const double *v) { return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); };
+    auto dot = [](const double *a, const double *b) { return ((a[0] * b[0]) + (a[1] * b[1])) + (a[2] * b[2]); };
+    auto store = [&]() {
+        for (int k = 0; k < 3; k++) {
+            t.p1[k] = P1[k];
+            t.p2[k] = P2[k];
+            t.p3[k] = P3[k];
+            t.n1[k] = N1[k];
+            t.n2[k] = N2[k];
+            t.n3[k] = N3[k];
+        }
+    };
+    t.smooth = smooth ? 1 : 0;
+    if (smooth) {
+        double *Ns[3] = {N1, N2, N3};
+        bool degen = false;
+        for (int j = 0; j < 3; j++) {
+            for (int k = 0; k < 3; k++) Ns[j][k] = N[3 * j + k];
+            const double l = length(Ns[j]);
+            if (l == 0.0)
+                degen = true;
+            else
+                for (int k = 0; k < 3; k++) Ns[j][k] /= l;
+        }
+        if (degen) { // a zero normal: no triangle is computed
+            t.degenerate = 1;
+            store();
+            return t;
+        }
+    }
+    double V1[3], V2[3], Nv[3];
+    for (int k = 0; k < 3; k++) {
+        V1[k] = P1[k] - P2[k];
+        V2[k] = P3[k] - P2[k];
+    }
+    Nv[0] = (V1[1] * V2[2]) - (V1[2] * V2[1]);
+    Nv[1] = (V1[2] * V2[0]) - (V1[0] * V2[2]);
+    Nv[2] = (V1[0] * V2[1]) - (V1[1] * V2[0]);
+    const double len = length(Nv);
+    if (len == 0.0) { // DEGENERATE_FLAG
+        t.degenerate = 1;
+        store();
+        return t;
+    }
+    for (int k = 0; k < 3; k++) Nv[k] /= len;
+    const double dist = -dot(Nv, P1);
+    const double ax = std::fabs(Nv[0]), ay = std::fabs(Nv[1]), az = std::fabs(Nv[2]);
+    const int axis = (ax > ay) ? ((ax > az) ? 0 : 2) : ((ay > az) ? 1 : 2);
+    bool swap = false;
+    switch (axis) {
+    case 0: swap = (P2[1] - P3[1]) * (P2[2] - P1[2]) < (P2[2] - P3[2]) * (P2[1] - P1[1]); break;
+    case 1: swap = (P2[0] - P3[0]) * (P2[2] - P1[2]) < (P2[2] - P3[2]) * (P2[0] - P1[0]); break;
+    default: swap = (P2[0] - P3[0]) * (P2[1] - P1[1]) < (P2[1] - P3[1]) * (P2[0] - P1[0]); break;
+    }
+    if (swap) {
+        for (int k = 0; k < 3; k++) {
+            std::swap(P1[k], P2[k]);
+            std::swap(N1[k], N2[k]);
+        }
+    }
+    for (int k = 0; k < 3; k++) t.n[k] = Nv[k];
+    t.dist = dist;
+    t.axis = axis;
+    if (smooth) { // Compute_Smooth_Triangle
+        double d32[3], v1[3], v2[3], perp[3];
+        for (int k = 0; k < 3; k++) d32[k] = P3[k] - P2[k];
+        const double x = std::fabs(d32[0]), y = std::fabs(d32[1]), z = std::fabs(d32[2]);
+        t.vaxis = (x > y) ? ((x > z) ? 0 : 2) : ((y > z) ? 1 : 2);
+        for (int k = 0; k < 3; k++) v1[k] = P2[k] - P3[k];
+        const double l1 = length(v1);
+        if (l1 != 0)
+            for (int k = 0; k < 3; k++) v1[k] = v1[k] / l1;
+        for (int k = 0; k < 3; k++) v2[k] = P1[k] - P3[k];
+        const double proj = dot(v2, v1);
+        for (int k = 0; k < 3; k++) v1[k] *= proj;
+        for (int k = 0; k < 3; k++) perp[k] = v1[k] - v2[k];
+        const double lp = length(perp);
+        if (lp != 0)
+            for (int k = 0; k < 3; k++) perp[k] = perp[k] / lp;
+        const double uden = dot(v2, perp);
+        for (int k = 0; k < 3; k++) perp[k] /= -uden;
+        for (int k = 0; k < 3; k++) t.perp[k] = perp[k];
+        // smooth normals more than 90 degrees from the face normal (or its
+        // inverse) on some vertex: DEGENERATE_FLAG
+        const double dx = dot(Nv, N1), dy = dot(Nv, N2), dz = dot(Nv, N3);
+        if (!(((dx < 0.0) && (dy < 0.0) && (dz < 0.0)) || ((dx > 0.0) && (dy > 0.0) && (dz > 0.0))))
+            t.degenerate = 1;
+    }
+    store();
+    return t;
+}
+
 } // namespace
 
 int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene &out,
@@ -140,6 +246,7 @@ int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene
     if (!need(d.n_transforms == 0 || d.transforms, "transforms")) return PVX_EINVAL;
     if (!need(d.n_meshes == 0 || d.meshes, "meshes")) return PVX_EINVAL;
     if (!need(d.n_csg == 0 || (d.csg && d.csg_children), "csg")) return PVX_EINVAL;
+    if (!need(d.n_triangles == 0 || d.triangle_vertices, "triangles")) return PVX_EINVAL;
 
     auto check_ref = [&](uint32_t type, uint32_t index) -> bool {
         uint32_t lim = 0;
@@ -150,6 +257,8 @@ int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene
         case PVX_PLANE: lim = d.n_planes; break;
         case PVX_MESH: lim = d.n_meshes; break;
         case PVX_CSG: lim = d.n_csg; break;
+        case PVX_TRIANGLE:
+        case PVX_SMOOTH_TRIANGLE: lim = d.n_triangles; break;
         default:
             err = "unknown object type " + std::to_string(type);
             return false;
@@ -218,6 +327,31 @@ int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene
         t.pad = 0;
     }
 
+    // stand-alone triangles, flat or smooth as their objects say
+    {
+        std::vector<uint8_t> kind(d.n_triangles, 0);
+        for (uint32_t o = 0; o < d.n_objects; o++) {
+            const pvx_object &ob = d.objects[o];
+            if (ob.type != PVX_TRIANGLE && ob.type != PVX_SMOOTH_TRIANGLE) continue;
+            if (ob.index >= d.n_triangles) continue; // reported by check_ref below
+            const uint8_t k = ob.type == PVX_SMOOTH_TRIANGLE ? 2 : 1;
+            if (kind[ob.index] && kind[ob.index] != k) {
+                err = "triangle " + std::to_string(ob.index) + " listed as both triangle and smooth_triangle";
+                return PVX_EINVAL;
+            }
+            if (k == 2 && !d.triangle_normals) {
+                err = "smooth triangles need triangle_normals";
+                return PVX_EINVAL;
+            }
+            kind[ob.index] = k;
+        }
+        out.triangles.resize(d.n_triangles);
+        for (uint32_t i = 0; i < d.n_triangles; i++)
+            out.triangles[i] = make_triangle(d.triangle_vertices + 9 * (size_t)i,
+                                             kind[i] == 2 ? d.triangle_normals + 9 * (size_t)i : nullptr,
+                                             kind[i] == 2);
+    }
+
     // --- CSG tables ------------------------------------------------------------
     out.csg.resize(d.n_csg);
     out.csg_children.resize(d.n_csg_children);
@@ -237,8 +371,8 @@ int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene
             uint32_t slot = n.first_child + k;
             const pvx_object &c = d.csg_children[slot];
             if (!check_ref(c.type, c.index)) return PVX_EINVAL;
-            if (c.type == PVX_MESH) {
-                err = "meshes inside CSG are not supported";
+            if (c.type == PVX_MESH || c.type == PVX_TRIANGLE || c.type == PVX_SMOOTH_TRIANGLE) {
+                err = "meshes and triangles inside CSG are not supported";
                 return PVX_EUNSUPPORTED;
             }
             out.csg_children[slot] = CsgChild{c.type, c.index, c.flags, i};
@@ -420,6 +554,23 @@ int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene
         case PVX_PLANE:
             out.inf.push_back(InfRec{kPrimPlane, ob.index, (int32_t)o, 0});
             break;
+        case PVX_TRIANGLE:
+        case PVX_SMOOTH_TRIANGLE: {
+            const TriangleRec &t = out.triangles[ob.index];
+            out.has_complex = std::max<uint32_t>(out.has_complex, 1u);
+            if (t.degenerate) break; // never hit
+            DBox bb = DBox::empty(); // Triangle::Compute_BBox (triangle.cpp:1296-1311): vertices +- EPSILON
+            const double *P = d.triangle_vertices + 9 * (size_t)ob.index;
+            for (int k = 0; k < 3; k++) {
+                bb.lo[k] = std::min(P[k], std::min(P[3 + k], P[6 + k])) - kEpsilon;
+                bb.hi[k] = std::max(P[k], std::max(P[3 + k], P[6 + k])) + kEpsilon;
+            }
+            pr.kind_axis = kPrimTriangle64;
+            pr.ref.index = ob.index;
+            to_fbox(bb, lo, hi);
+            add_prim(pr, lo, hi);
+            break;
+        }
         case PVX_MESH: {
             const pvx_mesh_desc &m = d.meshes[ob.index];
             if (!m.vertices || !m.indices) {

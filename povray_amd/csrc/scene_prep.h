@@ -22,6 +22,7 @@ struct HostScene {
     std::vector<uint32_t> obj_type;
     std::vector<PlaneRec> planes;
     std::vector<BoxRec> boxes;
+    std::vector<TriangleRec> triangles;
     std::vector<TorusRec> tori;
     std::vector<Xform> xforms;
     std::vector<double> sph; // 4 per sphere: centre, radius

@@ -39,6 +39,25 @@ __device__ inline D3 xf_dir(const double *m, D3 p) {
 // ---------------------------------------------------------------------------
 // mesh2 triangle: Mesh::intersect_mesh_triangle (core/shape/mesh.cpp:1040-1136)
 // ---------------------------------------------------------------------------
+// Triangle::Intersect (core/shape/triangle.cpp:479-572) on the FP64
+// record: plane depth in [DEPTH_TOLERANCE, MAX_DISTANCE], then the three
+// edge tests in the dominant-axis projection (an exact tie counts inside).
+__device__ inline bool tri_rec_hit(const TriangleRec &t, D3 o, D3 d, double &depth) {
+    if (t.degenerate) return false;
+    const double ndd = ((t.n[0] * d.x) + (t.n[1] * d.y)) + (t.n[2] * d.z);
+    if (fabs(ndd) < kEpsilon) return false;
+    const double ndo = ((t.n[0] * o.x) + (t.n[1] * o.y)) + (t.n[2] * o.z);
+    depth = -(t.dist + ndo) / ndd;
+    if ((depth < 1.0e-6) || (depth > kMaxDistance)) return false;
+    const int a = t.axis == 0 ? 1 : 0, b = t.axis == 2 ? 1 : 2; // projected axes
+    const double s = comp(o, a) + depth * comp(d, a);
+    const double u = comp(o, b) + depth * comp(d, b);
+    if ((t.p2[a] - s) * (t.p2[b] - t.p1[b]) < (t.p2[b] - u) * (t.p2[a] - t.p1[a])) return false;
+    if ((t.p3[a] - s) * (t.p3[b] - t.p2[b]) < (t.p3[b] - u) * (t.p3[a] - t.p2[a])) return false;
+    if ((t.p1[a] - s) * (t.p1[b] - t.p3[b]) < (t.p1[b] - u) * (t.p1[a] - t.p3[a])) return false;
+    return true;
+}
+
 __device__ inline bool tri_hit64(const TriRec &t, uint32_t axis, D3 o, D3 d, double &depth) {
     double nx = t.n[0], ny = t.n[1], nz = t.n[2];
     double ndd = nx * d.x + ny * d.y + nz * d.z;
@@ -694,7 +713,7 @@ __device__ bool csg_inside(const SceneView &sv, uint32_t node, D3 p) {
     }
 }
 
-// Hits of one leaf shape (sphere/box/torus) in push order.
+// Hits of one leaf shape (sphere/box/torus/triangle) in push order.
 __device__ inline int leaf_hits(const SceneView &sv, uint32_t type, uint32_t index, D3 o, D3 d,
                                 double *out) {
     int side[2];
@@ -707,6 +726,13 @@ __device__ inline int leaf_hits(const SceneView &sv, uint32_t type, uint32_t ind
         return box_hits(sv.boxes[index], sv.xforms, o, d, out, side);
     case 3:
         return torus_hits(sv.tori[index], sv.xforms, o, d, out);
+    case 7:
+    case 8: { // stand-alone triangle (not a CSG child; All_Intersections)
+        double dep;
+        if (!tri_rec_hit(sv.triangles[index], o, d, dep)) return 0;
+        out[0] = dep;
+        return 1;
+    }
     default:
         return 0;
     }

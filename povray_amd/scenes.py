@@ -91,6 +91,8 @@ class Scene:
     meshes: list = field(default_factory=list)  # (verts f32 (n,3), idx i32 (m,3), normals f32|None)
     csg: np.ndarray = None                   # (n, 4) uint32: op, first, count, pad
     csg_children: np.ndarray = None          # (n, 4) uint32 like objects
+    triangle_vertices: np.ndarray = None     # (n, 9) float64: P1, P2, P3
+    triangle_normals: np.ndarray = None      # (n, 9) float64: N1, N2, N3 (smooth triangles)
     camera: dict = None
     lights: list = field(default_factory=list)
     materials: dict = field(default_factory=dict)  # per top-level object arrays
@@ -156,6 +158,11 @@ class Scene:
             keep.append(md)
             d.n_meshes = len(self.meshes)
             d.meshes = C.cast(md, C.POINTER(A.PvxMeshDesc))
+        if self.triangle_vertices is not None and len(self.triangle_vertices):
+            d.triangle_vertices, _ = arr(np.asarray(self.triangle_vertices).reshape(-1), np.float64, C.c_double)
+            d.n_triangles = int(np.asarray(self.triangle_vertices).reshape(-1, 9).shape[0])
+            if self.triangle_normals is not None:
+                d.triangle_normals, _ = arr(np.asarray(self.triangle_normals).reshape(-1), np.float64, C.c_double)
         if self.csg is not None and len(self.csg):
             c = np.ascontiguousarray(self.csg, np.uint32).reshape(-1, 4)
             ch = np.ascontiguousarray(self.csg_children, np.uint32).reshape(-1, 4)
@@ -692,3 +699,46 @@ def batch_scenes(plan=None, **kw):
     plan = plan if plan is not None else batch_plan(**kw)
     return [batch_scene(p["seed"], p["n_tris"], p["n_spheres"], p["width"], p["height"],
                         name=f"batch{k:02d}") for k, p in enumerate(plan)]
+
+
+# --------------------------------------------------------------------------
+# stand-alone triangles
+# --------------------------------------------------------------------------
+def triangle_scene(seed: int = 0xC0FFEE + 5, n: int = 400, n_spheres: int = 40, width=320, height=240) -> Scene:
+    """`n` seeded stand-alone triangles (every other one a smooth_triangle
+    whose vertex normals are the face normal tilted by up to ~35 degrees;
+    one in 16 of those with a vertex normal flipped, which the reference
+    marks degenerate) in a 12-unit cube, `n_spheres` spheres, a ground
+    plane, one point light; 10% of the triangles reflective."""
+    r = Rng(seed)
+    c = r.uniform(3 * n, -6.0, 6.0).reshape(n, 1, 3)
+    e = r.uniform(9 * n, -1.2, 1.2).reshape(n, 3, 3)
+    v = (c + e).reshape(n, 9)
+    fn = np.cross(e[:, 0] - e[:, 1], e[:, 2] - e[:, 1])
+    fn /= np.maximum(np.linalg.norm(fn, axis=1, keepdims=True), 1e-300)
+    tilt = r.uniform(9 * n, -0.7, 0.7).reshape(n, 3, 3)
+    nv = fn[:, None, :] + tilt
+    flip = (np.arange(n) % 32) == 31
+    nv[flip, 2] *= -1.0
+    smooth = (np.arange(n) % 2) == 1
+    cs = r.uniform(3 * n_spheres, -6.0, 6.0).reshape(n_spheres, 3)
+    rs = r.uniform(n_spheres, 0.2, 0.7)
+    m = n + n_spheres + 1
+    objects = np.zeros((m, 4), np.uint32)
+    objects[:n, 0] = np.where(smooth, A.PVX_SMOOTH_TRIANGLE, A.PVX_TRIANGLE)
+    objects[:n, 1] = np.arange(n)
+    objects[n:n + n_spheres, 0] = A.PVX_SPHERE
+    objects[n:n + n_spheres, 1] = np.arange(n_spheres)
+    objects[-1, 0] = A.PVX_PLANE
+    alb = r.uniform(3 * m, 0.2, 0.95).reshape(m, 3)
+    refl = np.where(r.uniform(m) < 0.1, 0.3, 0.0)
+    return Scene(
+        name=f"triangles{n}", objects=objects, triangle_vertices=v, triangle_normals=nv.reshape(n, 9),
+        sphere_center=cs, sphere_radius=rs,
+        plane_normal=np.array([[0.0, 1.0, 0.0]]), plane_distance=np.array([7.5]),
+        camera=look_at_camera((0.0, 6.0, -20.0), (0.0, 0.0, 0.0), width, height),
+        lights=[dict(kind="point", position=np.array([15.0, 25.0, -15.0]), colour=np.ones(3))],
+        materials=dict(pigment=alb, diffuse=np.full(m, 0.6), ambient=np.full(m, 0.1), phong=np.full(m, 0.4),
+                       phong_size=np.full(m, 40.0), reflection=refl, ior=np.ones(m), transmit=np.zeros(m)),
+        settings=dict(max_trace_level=5, adc_bailout=1.0 / 255.0),
+    )
