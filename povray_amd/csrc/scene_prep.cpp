@@ -230,8 +230,8 @@ TriangleRec make_triangle(const double *P, const double *N, bool smooth) {
 int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene &out,
                   std::string &err) {
     auto t0 = std::chrono::steady_clock::now();
-    if (d.n_objects == 0 || !d.objects) {
-        err = "scene has no objects";
+    if (d.n_objects > 0 && !d.objects) { // an empty scene is valid: every ray misses
+        err = "null object table";
         return PVX_EINVAL;
     }
     // --- validate arrays -----------------------------------------------------
