@@ -8,6 +8,14 @@
  *
  * Each entry point names the reference interface it replaces (paths relative
  * to the POV-Ray 3.8 source tree).
+ *
+ * Streams: the per-ray entry points (pvx_trace_closest[_counted],
+ * pvx_all_intersections, pvx_inside) may run on one scene from several
+ * streams at once — each stream gets its own traversal-stack area, allocated
+ * on its first launch.  The frame-level calls (pvx_render, pvx_render_samples,
+ * pvx_radiosity_render) keep per-scene work buffers, and a batch keeps its
+ * chunk table and stack area: calls on one scene's frames, or on one batch,
+ * must not overlap in time.
  */
 #ifndef PVX_H
 #define PVX_H

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -74,7 +75,11 @@ struct pvx_scene {
     pvx::SceneView view{};
     std::vector<void *> allocs;
     uint32_t *d_obj_type = nullptr, *d_obj_index = nullptr;
-    int32_t *d_spill = nullptr;
+    // Traversal-stack spill areas, one per stream that has traced this scene
+    // (launches on different streams may overlap; each needs its own).
+    std::mutex spill_mu;
+    std::vector<std::pair<hipStream_t, int32_t *>> spills;
+    size_t spill_bytes = 0;
     uint32_t grid = 0;
     uint64_t device_bytes = 0;
     uint32_t n_objects = 0;

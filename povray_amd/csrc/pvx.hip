@@ -765,13 +765,26 @@ extern "C" int pvx_scene_create(const pvx_scene_desc *desc, const pvx_build_opts
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, kBlock, 0));
     if (per_cu < 1) per_cu = 1;
     s->grid = (uint32_t)(cus * per_cu);
-    void *spill = nullptr;
-    size_t spill_bytes = (size_t)s->grid * kBlock * kStackSpill * sizeof(int32_t);
-    HIP_TRY(hipMalloc(&spill, spill_bytes));
-    s->allocs.push_back(spill);
-    s->d_spill = static_cast<int32_t *>(spill);
-    s->device_bytes += spill_bytes;
+    s->spill_bytes = (size_t)s->grid * kBlock * kStackSpill * sizeof(int32_t);
     *out = s;
+    return PVX_OK;
+}
+
+// The scene's spill area for launches on `st` (allocated on the stream's
+// first launch; freed with the scene).
+static int scene_spill(pvx_scene *s, hipStream_t st, int32_t **out) {
+    std::lock_guard<std::mutex> lock(s->spill_mu);
+    for (const auto &e : s->spills)
+        if (e.first == st) {
+            *out = e.second;
+            return PVX_OK;
+        }
+    void *p = nullptr;
+    HIP_TRY(hipMalloc(&p, s->spill_bytes));
+    s->allocs.push_back(p);
+    s->device_bytes += s->spill_bytes;
+    s->spills.emplace_back(st, static_cast<int32_t *>(p));
+    *out = static_cast<int32_t *>(p);
     return PVX_OK;
 }
 
@@ -806,13 +819,16 @@ static int launch_closest(pvx_scene *s, const pvx_ray *rays, const double *max_d
     if (!rays || !hits) return fail(PVX_EINVAL, "null ray or hit buffer");
     if (n > 0xFFFFFFFFull) return fail(PVX_EINVAL, "too many rays in one call");
     hipStream_t st = static_cast<hipStream_t>(stream);
+    int32_t *spill = nullptr;
+    int r;
+    if ((r = scene_spill(s, st, &spill))) return r;
     uint32_t nblocks = (uint32_t)std::min<size_t>(s->grid, (n + kBlock - 1) / kBlock);
     bool flags = s->view.any_flags != 0 && (ray_kind != PVX_RAY_PRIMARY || kinds);
     uint32_t nn = (uint32_t)n;
 #define PVX_LAUNCH(L, F) \
-    k_trace_closest<L, F, kCount><<<nblocks, kBlock, 0, st>>>(s->view, rays, max_depth, ray_kind, kinds, nn, hits, s->d_spill, nv, pt)
+    k_trace_closest<L, F, kCount><<<nblocks, kBlock, 0, st>>>(s->view, rays, max_depth, ray_kind, kinds, nn, hits, spill, nv, pt)
 #define PVX_LAUNCH_WW(L, F) \
-    k_trace_closest_ww<F, kCount><<<nblocks, kBlock, 0, st>>>(s->view, rays, max_depth, ray_kind, kinds, nn, hits, s->d_spill, nv, pt)
+    k_trace_closest_ww<F, kCount><<<nblocks, kBlock, 0, st>>>(s->view, rays, max_depth, ray_kind, kinds, nn, hits, spill, nv, pt)
     const uint32_t level = kernel_level(s->view);
     if (level >= 2) {
         if (flags) PVX_LAUNCH(2, true); else PVX_LAUNCH(2, false);

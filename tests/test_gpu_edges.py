@@ -88,3 +88,26 @@ def test_zero_area_mesh_triangles(gpu):
                      camera=scenes.look_at_camera((0.0, 0.3, -3.0), (0.0, 0.0, 0.0), 160, 120))
     r = scenes.camera_rays_np(s.camera)
     _same(gpu.GpuScene(s).find_intersection(r), OracleScene(s).find_intersection(r), "zero-area triangles")
+
+
+def test_overlapping_launches_on_two_streams(gpu):
+    """Two streams tracing the same scene at once (each with its own stack
+    spill area) give the single-stream results."""
+    from povray_amd import hip
+
+    s = scenes.mesh_scene(level=6, width=640, height=360)
+    gs = gpu.GpuScene(s)
+    rays = scenes.camera_rays_np(s.camera)
+    n = rays.shape[0]
+    ref = gs.find_intersection(rays)
+    d_r = hip.DeviceBuffer.from_array(rays)
+    outs = [hip.DeviceBuffer(n * 16) for _ in range(2)]
+    sts = [hip.Stream(), hip.Stream()]
+    for _ in range(3):
+        for k in range(2):
+            gs.trace_closest(d_r, n, outs[k], stream=sts[k])
+    for st in sts:
+        st.synchronize()
+    for k in range(2):
+        got = outs[k].download(np.uint8, n * 16).view(gpu.HIT_DTYPE)
+        assert np.array_equal(got, ref)
