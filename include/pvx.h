@@ -183,14 +183,24 @@ typedef struct pvx_hit {
     int32_t prim;
 } pvx_hit;
 
+/* Acceleration structure the closest-hit calls use. */
+enum {
+    PVX_ACCEL_BVH = 0, /* flattened BVH (4-wide for triangle/sphere scenes); default  */
+    PVX_ACCEL_BSP = 1  /* kd-tree with the cost model of BSPTree (+BM2,
+                          core/bounding/bsptree.cpp:276-705); same hits, slower      */
+};
+
 typedef struct pvx_build_opts {
     uint32_t max_leaf_size; /* triangles per BVH leaf (0 = default 4) */
     uint32_t sah_bins;      /* binned-SAH bins (0 = default 16)        */
-    uint32_t reserved[6];
+    uint32_t accel;         /* PVX_ACCEL_BVH / PVX_ACCEL_BSP          */
+    uint32_t bsp_max_depth; /* BSP tree depth limit (0 = 64, at most 64) */
+    uint32_t reserved[4];
 } pvx_build_opts;
 
-/* Counts describe the tree the closest-hit kernels walk: the 4-wide tree
- * (128 B nodes) for triangle / sphere scenes, the binary tree (64 B) else. */
+/* Counts describe the tree the closest-hit kernels walk: the kd-tree (8 B
+ * nodes; n_prims = leaf list entries) for PVX_ACCEL_BSP, else the 4-wide tree
+ * (128 B nodes) for triangle / sphere scenes and the binary tree (64 B). */
 typedef struct pvx_scene_stats {
     uint64_t n_nodes;        /* BVH nodes                               */
     uint64_t n_prims;        /* BVH primitive references                */

@@ -74,7 +74,11 @@ class PvxHit(C.Structure):
 
 class PvxBuildOpts(C.Structure):
     _fields_ = [("max_leaf_size", C.c_uint32), ("sah_bins", C.c_uint32),
-                ("reserved", C.c_uint32 * 6)]
+                ("accel", C.c_uint32), ("bsp_max_depth", C.c_uint32),
+                ("reserved", C.c_uint32 * 4)]
+
+
+PVX_ACCEL_BVH, PVX_ACCEL_BSP = 0, 1
 
 
 class PvxSceneStats(C.Structure):

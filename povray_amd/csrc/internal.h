@@ -73,6 +73,8 @@ struct pvx_scene {
     int device = 0;
     pvx::HostScene host;
     pvx::SceneView view{};
+    pvx::KdView kd{};     // kd-tree of a PVX_ACCEL_BSP scene
+    bool use_kd = false;
     std::vector<void *> allocs;
     uint32_t *d_obj_type = nullptr, *d_obj_index = nullptr;
     // Traversal-stack spill areas, one per stream that has traced this scene

@@ -513,6 +513,165 @@ __global__ void k_clock_init(unsigned long long *clocks, uint32_t n) {
     if (i < n) clocks[i] = i == 0 ? ~0ull : 0ull;
 }
 
+// Closest valid hit among one primitive record of any kind, folded into
+// (bt, bo, bp) as the leaf loops of k_trace_closest do.
+template <bool kFlags>
+__device__ __forceinline__ void prim_test_any(const SceneView &sv, const PrimRec &p, D3 o, D3 d, bool shadow,
+                                              uint32_t rk, double &bt, int32_t &bo, int32_t &bp) {
+    const uint32_t ka = p.kind_axis;
+    const uint32_t pk = ka & 15u;
+    const int32_t obj = p.obj;
+    if (kFlags && !precondition(sv.obj_flags[obj], rk)) return;
+    if (pk == kPrimTriangle) {
+        double dep;
+        if (tri_hit64(p.tri, ka >> 4, o, d, dep) && depth_ok(dep, shadow) && better(dep, obj, p.tri.prim, bt, bo, bp)) {
+            bt = dep;
+            bo = obj;
+            bp = p.tri.prim;
+        }
+    } else if (pk == kPrimSphere) {
+        double hs[2];
+        int nh = sphere_hits(d3(p.sph.c[0], p.sph.c[1], p.sph.c[2]), p.sph.r, o, d, hs);
+        for (int j = 0; j < nh; j++)
+            if (depth_ok(hs[j], shadow) && better(hs[j], obj, 0, bt, bo, bp)) {
+                bt = hs[j];
+                bo = obj;
+                bp = 0;
+            }
+    } else {
+        double dep;
+        int32_t prim = 0;
+        bool ok;
+        if (pk == kPrimCsg)
+            ok = csg_closest(sv, p.ref.index, o, d, shadow, bt, dep, prim);
+        else
+            ok = simple_closest(sv, pk, p.ref.index, o, d, shadow, dep);
+        if (ok && better(dep, obj, prim, bt, bo, bp)) {
+            bt = dep;
+            bo = obj;
+            bp = prim;
+        }
+    }
+}
+
+// Closest hit through the kd-tree (BSPTree::operator()(ray),
+// core/bounding/bsptree.cpp:128-226): the ray's range inside the tree's box,
+// the near child first with the far one remembered with its [entry, exit)
+// range, every primitive of a reached leaf tested, and a stop once the next
+// range starts beyond the closest hit.  Each primitive is decided as in
+// k_trace_closest; one listed in several leaves is tested again (the
+// reference's mailbox only saves that work).
+template <bool kFlags, bool kCount>
+__global__ __launch_bounds__(kBlock) void k_trace_closest_kd(SceneView sv, KdView kd, const pvx_ray *__restrict__ rays,
+                                                           const double *__restrict__ max_depth, uint32_t kind,
+                                                           const uint8_t *__restrict__ kinds, uint32_t n,
+                                                           pvx_hit *__restrict__ hits, uint32_t *__restrict__ node_visits,
+                                                           uint32_t *__restrict__ prim_tests) {
+    struct Entry {
+        uint32_t node;
+        float rentry, rexit;
+    };
+    const bool shadow = (kind == PVX_RAY_SHADOW);
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const pvx_ray r = rays[i];
+        const uint32_t rk = (kFlags && kinds) ? (uint32_t)kinds[i] : kind;
+        const D3 o = d3(r.o[0], r.o[1], r.o[2]), d = d3(r.d[0], r.d[1], r.d[2]);
+        double bt = kBoundHuge;
+        if (max_depth) {
+            const double md = max_depth[i];
+            if (md >= kEpsilon) bt = md;
+        }
+        int32_t bo = -1, bp = 0;
+        uint32_t cn = 0, cp = 0;
+        for (uint32_t k = 0; k < sv.n_inf; k++) {
+            const InfRec inf = sv.inf[k];
+            if (kFlags && !precondition(sv.obj_flags[inf.obj], rk)) continue;
+            double dep;
+            int32_t prim = 0;
+            bool ok = inf.kind == kPrimPlane ? (plane_hit(sv.planes[inf.index], o, d, dep) && depth_ok(dep, shadow))
+                                             : csg_closest(sv, inf.index, o, d, shadow, bt, dep, prim);
+            if (ok && better(dep, inf.obj, prim, bt, bo, bp)) {
+                bt = dep;
+                bo = inf.obj;
+                bp = prim;
+            }
+        }
+        // the ray's range inside the tree's box (FP64 slabs)
+        double t0 = -INFINITY, t1 = INFINITY;
+        bool inbox = true;
+        for (int a = 0; a < 3; a++) {
+            const double oa = comp(o, a), da = comp(d, a), lo = kd.bmin[a], hi = kd.bmax[a];
+            if (da == 0.0) {
+                if (oa < lo || oa > hi) inbox = false;
+                continue;
+            }
+            double ta = (lo - oa) / da, tb2 = (hi - oa) / da;
+            if (ta > tb2) {
+                const double x = ta;
+                ta = tb2;
+                tb2 = x;
+            }
+            t0 = fmax(t0, ta);
+            t1 = fmin(t1, tb2);
+        }
+        if (inbox && t0 <= t1 && t1 >= 0.0) {
+            double rentry = fmax(t0, 0.0), rexit = t1;
+            Entry st[kKdMaxDepth + 1];
+            int sp = 0;
+            uint32_t inode = 0;
+            while (rentry < bt) {
+                const KdNode nd = kd.nodes[inode];
+                if (!(nd.info >> 31)) { // split
+                    if (kCount) cn++;
+                    const uint32_t axis = (nd.info >> 29) & 3u;
+                    uint32_t left = nd.info & 0x1FFFFFFFu, right = left + 1;
+                    const float plane = nd.u.plane;
+                    const double oa = comp(o, axis), da = comp(d, axis);
+                    if (da == 0.0) { // parallel to the plane: one side, or both when on it
+                        if (oa == (double)plane) {
+                            st[sp++] = Entry{right, (float)rentry, (float)rexit};
+                            inode = left;
+                        } else {
+                            inode = oa > (double)plane ? right : left;
+                        }
+                        continue;
+                    }
+                    const float rdist = (float)(((double)plane - oa) / da);
+                    if ((oa > (double)plane) || ((rdist == 0.0f) && (da < 0))) {
+                        const uint32_t x = left;
+                        left = right;
+                        right = x;
+                    }
+                    if ((rdist < 0.0f) || (rdist > rexit)) {
+                        inode = left;
+                    } else if (rdist < rentry) {
+                        inode = right;
+                    } else {
+                        st[sp++] = Entry{right, rdist, (float)rexit};
+                        inode = left;
+                        rexit = rdist;
+                    }
+                } else { // leaf
+                    const uint32_t count = nd.info & 0x7FFFFFFFu, first = nd.u.first;
+                    if (kCount) cp += count;
+                    for (uint32_t k = 0; k < count; k++)
+                        prim_test_any<kFlags>(sv, sv.prims[kd.list[first + k]], o, d, shadow, rk, bt, bo, bp);
+                    if (sp == 0) break;
+                    sp--;
+                    inode = st[sp].node;
+                    rentry = st[sp].rentry;
+                    rexit = st[sp].rexit;
+                }
+            }
+        }
+        hits[i] = pvx_hit{bt, bo, bp};
+        if (kCount) {
+            node_visits[i] = cn;
+            prim_tests[i] = cp;
+        }
+    }
+}
+
 // Perspective camera rays (tracepixel.cpp:357-372, 918-926).
 __global__ void k_camera_rays(pvx_camera cam, pvx_ray *rays) {
     uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -736,6 +895,18 @@ extern "C" int pvx_scene_create(const pvx_scene_desc *desc, const pvx_build_opts
     UP(h.nodes4, nodes4);
     UP(h.triangles, triangles);
 #undef UP
+    if (h.use_kd) {
+        if ((r = upload(s, h.kd_nodes, &s->kd.nodes)) != PVX_OK || (r = upload(s, h.kd_list, &s->kd.list)) != PVX_OK) {
+            scene_free(s);
+            delete s;
+            return r;
+        }
+        for (int a = 0; a < 3; a++) {
+            s->kd.bmin[a] = h.kd_bmin[a];
+            s->kd.bmax[a] = h.kd_bmax[a];
+        }
+        s->use_kd = true;
+    }
     const uint32_t *tmp = nullptr;
     if ((r = upload(s, h.obj_type, &tmp)) != PVX_OK) { scene_free(s); delete s; return r; }
     s->d_obj_type = const_cast<uint32_t *>(tmp);
@@ -799,12 +970,12 @@ extern "C" int pvx_scene_destroy(pvx_scene *s) {
 extern "C" int pvx_scene_get_stats(const pvx_scene *s, pvx_scene_stats *out) {
     if (!s || !out) return fail(PVX_EINVAL, "null argument");
     const bool wide = s->view.wide != 0; // the tree the closest-hit kernels walk
-    out->n_nodes = wide ? s->host.nodes4.size() : s->host.nodes.size();
-    out->n_prims = s->host.prims.size();
-    out->node_bytes = wide ? sizeof(Node4) : sizeof(Node);
+    out->n_nodes = s->use_kd ? s->host.kd_nodes.size() : wide ? s->host.nodes4.size() : s->host.nodes.size();
+    out->n_prims = s->use_kd ? s->host.kd_list.size() : s->host.prims.size();
+    out->node_bytes = s->use_kd ? sizeof(KdNode) : wide ? sizeof(Node4) : sizeof(Node);
     out->tri_bytes = sizeof(PrimRec);
     out->device_bytes = s->device_bytes;
-    out->bvh_depth = wide ? s->host.depth4 : s->host.depth;
+    out->bvh_depth = s->use_kd ? s->host.kd_depth : wide ? s->host.depth4 : s->host.depth;
     out->n_infinite = (uint32_t)s->host.inf.size();
     out->build_ms = s->host.build_ms;
     return PVX_OK;
@@ -830,7 +1001,14 @@ static int launch_closest(pvx_scene *s, const pvx_ray *rays, const double *max_d
 #define PVX_LAUNCH_WW(L, F) \
     k_trace_closest_ww<F, kCount><<<nblocks, kBlock, 0, st>>>(s->view, rays, max_depth, ray_kind, kinds, nn, hits, spill, nv, pt)
     const uint32_t level = kernel_level(s->view);
-    if (level >= 2) {
+    if (s->use_kd) {
+        if (flags)
+            k_trace_closest_kd<true, kCount><<<nblocks, kBlock, 0, st>>>(s->view, s->kd, rays, max_depth, ray_kind,
+                                                                        kinds, nn, hits, nv, pt);
+        else
+            k_trace_closest_kd<false, kCount><<<nblocks, kBlock, 0, st>>>(s->view, s->kd, rays, max_depth, ray_kind,
+                                                                         kinds, nn, hits, nv, pt);
+    } else if (level >= 2) {
         if (flags) PVX_LAUNCH(2, true); else PVX_LAUNCH(2, false);
     } else if (level == 1) {
         if (flags) PVX_LAUNCH(1, true); else PVX_LAUNCH(1, false);

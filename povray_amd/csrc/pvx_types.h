@@ -52,6 +52,26 @@ struct alignas(16) Node4 {
 };
 static_assert(sizeof(Node4) == 128, "wide node is 128 B");
 
+// kd-tree node (the reference's BSP tree, core/bounding/bsptree.h:64-115),
+// 8 B.  info bit 31: leaf; leaf: count in bits 0..30, u.first = offset into
+// the primitive list; split: axis in bits 29..30, left child in bits 0..28
+// (right = left + 1), u.plane = split plane.
+constexpr uint32_t kKdMaxDepth = 64;
+struct KdNode {
+    uint32_t info;
+    union {
+        float plane;
+        uint32_t first;
+    } u;
+};
+static_assert(sizeof(KdNode) == 8, "kd node is 8 B");
+
+struct KdView {
+    const KdNode *nodes;
+    const uint32_t *list; // leaf primitive lists (indices into prims)
+    float bmin[3], bmax[3];
+};
+
 // Mesh triangle after Compute_Mesh_Triangle (core/shape/mesh.cpp:838-945):
 // FP32 face normal (Normals[Normal_Ind]), FP32 Distance, vertices P1..P3 in
 // post-swap order, dominant axis.

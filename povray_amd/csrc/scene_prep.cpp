@@ -19,6 +19,7 @@
 #include <functional>
 
 #include "bvh_build.h"
+#include "kd_build.h"
 
 namespace pvx {
 
@@ -675,6 +676,27 @@ int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene
     out.prims.resize(prims.size());
     for (size_t k = 0; k < prims.size(); k++) out.prims[k] = prims[bvh.order[k]];
     if (out.has_complex == 0) collapse_bvh4(out.nodes, out.root, out.nodes4, out.root4, out.depth4);
+    if (opts && opts->accel == PVX_ACCEL_BSP) { // kd-tree over the same records, in their final order
+        std::vector<float> klo(plo.size()), khi(phi.size());
+        for (size_t k = 0; k < prims.size(); k++)
+            for (int a = 0; a < 3; a++) {
+                klo[3 * k + a] = plo[3 * (size_t)bvh.order[k] + a];
+                khi[3 * k + a] = phi[3 * (size_t)bvh.order[k] + a];
+            }
+        const uint32_t md = opts->bsp_max_depth ? std::min<uint32_t>(opts->bsp_max_depth, kKdMaxDepth) : kKdMaxDepth;
+        KdTree kd = build_kd(klo.data(), khi.data(), (uint32_t)prims.size(), md);
+        out.kd_nodes = std::move(kd.nodes);
+        out.kd_list = std::move(kd.list);
+        for (int a = 0; a < 3; a++) {
+            out.kd_bmin[a] = kd.bmin[a];
+            out.kd_bmax[a] = kd.bmax[a];
+        }
+        out.kd_depth = kd.depth;
+        out.use_kd = true;
+    } else if (opts && opts->accel != PVX_ACCEL_BVH) {
+        err = "unknown acceleration structure";
+        return PVX_EINVAL;
+    }
     out.build_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return PVX_OK;

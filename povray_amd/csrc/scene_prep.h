@@ -14,6 +14,11 @@ struct HostScene {
     std::vector<Node> nodes;
     int32_t root = INT32_MIN;
     uint32_t depth = 0;
+    std::vector<KdNode> kd_nodes; // kd-tree (PVX_ACCEL_BSP): nodes, leaf lists, bounds
+    std::vector<uint32_t> kd_list;
+    float kd_bmin[3] = {0, 0, 0}, kd_bmax[3] = {0, 0, 0};
+    uint32_t kd_depth = 0;
+    bool use_kd = false;
     std::vector<Node4> nodes4; // 4-wide tree (triangle / sphere scenes)
     int32_t root4 = INT32_MIN;
     uint32_t depth4 = 0;

@@ -104,13 +104,16 @@ def _ptr(buf):
 class GpuScene:
     """A scene uploaded to the GPU with its BVH."""
 
-    def __init__(self, scene, max_leaf_size: int = 0, sah_bins: int = 0):
+    def __init__(self, scene, max_leaf_size: int = 0, sah_bins: int = 0, accel: int = A.PVX_ACCEL_BVH,
+                 bsp_max_depth: int = 0):
         L = lib()
         self.scene = scene
         desc, keep = scene.desc()
         opts = A.PvxBuildOpts()
         opts.max_leaf_size = max_leaf_size
         opts.sah_bins = sah_bins
+        opts.accel = accel
+        opts.bsp_max_depth = bsp_max_depth
         h = C.c_void_p()
         _check(L.pvx_scene_create(C.byref(desc), C.byref(opts), C.byref(h)), "pvx_scene_create")
         self.h = h
