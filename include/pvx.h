@@ -253,6 +253,38 @@ int pvx_trace_closest_counted(pvx_scene *scene, const pvx_ray *rays,
                               pvx_hit *hits, uint32_t *node_visits,
                               uint32_t *prim_tests, void *stream);
 
+/* Traversal variants (every one gives the same hits). */
+enum {
+    PVX_TRAVERSAL_DEFAULT = 0,   /* by scene content and ray kind                     */
+    PVX_TRAVERSAL_WIDE_WW = 1,   /* 4-wide tree, while-while (triangle/sphere scenes) */
+    PVX_TRAVERSAL_WIDE_STEP = 2, /* 4-wide tree, one node or leaf per pass            */
+    PVX_TRAVERSAL_BINARY = 3     /* binary tree (scenes without tori / CSG)           */
+};
+
+typedef struct pvx_tune_result {
+    uint32_t variant;      /* chosen PVX_TRAVERSAL_*                        */
+    uint32_t grid_blocks;  /* chosen persistent grid (blocks of 256 lanes)  */
+    double best_ms;        /* its time per launch on the sample              */
+    double default_ms;     /* the untuned choice's time on the same sample   */
+} pvx_tune_result;
+
+/* Autotuner: times the traversal variants this scene admits on persistent
+ * grids of one and two times the resident blocks, over the caller's sample
+ * rays (device), and keeps the fastest for later closest-hit calls on rays
+ * of the same class (PVX_RAY_SHADOW, or every other kind).  No reference
+ * counterpart (the reference has one traversal).  Results of later calls are
+ * unchanged; only their speed is. */
+int pvx_scene_autotune(pvx_scene *scene, const pvx_ray *rays, size_t n, uint32_t ray_kind,
+                       pvx_tune_result *out, void *stream);
+
+/* Sets the variant and grid for rays of ray_kind's class directly (e.g. a
+ * choice saved from an earlier pvx_scene_autotune); variant
+ * PVX_TRAVERSAL_DEFAULT with grid_blocks 0 restores the untuned behaviour.
+ * PVX_EINVAL for a variant the scene does not admit or grid_blocks above
+ * twice the resident blocks.  Not to be called while launches on the scene
+ * are being issued from another thread. */
+int pvx_scene_set_traversal(pvx_scene *scene, uint32_t ray_kind, uint32_t variant, uint32_t grid_blocks);
+
 /* Per-object intersection list for n (ray, object) pairs.
  * Replaces virtual bool ObjectBase::All_Intersections(const Ray&, IStack&,
  *   TraceThreadData*) (core/scene/object.h:238) for the kinds above.

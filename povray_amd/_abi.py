@@ -80,6 +80,14 @@ class PvxBuildOpts(C.Structure):
 
 PVX_ACCEL_BVH, PVX_ACCEL_BSP = 0, 1
 
+# traversal variants (pvx_scene_autotune)
+PVX_TRAVERSAL_DEFAULT, PVX_TRAVERSAL_WIDE_WW, PVX_TRAVERSAL_WIDE_STEP, PVX_TRAVERSAL_BINARY = 0, 1, 2, 3
+
+
+class PvxTuneResult(C.Structure):
+    _fields_ = [("variant", C.c_uint32), ("grid_blocks", C.c_uint32),
+                ("best_ms", C.c_double), ("default_ms", C.c_double)]
+
 
 class PvxSceneStats(C.Structure):
     _fields_ = [("n_nodes", C.c_uint64), ("n_prims", C.c_uint64),

@@ -73,6 +73,8 @@ struct pvx_scene {
     int device = 0;
     pvx::HostScene host;
     pvx::SceneView view{};
+    uint32_t tune_variant[2] = {0, 0}; // autotuner's choice: [0] camera/bounce rays, [1] light rays
+    uint32_t tune_grid[2] = {0, 0};    // (0: the default grid)
     pvx::KdView kd{};     // kd-tree of a PVX_ACCEL_BSP scene
     bool use_kd = false;
     std::vector<void *> allocs;

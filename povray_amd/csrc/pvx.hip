@@ -936,7 +936,7 @@ extern "C" int pvx_scene_create(const pvx_scene_desc *desc, const pvx_build_opts
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, kBlock, 0));
     if (per_cu < 1) per_cu = 1;
     s->grid = (uint32_t)(cus * per_cu);
-    s->spill_bytes = (size_t)s->grid * kBlock * kStackSpill * sizeof(int32_t);
+    s->spill_bytes = (size_t)2 * s->grid * kBlock * kStackSpill * sizeof(int32_t); // grids up to 2x (autotuner)
     *out = s;
     return PVX_OK;
 }
@@ -981,19 +981,18 @@ extern "C" int pvx_scene_get_stats(const pvx_scene *s, pvx_scene_stats *out) {
     return PVX_OK;
 }
 
+// One closest-hit launch with an explicit traversal variant and grid.
+// variant: PVX_TRAVERSAL_DEFAULT (by scene content and ray kind), _WIDE_WW,
+// _WIDE_STEP (4-wide tree; triangle/sphere scenes) or _BINARY (binary tree,
+// scenes without tori/CSG); every variant gives the same hits.
 template <bool kCount>
-static int launch_closest(pvx_scene *s, const pvx_ray *rays, const double *max_depth,
-                          uint32_t ray_kind, const uint8_t *kinds, size_t n, pvx_hit *hits, uint32_t *nv,
-                          uint32_t *pt, void *stream) {
-    if (!s) return fail(PVX_EINVAL, "null scene");
-    if (n == 0) return PVX_OK;
-    if (!rays || !hits) return fail(PVX_EINVAL, "null ray or hit buffer");
-    if (n > 0xFFFFFFFFull) return fail(PVX_EINVAL, "too many rays in one call");
-    hipStream_t st = static_cast<hipStream_t>(stream);
+static int launch_variant(pvx_scene *s, const pvx_ray *rays, const double *max_depth, uint32_t ray_kind,
+                          const uint8_t *kinds, size_t n, pvx_hit *hits, uint32_t *nv, uint32_t *pt,
+                          hipStream_t st, uint32_t variant, uint32_t grid) {
     int32_t *spill = nullptr;
     int r;
     if ((r = scene_spill(s, st, &spill))) return r;
-    uint32_t nblocks = (uint32_t)std::min<size_t>(s->grid, (n + kBlock - 1) / kBlock);
+    uint32_t nblocks = (uint32_t)std::min<size_t>(grid, (n + kBlock - 1) / kBlock);
     bool flags = s->view.any_flags != 0 && (ray_kind != PVX_RAY_PRIMARY || kinds);
     uint32_t nn = (uint32_t)n;
 #define PVX_LAUNCH(L, F) \
@@ -1010,9 +1009,10 @@ static int launch_closest(pvx_scene *s, const pvx_ray *rays, const double *max_d
                                                                          kinds, nn, hits, nv, pt);
     } else if (level >= 2) {
         if (flags) PVX_LAUNCH(2, true); else PVX_LAUNCH(2, false);
-    } else if (level == 1) {
+    } else if (level == 1 || variant == PVX_TRAVERSAL_BINARY) {
         if (flags) PVX_LAUNCH(1, true); else PVX_LAUNCH(1, false);
-    } else if (ray_kind != PVX_RAY_SHADOW) {
+    } else if (variant == PVX_TRAVERSAL_WIDE_WW ||
+               (variant == PVX_TRAVERSAL_DEFAULT && ray_kind != PVX_RAY_SHADOW)) {
         if (flags) PVX_LAUNCH_WW(0, true); else PVX_LAUNCH_WW(0, false);
     } else {
         if (flags) PVX_LAUNCH(0, true); else PVX_LAUNCH(0, false);
@@ -1020,6 +1020,111 @@ static int launch_closest(pvx_scene *s, const pvx_ray *rays, const double *max_d
 #undef PVX_LAUNCH
 #undef PVX_LAUNCH_WW
     HIP_TRY(hipGetLastError());
+    return PVX_OK;
+}
+
+template <bool kCount>
+static int launch_closest(pvx_scene *s, const pvx_ray *rays, const double *max_depth,
+                          uint32_t ray_kind, const uint8_t *kinds, size_t n, pvx_hit *hits, uint32_t *nv,
+                          uint32_t *pt, void *stream) {
+    if (!s) return fail(PVX_EINVAL, "null scene");
+    if (n == 0) return PVX_OK;
+    if (!rays || !hits) return fail(PVX_EINVAL, "null ray or hit buffer");
+    if (n > 0xFFFFFFFFull) return fail(PVX_EINVAL, "too many rays in one call");
+    const int cls = ray_kind == PVX_RAY_SHADOW ? 1 : 0; // the autotuner's choice for this ray class
+    const uint32_t grid = s->tune_grid[cls] ? s->tune_grid[cls] : s->grid;
+    return launch_variant<kCount>(s, rays, max_depth, ray_kind, kinds, n, hits, nv, pt,
+                                  static_cast<hipStream_t>(stream), s->tune_variant[cls], grid);
+}
+
+// Variants a scene admits: the kd-tree and tori/CSG scenes have one walk;
+// box/triangle scenes the binary tree; triangle/sphere scenes all three.
+static std::vector<uint32_t> scene_variants(const pvx_scene *s) {
+    const uint32_t level = kernel_level(s->view);
+    if (s->use_kd || level >= 2) return {PVX_TRAVERSAL_DEFAULT};
+    if (level == 1) return {PVX_TRAVERSAL_BINARY};
+    return {PVX_TRAVERSAL_WIDE_WW, PVX_TRAVERSAL_WIDE_STEP, PVX_TRAVERSAL_BINARY};
+}
+
+extern "C" int pvx_scene_set_traversal(pvx_scene *s, uint32_t ray_kind, uint32_t variant, uint32_t grid_blocks) {
+    if (!s) return fail(PVX_EINVAL, "null scene");
+    if (grid_blocks > 2 * s->grid)
+        return fail(PVX_EINVAL, "grid_blocks above twice the resident blocks (" + std::to_string(2 * s->grid) + ")");
+    if (variant != PVX_TRAVERSAL_DEFAULT) {
+        auto v = scene_variants(s);
+        if (std::find(v.begin(), v.end(), variant) == v.end())
+            return fail(PVX_EINVAL, "traversal variant " + std::to_string(variant) + " not available for this scene");
+    }
+    const int cls = ray_kind == PVX_RAY_SHADOW ? 1 : 0;
+    s->tune_variant[cls] = variant;
+    s->tune_grid[cls] = grid_blocks;
+    return PVX_OK;
+}
+
+// Times every traversal variant the scene admits, on persistent grids of
+// one and two times the resident blocks, over the caller's rays (1 warm-up +
+// 3 timed launches each), and keeps the fastest for rays of this class
+// (light rays, or camera / bounce / radiosity rays).
+extern "C" int pvx_scene_autotune(pvx_scene *s, const pvx_ray *rays, size_t n, uint32_t ray_kind,
+                                  pvx_tune_result *out, void *stream) {
+    if (!s || !rays) return fail(PVX_EINVAL, "null argument");
+    if (n == 0 || n > 0xFFFFFFFFull) return fail(PVX_EINVAL, "n must be in [1, 2^32)");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const std::vector<uint32_t> variants = scene_variants(s);
+    void *hits = nullptr;
+    HIP_TRY(hipMalloc(&hits, n * sizeof(pvx_hit)));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int r = PVX_OK;
+    double best = 1e30, dflt = 0.0;
+    uint32_t best_v = PVX_TRAVERSAL_DEFAULT, best_g = s->grid;
+    auto time_one = [&](uint32_t v, uint32_t g, double &ms) -> int {
+        int rr;
+        if ((rr = launch_variant<false>(s, rays, nullptr, ray_kind, nullptr, n, static_cast<pvx_hit *>(hits), nullptr,
+                                        nullptr, st, v, g)))
+            return rr;
+        HIP_TRY(hipEventRecord(e0, st));
+        for (int k = 0; k < 3; k++)
+            if ((rr = launch_variant<false>(s, rays, nullptr, ray_kind, nullptr, n, static_cast<pvx_hit *>(hits),
+                                            nullptr, nullptr, st, v, g)))
+                return rr;
+        HIP_TRY(hipEventRecord(e1, st));
+        HIP_TRY(hipEventSynchronize(e1));
+        float f = 0;
+        HIP_TRY(hipEventElapsedTime(&f, e0, e1));
+        ms = f / 3.0;
+        return PVX_OK;
+    };
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e != hipSuccess) {
+        r = fail(PVX_EDEVICE, std::string("autotune events: ") + hipGetErrorString(e));
+    } else {
+        r = time_one(PVX_TRAVERSAL_DEFAULT, s->grid, dflt);
+        for (uint32_t v : variants)
+            for (uint32_t g : {s->grid, 2 * s->grid}) {
+                if (r) break;
+                double ms = 0;
+                r = time_one(v, g, ms);
+                if (!r && ms < best) {
+                    best = ms;
+                    best_v = v;
+                    best_g = g;
+                }
+            }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(hits);
+    if (r) return r;
+    const int cls = ray_kind == PVX_RAY_SHADOW ? 1 : 0;
+    s->tune_variant[cls] = best_v;
+    s->tune_grid[cls] = best_g;
+    if (out) {
+        out->variant = best_v;
+        out->grid_blocks = best_g;
+        out->best_ms = best;
+        out->default_ms = dflt;
+    }
     return PVX_OK;
 }
 

@@ -61,6 +61,8 @@ def lib():
         L.pvx_batch_trace_closest.argtypes = [vp, vp, vp, vp, u32, vp, vp, vp]
         L.pvx_batch_trace_closest_counted.argtypes = [vp, vp, vp, vp, u32, vp, vp, vp, vp, vp]
         L.pvx_clock_rate.argtypes = [C.POINTER(C.c_double)]
+        L.pvx_scene_autotune.argtypes = [vp, vp, sz, u32, C.POINTER(A.PvxTuneResult), vp]
+        L.pvx_scene_set_traversal.argtypes = [vp, u32, u32, u32]
         fp = C.POINTER(C.c_float)
         L.pvx_surrogate_create.argtypes = [fp, C.c_uint64, C.POINTER(vp)]
         L.pvx_surrogate_destroy.argtypes = [vp]
@@ -81,7 +83,7 @@ EXPORTED_SYMBOLS = (
     "pvx_radiosity_render", "pvx_surrogate_create", "pvx_surrogate_destroy",
     "pvx_surrogate_get_params", "pvx_surrogate_grad", "pvx_surrogate_train", "pvx_surrogate_infer",
     "pvx_batch_create", "pvx_batch_destroy", "pvx_batch_trace_closest", "pvx_batch_trace_closest_counted",
-    "pvx_clock_rate",
+    "pvx_clock_rate", "pvx_scene_autotune", "pvx_scene_set_traversal",
 )
 
 
@@ -148,6 +150,24 @@ class GpuScene:
                                                _ptr(hits), _ptr(node_visits), _ptr(prim_tests),
                                                stream.h if stream else None),
                "pvx_trace_closest_counted")
+
+    def autotune(self, rays, n=None, ray_kind=A.PVX_RAY_IMAGE, stream=None) -> dict:
+        """pvx_scene_autotune over sample rays (a device buffer with n, or an
+        (n, 6) array); later closest-hit calls of this ray class use the
+        fastest variant and grid.  Returns the choice and its timings."""
+        keep = None
+        if not isinstance(rays, hip.DeviceBuffer):
+            arr = np.ascontiguousarray(rays, np.float64).reshape(-1, 6)
+            n = arr.shape[0]
+            keep = rays = hip.DeviceBuffer.from_array(arr)
+        r = A.PvxTuneResult()
+        _check(lib().pvx_scene_autotune(self.h, _ptr(rays), n, ray_kind, C.byref(r),
+                                        stream.h if stream else None), "pvx_scene_autotune")
+        del keep
+        return {k: getattr(r, k) for k, _ in A.PvxTuneResult._fields_}
+
+    def set_traversal(self, variant=A.PVX_TRAVERSAL_DEFAULT, grid_blocks=0, ray_kind=A.PVX_RAY_IMAGE):
+        _check(lib().pvx_scene_set_traversal(self.h, ray_kind, variant, grid_blocks), "pvx_scene_set_traversal")
 
     # -- numpy convenience (upload, run, download) -------------------------------
     def find_intersection(self, rays: np.ndarray, max_depth=None, ray_kind=A.PVX_RAY_IMAGE):
