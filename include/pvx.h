@@ -444,6 +444,14 @@ int pvx_render(pvx_scene *scene, const pvx_camera *cam, float *rgb, pvx_render_s
 int pvx_render_samples(pvx_scene *scene, const pvx_camera *cam, uint32_t n_samples, const double *offsets,
                        float *rgb, pvx_render_stats *stats, void *stream);
 
+/* pvx_render that first runs pvx_scene_autotune on the frame's own camera
+ * rays (tune[0]) and first wavefront of light rays (tune[1], all zero when
+ * the frame has none); the image is the one pvx_render gives.  tune may be
+ * NULL.  stats->total_ms includes the tuning launches; trace_ms and
+ * shadow_ms do not. */
+int pvx_render_autotune(pvx_scene *scene, const pvx_camera *cam, float *rgb, pvx_tune_result *tune,
+                        pvx_render_stats *stats, void *stream);
+
 /* ---- radiosity ---------------------------------------------------------- */
 enum { PVX_RADIOSITY_PLAIN = 0, PVX_RADIOSITY_LEARNED = 1 };
 

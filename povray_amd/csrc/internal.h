@@ -116,6 +116,9 @@ namespace pvx {
 int trace_closest_internal(pvx_scene *s, const pvx_ray *rays, const double *max_depth,
                            uint32_t ray_kind, size_t n, pvx_hit *hits, hipStream_t stream,
                            const uint8_t *kinds = nullptr);
+// The autotuner over sample rays with optional per-ray max depths.
+int autotune_internal(pvx_scene *s, const pvx_ray *rays, const double *max_depth, size_t n,
+                      uint32_t ray_kind, pvx_tune_result *out, hipStream_t stream);
 } // namespace pvx
 
 #endif

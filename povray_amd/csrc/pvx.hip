@@ -1065,11 +1065,10 @@ extern "C" int pvx_scene_set_traversal(pvx_scene *s, uint32_t ray_kind, uint32_t
 // one and two times the resident blocks, over the caller's rays (1 warm-up +
 // 3 timed launches each), and keeps the fastest for rays of this class
 // (light rays, or camera / bounce / radiosity rays).
-extern "C" int pvx_scene_autotune(pvx_scene *s, const pvx_ray *rays, size_t n, uint32_t ray_kind,
-                                  pvx_tune_result *out, void *stream) {
+int pvx::autotune_internal(pvx_scene *s, const pvx_ray *rays, const double *max_depth, size_t n,
+                           uint32_t ray_kind, pvx_tune_result *out, hipStream_t st) {
     if (!s || !rays) return fail(PVX_EINVAL, "null argument");
     if (n == 0 || n > 0xFFFFFFFFull) return fail(PVX_EINVAL, "n must be in [1, 2^32)");
-    hipStream_t st = static_cast<hipStream_t>(stream);
     const std::vector<uint32_t> variants = scene_variants(s);
     void *hits = nullptr;
     HIP_TRY(hipMalloc(&hits, n * sizeof(pvx_hit)));
@@ -1079,12 +1078,12 @@ extern "C" int pvx_scene_autotune(pvx_scene *s, const pvx_ray *rays, size_t n, u
     uint32_t best_v = PVX_TRAVERSAL_DEFAULT, best_g = s->grid;
     auto time_one = [&](uint32_t v, uint32_t g, double &ms) -> int {
         int rr;
-        if ((rr = launch_variant<false>(s, rays, nullptr, ray_kind, nullptr, n, static_cast<pvx_hit *>(hits), nullptr,
-                                        nullptr, st, v, g)))
+        if ((rr = launch_variant<false>(s, rays, max_depth, ray_kind, nullptr, n, static_cast<pvx_hit *>(hits),
+                                        nullptr, nullptr, st, v, g)))
             return rr;
         HIP_TRY(hipEventRecord(e0, st));
         for (int k = 0; k < 3; k++)
-            if ((rr = launch_variant<false>(s, rays, nullptr, ray_kind, nullptr, n, static_cast<pvx_hit *>(hits),
+            if ((rr = launch_variant<false>(s, rays, max_depth, ray_kind, nullptr, n, static_cast<pvx_hit *>(hits),
                                             nullptr, nullptr, st, v, g)))
                 return rr;
         HIP_TRY(hipEventRecord(e1, st));
@@ -1126,6 +1125,11 @@ extern "C" int pvx_scene_autotune(pvx_scene *s, const pvx_ray *rays, size_t n, u
         out->default_ms = dflt;
     }
     return PVX_OK;
+}
+
+extern "C" int pvx_scene_autotune(pvx_scene *s, const pvx_ray *rays, size_t n, uint32_t ray_kind,
+                                  pvx_tune_result *out, void *stream) {
+    return pvx::autotune_internal(s, rays, nullptr, n, ray_kind, out, static_cast<hipStream_t>(stream));
 }
 
 int pvx::trace_closest_internal(pvx_scene *s, const pvx_ray *rays, const double *max_depth,

@@ -580,8 +580,26 @@ extern "C" int pvx_render(pvx_scene *s, const pvx_camera *cam, float *rgb, pvx_r
     return pvx_render_samples(s, cam, 1, off, rgb, stats, stream);
 }
 
+static int render_frame(pvx_scene *s, const pvx_camera *cam, uint32_t S, const double *offsets, float *rgb,
+                        pvx_render_stats *stats, void *stream, pvx_tune_result *tune);
+
 extern "C" int pvx_render_samples(pvx_scene *s, const pvx_camera *cam, uint32_t S, const double *offsets,
                                   float *rgb, pvx_render_stats *stats, void *stream) {
+    return render_frame(s, cam, S, offsets, rgb, stats, stream, nullptr);
+}
+
+// One frame that first tunes the closest-hit walk on its own camera rays
+// and on its first wavefront of light rays.
+extern "C" int pvx_render_autotune(pvx_scene *s, const pvx_camera *cam, float *rgb, pvx_tune_result *tune,
+                                   pvx_render_stats *stats, void *stream) {
+    if (!cam) return fail(PVX_EINVAL, "null argument");
+    pvx_tune_result local[2];
+    const double off[2] = {cam->jitter_x, cam->jitter_y};
+    return render_frame(s, cam, 1, off, rgb, stats, stream, tune ? tune : local);
+}
+
+static int render_frame(pvx_scene *s, const pvx_camera *cam, uint32_t S, const double *offsets, float *rgb,
+                        pvx_render_stats *stats, void *stream, pvx_tune_result *tune) {
     if (!s || !cam || !rgb || !offsets) return fail(PVX_EINVAL, "null argument");
     if (S == 0 || S > 256) return fail(PVX_EINVAL, "n_samples must be in [1, 256]");
     ShadingState &sh = s->shading;
@@ -660,6 +678,8 @@ extern "C" int pvx_render_samples(pvx_scene *s, const pvx_camera *cam, uint32_t 
     while (n > 0 && bounces < max_bounces) {
         const pvx_ray *rays = static_cast<const pvx_ray *>(b.rays[cur].p);
         const PathState *pst = static_cast<const PathState *>(b.state[cur].p);
+        if (tune && bounces == 0 && (rc = autotune_internal(s, rays, nullptr, n, PVX_RAY_IMAGE, &tune[0], st)))
+            return rc;
         HIP_TRY(hipEventRecord(t0, st));
         rc = trace_closest_internal(s, rays, nullptr, bounces == 0 ? (uint32_t)PVX_RAY_IMAGE : (uint32_t)PVX_RAY_REFLECTION,
                                     n, static_cast<pvx_hit *>(b.hits.p), st,
@@ -694,6 +714,16 @@ extern "C" int pvx_render_samples(pvx_scene *s, const pvx_camera *cam, uint32_t 
         // light rays, followed through transparent objects
         uint32_t ns = cnt[1];
         int sq = 0;
+        if (tune && bounces == 0) {
+            if (ns > 0) {
+                if ((rc = autotune_internal(s, static_cast<const pvx_ray *>(b.srays[0].p),
+                                            static_cast<const double *>(b.sdepth[0].p), ns, PVX_RAY_SHADOW, &tune[1],
+                                            st)))
+                    return rc;
+            } else {
+                tune[1] = pvx_tune_result{0, 0, 0.0, 0.0}; // no light rays: nothing to tune
+            }
+        }
         for (int step = 0; ns > 0 && step < kMaxShadowSteps; step++) {
             HIP_TRY(hipEventRecord(t0, st));
             rc = trace_closest_internal(s, static_cast<const pvx_ray *>(b.srays[sq].p),

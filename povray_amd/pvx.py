@@ -63,6 +63,8 @@ def lib():
         L.pvx_clock_rate.argtypes = [C.POINTER(C.c_double)]
         L.pvx_scene_autotune.argtypes = [vp, vp, sz, u32, C.POINTER(A.PvxTuneResult), vp]
         L.pvx_scene_set_traversal.argtypes = [vp, u32, u32, u32]
+        L.pvx_render_autotune.argtypes = [vp, C.POINTER(A.PvxCamera), vp, C.POINTER(A.PvxTuneResult),
+                                          C.POINTER(A.PvxRenderStats), vp]
         fp = C.POINTER(C.c_float)
         L.pvx_surrogate_create.argtypes = [fp, C.c_uint64, C.POINTER(vp)]
         L.pvx_surrogate_destroy.argtypes = [vp]
@@ -84,6 +86,7 @@ EXPORTED_SYMBOLS = (
     "pvx_surrogate_get_params", "pvx_surrogate_grad", "pvx_surrogate_train", "pvx_surrogate_infer",
     "pvx_batch_create", "pvx_batch_destroy", "pvx_batch_trace_closest", "pvx_batch_trace_closest_counted",
     "pvx_clock_rate", "pvx_scene_autotune", "pvx_scene_set_traversal",
+    "pvx_render_autotune",
 )
 
 
@@ -228,6 +231,21 @@ class GpuScene:
         img = out.download(np.float32, cam.width * cam.height * 3)
         return (img.reshape(cam.height, cam.width, 3),
                 {k: getattr(st, k) for k, _ in A.PvxRenderStats._fields_ if k != "pad"})
+
+    def render_autotune(self, width=None, height=None):
+        """pvx_render_autotune: one frame that tunes the closest-hit walk on its
+        camera rays and first light rays.  Returns (image, stats, [camera, light] choices)."""
+        cam = self.scene.camera_struct(width=width, height=height)
+        out = hip.DeviceBuffer(cam.width * cam.height * 12)
+        st = A.PvxRenderStats()
+        tune = (A.PvxTuneResult * 2)()
+        _check(lib().pvx_render_autotune(self.h, C.byref(cam), out.value, tune, C.byref(st), None),
+               "pvx_render_autotune")
+        hip.synchronize()
+        img = out.download(np.float32, cam.width * cam.height * 3)
+        return (img.reshape(cam.height, cam.width, 3),
+                {k: getattr(st, k) for k, _ in A.PvxRenderStats._fields_ if k != "pad"},
+                [{k: getattr(t, k) for k, _ in A.PvxTuneResult._fields_} for t in tune])
 
     def render(self, width=None, height=None):
         """Render one frame; returns ((h, w, 3) float32 linear RGB, stats)."""

@@ -84,3 +84,22 @@ def test_autotune_bad_arguments(gpu):
         gs.set_traversal(99)
     with pytest.raises(pvx.PvxError):
         gs.autotune(np.zeros((0, 6)))
+
+
+@pytest.mark.parametrize("name", ["mesh", "spheres"])
+def test_render_autotune_same_image(gpu, name):
+    """pvx_render_autotune tunes on the frame's own camera and light rays;
+    the frame, and every later frame, is the untuned one bit for bit."""
+    s = {"mesh": lambda: scenes.mesh_scene(level=5, width=320, height=180),
+         "spheres": lambda: scenes.sphere_scene(width=320, height=240)}[name]()
+    gs = gpu.GpuScene(s)
+    gs.set_shading()
+    ref, st0 = gs.render()
+    img, st, tune = gs.render_autotune()
+    print(f"[render autotune {name}] camera {tune[0]} light {tune[1]}", flush=True)
+    assert np.array_equal(img, ref)
+    assert st["closest_rays"] == st0["closest_rays"] and st["shadow_rays"] == st0["shadow_rays"]
+    for t in tune:
+        assert t["variant"] in V_ALL and t["best_ms"] > 0
+    again, _ = gs.render()
+    assert np.array_equal(again, ref)
