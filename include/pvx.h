@@ -190,12 +190,20 @@ enum {
                           core/bounding/bsptree.cpp:276-705); same hits, slower      */
 };
 
+/* Who builds the BVH (PVX_ACCEL_BVH).  Both trees give the same hits. */
+enum {
+    PVX_BUILDER_HOST_SAH = 0, /* binned SAH on the host; default                    */
+    PVX_BUILDER_GPU = 1       /* linear BVH (Morton order) built on the GPU: a much
+                                 faster build, a somewhat slower tree to traverse  */
+};
+
 typedef struct pvx_build_opts {
     uint32_t max_leaf_size; /* triangles per BVH leaf (0 = default 4) */
     uint32_t sah_bins;      /* binned-SAH bins (0 = default 16)        */
     uint32_t accel;         /* PVX_ACCEL_BVH / PVX_ACCEL_BSP          */
     uint32_t bsp_max_depth; /* BSP tree depth limit (0 = 64, at most 64) */
-    uint32_t reserved[4];
+    uint32_t builder;       /* PVX_BUILDER_HOST_SAH / PVX_BUILDER_GPU (BVH only) */
+    uint32_t reserved[3];
 } pvx_build_opts;
 
 /* Counts describe the tree the closest-hit kernels walk: the kd-tree (8 B
@@ -209,7 +217,8 @@ typedef struct pvx_scene_stats {
     uint64_t device_bytes;   /* total device bytes held by the scene    */
     uint32_t bvh_depth;      /* max BVH depth                           */
     uint32_t n_infinite;     /* objects tested outside the BVH (planes) */
-    double build_ms;         /* host BVH build time                     */
+    double build_ms;         /* scene preparation + tree build, wall    */
+    double bvh_build_ms;     /* the tree build alone (host or GPU), wall */
 } pvx_scene_stats;
 
 /* ---- library ----------------------------------------------------------- */

@@ -75,10 +75,11 @@ class PvxHit(C.Structure):
 class PvxBuildOpts(C.Structure):
     _fields_ = [("max_leaf_size", C.c_uint32), ("sah_bins", C.c_uint32),
                 ("accel", C.c_uint32), ("bsp_max_depth", C.c_uint32),
-                ("reserved", C.c_uint32 * 4)]
+                ("builder", C.c_uint32), ("reserved", C.c_uint32 * 3)]
 
 
 PVX_ACCEL_BVH, PVX_ACCEL_BSP = 0, 1
+PVX_BUILDER_HOST_SAH, PVX_BUILDER_GPU = 0, 1
 
 # traversal variants (pvx_scene_autotune)
 PVX_TRAVERSAL_DEFAULT, PVX_TRAVERSAL_WIDE_WW, PVX_TRAVERSAL_WIDE_STEP, PVX_TRAVERSAL_BINARY = 0, 1, 2, 3
@@ -93,7 +94,8 @@ class PvxSceneStats(C.Structure):
     _fields_ = [("n_nodes", C.c_uint64), ("n_prims", C.c_uint64),
                 ("node_bytes", C.c_uint64), ("tri_bytes", C.c_uint64),
                 ("device_bytes", C.c_uint64), ("bvh_depth", C.c_uint32),
-                ("n_infinite", C.c_uint32), ("build_ms", C.c_double)]
+                ("n_infinite", C.c_uint32), ("build_ms", C.c_double),
+                ("bvh_build_ms", C.c_double)]
 
 
 class PvxCamera(C.Structure):

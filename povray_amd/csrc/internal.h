@@ -86,6 +86,7 @@ struct pvx_scene {
     size_t spill_bytes = 0;
     uint32_t grid = 0;
     uint64_t device_bytes = 0;
+    uint64_t n_nodes2 = 0, n_nodes4 = 0, n_prims = 0; // tree sizes (host- or GPU-built)
     uint32_t n_objects = 0;
     pvx::ShadingState shading;
     pvx::RenderBuffers rb;

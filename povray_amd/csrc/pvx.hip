@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "../../include/pvx.h"
+#include "bvh_gpu.h"
 #include "internal.h"
 #include "scene_prep.h"
 #include "shapes_device.h"
@@ -822,6 +823,67 @@ static int upload(pvx_scene *s, const std::vector<T> &v, const T **dst) {
     return PVX_OK;
 }
 
+static void scene_free(pvx_scene *s);
+
+// PVX_BUILDER_GPU: upload the records (input order) and their bounds, build
+// the tree on the device, keep the records in leaf order.
+static int build_on_gpu(pvx_scene *s, const pvx_build_opts *opts) {
+    HostScene &h = s->host;
+    const auto t0 = std::chrono::steady_clock::now();
+    const uint32_t n = (uint32_t)h.prims.size();
+    s->n_prims = n;
+    void *d_in = nullptr, *d_lo = nullptr, *d_hi = nullptr, *d_out = nullptr;
+    int r = PVX_OK;
+    auto cleanup = [&]() {
+        if (d_in) (void)hipFree(d_in);
+        if (d_lo) (void)hipFree(d_lo);
+        if (d_hi) (void)hipFree(d_hi);
+    };
+    if (n > 0) {
+        hipError_t e = hipMalloc(&d_out, (size_t)n * sizeof(PrimRec)); // kept: freed with the scene
+        if (e == hipSuccess) {
+            s->allocs.push_back(d_out);
+            s->device_bytes += (size_t)n * sizeof(PrimRec);
+            e = hipMalloc(&d_in, (size_t)n * sizeof(PrimRec));
+        }
+        if (e == hipSuccess) e = hipMalloc(&d_lo, (size_t)n * 12);
+        if (e == hipSuccess) e = hipMalloc(&d_hi, (size_t)n * 12);
+        if (e == hipSuccess) e = hipMemcpy(d_in, h.prims.data(), (size_t)n * sizeof(PrimRec), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(d_lo, h.prim_lo.data(), (size_t)n * 12, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(d_hi, h.prim_hi.data(), (size_t)n * 12, hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            cleanup();
+            return fail(PVX_EDEVICE, std::string("GPU BVH build upload: ") + hipGetErrorString(e));
+        }
+    }
+    const auto tb = std::chrono::steady_clock::now();
+    GpuBvh g;
+    const int leaf = opts->max_leaf_size ? (int)opts->max_leaf_size : 4;
+    r = gpu_build_bvh(static_cast<const float *>(d_lo), static_cast<const float *>(d_hi),
+                      static_cast<const PrimRec *>(d_in), static_cast<PrimRec *>(d_out), n, leaf, h.has_complex == 0,
+                      nullptr, g, s->allocs);
+    const auto te = std::chrono::steady_clock::now();
+    cleanup();
+    if (r) return r;
+    s->device_bytes += g.device_bytes;
+    s->view.nodes = g.nodes;
+    s->view.prims = static_cast<const PrimRec *>(d_out);
+    s->view.nodes4 = g.nodes4;
+    h.root = g.root;
+    h.depth = g.depth;
+    h.root4 = g.root4;
+    h.depth4 = g.depth4;
+    s->n_nodes2 = g.n_nodes;
+    s->n_nodes4 = g.n_nodes4;
+    h.bvh_ms = std::chrono::duration<double, std::milli>(te - tb).count();
+    h.build_ms += std::chrono::duration<double, std::milli>(te - t0).count();
+    // the host copies are no longer needed
+    std::vector<PrimRec>().swap(h.prims);
+    std::vector<float>().swap(h.prim_lo);
+    std::vector<float>().swap(h.prim_hi);
+    return PVX_OK;
+}
+
 static void scene_free(pvx_scene *s) {
     if (s->rad_engine_free && s->rad_engine) s->rad_engine_free(s->rad_engine);
     s->rad_engine = nullptr;
@@ -862,21 +924,36 @@ extern "C" int pvx_scene_create(const pvx_scene_desc *desc, const pvx_build_opts
         delete s;
         return fail(rc, err);
     }
-    const HostScene &h = s->host;
+    HostScene &h = s->host;
+    SceneView &v = s->view;
+    int r = PVX_OK;
+    if (h.gpu_build) {
+        if ((r = build_on_gpu(s, opts)) != PVX_OK) {
+            scene_free(s);
+            delete s;
+            return r;
+        }
+    } else {
+        s->n_nodes2 = h.nodes.size();
+        s->n_nodes4 = h.nodes4.size();
+        s->n_prims = h.prims.size();
+    }
     if (h.depth + 1 >= (uint32_t)kStackMax) {
+        scene_free(s);
         delete s;
         return fail(PVX_EUNSUPPORTED, "BVH deeper than the traversal stack");
     }
-    SceneView &v = s->view;
-    int r = PVX_OK;
 #define UP(vec, field)                                          \
     if ((r = upload(s, vec, &v.field)) != PVX_OK) {             \
         scene_free(s);                                          \
         delete s;                                               \
         return r;                                               \
     }
-    UP(h.nodes, nodes);
-    UP(h.prims, prims);
+    if (!h.gpu_build) {
+        UP(h.nodes, nodes);
+        UP(h.prims, prims);
+        UP(h.nodes4, nodes4);
+    }
     UP(h.inf, inf);
     UP(h.obj_flags, obj_flags);
     UP(h.planes, planes);
@@ -892,7 +969,6 @@ extern "C" int pvx_scene_create(const pvx_scene_desc *desc, const pvx_build_opts
     UP(h.obj_tri_base, obj_tri_base);
     UP(h.mesh_vtx, mesh_vtx);
     UP(h.mesh_nrm, mesh_nrm);
-    UP(h.nodes4, nodes4);
     UP(h.triangles, triangles);
 #undef UP
     if (h.use_kd) {
@@ -970,14 +1046,15 @@ extern "C" int pvx_scene_destroy(pvx_scene *s) {
 extern "C" int pvx_scene_get_stats(const pvx_scene *s, pvx_scene_stats *out) {
     if (!s || !out) return fail(PVX_EINVAL, "null argument");
     const bool wide = s->view.wide != 0; // the tree the closest-hit kernels walk
-    out->n_nodes = s->use_kd ? s->host.kd_nodes.size() : wide ? s->host.nodes4.size() : s->host.nodes.size();
-    out->n_prims = s->use_kd ? s->host.kd_list.size() : s->host.prims.size();
+    out->n_nodes = s->use_kd ? s->host.kd_nodes.size() : wide ? s->n_nodes4 : s->n_nodes2;
+    out->n_prims = s->use_kd ? s->host.kd_list.size() : s->n_prims;
     out->node_bytes = s->use_kd ? sizeof(KdNode) : wide ? sizeof(Node4) : sizeof(Node);
     out->tri_bytes = sizeof(PrimRec);
     out->device_bytes = s->device_bytes;
     out->bvh_depth = s->use_kd ? s->host.kd_depth : wide ? s->host.depth4 : s->host.depth;
     out->n_infinite = (uint32_t)s->host.inf.size();
     out->build_ms = s->host.build_ms;
+    out->bvh_build_ms = s->host.bvh_ms;
     return PVX_OK;
 }
 

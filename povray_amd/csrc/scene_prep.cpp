@@ -669,6 +669,25 @@ int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene
         err = "too many primitives for the 28-bit leaf encoding";
         return PVX_EUNSUPPORTED;
     }
+    const uint32_t builder = opts ? opts->builder : (uint32_t)PVX_BUILDER_HOST_SAH;
+    if (builder != PVX_BUILDER_HOST_SAH && builder != PVX_BUILDER_GPU) {
+        err = "unknown BVH builder";
+        return PVX_EINVAL;
+    }
+    if (builder == PVX_BUILDER_GPU) {
+        if (opts->accel != PVX_ACCEL_BVH) {
+            err = "the GPU builder builds the BVH only (PVX_ACCEL_BVH)";
+            return PVX_EINVAL;
+        }
+        out.gpu_build = true;
+        out.prims = std::move(prims);
+        out.prim_lo = std::move(plo);
+        out.prim_hi = std::move(phi);
+        out.build_ms =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        return PVX_OK;
+    }
+    auto tb = std::chrono::steady_clock::now();
     BvhResult bvh = build_bvh(plo.data(), phi.data(), (uint32_t)prims.size(), leaf, bins);
     out.nodes = std::move(bvh.nodes);
     out.root = bvh.root;
@@ -676,6 +695,7 @@ int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene
     out.prims.resize(prims.size());
     for (size_t k = 0; k < prims.size(); k++) out.prims[k] = prims[bvh.order[k]];
     if (out.has_complex == 0) collapse_bvh4(out.nodes, out.root, out.nodes4, out.root4, out.depth4);
+    out.bvh_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tb).count();
     if (opts && opts->accel == PVX_ACCEL_BSP) { // kd-tree over the same records, in their final order
         std::vector<float> klo(plo.size()), khi(phi.size());
         for (size_t k = 0; k < prims.size(); k++)

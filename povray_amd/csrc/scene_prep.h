@@ -42,6 +42,9 @@ struct HostScene {
     uint32_t has_complex = 0; // 0: triangles/spheres, 1: + boxes, 2: + tori or CSG
     uint64_t n_triangles = 0;
     double build_ms = 0.0;
+    double bvh_ms = 0.0;         // the tree build alone
+    bool gpu_build = false;      // PVX_BUILDER_GPU: prims in input order, tree built on the device
+    std::vector<float> prim_lo, prim_hi; // their FP32 bounds (GPU build)
 };
 
 // Validates the description and builds every host array plus the BVH.

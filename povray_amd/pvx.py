@@ -110,7 +110,7 @@ class GpuScene:
     """A scene uploaded to the GPU with its BVH."""
 
     def __init__(self, scene, max_leaf_size: int = 0, sah_bins: int = 0, accel: int = A.PVX_ACCEL_BVH,
-                 bsp_max_depth: int = 0):
+                 bsp_max_depth: int = 0, builder: int = A.PVX_BUILDER_HOST_SAH):
         L = lib()
         self.scene = scene
         desc, keep = scene.desc()
@@ -119,6 +119,7 @@ class GpuScene:
         opts.sah_bins = sah_bins
         opts.accel = accel
         opts.bsp_max_depth = bsp_max_depth
+        opts.builder = builder
         h = C.c_void_p()
         _check(L.pvx_scene_create(C.byref(desc), C.byref(opts), C.byref(h)), "pvx_scene_create")
         self.h = h

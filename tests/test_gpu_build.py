@@ -1,0 +1,107 @@
+"""The GPU BVH builder (PVX_BUILDER_GPU, a linear BVH built on the device):
+every closest hit, intersection list and shaded frame must be the one the
+host-built (binned SAH) tree gives, bit for bit, on every scene kind, at
+every leaf size and on degenerate inputs (one primitive, identical
+primitives, empty scenes)."""
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleScene
+from povray_amd import _abi as A
+from povray_amd import pvx, scenes
+
+pytestmark = pytest.mark.gpu
+
+GPU = A.PVX_BUILDER_GPU
+
+
+def _same(a, b, what):
+    bad = (a["obj"] != b["obj"]) | (a["prim"] != b["prim"])
+    assert not bad.any(), f"{what}: {int(bad.sum())} id mismatches, first {np.nonzero(bad)[0][:4]}"
+    assert np.array_equal(a["t"], b["t"]), f"{what}: depths differ"
+
+
+SCENES = {
+    "spheres": lambda: scenes.sphere_scene(width=200, height=150),
+    "mesh": lambda: scenes.mesh_scene(level=6, width=240, height=135),
+    "csg": lambda: scenes.csg_scene(n_objects=600, width=160, height=90),
+    "triangles": lambda: scenes.triangle_scene(width=160, height=120),
+    "room": lambda: scenes.radiosity_scene(n_boxes=400, width=160, height=90),
+    "terrain": lambda: scenes.batch_scene(11, 20000, 300, 200, 150),
+}
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+def test_gpu_build_matches_host_build(gpu, name):
+    s = SCENES[name]()
+    rays = scenes.camera_rays_np(s.camera)
+    host = gpu.GpuScene(s)
+    dev = gpu.GpuScene(s, builder=GPU)
+    sh, sd = host.stats(), dev.stats()
+    print(f"[gpu build {name}] host tree {sh['n_nodes']} nodes depth {sh['bvh_depth']} "
+          f"{sh['bvh_build_ms']:.1f} ms; gpu tree {sd['n_nodes']} nodes depth {sd['bvh_depth']} "
+          f"{sd['bvh_build_ms']:.1f} ms", flush=True)
+    assert sd["n_prims"] == sh["n_prims"] and sd["node_bytes"] == sh["node_bytes"]
+    assert sd["n_nodes"] > 0 and sd["bvh_build_ms"] > 0
+    _same(dev.find_intersection(rays), host.find_intersection(rays), f"{name}: camera rays")
+    md = np.random.default_rng(2).uniform(2.0, 25.0, rays.shape[0])
+    _same(dev.find_intersection(rays, md, A.PVX_RAY_SHADOW), host.find_intersection(rays, md, A.PVX_RAY_SHADOW),
+          f"{name}: light rays")
+    if name == "triangles":
+        _same(dev.find_intersection(rays), OracleScene(s).find_intersection(rays, method=1), f"{name}: oracle")
+
+
+@pytest.mark.parametrize("leaf", [1, 2, 8])
+def test_gpu_build_leaf_sizes(gpu, leaf):
+    s = scenes.mesh_scene(level=5, width=160, height=90)
+    rays = scenes.camera_rays_np(s.camera)
+    ref = gpu.GpuScene(s).find_intersection(rays)
+    dev = gpu.GpuScene(s, max_leaf_size=leaf, builder=GPU)
+    _same(dev.find_intersection(rays), ref, f"leaf size {leaf}")
+
+
+def test_gpu_build_degenerate_inputs(gpu):
+    base = scenes.sphere_scene(n=200, width=64, height=48)
+    rays = scenes.camera_rays_np(base.camera)
+    # identical primitives: every Morton code equal (the tree splits by index)
+    same = scenes.sphere_scene(n=200, width=64, height=48)
+    same.sphere_center[:] = same.sphere_center[0]
+    # a handful of primitives (one leaf), one primitive, none (planes only)
+    few = scenes.sphere_scene(n=3, width=64, height=48)
+    one = scenes.sphere_scene(n=1, width=64, height=48)
+    for what, s in (("identical", same), ("three", few), ("one", one), ("base", base)):
+        ref = gpu.GpuScene(s).find_intersection(rays)
+        _same(gpu.GpuScene(s, builder=GPU).find_intersection(rays), ref, what)
+    # every triangle listed twice: pairs of equal codes, ties broken by triangle id
+    m = scenes.mesh_scene(level=3, width=64, height=48)
+    v, f, nrm = m.meshes[0]
+    m.meshes[0] = (v, np.concatenate([f, f]), nrm)
+    mr = scenes.camera_rays_np(m.camera)
+    _same(gpu.GpuScene(m, builder=GPU).find_intersection(mr), gpu.GpuScene(m).find_intersection(mr), "doubled mesh")
+
+
+def test_gpu_build_all_intersections_and_render(gpu):
+    s = scenes.mesh_scene(level=5, width=160, height=90)
+    host, dev = gpu.GpuScene(s), gpu.GpuScene(s, builder=GPU)
+    rays = scenes.camera_rays_np(s.camera)[:4000]
+    mesh_obj = int(np.nonzero(s.objects[:, 0] == A.PVX_MESH)[0][0])
+    obj = np.full(rays.shape[0], mesh_obj, np.uint32)
+    for a, b in zip(dev.all_intersections(rays, obj, 8), host.all_intersections(rays, obj, 8)):
+        assert np.array_equal(a, b)
+    host.set_shading()
+    dev.set_shading()
+    ih, _ = host.render()
+    idv, _ = dev.render()
+    assert np.array_equal(idv, ih)
+    # the autotuner works over the GPU-built trees too
+    t = dev.autotune(scenes.camera_rays_np(s.camera))
+    assert t["variant"] in (A.PVX_TRAVERSAL_WIDE_WW, A.PVX_TRAVERSAL_WIDE_STEP, A.PVX_TRAVERSAL_BINARY)
+    _same(dev.find_intersection(rays), host.find_intersection(rays), "tuned GPU-built tree")
+
+
+def test_gpu_build_options_rejected(gpu):
+    s = scenes.sphere_scene(n=50, width=32, height=24)
+    with pytest.raises(pvx.PvxError):
+        gpu.GpuScene(s, accel=A.PVX_ACCEL_BSP, builder=GPU)
+    with pytest.raises(pvx.PvxError):
+        gpu.GpuScene(s, builder=7)
