@@ -50,8 +50,15 @@ enum {
     PVX_PLANE = 4,
     PVX_MESH = 5,
     PVX_CSG = 6,
-    PVX_TRIANGLE = 7,       /* stand-alone triangle (triangle.cpp, Triangle)              */
-    PVX_SMOOTH_TRIANGLE = 8 /* stand-alone smooth triangle (triangle.cpp, SmoothTriangle) */
+    PVX_TRIANGLE = 7,        /* stand-alone triangle (triangle.cpp, Triangle)              */
+    PVX_SMOOTH_TRIANGLE = 8, /* stand-alone smooth triangle (triangle.cpp, SmoothTriangle) */
+    PVX_CONE = 9             /* cone or cylinder (cone.cpp, Cone)                          */
+};
+
+/* pvx_scene_desc.cone_flags bits */
+enum {
+    PVX_CONE_OPEN = 1u << 0,    /* `open`: no end caps (CLOSED_FLAG cleared)               */
+    PVX_CONE_CYLINDER = 1u << 1 /* a `cylinder` statement (Parse_Cylinder): one radius     */
 };
 
 /* Object flags: the bits of ObjectBase::Flags that the hot path reads
@@ -163,6 +170,19 @@ typedef struct pvx_scene_desc {
     const double *triangle_vertices; /* 9n: P1, P2, P3                          */
     const double *triangle_normals;  /* 9n: N1, N2, N3 (smooth ones; may be NULL
                                         when no smooth triangle is listed)      */
+
+    /* cones and cylinders as Parse_Cone / Parse_Cylinder read them
+     * (parser/parser.cpp:2255-2315): the first point and radius are the
+     * shape's `apex`, the second its `base`; a cylinder's one radius is
+     * cone_radius[2i].  The shape's own placement (Compute_Cone_Data /
+     * Compute_Cylinder_Data, cone.cpp:715-840) is computed from these, then
+     * the optional transform is composed after it (Cone::Transform). */
+    uint32_t n_cones;
+    const double *cone_apex;        /* 3n: first point                          */
+    const double *cone_base;        /* 3n: second point                         */
+    const double *cone_radius;      /* 2n: first radius, second radius          */
+    const uint8_t *cone_flags;      /* n: PVX_CONE_* bits, or NULL (closed cones) */
+    const int32_t *cone_transform;  /* n: index into transforms or -1; or NULL  */
 } pvx_scene_desc;
 
 /* ---- rays and hits ----------------------------------------------------- */

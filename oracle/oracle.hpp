@@ -172,6 +172,20 @@ FBox box_bbox(V3 c1, V3 c2, const Transform *tr);
 FBox torus_bbox(double R, double rr, const Transform *tr);
 bool torus_inside(V3 p, const Transform *tr, double R, double rr, bool inverted);
 
+// Cone / cylinder (cone.cpp): the shape's placement and flags after
+// Compute_Cone_Data / Compute_Cylinder_Data and the object's transform.
+struct Cone {
+    Transform tr;
+    double dist = 0.0;
+    bool cylinder = false, closed = true;
+};
+Cone cone_prepare(const double *apex, const double *base, const double *radius, uint8_t flags,
+                  const Transform *user);
+int cone_intersect(const Cone &c, const Ray &r, double *depth, int *part); // up to 4 hits
+bool cone_inside(const Cone &c, V3 p, bool inverted);
+V3 cone_normal(const Cone &c, int part, V3 ip);
+FBox cone_bbox(const Cone &c);
+
 // polynomialsolver.cpp:1585-1732
 int solve_polynomial(int n, const double *c0, double *r, bool sturm, double epsilon);
 
@@ -209,6 +223,7 @@ struct Scene {
     std::vector<double> tor_R, tor_r;
     std::vector<int32_t> tor_tr;
     std::vector<uint8_t> tor_sturm;
+    std::vector<Cone> cones;
     std::vector<Transform> trs;
     std::vector<Mesh> meshes;
     std::vector<pvx_csg_node> csg;

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cstring>
+#include <stdexcept>
 #include <thread>
 
 #include "oracle.hpp"
@@ -50,6 +51,9 @@ FBox Scene::shape_bbox(uint32_t type, uint32_t index, uint32_t flags, bool *infi
         break;
     case PVX_TORUS:
         b = torus_bbox(tor_R[index], tor_r[index], tr(tor_tr[index]));
+        break;
+    case PVX_CONE:
+        b = cone_bbox(cones[index]);
         break;
     case PVX_TRIANGLE:
     case PVX_SMOOTH_TRIANGLE:
@@ -168,6 +172,13 @@ bool Scene::all_intersections(uint32_t type, uint32_t index, uint32_t flags, con
     case PVX_TORUS:
         return torus_all_intersections(r, tr(tor_tr[index]), tor_R[index], tor_r[index],
                                        tor_sturm[index] != 0, st, slot);
+    case PVX_CONE: { // Cone::All_Intersections (cone.cpp:104-129), no clipped_by
+        double dep[4];
+        int part[4];
+        const int n = cone_intersect(cones[index], r, dep, part);
+        for (int k = 0; k < n; k++) st.push_back(Isect{dep[k], slot, part[k], slot});
+        return n > 0;
+    }
     case PVX_TRIANGLE:
     case PVX_SMOOTH_TRIANGLE: { // Triangle::All_Intersections (triangle.cpp:426-445)
         double depth;
@@ -240,6 +251,8 @@ bool Scene::inside(uint32_t type, uint32_t index, uint32_t flags, V3 p) const {
         return box_inside(p, tr(box_tr[index]), box_c1[index], box_c2[index], inv);
     case PVX_TORUS:
         return torus_inside(p, tr(tor_tr[index]), tor_R[index], tor_r[index], inv);
+    case PVX_CONE:
+        return cone_inside(cones[index], p, inv);
     case PVX_CSG: {
         // CSGUnion/CSGMerge::Inside (csg.cpp:405-424), CSGIntersection::Inside
         // (csg.cpp:440-454); the CSG's own INVERTED flag is carried by its
@@ -324,6 +337,12 @@ static Scene *scene_create(const pvx_scene_desc &d) {
         s->tor_r.push_back(d.torus_minor[i]);
         s->tor_tr.push_back(d.torus_transform ? d.torus_transform[i] : -1);
         s->tor_sturm.push_back(d.torus_sturm ? d.torus_sturm[i] : 0);
+    }
+    for (uint32_t i = 0; i < d.n_cones; i++) {
+        const int32_t t = d.cone_transform ? d.cone_transform[i] : -1;
+        s->cones.push_back(cone_prepare(d.cone_apex + 3 * (size_t)i, d.cone_base + 3 * (size_t)i,
+                                        d.cone_radius + 2 * (size_t)i, d.cone_flags ? d.cone_flags[i] : 0,
+                                        t >= 0 ? &s->trs[t] : nullptr));
     }
     if (d.n_triangles) s->tri_p.assign(d.triangle_vertices, d.triangle_vertices + 9 * (size_t)d.n_triangles);
     s->tris.resize(d.n_triangles);
@@ -439,7 +458,11 @@ extern "C" {
 
 void *orc_scene_create(const pvx_scene_desc *desc) {
     if (!desc) return nullptr;
-    return scene_create(*desc);
+    try {
+        return scene_create(*desc);
+    } catch (const std::exception &) { // e.g. a degenerate cone (the reference throws too)
+        return nullptr;
+    }
 }
 
 void orc_scene_destroy(void *s) { delete static_cast<Scene *>(s); }

@@ -10,6 +10,8 @@ PVX_OK, PVX_EINVAL, PVX_ENOMEM, PVX_EDEVICE, PVX_EUNSUPPORTED, PVX_ESTATE = rang
 
 PVX_SPHERE, PVX_BOX, PVX_TORUS, PVX_PLANE, PVX_MESH, PVX_CSG = 1, 2, 3, 4, 5, 6
 PVX_TRIANGLE, PVX_SMOOTH_TRIANGLE = 7, 8
+PVX_CONE = 9
+PVX_CONE_OPEN, PVX_CONE_CYLINDER = 1, 2
 
 PVX_FLAG_NO_SHADOW = 1 << 0
 PVX_FLAG_NO_IMAGE = 1 << 1
@@ -61,6 +63,8 @@ class PvxSceneDesc(C.Structure):
         ("n_csg", C.c_uint32), ("csg", C.POINTER(PvxCsgNode)),
         ("n_csg_children", C.c_uint32), ("csg_children", C.POINTER(PvxObject)),
         ("n_triangles", C.c_uint32), ("triangle_vertices", c_dp), ("triangle_normals", c_dp),
+        ("n_cones", C.c_uint32), ("cone_apex", c_dp), ("cone_base", c_dp), ("cone_radius", c_dp),
+        ("cone_flags", c_u8p), ("cone_transform", c_i32p),
     ]
 
 

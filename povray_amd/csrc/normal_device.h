@@ -49,6 +49,21 @@ __device__ inline D3 leaf_normal(const NormalParams &p, uint32_t type, uint32_t 
         if (b.xf >= 0) n = normalize3(mtrans_normal(sv.xforms[b.xf].inv, n));
         return n;
     }
+    case PVX_CONE: { // Cone::Normal (cone.cpp:440-480): the part of the hit at depth t
+        const ConeRec &c = sv.cones[index];
+        double hs[4];
+        int part[4] = {0, 0, 0, 0};
+        const int nh = cone_hits(c, sv.xforms, o, d, hs, part);
+        int pt = 0;
+        for (int k = 0; k < nh; k++)
+            if (hs[k] == t) pt = part[k]; // the last one pushed, as the IStack scan from the top
+        const Xform &x = sv.xforms[c.xf];
+        D3 n = xf_point(x.inv, ip);
+        if (pt == 3) n.z = (c.flags & kConeCylinder) ? 0.0 : -n.z;
+        else if (pt == 1) n = D3{0.0, 0.0, -1.0};
+        else if (pt == 2) n = D3{0.0, 0.0, 1.0};
+        return normalize3(mtrans_normal(x.inv, n));
+    }
     case PVX_TORUS: { // Torus::Normal (torus.cpp:458-490)
         const TorusRec &tr = sv.tori[index];
         const Xform &x = sv.xforms[tr.xf];

@@ -123,9 +123,10 @@ __device__ inline bool simple_closest(const SceneView &sv, uint32_t kind, uint32
         return true;
     }
     double hs[kMaxLeafHits];
-    int side[2];
-    int nh = kind == kPrimBox ? box_hits(sv.boxes[index], sv.xforms, o, d, hs, side)
-                              : torus_hits(sv.tori[index], sv.xforms, o, d, hs);
+    int side[4];
+    int nh = kind == kPrimBox    ? box_hits(sv.boxes[index], sv.xforms, o, d, hs, side)
+             : kind == kPrimCone ? cone_hits(sv.cones[index], sv.xforms, o, d, hs, side)
+                                 : torus_hits(sv.tori[index], sv.xforms, o, d, hs);
     bool found = false;
     double best = 0.0;
     for (int j = 0; j < nh; j++) {
@@ -959,6 +960,7 @@ extern "C" int pvx_scene_create(const pvx_scene_desc *desc, const pvx_build_opts
     UP(h.planes, planes);
     UP(h.boxes, boxes);
     UP(h.tori, tori);
+    UP(h.cones, cones);
     UP(h.xforms, xforms);
     UP(h.sph, sph);
     UP(h.csg, csg);

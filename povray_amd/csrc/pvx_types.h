@@ -32,6 +32,7 @@ enum PrimKind : uint32_t {
     kPrimCsg = 4,
     kPrimPlane = 5,
     kPrimTriangle64 = 6, // stand-alone triangle, by reference (FP64 record)
+    kPrimCone = 7,       // cone / cylinder, by reference
 };
 
 struct alignas(16) Node {
@@ -137,6 +138,18 @@ struct TorusRec {
     uint32_t pad;
 };
 
+// Cone / cylinder after Compute_Cone_Data / Compute_Cylinder_Data
+// (core/shape/cone.cpp:715-840): the canonical shape is the frustum of
+// x^2 + y^2 = z^2 for z in [dist, 1] (a cone) or the unit cylinder
+// x^2 + y^2 = 1 for z in [0, 1], placed by xforms[xf].
+constexpr uint32_t kConeCylinder = 1u; // CYLINDER_FLAG
+constexpr uint32_t kConeClosed = 2u;   // CLOSED_FLAG (end caps)
+struct ConeRec {
+    double dist;
+    int32_t xf;
+    uint32_t flags;
+};
+
 struct PlaneRec {
     double n[3];
     double d;
@@ -207,6 +220,7 @@ struct SceneView {
     const PlaneRec *planes;
     const BoxRec *boxes;
     const TorusRec *tori;
+    const ConeRec *cones;
     const Xform *xforms;
     const double *sph; // sphere array (cx, cy, cz, r) per sphere, for CSG leaves
     const CsgNode *csg;

@@ -95,6 +95,116 @@ DBox transformed_box(const double *lo, const double *hi, const double *m) {
     return b;
 }
 
+// TRANSFORM builders in the reference's arithmetic (core/math/matrix.cpp),
+// on the flat row-major Xform layout: m[4 * row + col].
+void xf_identity(Xform &x) {
+    for (int k = 0; k < 16; k++) x.m[k] = x.inv[k] = (k % 5 == 0) ? 1.0 : 0.0;
+}
+// Compose_Transforms (matrix.cpp:788-793): m = m * a.m (MTimesA),
+// inv = a.inv * inv (MTimesB), each entry summed over k = 0..3 in order.
+void xf_compose(Xform &x, const Xform &a) {
+    double r[16];
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++)
+            r[4 * i + j] = x.m[4 * i] * a.m[j] + x.m[4 * i + 1] * a.m[4 + j] + x.m[4 * i + 2] * a.m[8 + j] +
+                           x.m[4 * i + 3] * a.m[12 + j];
+    std::memcpy(x.m, r, sizeof r);
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++)
+            r[4 * i + j] = a.inv[4 * i] * x.inv[j] + a.inv[4 * i + 1] * x.inv[4 + j] +
+                           a.inv[4 * i + 2] * x.inv[8 + j] + a.inv[4 * i + 3] * x.inv[12 + j];
+    std::memcpy(x.inv, r, sizeof r);
+}
+// Compute_Coordinate_Transform (matrix.cpp:880-906): scale (radius, radius,
+// length), rotate z onto `up` about (-up.y, up.x, 0) (Compute_Axis_Rotation_
+// Transform, matrix.cpp:825-850), translate to `origin`.
+void xf_coordinate(Xform &x, const double *origin, double *up, double radius, double length) {
+    xf_identity(x);
+    x.m[0] = radius;
+    x.m[5] = radius;
+    x.m[10] = length;
+    x.inv[0] = 1.0 / radius;
+    x.inv[5] = 1.0 / radius;
+    x.inv[10] = 1.0 / length;
+    double ax[3];
+    if (std::fabs(up[2]) > 1.0 - 1.0e-10) {
+        ax[0] = 1.0, ax[1] = 0.0, ax[2] = 0.0;
+        up[2] = up[2] < 0.0 ? -1.0 : 1.0;
+    } else {
+        ax[0] = -up[1], ax[1] = up[0], ax[2] = 0.0;
+    }
+    const double angle = std::acos(up[2]);
+    const double l = std::sqrt(ax[0] * ax[0] + ax[1] * ax[1] + ax[2] * ax[2]);
+    const double v[3] = {l != 0 ? ax[0] / l : ax[0], l != 0 ? ax[1] / l : ax[1], l != 0 ? ax[2] / l : ax[2]};
+    const double c = std::cos(angle), s = std::sin(angle);
+    Xform rot;
+    xf_identity(rot);
+    rot.m[0] = v[0] * v[0] + c * (1.0 - v[0] * v[0]);
+    rot.m[1] = v[0] * v[1] * (1.0 - c) + v[2] * s;
+    rot.m[2] = v[0] * v[2] * (1.0 - c) - v[1] * s;
+    rot.m[4] = v[0] * v[1] * (1.0 - c) - v[2] * s;
+    rot.m[5] = v[1] * v[1] + c * (1.0 - v[1] * v[1]);
+    rot.m[6] = v[1] * v[2] * (1.0 - c) + v[0] * s;
+    rot.m[8] = v[0] * v[2] * (1.0 - c) + v[1] * s;
+    rot.m[9] = v[1] * v[2] * (1.0 - c) - v[0] * s;
+    rot.m[10] = v[2] * v[2] + c * (1.0 - v[2] * v[2]);
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) rot.inv[4 * i + j] = rot.m[4 * j + i];
+    xf_compose(x, rot);
+    Xform tr;
+    xf_identity(tr);
+    for (int k = 0; k < 3; k++) {
+        tr.m[12 + k] = origin[k];
+        tr.inv[12 + k] = -origin[k];
+    }
+    xf_compose(x, tr);
+}
+
+// A cone / cylinder statement as Parse_Cone / Parse_Cylinder leave it
+// (parser.cpp:2255-2315) after Compute_Cone_Data / Compute_Cylinder_Data
+// (cone.cpp:715-840).  Returns false for a degenerate shape (the reference
+// throws "Degenerate cone/cylinder").
+bool make_cone(const double *apex_in, const double *base_in, const double *radius, uint8_t flags, ConeRec &c,
+               Xform &x) {
+    double apex[3], base[3];
+    std::memcpy(apex, apex_in, sizeof apex);
+    std::memcpy(base, base_in, sizeof base);
+    const bool cyl_stmt = (flags & PVX_CONE_CYLINDER) != 0;
+    double ar = radius[0], br = cyl_stmt ? radius[0] : radius[1];
+    c.flags = (flags & PVX_CONE_OPEN) ? 0u : kConeClosed;
+    c.dist = 0.0;
+    auto cylinder_data = [&]() -> bool {
+        double axis[3] = {apex[0] - base[0], apex[1] - base[1], apex[2] - base[2]};
+        const double l = std::sqrt(axis[0] * axis[0] + axis[1] * axis[1] + axis[2] * axis[2]);
+        if (l < kEpsilon) return false;
+        for (int k = 0; k < 3; k++) axis[k] = axis[k] / l;
+        xf_coordinate(x, base, axis, ar, l);
+        c.flags |= kConeCylinder;
+        c.dist = 0.0;
+        return true;
+    };
+    if (cyl_stmt) return cylinder_data();
+    double axis[3] = {apex[0] - base[0], apex[1] - base[1], apex[2] - base[2]};
+    const double len = std::sqrt(axis[0] * axis[0] + axis[1] * axis[1] + axis[2] * axis[2]);
+    if (len < kEpsilon) return false;
+    for (int k = 0; k < 3; k++) axis[k] = axis[k] / len;
+    if (std::fabs(ar - br) < kEpsilon) return cylinder_data();
+    if (ar < br) { // the bigger end at the top
+        for (int k = 0; k < 3; k++) {
+            std::swap(apex[k], base[k]);
+            axis[k] = -axis[k];
+        }
+        std::swap(ar, br);
+    }
+    const double tmpf = br * len / (ar - br);
+    const double origin[3] = {base[0] - axis[0] * tmpf, base[1] - axis[1] * tmpf, base[2] - axis[2] * tmpf};
+    const double tlen = tmpf + len;
+    if (((ar - br) * len / tlen) < kEpsilon) return cylinder_data();
+    c.dist = tmpf / tlen;
+    xf_coordinate(x, origin, axis, ar, tlen);
+    return true;
+}
+
 inline int max3_axis(double x, double y, double z) { // mesh.cpp:94 max3_coordinate
     return (x > y) ? ((x > z) ? 0 : 2) : ((y > z) ? 1 : 2);
 }
@@ -248,6 +358,7 @@ int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene
     if (!need(d.n_meshes == 0 || d.meshes, "meshes")) return PVX_EINVAL;
     if (!need(d.n_csg == 0 || (d.csg && d.csg_children), "csg")) return PVX_EINVAL;
     if (!need(d.n_triangles == 0 || d.triangle_vertices, "triangles")) return PVX_EINVAL;
+    if (!need(d.n_cones == 0 || (d.cone_apex && d.cone_base && d.cone_radius), "cones")) return PVX_EINVAL;
 
     auto check_ref = [&](uint32_t type, uint32_t index) -> bool {
         uint32_t lim = 0;
@@ -260,6 +371,7 @@ int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene
         case PVX_CSG: lim = d.n_csg; break;
         case PVX_TRIANGLE:
         case PVX_SMOOTH_TRIANGLE: lim = d.n_triangles; break;
+        case PVX_CONE: lim = d.n_cones; break;
         default:
             err = "unknown object type " + std::to_string(type);
             return false;
@@ -326,6 +438,30 @@ int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene
         t.sturm = d.torus_sturm ? d.torus_sturm[i] : 0;
         t.flags = 0;
         t.pad = 0;
+    }
+
+    // cones / cylinders: the shape's own placement, then the object's transform
+    out.cones.resize(d.n_cones);
+    for (uint32_t i = 0; i < d.n_cones; i++) {
+        Xform x;
+        if (!make_cone(d.cone_apex + 3 * (size_t)i, d.cone_base + 3 * (size_t)i, d.cone_radius + 2 * (size_t)i,
+                       d.cone_flags ? d.cone_flags[i] : 0, out.cones[i], x)) {
+            err = "degenerate cone/cylinder " + std::to_string(i) + " (base point = apex point)";
+            return PVX_EINVAL;
+        }
+        const int32_t t = d.cone_transform ? d.cone_transform[i] : -1;
+        if (t >= (int32_t)d.n_transforms) {
+            err = "cone transform index out of range";
+            return PVX_EINVAL;
+        }
+        if (t >= 0) {
+            Xform u;
+            std::memcpy(u.m, d.transforms + 32 * (size_t)t, 16 * sizeof(double));
+            std::memcpy(u.inv, d.transforms + 32 * (size_t)t + 16, 16 * sizeof(double));
+            xf_compose(x, u);
+        }
+        out.cones[i].xf = (int32_t)out.xforms.size();
+        out.xforms.push_back(x);
     }
 
     // stand-alone triangles, flat or smooth as their objects say
@@ -451,6 +587,11 @@ int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene
             double lo[3] = {-a, -t.r, -a}, hi[3] = {a, t.r, a};
             return transformed_box(lo, hi, out.xforms[t.xf].m);
         }
+        case PVX_CONE: { // Cone::Compute_BBox (cone.cpp:870-880)
+            const ConeRec &c = out.cones[index];
+            double lo[3] = {-1.0, -1.0, c.dist}, hi[3] = {1.0, 1.0, 1.0};
+            return transformed_box(lo, hi, out.xforms[c.xf].m);
+        }
         default:
             return DBox::unbounded();
         }
@@ -527,8 +668,9 @@ int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene
             break;
         }
         case PVX_BOX:
-        case PVX_TORUS: {
-            pr.kind_axis = ob.type == PVX_BOX ? kPrimBox : kPrimTorus;
+        case PVX_TORUS:
+        case PVX_CONE: {
+            pr.kind_axis = ob.type == PVX_BOX ? kPrimBox : ob.type == PVX_TORUS ? kPrimTorus : kPrimCone;
             pr.ref.index = ob.index;
             to_fbox(shape_bnd(ob.type, ob.index, 0).S, lo, hi);
             add_prim(pr, lo, hi);

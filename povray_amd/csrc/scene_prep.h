@@ -29,6 +29,7 @@ struct HostScene {
     std::vector<BoxRec> boxes;
     std::vector<TriangleRec> triangles;
     std::vector<TorusRec> tori;
+    std::vector<ConeRec> cones;
     std::vector<Xform> xforms;
     std::vector<double> sph; // 4 per sphere: centre, radius
     std::vector<CsgNode> csg;

@@ -663,6 +663,85 @@ __device__ inline bool torus_inside(const TorusRec &tr, const Xform *xf, D3 p, b
 }
 
 // ---------------------------------------------------------------------------
+// Cone / cylinder (core/shape/cone.cpp)
+// ---------------------------------------------------------------------------
+// Cone::Intersect (cone.cpp:140-300): the ray in canonical space (direction
+// normalised, its length kept), side hits, then the cap (z = 1) and the base
+// (z = dist) of a closed shape.  part: 1 base, 2 cap, 3 side (the
+// Intersection::i1 Cone::Normal reads).  Returns the hit count (<= 4) in
+// All_Intersections' push order.
+__device__ inline int cone_hits(const ConeRec &c, const Xform *xf, D3 o, D3 d, double *out, int *part) {
+    constexpr double kTol = 1.0e-9; // Cone_Tolerance (cone.cpp:64)
+    const Xform &x = xf[c.xf];
+    const D3 P = xf_point(x.inv, o);
+    D3 D = xf_dir(x.inv, d);
+    const double len = sqrt(D.x * D.x + D.y * D.y + D.z * D.z);
+    D = D3{D.x / len, D.y / len, D.z / len};
+    int n = 0;
+    auto side = [&](double t, double zlo) {
+        const double z = P.z + t * D.z;
+        if (t > kTol && t < kMaxDistance && z >= zlo && z <= 1.0) {
+            out[n] = t / len;
+            part[n++] = 3;
+        }
+    };
+    if (c.flags & kConeCylinder) {
+        const double a = D.x * D.x + D.y * D.y;
+        if (a > kEpsilon) {
+            const double b = P.x * D.x + P.y * D.y;
+            const double cc = P.x * P.x + P.y * P.y - 1.0;
+            double dd = b * b - a * cc;
+            if (dd >= 0.0) {
+                dd = sqrt(dd);
+                side((-b + dd) / a, 0.0);
+                side((-b - dd) / a, 0.0);
+            }
+        }
+    } else {
+        const double a = D.x * D.x + D.y * D.y - D.z * D.z;
+        const double b = D.x * P.x + D.y * P.y - D.z * P.z;
+        const double cc = P.x * P.x + P.y * P.y - P.z * P.z;
+        if (fabs(a) < kEpsilon) {
+            if (fabs(b) > kEpsilon) side(-0.5 * cc / b, c.dist);
+        } else {
+            double dd = b * b - a * cc;
+            if (dd >= 0.0) {
+                dd = sqrt(dd);
+                side((-b - dd) / a, c.dist);
+                side((-b + dd) / a, c.dist);
+            }
+        }
+    }
+    if ((c.flags & kConeClosed) && fabs(D.z) > kEpsilon) {
+        double t = (1.0 - P.z) / D.z;
+        double u = P.x + t * D.x, v = P.y + t * D.y;
+        if ((u * u + v * v) <= 1.0 && t > kTol && t < kMaxDistance) {
+            out[n] = t / len;
+            part[n++] = 2;
+        }
+        t = (c.dist - P.z) / D.z;
+        u = P.x + t * D.x;
+        v = P.y + t * D.y;
+        if ((u * u + v * v) <= ((c.flags & kConeCylinder) ? 1.0 : c.dist * c.dist) && t > kTol && t < kMaxDistance) {
+            out[n] = t / len;
+            part[n++] = 1;
+        }
+    }
+    return n;
+}
+
+// Cone::Inside (cone.cpp:360-410): a closed shape shrinks by EPSILON, an
+// open one grows by it.
+__device__ inline bool cone_inside(const ConeRec &c, const Xform *xf, D3 p, bool inv) {
+    const double off = (c.flags & kConeClosed) ? -kEpsilon : kEpsilon;
+    const D3 q = xf_point(xf[c.xf].inv, p);
+    const double w2 = q.x * q.x + q.y * q.y;
+    const bool out = (c.flags & kConeCylinder) ? (w2 > 1.0 + off || q.z < 0.0 - off || q.z > 1.0 + off)
+                                               : (w2 > q.z * q.z + off || q.z < c.dist - off || q.z > 1.0 + off);
+    return out ? inv : !inv;
+}
+
+// ---------------------------------------------------------------------------
 // CSG (core/shape/csg.cpp): point membership with bounded nesting.
 // ---------------------------------------------------------------------------
 template <int D>
@@ -681,6 +760,8 @@ __device__ inline bool shape_inside(const SceneView &sv, uint32_t type, uint32_t
         return box_inside(sv.boxes[index], sv.xforms, p, inv);
     case 3:
         return torus_inside(sv.tori[index], sv.xforms, p, inv);
+    case 9:
+        return cone_inside(sv.cones[index], sv.xforms, p, inv);
     case 4: { // plane: Plane::Inside (plane.cpp:192-215)
         const PlaneRec &pl = sv.planes[index];
         double t = p.x * pl.n[0] + p.y * pl.n[1] + p.z * pl.n[2];
@@ -713,7 +794,7 @@ __device__ bool csg_inside(const SceneView &sv, uint32_t node, D3 p) {
     }
 }
 
-// Hits of one leaf shape (sphere/box/torus/triangle) in push order.
+// Hits of one leaf shape (sphere/box/torus/cone/triangle) in push order.
 __device__ inline int leaf_hits(const SceneView &sv, uint32_t type, uint32_t index, D3 o, D3 d,
                                 double *out) {
     int side[2];
@@ -726,6 +807,10 @@ __device__ inline int leaf_hits(const SceneView &sv, uint32_t type, uint32_t ind
         return box_hits(sv.boxes[index], sv.xforms, o, d, out, side);
     case 3:
         return torus_hits(sv.tori[index], sv.xforms, o, d, out);
+    case 9: {
+        int part[4];
+        return cone_hits(sv.cones[index], sv.xforms, o, d, out, part);
+    }
     case 7:
     case 8: { // stand-alone triangle (not a CSG child; All_Intersections)
         double dep;

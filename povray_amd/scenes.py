@@ -93,6 +93,11 @@ class Scene:
     csg_children: np.ndarray = None          # (n, 4) uint32 like objects
     triangle_vertices: np.ndarray = None     # (n, 9) float64: P1, P2, P3
     triangle_normals: np.ndarray = None      # (n, 9) float64: N1, N2, N3 (smooth triangles)
+    cone_apex: np.ndarray = None             # (n, 3) float64: first point of cone{} / cylinder{}
+    cone_base: np.ndarray = None             # (n, 3) float64: second point
+    cone_radius: np.ndarray = None           # (n, 2) float64: first, second radius
+    cone_flags: np.ndarray = None            # (n,) uint8: PVX_CONE_OPEN | PVX_CONE_CYLINDER
+    cone_transform: np.ndarray = None        # (n,) int32: transform index or -1
     camera: dict = None
     lights: list = field(default_factory=list)
     materials: dict = field(default_factory=dict)  # per top-level object arrays
@@ -163,6 +168,14 @@ class Scene:
             d.n_triangles = int(np.asarray(self.triangle_vertices).reshape(-1, 9).shape[0])
             if self.triangle_normals is not None:
                 d.triangle_normals, _ = arr(np.asarray(self.triangle_normals).reshape(-1), np.float64, C.c_double)
+        if self.cone_apex is not None and len(self.cone_apex):
+            d.cone_apex, d.n_cones = arr(np.asarray(self.cone_apex).reshape(-1, 3), np.float64, C.c_double)
+            d.cone_base, _ = arr(np.asarray(self.cone_base).reshape(-1, 3), np.float64, C.c_double)
+            d.cone_radius, _ = arr(np.asarray(self.cone_radius).reshape(-1, 2), np.float64, C.c_double)
+            if self.cone_flags is not None:
+                d.cone_flags, _ = arr(self.cone_flags, np.uint8, C.c_uint8)
+            if self.cone_transform is not None:
+                d.cone_transform, _ = arr(self.cone_transform, np.int32, C.c_int32)
         if self.csg is not None and len(self.csg):
             c = np.ascontiguousarray(self.csg, np.uint32).reshape(-1, 4)
             ch = np.ascontiguousarray(self.csg_children, np.uint32).reshape(-1, 4)
@@ -740,5 +753,96 @@ def triangle_scene(seed: int = 0xC0FFEE + 5, n: int = 400, n_spheres: int = 40, 
         lights=[dict(kind="point", position=np.array([15.0, 25.0, -15.0]), colour=np.ones(3))],
         materials=dict(pigment=alb, diffuse=np.full(m, 0.6), ambient=np.full(m, 0.1), phong=np.full(m, 0.4),
                        phong_size=np.full(m, 40.0), reflection=refl, ior=np.ones(m), transmit=np.zeros(m)),
+        settings=dict(max_trace_level=5, adc_bailout=1.0 / 255.0),
+    )
+
+
+# --------------------------------------------------------------------------
+# cones and cylinders
+# --------------------------------------------------------------------------
+def cone_scene(seed: int = 0xC0FFEE + 6, n: int = 300, n_csg: int = 40, width=320, height=240) -> Scene:
+    """`n` seeded cones and cylinders in a 12-unit cube: a third `cylinder`
+    statements, the rest cones with radii U[0, 0.8] (a pointed end at radius
+    0 for one in eight; one in eight with equal radii, which the reference
+    turns into a cylinder; either end may be the bigger); one in four `open`;
+    one in three with a random rotation composed after the shape's own
+    placement.  `n_csg` CSG objects: a box minus a cylinder (difference) or
+    a cone intersected with a sphere.  A ground plane, one point light; 10%
+    reflective, 10% glass."""
+    r = Rng(seed)
+    m = n + 2 * n_csg
+    apex = r.uniform(3 * m, -6.0, 6.0).reshape(m, 3)
+    axis = r.uniform(3 * m, -1.0, 1.0).reshape(m, 3)
+    axis /= np.maximum(np.linalg.norm(axis, axis=1, keepdims=True), 1e-9)
+    base = apex + axis * r.uniform(m, 0.5, 2.5)[:, None]
+    rad = r.uniform(2 * m, 0.0, 0.8).reshape(m, 2) + 0.05
+    k = np.arange(m)
+    rad[k % 8 == 3, 1] = 0.0            # pointed cone
+    rad[k % 8 == 5, 1] = rad[k % 8 == 5, 0]  # equal radii: the cone becomes a cylinder
+    flags = np.where(k % 3 == 0, A.PVX_CONE_CYLINDER, 0).astype(np.uint8)
+    flags |= np.where(k % 4 == 1, A.PVX_CONE_OPEN, 0).astype(np.uint8)
+    xforms, tr = [], np.full(m, -1, np.int32)
+    for i in range(m):
+        if i % 3 == 2:
+            xforms.append(_xf_row(rotation_transform(r.uniform(3, 0.0, 360.0))))
+            tr[i] = len(xforms) - 1
+    # CSG pieces
+    box_c1, box_c2, sph_c, sph_r = [], [], [], []
+    csg_nodes, csg_children = [], []
+    roots = []
+    for j in range(n_csg):
+        ca, cb = n + 2 * j, n + 2 * j + 1
+        c = r.uniform(3, -5.0, 5.0)
+        if j % 2 == 0:  # box minus a cylinder through it
+            e = r.uniform(3, 0.6, 1.4)
+            box_c1.append(c - e)
+            box_c2.append(c + e)
+            apex[ca] = c - np.array([0.0, 2.0, 0.0])
+            base[ca] = c + np.array([0.0, 2.0, 0.0])
+            rad[ca] = 0.4
+            flags[ca] = A.PVX_CONE_CYLINDER
+            tr[ca] = -1
+            kids = [[A.PVX_BOX, len(box_c1) - 1, 0, 0], [A.PVX_CONE, ca, A.PVX_FLAG_INVERTED, 0]]
+            op = A.PVX_CSG_INTERSECTION
+        else:  # cone intersected with a sphere
+            apex[cb] = c + np.array([0.0, 1.2, 0.0])
+            base[cb] = c - np.array([0.0, 1.2, 0.0])
+            rad[cb] = (0.0, 1.1)
+            flags[cb] = 0
+            tr[cb] = -1
+            sph_c.append(c)
+            sph_r.append(1.0)
+            kids = [[A.PVX_CONE, cb, 0, 0], [A.PVX_SPHERE, len(sph_c) - 1, 0, 0]]
+            op = A.PVX_CSG_INTERSECTION
+        first = len(csg_children)
+        csg_children.extend(kids)
+        csg_nodes.append([op, first, 2, 0])
+        roots.append(len(csg_nodes) - 1)
+    n_obj = n + n_csg + 1
+    objects = np.zeros((n_obj, 4), np.uint32)
+    objects[:n, 0] = A.PVX_CONE
+    objects[:n, 1] = np.arange(n)
+    objects[n:n + n_csg, 0] = A.PVX_CSG
+    objects[n:n + n_csg, 1] = roots
+    objects[-1, 0] = A.PVX_PLANE
+    alb = r.uniform(3 * n_obj, 0.2, 0.95).reshape(n_obj, 3)
+    u = r.uniform(n_obj)
+    refl = np.where(u < 0.1, 0.3, 0.0)
+    glass = (u >= 0.1) & (u < 0.2)
+    return Scene(
+        name=f"cones{n}", objects=objects,
+        cone_apex=apex, cone_base=base, cone_radius=rad, cone_flags=flags, cone_transform=tr,
+        transforms=np.array(xforms).reshape(-1, 32),
+        box_corner1=np.array(box_c1).reshape(-1, 3), box_corner2=np.array(box_c2).reshape(-1, 3),
+        box_transform=np.full(len(box_c1), -1, np.int32),
+        sphere_center=np.array(sph_c).reshape(-1, 3), sphere_radius=np.array(sph_r),
+        plane_normal=np.array([[0.0, 1.0, 0.0]]), plane_distance=np.array([7.5]),
+        csg=np.array(csg_nodes, np.uint32).reshape(-1, 4),
+        csg_children=np.array(csg_children, np.uint32).reshape(-1, 4),
+        camera=look_at_camera((0.0, 6.0, -20.0), (0.0, 0.0, 0.0), width, height),
+        lights=[dict(kind="point", position=np.array([15.0, 25.0, -15.0]), colour=np.ones(3))],
+        materials=dict(pigment=alb, diffuse=np.full(n_obj, 0.6), ambient=np.full(n_obj, 0.1),
+                       phong=np.full(n_obj, 0.4), phong_size=np.full(n_obj, 40.0), reflection=refl,
+                       ior=np.where(glass, 1.5, 1.0), transmit=np.where(glass, 0.9, 0.0)),
         settings=dict(max_trace_level=5, adc_bailout=1.0 / 255.0),
     )
