@@ -52,7 +52,8 @@ enum {
     PVX_CSG = 6,
     PVX_TRIANGLE = 7,        /* stand-alone triangle (triangle.cpp, Triangle)              */
     PVX_SMOOTH_TRIANGLE = 8, /* stand-alone smooth triangle (triangle.cpp, SmoothTriangle) */
-    PVX_CONE = 9             /* cone or cylinder (cone.cpp, Cone)                          */
+    PVX_CONE = 9,            /* cone or cylinder (cone.cpp, Cone)                          */
+    PVX_DISC = 10            /* disc, optionally with a hole (disc.cpp, Disc)              */
 };
 
 /* pvx_scene_desc.cone_flags bits */
@@ -183,6 +184,15 @@ typedef struct pvx_scene_desc {
     const double *cone_radius;      /* 2n: first radius, second radius          */
     const uint8_t *cone_flags;      /* n: PVX_CONE_* bits, or NULL (closed cones) */
     const int32_t *cone_transform;  /* n: index into transforms or -1; or NULL  */
+
+    /* discs as Parse_Disc reads them: centre, normal (normalised here),
+     * radius and hole radius; the placement (Compute_Disc) is computed from
+     * them, the optional transform composed after it (Disc::Transform). */
+    uint32_t n_discs;
+    const double *disc_center;      /* 3n                                       */
+    const double *disc_normal;      /* 3n                                       */
+    const double *disc_radius;      /* 2n: radius, hole radius (0: none)        */
+    const int32_t *disc_transform;  /* n: index into transforms or -1; or NULL  */
 } pvx_scene_desc;
 
 /* ---- rays and hits ----------------------------------------------------- */

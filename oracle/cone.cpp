@@ -11,15 +11,13 @@
 
 namespace orc {
 
-namespace {
-
-void identity(double (*m)[4]) {
+static void identity(double (*m)[4]) {
     for (int i = 0; i < 4; i++)
         for (int j = 0; j < 4; j++) m[i][j] = i == j ? 1.0 : 0.0;
 }
 
 // Compute_Scaling_Transform (matrix.cpp:535-548)
-void scaling(Transform &t, V3 v) {
+static void scaling(Transform &t, V3 v) {
     identity(t.m);
     t.m[0][0] = v.x;
     t.m[1][1] = v.y;
@@ -31,7 +29,7 @@ void scaling(Transform &t, V3 v) {
 }
 
 // Compute_Translation_Transform (matrix.cpp:644-657)
-void translation(Transform &t, V3 v) {
+static void translation(Transform &t, V3 v) {
     identity(t.m);
     t.m[3][0] = v.x;
     t.m[3][1] = v.y;
@@ -43,7 +41,7 @@ void translation(Transform &t, V3 v) {
 }
 
 // Compute_Axis_Rotation_Transform (matrix.cpp:825-850)
-void axis_rotation(Transform &t, V3 axis, double angle) {
+static void axis_rotation(Transform &t, V3 axis, double angle) {
     const double l = length(axis);
     const V3 v = l != 0 ? axis / l : axis;
     identity(t.m);
@@ -76,8 +74,9 @@ void compose(Transform &t, const Transform &a) {
     std::memcpy(t.inv, r, sizeof r);
 }
 
-// Compute_Coordinate_Transform (matrix.cpp:880-906)
-void coordinate_transform(Transform &t, V3 origin, V3 up, double radius, double len) {
+// Compute_Coordinate_Transform (matrix.cpp:880-906); like the reference it
+// snaps the caller's `up` to +-z when it is within EPSILON of it.
+void coordinate_transform(Transform &t, V3 origin, V3 &up, double radius, double len) {
     scaling(t, V3(radius, radius, len));
     V3 tmpv;
     if (std::fabs(up.z) > 1.0 - kEpsilon) {
@@ -92,8 +91,6 @@ void coordinate_transform(Transform &t, V3 origin, V3 up, double radius, double 
     translation(t2, origin);
     compose(t, t2);
 }
-
-} // namespace
 
 // Parse_Cone / Parse_Cylinder + Compute_Cone_Data / Compute_Cylinder_Data
 // (parser.cpp:2255-2315, cone.cpp:715-840) + Cone::Transform (cone.cpp:660).

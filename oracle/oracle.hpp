@@ -172,6 +172,23 @@ FBox box_bbox(V3 c1, V3 c2, const Transform *tr);
 FBox torus_bbox(double R, double rr, const Transform *tr);
 bool torus_inside(V3 p, const Transform *tr, double R, double rr, bool inverted);
 
+// TRANSFORM builders (core/math/matrix.cpp): Compose_Transforms
+// (matrix.cpp:788-793) and Compute_Coordinate_Transform (matrix.cpp:880-906).
+void compose(Transform &t, const Transform &add);
+void coordinate_transform(Transform &t, V3 origin, V3 &up, double radius, double len);
+
+// Disc (disc.cpp): placement, normal and radii after Parse_Disc /
+// Compute_Disc and the object's transform.
+struct Disc {
+    Transform tr;
+    V3 normal;
+    double iradius2 = 0.0, oradius2 = 1.0;
+};
+Disc disc_prepare(const double *center, const double *normal, const double *radius, const Transform *user);
+bool disc_intersect(const Disc &c, const Ray &r, double *depth);
+bool disc_inside(const Disc &c, V3 p, bool inverted);
+FBox disc_bbox(const Disc &c);
+
 // Cone / cylinder (cone.cpp): the shape's placement and flags after
 // Compute_Cone_Data / Compute_Cylinder_Data and the object's transform.
 struct Cone {
@@ -224,6 +241,7 @@ struct Scene {
     std::vector<int32_t> tor_tr;
     std::vector<uint8_t> tor_sturm;
     std::vector<Cone> cones;
+    std::vector<Disc> discs;
     std::vector<Transform> trs;
     std::vector<Mesh> meshes;
     std::vector<pvx_csg_node> csg;

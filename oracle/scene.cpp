@@ -55,6 +55,9 @@ FBox Scene::shape_bbox(uint32_t type, uint32_t index, uint32_t flags, bool *infi
     case PVX_CONE:
         b = cone_bbox(cones[index]);
         break;
+    case PVX_DISC:
+        b = disc_bbox(discs[index]);
+        break;
     case PVX_TRIANGLE:
     case PVX_SMOOTH_TRIANGLE:
         b = tri_bbox(&tri_p[9 * (size_t)index]);
@@ -179,6 +182,12 @@ bool Scene::all_intersections(uint32_t type, uint32_t index, uint32_t flags, con
         for (int k = 0; k < n; k++) st.push_back(Isect{dep[k], slot, part[k], slot});
         return n > 0;
     }
+    case PVX_DISC: { // Disc::All_Intersections (disc.cpp), no clipped_by
+        double dep;
+        if (!disc_intersect(discs[index], r, &dep)) return false;
+        st.push_back(Isect{dep, slot, 0, slot});
+        return true;
+    }
     case PVX_TRIANGLE:
     case PVX_SMOOTH_TRIANGLE: { // Triangle::All_Intersections (triangle.cpp:426-445)
         double depth;
@@ -253,6 +262,8 @@ bool Scene::inside(uint32_t type, uint32_t index, uint32_t flags, V3 p) const {
         return torus_inside(p, tr(tor_tr[index]), tor_R[index], tor_r[index], inv);
     case PVX_CONE:
         return cone_inside(cones[index], p, inv);
+    case PVX_DISC:
+        return disc_inside(discs[index], p, inv);
     case PVX_CSG: {
         // CSGUnion/CSGMerge::Inside (csg.cpp:405-424), CSGIntersection::Inside
         // (csg.cpp:440-454); the CSG's own INVERTED flag is carried by its
@@ -343,6 +354,11 @@ static Scene *scene_create(const pvx_scene_desc &d) {
         s->cones.push_back(cone_prepare(d.cone_apex + 3 * (size_t)i, d.cone_base + 3 * (size_t)i,
                                         d.cone_radius + 2 * (size_t)i, d.cone_flags ? d.cone_flags[i] : 0,
                                         t >= 0 ? &s->trs[t] : nullptr));
+    }
+    for (uint32_t i = 0; i < d.n_discs; i++) {
+        const int32_t t = d.disc_transform ? d.disc_transform[i] : -1;
+        s->discs.push_back(disc_prepare(d.disc_center + 3 * (size_t)i, d.disc_normal + 3 * (size_t)i,
+                                        d.disc_radius + 2 * (size_t)i, t >= 0 ? &s->trs[t] : nullptr));
     }
     if (d.n_triangles) s->tri_p.assign(d.triangle_vertices, d.triangle_vertices + 9 * (size_t)d.n_triangles);
     s->tris.resize(d.n_triangles);

@@ -11,6 +11,7 @@ PVX_OK, PVX_EINVAL, PVX_ENOMEM, PVX_EDEVICE, PVX_EUNSUPPORTED, PVX_ESTATE = rang
 PVX_SPHERE, PVX_BOX, PVX_TORUS, PVX_PLANE, PVX_MESH, PVX_CSG = 1, 2, 3, 4, 5, 6
 PVX_TRIANGLE, PVX_SMOOTH_TRIANGLE = 7, 8
 PVX_CONE = 9
+PVX_DISC = 10
 PVX_CONE_OPEN, PVX_CONE_CYLINDER = 1, 2
 
 PVX_FLAG_NO_SHADOW = 1 << 0
@@ -65,6 +66,8 @@ class PvxSceneDesc(C.Structure):
         ("n_triangles", C.c_uint32), ("triangle_vertices", c_dp), ("triangle_normals", c_dp),
         ("n_cones", C.c_uint32), ("cone_apex", c_dp), ("cone_base", c_dp), ("cone_radius", c_dp),
         ("cone_flags", c_u8p), ("cone_transform", c_i32p),
+        ("n_discs", C.c_uint32), ("disc_center", c_dp), ("disc_normal", c_dp), ("disc_radius", c_dp),
+        ("disc_transform", c_i32p),
     ]
 
 

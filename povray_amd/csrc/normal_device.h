@@ -49,6 +49,10 @@ __device__ inline D3 leaf_normal(const NormalParams &p, uint32_t type, uint32_t 
         if (b.xf >= 0) n = normalize3(mtrans_normal(sv.xforms[b.xf].inv, n));
         return n;
     }
+    case PVX_DISC: { // Disc::Normal (disc.cpp): the stored normal
+        const DiscRec &c = sv.discs[index];
+        return D3{c.n[0], c.n[1], c.n[2]};
+    }
     case PVX_CONE: { // Cone::Normal (cone.cpp:440-480): the part of the hit at depth t
         const ConeRec &c = sv.cones[index];
         double hs[4];

@@ -116,9 +116,11 @@ __device__ inline bool csg_closest(const SceneView &sv, uint32_t root, D3 o, D3 
 // (Find_Intersection over its IStack).
 __device__ inline bool simple_closest(const SceneView &sv, uint32_t kind, uint32_t index, D3 o,
                                       D3 d, bool shadow, double &dep_out) {
-    if (kind == kPrimTriangle64) {
+    if (kind == kPrimTriangle64 || kind == kPrimDisc) {
         double dep;
-        if (!tri_rec_hit(sv.triangles[index], o, d, dep) || !depth_ok(dep, shadow)) return false;
+        const bool hit = kind == kPrimDisc ? disc_hit(sv.discs[index], sv.xforms, o, d, dep)
+                                           : tri_rec_hit(sv.triangles[index], o, d, dep);
+        if (!hit || !depth_ok(dep, shadow)) return false;
         dep_out = dep;
         return true;
     }
@@ -961,6 +963,7 @@ extern "C" int pvx_scene_create(const pvx_scene_desc *desc, const pvx_build_opts
     UP(h.boxes, boxes);
     UP(h.tori, tori);
     UP(h.cones, cones);
+    UP(h.discs, discs);
     UP(h.xforms, xforms);
     UP(h.sph, sph);
     UP(h.csg, csg);

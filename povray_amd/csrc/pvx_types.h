@@ -33,6 +33,7 @@ enum PrimKind : uint32_t {
     kPrimPlane = 5,
     kPrimTriangle64 = 6, // stand-alone triangle, by reference (FP64 record)
     kPrimCone = 7,       // cone / cylinder, by reference
+    kPrimDisc = 8,       // disc, by reference
 };
 
 struct alignas(16) Node {
@@ -150,6 +151,15 @@ struct ConeRec {
     uint32_t flags;
 };
 
+// Disc after Parse_Disc / Compute_Disc (core/shape/disc.cpp): the plane
+// z = 0 of xforms[xf], squared radii, the stored (world) normal.
+struct DiscRec {
+    double n[3];
+    double ir2, or2;
+    int32_t xf;
+    uint32_t pad;
+};
+
 struct PlaneRec {
     double n[3];
     double d;
@@ -221,6 +231,7 @@ struct SceneView {
     const BoxRec *boxes;
     const TorusRec *tori;
     const ConeRec *cones;
+    const DiscRec *discs;
     const Xform *xforms;
     const double *sph; // sphere array (cx, cy, cz, r) per sphere, for CSG leaves
     const CsgNode *csg;

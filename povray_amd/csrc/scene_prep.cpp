@@ -205,6 +205,26 @@ bool make_cone(const double *apex_in, const double *base_in, const double *radiu
     return true;
 }
 
+// MTransNormal (matrix.cpp) with a transform's inverse: row i of the inverse.
+void xf_normal(const Xform &x, const double *n, double *out) {
+    for (int i = 0; i < 3; i++) out[i] = n[0] * x.inv[4 * i] + n[1] * x.inv[4 * i + 1] + n[2] * x.inv[4 * i + 2];
+}
+
+// Parse_Disc (parser.cpp) + Compute_Disc (disc.cpp): the normal normalised,
+// the placement Compute_Coordinate_Transform(center, normal, 1, 1), which
+// may snap the stored normal's z to +-1.
+void make_disc(const double *center, const double *normal, const double *radius, DiscRec &c, Xform &x) {
+    double n[3] = {normal[0], normal[1], normal[2]};
+    const double l = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    if (l != 0)
+        for (int k = 0; k < 3; k++) n[k] = n[k] / l;
+    c.or2 = radius[0] * radius[0];
+    c.ir2 = radius[1] * radius[1];
+    xf_coordinate(x, center, n, 1.0, 1.0);
+    std::memcpy(c.n, n, sizeof n);
+    c.pad = 0;
+}
+
 inline int max3_axis(double x, double y, double z) { // mesh.cpp:94 max3_coordinate
     return (x > y) ? ((x > z) ? 0 : 2) : ((y > z) ? 1 : 2);
 }
@@ -359,6 +379,7 @@ int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene
     if (!need(d.n_csg == 0 || (d.csg && d.csg_children), "csg")) return PVX_EINVAL;
     if (!need(d.n_triangles == 0 || d.triangle_vertices, "triangles")) return PVX_EINVAL;
     if (!need(d.n_cones == 0 || (d.cone_apex && d.cone_base && d.cone_radius), "cones")) return PVX_EINVAL;
+    if (!need(d.n_discs == 0 || (d.disc_center && d.disc_normal && d.disc_radius), "discs")) return PVX_EINVAL;
 
     auto check_ref = [&](uint32_t type, uint32_t index) -> bool {
         uint32_t lim = 0;
@@ -372,6 +393,7 @@ int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene
         case PVX_TRIANGLE:
         case PVX_SMOOTH_TRIANGLE: lim = d.n_triangles; break;
         case PVX_CONE: lim = d.n_cones; break;
+        case PVX_DISC: lim = d.n_discs; break;
         default:
             err = "unknown object type " + std::to_string(type);
             return false;
@@ -461,6 +483,32 @@ int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene
             xf_compose(x, u);
         }
         out.cones[i].xf = (int32_t)out.xforms.size();
+        out.xforms.push_back(x);
+    }
+
+    // discs: placement, then the object's transform (Disc::Transform carries
+    // the normal with MTransNormal and normalises it)
+    out.discs.resize(d.n_discs);
+    for (uint32_t i = 0; i < d.n_discs; i++) {
+        Xform x;
+        DiscRec &c = out.discs[i];
+        make_disc(d.disc_center + 3 * (size_t)i, d.disc_normal + 3 * (size_t)i, d.disc_radius + 2 * (size_t)i, c, x);
+        const int32_t t = d.disc_transform ? d.disc_transform[i] : -1;
+        if (t >= (int32_t)d.n_transforms) {
+            err = "disc transform index out of range";
+            return PVX_EINVAL;
+        }
+        if (t >= 0) {
+            Xform u;
+            std::memcpy(u.m, d.transforms + 32 * (size_t)t, 16 * sizeof(double));
+            std::memcpy(u.inv, d.transforms + 32 * (size_t)t + 16, 16 * sizeof(double));
+            double nn[3];
+            xf_normal(u, c.n, nn);
+            const double l = std::sqrt(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]);
+            for (int k = 0; k < 3; k++) c.n[k] = l != 0 ? nn[k] / l : nn[k];
+            xf_compose(x, u);
+        }
+        c.xf = (int32_t)out.xforms.size();
         out.xforms.push_back(x);
     }
 
@@ -592,6 +640,13 @@ int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene
             double lo[3] = {-1.0, -1.0, c.dist}, hi[3] = {1.0, 1.0, 1.0};
             return transformed_box(lo, hi, out.xforms[c.xf].m);
         }
+        case PVX_DISC: { // Disc::Compute_BBox (disc.cpp): the thin box also bounds the
+                         // disc as a CSG intersection operand, as in the reference
+            const DiscRec &c = out.discs[index];
+            const double rad = std::sqrt(c.or2);
+            double lo[3] = {-rad, -rad, -kSmallTolerance}, hi[3] = {rad, rad, kSmallTolerance};
+            return transformed_box(lo, hi, out.xforms[c.xf].m);
+        }
         default:
             return DBox::unbounded();
         }
@@ -669,8 +724,12 @@ int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene
         }
         case PVX_BOX:
         case PVX_TORUS:
-        case PVX_CONE: {
-            pr.kind_axis = ob.type == PVX_BOX ? kPrimBox : ob.type == PVX_TORUS ? kPrimTorus : kPrimCone;
+        case PVX_CONE:
+        case PVX_DISC: {
+            pr.kind_axis = ob.type == PVX_BOX     ? kPrimBox
+                           : ob.type == PVX_TORUS ? kPrimTorus
+                           : ob.type == PVX_CONE  ? kPrimCone
+                                                  : kPrimDisc;
             pr.ref.index = ob.index;
             to_fbox(shape_bnd(ob.type, ob.index, 0).S, lo, hi);
             add_prim(pr, lo, hi);

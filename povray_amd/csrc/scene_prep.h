@@ -30,6 +30,7 @@ struct HostScene {
     std::vector<TriangleRec> triangles;
     std::vector<TorusRec> tori;
     std::vector<ConeRec> cones;
+    std::vector<DiscRec> discs;
     std::vector<Xform> xforms;
     std::vector<double> sph; // 4 per sphere: centre, radius
     std::vector<CsgNode> csg;

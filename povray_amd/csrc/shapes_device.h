@@ -742,6 +742,37 @@ __device__ inline bool cone_inside(const ConeRec &c, const Xform *xf, D3 p, bool
 }
 
 // ---------------------------------------------------------------------------
+// Disc (core/shape/disc.cpp)
+// ---------------------------------------------------------------------------
+// Disc::Intersect: the plane z = 0 of disc space, squared radius within
+// [iradius2, oradius2], depth in (DEPTH_TOLERANCE = 1e-6, MAX_DISTANCE).
+__device__ inline bool disc_hit(const DiscRec &c, const Xform *xf, D3 o, D3 d, double &depth) {
+    const Xform &x = xf[c.xf];
+    const D3 P = xf_point(x.inv, o);
+    D3 D = xf_dir(x.inv, d);
+    const double len = sqrt(D.x * D.x + D.y * D.y + D.z * D.z);
+    D = D3{D.x / len, D.y / len, D.z / len};
+    if (fabs(D.z) > kEpsilon) {
+        const double t = -P.z / D.z;
+        if (t >= 0.0) {
+            const double u = P.x + t * D.x, v = P.y + t * D.y;
+            const double r2 = u * u + v * v;
+            if (r2 >= c.ir2 && r2 <= c.or2) {
+                depth = t / len;
+                return depth > 1.0e-6 && depth < kMaxDistance;
+            }
+        }
+    }
+    return false;
+}
+
+// Disc::Inside: the half-space z < 0 of disc space.
+__device__ inline bool disc_inside(const DiscRec &c, const Xform *xf, D3 p, bool inv) {
+    const D3 q = xf_point(xf[c.xf].inv, p);
+    return q.z >= 0.0 ? inv : !inv;
+}
+
+// ---------------------------------------------------------------------------
 // CSG (core/shape/csg.cpp): point membership with bounded nesting.
 // ---------------------------------------------------------------------------
 template <int D>
@@ -762,6 +793,8 @@ __device__ inline bool shape_inside(const SceneView &sv, uint32_t type, uint32_t
         return torus_inside(sv.tori[index], sv.xforms, p, inv);
     case 9:
         return cone_inside(sv.cones[index], sv.xforms, p, inv);
+    case 10:
+        return disc_inside(sv.discs[index], sv.xforms, p, inv);
     case 4: { // plane: Plane::Inside (plane.cpp:192-215)
         const PlaneRec &pl = sv.planes[index];
         double t = p.x * pl.n[0] + p.y * pl.n[1] + p.z * pl.n[2];
@@ -794,7 +827,7 @@ __device__ bool csg_inside(const SceneView &sv, uint32_t node, D3 p) {
     }
 }
 
-// Hits of one leaf shape (sphere/box/torus/cone/triangle) in push order.
+// Hits of one leaf shape (sphere/box/torus/cone/disc/triangle) in push order.
 __device__ inline int leaf_hits(const SceneView &sv, uint32_t type, uint32_t index, D3 o, D3 d,
                                 double *out) {
     int side[2];
@@ -811,6 +844,8 @@ __device__ inline int leaf_hits(const SceneView &sv, uint32_t type, uint32_t ind
         int part[4];
         return cone_hits(sv.cones[index], sv.xforms, o, d, out, part);
     }
+    case 10:
+        return disc_hit(sv.discs[index], sv.xforms, o, d, out[0]) ? 1 : 0;
     case 7:
     case 8: { // stand-alone triangle (not a CSG child; All_Intersections)
         double dep;

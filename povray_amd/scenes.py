@@ -98,6 +98,10 @@ class Scene:
     cone_radius: np.ndarray = None           # (n, 2) float64: first, second radius
     cone_flags: np.ndarray = None            # (n,) uint8: PVX_CONE_OPEN | PVX_CONE_CYLINDER
     cone_transform: np.ndarray = None        # (n,) int32: transform index or -1
+    disc_center: np.ndarray = None           # (n, 3) float64
+    disc_normal: np.ndarray = None           # (n, 3) float64
+    disc_radius: np.ndarray = None           # (n, 2) float64: radius, hole radius
+    disc_transform: np.ndarray = None        # (n,) int32: transform index or -1
     camera: dict = None
     lights: list = field(default_factory=list)
     materials: dict = field(default_factory=dict)  # per top-level object arrays
@@ -176,6 +180,12 @@ class Scene:
                 d.cone_flags, _ = arr(self.cone_flags, np.uint8, C.c_uint8)
             if self.cone_transform is not None:
                 d.cone_transform, _ = arr(self.cone_transform, np.int32, C.c_int32)
+        if self.disc_center is not None and len(self.disc_center):
+            d.disc_center, d.n_discs = arr(np.asarray(self.disc_center).reshape(-1, 3), np.float64, C.c_double)
+            d.disc_normal, _ = arr(np.asarray(self.disc_normal).reshape(-1, 3), np.float64, C.c_double)
+            d.disc_radius, _ = arr(np.asarray(self.disc_radius).reshape(-1, 2), np.float64, C.c_double)
+            if self.disc_transform is not None:
+                d.disc_transform, _ = arr(self.disc_transform, np.int32, C.c_int32)
         if self.csg is not None and len(self.csg):
             c = np.ascontiguousarray(self.csg, np.uint32).reshape(-1, 4)
             ch = np.ascontiguousarray(self.csg_children, np.uint32).reshape(-1, 4)
@@ -760,15 +770,18 @@ def triangle_scene(seed: int = 0xC0FFEE + 5, n: int = 400, n_spheres: int = 40, 
 # --------------------------------------------------------------------------
 # cones and cylinders
 # --------------------------------------------------------------------------
-def cone_scene(seed: int = 0xC0FFEE + 6, n: int = 300, n_csg: int = 40, width=320, height=240) -> Scene:
+def cone_scene(seed: int = 0xC0FFEE + 6, n: int = 300, n_csg: int = 40, n_discs: int = 60, width=320,
+               height=240) -> Scene:
     """`n` seeded cones and cylinders in a 12-unit cube: a third `cylinder`
     statements, the rest cones with radii U[0, 0.8] (a pointed end at radius
     0 for one in eight; one in eight with equal radii, which the reference
     turns into a cylinder; either end may be the bigger); one in four `open`;
     one in three with a random rotation composed after the shape's own
     placement.  `n_csg` CSG objects: a box minus a cylinder (difference) or
-    a cone intersected with a sphere.  A ground plane, one point light; 10%
-    reflective, 10% glass."""
+    a cone intersected with a sphere, or (one in four) a sphere minus the
+    half-space below a disc (the disc inverted).  `n_discs` discs (radius U[0.3, 1.2], a hole in one
+    of three, a rotation composed after one in three).  A ground plane, one
+    point light; 10% reflective, 10% glass."""
     r = Rng(seed)
     m = n + 2 * n_csg
     apex = r.uniform(3 * m, -6.0, 6.0).reshape(m, 3)
@@ -786,6 +799,18 @@ def cone_scene(seed: int = 0xC0FFEE + 6, n: int = 300, n_csg: int = 40, width=32
         if i % 3 == 2:
             xforms.append(_xf_row(rotation_transform(r.uniform(3, 0.0, 360.0))))
             tr[i] = len(xforms) - 1
+    # discs: the free-standing ones first, CSG ones appended below
+    dc = list(r.uniform(3 * n_discs, -6.0, 6.0).reshape(n_discs, 3))
+    dn = list(r.uniform(3 * n_discs, -1.0, 1.0).reshape(n_discs, 3))
+    drad = r.uniform(n_discs, 0.3, 1.2)
+    dr = [(float(drad[i]), float(drad[i]) * 0.5 if i % 3 == 1 else 0.0) for i in range(n_discs)]
+    dt = []
+    for i in range(n_discs):
+        if i % 3 == 2:
+            xforms.append(_xf_row(rotation_transform(r.uniform(3, 0.0, 360.0))))
+            dt.append(len(xforms) - 1)
+        else:
+            dt.append(-1)
     # CSG pieces
     box_c1, box_c2, sph_c, sph_r = [], [], [], []
     csg_nodes, csg_children = [], []
@@ -793,7 +818,16 @@ def cone_scene(seed: int = 0xC0FFEE + 6, n: int = 300, n_csg: int = 40, width=32
     for j in range(n_csg):
         ca, cb = n + 2 * j, n + 2 * j + 1
         c = r.uniform(3, -5.0, 5.0)
-        if j % 2 == 0:  # box minus a cylinder through it
+        if j % 4 == 3:  # a sphere minus the half-space below a disc
+            sph_c.append(c)
+            sph_r.append(1.0)
+            dc.append(c + r.uniform(3, -0.3, 0.3))
+            dn.append(r.uniform(3, -1.0, 1.0))
+            dr.append((1.5, 0.0))
+            dt.append(-1)
+            kids = [[A.PVX_SPHERE, len(sph_c) - 1, 0, 0], [A.PVX_DISC, len(dc) - 1, A.PVX_FLAG_INVERTED, 0]]
+            op = A.PVX_CSG_INTERSECTION
+        elif j % 2 == 0:  # box minus a cylinder through it
             e = r.uniform(3, 0.6, 1.4)
             box_c1.append(c - e)
             box_c2.append(c + e)
@@ -818,12 +852,14 @@ def cone_scene(seed: int = 0xC0FFEE + 6, n: int = 300, n_csg: int = 40, width=32
         csg_children.extend(kids)
         csg_nodes.append([op, first, 2, 0])
         roots.append(len(csg_nodes) - 1)
-    n_obj = n + n_csg + 1
+    n_obj = n + n_csg + n_discs + 1
     objects = np.zeros((n_obj, 4), np.uint32)
     objects[:n, 0] = A.PVX_CONE
     objects[:n, 1] = np.arange(n)
     objects[n:n + n_csg, 0] = A.PVX_CSG
     objects[n:n + n_csg, 1] = roots
+    objects[n + n_csg:n + n_csg + n_discs, 0] = A.PVX_DISC
+    objects[n + n_csg:n + n_csg + n_discs, 1] = np.arange(n_discs)
     objects[-1, 0] = A.PVX_PLANE
     alb = r.uniform(3 * n_obj, 0.2, 0.95).reshape(n_obj, 3)
     u = r.uniform(n_obj)
@@ -832,6 +868,8 @@ def cone_scene(seed: int = 0xC0FFEE + 6, n: int = 300, n_csg: int = 40, width=32
     return Scene(
         name=f"cones{n}", objects=objects,
         cone_apex=apex, cone_base=base, cone_radius=rad, cone_flags=flags, cone_transform=tr,
+        disc_center=np.array(dc).reshape(-1, 3), disc_normal=np.array(dn).reshape(-1, 3),
+        disc_radius=np.array(dr).reshape(-1, 2), disc_transform=np.array(dt, np.int32),
         transforms=np.array(xforms).reshape(-1, 32),
         box_corner1=np.array(box_c1).reshape(-1, 3), box_corner2=np.array(box_c2).reshape(-1, 3),
         box_transform=np.full(len(box_c1), -1, np.int32),

@@ -1,4 +1,4 @@
-"""Cones and cylinders (core/shape/cone.cpp).
+"""Cones and cylinders (core/shape/cone.cpp) and discs (core/shape/disc.cpp).
 
 CPU: the oracle's cone (the shape's own placement from its two points and
 radii, the canonical-space intersection, Inside) is pinned against closed
@@ -18,7 +18,7 @@ from povray_amd import scenes
 
 
 def _single(apex, base, r1, r2, flags, transform=None):
-    s = scenes.cone_scene(n=1, n_csg=0, width=8, height=8)
+    s = scenes.cone_scene(n=1, n_csg=0, n_discs=0, width=8, height=8)
     s.cone_apex = np.array([apex], float)
     s.cone_base = np.array([base], float)
     s.cone_radius = np.array([[r1, r2]], float)
@@ -154,12 +154,81 @@ def test_oracle_cone_inside_and_transform():
 
 def test_cone_scene_generator_and_tree():
     s = scenes.cone_scene()
-    assert s.n_objects == 300 + 40 + 1
+    assert s.n_objects == 300 + 40 + 60 + 1
     o = OracleScene(s)
     rays = scenes.camera_rays_np(s.camera)
     h1, h0 = o.find_intersection(rays, method=1), o.find_intersection(rays, method=0)
     assert np.array_equal(h1["obj"], h0["obj"]) and np.array_equal(h1["t"], h0["t"])
     assert np.mean((h1["obj"] >= 0) & (h1["obj"] < 300)) > 0.1
+    assert np.mean((h1["obj"] >= 340) & (h1["obj"] < 400)) > 0.01  # discs are seen
+
+
+def _single_disc(center, normal, radius, hole, transform=None):
+    s = scenes.cone_scene(n=1, n_csg=0, n_discs=1, width=8, height=8)
+    s.disc_center = np.array([center], float)
+    s.disc_normal = np.array([normal], float)
+    s.disc_radius = np.array([[radius, hole]], float)
+    if transform is None:
+        s.disc_transform = np.array([-1], np.int32)
+        s.transforms = None
+        s.cone_transform = np.array([-1], np.int32)
+    else:
+        s.disc_transform = np.array([0], np.int32)
+        s.cone_transform = np.array([-1], np.int32)
+        s.transforms = np.asarray(transform).reshape(1, 32)
+    s.objects = s.objects[1:2]  # the disc
+    s.plane_normal = s.plane_distance = None
+    s.materials = {k: v[1:2] for k, v in s.materials.items()}
+    return s
+
+
+DISCS = [((0.2, -0.3, 0.5), (0.3, 0.8, -0.2), 1.1, 0.0),
+         ((0.2, -0.3, 0.5), (0.3, 0.8, -0.2), 1.1, 0.6),   # with a hole
+         ((0.0, 0.0, 0.0), (0.0, 0.0, -2.0), 0.9, 0.0),    # normal along -z
+         ((1.0, 2.0, 3.0), (1e-12, 0.0, 1.0), 0.7, 0.2)]   # within EPSILON of +z: snapped
+
+
+@pytest.mark.parametrize("case", range(len(DISCS)))
+def test_oracle_disc_closed_form(case):
+    c, nrm, rad, hole = DISCS[case]
+    s = _single_disc(c, nrm, rad, hole)
+    rays = _rays_at(c, np.add(c, 0.1), 3000, 20 + case)
+    d, _, cnt = OracleScene(s).all_intersections(rays, np.zeros(rays.shape[0], np.uint32), 4)
+    n = np.asarray(nrm) / np.linalg.norm(nrm)
+    o, dd = rays[:, :3], rays[:, 3:]
+    t = ((np.asarray(c) - o) @ n) / (dd @ n)
+    p = o + t[:, None] * dd
+    r2 = ((p - np.asarray(c)) ** 2).sum(1)
+    exp = (t > 1e-6) & (r2 >= hole * hole) & (r2 <= rad * rad)
+    edge = (np.abs(r2 - rad * rad) < 1e-7) | (np.abs(r2 - hole * hole) < 1e-7)
+    got = cnt == 1
+    assert (got == exp)[~edge].all()
+    assert exp.sum() > 200
+    assert np.allclose(d[got, 0], t[got], rtol=1e-9)
+    # Inside: the half-space behind the normal
+    pts = np.asarray(c) + np.random.default_rng(case).uniform(-2, 2, (5000, 3))
+    side = (pts - np.asarray(c)) @ n
+    ins = OracleScene(s).inside(pts, np.zeros(pts.shape[0], np.uint32))
+    far = np.abs(side) > 1e-9
+    assert (ins[far] == (side[far] < 0)).all()
+
+
+def test_oracle_disc_transform():
+    rot = scenes.rotation_transform((20.0, 40.0, 60.0))
+    xf = np.asarray(scenes._xf_row(rot))
+    m = xf[:16].reshape(4, 4)
+    c, nrm = np.array([0.3, 0.1, -0.4]), np.array([0.2, 1.0, 0.3])
+    rc = c @ m[:3, :3] + m[3, :3]
+    rn = nrm @ m[:3, :3]  # a rotation carries normals like points' directions
+    s1 = _single_disc(tuple(c), tuple(nrm), 1.0, 0.3, transform=xf)
+    s2 = _single_disc(tuple(rc), tuple(rn), 1.0, 0.3)
+    rays = _rays_at(tuple(rc), tuple(rc + 0.1), 2000, 31)
+    h1 = OracleScene(s1).find_intersection(rays, method=0)
+    h2 = OracleScene(s2).find_intersection(rays, method=0)
+    assert np.array_equal(h1["obj"], h2["obj"])
+    ok = h1["obj"] >= 0
+    assert ok.sum() > 200
+    assert np.allclose(h1["t"][ok], h2["t"][ok], rtol=1e-9)
 
 
 @pytest.mark.gpu
