@@ -54,6 +54,7 @@ struct ShadingState {
     uint32_t max_level = 5;
     float bailout = 1.0f / 255.0f;
     uint32_t samples_per_hit = 0; // shadow samples over all lights
+    bool all_opaque = false;      // no filter/transmit anywhere: light rays only test occlusion
     MatRec *d_mats = nullptr;
     LightRec *d_lights = nullptr;
 };

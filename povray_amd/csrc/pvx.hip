@@ -162,6 +162,10 @@ __device__ __forceinline__ void trace_closest_body(const SceneView &sv0, const S
     const uint32_t lane = threadIdx.x;
     const uint32_t gtid = blockIdx.x * kBlock + lane;
     int32_t *my_spill = spill + (size_t)gtid * kStackSpill;
+    // kRayOcclusion on a light-ray launch: stop at the first hit that the
+    // caller counts as blocking (closer than max depth - SHADOW_TOLERANCE)
+    const bool occl = (kind & kRayOcclusion) != 0;
+    kind &= 0xFFu;
     const bool shadow = (kind == PVX_RAY_SHADOW);
     if constexpr (kMulti) {
         if (clocks && (lane & 63u) == 0) atomicMin(&clocks[0], (unsigned long long)wall_clock64());
@@ -186,6 +190,7 @@ __device__ __forceinline__ void trace_closest_body(const SceneView &sv0, const S
             double md = max_depth[ri];
             if (md >= kEpsilon) bt = md; // trace.cpp:157-158
         }
+        const double occl_lim = bt - kShadowTolerance;
         int32_t bo = -1, bp = 0;
 
         // objects outside the BVH (planes, unbounded CSG)
@@ -218,7 +223,7 @@ __device__ __forceinline__ void trace_closest_body(const SceneView &sv0, const S
         rf.oiz = (float)o.z * rf.iz;
         float tb = (float)bt * 1.000001f;
 
-        int32_t node = kLevel == 0 ? sv.root4 : sv.root;
+        int32_t node = (occl && bo >= 0 && bt < occl_lim) ? kEmpty : kLevel == 0 ? sv.root4 : sv.root;
         int sp = 0;
         uint32_t cnt_nodes = 0, cnt_prims = 0;
         // Triangle / sphere scenes traverse the 4-wide tree; camera and
@@ -319,6 +324,10 @@ __device__ __forceinline__ void trace_closest_body(const SceneView &sv0, const S
                 }
             }
             tb = (float)bt * 1.000001f;
+            if (occl && bo >= 0 && bt < occl_lim) {
+                node = kEmpty;
+                return;
+            }
             PVX_POP();
         };
         if constexpr (kLevel == 0) {
@@ -443,6 +452,10 @@ __device__ __forceinline__ void trace_closest_body(const SceneView &sv0, const S
                         }
                     }
                     tb = (float)bt * 1.000001f;
+                    if (occl && bo >= 0 && bt < occl_lim) {
+                        node = kEmpty;
+                        continue;
+                    }
                 }
                 // pop
                 if (sp == 0) {
@@ -575,6 +588,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_closest_kd(SceneView sv, KdVie
         uint32_t node;
         float rentry, rexit;
     };
+    kind &= 0xFFu; // closest hits only (the occlusion bit is a hint)
     const bool shadow = (kind == PVX_RAY_SHADOW);
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
         const pvx_ray r = rays[i];
@@ -1094,7 +1108,7 @@ static int launch_variant(pvx_scene *s, const pvx_ray *rays, const double *max_d
     } else if (level == 1 || variant == PVX_TRAVERSAL_BINARY) {
         if (flags) PVX_LAUNCH(1, true); else PVX_LAUNCH(1, false);
     } else if (variant == PVX_TRAVERSAL_WIDE_WW ||
-               (variant == PVX_TRAVERSAL_DEFAULT && ray_kind != PVX_RAY_SHADOW)) {
+               (variant == PVX_TRAVERSAL_DEFAULT && (ray_kind & 0xFFu) != PVX_RAY_SHADOW)) {
         if (flags) PVX_LAUNCH_WW(0, true); else PVX_LAUNCH_WW(0, false);
     } else {
         if (flags) PVX_LAUNCH(0, true); else PVX_LAUNCH(0, false);
@@ -1113,7 +1127,7 @@ static int launch_closest(pvx_scene *s, const pvx_ray *rays, const double *max_d
     if (n == 0) return PVX_OK;
     if (!rays || !hits) return fail(PVX_EINVAL, "null ray or hit buffer");
     if (n > 0xFFFFFFFFull) return fail(PVX_EINVAL, "too many rays in one call");
-    const int cls = ray_kind == PVX_RAY_SHADOW ? 1 : 0; // the autotuner's choice for this ray class
+    const int cls = (ray_kind & 0xFFu) == PVX_RAY_SHADOW ? 1 : 0; // the autotuner's choice for this ray class
     const uint32_t grid = s->tune_grid[cls] ? s->tune_grid[cls] : s->grid;
     return launch_variant<kCount>(s, rays, max_depth, ray_kind, kinds, n, hits, nv, pt,
                                   static_cast<hipStream_t>(stream), s->tune_variant[cls], grid);
@@ -1137,7 +1151,7 @@ extern "C" int pvx_scene_set_traversal(pvx_scene *s, uint32_t ray_kind, uint32_t
         if (std::find(v.begin(), v.end(), variant) == v.end())
             return fail(PVX_EINVAL, "traversal variant " + std::to_string(variant) + " not available for this scene");
     }
-    const int cls = ray_kind == PVX_RAY_SHADOW ? 1 : 0;
+    const int cls = (ray_kind & 0xFFu) == PVX_RAY_SHADOW ? 1 : 0;
     s->tune_variant[cls] = variant;
     s->tune_grid[cls] = grid_blocks;
     return PVX_OK;
@@ -1197,7 +1211,7 @@ int pvx::autotune_internal(pvx_scene *s, const pvx_ray *rays, const double *max_
     if (e1) (void)hipEventDestroy(e1);
     (void)hipFree(hits);
     if (r) return r;
-    const int cls = ray_kind == PVX_RAY_SHADOW ? 1 : 0;
+    const int cls = (ray_kind & 0xFFu) == PVX_RAY_SHADOW ? 1 : 0;
     s->tune_variant[cls] = best_v;
     s->tune_grid[cls] = best_g;
     if (out) {
@@ -1222,6 +1236,7 @@ int pvx::trace_closest_internal(pvx_scene *s, const pvx_ray *rays, const double 
 
 extern "C" int pvx_trace_closest(pvx_scene *s, const pvx_ray *rays, const double *max_depth,
                                  uint32_t ray_kind, size_t n, pvx_hit *hits, void *stream) {
+    if (ray_kind & ~0xFFu) return fail(PVX_EINVAL, "unknown ray kind bits");
     return launch_closest<false>(s, rays, max_depth, ray_kind, nullptr, n, hits, nullptr, nullptr, stream);
 }
 
@@ -1230,6 +1245,7 @@ extern "C" int pvx_trace_closest_counted(pvx_scene *s, const pvx_ray *rays,
                                          pvx_hit *hits, uint32_t *node_visits,
                                          uint32_t *prim_tests, void *stream) {
     if (!node_visits || !prim_tests) return fail(PVX_EINVAL, "null counter buffer");
+    if (ray_kind & ~0xFFu) return fail(PVX_EINVAL, "unknown ray kind bits");
     return launch_closest<true>(s, rays, max_depth, ray_kind, nullptr, n, hits, node_visits, prim_tests,
                                 stream);
 }
@@ -1389,6 +1405,7 @@ static int launch_batch(pvx_batch *b, const pvx_ray *rays, const uint64_t *first
 extern "C" int pvx_batch_trace_closest(pvx_batch *b, const pvx_ray *rays, const uint64_t *ray_first,
                                        const double *max_depth, uint32_t ray_kind, pvx_hit *hits,
                                        uint64_t *scene_clock, void *stream) {
+    if (ray_kind & ~0xFFu) return fail(PVX_EINVAL, "unknown ray kind bits");
     return launch_batch<false>(b, rays, ray_first, max_depth, ray_kind, hits, scene_clock, nullptr, nullptr,
                                stream);
 }
@@ -1398,6 +1415,7 @@ extern "C" int pvx_batch_trace_closest_counted(pvx_batch *b, const pvx_ray *rays
                                                uint32_t *node_visits, uint32_t *prim_tests,
                                                uint64_t *scene_clock, void *stream) {
     if (!node_visits || !prim_tests) return fail(PVX_EINVAL, "null counter buffer");
+    if (ray_kind & ~0xFFu) return fail(PVX_EINVAL, "unknown ray kind bits");
     return launch_batch<true>(b, rays, ray_first, max_depth, ray_kind, hits, scene_clock, node_visits, prim_tests,
                               stream);
 }

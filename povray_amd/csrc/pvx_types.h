@@ -23,6 +23,10 @@ constexpr double kBoundHuge = 2.0e10;
 constexpr double kSmallTolerance = 1.0e-3;
 constexpr double kMaxDistance = 1.0e7;
 constexpr double kMinIsectDepth = 1.0e-4;
+constexpr double kShadowTolerance = 1.0e-3; // SHADOW_TOLERANCE (render/trace.cpp:81)
+// Internal ray-kind bit: a light-ray launch whose caller only needs to know
+// whether something blocks the light (every material opaque).
+constexpr uint32_t kRayOcclusion = 0x100u;
 
 enum PrimKind : uint32_t {
     kPrimTriangle = 0,

@@ -30,7 +30,6 @@ using namespace pvx;
 namespace {
 
 constexpr uint32_t kNone = 0xFFFFFFFFu;
-constexpr double kShadowTolerance = 1.0e-3; // trace.cpp:81
 constexpr int kMaxInteriors = 10;           // objects a ray can be inside at once
 constexpr int kMaxShadowSteps = 64;         // transparent layers followed per light ray
 
@@ -507,6 +506,7 @@ extern "C" int pvx_scene_set_shading(pvx_scene *s, const pvx_shading_desc *d) {
     if (!d->materials) return fail(PVX_EINVAL, "null material array");
     ShadingState &sh = s->shading;
     sh.mats.resize(d->n_materials);
+    bool all_opaque = true;
     for (uint32_t i = 0; i < d->n_materials; i++) {
         const pvx_material &a = d->materials[i];
         MatRec &m = sh.mats[i];
@@ -536,7 +536,9 @@ extern "C" int pvx_scene_set_shading(pvx_scene *s, const pvx_shading_desc *d) {
         m.ior = a.ior > 0.0f ? a.ior : 1.0f; // every object has an interior (parser.cpp:9226-9230)
         m.opaque = (a.filter == 0.0f && a.transmit == 0.0f) ? 1u : 0u;
         if (m.trans > 0.0f) m.reflective |= 2u;
+        if (!m.opaque) all_opaque = false;
     }
+    sh.all_opaque = all_opaque;
     sh.lights.resize(d->n_lights);
     sh.samples_per_hit = 0;
     for (uint32_t i = 0; i < d->n_lights; i++) {
@@ -672,6 +674,9 @@ static int render_frame(pvx_scene *s, const pvx_camera *cam, uint32_t S, const d
                                                    static_cast<uint8_t *>(b.kinds[0].p));
     int rc = PVX_OK;
     int cur = 0;
+    // with every material opaque a light ray only needs to know whether
+    // something blocks it: the traversal may stop at the first blocking hit
+    const uint32_t light_kind = PVX_RAY_SHADOW | (sh.all_opaque ? kRayOcclusion : 0u);
     // every bounce raises the trace level of all but continued rays; the cap
     // only guards against pathological chains of continued rays
     const uint32_t max_bounces = 4 * std::max<uint32_t>(1, sh.max_level) + 16;
@@ -717,7 +722,7 @@ static int render_frame(pvx_scene *s, const pvx_camera *cam, uint32_t S, const d
         if (tune && bounces == 0) {
             if (ns > 0) {
                 if ((rc = autotune_internal(s, static_cast<const pvx_ray *>(b.srays[0].p),
-                                            static_cast<const double *>(b.sdepth[0].p), ns, PVX_RAY_SHADOW, &tune[1],
+                                            static_cast<const double *>(b.sdepth[0].p), ns, light_kind, &tune[1],
                                             st)))
                     return rc;
             } else {
@@ -727,7 +732,7 @@ static int render_frame(pvx_scene *s, const pvx_camera *cam, uint32_t S, const d
         for (int step = 0; ns > 0 && step < kMaxShadowSteps; step++) {
             HIP_TRY(hipEventRecord(t0, st));
             rc = trace_closest_internal(s, static_cast<const pvx_ray *>(b.srays[sq].p),
-                                        static_cast<const double *>(b.sdepth[sq].p), PVX_RAY_SHADOW, ns,
+                                        static_cast<const double *>(b.sdepth[sq].p), light_kind, ns,
                                         static_cast<pvx_hit *>(b.shits.p), st);
             if (rc) return rc;
             HIP_TRY(hipEventRecord(t1, st));
