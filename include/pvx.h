@@ -53,7 +53,8 @@ enum {
     PVX_TRIANGLE = 7,        /* stand-alone triangle (triangle.cpp, Triangle)              */
     PVX_SMOOTH_TRIANGLE = 8, /* stand-alone smooth triangle (triangle.cpp, SmoothTriangle) */
     PVX_CONE = 9,            /* cone or cylinder (cone.cpp, Cone)                          */
-    PVX_DISC = 10            /* disc, optionally with a hole (disc.cpp, Disc)              */
+    PVX_DISC = 10,           /* disc, optionally with a hole (disc.cpp, Disc)              */
+    PVX_QUADRIC = 11         /* quadric surface (quadric.cpp, Quadric)                     */
 };
 
 /* pvx_scene_desc.cone_flags bits */
@@ -193,6 +194,15 @@ typedef struct pvx_scene_desc {
     const double *disc_normal;      /* 3n                                       */
     const double *disc_radius;      /* 2n: radius, hole radius (0: none)        */
     const int32_t *disc_transform;  /* n: index into transforms or -1; or NULL  */
+
+    /* quadrics as Parse_Quadric reads them: <A,B,C> (Square_Terms),
+     * <D,E,F> (Mixed_Terms), <G,H,I> (Terms), J (Constant), i.e. the
+     * surface A x^2 + B y^2 + C z^2 + D xy + E xz + F yz + G x + H y + I z + J
+     * = 0; the optional transform is applied to the coefficients
+     * (Quadric::Transform).  Quadrics are tested outside the BVH, like planes. */
+    uint32_t n_quadrics;
+    const double *quadric_coef;     /* 10n                                      */
+    const int32_t *quadric_transform; /* n: index into transforms or -1; or NULL */
 } pvx_scene_desc;
 
 /* ---- rays and hits ----------------------------------------------------- */

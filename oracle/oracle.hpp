@@ -189,6 +189,15 @@ bool disc_intersect(const Disc &c, const Ray &r, double *depth);
 bool disc_inside(const Disc &c, V3 p, bool inverted);
 FBox disc_bbox(const Disc &c);
 
+// Quadric (quadric.cpp): coefficients A..J after the object's transform.
+struct Quadric {
+    double c[10];
+};
+Quadric quadric_prepare(const double *coef, const Transform *user);
+int quadric_intersect(const Quadric &q, const Ray &r, double *depth); // up to 2 hits
+bool quadric_inside(const Quadric &q, V3 p, bool inverted);
+V3 quadric_normal(const Quadric &q, V3 p);
+
 // Cone / cylinder (cone.cpp): the shape's placement and flags after
 // Compute_Cone_Data / Compute_Cylinder_Data and the object's transform.
 struct Cone {
@@ -242,6 +251,7 @@ struct Scene {
     std::vector<uint8_t> tor_sturm;
     std::vector<Cone> cones;
     std::vector<Disc> discs;
+    std::vector<Quadric> quadrics;
     std::vector<Transform> trs;
     std::vector<Mesh> meshes;
     std::vector<pvx_csg_node> csg;

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cstring>
+#include <memory>
 #include <stdexcept>
 #include <thread>
 
@@ -57,6 +58,12 @@ FBox Scene::shape_bbox(uint32_t type, uint32_t index, uint32_t flags, bool *infi
         break;
     case PVX_DISC:
         b = disc_bbox(discs[index]);
+        break;
+    case PVX_QUADRIC: // unbounded (see quadric.cpp)
+        for (int k = 0; k < 3; k++) {
+            b.ll[k] = (float)(-kBoundHuge / 2);
+            b.size[k] = (float)kBoundHuge;
+        }
         break;
     case PVX_TRIANGLE:
     case PVX_SMOOTH_TRIANGLE:
@@ -182,6 +189,12 @@ bool Scene::all_intersections(uint32_t type, uint32_t index, uint32_t flags, con
         for (int k = 0; k < n; k++) st.push_back(Isect{dep[k], slot, part[k], slot});
         return n > 0;
     }
+    case PVX_QUADRIC: { // Quadric::All_Intersections (quadric.cpp:100-135), no clipped_by
+        double dep[2];
+        const int n = quadric_intersect(quadrics[index], r, dep);
+        for (int k = 0; k < n; k++) st.push_back(Isect{dep[k], slot, 0, slot});
+        return n > 0;
+    }
     case PVX_DISC: { // Disc::All_Intersections (disc.cpp), no clipped_by
         double dep;
         if (!disc_intersect(discs[index], r, &dep)) return false;
@@ -264,6 +277,8 @@ bool Scene::inside(uint32_t type, uint32_t index, uint32_t flags, V3 p) const {
         return cone_inside(cones[index], p, inv);
     case PVX_DISC:
         return disc_inside(discs[index], p, inv);
+    case PVX_QUADRIC:
+        return quadric_inside(quadrics[index], p, inv);
     case PVX_CSG: {
         // CSGUnion/CSGMerge::Inside (csg.cpp:405-424), CSGIntersection::Inside
         // (csg.cpp:440-454); the CSG's own INVERTED flag is carried by its
@@ -319,7 +334,8 @@ static bool precondition(uint32_t flags, uint32_t kind) {
 
 
 static Scene *scene_create(const pvx_scene_desc &d) {
-    Scene *s = new Scene();
+    std::unique_ptr<Scene> holder(new Scene()); // freed if preparation throws
+    Scene *s = holder.get();
     for (uint32_t i = 0; i < d.n_spheres; i++) {
         s->sph_c.push_back(V3(d.sphere_center[3 * i], d.sphere_center[3 * i + 1],
                               d.sphere_center[3 * i + 2]));
@@ -349,16 +365,26 @@ static Scene *scene_create(const pvx_scene_desc &d) {
         s->tor_tr.push_back(d.torus_transform ? d.torus_transform[i] : -1);
         s->tor_sturm.push_back(d.torus_sturm ? d.torus_sturm[i] : 0);
     }
+    auto check_tr = [&](int32_t t) {
+        if (t >= (int32_t)d.n_transforms) throw std::runtime_error("transform index out of range");
+    };
     for (uint32_t i = 0; i < d.n_cones; i++) {
         const int32_t t = d.cone_transform ? d.cone_transform[i] : -1;
+        check_tr(t);
         s->cones.push_back(cone_prepare(d.cone_apex + 3 * (size_t)i, d.cone_base + 3 * (size_t)i,
                                         d.cone_radius + 2 * (size_t)i, d.cone_flags ? d.cone_flags[i] : 0,
                                         t >= 0 ? &s->trs[t] : nullptr));
     }
     for (uint32_t i = 0; i < d.n_discs; i++) {
         const int32_t t = d.disc_transform ? d.disc_transform[i] : -1;
+        check_tr(t);
         s->discs.push_back(disc_prepare(d.disc_center + 3 * (size_t)i, d.disc_normal + 3 * (size_t)i,
                                         d.disc_radius + 2 * (size_t)i, t >= 0 ? &s->trs[t] : nullptr));
+    }
+    for (uint32_t i = 0; i < d.n_quadrics; i++) {
+        const int32_t t = d.quadric_transform ? d.quadric_transform[i] : -1;
+        check_tr(t);
+        s->quadrics.push_back(quadric_prepare(d.quadric_coef + 10 * (size_t)i, t >= 0 ? &s->trs[t] : nullptr));
     }
     if (d.n_triangles) s->tri_p.assign(d.triangle_vertices, d.triangle_vertices + 9 * (size_t)d.n_triangles);
     s->tris.resize(d.n_triangles);
@@ -393,7 +419,7 @@ static Scene *scene_create(const pvx_scene_desc &d) {
     // when objects >= Bounding_Threshold (3), else brute force
     // (boundingtask.cpp:168-210).
     s->tree.build(fb, fi, ib, ii);
-    return s;
+    return holder.release();
 }
 
 // Trace::FindIntersection (trace.cpp:285-344), method 0 or 1.

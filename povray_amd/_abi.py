@@ -12,6 +12,7 @@ PVX_SPHERE, PVX_BOX, PVX_TORUS, PVX_PLANE, PVX_MESH, PVX_CSG = 1, 2, 3, 4, 5, 6
 PVX_TRIANGLE, PVX_SMOOTH_TRIANGLE = 7, 8
 PVX_CONE = 9
 PVX_DISC = 10
+PVX_QUADRIC = 11
 PVX_CONE_OPEN, PVX_CONE_CYLINDER = 1, 2
 
 PVX_FLAG_NO_SHADOW = 1 << 0
@@ -68,6 +69,7 @@ class PvxSceneDesc(C.Structure):
         ("cone_flags", c_u8p), ("cone_transform", c_i32p),
         ("n_discs", C.c_uint32), ("disc_center", c_dp), ("disc_normal", c_dp), ("disc_radius", c_dp),
         ("disc_transform", c_i32p),
+        ("n_quadrics", C.c_uint32), ("quadric_coef", c_dp), ("quadric_transform", c_i32p),
     ]
 
 

@@ -49,6 +49,13 @@ __device__ inline D3 leaf_normal(const NormalParams &p, uint32_t type, uint32_t 
         if (b.xf >= 0) n = normalize3(mtrans_normal(sv.xforms[b.xf].inv, n));
         return n;
     }
+    case PVX_QUADRIC: { // Quadric::Normal (quadric.cpp): the gradient, (1,0,0) where it vanishes
+        const QuadricRec &q = sv.quadrics[index];
+        D3 n{2.0 * q.qa * ip.x + q.qb * ip.y + q.qc * ip.z + q.qd, q.qb * ip.x + 2.0 * q.qe * ip.y + q.qf * ip.z + q.qg,
+             q.qc * ip.x + q.qf * ip.y + 2.0 * q.qh * ip.z + q.qi};
+        const double len = sqrt(n.x * n.x + n.y * n.y + n.z * n.z);
+        return len == 0.0 ? D3{1.0, 0.0, 0.0} : D3{n.x / len, n.y / len, n.z / len};
+    }
     case PVX_DISC: { // Disc::Normal (disc.cpp): the stored normal
         const DiscRec &c = sv.discs[index];
         return D3{c.n[0], c.n[1], c.n[2]};

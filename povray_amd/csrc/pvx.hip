@@ -112,6 +112,21 @@ __device__ inline bool csg_closest(const SceneView &sv, uint32_t root, D3 o, D3 
     return found;
 }
 
+// Closest valid hit of a quadric (Find_Intersection over its IStack).
+__device__ inline bool quadric_closest(const QuadricRec &q, D3 o, D3 d, bool shadow, double &dep_out) {
+    double hs[2];
+    const int nh = quadric_hits(q, o, d, hs);
+    bool found = false;
+    double best = 0.0;
+    for (int j = 0; j < nh; j++)
+        if (depth_ok(hs[j], shadow) && (!found || hs[j] < best)) {
+            best = hs[j];
+            found = true;
+        }
+    dep_out = best;
+    return found;
+}
+
 // Closest valid hit of a box / torus / stand-alone triangle object
 // (Find_Intersection over its IStack).
 __device__ inline bool simple_closest(const SceneView &sv, uint32_t kind, uint32_t index, D3 o,
@@ -203,7 +218,8 @@ __device__ __forceinline__ void trace_closest_body(const SceneView &sv0, const S
             if (inf.kind == kPrimPlane) {
                 ok = plane_hit(sv.planes[inf.index], o, d, dep) && depth_ok(dep, shadow);
             } else if (kLevel >= 2) {
-                ok = csg_closest(sv, inf.index, o, d, shadow, bt, dep, prim);
+                ok = inf.kind == kPrimQuadric ? quadric_closest(sv.quadrics[inf.index], o, d, shadow, dep)
+                                              : csg_closest(sv, inf.index, o, d, shadow, bt, dep, prim);
             } else {
                 ok = false;
             }
@@ -606,8 +622,9 @@ __global__ __launch_bounds__(kBlock) void k_trace_closest_kd(SceneView sv, KdVie
             if (kFlags && !precondition(sv.obj_flags[inf.obj], rk)) continue;
             double dep;
             int32_t prim = 0;
-            bool ok = inf.kind == kPrimPlane ? (plane_hit(sv.planes[inf.index], o, d, dep) && depth_ok(dep, shadow))
-                                             : csg_closest(sv, inf.index, o, d, shadow, bt, dep, prim);
+            bool ok = inf.kind == kPrimPlane     ? (plane_hit(sv.planes[inf.index], o, d, dep) && depth_ok(dep, shadow))
+                      : inf.kind == kPrimQuadric ? quadric_closest(sv.quadrics[inf.index], o, d, shadow, dep)
+                                                 : csg_closest(sv, inf.index, o, d, shadow, bt, dep, prim);
             if (ok && better(dep, inf.obj, prim, bt, bo, bp)) {
                 bt = dep;
                 bo = inf.obj;
@@ -978,6 +995,7 @@ extern "C" int pvx_scene_create(const pvx_scene_desc *desc, const pvx_build_opts
     UP(h.tori, tori);
     UP(h.cones, cones);
     UP(h.discs, discs);
+    UP(h.quadrics, quadrics);
     UP(h.xforms, xforms);
     UP(h.sph, sph);
     UP(h.csg, csg);

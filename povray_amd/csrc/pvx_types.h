@@ -38,6 +38,7 @@ enum PrimKind : uint32_t {
     kPrimTriangle64 = 6, // stand-alone triangle, by reference (FP64 record)
     kPrimCone = 7,       // cone / cylinder, by reference
     kPrimDisc = 8,       // disc, by reference
+    kPrimQuadric = 9,    // quadric (outside the BVH)
 };
 
 struct alignas(16) Node {
@@ -164,6 +165,13 @@ struct DiscRec {
     uint32_t pad;
 };
 
+// Quadric (core/shape/quadric.cpp) in world space after its transforms:
+// QA..QJ of quadric.cpp:80-92 (QA/QE/QH square, QB/QC/QF mixed xy/xz/yz,
+// QD/QG/QI linear, QJ constant).
+struct QuadricRec {
+    double qa, qe, qh, qb, qc, qf, qd, qg, qi, qj;
+};
+
 struct PlaneRec {
     double n[3];
     double d;
@@ -219,7 +227,7 @@ struct LightRec {
 
 // An object tested outside the BVH.
 struct InfRec {
-    uint32_t kind; // kPrimPlane / kPrimCsg
+    uint32_t kind; // kPrimPlane / kPrimCsg / kPrimQuadric
     uint32_t index;
     int32_t obj;
     uint32_t pad;
@@ -236,6 +244,7 @@ struct SceneView {
     const TorusRec *tori;
     const ConeRec *cones;
     const DiscRec *discs;
+    const QuadricRec *quadrics;
     const Xform *xforms;
     const double *sph; // sphere array (cx, cy, cz, r) per sphere, for CSG leaves
     const CsgNode *csg;

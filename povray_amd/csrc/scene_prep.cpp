@@ -380,6 +380,7 @@ int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene
     if (!need(d.n_triangles == 0 || d.triangle_vertices, "triangles")) return PVX_EINVAL;
     if (!need(d.n_cones == 0 || (d.cone_apex && d.cone_base && d.cone_radius), "cones")) return PVX_EINVAL;
     if (!need(d.n_discs == 0 || (d.disc_center && d.disc_normal && d.disc_radius), "discs")) return PVX_EINVAL;
+    if (!need(d.n_quadrics == 0 || d.quadric_coef, "quadrics")) return PVX_EINVAL;
 
     auto check_ref = [&](uint32_t type, uint32_t index) -> bool {
         uint32_t lim = 0;
@@ -394,6 +395,7 @@ int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene
         case PVX_SMOOTH_TRIANGLE: lim = d.n_triangles; break;
         case PVX_CONE: lim = d.n_cones; break;
         case PVX_DISC: lim = d.n_discs; break;
+        case PVX_QUADRIC: lim = d.n_quadrics; break;
         default:
             err = "unknown object type " + std::to_string(type);
             return false;
@@ -510,6 +512,35 @@ int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene
         }
         c.xf = (int32_t)out.xforms.size();
         out.xforms.push_back(x);
+    }
+
+    // quadrics: Quadric::Transform (quadric.cpp) on the coefficient matrix
+    // (Quadric_To_Matrix; M = inverse * M (MTimesB); M = M * inverse^T
+    // (MTimesA); Matrix_To_Quadric)
+    out.quadrics.resize(d.n_quadrics);
+    for (uint32_t i = 0; i < d.n_quadrics; i++) {
+        const double *c = d.quadric_coef + 10 * (size_t)i;
+        QuadricRec &q = out.quadrics[i];
+        q = QuadricRec{c[0], c[1], c[2], c[3], c[4], c[5], c[6], c[7], c[8], c[9]};
+        const int32_t t = d.quadric_transform ? d.quadric_transform[i] : -1;
+        if (t >= (int32_t)d.n_transforms) {
+            err = "quadric transform index out of range";
+            return PVX_EINVAL;
+        }
+        if (t < 0) continue;
+        const double *inv = d.transforms + 32 * (size_t)t + 16;
+        double M[16] = {q.qa, q.qb, q.qc, q.qd, 0.0, q.qe, q.qf, q.qg, 0.0, 0.0, q.qh, q.qi, 0.0, 0.0, 0.0, q.qj};
+        double R[16];
+        for (int j = 0; j < 4; j++) // MTimesB(inverse, M): column j of inverse * M
+            for (int r = 0; r < 4; r++)
+                R[4 * r + j] = inv[4 * r] * M[j] + inv[4 * r + 1] * M[4 + j] + inv[4 * r + 2] * M[8 + j] +
+                               inv[4 * r + 3] * M[12 + j];
+        for (int r = 0; r < 4; r++) // MTimesA(R, inverse^T): row r of R * inverse^T
+            for (int j = 0; j < 4; j++)
+                M[4 * r + j] = R[4 * r] * inv[4 * j] + R[4 * r + 1] * inv[4 * j + 1] + R[4 * r + 2] * inv[4 * j + 2] +
+                               R[4 * r + 3] * inv[4 * j + 3];
+        q = QuadricRec{M[0], M[5], M[10], M[1] + M[4], M[2] + M[8], M[6] + M[9], M[3] + M[12], M[7] + M[13],
+                       M[11] + M[14], M[15]};
     }
 
     // stand-alone triangles, flat or smooth as their objects say
@@ -755,6 +786,10 @@ int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene
         }
         case PVX_PLANE:
             out.inf.push_back(InfRec{kPrimPlane, ob.index, (int32_t)o, 0});
+            break;
+        case PVX_QUADRIC: // unbounded here: Compute_BBox's box only culls (quadric.cpp:764-1000)
+            out.inf.push_back(InfRec{kPrimQuadric, ob.index, (int32_t)o, 0});
+            out.has_complex = 2;
             break;
         case PVX_TRIANGLE:
         case PVX_SMOOTH_TRIANGLE: {

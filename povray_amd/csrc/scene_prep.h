@@ -31,6 +31,7 @@ struct HostScene {
     std::vector<TorusRec> tori;
     std::vector<ConeRec> cones;
     std::vector<DiscRec> discs;
+    std::vector<QuadricRec> quadrics;
     std::vector<Xform> xforms;
     std::vector<double> sph; // 4 per sphere: centre, radius
     std::vector<CsgNode> csg;

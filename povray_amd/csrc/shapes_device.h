@@ -773,6 +773,44 @@ __device__ inline bool disc_inside(const DiscRec &c, const Xform *xf, D3 p, bool
 }
 
 // ---------------------------------------------------------------------------
+// Quadric (core/shape/quadric.cpp)
+// ---------------------------------------------------------------------------
+// Quadric::Intersect + All_Intersections (quadric.cpp:100-200): the world
+// ray in the quadratic a t^2 + 2 b t + c; roots kept in (1e-6, MAX_DISTANCE).
+__device__ inline int quadric_hits(const QuadricRec &q, D3 o, D3 d, double *out) {
+    const double a = d.x * (q.qa * d.x + q.qb * d.y + q.qc * d.z) + d.y * (q.qe * d.y + q.qf * d.z) + d.z * q.qh * d.z;
+    const double b = d.x * (q.qa * o.x + 0.5 * (q.qb * o.y + q.qc * o.z + q.qd)) +
+                     d.y * (q.qe * o.y + 0.5 * (q.qb * o.x + q.qf * o.z + q.qg)) +
+                     d.z * (q.qh * o.z + 0.5 * (q.qc * o.x + q.qf * o.y + q.qi));
+    const double c = o.x * (q.qa * o.x + q.qb * o.y + q.qc * o.z + q.qd) + o.y * (q.qe * o.y + q.qf * o.z + q.qg) +
+                     o.z * (q.qh * o.z + q.qi) + q.qj;
+    double t1, t2;
+    if (a != 0.0) {
+        double dd = b * b - a * c;
+        if (dd <= 0.0) return 0;
+        dd = sqrt(dd);
+        t1 = (-b + dd) / a;
+        t2 = (-b - dd) / a;
+    } else {
+        if (b == 0.0) return 0;
+        t1 = -0.5 * c / b;
+        t2 = kMaxDistance;
+    }
+    int n = 0;
+    if (t1 > 1.0e-6 && t1 < kMaxDistance) out[n++] = t1;
+    if (t2 > 1.0e-6 && t2 < kMaxDistance) out[n++] = t2;
+    return n;
+}
+
+// Quadric::Inside (quadric.cpp): q(p) <= 0.  Invert negates every
+// coefficient (Quadric::Invert), so an inverted quadric holds q(p) >= 0.
+__device__ inline bool quadric_inside(const QuadricRec &q, D3 p, bool inv) {
+    const double v = p.x * (q.qa * p.x + q.qb * p.y + q.qd) + p.y * (q.qe * p.y + q.qf * p.z + q.qg) +
+                     p.z * (q.qh * p.z + q.qc * p.x + q.qi) + q.qj;
+    return inv ? (-v <= 0.0) : (v <= 0.0);
+}
+
+// ---------------------------------------------------------------------------
 // CSG (core/shape/csg.cpp): point membership with bounded nesting.
 // ---------------------------------------------------------------------------
 template <int D>
@@ -795,6 +833,8 @@ __device__ inline bool shape_inside(const SceneView &sv, uint32_t type, uint32_t
         return cone_inside(sv.cones[index], sv.xforms, p, inv);
     case 10:
         return disc_inside(sv.discs[index], sv.xforms, p, inv);
+    case 11:
+        return quadric_inside(sv.quadrics[index], p, inv);
     case 4: { // plane: Plane::Inside (plane.cpp:192-215)
         const PlaneRec &pl = sv.planes[index];
         double t = p.x * pl.n[0] + p.y * pl.n[1] + p.z * pl.n[2];
@@ -827,7 +867,9 @@ __device__ bool csg_inside(const SceneView &sv, uint32_t node, D3 p) {
     }
 }
 
-// Hits of one leaf shape (sphere/box/torus/cone/disc/triangle) in push order.
+// Hits of one leaf shape (sphere/box/torus/cone/disc/quadric/triangle) in
+// push order (an inverted quadric pushes its two roots the other way round;
+// the set is the same).
 __device__ inline int leaf_hits(const SceneView &sv, uint32_t type, uint32_t index, D3 o, D3 d,
                                 double *out) {
     int side[2];
@@ -846,6 +888,8 @@ __device__ inline int leaf_hits(const SceneView &sv, uint32_t type, uint32_t ind
     }
     case 10:
         return disc_hit(sv.discs[index], sv.xforms, o, d, out[0]) ? 1 : 0;
+    case 11:
+        return quadric_hits(sv.quadrics[index], o, d, out);
     case 7:
     case 8: { // stand-alone triangle (not a CSG child; All_Intersections)
         double dep;

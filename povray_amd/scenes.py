@@ -102,6 +102,8 @@ class Scene:
     disc_normal: np.ndarray = None           # (n, 3) float64
     disc_radius: np.ndarray = None           # (n, 2) float64: radius, hole radius
     disc_transform: np.ndarray = None        # (n,) int32: transform index or -1
+    quadric_coef: np.ndarray = None          # (n, 10) float64: A..J as quadric { <A,B,C>, <D,E,F>, <G,H,I>, J }
+    quadric_transform: np.ndarray = None     # (n,) int32: transform index or -1
     camera: dict = None
     lights: list = field(default_factory=list)
     materials: dict = field(default_factory=dict)  # per top-level object arrays
@@ -186,6 +188,10 @@ class Scene:
             d.disc_radius, _ = arr(np.asarray(self.disc_radius).reshape(-1, 2), np.float64, C.c_double)
             if self.disc_transform is not None:
                 d.disc_transform, _ = arr(self.disc_transform, np.int32, C.c_int32)
+        if self.quadric_coef is not None and len(self.quadric_coef):
+            d.quadric_coef, d.n_quadrics = arr(np.asarray(self.quadric_coef).reshape(-1, 10), np.float64, C.c_double)
+            if self.quadric_transform is not None:
+                d.quadric_transform, _ = arr(self.quadric_transform, np.int32, C.c_int32)
         if self.csg is not None and len(self.csg):
             c = np.ascontiguousarray(self.csg, np.uint32).reshape(-1, 4)
             ch = np.ascontiguousarray(self.csg_children, np.uint32).reshape(-1, 4)
@@ -770,8 +776,8 @@ def triangle_scene(seed: int = 0xC0FFEE + 5, n: int = 400, n_spheres: int = 40, 
 # --------------------------------------------------------------------------
 # cones and cylinders
 # --------------------------------------------------------------------------
-def cone_scene(seed: int = 0xC0FFEE + 6, n: int = 300, n_csg: int = 40, n_discs: int = 60, width=320,
-               height=240) -> Scene:
+def cone_scene(seed: int = 0xC0FFEE + 6, n: int = 300, n_csg: int = 40, n_discs: int = 60, n_quadrics: int = 8,
+               width=320, height=240) -> Scene:
     """`n` seeded cones and cylinders in a 12-unit cube: a third `cylinder`
     statements, the rest cones with radii U[0, 0.8] (a pointed end at radius
     0 for one in eight; one in eight with equal radii, which the reference
@@ -780,8 +786,11 @@ def cone_scene(seed: int = 0xC0FFEE + 6, n: int = 300, n_csg: int = 40, n_discs:
     placement.  `n_csg` CSG objects: a box minus a cylinder (difference) or
     a cone intersected with a sphere, or (one in four) a sphere minus the
     half-space below a disc (the disc inverted).  `n_discs` discs (radius U[0.3, 1.2], a hole in one
-    of three, a rotation composed after one in three).  A ground plane, one
-    point light; 10% reflective, 10% glass."""
+    of three, a rotation composed after one in three).  `n_quadrics`
+    ellipsoid quadrics (semi-axes U[0.6, 1.5], rotated and moved by their
+    transform) and two CSG objects of unbounded quadrics: an infinite
+    cylinder x^2 + z^2 = 0.25 cut by a box, a paraboloid y = x^2 + z^2 cut by
+    a sphere.  A ground plane, one point light; 10% reflective, 10% glass."""
     r = Rng(seed)
     m = n + 2 * n_csg
     apex = r.uniform(3 * m, -6.0, 6.0).reshape(m, 3)
@@ -852,7 +861,35 @@ def cone_scene(seed: int = 0xC0FFEE + 6, n: int = 300, n_csg: int = 40, n_discs:
         csg_children.extend(kids)
         csg_nodes.append([op, first, 2, 0])
         roots.append(len(csg_nodes) - 1)
-    n_obj = n + n_csg + n_discs + 1
+    # quadrics: ellipsoids, then the two unbounded ones used in CSG
+    quad, qtr = [], []
+    for i in range(n_quadrics):
+        ax = r.uniform(3, 0.6, 1.5)
+        quad.append([1.0 / ax[0] ** 2, 1.0 / ax[1] ** 2, 1.0 / ax[2] ** 2, 0, 0, 0, 0, 0, 0, -1.0])
+        t = compose(rotation_transform(r.uniform(3, 0.0, 360.0)), translation_transform(r.uniform(3, -5.0, 5.0)))
+        xforms.append(_xf_row(t))
+        qtr.append(len(xforms) - 1)
+    qroots = []
+    if n_quadrics:
+        for k, (coef, c) in enumerate((([1.0, 0, 1.0, 0, 0, 0, 0, 0, 0, -0.25], np.array([-3.0, 1.0, -2.0])),
+                                      ([1.0, 0, 1.0, 0, 0, 0, 0, -1.0, 0, 0.0], np.array([3.0, -1.0, -2.0])))):
+            quad.append(coef)
+            xforms.append(_xf_row(translation_transform(c)))
+            qtr.append(len(xforms) - 1)
+            if k == 0:
+                box_c1.append(c - np.array([1.0, 1.5, 1.0]))
+                box_c2.append(c + np.array([1.0, 1.5, 1.0]))
+                other = [A.PVX_BOX, len(box_c1) - 1, 0, 0]
+            else:
+                sph_c.append(c + np.array([0.0, 1.0, 0.0]))
+                sph_r.append(1.2)
+                other = [A.PVX_SPHERE, len(sph_c) - 1, 0, 0]
+            first = len(csg_children)
+            csg_children.extend([[A.PVX_QUADRIC, len(quad) - 1, 0, 0], other])
+            csg_nodes.append([A.PVX_CSG_INTERSECTION, first, 2, 0])
+            qroots.append(len(csg_nodes) - 1)
+    n_q = n_quadrics + len(qroots)
+    n_obj = n + n_csg + n_discs + n_q + 1
     objects = np.zeros((n_obj, 4), np.uint32)
     objects[:n, 0] = A.PVX_CONE
     objects[:n, 1] = np.arange(n)
@@ -860,6 +897,11 @@ def cone_scene(seed: int = 0xC0FFEE + 6, n: int = 300, n_csg: int = 40, n_discs:
     objects[n:n + n_csg, 1] = roots
     objects[n + n_csg:n + n_csg + n_discs, 0] = A.PVX_DISC
     objects[n + n_csg:n + n_csg + n_discs, 1] = np.arange(n_discs)
+    q0 = n + n_csg + n_discs
+    objects[q0:q0 + n_quadrics, 0] = A.PVX_QUADRIC
+    objects[q0:q0 + n_quadrics, 1] = np.arange(n_quadrics)
+    objects[q0 + n_quadrics:q0 + n_q, 0] = A.PVX_CSG
+    objects[q0 + n_quadrics:q0 + n_q, 1] = qroots
     objects[-1, 0] = A.PVX_PLANE
     alb = r.uniform(3 * n_obj, 0.2, 0.95).reshape(n_obj, 3)
     u = r.uniform(n_obj)
@@ -870,6 +912,7 @@ def cone_scene(seed: int = 0xC0FFEE + 6, n: int = 300, n_csg: int = 40, n_discs:
         cone_apex=apex, cone_base=base, cone_radius=rad, cone_flags=flags, cone_transform=tr,
         disc_center=np.array(dc).reshape(-1, 3), disc_normal=np.array(dn).reshape(-1, 3),
         disc_radius=np.array(dr).reshape(-1, 2), disc_transform=np.array(dt, np.int32),
+        quadric_coef=np.array(quad, float).reshape(-1, 10), quadric_transform=np.array(qtr, np.int32),
         transforms=np.array(xforms).reshape(-1, 32),
         box_corner1=np.array(box_c1).reshape(-1, 3), box_corner2=np.array(box_c2).reshape(-1, 3),
         box_transform=np.full(len(box_c1), -1, np.int32),

@@ -1,4 +1,5 @@
-"""Cones and cylinders (core/shape/cone.cpp) and discs (core/shape/disc.cpp).
+"""Cones and cylinders (core/shape/cone.cpp), discs (core/shape/disc.cpp)
+and quadrics (core/shape/quadric.cpp).
 
 CPU: the oracle's cone (the shape's own placement from its two points and
 radii, the canonical-space intersection, Inside) is pinned against closed
@@ -18,7 +19,7 @@ from povray_amd import scenes
 
 
 def _single(apex, base, r1, r2, flags, transform=None):
-    s = scenes.cone_scene(n=1, n_csg=0, n_discs=0, width=8, height=8)
+    s = scenes.cone_scene(n=1, n_csg=0, n_discs=0, n_quadrics=0, width=8, height=8)
     s.cone_apex = np.array([apex], float)
     s.cone_base = np.array([base], float)
     s.cone_radius = np.array([[r1, r2]], float)
@@ -154,17 +155,18 @@ def test_oracle_cone_inside_and_transform():
 
 def test_cone_scene_generator_and_tree():
     s = scenes.cone_scene()
-    assert s.n_objects == 300 + 40 + 60 + 1
+    assert s.n_objects == 300 + 40 + 60 + 8 + 2 + 1
     o = OracleScene(s)
     rays = scenes.camera_rays_np(s.camera)
     h1, h0 = o.find_intersection(rays, method=1), o.find_intersection(rays, method=0)
     assert np.array_equal(h1["obj"], h0["obj"]) and np.array_equal(h1["t"], h0["t"])
     assert np.mean((h1["obj"] >= 0) & (h1["obj"] < 300)) > 0.1
     assert np.mean((h1["obj"] >= 340) & (h1["obj"] < 400)) > 0.01  # discs are seen
+    assert np.mean((h1["obj"] >= 400) & (h1["obj"] < 410)) > 0.003  # quadrics are seen
 
 
 def _single_disc(center, normal, radius, hole, transform=None):
-    s = scenes.cone_scene(n=1, n_csg=0, n_discs=1, width=8, height=8)
+    s = scenes.cone_scene(n=1, n_csg=0, n_discs=1, n_quadrics=0, width=8, height=8)
     s.disc_center = np.array([center], float)
     s.disc_normal = np.array([normal], float)
     s.disc_radius = np.array([[radius, hole]], float)
@@ -229,6 +231,47 @@ def test_oracle_disc_transform():
     ok = h1["obj"] >= 0
     assert ok.sum() > 200
     assert np.allclose(h1["t"][ok], h2["t"][ok], rtol=1e-9)
+
+
+def test_oracle_quadric_ellipsoid_closed_form():
+    """An ellipsoid quadric moved by a rotation + translation: the oracle's
+    transformed coefficients give the hits of the ray taken into the
+    ellipsoid's own frame and solved there."""
+    ax = np.array([0.7, 1.3, 0.9])
+    rot = scenes.rotation_transform((25.0, 70.0, 140.0))
+    t = scenes.compose(rot, scenes.translation_transform(np.array([0.4, -0.2, 0.3])))
+    xf = np.asarray(scenes._xf_row(t))
+    s = scenes.cone_scene(n=1, n_csg=0, n_discs=0, n_quadrics=0, width=8, height=8)
+    s.quadric_coef = np.array([[1 / ax[0] ** 2, 1 / ax[1] ** 2, 1 / ax[2] ** 2, 0, 0, 0, 0, 0, 0, -1.0]])
+    s.quadric_transform = np.array([0], np.int32)
+    s.transforms = xf.reshape(1, 32)
+    s.cone_transform = np.array([-1], np.int32)
+    s.objects = np.array([[A.PVX_QUADRIC, 0, 0, 0]], np.uint32)
+    s.plane_normal = s.plane_distance = None
+    s.materials = {k: v[:1] for k, v in s.materials.items()}
+    inv = xf[16:].reshape(4, 4)
+    rays = _rays_at((0.4, -0.2, 0.3), (0.4, -0.2, 0.3), 3000, 41)
+    d, _, cnt = OracleScene(s).all_intersections(rays, np.zeros(rays.shape[0], np.uint32), 4)
+    o = rays[:, :3] @ inv[:3, :3] + inv[3, :3]
+    dd = rays[:, 3:] @ inv[:3, :3]
+    A2 = ((dd / ax) ** 2).sum(1)
+    B2 = 2 * ((o * dd) / ax ** 2).sum(1)
+    C2 = ((o / ax) ** 2).sum(1) - 1
+    disc = B2 * B2 - 4 * A2 * C2
+    hit = disc > 1e-9
+    n_hit = 0
+    for i in np.nonzero(hit)[0]:
+        q = np.sqrt(disc[i])
+        ts = sorted(t for t in ((-B2[i] - q) / (2 * A2[i]), (-B2[i] + q) / (2 * A2[i])) if t > 1e-6)
+        assert cnt[i] == len(ts) and np.allclose(d[i, :cnt[i]], ts, rtol=1e-9), (i, d[i, :cnt[i]], ts)
+        n_hit += 1
+    assert (cnt[disc < -1e-9] == 0).all() and n_hit > 300
+    pts = np.random.default_rng(5).uniform(-2, 2, (5000, 3))
+    lp = pts @ inv[:3, :3] + inv[3, :3]
+    val = ((lp / ax) ** 2).sum(1) - 1
+    ins = OracleScene(s).inside(pts, np.zeros(pts.shape[0], np.uint32))
+    far = np.abs(val) > 1e-9
+    assert (ins[far] == (val[far] < 0)).all()
 
 
 @pytest.mark.gpu
