@@ -555,6 +555,17 @@ typedef struct pvx_radiosity_stats {
  * takes the depth-0 network's answer at the misses, except in tiles whose
  * held-out probe gathers disagree (probe_threshold), which fall back to the
  * plain gather. */
+/* Radiosity cache files (`save_file` / `load_file`, the .rca format of
+ * RadiosityCache::Load and ot_write_block, core/lighting/radiosity.cpp:909-1000,
+ * core/support/octree.cpp:1345-1370): one text line per record,
+ *   C<depth+1> x y z <normal hex> r g b <harmonic mean> <nearest> <to-nearest hex>.
+ * pvx_radiosity_save writes the records of the scene's last radiosity frame;
+ * pvx_radiosity_load reads records that seed every later frame on the scene
+ * (path NULL drops them); lines that are not 11-field "C" records are
+ * skipped, as the reference does.  Host paths; n_* may be NULL. */
+int pvx_radiosity_save(pvx_scene *scene, const char *path, uint64_t *n_written);
+int pvx_radiosity_load(pvx_scene *scene, const char *path, uint64_t *n_read);
+
 int pvx_radiosity_render(pvx_scene *scene, const pvx_camera *cam,
                          const pvx_radiosity_settings *settings, float *rgb,
                          pvx_radiosity_stats *stats, void *stream);

@@ -754,7 +754,7 @@ int RadiosityEngine::begin(const pvx_radiosity_settings &set, const pvx_camera &
     stream = st;
     for (int d = 0; d < kMaxRadDepth; d++) {
         RadCache &cc = caches[d];
-        cc.recs.clear();
+        cc.recs = preload[d]; // a loaded cache file seeds the frame (RadiosityCache::Load)
         cc.uploaded = cc.indexed = 0;
         cc.all.n = cc.fresh.n = 0;
         cc.reach = (float)(rs.error_bound + kWeightErrOffset) * std::pow(2.0f, (float)d);
@@ -1535,6 +1535,102 @@ int RadiosityEngine::pixel_pass(const std::vector<uint32_t> &px, bool final, flo
 
 } // namespace pvx
 
+static RadiosityEngine *engine_of(pvx_scene *s) {
+    if (!s->rad_engine) {
+        RadiosityEngine *e = new (std::nothrow) RadiosityEngine(s);
+        if (!e) return nullptr;
+        s->rad_engine = e;
+        s->rad_engine_free = [](void *p) { delete static_cast<RadiosityEngine *>(p); };
+    }
+    return static_cast<RadiosityEngine *>(s->rad_engine);
+}
+
+// The packed unit vector of a cache file: three 2-digit hex bytes,
+// (v + 1) / 2 * 254 rounded (octree.cpp:1345-1370); read back as
+// byte / 254 * 2 - 1, normalised (radiosity.cpp:970-985).
+static void pack_dir(const float *v, char *out) {
+    int b[3];
+    for (int k = 0; k < 3; k++) b[k] = (int)(((double)v[k] + 1.) * .5 * 254. + .499999);
+    std::snprintf(out, 8, "%02x%02x%02x", b[0] & 0xFF, b[1] & 0xFF, b[2] & 0xFF);
+}
+static bool unpack_dir(const char *s, float *v) {
+    unsigned tx, ty, tz;
+    if (std::sscanf(s, "%02x%02x%02x", &tx, &ty, &tz) != 3) return false;
+    double d[3] = {((double)tx * (1. / 254.)) * 2. - 1., ((double)ty * (1. / 254.)) * 2. - 1.,
+                   ((double)tz * (1. / 254.)) * 2. - 1.};
+    const double l = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    for (int k = 0; k < 3; k++) v[k] = (float)(l != 0 ? d[k] / l : d[k]);
+    return true;
+}
+
+// Writes the last frame's irradiance cache as a POV-Ray radiosity cache file
+// (.rca): one "C" line per record with 1-based bounce depth, point, packed
+// normal, illuminance, harmonic mean distance, nearest distance and packed
+// direction to the nearest surface (ot_write_block, octree.cpp:1345-1370).
+extern "C" int pvx_radiosity_save(pvx_scene *s, const char *path, uint64_t *n_written) {
+    if (!s || !path) return fail(PVX_EINVAL, "null argument");
+    if (!s->rad_engine) return fail(PVX_ESTATE, "no radiosity frame rendered on this scene");
+    const RadiosityEngine &eng = *static_cast<RadiosityEngine *>(s->rad_engine);
+    FILE *f = std::fopen(path, "w");
+    if (!f) return fail(PVX_EINVAL, std::string("cannot open ") + path + " for writing");
+    uint64_t n = 0;
+    for (int d = 0; d < kMaxRadDepth; d++)
+        for (const RadRec &r : eng.caches[d].recs) {
+            char ns[8], ts[8];
+            pack_dir(r.n, ns);
+            pack_dir(r.tn, ts);
+            std::fprintf(f, "C%d\t%g\t%g\t%g\t%s\t%.4f\t%.4f\t%.4f\t%g\t%g\t%s\n", d + 1, (double)r.p[0],
+                         (double)r.p[1], (double)r.p[2], ns, (double)r.E[0], (double)r.E[1], (double)r.E[2],
+                         (double)r.r, (double)r.nearest, ts);
+            n++;
+        }
+    const bool ok = std::fclose(f) == 0;
+    if (!ok) return fail(PVX_EINVAL, std::string("error writing ") + path);
+    if (n_written) *n_written = n;
+    return PVX_OK;
+}
+
+// Reads a radiosity cache file into the records every later frame on this
+// scene starts from (RadiosityCache::Load, radiosity.cpp:909-1000): "C" lines
+// of 11 fields are taken, other lines are skipped.  path NULL drops them.
+extern "C" int pvx_radiosity_load(pvx_scene *s, const char *path, uint64_t *n_read) {
+    if (!s) return fail(PVX_EINVAL, "null scene");
+    RadiosityEngine *e = engine_of(s);
+    if (!e) return fail(PVX_ENOMEM, "host allocation failed");
+    for (int d = 0; d < kMaxRadDepth; d++) e->preload[d].clear();
+    if (n_read) *n_read = 0;
+    if (!path) return PVX_OK;
+    FILE *f = std::fopen(path, "r");
+    if (!f) return fail(PVX_EINVAL, std::string("cannot open ") + path);
+    char line[256];
+    uint64_t n = 0;
+    while (std::fgets(line, sizeof line, f)) {
+        if (line[0] != 'C') continue;
+        int depth;
+        double p[3], hm, nearest;
+        float E[3];
+        char ns[32], ts[32];
+        if (std::sscanf(line, "C%d %lf %lf %lf %31s %f %f %f %lf %lf %31s", &depth, &p[0], &p[1], &p[2], ns, &E[0],
+                        &E[1], &E[2], &hm, &nearest, ts) != 11)
+            continue;
+        depth -= 1; // 1-based in the file
+        if (depth < 0 || depth >= kMaxRadDepth) continue;
+        RadRec r{};
+        for (int k = 0; k < 3; k++) {
+            r.p[k] = (float)p[k];
+            r.E[k] = E[k];
+        }
+        if (!unpack_dir(ns, r.n) || !unpack_dir(ts, r.tn)) continue;
+        r.r = (float)hm;
+        r.nearest = (float)nearest;
+        e->preload[depth].push_back(r);
+        n++;
+    }
+    std::fclose(f);
+    if (n_read) *n_read = n;
+    return PVX_OK;
+}
+
 extern "C" int pvx_radiosity_render(pvx_scene *s, const pvx_camera *cam, const pvx_radiosity_settings *set,
                                     float *rgb, pvx_radiosity_stats *stats, void *stream) {
     if (!s || !cam || !set || !rgb) return fail(PVX_EINVAL, "null argument");
@@ -1559,12 +1655,7 @@ extern "C" int pvx_radiosity_render(pvx_scene *s, const pvx_camera *cam, const p
         if (m.reflective)
             return fail(PVX_EUNSUPPORTED, "radiosity frames take plain diffuse / emissive finishes only");
     hipStream_t st = static_cast<hipStream_t>(stream);
-    if (!s->rad_engine) {
-        RadiosityEngine *e = new (std::nothrow) RadiosityEngine(s);
-        if (!e) return fail(PVX_ENOMEM, "host allocation failed");
-        s->rad_engine = e;
-        s->rad_engine_free = [](void *p) { delete static_cast<RadiosityEngine *>(p); };
-    }
+    if (!engine_of(s)) return fail(PVX_ENOMEM, "host allocation failed");
     RadiosityEngine &eng = *static_cast<RadiosityEngine *>(s->rad_engine);
     int r;
     if ((r = eng.begin(*set, *cam, st))) return r;

@@ -103,6 +103,7 @@ struct RadiosityEngine {
     Surrogate *amb_net = nullptr;
     float nlo[3] = {0, 0, 0}, ninv = 1.0f;
     std::vector<RadRec> last_fetched[kMaxRadDepth]; // records of the last gather per depth
+    std::vector<RadRec> preload[kMaxRadDepth];      // records loaded from a cache file (seed every frame)
     double train_ms = 0.0;
     double t_sync_ms = 0.0, t_grid_ms = 0.0; // host-side waits and grid builds
     uint64_t n_sync = 0;

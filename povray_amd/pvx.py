@@ -63,6 +63,8 @@ def lib():
         L.pvx_clock_rate.argtypes = [C.POINTER(C.c_double)]
         L.pvx_scene_autotune.argtypes = [vp, vp, sz, u32, C.POINTER(A.PvxTuneResult), vp]
         L.pvx_scene_set_traversal.argtypes = [vp, u32, u32, u32]
+        L.pvx_radiosity_save.argtypes = [vp, C.c_char_p, C.POINTER(C.c_uint64)]
+        L.pvx_radiosity_load.argtypes = [vp, C.c_char_p, C.POINTER(C.c_uint64)]
         L.pvx_render_autotune.argtypes = [vp, C.POINTER(A.PvxCamera), vp, C.POINTER(A.PvxTuneResult),
                                           C.POINTER(A.PvxRenderStats), vp]
         fp = C.POINTER(C.c_float)
@@ -86,7 +88,7 @@ EXPORTED_SYMBOLS = (
     "pvx_surrogate_get_params", "pvx_surrogate_grad", "pvx_surrogate_train", "pvx_surrogate_infer",
     "pvx_batch_create", "pvx_batch_destroy", "pvx_batch_trace_closest", "pvx_batch_trace_closest_counted",
     "pvx_clock_rate", "pvx_scene_autotune", "pvx_scene_set_traversal",
-    "pvx_render_autotune",
+    "pvx_render_autotune", "pvx_radiosity_save", "pvx_radiosity_load",
 )
 
 
@@ -280,6 +282,19 @@ class GpuScene:
         hip.synchronize()
         img = out.download(np.float32, cam.width * cam.height * 3)
         return img.reshape(cam.height, cam.width, 3), stats
+
+    def radiosity_save(self, path) -> int:
+        """Write the last radiosity frame's cache as a .rca file; returns the record count."""
+        n = C.c_uint64()
+        _check(lib().pvx_radiosity_save(self.h, str(path).encode(), C.byref(n)), "pvx_radiosity_save")
+        return n.value
+
+    def radiosity_load(self, path) -> int:
+        """Seed later radiosity frames with a .rca file's records (None drops them)."""
+        n = C.c_uint64()
+        _check(lib().pvx_radiosity_load(self.h, None if path is None else str(path).encode(), C.byref(n)),
+               "pvx_radiosity_load")
+        return n.value
 
     def inside(self, pts: np.ndarray, obj):
         pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
