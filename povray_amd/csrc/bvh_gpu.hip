@@ -344,16 +344,20 @@ __global__ void __launch_bounds__(kB) k_wide_emit(const Node *nodes, const int32
 
 inline unsigned blocks(size_t n) { return (unsigned)((n + kB - 1) / kB); }
 
-// Temporaries freed on every return path.
+// Temporaries from the stream-ordered allocator (no device-wide wait on
+// free), released on every return path after the work queued on `st`.
 struct Temps {
+    hipStream_t st;
     std::vector<void *> p;
+    explicit Temps(hipStream_t s) : st(s) {}
     ~Temps() {
-        for (void *q : p) (void)hipFree(q);
+        for (void *q : p) (void)hipFreeAsync(q, st);
+        (void)hipStreamSynchronize(st);
     }
     template <class T>
     int get(T **out, size_t count) {
         void *q = nullptr;
-        HIP_TRY(hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T)));
+        HIP_TRY(hipMallocAsync(&q, std::max<size_t>(count, 1) * sizeof(T), st));
         p.push_back(q);
         *out = static_cast<T *>(q);
         return PVX_OK;
@@ -371,7 +375,7 @@ int gpu_build_bvh(const float *d_lo, const float *d_hi, const PrimRec *d_prims_i
         out.root4 = out.root = INT32_MIN;
         return PVX_OK;
     }
-    Temps tmp;
+    Temps tmp(st);
     int r;
     uint32_t *vals = nullptr;
     if ((r = tmp.get(&vals, n))) return r;
