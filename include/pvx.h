@@ -41,8 +41,9 @@ enum {
 typedef struct pvx_scene pvx_scene;
 
 /* ---- object model ------------------------------------------------------ */
-/* Shape kinds: the subset of core/shape on the hot path
- * (sphere.cpp, box.cpp, torus.cpp, plane.cpp, mesh.cpp, csg.cpp). */
+/* Shape kinds: the subset of core/shape on the hot path (sphere.cpp,
+ * box.cpp, torus.cpp, plane.cpp, mesh.cpp, csg.cpp, triangle.cpp, cone.cpp,
+ * disc.cpp, quadric.cpp). */
 enum {
     PVX_SPHERE = 1,
     PVX_BOX = 2,
