@@ -211,14 +211,14 @@ class Scene:
         for i in range(n):
             t = mats[i]
             t.pigment[:] = [float(x) for x in m["pigment"][i]]
-            t.filter = 0.0
+            t.filter = float(m["filter"][i]) if "filter" in m else 0.0
             t.transmit = float(m.get("transmit", np.zeros(n))[i])
             amb = float(m.get("ambient", np.full(n, 0.1))[i])
             t.ambient[:] = [amb, amb, amb]
             em = m.get("emission", None)
             t.emission[:] = [float(x) for x in em[i]] if em is not None else [0.0, 0.0, 0.0]
             t.diffuse = float(m.get("diffuse", np.full(n, 0.6))[i])
-            t.brilliance = 1.0
+            t.brilliance = float(m["brilliance"][i]) if "brilliance" in m else 1.0
             t.phong = float(m.get("phong", np.zeros(n))[i])
             t.phong_size = float(m.get("phong_size", np.full(n, 40.0))[i])
             r = float(m.get("reflection", np.zeros(n))[i])
@@ -226,7 +226,7 @@ class Scene:
             t.reflection_max[:] = [r, r, r]
             t.reflection_falloff = 1.0
             ior = float(m.get("ior", np.ones(n))[i])
-            t.ior = ior if t.transmit > 0.0 else 0.0
+            t.ior = ior if (t.transmit > 0.0 or t.filter > 0.0) else 0.0
         lights = (A.PvxLight * max(1, len(self.lights)))()
         for i, L in enumerate(self.lights):
             lights[i].kind = A.PVX_LIGHT_AREA if L["kind"] == "area" else A.PVX_LIGHT_POINT
@@ -243,8 +243,8 @@ class Scene:
         d.materials = C.cast(mats, C.POINTER(A.PvxMaterial))
         d.n_lights = len(self.lights)
         d.lights = C.cast(lights, C.POINTER(A.PvxLight))
-        d.ambient_light[:] = [1.0, 1.0, 1.0]
-        d.background[:] = [0.0, 0.0, 0.0]
+        d.ambient_light[:] = [float(x) for x in self.settings.get("ambient_light", (1.0, 1.0, 1.0))]
+        d.background[:] = [float(x) for x in self.settings.get("background", (0.0, 0.0, 0.0))]
         d.max_trace_level = int(self.settings.get("max_trace_level", 5))
         d.adc_bailout = float(self.settings.get("adc_bailout", 1.0 / 255.0))
         return d, [mats, lights]
