@@ -43,7 +43,7 @@ typedef struct pvx_scene pvx_scene;
 /* ---- object model ------------------------------------------------------ */
 /* Shape kinds: the subset of core/shape on the hot path (sphere.cpp,
  * box.cpp, torus.cpp, plane.cpp, mesh.cpp, csg.cpp, triangle.cpp, cone.cpp,
- * disc.cpp, quadric.cpp). */
+ * disc.cpp, quadric.cpp, polygon.cpp). */
 enum {
     PVX_SPHERE = 1,
     PVX_BOX = 2,
@@ -55,7 +55,8 @@ enum {
     PVX_SMOOTH_TRIANGLE = 8, /* stand-alone smooth triangle (triangle.cpp, SmoothTriangle) */
     PVX_CONE = 9,            /* cone or cylinder (cone.cpp, Cone)                          */
     PVX_DISC = 10,           /* disc, optionally with a hole (disc.cpp, Disc)              */
-    PVX_QUADRIC = 11         /* quadric surface (quadric.cpp, Quadric)                     */
+    PVX_QUADRIC = 11,        /* quadric surface (quadric.cpp, Quadric)                     */
+    PVX_POLYGON = 12         /* planar polygon (polygon.cpp, Polygon; not a CSG operand)   */
 };
 
 /* pvx_scene_desc.cone_flags bits */
@@ -204,6 +205,16 @@ typedef struct pvx_scene_desc {
     uint32_t n_quadrics;
     const double *quadric_coef;     /* 10n                                      */
     const int32_t *quadric_transform; /* n: index into transforms or -1; or NULL */
+
+    /* polygons as Parse_Polygon reads them: polygon i has the points
+     * polygon_points[3 * polygon_first[i] .. 3 * polygon_first[i + 1]) in
+     * statement order (sub-polygons closed by repeating their first point;
+     * an open last one is closed as the parser does); the optional transform
+     * is composed after the polygon's own placement (Polygon::Transform). */
+    uint32_t n_polygons;
+    const uint32_t *polygon_first;  /* n + 1 offsets, in points                 */
+    const double *polygon_points;   /* 3 per point                              */
+    const int32_t *polygon_transform; /* n: index into transforms or -1; or NULL */
 } pvx_scene_desc;
 
 /* ---- rays and hits ----------------------------------------------------- */

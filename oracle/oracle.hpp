@@ -189,6 +189,19 @@ bool disc_intersect(const Disc &c, const Ray &r, double *depth);
 bool disc_inside(const Disc &c, V3 p, bool inverted);
 FBox disc_bbox(const Disc &c);
 
+// Polygon (polygon.cpp): points in the polygon's own u,v plane, the
+// world-to-plane transform (inverse only), S_Normal, the bounding box.
+struct PolygonO {
+    Transform tr;
+    std::vector<double> pts;
+    int number = 0;
+    bool degenerate = false;
+    V3 normal;
+    FBox bbox;
+};
+PolygonO polygon_prepare(const double *pts, int number, const Transform *user);
+bool polygon_intersect(const PolygonO &g, const Ray &r, double *depth);
+
 // Quadric (quadric.cpp): coefficients A..J after the object's transform.
 struct Quadric {
     double c[10];
@@ -252,6 +265,7 @@ struct Scene {
     std::vector<Cone> cones;
     std::vector<Disc> discs;
     std::vector<Quadric> quadrics;
+    std::vector<PolygonO> polygons;
     std::vector<Transform> trs;
     std::vector<Mesh> meshes;
     std::vector<pvx_csg_node> csg;

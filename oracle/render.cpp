@@ -90,6 +90,8 @@ V3 shape_normal(const Scene &s, uint32_t type, uint32_t index, const Isect &h, V
         if (dist > kEpsilon) M = V3(s.tor_R[index] * P.x / dist, 0.0, s.tor_R[index] * P.z / dist);
         return normalized(minv_trans_normal(tr->inv, P - M));
     }
+    case PVX_POLYGON: // Polygon::Normal (polygon.cpp:308): S_Normal
+        return s.polygons[index].normal;
     case PVX_QUADRIC: // Quadric::Normal (quadric.cpp)
         return quadric_normal(s.quadrics[index], ip);
     case PVX_DISC: // Disc::Normal (disc.cpp): the stored normal

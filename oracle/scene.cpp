@@ -59,6 +59,9 @@ FBox Scene::shape_bbox(uint32_t type, uint32_t index, uint32_t flags, bool *infi
     case PVX_DISC:
         b = disc_bbox(discs[index]);
         break;
+    case PVX_POLYGON:
+        b = polygons[index].bbox;
+        break;
     case PVX_QUADRIC: // unbounded (see quadric.cpp)
         for (int k = 0; k < 3; k++) {
             b.ll[k] = (float)(-kBoundHuge / 2);
@@ -189,6 +192,12 @@ bool Scene::all_intersections(uint32_t type, uint32_t index, uint32_t flags, con
         for (int k = 0; k < n; k++) st.push_back(Isect{dep[k], slot, part[k], slot});
         return n > 0;
     }
+    case PVX_POLYGON: { // Polygon::All_Intersections (polygon.cpp:131-160)
+        double dep;
+        if (!polygon_intersect(polygons[index], r, &dep)) return false;
+        st.push_back(Isect{dep, slot, 0, slot});
+        return true;
+    }
     case PVX_QUADRIC: { // Quadric::All_Intersections (quadric.cpp:100-135), no clipped_by
         double dep[2];
         const int n = quadric_intersect(quadrics[index], r, dep);
@@ -266,6 +275,7 @@ bool Scene::inside(uint32_t type, uint32_t index, uint32_t flags, V3 p) const {
         return inv ? !((t + pl_d[index]) < kEpsilon) : ((t + pl_d[index]) < kEpsilon);
     }
     case PVX_MESH:
+    case PVX_POLYGON: // Polygon::Inside (polygon.cpp:269-272): never inside
     case PVX_TRIANGLE: // Triangle::Inside (triangle.cpp:608-611): never inside
     case PVX_SMOOTH_TRIANGLE:
         return inv;
@@ -385,6 +395,14 @@ static Scene *scene_create(const pvx_scene_desc &d) {
         const int32_t t = d.quadric_transform ? d.quadric_transform[i] : -1;
         check_tr(t);
         s->quadrics.push_back(quadric_prepare(d.quadric_coef + 10 * (size_t)i, t >= 0 ? &s->trs[t] : nullptr));
+    }
+    for (uint32_t i = 0; i < d.n_polygons; i++) {
+        const int32_t t = d.polygon_transform ? d.polygon_transform[i] : -1;
+        check_tr(t);
+        const uint32_t b = d.polygon_first[i], e = d.polygon_first[i + 1];
+        if (e < b + 3) throw std::runtime_error("polygon with fewer than three points");
+        s->polygons.push_back(polygon_prepare(d.polygon_points + 3 * (size_t)b, (int)(e - b),
+                                              t >= 0 ? &s->trs[t] : nullptr));
     }
     if (d.n_triangles) s->tri_p.assign(d.triangle_vertices, d.triangle_vertices + 9 * (size_t)d.n_triangles);
     s->tris.resize(d.n_triangles);

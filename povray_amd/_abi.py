@@ -13,6 +13,7 @@ PVX_TRIANGLE, PVX_SMOOTH_TRIANGLE = 7, 8
 PVX_CONE = 9
 PVX_DISC = 10
 PVX_QUADRIC = 11
+PVX_POLYGON = 12
 PVX_CONE_OPEN, PVX_CONE_CYLINDER = 1, 2
 
 PVX_FLAG_NO_SHADOW = 1 << 0
@@ -70,6 +71,8 @@ class PvxSceneDesc(C.Structure):
         ("n_discs", C.c_uint32), ("disc_center", c_dp), ("disc_normal", c_dp), ("disc_radius", c_dp),
         ("disc_transform", c_i32p),
         ("n_quadrics", C.c_uint32), ("quadric_coef", c_dp), ("quadric_transform", c_i32p),
+        ("n_polygons", C.c_uint32), ("polygon_first", C.POINTER(C.c_uint32)), ("polygon_points", c_dp),
+        ("polygon_transform", c_i32p),
     ]
 
 

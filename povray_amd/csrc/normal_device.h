@@ -56,6 +56,10 @@ __device__ inline D3 leaf_normal(const NormalParams &p, uint32_t type, uint32_t 
         const double len = sqrt(n.x * n.x + n.y * n.y + n.z * n.z);
         return len == 0.0 ? D3{1.0, 0.0, 0.0} : D3{n.x / len, n.y / len, n.z / len};
     }
+    case PVX_POLYGON: { // Polygon::Normal (polygon.cpp:308): S_Normal
+        const PolygonRec &pg = sv.polygons[index];
+        return D3{pg.n[0], pg.n[1], pg.n[2]};
+    }
     case PVX_DISC: { // Disc::Normal (disc.cpp): the stored normal
         const DiscRec &c = sv.discs[index];
         return D3{c.n[0], c.n[1], c.n[2]};

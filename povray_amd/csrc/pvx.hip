@@ -131,10 +131,11 @@ __device__ inline bool quadric_closest(const QuadricRec &q, D3 o, D3 d, bool sha
 // (Find_Intersection over its IStack).
 __device__ inline bool simple_closest(const SceneView &sv, uint32_t kind, uint32_t index, D3 o,
                                       D3 d, bool shadow, double &dep_out) {
-    if (kind == kPrimTriangle64 || kind == kPrimDisc) {
+    if (kind == kPrimTriangle64 || kind == kPrimDisc || kind == kPrimPolygon) {
         double dep;
-        const bool hit = kind == kPrimDisc ? disc_hit(sv.discs[index], sv.xforms, o, d, dep)
-                                           : tri_rec_hit(sv.triangles[index], o, d, dep);
+        const bool hit = kind == kPrimDisc      ? disc_hit(sv.discs[index], sv.xforms, o, d, dep)
+                         : kind == kPrimPolygon ? polygon_hit(sv.polygons[index], sv.poly_pts, sv.xforms, o, d, dep)
+                                                : tri_rec_hit(sv.triangles[index], o, d, dep);
         if (!hit || !depth_ok(dep, shadow)) return false;
         dep_out = dep;
         return true;
@@ -996,6 +997,8 @@ extern "C" int pvx_scene_create(const pvx_scene_desc *desc, const pvx_build_opts
     UP(h.cones, cones);
     UP(h.discs, discs);
     UP(h.quadrics, quadrics);
+    UP(h.polygons, polygons);
+    UP(h.poly_pts, poly_pts);
     UP(h.xforms, xforms);
     UP(h.sph, sph);
     UP(h.csg, csg);

@@ -39,6 +39,7 @@ enum PrimKind : uint32_t {
     kPrimCone = 7,       // cone / cylinder, by reference
     kPrimDisc = 8,       // disc, by reference
     kPrimQuadric = 9,    // quadric (outside the BVH)
+    kPrimPolygon = 10,   // polygon, by reference
 };
 
 struct alignas(16) Node {
@@ -172,6 +173,17 @@ struct QuadricRec {
     double qa, qe, qh, qb, qc, qf, qd, qg, qi, qj;
 };
 
+// Polygon after Compute_Polygon (core/shape/polygon.cpp:646-805): the
+// points in its own u,v plane (poly_pts, 2 doubles each, from `first`),
+// xforms[xf].inv maps the world to that plane, S_Normal in world space.
+struct PolygonRec {
+    double n[3];
+    int32_t xf;
+    uint32_t first, count;
+    uint32_t degenerate;
+    uint32_t pad;
+};
+
 struct PlaneRec {
     double n[3];
     double d;
@@ -245,6 +257,8 @@ struct SceneView {
     const ConeRec *cones;
     const DiscRec *discs;
     const QuadricRec *quadrics;
+    const PolygonRec *polygons;
+    const double *poly_pts;
     const Xform *xforms;
     const double *sph; // sphere array (cx, cy, cz, r) per sphere, for CSG leaves
     const CsgNode *csg;

@@ -381,6 +381,7 @@ int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene
     if (!need(d.n_cones == 0 || (d.cone_apex && d.cone_base && d.cone_radius), "cones")) return PVX_EINVAL;
     if (!need(d.n_discs == 0 || (d.disc_center && d.disc_normal && d.disc_radius), "discs")) return PVX_EINVAL;
     if (!need(d.n_quadrics == 0 || d.quadric_coef, "quadrics")) return PVX_EINVAL;
+    if (!need(d.n_polygons == 0 || (d.polygon_first && d.polygon_points), "polygons")) return PVX_EINVAL;
 
     auto check_ref = [&](uint32_t type, uint32_t index) -> bool {
         uint32_t lim = 0;
@@ -396,6 +397,7 @@ int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene
         case PVX_CONE: lim = d.n_cones; break;
         case PVX_DISC: lim = d.n_discs; break;
         case PVX_QUADRIC: lim = d.n_quadrics; break;
+        case PVX_POLYGON: lim = d.n_polygons; break;
         default:
             err = "unknown object type " + std::to_string(type);
             return false;
@@ -543,6 +545,123 @@ int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene
                        M[11] + M[14], M[15]};
     }
 
+    std::vector<DBox> poly_boxes;
+    // polygons: Parse_Polygon's closing of the point list, then Compute_Polygon
+    // (polygon.cpp:646-805) and Polygon::Transform
+    out.polygons.resize(d.n_polygons);
+    for (uint32_t i = 0; i < d.n_polygons; i++) {
+        const uint32_t b = d.polygon_first[i], e = d.polygon_first[i + 1];
+        if (e < b + 3) {
+            err = "polygon " + std::to_string(i) + " needs at least three points";
+            return PVX_EINVAL;
+        }
+        std::vector<double> P(d.polygon_points + 3 * (size_t)b, d.polygon_points + 3 * (size_t)e);
+        int number = (int)(e - b);
+        bool closed = false;
+        double Q[3] = {P[0], P[1], P[2]};
+        for (int k = 1; k < number; k++) {
+            closed = false;
+            if (std::fabs(Q[0] - P[3 * k]) < kEpsilon && std::fabs(Q[1] - P[3 * k + 1]) < kEpsilon &&
+                std::fabs(Q[2] - P[3 * k + 2]) < kEpsilon) {
+                for (int a = 0; a < 3; a++) P[3 * k + a] = Q[a];
+                k++;
+                if (k < number)
+                    for (int a = 0; a < 3; a++) Q[a] = P[3 * k + a];
+                closed = true;
+            }
+        }
+        if (!closed) {
+            P.insert(P.end(), Q, Q + 3);
+            number++;
+        }
+        PolygonRec &pg = out.polygons[i];
+        pg.degenerate = 0;
+        pg.pad = 0;
+        auto sub = [&](int k, double *r) {
+            for (int a = 0; a < 3; a++) r[a] = P[3 * k + a] - P[a];
+        };
+        auto cross = [](const double *x, const double *y, double *r) {
+            r[0] = x[1] * y[2] - x[2] * y[1];
+            r[1] = x[2] * y[0] - x[0] * y[2];
+            r[2] = x[0] * y[1] - x[1] * y[0];
+        };
+        auto len2 = [](const double *x) { return x[0] * x[0] + x[1] * x[1] + x[2] * x[2]; };
+        double u[3] = {0, 0, 0}, v[3] = {0, 0, 0}, w[3] = {0, 0, 0};
+        int k = 1;
+        for (; k < number; k++) {
+            sub(k, u);
+            if (len2(u) > kEpsilon) break;
+        }
+        if (k == number) pg.degenerate = 1;
+        for (k++; k < number; k++) {
+            sub(k, v);
+            cross(u, v, w);
+            if (len2(v) > kEpsilon && len2(w) > kEpsilon) break;
+        }
+        if (k == number) pg.degenerate = 1;
+        double t[3];
+        cross(v, w, t);
+        std::memcpy(u, t, sizeof t);
+        cross(w, u, t);
+        std::memcpy(v, t, sizeof t);
+        for (double *x : {u, v, w}) {
+            const double l = std::sqrt(len2(x));
+            if (l != 0)
+                for (int a = 0; a < 3; a++) x[a] = x[a] / l;
+        }
+        // inverse = A * B (MTimesC): A translates by -o, B's columns are u, v, w
+        Xform x;
+        xf_identity(x);
+        double A_[16], B_[16];
+        for (int q = 0; q < 16; q++) A_[q] = B_[q] = (q % 5 == 0) ? 1.0 : 0.0;
+        for (int a = 0; a < 3; a++) {
+            A_[12 + a] = -P[a];
+            B_[4 * a + 0] = u[a];
+            B_[4 * a + 1] = v[a];
+            B_[4 * a + 2] = w[a];
+        }
+        for (int r = 0; r < 4; r++)
+            for (int c = 0; c < 4; c++)
+                x.inv[4 * r + c] = A_[4 * r] * B_[c] + A_[4 * r + 1] * B_[4 + c] + A_[4 * r + 2] * B_[8 + c] +
+                                   A_[4 * r + 3] * B_[12 + c];
+        // (x.m is not used: hits need the inverse only; the box comes from the points)
+        pg.first = (uint32_t)(out.poly_pts.size() / 2);
+        pg.count = (uint32_t)number;
+        for (int q = 0; q < number; q++) {
+            const double px = P[3 * q] - P[0], py = P[3 * q + 1] - P[1], pz = P[3 * q + 2] - P[2];
+            const double dd = px * w[0] + py * w[1] + pz * w[2];
+            if (std::fabs(dd) > 1.0e-10) pg.degenerate = 1; // not co-planar (ZERO_TOLERANCE)
+            out.poly_pts.push_back(px * u[0] + py * u[1] + pz * u[2]);
+            out.poly_pts.push_back(px * v[0] + py * v[1] + pz * v[2]);
+        }
+        const int32_t tix = d.polygon_transform ? d.polygon_transform[i] : -1;
+        if (tix >= (int32_t)d.n_transforms) {
+            err = "polygon transform index out of range";
+            return PVX_EINVAL;
+        }
+        // bounds: the points (then the transform's image of their box), +-SMALL_TOLERANCE
+        DBox bb = DBox::empty();
+        for (int q = 0; q < number; q++) bb.grow(&P[3 * q]);
+        for (int a = 0; a < 3; a++) {
+            bb.lo[a] -= kSmallTolerance;
+            bb.hi[a] += kSmallTolerance;
+        }
+        if (tix >= 0) {
+            Xform u2;
+            std::memcpy(u2.m, d.transforms + 32 * (size_t)tix, 16 * sizeof(double));
+            std::memcpy(u2.inv, d.transforms + 32 * (size_t)tix + 16, 16 * sizeof(double));
+            xf_compose(x, u2); // only its inverse is kept
+            bb = transformed_box(bb.lo, bb.hi, u2.m);
+        }
+        // S_Normal = MTransNormal(<0,0,1>, Trans): column 2 of the inverse rows
+        double nn[3] = {x.inv[2], x.inv[6], x.inv[10]};
+        const double nl = std::sqrt(len2(nn));
+        for (int a = 0; a < 3; a++) pg.n[a] = nl != 0 ? nn[a] / nl : nn[a];
+        pg.xf = (int32_t)out.xforms.size();
+        out.xforms.push_back(x);
+        poly_boxes.push_back(bb);
+    }
+
     // stand-alone triangles, flat or smooth as their objects say
     {
         std::vector<uint8_t> kind(d.n_triangles, 0);
@@ -587,8 +706,9 @@ int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene
             uint32_t slot = n.first_child + k;
             const pvx_object &c = d.csg_children[slot];
             if (!check_ref(c.type, c.index)) return PVX_EINVAL;
-            if (c.type == PVX_MESH || c.type == PVX_TRIANGLE || c.type == PVX_SMOOTH_TRIANGLE) {
-                err = "meshes and triangles inside CSG are not supported";
+            if (c.type == PVX_MESH || c.type == PVX_TRIANGLE || c.type == PVX_SMOOTH_TRIANGLE ||
+                c.type == PVX_POLYGON) {
+                err = "meshes, triangles and polygons inside CSG are not supported";
                 return PVX_EUNSUPPORTED;
             }
             out.csg_children[slot] = CsgChild{c.type, c.index, c.flags, i};
@@ -787,6 +907,15 @@ int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene
         case PVX_PLANE:
             out.inf.push_back(InfRec{kPrimPlane, ob.index, (int32_t)o, 0});
             break;
+        case PVX_POLYGON: {
+            if (out.polygons[ob.index].degenerate) break; // never hit (Polygon::Intersect)
+            pr.kind_axis = kPrimPolygon;
+            pr.ref.index = ob.index;
+            to_fbox(poly_boxes[ob.index], lo, hi);
+            add_prim(pr, lo, hi);
+            out.has_complex = 2;
+            break;
+        }
         case PVX_QUADRIC: // unbounded here: Compute_BBox's box only culls (quadric.cpp:764-1000)
             out.inf.push_back(InfRec{kPrimQuadric, ob.index, (int32_t)o, 0});
             out.has_complex = 2;

@@ -32,6 +32,8 @@ struct HostScene {
     std::vector<ConeRec> cones;
     std::vector<DiscRec> discs;
     std::vector<QuadricRec> quadrics;
+    std::vector<PolygonRec> polygons;
+    std::vector<double> poly_pts;
     std::vector<Xform> xforms;
     std::vector<double> sph; // 4 per sphere: centre, radius
     std::vector<CsgNode> csg;

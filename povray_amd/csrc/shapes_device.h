@@ -811,6 +811,54 @@ __device__ inline bool quadric_inside(const QuadricRec &q, D3 p, bool inv) {
 }
 
 // ---------------------------------------------------------------------------
+// Polygon (core/shape/polygon.cpp)
+// ---------------------------------------------------------------------------
+// Polygon::in_polygon (polygon.cpp:905-984): crossings of the +u ray from
+// (u, v) with the edges; a point equal to its sub-polygon's first point
+// closes that sub-polygon and the next starts after it.
+__device__ inline bool in_polygon(const double *pts, uint32_t number, double tx, double ty) {
+    const double *vtx0 = pts, *vtx1 = pts + 2, *first = pts;
+    bool yflag0 = vtx0[1] >= ty, inside = false;
+    for (uint32_t i = 1; i < number;) {
+        const bool yflag1 = vtx1[1] >= ty;
+        if (yflag0 != yflag1 &&
+            (((vtx1[1] - ty) * (vtx0[0] - vtx1[0]) >= (vtx1[0] - tx) * (vtx0[1] - vtx1[1])) == yflag1))
+            inside = !inside;
+        if (i + 2 < number && vtx1[0] == first[0] && vtx1[1] == first[1]) {
+            vtx0 = pts + 2 * (size_t)(++i);
+            vtx1 = pts + 2 * (size_t)(++i);
+            yflag0 = vtx0[1] >= ty;
+            first = vtx0;
+        } else {
+            vtx0 = vtx1;
+            ++i;
+            if (i < number) vtx1 = pts + 2 * (size_t)i;
+            yflag0 = yflag1;
+        }
+    }
+    return inside;
+}
+
+// Polygon::Intersect (polygon.cpp:187-230): the plane z = 0 of polygon
+// space, depth in [1e-8, MAX_DISTANCE] before the direction's length is
+// divided out, then the crossing test.
+__device__ inline bool polygon_hit(const PolygonRec &pg, const double *pts, const Xform *xf, D3 o, D3 d,
+                                   double &depth) {
+    if (pg.degenerate) return false;
+    const Xform &x = xf[pg.xf];
+    const D3 P = xf_point(x.inv, o);
+    D3 D = xf_dir(x.inv, d);
+    const double len = sqrt(D.x * D.x + D.y * D.y + D.z * D.z);
+    D = D3{D.x / len, D.y / len, D.z / len};
+    if (fabs(D.z) < 1.0e-10) return false;
+    double t = -P.z / D.z;
+    if (t < 1.0e-8 || t > kMaxDistance) return false;
+    if (!in_polygon(pts + 2 * (size_t)pg.first, pg.count, P.x + t * D.x, P.y + t * D.y)) return false;
+    depth = t / len;
+    return true;
+}
+
+// ---------------------------------------------------------------------------
 // CSG (core/shape/csg.cpp): point membership with bounded nesting.
 // ---------------------------------------------------------------------------
 template <int D>
@@ -890,6 +938,8 @@ __device__ inline int leaf_hits(const SceneView &sv, uint32_t type, uint32_t ind
         return disc_hit(sv.discs[index], sv.xforms, o, d, out[0]) ? 1 : 0;
     case 11:
         return quadric_hits(sv.quadrics[index], o, d, out);
+    case 12: // polygon (not a CSG operand; All_Intersections)
+        return polygon_hit(sv.polygons[index], sv.poly_pts, sv.xforms, o, d, out[0]) ? 1 : 0;
     case 7:
     case 8: { // stand-alone triangle (not a CSG child; All_Intersections)
         double dep;

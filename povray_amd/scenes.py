@@ -104,6 +104,9 @@ class Scene:
     disc_transform: np.ndarray = None        # (n,) int32: transform index or -1
     quadric_coef: np.ndarray = None          # (n, 10) float64: A..J as quadric { <A,B,C>, <D,E,F>, <G,H,I>, J }
     quadric_transform: np.ndarray = None     # (n,) int32: transform index or -1
+    polygon_first: np.ndarray = None         # (n + 1,) uint32: offsets into polygon_points (in points)
+    polygon_points: np.ndarray = None        # (m, 3) float64: the statements' points
+    polygon_transform: np.ndarray = None     # (n,) int32: transform index or -1
     camera: dict = None
     lights: list = field(default_factory=list)
     materials: dict = field(default_factory=dict)  # per top-level object arrays
@@ -192,6 +195,12 @@ class Scene:
             d.quadric_coef, d.n_quadrics = arr(np.asarray(self.quadric_coef).reshape(-1, 10), np.float64, C.c_double)
             if self.quadric_transform is not None:
                 d.quadric_transform, _ = arr(self.quadric_transform, np.int32, C.c_int32)
+        if self.polygon_first is not None and len(self.polygon_first) > 1:
+            d.polygon_first, n1 = arr(self.polygon_first, np.uint32, C.c_uint32)
+            d.n_polygons = n1 - 1
+            d.polygon_points, _ = arr(np.asarray(self.polygon_points).reshape(-1, 3), np.float64, C.c_double)
+            if self.polygon_transform is not None:
+                d.polygon_transform, _ = arr(self.polygon_transform, np.int32, C.c_int32)
         if self.csg is not None and len(self.csg):
             c = np.ascontiguousarray(self.csg, np.uint32).reshape(-1, 4)
             ch = np.ascontiguousarray(self.csg_children, np.uint32).reshape(-1, 4)
@@ -777,7 +786,7 @@ def triangle_scene(seed: int = 0xC0FFEE + 5, n: int = 400, n_spheres: int = 40, 
 # cones and cylinders
 # --------------------------------------------------------------------------
 def cone_scene(seed: int = 0xC0FFEE + 6, n: int = 300, n_csg: int = 40, n_discs: int = 60, n_quadrics: int = 8,
-               width=320, height=240) -> Scene:
+               n_polygons: int = 10, width=320, height=240) -> Scene:
     """`n` seeded cones and cylinders in a 12-unit cube: a third `cylinder`
     statements, the rest cones with radii U[0, 0.8] (a pointed end at radius
     0 for one in eight; one in eight with equal radii, which the reference
@@ -790,7 +799,10 @@ def cone_scene(seed: int = 0xC0FFEE + 6, n: int = 300, n_csg: int = 40, n_discs:
     ellipsoid quadrics (semi-axes U[0.6, 1.5], rotated and moved by their
     transform) and two CSG objects of unbounded quadrics: an infinite
     cylinder x^2 + z^2 = 0.25 cut by a box, a paraboloid y = x^2 + z^2 cut by
-    a sphere.  A ground plane, one point light; 10% reflective, 10% glass."""
+    a sphere.  `n_polygons` planar star polygons (5 to 8 tips, every other
+    one with a square hole as a second sub-polygon, one in three rotated by
+    its transform).  A ground plane, one point light; 10% reflective, 10%
+    glass."""
     r = Rng(seed)
     m = n + 2 * n_csg
     apex = r.uniform(3 * m, -6.0, 6.0).reshape(m, 3)
@@ -889,7 +901,31 @@ def cone_scene(seed: int = 0xC0FFEE + 6, n: int = 300, n_csg: int = 40, n_discs:
             csg_nodes.append([A.PVX_CSG_INTERSECTION, first, 2, 0])
             qroots.append(len(csg_nodes) - 1)
     n_q = n_quadrics + len(qroots)
-    n_obj = n + n_csg + n_discs + n_q + 1
+    # polygons
+    pfirst, ppts, ptr = [0], [], []
+    for k in range(n_polygons):
+        c = r.uniform(3, -5.0, 5.0)
+        a = r.uniform(3, -1.0, 1.0)
+        a /= np.linalg.norm(a)
+        u = np.cross(a, [0.0, 0.0, 1.0] if abs(a[2]) < 0.9 else [1.0, 0.0, 0.0])
+        u /= np.linalg.norm(u)
+        v = np.cross(a, u)
+        tips = 5 + k % 4
+        ang = np.arange(2 * tips) * np.pi / tips
+        prad = np.where(np.arange(2 * tips) % 2 == 0, r.uniform(1, 0.8, 1.3)[0], r.uniform(1, 0.35, 0.6)[0])
+        outer = [c + rr * (np.cos(t) * u + np.sin(t) * v) for t, rr in zip(ang, prad)]
+        pts = outer + [outer[0]]
+        if k % 2 == 1:  # a square hole
+            sq = [c + 0.15 * (sx * u + sy * v) for sx, sy in ((-1, -1), (1, -1), (1, 1), (-1, 1))]
+            pts += sq + [sq[0]]
+        ppts.extend(pts)
+        pfirst.append(len(ppts))
+        if k % 3 == 2:
+            xforms.append(_xf_row(rotation_transform(r.uniform(3, 0.0, 30.0))))
+            ptr.append(len(xforms) - 1)
+        else:
+            ptr.append(-1)
+    n_obj = n + n_csg + n_discs + n_q + n_polygons + 1
     objects = np.zeros((n_obj, 4), np.uint32)
     objects[:n, 0] = A.PVX_CONE
     objects[:n, 1] = np.arange(n)
@@ -902,6 +938,9 @@ def cone_scene(seed: int = 0xC0FFEE + 6, n: int = 300, n_csg: int = 40, n_discs:
     objects[q0:q0 + n_quadrics, 1] = np.arange(n_quadrics)
     objects[q0 + n_quadrics:q0 + n_q, 0] = A.PVX_CSG
     objects[q0 + n_quadrics:q0 + n_q, 1] = qroots
+    p0 = q0 + n_q
+    objects[p0:p0 + n_polygons, 0] = A.PVX_POLYGON
+    objects[p0:p0 + n_polygons, 1] = np.arange(n_polygons)
     objects[-1, 0] = A.PVX_PLANE
     alb = r.uniform(3 * n_obj, 0.2, 0.95).reshape(n_obj, 3)
     u = r.uniform(n_obj)
@@ -913,6 +952,8 @@ def cone_scene(seed: int = 0xC0FFEE + 6, n: int = 300, n_csg: int = 40, n_discs:
         disc_center=np.array(dc).reshape(-1, 3), disc_normal=np.array(dn).reshape(-1, 3),
         disc_radius=np.array(dr).reshape(-1, 2), disc_transform=np.array(dt, np.int32),
         quadric_coef=np.array(quad, float).reshape(-1, 10), quadric_transform=np.array(qtr, np.int32),
+        polygon_first=np.array(pfirst, np.uint32), polygon_points=np.array(ppts, float).reshape(-1, 3),
+        polygon_transform=np.array(ptr, np.int32),
         transforms=np.array(xforms).reshape(-1, 32),
         box_corner1=np.array(box_c1).reshape(-1, 3), box_corner2=np.array(box_c2).reshape(-1, 3),
         box_transform=np.full(len(box_c1), -1, np.int32),

@@ -19,7 +19,7 @@ from povray_amd import scenes
 
 
 def _single(apex, base, r1, r2, flags, transform=None):
-    s = scenes.cone_scene(n=1, n_csg=0, n_discs=0, n_quadrics=0, width=8, height=8)
+    s = scenes.cone_scene(n=1, n_csg=0, n_discs=0, n_quadrics=0, n_polygons=0, width=8, height=8)
     s.cone_apex = np.array([apex], float)
     s.cone_base = np.array([base], float)
     s.cone_radius = np.array([[r1, r2]], float)
@@ -155,7 +155,7 @@ def test_oracle_cone_inside_and_transform():
 
 def test_cone_scene_generator_and_tree():
     s = scenes.cone_scene()
-    assert s.n_objects == 300 + 40 + 60 + 8 + 2 + 1
+    assert s.n_objects == 300 + 40 + 60 + 8 + 2 + 10 + 1
     o = OracleScene(s)
     rays = scenes.camera_rays_np(s.camera)
     h1, h0 = o.find_intersection(rays, method=1), o.find_intersection(rays, method=0)
@@ -163,10 +163,11 @@ def test_cone_scene_generator_and_tree():
     assert np.mean((h1["obj"] >= 0) & (h1["obj"] < 300)) > 0.1
     assert np.mean((h1["obj"] >= 340) & (h1["obj"] < 400)) > 0.01  # discs are seen
     assert np.mean((h1["obj"] >= 400) & (h1["obj"] < 410)) > 0.003  # quadrics are seen
+    assert np.mean((h1["obj"] >= 410) & (h1["obj"] < 420)) > 0.002  # polygons are seen
 
 
 def _single_disc(center, normal, radius, hole, transform=None):
-    s = scenes.cone_scene(n=1, n_csg=0, n_discs=1, n_quadrics=0, width=8, height=8)
+    s = scenes.cone_scene(n=1, n_csg=0, n_discs=1, n_quadrics=0, n_polygons=0, width=8, height=8)
     s.disc_center = np.array([center], float)
     s.disc_normal = np.array([normal], float)
     s.disc_radius = np.array([[radius, hole]], float)
@@ -241,7 +242,7 @@ def test_oracle_quadric_ellipsoid_closed_form():
     rot = scenes.rotation_transform((25.0, 70.0, 140.0))
     t = scenes.compose(rot, scenes.translation_transform(np.array([0.4, -0.2, 0.3])))
     xf = np.asarray(scenes._xf_row(t))
-    s = scenes.cone_scene(n=1, n_csg=0, n_discs=0, n_quadrics=0, width=8, height=8)
+    s = scenes.cone_scene(n=1, n_csg=0, n_discs=0, n_quadrics=0, n_polygons=0, width=8, height=8)
     s.quadric_coef = np.array([[1 / ax[0] ** 2, 1 / ax[1] ** 2, 1 / ax[2] ** 2, 0, 0, 0, 0, 0, 0, -1.0]])
     s.quadric_transform = np.array([0], np.int32)
     s.transforms = xf.reshape(1, 32)
@@ -272,6 +273,41 @@ def test_oracle_quadric_ellipsoid_closed_form():
     ins = OracleScene(s).inside(pts, np.zeros(pts.shape[0], np.uint32))
     far = np.abs(val) > 1e-9
     assert (ins[far] == (val[far] < 0)).all()
+
+
+def test_oracle_polygon_closed_form():
+    """A square with a square hole, given as two closed sub-polygons (the
+    outer one written without its closing point, which the reader adds):
+    hits are where the ray meets the plane inside the outer square and
+    outside the hole."""
+    s = scenes.cone_scene(n=1, n_csg=0, n_discs=0, n_quadrics=0, n_polygons=0, width=8, height=8)
+    c = np.array([0.3, -0.2, 0.5])
+    u = np.array([0.6, 0.8, 0.0])
+    v = np.array([0.0, 0.0, 1.0])
+    sq = lambda h: [c + h * (a * u + b * v) for a, b in ((-1, -1), (1, -1), (1, 1), (-1, 1))]
+    hole = sq(0.4)
+    pts = sq(1.0) + [sq(1.0)[0]] + hole + [hole[0]]
+    s.polygon_first = np.array([0, len(pts)], np.uint32)
+    s.polygon_points = np.array(pts)
+    s.polygon_transform = None
+    s.objects = np.array([[A.PVX_POLYGON, 0, 0, 0]], np.uint32)
+    s.plane_normal = s.plane_distance = None
+    s.cone_transform = np.array([-1], np.int32)
+    s.transforms = None
+    s.materials = {k: v_[:1] for k, v_ in s.materials.items()}
+    rays = _rays_at(tuple(c), tuple(c), 4000, 51)
+    d, _, cnt = OracleScene(s).all_intersections(rays, np.zeros(rays.shape[0], np.uint32), 2)
+    n = np.cross(u, v)
+    o, dd = rays[:, :3], rays[:, 3:]
+    t = ((c - o) @ n) / (dd @ n)
+    p = o + t[:, None] * dd - c
+    a, b = np.abs(p @ u), np.abs(p @ v)
+    exp = (t > 1e-7) & (np.maximum(a, b) <= 1.0) & (np.maximum(a, b) >= 0.4)
+    edge = (np.abs(np.maximum(a, b) - 1.0) < 1e-9) | (np.abs(np.maximum(a, b) - 0.4) < 1e-9)
+    got = cnt == 1
+    assert (got == exp)[~edge].all()
+    assert exp.sum() > 300 and (~exp).sum() > 300
+    assert np.allclose(d[got, 0], t[got], rtol=1e-9)
 
 
 @pytest.mark.gpu
