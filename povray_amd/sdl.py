@@ -18,7 +18,7 @@ Parse_Object_Mods; parser_expressions.cpp for expressions):
   max_trace_level, adc_bailout, radiosity count / recursion_limit /
   error_bound / nearest_count);
 * sphere, box, plane, cylinder, cone, disc, torus (sturm), quadric, triangle,
-  smooth_triangle, mesh2 (vertex_vectors, normal_vectors, face_indices),
+  smooth_triangle, polygon, mesh2 (vertex_vectors, normal_vectors, face_indices),
   union / intersection / difference / merge, object { IDENT };
 * object modifiers: pigment / color / rgb / rgbf / rgbt, finish (ambient,
   diffuse, brilliance, phong, phong_size, reflection, emission), interior
@@ -70,7 +70,7 @@ _COLOURS = {
 }
 
 _SHAPES = ("sphere", "box", "plane", "cylinder", "cone", "disc", "torus", "quadric", "triangle",
-           "smooth_triangle", "mesh2", "union", "intersection", "difference", "merge", "object")
+           "smooth_triangle", "polygon", "mesh2", "union", "intersection", "difference", "merge", "object")
 
 
 def _identity():
@@ -514,6 +514,16 @@ class _Parser:
                 self.comma()
                 n.append(self.vector())
             o = _Obj("triangle", dict(p=np.array(p), n=np.array(n)), ln)
+        elif kw == "polygon":
+            cnt = int(self.flt())
+            if cnt < 3:
+                raise SdlError(f"line {ln}: polygon needs at least three points")
+            pts = []
+            for _ in range(cnt):
+                self.comma()
+                pts.append(self.vector())
+            self.accept(",")
+            o = _Obj("polygon", dict(p=np.array(pts)), ln)
         elif kw == "mesh2":
             o = self.mesh2(ln)
         else:  # CSG
@@ -662,6 +672,7 @@ class _Parser:
         arrays = dict(sph_c=[], sph_r=[], pl_n=[], pl_d=[], box_c1=[], box_c2=[], box_tr=[], tor_R=[], tor_r=[],
                       tor_tr=[], tor_s=[], xf=[], cone_a=[], cone_b=[], cone_r=[], cone_f=[], cone_t=[],
                       disc_c=[], disc_n=[], disc_r=[], disc_t=[], q=[], q_t=[], tri_p=[], tri_n=[], meshes=[],
+                      poly_first=[0], poly_pts=[], poly_t=[],
                       csg=[], kids=[])
 
         def xf_index(o):
@@ -716,6 +727,11 @@ class _Parser:
                 arrays["tri_n"].append(dd["n"].ravel() if dd["n"] is not None else np.zeros(9))
                 return [A.PVX_SMOOTH_TRIANGLE if dd["n"] is not None else A.PVX_TRIANGLE,
                         len(arrays["tri_p"]) - 1, f, 0]
+            if k == "polygon":
+                arrays["poly_pts"].extend(dd["p"])
+                arrays["poly_first"].append(len(arrays["poly_pts"]))
+                arrays["poly_t"].append(xf_index(o))
+                return [A.PVX_POLYGON, len(arrays["poly_t"]) - 1, f, 0]
             if k == "mesh":
                 arrays["meshes"].append((dd["v"].astype(np.float32), dd["f"].astype(np.int32),
                                          None if dd["n"] is None else dd["n"].astype(np.float32)))
@@ -798,6 +814,8 @@ class _Parser:
             disc_center=arr("disc_c", (-1, 3)), disc_normal=arr("disc_n", (-1, 3)),
             disc_radius=arr("disc_r", (-1, 2)), disc_transform=arr("disc_t", (-1,), np.int32),
             quadric_coef=arr("q", (-1, 10)), quadric_transform=arr("q_t", (-1,), np.int32),
+            polygon_first=np.array(arrays["poly_first"], np.uint32) if arrays["poly_t"] else None,
+            polygon_points=arr("poly_pts", (-1, 3)), polygon_transform=arr("poly_t", (-1,), np.int32),
             camera=self.camera, lights=self.lights, materials=materials, settings=settings)
 
 

@@ -36,6 +36,7 @@ difference {
   interior { ior 1.5 }
 }
 merge { sphere { <-3, 1, 3>, 0.5 } sphere { <-2.6, 1, 3>, 0.5 } pigment { rgb 0.7 } no_shadow }
+polygon { 5, <-4, 2, 2>, <-3, 2, 2>, <-3, 3, 2>, <-4, 3, 2>, <-4, 2, 2> pigment { rgb <0.3, 0.3, 0.9> } }
 mesh2 {
   vertex_vectors { 4, <0, 2.5, 0>, <1, 2.5, 0>, <1, 3.5, 0>, <0, 3.5, 0> }
   face_indices { 2, <0, 1, 2>, <0, 2, 3> }
@@ -49,7 +50,8 @@ def test_sdl_reads_the_scene():
     s = sdl.parse(SCENE, 160, 120)
     types = [int(t) for t in s.objects[:, 0]]
     assert types == [A.PVX_PLANE, A.PVX_SPHERE, A.PVX_BOX, A.PVX_CONE, A.PVX_CONE, A.PVX_DISC, A.PVX_TORUS,
-                     A.PVX_QUADRIC, A.PVX_TRIANGLE, A.PVX_SMOOTH_TRIANGLE, A.PVX_CSG, A.PVX_CSG, A.PVX_MESH]
+                     A.PVX_QUADRIC, A.PVX_TRIANGLE, A.PVX_SMOOTH_TRIANGLE, A.PVX_CSG, A.PVX_CSG, A.PVX_POLYGON,
+                     A.PVX_MESH]
     # Parse_Plane: unit normal, Distance = -d
     assert np.allclose(s.plane_normal[0], [0, 1, 0]) and s.plane_distance[0] == 1.0
     # the sphere moved by its translate, radius from the declared value
@@ -100,6 +102,7 @@ def test_sdl_scene_through_the_oracle():
     assert np.array_equal(h1["obj"], h0["obj"]) and np.array_equal(h1["t"], h0["t"])
     seen = set(int(x) for x in np.unique(h1["obj"]) if x >= 0)
     assert len(seen) >= 10  # most objects are in view
+    assert list(s.polygon_first) == [0, 5] and s.polygon_points.shape == (5, 3)
 
 
 @pytest.mark.gpu
