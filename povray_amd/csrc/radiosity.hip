@@ -1256,7 +1256,7 @@ int RadiosityEngine::compact(const uint8_t *flags, uint32_t n, uint32_t *out, ui
 //     would; the misses take the network's answer, except in tiles whose
 //     probes disagree by more than probe_threshold, which fall back to the
 //     plain method (gather rounds over the same cache).
-// Each network is judged only by plain gathers it was not trained on.
+// Each network is scored only on plain gathers it was not trained on.
 int RadiosityEngine::resolve_learned(const RadQuery *q, uint32_t n, const uint32_t *d_pix, float *E) {
     if (n == 0) return PVX_OK;
     int r;

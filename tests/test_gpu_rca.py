@@ -47,7 +47,7 @@ def test_rca_round_trip(gpu, tmp_path):
     print(f"[rca] {n} records; gathers {st['gathers']} -> {st2['gathers']}", flush=True)
     assert sum(st2["gathers"]) * 4 < sum(st["gathers"])
     # Different gathers mean different Monte Carlo noise (two plain frames of
-    # this count differ by ~6/255 RMS), so the seeded frame is judged against
+    # this count differ by ~6/255 RMS), so the seeded frame is compared with
     # a converged frame: it must be as close to it as the frame that wrote the
     # file, and as bright.
     conv, _ = g.radiosity_render(count=1600, error_bound=0.2, nearest_count=10)
