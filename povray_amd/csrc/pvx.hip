@@ -24,7 +24,7 @@
 #include "bvh_gpu.h"
 #include "internal.h"
 #include "scene_prep.h"
-#include "shapes_device.h"
+#include "trace_launch.h"
 
 using namespace pvx;
 
@@ -43,669 +43,9 @@ extern "C" const char *pvx_last_error(void) { return pvx::g_err.c_str(); }
 // ----------------------------------------------------------------------------
 namespace {
 
-constexpr int kBlock = 256;
-constexpr int kStackLds = 24;
-constexpr int kStackSpill = 40;
-constexpr int kStackMax = kStackLds + kStackSpill;
-constexpr int32_t kEmpty = INT32_MIN;
-
-struct RayF {
-    float ix, iy, iz;    // 1 / direction (FP32, zero components clamped)
-    float oix, oiy, oiz; // origin * inverse direction
-};
-
-__device__ inline float safe_inv(double d) {
-    float f = (float)d;
-    if (fabsf(f) < 1.0e-20f) f = f < 0.0f ? -1.0e-20f : 1.0e-20f;
-    return 1.0f / f;
-}
-
-__device__ inline bool better(double dep, int32_t obj, int32_t prim, double bt, int32_t bo,
-                              int32_t bp) {
-    // strictly closer wins; an exact tie between two hits goes to the lower
-    // (object, primitive) key so results do not depend on traversal order.
-    if (dep < bt) return true;
-    return dep == bt && bo >= 0 && (obj < bo || (obj == bo && prim < bp));
-}
-
-// Find_Intersection's depth filter (object.cpp:149-161) with the shadow-ray
-// postcondition dist > SMALL_TOLERANCE (trace.cpp:1946-1949).
-__device__ inline bool depth_ok(double dep, bool shadow) {
-    return dep >= kMinIsectDepth && (!shadow || dep > kSmallTolerance);
-}
-
-// NoSomethingFlagRayObjectCondition (trace.cpp:84-95) /
-// NoShadowFlagRayObjectCondition (trace.cpp:1941-1944).
-__device__ inline bool precondition(uint32_t flags, uint32_t kind) {
-    if (kind == PVX_RAY_SHADOW) return !(flags & PVX_FLAG_NO_SHADOW);
-    if ((kind & PVX_RAY_IMAGE) && (flags & PVX_FLAG_NO_IMAGE)) return false;
-    if ((kind & PVX_RAY_REFLECTION) && (flags & PVX_FLAG_NO_REFLECTION)) return false;
-    if ((kind & PVX_RAY_RADIOSITY) && (flags & PVX_FLAG_NO_RADIOSITY)) return false;
-    return true;
-}
-
-// Closest valid hit of one CSG object (its leaves' hits that survive every
-// CSG filter up to the root); returns false when none beats `limit`.
-__device__ inline bool csg_closest(const SceneView &sv, uint32_t root, D3 o, D3 d, bool shadow,
-                                   double limit, double &dep_out, int32_t &slot_out) {
-    uint32_t b = sv.csg_leaf_range[2 * root], e = sv.csg_leaf_range[2 * root + 1];
-    bool found = false;
-    double best = limit;
-    for (uint32_t k = b; k < e; k++) {
-        uint32_t slot = sv.csg_leaves[k];
-        const CsgChild &c = sv.csg_children[slot];
-        double hs[kMaxLeafHits];
-        int nh = leaf_hits(sv, c.type, c.index, o, d, hs);
-        for (int j = 0; j < nh; j++) {
-            double dep = hs[j];
-            if (!depth_ok(dep, shadow)) continue;
-            if (!(dep < best || (found && dep == best && (int32_t)slot < slot_out))) continue;
-            D3 ip = add3(o, mul3(d, dep));
-            if (csg_survives(sv, slot, ip)) {
-                best = dep;
-                slot_out = (int32_t)slot;
-                found = true;
-            }
-        }
-    }
-    dep_out = best;
-    return found;
-}
-
-// Closest valid hit of a quadric (Find_Intersection over its IStack).
-__device__ inline bool quadric_closest(const QuadricRec &q, D3 o, D3 d, bool shadow, double &dep_out) {
-    double hs[2];
-    const int nh = quadric_hits(q, o, d, hs);
-    bool found = false;
-    double best = 0.0;
-    for (int j = 0; j < nh; j++)
-        if (depth_ok(hs[j], shadow) && (!found || hs[j] < best)) {
-            best = hs[j];
-            found = true;
-        }
-    dep_out = best;
-    return found;
-}
-
-// Closest valid hit of a box / torus / stand-alone triangle object
-// (Find_Intersection over its IStack).
-__device__ inline bool simple_closest(const SceneView &sv, uint32_t kind, uint32_t index, D3 o,
-                                      D3 d, bool shadow, double &dep_out) {
-    if (kind == kPrimTriangle64 || kind == kPrimDisc || kind == kPrimPolygon) {
-        double dep;
-        const bool hit = kind == kPrimDisc      ? disc_hit(sv.discs[index], sv.xforms, o, d, dep)
-                         : kind == kPrimPolygon ? polygon_hit(sv.polygons[index], sv.poly_pts, sv.xforms, o, d, dep)
-                                                : tri_rec_hit(sv.triangles[index], o, d, dep);
-        if (!hit || !depth_ok(dep, shadow)) return false;
-        dep_out = dep;
-        return true;
-    }
-    double hs[kMaxLeafHits];
-    int side[4];
-    int nh = kind == kPrimBox    ? box_hits(sv.boxes[index], sv.xforms, o, d, hs, side)
-             : kind == kPrimCone ? cone_hits(sv.cones[index], sv.xforms, o, d, hs, side)
-                                 : torus_hits(sv.tori[index], sv.xforms, o, d, hs);
-    bool found = false;
-    double best = 0.0;
-    for (int j = 0; j < nh; j++) {
-        if (depth_ok(hs[j], shadow) && (!found || hs[j] < best)) {
-            best = hs[j];
-            found = true;
-        }
-    }
-    dep_out = best;
-    return found;
-}
-
-// kLevel 0: triangles and spheres; 1: also boxes; 2: also tori and CSG.
-// kWW: while-while traversal (triangle scenes, camera and bounce rays).
-// kMulti: rays of many scenes in one launch.  `n` is then 64 x the number of
-// chunks; chunk c = {scene, first ray, ray count <= 64} covers the wavefront
-// whose loop index is 64c..64c+63, so the scene (and its SceneView, read from
-// `views`) is uniform across the wavefront.  With `clocks`, lane 0 of each
-// wavefront records the wall clock at its start (minimum into clocks[0]) and
-// at the end of each chunk (maximum into clocks[1 + scene]).
-template <int kLevel, bool kFlags, bool kCount, bool kWW, bool kMulti = false>
-__device__ __forceinline__ void trace_closest_body(const SceneView &sv0, const SceneView *__restrict__ views,
-                                                   const uint4 *__restrict__ chunks,
-                                                   unsigned long long *__restrict__ clocks,
-                                                   const pvx_ray *__restrict__ rays,
-                                                   const double *__restrict__ max_depth, uint32_t kind,
-                                                   const uint8_t *__restrict__ kinds, uint32_t n,
-                                                   pvx_hit *__restrict__ hits, int32_t *__restrict__ spill,
-                                                   uint32_t *__restrict__ node_visits,
-                                                   uint32_t *__restrict__ prim_tests, int32_t *s_stack) {
-    const uint32_t lane = threadIdx.x;
-    const uint32_t gtid = blockIdx.x * kBlock + lane;
-    int32_t *my_spill = spill + (size_t)gtid * kStackSpill;
-    // kRayOcclusion on a light-ray launch: stop at the first hit that the
-    // caller counts as blocking (closer than max depth - SHADOW_TOLERANCE)
-    const bool occl = (kind & kRayOcclusion) != 0;
-    kind &= 0xFFu;
-    const bool shadow = (kind == PVX_RAY_SHADOW);
-    if constexpr (kMulti) {
-        if (clocks && (lane & 63u) == 0) atomicMin(&clocks[0], (unsigned long long)wall_clock64());
-    }
-
-    for (uint32_t i = gtid; i < n; i += gridDim.x * kBlock) {
-        uint32_t ri = i, scene = 0;
-        if constexpr (kMulti) {
-            const uint4 ch = chunks[__builtin_amdgcn_readfirstlane(i >> 6)];
-            scene = ch.x;
-            if ((i & 63u) >= ch.z) continue; // lane 0 always has a ray
-            ri = ch.y + (i & 63u);
-        }
-        const SceneView sv = kMulti ? views[scene] : sv0;
-        const pvx_ray r = rays[ri];
-        // per-ray kind (precondition only; shadow rays come in launches of their own)
-        const uint32_t rk = (kFlags && kinds) ? (uint32_t)kinds[ri] : kind;
-        D3 o = d3(r.o[0], r.o[1], r.o[2]);
-        D3 d = d3(r.d[0], r.d[1], r.d[2]);
-        double bt = kBoundHuge;
-        if (max_depth) {
-            double md = max_depth[ri];
-            if (md >= kEpsilon) bt = md; // trace.cpp:157-158
-        }
-        const double occl_lim = bt - kShadowTolerance;
-        int32_t bo = -1, bp = 0;
-
-        // objects outside the BVH (planes, unbounded CSG)
-        for (uint32_t k = 0; k < sv.n_inf; k++) {
-            const InfRec inf = sv.inf[k];
-            if (kFlags && !precondition(sv.obj_flags[inf.obj], rk)) continue;
-            double dep;
-            int32_t prim = 0;
-            bool ok;
-            if (inf.kind == kPrimPlane) {
-                ok = plane_hit(sv.planes[inf.index], o, d, dep) && depth_ok(dep, shadow);
-            } else if (kLevel >= 2) {
-                ok = inf.kind == kPrimQuadric ? quadric_closest(sv.quadrics[inf.index], o, d, shadow, dep)
-                                              : csg_closest(sv, inf.index, o, d, shadow, bt, dep, prim);
-            } else {
-                ok = false;
-            }
-            if (ok && better(dep, inf.obj, prim, bt, bo, bp)) {
-                bt = dep;
-                bo = inf.obj;
-                bp = prim;
-            }
-        }
-
-        RayF rf;
-        rf.ix = safe_inv(d.x);
-        rf.iy = safe_inv(d.y);
-        rf.iz = safe_inv(d.z);
-        rf.oix = (float)o.x * rf.ix;
-        rf.oiy = (float)o.y * rf.iy;
-        rf.oiz = (float)o.z * rf.iz;
-        float tb = (float)bt * 1.000001f;
-
-        int32_t node = (occl && bo >= 0 && bt < occl_lim) ? kEmpty : kLevel == 0 ? sv.root4 : sv.root;
-        int sp = 0;
-        uint32_t cnt_nodes = 0, cnt_prims = 0;
-        // Triangle / sphere scenes traverse the 4-wide tree; camera and
-        // bounce rays with while-while passes (the lanes of a wavefront visit
-        // inner nodes together, then leaves together), light rays one node or
-        // leaf per pass.  Scenes with boxes, tori and CSG, whose leaf tests
-        // are long, take the binary tree one step per pass.
-#define PVX_POP()                                                                       \
-    do {                                                                                \
-        if (sp == 0) {                                                                  \
-            node = kEmpty;                                                              \
-        } else {                                                                        \
-            sp--;                                                                       \
-            node = sp < kStackLds ? s_stack[sp * kBlock + lane] : my_spill[sp - kStackLds]; \
-        }                                                                               \
-    } while (0)
-        // one 4-wide inner-node step (triangle / sphere scenes): the hit
-        // children sorted by entry distance, the nearest taken, the others
-        // pushed farthest first; pops when no child is hit
-        auto step_node4 = [&]() {
-            if (kCount) cnt_nodes++;
-            const float4 *np = reinterpret_cast<const float4 *>(sv.nodes4 + node);
-            const float4 xl = np[0], xh = np[1], yl = np[2], yh = np[3], zl = np[4], zh = np[5];
-            const int4 cl = reinterpret_cast<const int4 *>(np)[6];
-            auto slab = [&](float lx, float hx, float ly, float hy, float lz, float hz, int32_t link) {
-                const float x0 = fmaf(lx, rf.ix, -rf.oix), x1 = fmaf(hx, rf.ix, -rf.oix);
-                const float y0 = fmaf(ly, rf.iy, -rf.oiy), y1 = fmaf(hy, rf.iy, -rf.oiy);
-                const float z0 = fmaf(lz, rf.iz, -rf.oiz), z1 = fmaf(hz, rf.iz, -rf.oiz);
-                const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
-                const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tb));
-                return (link != kEmpty && tn <= tf * 1.000001f) ? tn : INFINITY;
-            };
-            float d0 = slab(xl.x, xh.x, yl.x, yh.x, zl.x, zh.x, cl.x);
-            float d1 = slab(xl.y, xh.y, yl.y, yh.y, zl.y, zh.y, cl.y);
-            float d2 = slab(xl.z, xh.z, yl.z, yh.z, zl.z, zh.z, cl.z);
-            float d3 = slab(xl.w, xh.w, yl.w, yh.w, zl.w, zh.w, cl.w);
-            int32_t l0 = cl.x, l1 = cl.y, l2 = cl.z, l3 = cl.w;
-            auto cx = [](float &da, int32_t &la, float &db, int32_t &lb) {
-                if (db < da) {
-                    const float td = da;
-                    da = db;
-                    db = td;
-                    const int32_t tl = la;
-                    la = lb;
-                    lb = tl;
-                }
-            };
-            cx(d0, l0, d1, l1);
-            cx(d2, l2, d3, l3);
-            cx(d0, l0, d2, l2);
-            cx(d1, l1, d3, l3);
-            cx(d1, l1, d2, l2);
-            if (d0 == INFINITY) {
-                PVX_POP();
-                return;
-            }
-            auto push = [&](int32_t x) {
-                if (sp < kStackLds)
-                    s_stack[sp * kBlock + lane] = x;
-                else
-                    my_spill[sp - kStackLds] = x;
-                sp++;
-            };
-            if (d3 != INFINITY) push(l3);
-            if (d2 != INFINITY) push(l2);
-            if (d1 != INFINITY) push(l1);
-            node = l0;
-        };
-        // one leaf step of a triangle / sphere scene (tests its primitives, then pops)
-        auto step_leaf = [&]() {
-            // leaf: ~node = first << 3 | (count - 1)
-            const uint32_t leaf = (uint32_t)(~node);
-            const uint32_t first = leaf >> 3, count = (leaf & 7u) + 1u;
-            if (kCount) cnt_prims += count;
-            for (uint32_t k = 0; k < count; k++) {
-                const PrimRec &p = sv.prims[first + k];
-                const uint32_t ka = p.kind_axis;
-                const uint32_t pk = ka & 15u;
-                const int32_t obj = p.obj;
-                if (kFlags && !precondition(sv.obj_flags[obj], rk)) continue;
-                if (pk == kPrimTriangle) {
-                    double dep;
-                    if (tri_hit64(p.tri, ka >> 4, o, d, dep) && depth_ok(dep, shadow) &&
-                        better(dep, obj, p.tri.prim, bt, bo, bp)) {
-                        bt = dep;
-                        bo = obj;
-                        bp = p.tri.prim;
-                    }
-                } else if (pk == kPrimSphere) {
-                    double hs[2];
-                    int nh = sphere_hits(d3(p.sph.c[0], p.sph.c[1], p.sph.c[2]), p.sph.r, o, d, hs);
-                    for (int j = 0; j < nh; j++)
-                        if (depth_ok(hs[j], shadow) && better(hs[j], obj, 0, bt, bo, bp)) {
-                            bt = hs[j];
-                            bo = obj;
-                            bp = 0;
-                        }
-                }
-            }
-            tb = (float)bt * 1.000001f;
-            if (occl && bo >= 0 && bt < occl_lim) {
-                node = kEmpty;
-                return;
-            }
-            PVX_POP();
-        };
-        if constexpr (kLevel == 0) {
-            if constexpr (kWW) {
-                while (node != kEmpty) {
-                    while (node >= 0) step_node4();
-                    while (node < 0 && node != kEmpty) step_leaf();
-                }
-            } else {
-                while (node != kEmpty) {
-                    if (node >= 0)
-                        step_node4();
-                    else
-                        step_leaf();
-                }
-            }
-        } else {
-            while (node != kEmpty) {
-                if (node >= 0) {
-                    if (kCount) cnt_nodes++;
-                    const float4 *np = reinterpret_cast<const float4 *>(sv.nodes + node);
-                    const float4 a = np[0], b = np[1], c = np[2];
-                    const int4 l = reinterpret_cast<const int4 *>(np)[3];
-                    float x0 = fmaf(a.x, rf.ix, -rf.oix), x1 = fmaf(a.y, rf.ix, -rf.oix);
-                    float y0 = fmaf(a.z, rf.iy, -rf.oiy), y1 = fmaf(a.w, rf.iy, -rf.oiy);
-                    float z0 = fmaf(c.x, rf.iz, -rf.oiz), z1 = fmaf(c.y, rf.iz, -rf.oiz);
-                    float tn0 = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
-                    float tf0 = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tb));
-                    x0 = fmaf(b.x, rf.ix, -rf.oix);
-                    x1 = fmaf(b.y, rf.ix, -rf.oix);
-                    y0 = fmaf(b.z, rf.iy, -rf.oiy);
-                    y1 = fmaf(b.w, rf.iy, -rf.oiy);
-                    z0 = fmaf(c.z, rf.iz, -rf.oiz);
-                    z1 = fmaf(c.w, rf.iz, -rf.oiz);
-                    float tn1 = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
-                    float tf1 = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tb));
-                    bool h0 = tn0 <= tf0 * 1.000001f;
-                    bool h1 = tn1 <= tf1 * 1.000001f;
-                    if (h0 && h1) {
-                        int32_t nearc = l.x, farc = l.y;
-                        if (tn1 < tn0) {
-                            nearc = l.y;
-                            farc = l.x;
-                        }
-                        if (sp < kStackLds)
-                            s_stack[sp * kBlock + lane] = farc;
-                        else
-                            my_spill[sp - kStackLds] = farc;
-                        sp++;
-                        node = nearc;
-                        continue;
-                    }
-                    if (h0) {
-                        node = l.x;
-                        continue;
-                    }
-                    if (h1) {
-                        node = l.y;
-                        continue;
-                    }
-                } else {
-                    // leaf: ~node = first << 3 | (count - 1)
-                    const uint32_t leaf = (uint32_t)(~node);
-                    const uint32_t first = leaf >> 3, count = (leaf & 7u) + 1u;
-                    if (kCount) cnt_prims += count;
-                    for (uint32_t k = 0; k < count; k++) {
-                        const PrimRec &p = sv.prims[first + k];
-                        const uint32_t ka = p.kind_axis;
-                        const uint32_t pk = ka & 15u;
-                        const int32_t obj = p.obj;
-                        if (kFlags && !precondition(sv.obj_flags[obj], rk)) continue;
-                        if (pk == kPrimTriangle) {
-                            double dep;
-                            if (tri_hit64(p.tri, ka >> 4, o, d, dep) && depth_ok(dep, shadow) &&
-                                better(dep, obj, p.tri.prim, bt, bo, bp)) {
-                                bt = dep;
-                                bo = obj;
-                                bp = p.tri.prim;
-                            }
-                        } else if (pk == kPrimSphere) {
-                            double hs[2];
-                            int nh = sphere_hits(d3(p.sph.c[0], p.sph.c[1], p.sph.c[2]), p.sph.r, o, d, hs);
-                            for (int j = 0; j < nh; j++)
-                                if (depth_ok(hs[j], shadow) && better(hs[j], obj, 0, bt, bo, bp)) {
-                                    bt = hs[j];
-                                    bo = obj;
-                                    bp = 0;
-                                }
-                        } else if (kLevel == 1) { // boxes and stand-alone triangles
-                            if (pk == kPrimTriangle64) {
-                                double dep;
-                                if (tri_rec_hit(sv.triangles[p.ref.index], o, d, dep) && depth_ok(dep, shadow) &&
-                                    better(dep, obj, 0, bt, bo, bp)) {
-                                    bt = dep;
-                                    bo = obj;
-                                    bp = 0;
-                                }
-                            } else {
-                                double hs[2];
-                                int side[2];
-                                int nh = box_hits(sv.boxes[p.ref.index], sv.xforms, o, d, hs, side);
-                                for (int j = 0; j < nh; j++)
-                                    if (depth_ok(hs[j], shadow) && better(hs[j], obj, 0, bt, bo, bp)) {
-                                        bt = hs[j];
-                                        bo = obj;
-                                        bp = 0;
-                                    }
-                            }
-                        } else if (kLevel >= 2) {
-                            double dep;
-                            int32_t prim = 0;
-                            bool ok;
-                            if (pk == kPrimCsg)
-                                ok = csg_closest(sv, p.ref.index, o, d, shadow, bt, dep, prim);
-                            else
-                                ok = simple_closest(sv, pk, p.ref.index, o, d, shadow, dep);
-                            if (ok && better(dep, obj, prim, bt, bo, bp)) {
-                                bt = dep;
-                                bo = obj;
-                                bp = prim;
-                            }
-                        }
-                    }
-                    tb = (float)bt * 1.000001f;
-                    if (occl && bo >= 0 && bt < occl_lim) {
-                        node = kEmpty;
-                        continue;
-                    }
-                }
-                // pop
-                if (sp == 0) {
-                    node = kEmpty;
-                } else {
-                    sp--;
-                    node = sp < kStackLds ? s_stack[sp * kBlock + lane] : my_spill[sp - kStackLds];
-                }
-            }
-        }
-#undef PVX_POP
-        hits[ri] = pvx_hit{bt, bo, bp};
-        if (kCount) {
-            node_visits[ri] = cnt_nodes;
-            prim_tests[ri] = cnt_prims;
-        }
-        if constexpr (kMulti) {
-            if (clocks && (lane & 63u) == 0) atomicMax(&clocks[1 + scene], (unsigned long long)wall_clock64());
-        }
-    }
-}
-
-#define PVX_TRACE_ARGS                                                                                      \
-    SceneView sv, const pvx_ray *__restrict__ rays, const double *__restrict__ max_depth, uint32_t kind,    \
-        const uint8_t *__restrict__ kinds, uint32_t n, pvx_hit *__restrict__ hits,                          \
-        int32_t *__restrict__ spill, uint32_t *__restrict__ node_visits, uint32_t *__restrict__ prim_tests
-
-template <int kLevel, bool kFlags, bool kCount>
-__global__ __launch_bounds__(kBlock) void k_trace_closest(PVX_TRACE_ARGS) {
-    __shared__ int32_t s_stack[kStackLds * kBlock];
-    trace_closest_body<kLevel, kFlags, kCount, false>(sv, nullptr, nullptr, nullptr, rays, max_depth, kind, kinds, n,
-                                                      hits, spill, node_visits, prim_tests, s_stack);
-}
-
-// Triangle scenes, camera and bounce rays: while-while traversal (78 VGPRs
-// and a 24 KB LDS stack per block: 6 wavefronts per SIMD; forcing 7 or 8
-// costs more in register pressure than it gains).
-template <bool kFlags, bool kCount>
-__global__ __launch_bounds__(kBlock) void k_trace_closest_ww(
-    PVX_TRACE_ARGS) {
-    __shared__ int32_t s_stack[kStackLds * kBlock];
-    trace_closest_body<0, kFlags, kCount, true>(sv, nullptr, nullptr, nullptr, rays, max_depth, kind, kinds, n, hits,
-                                                spill, node_visits, prim_tests, s_stack);
-}
-#undef PVX_TRACE_ARGS
-
-// The same kernels over a batch of scenes (chunk table and SceneView table).
-#define PVX_MULTI_ARGS                                                                                      \
-    const SceneView *__restrict__ views, const uint4 *__restrict__ chunks,                                  \
-        unsigned long long *__restrict__ clocks, const pvx_ray *__restrict__ rays,                          \
-        const double *__restrict__ max_depth, uint32_t kind, const uint8_t *__restrict__ kinds, uint32_t n, \
-        pvx_hit *__restrict__ hits, int32_t *__restrict__ spill, uint32_t *__restrict__ node_visits,        \
-        uint32_t *__restrict__ prim_tests
-
-template <int kLevel, bool kFlags, bool kCount>
-__global__ __launch_bounds__(kBlock) void k_trace_closest_multi(PVX_MULTI_ARGS) {
-    __shared__ int32_t s_stack[kStackLds * kBlock];
-    trace_closest_body<kLevel, kFlags, kCount, false, true>(views[0], views, chunks, clocks, rays, max_depth, kind,
-                                                            kinds, n, hits, spill, node_visits, prim_tests, s_stack);
-}
-
-template <bool kFlags, bool kCount>
-__global__ __launch_bounds__(kBlock) void k_trace_closest_ww_multi(PVX_MULTI_ARGS) {
-    __shared__ int32_t s_stack[kStackLds * kBlock];
-    trace_closest_body<0, kFlags, kCount, true, true>(views[0], views, chunks, clocks, rays, max_depth, kind, kinds,
-                                                      n, hits, spill, node_visits, prim_tests, s_stack);
-}
-#undef PVX_MULTI_ARGS
-
 __global__ void k_clock_init(unsigned long long *clocks, uint32_t n) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) clocks[i] = i == 0 ? ~0ull : 0ull;
-}
-
-// Closest valid hit among one primitive record of any kind, folded into
-// (bt, bo, bp) as the leaf loops of k_trace_closest do.
-template <bool kFlags>
-__device__ __forceinline__ void prim_test_any(const SceneView &sv, const PrimRec &p, D3 o, D3 d, bool shadow,
-                                              uint32_t rk, double &bt, int32_t &bo, int32_t &bp) {
-    const uint32_t ka = p.kind_axis;
-    const uint32_t pk = ka & 15u;
-    const int32_t obj = p.obj;
-    if (kFlags && !precondition(sv.obj_flags[obj], rk)) return;
-    if (pk == kPrimTriangle) {
-        double dep;
-        if (tri_hit64(p.tri, ka >> 4, o, d, dep) && depth_ok(dep, shadow) && better(dep, obj, p.tri.prim, bt, bo, bp)) {
-            bt = dep;
-            bo = obj;
-            bp = p.tri.prim;
-        }
-    } else if (pk == kPrimSphere) {
-        double hs[2];
-        int nh = sphere_hits(d3(p.sph.c[0], p.sph.c[1], p.sph.c[2]), p.sph.r, o, d, hs);
-        for (int j = 0; j < nh; j++)
-            if (depth_ok(hs[j], shadow) && better(hs[j], obj, 0, bt, bo, bp)) {
-                bt = hs[j];
-                bo = obj;
-                bp = 0;
-            }
-    } else {
-        double dep;
-        int32_t prim = 0;
-        bool ok;
-        if (pk == kPrimCsg)
-            ok = csg_closest(sv, p.ref.index, o, d, shadow, bt, dep, prim);
-        else
-            ok = simple_closest(sv, pk, p.ref.index, o, d, shadow, dep);
-        if (ok && better(dep, obj, prim, bt, bo, bp)) {
-            bt = dep;
-            bo = obj;
-            bp = prim;
-        }
-    }
-}
-
-// Closest hit through the kd-tree (BSPTree::operator()(ray),
-// core/bounding/bsptree.cpp:128-226): the ray's range inside the tree's box,
-// the near child first with the far one remembered with its [entry, exit)
-// range, every primitive of a reached leaf tested, and a stop once the next
-// range starts beyond the closest hit.  Each primitive is decided as in
-// k_trace_closest; one listed in several leaves is tested again (the
-// reference's mailbox only saves that work).
-template <bool kFlags, bool kCount>
-__global__ __launch_bounds__(kBlock) void k_trace_closest_kd(SceneView sv, KdView kd, const pvx_ray *__restrict__ rays,
-                                                           const double *__restrict__ max_depth, uint32_t kind,
-                                                           const uint8_t *__restrict__ kinds, uint32_t n,
-                                                           pvx_hit *__restrict__ hits, uint32_t *__restrict__ node_visits,
-                                                           uint32_t *__restrict__ prim_tests) {
-    struct Entry {
-        uint32_t node;
-        float rentry, rexit;
-    };
-    kind &= 0xFFu; // closest hits only (the occlusion bit is a hint)
-    const bool shadow = (kind == PVX_RAY_SHADOW);
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-        const pvx_ray r = rays[i];
-        const uint32_t rk = (kFlags && kinds) ? (uint32_t)kinds[i] : kind;
-        const D3 o = d3(r.o[0], r.o[1], r.o[2]), d = d3(r.d[0], r.d[1], r.d[2]);
-        double bt = kBoundHuge;
-        if (max_depth) {
-            const double md = max_depth[i];
-            if (md >= kEpsilon) bt = md;
-        }
-        int32_t bo = -1, bp = 0;
-        uint32_t cn = 0, cp = 0;
-        for (uint32_t k = 0; k < sv.n_inf; k++) {
-            const InfRec inf = sv.inf[k];
-            if (kFlags && !precondition(sv.obj_flags[inf.obj], rk)) continue;
-            double dep;
-            int32_t prim = 0;
-            bool ok = inf.kind == kPrimPlane     ? (plane_hit(sv.planes[inf.index], o, d, dep) && depth_ok(dep, shadow))
-                      : inf.kind == kPrimQuadric ? quadric_closest(sv.quadrics[inf.index], o, d, shadow, dep)
-                                                 : csg_closest(sv, inf.index, o, d, shadow, bt, dep, prim);
-            if (ok && better(dep, inf.obj, prim, bt, bo, bp)) {
-                bt = dep;
-                bo = inf.obj;
-                bp = prim;
-            }
-        }
-        // the ray's range inside the tree's box (FP64 slabs)
-        double t0 = -INFINITY, t1 = INFINITY;
-        bool inbox = true;
-        for (int a = 0; a < 3; a++) {
-            const double oa = comp(o, a), da = comp(d, a), lo = kd.bmin[a], hi = kd.bmax[a];
-            if (da == 0.0) {
-                if (oa < lo || oa > hi) inbox = false;
-                continue;
-            }
-            double ta = (lo - oa) / da, tb2 = (hi - oa) / da;
-            if (ta > tb2) {
-                const double x = ta;
-                ta = tb2;
-                tb2 = x;
-            }
-            t0 = fmax(t0, ta);
-            t1 = fmin(t1, tb2);
-        }
-        if (inbox && t0 <= t1 && t1 >= 0.0) {
-            double rentry = fmax(t0, 0.0), rexit = t1;
-            Entry st[kKdMaxDepth + 1];
-            int sp = 0;
-            uint32_t inode = 0;
-            while (rentry < bt) {
-                const KdNode nd = kd.nodes[inode];
-                if (!(nd.info >> 31)) { // split
-                    if (kCount) cn++;
-                    const uint32_t axis = (nd.info >> 29) & 3u;
-                    uint32_t left = nd.info & 0x1FFFFFFFu, right = left + 1;
-                    const float plane = nd.u.plane;
-                    const double oa = comp(o, axis), da = comp(d, axis);
-                    if (da == 0.0) { // parallel to the plane: one side, or both when on it
-                        if (oa == (double)plane) {
-                            st[sp++] = Entry{right, (float)rentry, (float)rexit};
-                            inode = left;
-                        } else {
-                            inode = oa > (double)plane ? right : left;
-                        }
-                        continue;
-                    }
-                    const float rdist = (float)(((double)plane - oa) / da);
-                    if ((oa > (double)plane) || ((rdist == 0.0f) && (da < 0))) {
-                        const uint32_t x = left;
-                        left = right;
-                        right = x;
-                    }
-                    if ((rdist < 0.0f) || (rdist > rexit)) {
-                        inode = left;
-                    } else if (rdist < rentry) {
-                        inode = right;
-                    } else {
-                        st[sp++] = Entry{right, rdist, (float)rexit};
-                        inode = left;
-                        rexit = rdist;
-                    }
-                } else { // leaf
-                    const uint32_t count = nd.info & 0x7FFFFFFFu, first = nd.u.first;
-                    if (kCount) cp += count;
-                    for (uint32_t k = 0; k < count; k++)
-                        prim_test_any<kFlags>(sv, sv.prims[kd.list[first + k]], o, d, shadow, rk, bt, bo, bp);
-                    if (sp == 0) break;
-                    sp--;
-                    inode = st[sp].node;
-                    rentry = st[sp].rentry;
-                    rexit = st[sp].rexit;
-                }
-            }
-        }
-        hits[i] = pvx_hit{bt, bo, bp};
-        if (kCount) {
-            node_visits[i] = cn;
-            prim_tests[i] = cp;
-        }
-    }
 }
 
 // Perspective camera rays (tracepixel.cpp:357-372, 918-926).
@@ -729,114 +69,6 @@ __global__ void k_camera_rays(pvx_camera cam, pvx_ray *rays) {
     rays[(size_t)y * cam.width + x] = r;
 }
 
-// All intersections of one object per ray, sorted ascending
-// (ObjectBase::All_Intersections, object.h:238).
-__global__ void k_all_intersections(SceneView sv, const pvx_ray *rays, const uint32_t *objs,
-                                    uint32_t n, uint32_t max_hits, double *depths, int32_t *prims,
-                                    uint32_t *counts, const uint32_t *obj_type,
-                                    const uint32_t *obj_index) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const pvx_ray r = rays[i];
-    D3 o = d3(r.o[0], r.o[1], r.o[2]);
-    D3 d = d3(r.d[0], r.d[1], r.d[2]);
-    uint32_t ob = objs[i];
-    uint32_t type = obj_type[ob], index = obj_index[ob];
-    double buf[16];
-    int32_t pbuf[16];
-    uint32_t cnt = 0;
-    auto push = [&](double dep, int32_t prim) {
-        if (cnt < 16) {
-            // insertion sort keeps the list ascending
-            uint32_t j = cnt;
-            while (j > 0 && buf[j - 1] > dep) {
-                buf[j] = buf[j - 1];
-                pbuf[j] = pbuf[j - 1];
-                j--;
-            }
-            buf[j] = dep;
-            pbuf[j] = prim;
-        }
-        cnt++;
-    };
-    if (type == PVX_PLANE) {
-        double dep;
-        if (plane_hit(sv.planes[index], o, d, dep)) push(dep, 0);
-    } else if (type == PVX_CSG) {
-        uint32_t b = sv.csg_leaf_range[2 * index], e = sv.csg_leaf_range[2 * index + 1];
-        for (uint32_t k = b; k < e; k++) {
-            uint32_t slot = sv.csg_leaves[k];
-            const CsgChild &c = sv.csg_children[slot];
-            double hs[kMaxLeafHits];
-            int nh = leaf_hits(sv, c.type, c.index, o, d, hs);
-            for (int j = 0; j < nh; j++)
-                if (csg_survives(sv, slot, add3(o, mul3(d, hs[j])))) push(hs[j], (int32_t)slot);
-        }
-    } else if (type == PVX_MESH) {
-        // every triangle of the mesh: walk the whole BVH without culling by depth
-        int32_t stack[kStackMax];
-        int sp = 0;
-        int32_t node = sv.root;
-        float ix = safe_inv(d.x), iy = safe_inv(d.y), iz = safe_inv(d.z);
-        float oix = (float)o.x * ix, oiy = (float)o.y * iy, oiz = (float)o.z * iz;
-        while (node != kEmpty) {
-            if (node >= 0) {
-                const Node &nd = sv.nodes[node];
-                const float lo0[3] = {nd.lo0x, nd.lo0y, nd.lo0z}, hi0[3] = {nd.hi0x, nd.hi0y, nd.hi0z};
-                const float lo1[3] = {nd.lo1x, nd.lo1y, nd.lo1z}, hi1[3] = {nd.hi1x, nd.hi1y, nd.hi1z};
-                const float iv[3] = {ix, iy, iz}, oi[3] = {oix, oiy, oiz};
-                float tn0 = 0.0f, tf0 = 3.0e38f, tn1 = 0.0f, tf1 = 3.0e38f;
-                for (int k = 0; k < 3; k++) {
-                    float a0 = fmaf(lo0[k], iv[k], -oi[k]), b0 = fmaf(hi0[k], iv[k], -oi[k]);
-                    float a1 = fmaf(lo1[k], iv[k], -oi[k]), b1 = fmaf(hi1[k], iv[k], -oi[k]);
-                    tn0 = fmaxf(tn0, fminf(a0, b0));
-                    tf0 = fminf(tf0, fmaxf(a0, b0));
-                    tn1 = fmaxf(tn1, fminf(a1, b1));
-                    tf1 = fminf(tf1, fmaxf(a1, b1));
-                }
-                bool h0 = tn0 <= tf0 * 1.000001f, h1 = tn1 <= tf1 * 1.000001f;
-                if (h0 && h1 && sp < kStackMax) {
-                    stack[sp++] = nd.c1;
-                    node = nd.c0;
-                    continue;
-                }
-                if (h0) { node = nd.c0; continue; }
-                if (h1) { node = nd.c1; continue; }
-            } else {
-                uint32_t leaf = (uint32_t)(~node), first = leaf >> 3, count = (leaf & 7u) + 1u;
-                for (uint32_t k = 0; k < count; k++) {
-                    const PrimRec &p = sv.prims[first + k];
-                    if ((p.kind_axis & 15u) != kPrimTriangle || p.obj != (int32_t)ob) continue;
-                    double dep;
-                    if (tri_hit64(p.tri, p.kind_axis >> 4, o, d, dep)) push(dep, p.tri.prim);
-                }
-            }
-            node = sp == 0 ? kEmpty : stack[--sp];
-        }
-    } else {
-        double hs[kMaxLeafHits];
-        int nh = leaf_hits(sv, type, index, o, d, hs);
-        for (int j = 0; j < nh; j++) push(hs[j], 0);
-    }
-    counts[i] = cnt;
-    for (uint32_t k = 0; k < max_hits; k++) {
-        bool have = k < cnt && k < 16;
-        depths[(size_t)i * max_hits + k] = have ? buf[k] : HUGE_VAL;
-        prims[(size_t)i * max_hits + k] = have ? pbuf[k] : -1;
-    }
-}
-
-// Inside_Object per (point, object) (object.cpp:346-355).
-__global__ void k_inside(SceneView sv, const double *pts, const uint32_t *objs, uint32_t n,
-                         uint8_t *out, const uint32_t *obj_type, const uint32_t *obj_index,
-                         const uint32_t *obj_flags) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    D3 p = d3(pts[3 * (size_t)i], pts[3 * (size_t)i + 1], pts[3 * (size_t)i + 2]);
-    uint32_t ob = objs[i];
-    out[i] = shape_inside<kCsgMaxDepth>(sv, obj_type[ob], obj_index[ob], obj_flags[ob], p) ? 1 : 0;
-}
-
 __global__ void k_copy16(uint4 *__restrict__ dst, const uint4 *__restrict__ src, size_t n16) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16;
          i += (size_t)gridDim.x * blockDim.x)
@@ -844,6 +76,32 @@ __global__ void k_copy16(uint4 *__restrict__ dst, const uint4 *__restrict__ src,
 }
 
 } // namespace
+
+// The part of trace_kernels.hip that holds kernel `k`.
+hipError_t pvx::trace_launch(const TraceKernel &k, const TraceArgs &a, uint32_t nb, hipStream_t st, int *occ) {
+    if (!k.multi) {
+        switch (k.walk) {
+        case kWalkStep4:
+        case kWalkWW4: return trace_part0(k, a, nb, st, occ);
+        case kWalkBinary: return trace_part1(k, a, nb, st, occ);
+        case kWalkComplex:
+            return (k.flags ? (k.count ? trace_part5 : trace_part4) : (k.count ? trace_part3 : trace_part2))(k, a, nb, st,
+                                                                                                          occ);
+        case kWalkKd: return trace_part6(k, a, nb, st, occ);
+        default: return hipErrorInvalidValue;
+        }
+    }
+    switch (k.walk) {
+    case kWalkStep4:
+    case kWalkWW4:
+    case kWalkBinary: return trace_part7(k, a, nb, st, occ);
+    case kWalkComplex:
+        return (k.flags ? (k.count ? trace_part11 : trace_part10) : (k.count ? trace_part9 : trace_part8))(k, a, nb, st,
+                                                                                                        occ);
+    default: return hipErrorInvalidValue;
+    }
+}
+
 
 template <class T>
 static int upload(pvx_scene *s, const std::vector<T> &v, const T **dst) {
@@ -1043,13 +301,13 @@ extern "C" int pvx_scene_create(const pvx_scene_desc *desc, const pvx_build_opts
     int cus = 0;
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device));
     int per_cu = 0;
-    auto pick = [](uint32_t level, bool flags) -> const void * {
-        if (level >= 2) return flags ? (const void *)k_trace_closest<2, true, false> : (const void *)k_trace_closest<2, false, false>;
-        if (level == 1) return flags ? (const void *)k_trace_closest<1, true, false> : (const void *)k_trace_closest<1, false, false>;
-        return flags ? (const void *)k_trace_closest_ww<true, false> : (const void *)k_trace_closest_ww<false, false>;
-    };
-    auto kfn = pick(kernel_level(v), h.any_flags != 0);
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, kBlock, 0));
+    {
+        const uint32_t level = kernel_level(v);
+        const TraceKernel k{level >= 2 ? kWalkComplex : level == 1 ? kWalkBinary : kWalkWW4, h.any_flags != 0, false,
+                            false};
+        TraceArgs none{};
+        HIP_TRY(trace_launch(k, none, 0, nullptr, &per_cu));
+    }
     if (per_cu < 1) per_cu = 1;
     s->grid = (uint32_t)(cus * per_cu);
     s->spill_bytes = (size_t)2 * s->grid * kBlock * kStackSpill * sizeof(int32_t); // grids up to 2x (autotuner)
@@ -1105,38 +363,35 @@ extern "C" int pvx_scene_get_stats(const pvx_scene *s, pvx_scene_stats *out) {
 template <bool kCount>
 static int launch_variant(pvx_scene *s, const pvx_ray *rays, const double *max_depth, uint32_t ray_kind,
                           const uint8_t *kinds, size_t n, pvx_hit *hits, uint32_t *nv, uint32_t *pt,
-                          hipStream_t st, uint32_t variant, uint32_t grid) {
-    int32_t *spill = nullptr;
+                          hipStream_t st, uint32_t variant, uint32_t grid, int32_t *spill = nullptr) {
     int r;
-    if ((r = scene_spill(s, st, &spill))) return r;
+    if (!spill && (r = scene_spill(s, st, &spill))) return r;
     uint32_t nblocks = (uint32_t)std::min<size_t>(grid, (n + kBlock - 1) / kBlock);
-    bool flags = s->view.any_flags != 0 && (ray_kind != PVX_RAY_PRIMARY || kinds);
-    uint32_t nn = (uint32_t)n;
-#define PVX_LAUNCH(L, F) \
-    k_trace_closest<L, F, kCount><<<nblocks, kBlock, 0, st>>>(s->view, rays, max_depth, ray_kind, kinds, nn, hits, spill, nv, pt)
-#define PVX_LAUNCH_WW(L, F) \
-    k_trace_closest_ww<F, kCount><<<nblocks, kBlock, 0, st>>>(s->view, rays, max_depth, ray_kind, kinds, nn, hits, spill, nv, pt)
+    const bool flags = s->view.any_flags != 0 && (ray_kind != PVX_RAY_PRIMARY || kinds);
+    TraceKernel k{kWalkStep4, flags, kCount, false};
     const uint32_t level = kernel_level(s->view);
-    if (s->use_kd) {
-        if (flags)
-            k_trace_closest_kd<true, kCount><<<nblocks, kBlock, 0, st>>>(s->view, s->kd, rays, max_depth, ray_kind,
-                                                                        kinds, nn, hits, nv, pt);
-        else
-            k_trace_closest_kd<false, kCount><<<nblocks, kBlock, 0, st>>>(s->view, s->kd, rays, max_depth, ray_kind,
-                                                                         kinds, nn, hits, nv, pt);
-    } else if (level >= 2) {
-        if (flags) PVX_LAUNCH(2, true); else PVX_LAUNCH(2, false);
-    } else if (level == 1 || variant == PVX_TRAVERSAL_BINARY) {
-        if (flags) PVX_LAUNCH(1, true); else PVX_LAUNCH(1, false);
-    } else if (variant == PVX_TRAVERSAL_WIDE_WW ||
-               (variant == PVX_TRAVERSAL_DEFAULT && (ray_kind & 0xFFu) != PVX_RAY_SHADOW)) {
-        if (flags) PVX_LAUNCH_WW(0, true); else PVX_LAUNCH_WW(0, false);
-    } else {
-        if (flags) PVX_LAUNCH(0, true); else PVX_LAUNCH(0, false);
-    }
-#undef PVX_LAUNCH
-#undef PVX_LAUNCH_WW
-    HIP_TRY(hipGetLastError());
+    if (s->use_kd)
+        k.walk = kWalkKd;
+    else if (level >= 2)
+        k.walk = kWalkComplex;
+    else if (level == 1 || variant == PVX_TRAVERSAL_BINARY)
+        k.walk = kWalkBinary;
+    else if (variant == PVX_TRAVERSAL_WIDE_WW ||
+             (variant == PVX_TRAVERSAL_DEFAULT && (ray_kind & 0xFFu) != PVX_RAY_SHADOW))
+        k.walk = kWalkWW4;
+    TraceArgs ta{};
+    ta.sv = s->view;
+    ta.kd = s->kd;
+    ta.rays = rays;
+    ta.max_depth = max_depth;
+    ta.kind = ray_kind;
+    ta.kinds = kinds;
+    ta.n = (uint32_t)n;
+    ta.hits = hits;
+    ta.spill = spill;
+    ta.node_visits = nv;
+    ta.prim_tests = pt;
+    HIP_TRY(trace_launch(k, ta, nblocks, st));
     return PVX_OK;
 }
 
@@ -1308,17 +563,12 @@ extern "C" int pvx_batch_create(pvx_scene *const *scenes, uint32_t n_scenes, pvx
     }
     // persistent grids of the two kernel families the batch can launch
     int cus = 0, per_ww = 0, per = 0;
-    const void *kfn = level >= 2   ? (flags ? (const void *)k_trace_closest_multi<2, true, false>
-                                            : (const void *)k_trace_closest_multi<2, false, false>)
-                      : level == 1 ? (flags ? (const void *)k_trace_closest_multi<1, true, false>
-                                            : (const void *)k_trace_closest_multi<1, false, false>)
-                                   : (flags ? (const void *)k_trace_closest_multi<0, true, false>
-                                            : (const void *)k_trace_closest_multi<0, false, false>);
-    const void *kww = flags ? (const void *)k_trace_closest_ww_multi<true, false>
-                            : (const void *)k_trace_closest_ww_multi<false, false>;
+    const TraceKernel kb{level >= 2 ? kWalkComplex : level == 1 ? kWalkBinary : kWalkStep4, flags != 0, false, true};
+    const TraceKernel kw{kWalkWW4, flags != 0, false, true};
+    TraceArgs none{};
     e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kfn, kBlock, 0);
-    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_ww, kww, kBlock, 0);
+    if (e == hipSuccess) e = trace_launch(kb, none, 0, nullptr, &per);
+    if (e == hipSuccess) e = trace_launch(kw, none, 0, nullptr, &per_ww);
     if (e != hipSuccess) {
         pvx_batch_destroy(b);
         return fail(PVX_EDEVICE, std::string("batch occupancy: ") + hipGetErrorString(e));
@@ -1402,24 +652,21 @@ static int launch_batch(pvx_batch *b, const pvx_ray *rays, const uint64_t *first
     const SceneView *views = static_cast<const SceneView *>(b->views.p);
     const uint4 *chunks = static_cast<const uint4 *>(b->chunks.p);
     int32_t *spill = static_cast<int32_t *>(b->spill.p);
-#define PVX_LAUNCH(L, F)                                                                                     \
-    k_trace_closest_multi<L, F, kCount><<<nblocks, kBlock, 0, st>>>(views, chunks, clk, rays, max_depth,     \
-                                                                    ray_kind, nullptr, n, hits, spill, nv, pt)
-#define PVX_LAUNCH_WW(F)                                                                                     \
-    k_trace_closest_ww_multi<F, kCount><<<nblocks, kBlock, 0, st>>>(views, chunks, clk, rays, max_depth,     \
-                                                                    ray_kind, nullptr, n, hits, spill, nv, pt)
-    if (b->level >= 2) {
-        if (flags) PVX_LAUNCH(2, true); else PVX_LAUNCH(2, false);
-    } else if (b->level == 1) {
-        if (flags) PVX_LAUNCH(1, true); else PVX_LAUNCH(1, false);
-    } else if (ww) {
-        if (flags) PVX_LAUNCH_WW(true); else PVX_LAUNCH_WW(false);
-    } else {
-        if (flags) PVX_LAUNCH(0, true); else PVX_LAUNCH(0, false);
-    }
-#undef PVX_LAUNCH
-#undef PVX_LAUNCH_WW
-    HIP_TRY(hipGetLastError());
+    const TraceKernel k{b->level >= 2 ? kWalkComplex : b->level == 1 ? kWalkBinary : ww ? kWalkWW4 : kWalkStep4, flags,
+                        kCount, true};
+    TraceArgs ta{};
+    ta.views = views;
+    ta.chunks = chunks;
+    ta.clocks = clk;
+    ta.rays = rays;
+    ta.max_depth = max_depth;
+    ta.kind = ray_kind;
+    ta.n = n;
+    ta.hits = hits;
+    ta.spill = spill;
+    ta.node_visits = nv;
+    ta.prim_tests = pt;
+    HIP_TRY(trace_launch(k, ta, nblocks, st));
     return PVX_OK;
 }
 
@@ -1460,10 +707,8 @@ extern "C" int pvx_all_intersections(pvx_scene *s, const pvx_ray *rays, const ui
         return fail(PVX_EINVAL, "null buffer or max_hits == 0");
     if (n > 0xFFFFFFFFull) return fail(PVX_EINVAL, "too many rays in one call");
     hipStream_t st = static_cast<hipStream_t>(stream);
-    uint32_t nb = (uint32_t)((n + 127) / 128);
-    k_all_intersections<<<nb, 128, 0, st>>>(s->view, rays, obj, (uint32_t)n, max_hits, depths,
-                                            prims, counts, s->d_obj_type, s->d_obj_index);
-    HIP_TRY(hipGetLastError());
+    HIP_TRY(launch_all_intersections(s->view, rays, obj, (uint32_t)n, max_hits, depths, prims, counts,
+                                     s->d_obj_type, s->d_obj_index, st));
     return PVX_OK;
 }
 
@@ -1474,10 +719,8 @@ extern "C" int pvx_inside(pvx_scene *s, const double *points, const uint32_t *ob
     if (!points || !obj || !out) return fail(PVX_EINVAL, "null buffer");
     if (n > 0xFFFFFFFFull) return fail(PVX_EINVAL, "too many points in one call");
     hipStream_t st = static_cast<hipStream_t>(stream);
-    uint32_t nb = (uint32_t)((n + 255) / 256);
-    k_inside<<<nb, 256, 0, st>>>(s->view, points, obj, (uint32_t)n, out, s->d_obj_type,
-                                 s->d_obj_index, s->view.obj_flags);
-    HIP_TRY(hipGetLastError());
+    HIP_TRY(launch_inside(s->view, points, obj, (uint32_t)n, out, s->d_obj_type, s->d_obj_index,
+                          s->view.obj_flags, st));
     return PVX_OK;
 }
 

@@ -1,0 +1,349 @@
+// Traversal kernels: closest hit over the BVH (one-lane-per-ray persistent
+// grids), over the kd-tree, and the per-object All_Intersections / Inside
+// queries.  This file is compiled once per part (-DPVX_TRACE_PART=n, see the
+// Makefile); each part instantiates a few of the kernel templates, and
+// trace_launch() in pvx.hip calls the part that holds the kernel asked for.
+#include <hip/hip_runtime.h>
+
+#include "../../include/pvx.h"
+#include "trace_body.h"
+#include "trace_launch.h"
+
+#ifndef PVX_TRACE_PART
+#error "build with -DPVX_TRACE_PART=<0..12>"
+#endif
+
+namespace pvx {
+namespace {
+
+template <int kLevel, bool kFlags, bool kCount>
+__global__ __launch_bounds__(kBlock) void k_trace_closest(TraceArgs a) {
+    __shared__ int32_t s_stack[kStackLds * kBlock];
+    trace_closest_body<kLevel, kFlags, kCount, false>(a.sv, nullptr, nullptr, nullptr, a.rays, a.max_depth, a.kind,
+                                                      a.kinds, a.n, a.hits, a.spill, a.node_visits, a.prim_tests,
+                                                      s_stack);
+}
+
+// Triangle scenes, camera and bounce rays: while-while traversal (78 VGPRs
+// and a 24 KB LDS stack per block: 6 wavefronts per SIMD; forcing 7 or 8
+// costs more in register pressure than it gains).
+template <bool kFlags, bool kCount>
+__global__ __launch_bounds__(kBlock) void k_trace_closest_ww(TraceArgs a) {
+    __shared__ int32_t s_stack[kStackLds * kBlock];
+    trace_closest_body<0, kFlags, kCount, true>(a.sv, nullptr, nullptr, nullptr, a.rays, a.max_depth, a.kind,
+                                                a.kinds, a.n, a.hits, a.spill, a.node_visits, a.prim_tests, s_stack);
+}
+
+// The same kernels over a batch of scenes (chunk table and SceneView table).
+template <int kLevel, bool kFlags, bool kCount>
+__global__ __launch_bounds__(kBlock) void k_trace_closest_multi(TraceArgs a) {
+    __shared__ int32_t s_stack[kStackLds * kBlock];
+    trace_closest_body<kLevel, kFlags, kCount, false, true>(a.views[0], a.views, a.chunks, a.clocks, a.rays,
+                                                            a.max_depth, a.kind, a.kinds, a.n, a.hits, a.spill,
+                                                            a.node_visits, a.prim_tests, s_stack);
+}
+
+template <bool kFlags, bool kCount>
+__global__ __launch_bounds__(kBlock) void k_trace_closest_ww_multi(TraceArgs a) {
+    __shared__ int32_t s_stack[kStackLds * kBlock];
+    trace_closest_body<0, kFlags, kCount, true, true>(a.views[0], a.views, a.chunks, a.clocks, a.rays, a.max_depth,
+                                                      a.kind, a.kinds, a.n, a.hits, a.spill, a.node_visits,
+                                                      a.prim_tests, s_stack);
+}
+
+// Closest hit through the kd-tree (BSPTree::operator()(ray),
+// core/bounding/bsptree.cpp:128-226): the ray's range inside the tree's box,
+// the near child first with the far one remembered with its [entry, exit)
+// range, every primitive of a reached leaf tested, and a stop once the next
+// range starts beyond the closest hit.  Each primitive is decided as in
+// k_trace_closest; one listed in several leaves is tested again (the
+// reference's mailbox only saves that work).
+template <bool kFlags, bool kCount>
+__global__ __launch_bounds__(kBlock) void k_trace_closest_kd(TraceArgs a) {
+    const SceneView &sv = a.sv;
+    const KdView &kd = a.kd;
+    const pvx_ray *__restrict__ rays = a.rays;
+    const double *__restrict__ max_depth = a.max_depth;
+    uint32_t kind = a.kind;
+    const uint8_t *__restrict__ kinds = a.kinds;
+    const uint32_t n = a.n;
+    pvx_hit *__restrict__ hits = a.hits;
+    uint32_t *__restrict__ node_visits = a.node_visits;
+    uint32_t *__restrict__ prim_tests = a.prim_tests;
+    struct Entry {
+        uint32_t node;
+        float rentry, rexit;
+    };
+    kind &= 0xFFu; // closest hits only (the occlusion bit is a hint)
+    const bool shadow = (kind == PVX_RAY_SHADOW);
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const pvx_ray r = rays[i];
+        const uint32_t rk = (kFlags && kinds) ? (uint32_t)kinds[i] : kind;
+        const D3 o = d3(r.o[0], r.o[1], r.o[2]), d = d3(r.d[0], r.d[1], r.d[2]);
+        double bt = kBoundHuge;
+        if (max_depth) {
+            const double md = max_depth[i];
+            if (md >= kEpsilon) bt = md;
+        }
+        int32_t bo = -1, bp = 0;
+        uint32_t cn = 0, cp = 0;
+        for (uint32_t k = 0; k < sv.n_inf; k++) {
+            const InfRec inf = sv.inf[k];
+            if (kFlags && !precondition(sv.obj_flags[inf.obj], rk)) continue;
+            double dep;
+            int32_t prim = 0;
+            bool ok = inf.kind == kPrimPlane     ? (plane_hit(sv.planes[inf.index], o, d, dep) && depth_ok(dep, shadow))
+                      : inf.kind == kPrimQuadric ? quadric_closest(sv.quadrics[inf.index], o, d, shadow, dep)
+                                                 : csg_closest(sv, inf.index, o, d, shadow, bt, dep, prim);
+            if (ok && better(dep, inf.obj, prim, bt, bo, bp)) {
+                bt = dep;
+                bo = inf.obj;
+                bp = prim;
+            }
+        }
+        // the ray's range inside the tree's box (FP64 slabs)
+        double t0 = -INFINITY, t1 = INFINITY;
+        bool inbox = true;
+        for (int a = 0; a < 3; a++) {
+            const double oa = comp(o, a), da = comp(d, a), lo = kd.bmin[a], hi = kd.bmax[a];
+            if (da == 0.0) {
+                if (oa < lo || oa > hi) inbox = false;
+                continue;
+            }
+            double ta = (lo - oa) / da, tb2 = (hi - oa) / da;
+            if (ta > tb2) {
+                const double x = ta;
+                ta = tb2;
+                tb2 = x;
+            }
+            t0 = fmax(t0, ta);
+            t1 = fmin(t1, tb2);
+        }
+        if (inbox && t0 <= t1 && t1 >= 0.0) {
+            double rentry = fmax(t0, 0.0), rexit = t1;
+            Entry st[kKdMaxDepth + 1];
+            int sp = 0;
+            uint32_t inode = 0;
+            while (rentry < bt) {
+                const KdNode nd = kd.nodes[inode];
+                if (!(nd.info >> 31)) { // split
+                    if (kCount) cn++;
+                    const uint32_t axis = (nd.info >> 29) & 3u;
+                    uint32_t left = nd.info & 0x1FFFFFFFu, right = left + 1;
+                    const float plane = nd.u.plane;
+                    const double oa = comp(o, axis), da = comp(d, axis);
+                    if (da == 0.0) { // parallel to the plane: one side, or both when on it
+                        if (oa == (double)plane) {
+                            st[sp++] = Entry{right, (float)rentry, (float)rexit};
+                            inode = left;
+                        } else {
+                            inode = oa > (double)plane ? right : left;
+                        }
+                        continue;
+                    }
+                    const float rdist = (float)(((double)plane - oa) / da);
+                    if ((oa > (double)plane) || ((rdist == 0.0f) && (da < 0))) {
+                        const uint32_t x = left;
+                        left = right;
+                        right = x;
+                    }
+                    if ((rdist < 0.0f) || (rdist > rexit)) {
+                        inode = left;
+                    } else if (rdist < rentry) {
+                        inode = right;
+                    } else {
+                        st[sp++] = Entry{right, rdist, (float)rexit};
+                        inode = left;
+                        rexit = rdist;
+                    }
+                } else { // leaf
+                    const uint32_t count = nd.info & 0x7FFFFFFFu, first = nd.u.first;
+                    if (kCount) cp += count;
+                    for (uint32_t k = 0; k < count; k++)
+                        prim_test_any<kFlags>(sv, sv.prims[kd.list[first + k]], o, d, shadow, rk, bt, bo, bp);
+                    if (sp == 0) break;
+                    sp--;
+                    inode = st[sp].node;
+                    rentry = st[sp].rentry;
+                    rexit = st[sp].rexit;
+                }
+            }
+        }
+        hits[i] = pvx_hit{bt, bo, bp};
+        if (kCount) {
+            node_visits[i] = cn;
+            prim_tests[i] = cp;
+        }
+    }
+}
+
+#if PVX_TRACE_PART == 12
+// All intersections of one object per ray, sorted ascending
+// (ObjectBase::All_Intersections, object.h:238).
+__global__ void k_all_intersections(SceneView sv, const pvx_ray *rays, const uint32_t *objs,
+                                    uint32_t n, uint32_t max_hits, double *depths, int32_t *prims,
+                                    uint32_t *counts, const uint32_t *obj_type,
+                                    const uint32_t *obj_index) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const pvx_ray r = rays[i];
+    D3 o = d3(r.o[0], r.o[1], r.o[2]);
+    D3 d = d3(r.d[0], r.d[1], r.d[2]);
+    uint32_t ob = objs[i];
+    uint32_t type = obj_type[ob], index = obj_index[ob];
+    double buf[16];
+    int32_t pbuf[16];
+    uint32_t cnt = 0;
+    auto push = [&](double dep, int32_t prim) {
+        if (cnt < 16) {
+            // insertion sort keeps the list ascending
+            uint32_t j = cnt;
+            while (j > 0 && buf[j - 1] > dep) {
+                buf[j] = buf[j - 1];
+                pbuf[j] = pbuf[j - 1];
+                j--;
+            }
+            buf[j] = dep;
+            pbuf[j] = prim;
+        }
+        cnt++;
+    };
+    if (type == PVX_PLANE) {
+        double dep;
+        if (plane_hit(sv.planes[index], o, d, dep)) push(dep, 0);
+    } else if (type == PVX_CSG) {
+        uint32_t b = sv.csg_leaf_range[2 * index], e = sv.csg_leaf_range[2 * index + 1];
+        for (uint32_t k = b; k < e; k++) {
+            uint32_t slot = sv.csg_leaves[k];
+            const CsgChild &c = sv.csg_children[slot];
+            double hs[kMaxLeafHits];
+            int nh = leaf_hits(sv, c.type, c.index, o, d, hs);
+            for (int j = 0; j < nh; j++)
+                if (csg_survives(sv, slot, add3(o, mul3(d, hs[j])))) push(hs[j], (int32_t)slot);
+        }
+    } else if (type == PVX_MESH) {
+        // every triangle of the mesh: walk the whole BVH without culling by depth
+        int32_t stack[kStackMax];
+        int sp = 0;
+        int32_t node = sv.root;
+        float ix = safe_inv(d.x), iy = safe_inv(d.y), iz = safe_inv(d.z);
+        float oix = (float)o.x * ix, oiy = (float)o.y * iy, oiz = (float)o.z * iz;
+        while (node != kEmpty) {
+            if (node >= 0) {
+                const Node &nd = sv.nodes[node];
+                const float lo0[3] = {nd.lo0x, nd.lo0y, nd.lo0z}, hi0[3] = {nd.hi0x, nd.hi0y, nd.hi0z};
+                const float lo1[3] = {nd.lo1x, nd.lo1y, nd.lo1z}, hi1[3] = {nd.hi1x, nd.hi1y, nd.hi1z};
+                const float iv[3] = {ix, iy, iz}, oi[3] = {oix, oiy, oiz};
+                float tn0 = 0.0f, tf0 = 3.0e38f, tn1 = 0.0f, tf1 = 3.0e38f;
+                for (int k = 0; k < 3; k++) {
+                    float a0 = fmaf(lo0[k], iv[k], -oi[k]), b0 = fmaf(hi0[k], iv[k], -oi[k]);
+                    float a1 = fmaf(lo1[k], iv[k], -oi[k]), b1 = fmaf(hi1[k], iv[k], -oi[k]);
+                    tn0 = fmaxf(tn0, fminf(a0, b0));
+                    tf0 = fminf(tf0, fmaxf(a0, b0));
+                    tn1 = fmaxf(tn1, fminf(a1, b1));
+                    tf1 = fminf(tf1, fmaxf(a1, b1));
+                }
+                bool h0 = tn0 <= tf0 * 1.000001f, h1 = tn1 <= tf1 * 1.000001f;
+                if (h0 && h1 && sp < kStackMax) {
+                    stack[sp++] = nd.c1;
+                    node = nd.c0;
+                    continue;
+                }
+                if (h0) { node = nd.c0; continue; }
+                if (h1) { node = nd.c1; continue; }
+            } else {
+                uint32_t leaf = (uint32_t)(~node), first = leaf >> 3, count = (leaf & 7u) + 1u;
+                for (uint32_t k = 0; k < count; k++) {
+                    const PrimRec &p = sv.prims[first + k];
+                    if ((p.kind_axis & 15u) != kPrimTriangle || p.obj != (int32_t)ob) continue;
+                    double dep;
+                    if (tri_hit64(p.tri, p.kind_axis >> 4, o, d, dep)) push(dep, p.tri.prim);
+                }
+            }
+            node = sp == 0 ? kEmpty : stack[--sp];
+        }
+    } else {
+        double hs[kMaxLeafHits];
+        int nh = leaf_hits(sv, type, index, o, d, hs);
+        for (int j = 0; j < nh; j++) push(hs[j], 0);
+    }
+    counts[i] = cnt;
+    for (uint32_t k = 0; k < max_hits; k++) {
+        bool have = k < cnt && k < 16;
+        depths[(size_t)i * max_hits + k] = have ? buf[k] : HUGE_VAL;
+        prims[(size_t)i * max_hits + k] = have ? pbuf[k] : -1;
+    }
+}
+
+// Inside_Object per (point, object) (object.cpp:346-355).
+__global__ void k_inside(SceneView sv, const double *pts, const uint32_t *objs, uint32_t n,
+                         uint8_t *out, const uint32_t *obj_type, const uint32_t *obj_index,
+                         const uint32_t *obj_flags) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    D3 p = d3(pts[3 * (size_t)i], pts[3 * (size_t)i + 1], pts[3 * (size_t)i + 2]);
+    uint32_t ob = objs[i];
+    out[i] = shape_inside<kCsgMaxDepth>(sv, obj_type[ob], obj_index[ob], obj_flags[ob], p) ? 1 : 0;
+}
+#endif
+
+// One kernel: its occupancy, or a launch of nb blocks.
+template <class F>
+hipError_t run(F *kfn, const TraceArgs &a, uint32_t nb, hipStream_t st, int *occ) {
+    if (occ) return hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, reinterpret_cast<const void *>(kfn), kBlock, 0);
+    hipLaunchKernelGGL(kfn, dim3(nb), dim3(kBlock), 0, st, a);
+    return hipGetLastError();
+}
+
+} // namespace
+
+#define PVX_FN2(n) trace_part##n
+#define PVX_FN(n) PVX_FN2(n)
+// flags x count selection of one template
+#define PVX_FC(K, ...)                                                                                         \
+    (k.flags ? (k.count ? run(K<__VA_ARGS__ true, true>, a, nb, st, occ) : run(K<__VA_ARGS__ true, false>, a, nb, st, occ)) \
+             : (k.count ? run(K<__VA_ARGS__ false, true>, a, nb, st, occ) : run(K<__VA_ARGS__ false, false>, a, nb, st, occ)))
+
+#if PVX_TRACE_PART == 12
+hipError_t launch_all_intersections(const SceneView &sv, const pvx_ray *rays, const uint32_t *objs, uint32_t n,
+                                    uint32_t max_hits, double *depths, int32_t *prims, uint32_t *counts,
+                                    const uint32_t *obj_type, const uint32_t *obj_index, hipStream_t st) {
+    const uint32_t nb = (n + 127) / 128;
+    k_all_intersections<<<nb, 128, 0, st>>>(sv, rays, objs, n, max_hits, depths, prims, counts, obj_type, obj_index);
+    return hipGetLastError();
+}
+
+hipError_t launch_inside(const SceneView &sv, const double *pts, const uint32_t *objs, uint32_t n, uint8_t *out,
+                         const uint32_t *obj_type, const uint32_t *obj_index, const uint32_t *obj_flags,
+                         hipStream_t st) {
+    const uint32_t nb = (n + 255) / 256;
+    k_inside<<<nb, 256, 0, st>>>(sv, pts, objs, n, out, obj_type, obj_index, obj_flags);
+    return hipGetLastError();
+}
+#else
+hipError_t PVX_FN(PVX_TRACE_PART)(const TraceKernel &k, const TraceArgs &a, uint32_t nb, hipStream_t st, int *occ) {
+#if PVX_TRACE_PART == 0 // single scene, 4-wide walks
+    return k.walk == kWalkWW4 ? PVX_FC(k_trace_closest_ww, ) : PVX_FC(k_trace_closest, 0, );
+#elif PVX_TRACE_PART == 1 // single scene, binary walk
+    return PVX_FC(k_trace_closest, 1, );
+#elif PVX_TRACE_PART >= 2 && PVX_TRACE_PART <= 5 // single scene, every primitive kind
+    constexpr bool kF = (PVX_TRACE_PART - 2) >= 2, kC = (PVX_TRACE_PART - 2) & 1;
+    (void)k;
+    return run(k_trace_closest<2, kF, kC>, a, nb, st, occ);
+#elif PVX_TRACE_PART == 6 // kd-tree
+    return PVX_FC(k_trace_closest_kd, );
+#elif PVX_TRACE_PART == 7 // scene batches: 4-wide and binary walks
+    return k.walk == kWalkWW4    ? PVX_FC(k_trace_closest_ww_multi, )
+           : k.walk == kWalkStep4 ? PVX_FC(k_trace_closest_multi, 0, )
+                                  : PVX_FC(k_trace_closest_multi, 1, );
+#elif PVX_TRACE_PART >= 8 && PVX_TRACE_PART <= 11 // scene batches, every primitive kind
+    constexpr bool kF = (PVX_TRACE_PART - 8) >= 2, kC = (PVX_TRACE_PART - 8) & 1;
+    (void)k;
+    return run(k_trace_closest_multi<2, kF, kC>, a, nb, st, occ);
+#else
+#error "PVX_TRACE_PART out of range"
+#endif
+}
+#endif
+
+} // namespace pvx

@@ -1,7 +1,7 @@
 """Seeded synthetic scenes for the hot-path workloads.
 
-Every scene is generated from a splitmix64 stream so that the GPU engine, the
-CPU oracle and the `.pov` emitter see bit-identical inputs.  Geometry follows
+Every scene is generated from a splitmix64 stream so that the GPU engine and
+the CPU oracle see bit-identical inputs.  Geometry follows
 POV-Ray's conventions: left-handed camera basis built like the `look_at`
 keyword (parser/parser.cpp:1926-1969), mesh2 vertices stored as FP32
 (core/shape/mesh.h: MeshVector = SnglVector3d), planes as N.P + D = 0
