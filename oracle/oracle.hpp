@@ -205,8 +205,10 @@ bool polygon_intersect(const PolygonO &g, const Ray &r, double *depth);
 // Quadric (quadric.cpp): coefficients A..J after the object's transform.
 struct Quadric {
     double c[10];
+    FBox bbox; // Compute_BBox at parse time, moved by the transform
 };
 Quadric quadric_prepare(const double *coef, const Transform *user);
+FBox quadric_bbox(const double *coef, const Transform *user);
 int quadric_intersect(const Quadric &q, const Ray &r, double *depth); // up to 2 hits
 bool quadric_inside(const Quadric &q, V3 p, bool inverted);
 V3 quadric_normal(const Quadric &q, V3 p);

@@ -45,6 +45,16 @@ def lib():
                                            C.c_void_p]
         L.orc_solve_polynomial.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_double, C.c_void_p]
         L.orc_triangle_normal.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t, C.c_void_p]
+        L.orc_trace_rays.argtypes = [C.c_void_p, C.POINTER(A.PvxShadingDesc), C.c_void_p, C.c_size_t,
+                                     C.c_void_p, C.c_int]
+        L.orc_halton_pool.argtypes = [C.c_void_p]
+        L.orc_rad_average.argtypes = [C.c_void_p, C.c_size_t, C.c_double, C.c_void_p, C.c_void_p,
+                                      C.c_double, C.c_int, C.c_int, C.c_void_p]
+        L.orc_box_intersect.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_object_bbox.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p]
+        L.orc_difficult_coeffs.argtypes = [C.c_int, C.c_void_p]
+        L.orc_solve_quartic.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_polysolve.argtypes = [C.c_int, C.c_void_p, C.c_void_p]
         _LIB = L
     return _LIB
 
@@ -143,6 +153,21 @@ class OracleScene:
         return out, dict(gathers0=int(cnt[0]), gathers1=int(cnt[1]), rays=int(cnt[2]),
                          records0=int(cnt[3]), records1=int(cnt[4]))
 
+    def trace_rays(self, scene, rays, method=1):
+        """TraceRay (primary-ray flags) for (n, 6) rays -> (n, 3) float32 RGB."""
+        sd, keep = scene.shading()
+        rays = np.ascontiguousarray(rays, np.float64).reshape(-1, 6)
+        out = np.zeros((rays.shape[0], 3), np.float32)
+        lib().orc_trace_rays(self.h, C.byref(sd), _p(rays), rays.shape[0], _p(out), method)
+        del keep
+        return out
+
+    def object_bbox(self, obj):
+        """(lowerLeft (3,) float32, size (3,) float32, infinite flag) of an object."""
+        out = np.zeros(6, np.float32)
+        inf = lib().orc_object_bbox(self.h, obj, _p(out))
+        return out[:3].copy(), out[3:].copy(), bool(inf)
+
     def mesh_normal(self, mesh, tri, pts):
         tri = np.ascontiguousarray(tri, np.int32)
         pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
@@ -164,3 +189,56 @@ def solve_polynomial(coeffs, sturm=True, epsilon=1e-4):
     r = np.zeros(max(4, c.shape[0]))
     n = lib().orc_solve_polynomial(c.shape[0] - 1, _p(c), 1 if sturm else 0, epsilon, _p(r))
     return np.sort(r[:n])
+
+
+def halton_pool():
+    """The radiosity gathers' direction pool, (1600, 3)."""
+    out = np.zeros((1600, 3))
+    n = lib().orc_halton_pool(_p(out))
+    return out[:n]
+
+
+def rad_average(records, error_bound, point, normal, bound, depth=0, final=False):
+    """AverageNearBlock of one query over hand-built records: (n, 15) rows of
+    point, normal, irradiance, harmonic-mean distance, nearest distance,
+    direction to the nearest surface, depth.  Returns (irradiance (3,), weight sum)."""
+    rec = np.ascontiguousarray(records, np.float64).reshape(-1, 15)
+    p = np.ascontiguousarray(point, np.float64)
+    nrm = np.ascontiguousarray(normal, np.float64)
+    out = np.zeros(4)
+    lib().orc_rad_average(_p(rec), rec.shape[0], error_bound, _p(p), _p(nrm), bound, depth,
+                          1 if final else 0, _p(out))
+    return out[:3], out[3]
+
+
+def box_intersect(c1, c2, ray, transform=None):
+    """Box::Intersect: (hit, depth1, depth2, side1, side2); sides 1..6 =
+    -x, +x, -y, +y, -z, +z."""
+    c1 = np.ascontiguousarray(c1, np.float64)
+    c2 = np.ascontiguousarray(c2, np.float64)
+    ray = np.ascontiguousarray(ray, np.float64)
+    tr = None if transform is None else np.ascontiguousarray(transform, np.float64).reshape(32)
+    out = np.zeros(4)
+    hit = lib().orc_box_intersect(_p(c1), _p(c2), None if tr is None else _p(tr), _p(ray), _p(out))
+    return bool(hit), out[0], out[1], int(out[2]), int(out[3])
+
+
+def difficult_coeffs(c):
+    c = np.ascontiguousarray(c, np.float64)
+    return bool(lib().orc_difficult_coeffs(c.shape[0] - 1, _p(c)))
+
+
+def solve_quartic_vieta(c):
+    """The closed-form quartic branch of Solve_Polynomial (unsorted roots)."""
+    c = np.ascontiguousarray(c, np.float64)
+    r = np.zeros(4)
+    n = lib().orc_solve_quartic(_p(c), _p(r))
+    return r[:n].copy()
+
+
+def polysolve(c):
+    """The Sturm-sequence branch of Solve_Polynomial (unsorted roots)."""
+    c = np.ascontiguousarray(c, np.float64)
+    r = np.zeros(max(4, c.shape[0]))
+    n = lib().orc_polysolve(c.shape[0] - 1, _p(c), _p(r))
+    return r[:n].copy()

@@ -391,3 +391,10 @@ int solve_polynomial(int n, const double *c0, double *r, bool sturm, double epsi
 }
 
 } // namespace orc
+
+// Single branches of the solver, for tests that pin which branch
+// Solve_Polynomial takes: difficult_coeffs, the Ferrari/Vieta quartic and
+// the Sturm isolation (coefficients in descending powers).
+extern "C" int orc_difficult_coeffs(int n, const double *c) { return orc::difficult_coeffs(n, c); }
+extern "C" int orc_solve_quartic(const double *c, double *roots) { return orc::solve_quartic(c, roots); }
+extern "C" int orc_polysolve(int order, const double *c, double *roots) { return orc::polysolve(order, c, roots); }

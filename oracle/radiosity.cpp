@@ -297,3 +297,51 @@ extern "C" int orc_radiosity_render(void *sp, const pvx_shading_desc *sd, const 
     }
     return 0;
 }
+
+// The direction pool the gathers draw from (kPoolSize x 3).
+extern "C" int orc_halton_pool(double *out) {
+    static const Scene empty;
+    static const std::vector<pvx_material> no_mats;
+    pvx_radiosity_settings st{};
+    Rad rad(empty, no_mats, st, V3(0, 0, 0));
+    for (int i = 0; i < kPoolSize; i++) {
+        out[3 * i] = rad.pool[i].x;
+        out[3 * i + 1] = rad.pool[i].y;
+        out[3 * i + 2] = rad.pool[i].z;
+    }
+    return kPoolSize;
+}
+
+// FindReusableBlock + AverageNearBlock of one query against hand-built
+// records (15 doubles each: point 3, normal 3, irradiance 3, harmonic-mean
+// distance, nearest distance, direction to the nearest surface 3, depth).
+// `error_bound` places the records in the lookup grid like the cache's own
+// insertions; `bound` is the query's error bound.  out4 = averaged
+// irradiance (r, g, b) and the weight sum.
+extern "C" int orc_rad_average(const double *recs, size_t n, double error_bound, const double *p3, const double *n3,
+                               double bound, int depth, int final_pass, double *out4) {
+    static const Scene empty;
+    static const std::vector<pvx_material> no_mats;
+    pvx_radiosity_settings st{};
+    st.error_bound = (float)error_bound;
+    Rad rad(empty, no_mats, st, V3(0, 0, 0));
+    for (size_t i = 0; i < n; i++) {
+        const double *q = recs + 15 * i;
+        Record rc;
+        rc.p = V3(q[0], q[1], q[2]);
+        rc.n = V3(q[3], q[4], q[5]);
+        rc.E = Col{q[6], q[7], q[8]};
+        rc.r = q[9];
+        rc.nearest = q[10];
+        rc.to_nearest = V3(q[11], q[12], q[13]);
+        rc.depth = (int)q[14];
+        rad.add(rc);
+    }
+    Col E;
+    const double W = rad.lookup(V3(p3[0], p3[1], p3[2]), V3(n3[0], n3[1], n3[2]), bound, depth, final_pass != 0, E);
+    out4[0] = E.r;
+    out4[1] = E.g;
+    out4[2] = E.b;
+    out4[3] = W;
+    return 0;
+}

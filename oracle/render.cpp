@@ -369,3 +369,28 @@ extern "C" int orc_render(void *sp, const pvx_shading_desc *sd, const pvx_camera
     }
     return 0;
 }
+
+// TraceRay for caller-given camera rays (n x 6: origin, unit direction),
+// with the primary-ray flags of orc_render; rgb receives n x 3.
+extern "C" int orc_trace_rays(void *sp, const pvx_shading_desc *sd, const double *rays, size_t n, float *rgb,
+                              int method) {
+    const Scene &s = *static_cast<Scene *>(sp);
+    Shading sh;
+    sh.method = method;
+    sh.mats.assign(sd->materials, sd->materials + sd->n_materials);
+    sh.lights.assign(sd->lights, sd->lights + sd->n_lights);
+    std::memcpy(sh.amb, sd->ambient_light, sizeof sh.amb);
+    std::memcpy(sh.bg, sd->background, sizeof sh.bg);
+    sh.max_level = sd->max_trace_level;
+    sh.bailout = sd->adc_bailout;
+    Counters cnt;
+    for (size_t i = 0; i < n; i++) {
+        const double *q = rays + 6 * i;
+        Ray r{V3(q[0], q[1], q[2]), V3(q[3], q[4], q[5])};
+        const RGB c = trace_ray(s, sh, cnt, r, 1.0f, 0, RayFlags{true, false, false}, Interiors{}, false);
+        rgb[3 * i] = c.r;
+        rgb[3 * i + 1] = c.g;
+        rgb[3 * i + 2] = c.b;
+    }
+    return 0;
+}

@@ -62,11 +62,8 @@ FBox Scene::shape_bbox(uint32_t type, uint32_t index, uint32_t flags, bool *infi
     case PVX_POLYGON:
         b = polygons[index].bbox;
         break;
-    case PVX_QUADRIC: // unbounded (see quadric.cpp)
-        for (int k = 0; k < 3; k++) {
-            b.ll[k] = (float)(-kBoundHuge / 2);
-            b.size[k] = (float)kBoundHuge;
-        }
+    case PVX_QUADRIC: // Quadric::Compute_BBox at parse time (see quadric.cpp)
+        b = quadrics[index].bbox;
         break;
     case PVX_TRIANGLE:
     case PVX_SMOOTH_TRIANGLE:
@@ -310,12 +307,16 @@ bool Scene::inside(uint32_t type, uint32_t index, uint32_t flags, V3 p) const {
 }
 
 // Find_Intersection (object.cpp:118-224) / Trace::FindIntersection(object,...)
-// (trace.cpp:346-450): object bbox pre-test for shapes that do not override
-// Intersect_BBox (mesh, torus, CSG), then the IStack scan from the top.
+// (trace.cpp:346-450): the FP32 object-box pretest of ObjectBase::Intersect_BBox
+// (object.cpp:917-942, 1074-1108) for every shape that does not override it
+// (Sphere, Box, Plane, Triangle and Poly do: sphere.h:92, box.h:106,
+// plane.h:94, triangle.h:99, polynomial.h:104), then the IStack scan from the
+// top.
 bool Scene::find_object(uint32_t obj, const Ray &r, bool shadow, Isect &out) const {
     const Object &o = objects[obj];
     double closest = HUGE_VAL;
-    if (o.type == PVX_MESH || o.type == PVX_TORUS || o.type == PVX_CSG) {
+    if (o.type == PVX_MESH || o.type == PVX_TORUS || o.type == PVX_CSG || o.type == PVX_CONE ||
+        o.type == PVX_DISC || o.type == PVX_POLYGON || o.type == PVX_QUADRIC) {
         if (!intersect_bbox_dir(o.bbox, r, (float)kMinIsectDepth, (float)closest)) return false;
     }
     std::vector<Isect> st;
@@ -526,6 +527,40 @@ void *orc_scene_create(const pvx_scene_desc *desc) {
 }
 
 void orc_scene_destroy(void *s) { delete static_cast<Scene *>(s); }
+
+// Box::Intersect on one ray: corners c1 <= c2, optional transform (16
+// doubles matrix then 16 inverse, row-vector convention), ray origin +
+// direction; out = depth1, depth2, side1, side2 (kSideHit numbering 1..6 =
+// -x, +x, -y, +y, -z, +z, 0 = none).
+int orc_box_intersect(const double *c1, const double *c2, const double *tr32, const double *ray6,
+                      double *out4) {
+    Transform t;
+    if (tr32) {
+        std::memcpy(t.m, tr32, sizeof t.m);
+        std::memcpy(t.inv, tr32 + 16, sizeof t.inv);
+    }
+    Ray r{V3(ray6[0], ray6[1], ray6[2]), V3(ray6[3], ray6[4], ray6[5])};
+    double d1 = 0, d2 = 0;
+    int s1 = 0, s2 = 0;
+    const bool hit = box_intersect(r, tr32 ? &t : nullptr, V3(c1[0], c1[1], c1[2]), V3(c2[0], c2[1], c2[2]), &d1,
+                                   &d2, &s1, &s2);
+    out4[0] = d1;
+    out4[1] = d2;
+    out4[2] = s1;
+    out4[3] = s2;
+    return hit ? 1 : 0;
+}
+
+// The object's FP32 bounding box (lowerLeft, size) and infinite flag.
+int orc_object_bbox(void *sp, uint32_t obj, float *ll_size6) {
+    const Scene &s = *static_cast<Scene *>(sp);
+    const Object &o = s.objects[obj];
+    for (int k = 0; k < 3; k++) {
+        ll_size6[k] = o.bbox.ll[k];
+        ll_size6[3 + k] = o.bbox.size[k];
+    }
+    return o.infinite ? 1 : 0;
+}
 
 int orc_find_intersection(void *sp, const pvx_ray *rays, const double *max_depth,
                           uint32_t ray_kind, size_t n, int method, pvx_hit *hits, int threads) {
