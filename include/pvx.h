@@ -582,6 +582,41 @@ int pvx_radiosity_render(pvx_scene *scene, const pvx_camera *cam,
                          const pvx_radiosity_settings *settings, float *rgb,
                          pvx_radiosity_stats *stats, void *stream);
 
+/* Opens a per-query irradiance cache on the scene, as constructing a
+ * RadiosityFunction over the shared RadiosityCache does
+ * (core/lighting/radiosity.h:257-300, radiosity.cpp:177-260): settings as
+ * above (mode PVX_RADIOSITY_PLAIN only), cam->location is the camera
+ * position that scales the reuse distances (GatherLight's to_eye).  Clears
+ * the cache (records loaded by pvx_radiosity_load seed it); the cache then
+ * grows with every pvx_radiosity_compute_ambient call until the next
+ * pvx_radiosity_begin or pvx_radiosity_render on the scene. */
+int pvx_radiosity_begin(pvx_scene *scene, const pvx_camera *cam,
+                        const pvx_radiosity_settings *settings, void *stream);
+
+/* Ambient colour for n surface points.
+ * Replaces virtual void Trace::RadiosityFunctor::ComputeAmbient(
+ *   const Vector3d& ipoint, const Vector3d& raw_normal,
+ *   const Vector3d& layer_normal, double brilliance, MathColour& ambient_colour,
+ *   double weight, TraceTicket& ticket)  (core/render/trace.h:210-216),
+ * implemented by RadiosityFunction::ComputeAmbient
+ * (core/lighting/radiosity.h:287, radiosity.cpp:385-461), batched:
+ * ipoint, raw_normal: 3n doubles (device); layer_normal: 3n doubles or NULL
+ * (= raw_normal; a layer normal not `similar` to its raw normal, i.e. a
+ * perturbed normal, gives PVX_EUNSUPPORTED); weight: n doubles or NULL
+ * (1.0); depth: ticket.radiosityRecursionDepth (< recursion_limit);
+ * final_trace: RadiosityFunction::isFinalTrace.  brilliance is not an
+ * argument: with the default `brilliance off` ComputeAmbient uses 1.0.
+ * Each point reuses the cache where FindReusableBlock finds enough weight
+ * and gathers otherwise (GatherLight), filing the new records; the points of
+ * one call are gathered in rounds (a spread-out subset first), so later
+ * points of the call reuse earlier ones' records as in the sequential code.
+ * ambient_rgb: 3n floats (device), times `brightness` (gray_threshold 0).
+ * stats (may be NULL): gathers and rays of this call, cache sizes after it. */
+int pvx_radiosity_compute_ambient(pvx_scene *scene, size_t n, const double *ipoint,
+                                  const double *raw_normal, const double *layer_normal,
+                                  const double *weight, uint32_t depth, uint32_t final_trace,
+                                  float *ambient_rgb, pvx_radiosity_stats *stats, void *stream);
+
 /* ---- irradiance surrogate ------------------------------------------------ */
 /* No reference counterpart: a small MLP that stands in for
  * RadiosityFunction::GatherLight (core/lighting/radiosity.cpp:509-743) at

@@ -48,6 +48,13 @@ def lib():
         L.orc_trace_rays.argtypes = [C.c_void_p, C.POINTER(A.PvxShadingDesc), C.c_void_p, C.c_size_t,
                                      C.c_void_p, C.c_int]
         L.orc_halton_pool.argtypes = [C.c_void_p]
+        L.orc_rad_open.argtypes = [C.c_void_p, C.POINTER(A.PvxShadingDesc), C.POINTER(A.PvxCamera),
+                                   C.POINTER(A.PvxRadiositySettings)]
+        L.orc_rad_open.restype = C.c_void_p
+        L.orc_rad_close.argtypes = [C.c_void_p]
+        L.orc_rad_close.restype = None
+        L.orc_rad_ambient.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                      C.c_int, C.c_void_p, C.c_void_p]
         L.orc_rad_average.argtypes = [C.c_void_p, C.c_size_t, C.c_double, C.c_void_p, C.c_void_p,
                                       C.c_double, C.c_int, C.c_int, C.c_void_p]
         L.orc_box_intersect.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
@@ -55,6 +62,7 @@ def lib():
         L.orc_difficult_coeffs.argtypes = [C.c_int, C.c_void_p]
         L.orc_solve_quartic.argtypes = [C.c_void_p, C.c_void_p]
         L.orc_polysolve.argtypes = [C.c_int, C.c_void_p, C.c_void_p]
+        L.orc_solver_branches.argtypes = [C.c_void_p]
         _LIB = L
     return _LIB
 
@@ -153,6 +161,10 @@ class OracleScene:
         return out, dict(gathers0=int(cnt[0]), gathers1=int(cnt[1]), rays=int(cnt[2]),
                          records0=int(cnt[3]), records1=int(cnt[4]))
 
+    def radiosity_function(self, scene, **over):
+        """A sequential FP64 RadiosityFunction over one cache (see OracleRadiosity)."""
+        return OracleRadiosity(self, scene, **over)
+
     def trace_rays(self, scene, rays, method=1):
         """TraceRay (primary-ray flags) for (n, 6) rays -> (n, 3) float32 RGB."""
         sd, keep = scene.shading()
@@ -226,6 +238,44 @@ def box_intersect(c1, c2, ray, transform=None):
 def difficult_coeffs(c):
     c = np.ascontiguousarray(c, np.float64)
     return bool(lib().orc_difficult_coeffs(c.shape[0] - 1, _p(c)))
+
+
+class OracleRadiosity:
+    """RadiosityFunction::ComputeAmbient (radiosity.cpp:385-461) point by
+    point in the caller's order over one cache that persists between calls."""
+
+    def __init__(self, oscene, scene, **over):
+        self.o = oscene
+        self._sd, self._keep = scene.shading()
+        cam = scene.camera_struct()
+        rs = A.radiosity_settings(scene, **over)
+        self.h = lib().orc_rad_open(oscene.h, C.byref(self._sd), C.byref(cam), C.byref(rs))
+
+    def compute_ambient(self, ipoint, normal, weight=None, depth=0, final=True):
+        p = np.ascontiguousarray(ipoint, np.float64).reshape(-1, 3)
+        nr = np.ascontiguousarray(normal, np.float64).reshape(-1, 3)
+        w = None if weight is None else np.ascontiguousarray(weight, np.float64)
+        out = np.zeros((p.shape[0], 3), np.float32)
+        cnt = np.zeros(5, np.uint64)
+        lib().orc_rad_ambient(self.h, p.shape[0], _p(p), _p(nr), None if w is None else _p(w), depth,
+                              1 if final else 0, _p(out), _p(cnt))
+        self.counters = dict(gathers0=int(cnt[0]), gathers1=int(cnt[1]), rays=int(cnt[2]),
+                             records0=int(cnt[3]), records1=int(cnt[4]))
+        return out
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_rad_close(self.h)
+            self.h = None
+
+
+def solver_branches():
+    """Reads and clears how often the order-4 branches of Solve_Polynomial
+    ran since the last call: dict of closed-form quartic, difficult_coeffs
+    forcing Sturm, the quartic's d1 clamp to 0, and Sturm by request."""
+    out = np.zeros(4, np.uint64)
+    lib().orc_solver_branches(_p(out))
+    return dict(quartic=int(out[0]), difficult=int(out[1]), d1_clamp=int(out[2]), sturm=int(out[3]))
 
 
 def solve_quartic_vieta(c):

@@ -4,11 +4,19 @@
 // regula_falsa, polyeval) and the closed-form quadratic/cubic/quartic
 // solvers, with the branches the reference compiles by default
 // (TEST_SOLVER and USE_NEW_DIFFICULT_COEFFS undefined).
+#include <atomic>
 #include <cmath>
+#include <cstdint>
 
 #include "oracle.hpp"
 
 namespace orc {
+
+// How often Solve_Polynomial took each order-4 branch (for tests that show
+// which branches a scene's rays reach): [0] closed-form quartic,
+// [1] difficult_coeffs forcing Sturm, [2] the quartic's d1 clamp
+// (-SMALL_ENOUGH < d1 < 0 set to 0), [3] Sturm asked for by the caller.
+std::atomic<uint64_t> g_branch[4];
 
 namespace {
 
@@ -300,10 +308,12 @@ int solve_quartic(const double *x, double *results) { // #else branch :1325-1450
     double z = roots[0];
     double d1 = 2.0 * z - p, d2;
     if (d1 < 0.0) {
-        if (d1 > -kSmallEnough)
+        if (d1 > -kSmallEnough) {
+            g_branch[2]++;
             d1 = 0.0;
-        else
+        } else {
             return 0;
+        }
     }
     if (d1 < kSmallEnough) {
         d2 = z * z - r;
@@ -376,7 +386,14 @@ int solve_polynomial(int n, const double *c0, double *r, bool sturm, double epsi
             roots = sturm ? polysolve(3, c, r) : solve_cubic(c, r);
             break;
         }
-        if (difficult_coeffs(4, c)) sturm = true;
+        if (sturm) {
+            g_branch[3]++;
+        } else if (difficult_coeffs(4, c)) {
+            g_branch[1]++;
+            sturm = true;
+        } else {
+            g_branch[0]++;
+        }
         roots = sturm ? polysolve(4, c, r) : solve_quartic(c, r);
         break;
     default:
@@ -398,3 +415,7 @@ int solve_polynomial(int n, const double *c0, double *r, bool sturm, double epsi
 extern "C" int orc_difficult_coeffs(int n, const double *c) { return orc::difficult_coeffs(n, c); }
 extern "C" int orc_solve_quartic(const double *c, double *roots) { return orc::solve_quartic(c, roots); }
 extern "C" int orc_polysolve(int order, const double *c, double *roots) { return orc::polysolve(order, c, roots); }
+// Reads and clears the order-4 branch counters (four entries, see g_branch).
+extern "C" void orc_solver_branches(uint64_t *out) {
+    for (int i = 0; i < 4; i++) out[i] = orc::g_branch[i].exchange(0);
+}

@@ -299,6 +299,45 @@ extern "C" int orc_radiosity_render(void *sp, const pvx_shading_desc *sd, const 
 }
 
 // The direction pool the gathers draw from (kPoolSize x 3).
+// RadiosityFunction over one cache, queried point by point in the caller's
+// order (ComputeAmbient, radiosity.cpp:385-461): open, query, close.
+struct OrcRadiosity {
+    std::vector<pvx_material> mats;
+    Rad rad;
+    OrcRadiosity(const Scene &s, const pvx_shading_desc *sd, const pvx_radiosity_settings *rs, V3 cam)
+        : mats(sd->materials, sd->materials + sd->n_materials), rad(s, mats, *rs, cam) {}
+};
+
+extern "C" void *orc_rad_open(void *sp, const pvx_shading_desc *sd, const pvx_camera *cam,
+                              const pvx_radiosity_settings *rs) {
+    return new OrcRadiosity(*static_cast<Scene *>(sp), sd, rs,
+                            V3(cam->location[0], cam->location[1], cam->location[2]));
+}
+
+extern "C" void orc_rad_close(void *h) { delete static_cast<OrcRadiosity *>(h); }
+
+// ambient (n x 3) for points p, normals nrm (n x 3) and weights w (n, or
+// NULL = 1); counters: gathers at depth 0 / 1, rays, records at depth 0 / 1.
+extern "C" int orc_rad_ambient(void *h, size_t n, const double *p, const double *nrm, const double *w, int depth,
+                               int final, float *out, uint64_t *counters) {
+    Rad &rad = static_cast<OrcRadiosity *>(h)->rad;
+    for (size_t i = 0; i < n; i++) {
+        Col c = rad.ambient(V3(p[3 * i], p[3 * i + 1], p[3 * i + 2]),
+                            V3(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]), w ? w[i] : 1.0, depth, final != 0);
+        out[3 * i] = (float)c.r;
+        out[3 * i + 1] = (float)c.g;
+        out[3 * i + 2] = (float)c.b;
+    }
+    if (counters) {
+        counters[0] = rad.gathers[0];
+        counters[1] = rad.gathers[1];
+        counters[2] = rad.rays;
+        counters[3] = rad.recs[0].size();
+        counters[4] = rad.recs[1].size();
+    }
+    return 0;
+}
+
 extern "C" int orc_halton_pool(double *out) {
     static const Scene empty;
     static const std::vector<pvx_material> no_mats;

@@ -444,6 +444,30 @@ __global__ void k_apply_pred(const uint32_t *__restrict__ idx, uint32_t m, const
     acc[i].w = -1.0;
 }
 
+// ComputeAmbient's arguments -> queries (radiosity.cpp:385-400): the point,
+// the raw normal (the effective normal with `normal off`), the weight that
+// widens the error bound.  A layer normal that is not `similar` to the raw
+// one (vector.h:591, summed component difference < EPSILON) would make
+// GatherLight sample around the perturbed normal: flagged, not served.
+__global__ void k_ambient_queries(const double *__restrict__ ip, const double *__restrict__ rn,
+                                  const double *__restrict__ ln, const double *__restrict__ wt, size_t n,
+                                  RadQuery *__restrict__ q, uint32_t *__restrict__ bad) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    RadQuery qq;
+    for (int a = 0; a < 3; a++) {
+        qq.p[a] = ip[3 * i + a];
+        qq.n[a] = (float)rn[3 * i + a];
+    }
+    qq.w = wt ? (float)wt[i] : 1.0f;
+    if (ln) {
+        double s = fabs(ln[3 * i] - rn[3 * i]) + fabs(ln[3 * i + 1] - rn[3 * i + 1]) +
+                   fabs(ln[3 * i + 2] - rn[3 * i + 2]);
+        if (!(s < 1.0e-10)) atomicOr(bad, 1u);
+    }
+    q[i] = qq;
+}
+
 __global__ void k_fill_f32(float *__restrict__ p, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0.0f;
@@ -1631,13 +1655,8 @@ extern "C" int pvx_radiosity_load(pvx_scene *s, const char *path, uint64_t *n_re
     return PVX_OK;
 }
 
-extern "C" int pvx_radiosity_render(pvx_scene *s, const pvx_camera *cam, const pvx_radiosity_settings *set,
-                                    float *rgb, pvx_radiosity_stats *stats, void *stream) {
-    if (!s || !cam || !set || !rgb) return fail(PVX_EINVAL, "null argument");
+static int check_settings(pvx_scene *s, const pvx_radiosity_settings *set) {
     if (!s->shading.set) return fail(PVX_ESTATE, "pvx_scene_set_shading was not called");
-    if (cam->width == 0 || cam->height == 0) return fail(PVX_EINVAL, "bad image size");
-    size_t npix = (size_t)cam->width * cam->height;
-    if (npix > 0x3FFFFFFFu) return fail(PVX_EINVAL, "image too large");
     if (set->count == 0 || set->count > 1600) return fail(PVX_EINVAL, "count must be in [1, 1600]");
     if (set->nearest_count == 0 || set->nearest_count > 20)
         return fail(PVX_EINVAL, "nearest_count must be in [1, 20]");
@@ -1654,11 +1673,78 @@ extern "C" int pvx_radiosity_render(pvx_scene *s, const pvx_camera *cam, const p
     for (const MatRec &m : s->shading.mats)
         if (m.reflective)
             return fail(PVX_EUNSUPPORTED, "radiosity frames take plain diffuse / emissive finishes only");
+    return PVX_OK;
+}
+
+extern "C" int pvx_radiosity_begin(pvx_scene *s, const pvx_camera *cam, const pvx_radiosity_settings *set,
+                                   void *stream) {
+    if (!s || !cam || !set) return fail(PVX_EINVAL, "null argument");
+    int r;
+    if ((r = check_settings(s, set))) return r;
+    if (set->mode != PVX_RADIOSITY_PLAIN)
+        return fail(PVX_EUNSUPPORTED, "per-query ambient takes the plain gather (the surrogate works per frame)");
+    if (!engine_of(s)) return fail(PVX_ENOMEM, "host allocation failed");
+    RadiosityEngine &eng = *static_cast<RadiosityEngine *>(s->rad_engine);
+    if ((r = eng.begin(*set, *cam, static_cast<hipStream_t>(stream)))) return r;
+    eng.ambient_open = true;
+    return PVX_OK;
+}
+
+extern "C" int pvx_radiosity_compute_ambient(pvx_scene *s, size_t n, const double *ipoint, const double *raw_normal,
+                                             const double *layer_normal, const double *weight, uint32_t depth,
+                                             uint32_t final_trace, float *ambient_rgb, pvx_radiosity_stats *stats,
+                                             void *stream) {
+    if (!s) return fail(PVX_EINVAL, "null scene");
+    RadiosityEngine *e = static_cast<RadiosityEngine *>(s->rad_engine);
+    if (!e || !e->ambient_open) return fail(PVX_ESTATE, "pvx_radiosity_begin was not called");
+    if (n == 0) return PVX_OK;
+    if (!ipoint || !raw_normal || !ambient_rgb) return fail(PVX_EINVAL, "null argument");
+    if (n > 0x3FFFFFFFu) return fail(PVX_EINVAL, "too many queries");
+    RadiosityEngine &eng = *e;
+    if (depth >= eng.rs.recursion_limit)
+        return fail(PVX_EINVAL, "depth must be below recursion_limit (CheckRadiosityTraceLevel)");
+    eng.stream = static_cast<hipStream_t>(stream);
+    int r;
+    if ((r = eng.amb_q.reserve(n * sizeof(RadQuery))) || (r = eng.amb_flag.reserve(4))) return r;
+    uint32_t *flag = static_cast<uint32_t *>(eng.amb_flag.p);
+    k_fill_u32<<<1, 64, 0, eng.stream>>>(flag, 1);
+    RadQuery *q = static_cast<RadQuery *>(eng.amb_q.p);
+    k_ambient_queries<<<blocks(n), 256, 0, eng.stream>>>(ipoint, raw_normal, layer_normal, weight, n, q, flag);
+    HIP_TRY(hipGetLastError());
+    uint32_t bad = 0;
+    if ((r = eng.fetch(&bad, flag, 4))) return r;
+    if (bad) return fail(PVX_EUNSUPPORTED, "a layer normal differs from its raw normal (perturbed normals)");
+    const pvx_radiosity_stats before = eng.stats;
+    if ((r = eng.resolve((int)depth, q, (uint32_t)n, final_trace != 0, ambient_rgb))) return r;
+    HIP_TRY(eng.sync());
+    if (stats) {
+        *stats = eng.stats;
+        for (int d = 0; d < 2; d++) {
+            stats->gathers[d] -= before.gathers[d];
+            stats->records[d] = eng.caches[d].recs.size();
+        }
+        stats->gather_rays -= before.gather_rays;
+        stats->final_misses -= before.final_misses;
+    }
+    return PVX_OK;
+}
+
+extern "C" int pvx_radiosity_render(pvx_scene *s, const pvx_camera *cam, const pvx_radiosity_settings *set,
+                                    float *rgb, pvx_radiosity_stats *stats, void *stream) {
+    if (!s || !cam || !set || !rgb) return fail(PVX_EINVAL, "null argument");
+    if (cam->width == 0 || cam->height == 0) return fail(PVX_EINVAL, "bad image size");
+    size_t npix = (size_t)cam->width * cam->height;
+    if (npix > 0x3FFFFFFFu) return fail(PVX_EINVAL, "image too large");
+    {
+        int rc = check_settings(s, set);
+        if (rc) return rc;
+    }
     hipStream_t st = static_cast<hipStream_t>(stream);
     if (!engine_of(s)) return fail(PVX_ENOMEM, "host allocation failed");
     RadiosityEngine &eng = *static_cast<RadiosityEngine *>(s->rad_engine);
     int r;
     if ((r = eng.begin(*set, *cam, st))) return r;
+    eng.ambient_open = false; // a frame leaves the per-query cache closed
     hipEvent_t e0, e1, e2;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));

@@ -104,6 +104,8 @@ struct RadiosityEngine {
     float nlo[3] = {0, 0, 0}, ninv = 1.0f;
     std::vector<RadRec> last_fetched[kMaxRadDepth]; // records of the last gather per depth
     std::vector<RadRec> preload[kMaxRadDepth];      // records loaded from a cache file (seed every frame)
+    DevBuf amb_q, amb_flag; // pvx_radiosity_compute_ambient's queries and layer-normal flag
+    bool ambient_open = false; // pvx_radiosity_begin has opened a per-query cache
     double train_ms = 0.0;
     double t_sync_ms = 0.0, t_grid_ms = 0.0; // host-side waits and grid builds
     uint64_t n_sync = 0;

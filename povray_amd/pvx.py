@@ -63,6 +63,9 @@ def lib():
         L.pvx_clock_rate.argtypes = [C.POINTER(C.c_double)]
         L.pvx_scene_autotune.argtypes = [vp, vp, sz, u32, C.POINTER(A.PvxTuneResult), vp]
         L.pvx_scene_set_traversal.argtypes = [vp, u32, u32, u32]
+        L.pvx_radiosity_begin.argtypes = [vp, C.POINTER(A.PvxCamera), C.POINTER(A.PvxRadiositySettings), vp]
+        L.pvx_radiosity_compute_ambient.argtypes = [vp, sz, vp, vp, vp, vp, u32, u32, vp,
+                                                    C.POINTER(A.PvxRadiosityStats), vp]
         L.pvx_radiosity_save.argtypes = [vp, C.c_char_p, C.POINTER(C.c_uint64)]
         L.pvx_radiosity_load.argtypes = [vp, C.c_char_p, C.POINTER(C.c_uint64)]
         L.pvx_render_autotune.argtypes = [vp, C.POINTER(A.PvxCamera), vp, C.POINTER(A.PvxTuneResult),
@@ -89,6 +92,7 @@ EXPORTED_SYMBOLS = (
     "pvx_batch_create", "pvx_batch_destroy", "pvx_batch_trace_closest", "pvx_batch_trace_closest_counted",
     "pvx_clock_rate", "pvx_scene_autotune", "pvx_scene_set_traversal",
     "pvx_render_autotune", "pvx_radiosity_save", "pvx_radiosity_load",
+    "pvx_radiosity_begin", "pvx_radiosity_compute_ambient",
 )
 
 
@@ -283,6 +287,11 @@ class GpuScene:
         img = out.download(np.float32, cam.width * cam.height * 3)
         return img.reshape(cam.height, cam.width, 3), stats
 
+    def radiosity_function(self, **over) -> "RadiosityFunction":
+        """Open a per-query irradiance cache on this scene (pvx_radiosity_begin)
+        with the scene's radiosity settings and overrides; see RadiosityFunction."""
+        return RadiosityFunction(self, **over)
+
     def radiosity_save(self, path) -> int:
         """Write the last radiosity frame's cache as a .rca file; returns the record count."""
         n = C.c_uint64()
@@ -476,3 +485,49 @@ def camera_rays(scene, width=None, height=None, jitter=(0.5, 0.5), out=None, str
 def copy(dst, src, nbytes, stream=None):
     """Device copy kernel (byte-counter calibration workload)."""
     _check(lib().pvx_copy(_ptr(dst), _ptr(src), nbytes, stream.h if stream else None), "pvx_copy")
+
+
+class RadiosityFunction:
+    """RadiosityFunction::ComputeAmbient (core/lighting/radiosity.cpp:385-461)
+    in batch form over the scene's device irradiance cache.
+
+    Construction opens the cache (pvx_radiosity_begin: settings, camera
+    position for the reuse distances); every `compute_ambient` call looks
+    its points up, gathers where the cache cannot serve them (filing the new
+    records, so later calls reuse them) and returns the ambient colour
+    times `brightness`.  Only `mode` PLAIN is served per query."""
+
+    def __init__(self, gscene: "GpuScene", **over):
+        self.g = gscene
+        cam = gscene.scene.camera_struct()
+        rs = A.radiosity_settings(gscene.scene, mode=A.PVX_RADIOSITY_PLAIN, **over)
+        _check(lib().pvx_radiosity_begin(gscene.h, C.byref(cam), C.byref(rs), None), "pvx_radiosity_begin")
+        self.last_stats = None
+
+    def compute_ambient(self, ipoint, raw_normal, layer_normal=None, weight=None, depth=0, final=True):
+        """ipoint, raw_normal (, layer_normal): (n, 3); weight: (n,) or None
+        (1.0); depth: the ticket's radiosity recursion depth; final: the
+        final trace (isFinalTrace) rather than a pretrace.  Returns (n, 3)
+        float32 ambient colours."""
+        p = np.ascontiguousarray(ipoint, np.float64).reshape(-1, 3)
+        n = p.shape[0]
+        nr = np.ascontiguousarray(raw_normal, np.float64).reshape(n, 3)
+        bufs = [hip.DeviceBuffer.from_array(p), hip.DeviceBuffer.from_array(nr)]
+        ln = wt = None
+        if layer_normal is not None:
+            ln = hip.DeviceBuffer.from_array(np.ascontiguousarray(layer_normal, np.float64).reshape(n, 3))
+        if weight is not None:
+            wt = hip.DeviceBuffer.from_array(np.ascontiguousarray(weight, np.float64).reshape(n))
+        out = hip.DeviceBuffer(max(1, n) * 12)
+        st = A.PvxRadiosityStats()
+        _check(lib().pvx_radiosity_compute_ambient(self.g.h, n, bufs[0].value, bufs[1].value,
+                                                   ln.value if ln else None, wt.value if wt else None,
+                                                   depth, 1 if final else 0, out.value, C.byref(st), None),
+               "pvx_radiosity_compute_ambient")
+        hip.synchronize()
+        d = {}
+        for k, _ in A.PvxRadiosityStats._fields_:
+            v = getattr(st, k)
+            d[k] = list(v) if hasattr(v, "__len__") else v
+        self.last_stats = d
+        return out.download(np.float32, n * 3).reshape(n, 3)

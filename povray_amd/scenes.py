@@ -595,6 +595,70 @@ def csg_scene(seed: int = 0xC0FFEE + 2, n_objects: int = 10000, width=1280, heig
     )
 
 
+def scale_transform(s):
+    """Compute_Scaling_Transform for a uniform factor s."""
+    m = np.eye(4)
+    inv = np.eye(4)
+    m[0, 0] = m[1, 1] = m[2, 2] = s
+    inv[0, 0] = inv[1, 1] = inv[2, 2] = 1.0 / s
+    return m, inv
+
+
+def torus_scene(seed: int = 0xC0FFEE + 7, n: int = 48, width=320, height=240, sturm: bool = False) -> Scene:
+    """`n` top-level tori on an 8-wide grid, randomly rotated.  Two in three
+    are ordinary (R U[0.5,1.5], r = U[0.1,0.3]*R); every third is a thin ring
+    with R = 1e4, r = 200 in its own space scaled by 1e-4 (R 1, r 0.02 in the
+    world), whose quartic's coefficients span more than 12 decades, so that
+    difficult_coeffs (polynomialsolver.cpp:1047-1087) sends it to the Sturm
+    solver even without `sturm`.  In-plane rays through the centres come
+    from `torus_plane_rays`."""
+    r = Rng(seed)
+    R_, r_, xf = [], [], []
+    for i in range(n):
+        pos = np.array([(i % 8) * 3.0 - 10.5, (i // 8) * 3.0 - 3.0 * (n // 8) / 2.0 + 1.5, 10.0])
+        rot = rotation_transform(r.uniform(3, 0.0, 360.0))
+        if i % 3 == 2:
+            R, rr = 1.0e4, 200.0
+            t = compose(compose(scale_transform(1e-4), rot), translation_transform(pos))
+        else:
+            R = float(r.uniform(1, 0.5, 1.5)[0])
+            rr = R * float(r.uniform(1, 0.1, 0.3)[0])
+            t = compose(rot, translation_transform(pos))
+        R_.append(R)
+        r_.append(rr)
+        xf.append(_xf_row(t))
+    objects = np.zeros((n, 4), np.uint32)
+    objects[:, 0] = A.PVX_TORUS
+    objects[:, 1] = np.arange(n)
+    alb = r.uniform(3 * n).reshape(n, 3)
+    return Scene(
+        name=f"tori{n}", objects=objects,
+        torus_major=np.array(R_), torus_minor=np.array(r_), torus_transform=np.arange(n, dtype=np.int32),
+        torus_sturm=np.full(n, 1 if sturm else 0, np.uint8), transforms=np.array(xf).reshape(-1, 32),
+        camera=look_at_camera((0.0, 0.0, -8.0), (0.0, 0.0, 10.0), width, height),
+        lights=[dict(kind="point", position=np.array([30.0, 40.0, -30.0]), colour=np.ones(3))],
+        materials=dict(pigment=alb, diffuse=np.full(n, 0.6), ambient=np.full(n, 0.1),
+                       phong=np.full(n, 0.3), phong_size=np.full(n, 40.0)),
+    )
+
+
+def torus_plane_rays(scene: Scene, per_torus: int = 16, distance: float = 6.0) -> np.ndarray:
+    """Rays lying in each torus's own plane and passing through its centre,
+    from `distance` away.  Their quartic is symmetric about the centre (a
+    biquadratic after the depressing shift), so the resolvent cubic's root
+    makes d1 = 2z - p vanish up to rounding: the SMALL_ENOUGH clamp of
+    solve_quartic (polynomialsolver.cpp:1325-1450)."""
+    out = []
+    for row in scene.transforms[scene.torus_transform]:
+        m = row[:16].reshape(4, 4)
+        c = m[3, :3]
+        for a in np.linspace(0.0, np.pi, per_torus, endpoint=False):
+            u = np.cos(a) * m[0, :3] + np.sin(a) * m[2, :3]
+            u = u / np.linalg.norm(u)
+            out.append(np.concatenate([c + distance * u, -u]))
+    return np.array(out, np.float64)
+
+
 # --------------------------------------------------------------------------
 # radiosity room
 # --------------------------------------------------------------------------

@@ -106,6 +106,36 @@ def test_csg_scene(gpu):
     _compare(g, r, "csg800 vs brute force")
 
 
+def test_csg_scene_without_sturm(gpu):
+    """The device's closed-form quartic (solve_quartic, solve_cubic,
+    difficult_coeffs) on the CSG field's tori."""
+    s = scenes.csg_scene(n_objects=800, width=320, height=180, sturm=False)
+    rays = scenes.camera_rays_np(s.camera)
+    g = gpu.GpuScene(s).find_intersection(rays)
+    r = OracleScene(s).find_intersection(rays, method=0)
+    _compare(g, r, "csg800 no sturm vs brute force")
+
+
+def test_tori_without_sturm_every_branch(gpu):
+    """Top-level tori without `sturm`: ordinary ones take the Ferrari
+    quartic, thin scaled rings have coefficients over 12 decades apart so
+    difficult_coeffs switches them to Sturm, and in-plane rays through the
+    centres take the d1 clamp of solve_quartic (the oracle counts each)."""
+    from oracle import oracle as O
+
+    s = scenes.torus_scene()
+    rays = np.concatenate([scenes.camera_rays_np(s.camera), scenes.torus_plane_rays(s)])
+    O.solver_branches()
+    r = OracleScene(s).find_intersection(rays, method=1)
+    br = O.solver_branches()
+    assert br["quartic"] and br["difficult"] and br["d1_clamp"], br
+    g = gpu.GpuScene(s).find_intersection(rays)
+    # ids identical; depths within 1e-5 relative (the closed-form quartic
+    # goes through cbrt / acos / cos, whose last bits differ between the
+    # device's and the host's math libraries)
+    _compare(g, r, "tori48 no sturm (camera + in-plane rays)")
+
+
 def test_empty_batch(gpu):
     s = scenes.sphere_scene(n=10)
     assert gpu.GpuScene(s).find_intersection(np.zeros((0, 6))).shape == (0,)

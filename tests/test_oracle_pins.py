@@ -572,3 +572,42 @@ def test_grazing_hit_outside_fp32_box_is_culled(kind):
     # the shape's own test does hit the first ray (All_Intersections)
     d, _, cnt = o.all_intersections(rays[:1], [0])
     assert cnt[0] == 1 and d[0, 0] == pytest.approx(1.0)
+
+
+# ---------------------------------------------------------------------------
+# Torus without `sturm`: Ferrari quartic, difficult_coeffs, d1 clamp
+# (torus.cpp:241-329, polynomialsolver.cpp:1047-1087, 1325-1450)
+# ---------------------------------------------------------------------------
+def _torus_residual(s, obj, p):
+    """|(|q|^2 + R^2 - r^2)^2 - 4 R^2 (qx^2 + qz^2)| / (4 R^2 r^2) at the
+    torus-space point q of world point p (0 on the surface)."""
+    idx = s.objects[obj, 1]
+    inv = s.transforms[s.torus_transform[idx], 16:].reshape(4, 4)
+    q = np.c_[p, np.ones(len(p))] @ inv
+    R, r = s.torus_major[idx], s.torus_minor[idx]
+    k = (q[:, :3] ** 2).sum(1) + R * R - r * r
+    return np.abs(k * k - 4.0 * R * R * (q[:, 0] ** 2 + q[:, 2] ** 2)) / (4.0 * R * R * r * r)
+
+
+def test_torus_without_sturm_reaches_every_branch_and_hits_the_surface():
+    s = scenes.torus_scene(width=160, height=120)
+    rays = np.concatenate([scenes.camera_rays_np(s.camera), scenes.torus_plane_rays(s)])
+    o = OracleScene(s)
+    O.solver_branches()
+    h = o.find_intersection(rays, method=0)
+    br = O.solver_branches()
+    assert br["quartic"] > 0 and br["difficult"] > 0 and br["d1_clamp"] > 0 and br["sturm"] == 0, br
+    hit = h["obj"] >= 0
+    assert hit.sum() > 500
+    p = rays[hit, :3] + h["t"][hit, None] * rays[hit, 3:]
+    res = np.array([_torus_residual(s, ob, pp[None])[0] for ob, pp in zip(h["obj"][hit], p)])
+    # distance to the surface is about res * r / 2: within 5e-6 of the tube radius
+    assert res.max() < 1e-5, res.max()
+    # the closed-form and Sturm solvers find the same first crossings
+    s2 = scenes.torus_scene(width=160, height=120, sturm=True)
+    h2 = OracleScene(s2).find_intersection(rays, method=0)
+    assert O.solver_branches()["sturm"] > 0
+    same = h["obj"] == h2["obj"]
+    assert same.mean() > 0.999
+    both = same & hit
+    np.testing.assert_allclose(h["t"][both], h2["t"][both], rtol=1e-6)
