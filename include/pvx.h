@@ -314,6 +314,21 @@ int pvx_trace_closest_counted(pvx_scene *scene, const pvx_ray *rays,
                               pvx_hit *hits, uint32_t *node_visits,
                               uint32_t *prim_tests, void *stream);
 
+/* pvx_trace_closest without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, results and checks as pvx_trace_closest; the
+ * traversal-stack spill area is a fixed 32 MiB device array of the library
+ * (one per device, held until the process ends) instead of the per-stream
+ * area pvx_trace_closest allocates on a stream's first launch, so the
+ * persistent grid is capped at what it holds (819 blocks of 256 lanes).
+ * Every call on one device shares that array: calls that overlap in time on
+ * one device (other streams, other threads) race and must not be issued.
+ * The scene must live on the current device (PVX_EINVAL otherwise). */
+int pvx_trace_closest_noalloc(pvx_scene *scene, const pvx_ray *rays,
+                              const double *max_depth, uint32_t ray_kind, size_t n,
+                              pvx_hit *hits, void *stream);
+
 /* Traversal variants (every one gives the same hits). */
 enum {
     PVX_TRAVERSAL_DEFAULT = 0,   /* by scene content and ray kind                     */

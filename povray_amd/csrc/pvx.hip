@@ -193,13 +193,46 @@ static void scene_free(pvx_scene *s) {
 // is too deep for the traversal stack takes the binary tree (family 1).
 static uint32_t kernel_level(const SceneView &v) { return v.has_complex == 0 && !v.wide ? 1u : v.has_complex; }
 
+// The no-allocation closest-hit entry point keeps its traversal-stack spill
+// area in this fixed device array (one copy per device).
+namespace {
+
+constexpr size_t kNoAllocScratchBytes = (size_t)32 << 20; // 32 MiB per device
+constexpr uint32_t kNoAllocMaxBlocks = (uint32_t)(kNoAllocScratchBytes / (kBlock * kSpillPerThread));
+constexpr int kMaxDevices = 64;
+
+static __device__ __align__(256) int32_t scratch[kNoAllocScratchBytes / sizeof(int32_t)];
+
+std::mutex g_scratch_mu;
+int32_t *g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device (each device holds
+// its own copy), looked up once per device.
+int scratch_address(int32_t **out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(PVX_EDEVICE, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(PVX_EDEVICE, "hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch_ptr[dev] = static_cast<int32_t *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return PVX_OK;
+}
+
+} // namespace
+
 extern "C" int pvx_init(int device) {
     int n = 0;
     hipError_t e = hipGetDeviceCount(&n);
     if (e != hipSuccess || n == 0) return fail(PVX_EDEVICE, "no HIP device available");
     if (device < 0 || device >= n) return fail(PVX_EINVAL, "device index out of range");
     HIP_TRY(hipSetDevice(device));
-    return PVX_OK;
+    int32_t *p = nullptr;
+    return scratch_address(&p); // the no-allocation entry point's array on this device
 }
 
 extern "C" int pvx_scene_create(const pvx_scene_desc *desc, const pvx_build_opts *opts,
@@ -524,6 +557,30 @@ extern "C" int pvx_trace_closest_counted(pvx_scene *s, const pvx_ray *rays,
     if (ray_kind & ~0xFFu) return fail(PVX_EINVAL, "unknown ray kind bits");
     return launch_closest<true>(s, rays, max_depth, ray_kind, nullptr, n, hits, node_visits, prim_tests,
                                 stream);
+}
+
+// ----------------------------------------------------------------------------
+// closest hit without allocation: the traversal-stack spill area is a fixed
+// device array instead of the per-stream area allocated on first use
+// ----------------------------------------------------------------------------
+extern "C" int pvx_trace_closest_noalloc(pvx_scene *s, const pvx_ray *rays, const double *max_depth,
+                                         uint32_t ray_kind, size_t n, pvx_hit *hits, void *stream) {
+    if (!s) return fail(PVX_EINVAL, "null scene");
+    if (ray_kind & ~0xFFu) return fail(PVX_EINVAL, "unknown ray kind bits");
+    if (n == 0) return PVX_OK;
+    if (!rays || !hits) return fail(PVX_EINVAL, "null ray or hit buffer");
+    if (n > 0xFFFFFFFFull) return fail(PVX_EINVAL, "too many rays in one call (at most 2^32 - 1)");
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev != s->device) return fail(PVX_EINVAL, "the scene lives on another device than the current one");
+    int32_t *spill = nullptr;
+    int r;
+    if ((r = scratch_address(&spill))) return r;
+    const int cls = (ray_kind & 0xFFu) == PVX_RAY_SHADOW ? 1 : 0;
+    uint32_t grid = s->tune_grid[cls] ? s->tune_grid[cls] : s->grid;
+    grid = std::min(grid, kNoAllocMaxBlocks); // the persistent grid is what the scratch array holds
+    return launch_variant<false>(s, rays, max_depth, ray_kind, nullptr, n, hits, nullptr, nullptr,
+                                 static_cast<hipStream_t>(stream), s->tune_variant[cls], grid, spill);
 }
 
 // ----------------------------------------------------------------------------

@@ -46,6 +46,7 @@ def lib():
         L.pvx_scene_get_stats.argtypes = [vp, C.POINTER(A.PvxSceneStats)]
         L.pvx_trace_closest.argtypes = [vp, vp, vp, u32, sz, vp, vp]
         L.pvx_trace_closest_counted.argtypes = [vp, vp, vp, u32, sz, vp, vp, vp, vp]
+        L.pvx_trace_closest_noalloc.argtypes = [vp, vp, vp, u32, sz, vp, vp]
         L.pvx_all_intersections.argtypes = [vp, vp, vp, sz, u32, vp, vp, vp, vp]
         L.pvx_inside.argtypes = [vp, vp, vp, sz, vp, vp]
         L.pvx_camera_rays.argtypes = [C.POINTER(A.PvxCamera), vp, vp]
@@ -92,7 +93,7 @@ EXPORTED_SYMBOLS = (
     "pvx_batch_create", "pvx_batch_destroy", "pvx_batch_trace_closest", "pvx_batch_trace_closest_counted",
     "pvx_clock_rate", "pvx_scene_autotune", "pvx_scene_set_traversal",
     "pvx_render_autotune", "pvx_radiosity_save", "pvx_radiosity_load",
-    "pvx_radiosity_begin", "pvx_radiosity_compute_ambient",
+    "pvx_radiosity_begin", "pvx_radiosity_compute_ambient", "pvx_trace_closest_noalloc",
 )
 
 
