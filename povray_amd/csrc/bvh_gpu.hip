@@ -5,7 +5,7 @@
 // 834-927: largest-extent axis, sort by box centre, prefix/suffix surface-area
 // sweep).  Here the tree is built on the device as a linear BVH:
 //   1. centroid bounds (block reduction, ordered-int atomics);
-//   2. 63-bit Morton codes of the primitive centroids, radix-sorted (hipcub);
+//   2. 63-bit Morton codes of the primitive centroids, radix-sorted (rocPRIM);
 //   3. the binary radix tree over the sorted codes (Karras, "Maximizing
 //      Parallelism in the Construction of BVHs, Octrees, and k-d Trees",
 //      HPG 2012): one thread per internal node, equal codes split by index;
@@ -25,7 +25,8 @@
 // Any tree over the primitives gives the same closest hits (ties are broken
 // by object and primitive id), so the build changes speed, not results.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
 #include <cstring>
@@ -402,10 +403,10 @@ int gpu_build_bvh(const float *d_lo, const float *d_hi, const PrimRec *d_prims_i
         (r = tmp.get(&newidx, n)))
         return r;
 
-    // scratch for hipcub: the larger of the sort and the scans
+    // scratch for rocPRIM: the larger of the sort and the scans
     size_t sort_bytes = 0, scan_bytes = 0;
-    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, keys_in, keys, vals_in, vals, ni, 0, 63, st));
-    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, keep, newidx, ni, st));
+    HIP_TRY(rocprim::radix_sort_pairs(nullptr, sort_bytes, keys_in, keys, vals_in, vals, ni, 0u, 63u, st));
+    HIP_TRY(rocprim::exclusive_scan(nullptr, scan_bytes, keep, newidx, 0u, (size_t)ni, rocprim::plus<uint32_t>(), st));
     void *cub = nullptr;
     size_t cub_bytes = std::max(sort_bytes, scan_bytes);
     unsigned char *cub_p = nullptr;
@@ -417,7 +418,7 @@ int gpu_build_bvh(const float *d_lo, const float *d_hi, const PrimRec *d_prims_i
     k_centroid_bounds<<<std::min<unsigned>(blocks(n), 1024u), kB, 0, st>>>(d_lo, d_hi, n, bounds);
     k_morton<<<blocks(n), kB, 0, st>>>(d_lo, d_hi, n, bounds, keys_in, vals_in);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(cub, cub_bytes, keys_in, keys, vals_in, vals, ni, 0, 63, st));
+    HIP_TRY(rocprim::radix_sort_pairs(cub, cub_bytes, keys_in, keys, vals_in, vals, ni, 0u, 63u, st));
     // 3: radix tree
     k_fill_i32<<<blocks(n - 1), kB, 0, st>>>(done, n - 1, -1);
     k_radix_tree<<<blocks(n - 1), kB, 0, st>>>(keys, ni, child, range);
@@ -438,7 +439,7 @@ int gpu_build_bvh(const float *d_lo, const float *d_hi, const PrimRec *d_prims_i
     // 5: compaction into the binary layout
     k_keep<<<blocks(n), kB, 0, st>>>(range, ni, max_leaf, keep);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(cub, cub_bytes, keep, newidx, ni, st));
+    HIP_TRY(rocprim::exclusive_scan(cub, cub_bytes, keep, newidx, 0u, (size_t)ni, rocprim::plus<uint32_t>(), st));
     uint32_t n_nodes = 0;
     int32_t root_h = 0;
     HIP_TRY(hipMemcpyAsync(&n_nodes, newidx + (n - 1), 4, hipMemcpyDeviceToHost, st));
@@ -467,7 +468,7 @@ int gpu_build_bvh(const float *d_lo, const float *d_hi, const PrimRec *d_prims_i
             (r = tmp.get(&cnt, n_nodes + 1)) || (r = tmp.get(&offs, n_nodes + 1)))
             return r;
         size_t wscan = 0;
-        HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, wscan, cnt, offs, (int)n_nodes + 1, st));
+        HIP_TRY(rocprim::exclusive_scan(nullptr, wscan, cnt, offs, 0u, (size_t)n_nodes + 1, rocprim::plus<uint32_t>(), st));
         if (wscan > cub_bytes) return fail(PVX_EDEVICE, "GPU BVH build: scan scratch too small");
         k_fill_i32<<<1, kB, 0, st>>>(fa, 1, 0); // the root
         uint32_t m = 1, base = 0, levels = 0;
@@ -475,7 +476,7 @@ int gpu_build_bvh(const float *d_lo, const float *d_hi, const PrimRec *d_prims_i
             if (base + m > n_nodes) return fail(PVX_EDEVICE, "GPU BVH build: wide tree overflow");
             k_wide_count<<<blocks(m), kB, 0, st>>>(out.nodes, fa, m, cnt);
             HIP_TRY(hipGetLastError());
-            HIP_TRY(hipcub::DeviceScan::ExclusiveSum(cub, cub_bytes, cnt, offs, (int)m + 1, st));
+            HIP_TRY(rocprim::exclusive_scan(cub, cub_bytes, cnt, offs, 0u, (size_t)m + 1, rocprim::plus<uint32_t>(), st));
             uint32_t next = 0;
             HIP_TRY(hipMemcpyAsync(&next, offs + m, 4, hipMemcpyDeviceToHost, st));
             HIP_TRY(hipStreamSynchronize(st));

@@ -17,7 +17,8 @@
 // pass).  A query keeps running sums (sum w*E, sum w), so after a round only
 // the round's new records (their own small grid) are visited again.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_select.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
 
 #include <algorithm>
 #include <chrono>
@@ -1255,13 +1256,11 @@ int RadiosityEngine::compact(const uint8_t *flags, uint32_t n, uint32_t *out, ui
     if (n == 0) return PVX_OK;
     int r;
     if ((r = d_count.reserve(4))) return r;
-    hipcub::CountingInputIterator<uint32_t> it(0);
+    rocprim::counting_iterator<uint32_t> it(0u);
     size_t tb = 0;
-    HIP_TRY(hipcub::DeviceSelect::Flagged(nullptr, tb, it, flags, out, static_cast<uint32_t *>(d_count.p), n,
-                                          stream));
+    HIP_TRY(rocprim::select(nullptr, tb, it, flags, out, static_cast<uint32_t *>(d_count.p), (size_t)n, stream));
     if ((r = cub_tmp.reserve(std::max<size_t>(tb, 16)))) return r;
-    HIP_TRY(hipcub::DeviceSelect::Flagged(cub_tmp.p, tb, it, flags, out, static_cast<uint32_t *>(d_count.p), n,
-                                          stream));
+    HIP_TRY(rocprim::select(cub_tmp.p, tb, it, flags, out, static_cast<uint32_t *>(d_count.p), (size_t)n, stream));
     return fetch(&count, d_count.p, 4);
 }
 
