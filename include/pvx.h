@@ -118,7 +118,8 @@ typedef struct pvx_csg_node {
 /* mesh2 data as Parse_Mesh2 receives it (parser/parser.cpp:4048-4600):
  * FP32 vertex_vectors, face_indices, optional per-vertex normal_vectors
  * (normal indices = face indices, the "number_of_normals == number_of_vertices"
- * rule at parser.cpp:4494-4509).  Vertices are in world space. */
+ * rule at parser.cpp:4494-4509).  Vertices are in the mesh's own space
+ * (world space unless pvx_scene_desc.mesh_transform places the mesh). */
 typedef struct pvx_mesh_desc {
     uint32_t n_vertices;
     uint32_t n_triangles;
@@ -215,6 +216,14 @@ typedef struct pvx_scene_desc {
     const uint32_t *polygon_first;  /* n + 1 offsets, in points                 */
     const double *polygon_points;   /* 3 per point                              */
     const int32_t *polygon_transform; /* n: index into transforms or -1; or NULL */
+
+    /* the placement of each mesh (Mesh::Transform, core/shape/mesh.cpp:568-580):
+     * its vertices stay in the mesh's own space; a ray is moved into that space
+     * and its direction normalised (Mesh::Intersect, mesh.cpp:179-230), hit
+     * depths are divided by the direction's length (test_hit, mesh.cpp:1218-1243),
+     * normals leave through MTransNormal (Mesh::Normal, mesh.cpp:342-380) and
+     * the object box is the mesh box through Recompute_BBox */
+    const int32_t *mesh_transform;  /* n_meshes: index into transforms or -1; or NULL */
 } pvx_scene_desc;
 
 /* ---- rays and hits ----------------------------------------------------- */
@@ -631,6 +640,34 @@ int pvx_radiosity_compute_ambient(pvx_scene *scene, size_t n, const double *ipoi
                                   const double *raw_normal, const double *layer_normal,
                                   const double *weight, uint32_t depth, uint32_t final_trace,
                                   float *ambient_rgb, pvx_radiosity_stats *stats, void *stream);
+
+/* Radiosity frames of every scene of a batch.  No reference counterpart
+ * (the reference renders one scene per process): for each scene the image
+ * and statistics are exactly those of pvx_radiosity_render with its camera
+ * and settings (cams, settings: n_scenes host structs; rgb: n_scenes device
+ * buffers of 3*width*height floats).  The frames run on n_streams HIP
+ * streams at once (1..16), one host thread each, largest frames first; the
+ * scenes' BVHs stay device-resident side by side.  Learned-mode training
+ * launches (cooperative kernels) are serialised across the streams.
+ * scene_stats (n_scenes, may be NULL) receives each frame's statistics,
+ * scene_done_ms (n_scenes, may be NULL) the host wall time from the batch
+ * start to each scene's last kernel. */
+typedef struct pvx_batch_radiosity_stats {
+    double batch_ms;            /* host wall clock of the whole batch            */
+    double gpu_frame_ms;        /* sum of the frames' own GPU event times        */
+    uint64_t pixels;            /* camera rays, one per pixel                    */
+    uint64_t gather_rays;       /* rays traced by plain gathers                  */
+    uint64_t surrogate_queries; /* final-pass misses answered by the surrogate   */
+    uint64_t fallbacks;         /* misses sent back to the plain gather          */
+    uint64_t final_misses;      /* final-pass points with no reusable record     */
+    uint32_t streams;
+    uint32_t pad;
+} pvx_batch_radiosity_stats;
+
+int pvx_batch_radiosity_render(pvx_batch *batch, const pvx_camera *cams,
+                               const pvx_radiosity_settings *settings, float *const *rgb,
+                               uint32_t n_streams, pvx_radiosity_stats *scene_stats,
+                               double *scene_done_ms, pvx_batch_radiosity_stats *stats);
 
 /* ---- irradiance surrogate ------------------------------------------------ */
 /* No reference counterpart: a small MLP that stands in for

@@ -157,9 +157,14 @@ struct Mesh {
 };
 void mesh_prepare(Mesh &m, const pvx_mesh_desc &d);                // Parse_Mesh2 tail + Build_Mesh_BBox_Tree
 bool mesh_triangle_intersect(const Mesh &m, const MeshTri &t, const Ray &r, double *depth); // mesh.cpp:1040-1136
-void mesh_all_intersections(const Mesh &m, const Ray &r, std::vector<Isect> &stack, bool use_tree);
+// Mesh::Intersect (mesh.cpp:179-230): tr places the mesh (NULL: none); the
+// depths pushed are world depths (test_hit, mesh.cpp:1218-1243)
+void mesh_all_intersections(const Mesh &m, const Transform *tr, const Ray &r, std::vector<Isect> &stack,
+                            bool use_tree);
 bool mesh_inside(const Mesh &m, V3 p);
-V3 mesh_normal(const Mesh &m, int tri, V3 ipoint);                   // Mesh::Normal + Smooth_Mesh_Normal
+V3 mesh_normal(const Mesh &m, int tri, V3 ipoint);                   // Smooth_Mesh_Normal, mesh space
+V3 mesh_normal_world(const Mesh &m, const Transform *tr, int tri, V3 ipoint); // Mesh::Normal (mesh.cpp:342-380)
+FBox recompute_bbox(const FBox &b, const Transform *tr);             // Recompute_BBox
 
 // Box::Intersect (box.cpp:172-505)
 bool box_intersect(const Ray &r, const Transform *tr, V3 c1, V3 c2, double *d1, double *d2,
@@ -270,6 +275,7 @@ struct Scene {
     std::vector<PolygonO> polygons;
     std::vector<Transform> trs;
     std::vector<Mesh> meshes;
+    std::vector<int32_t> mesh_tr;     // each mesh's placement (trs index) or -1
     std::vector<pvx_csg_node> csg;
     std::vector<pvx_object> csg_children;
     BBoxTree tree;

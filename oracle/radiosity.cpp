@@ -68,12 +68,14 @@ struct Rad {
     std::vector<V3> pool;
     size_t cursor[8] = {0};
     std::vector<Record> recs[8];
-    double cell = 0.25;
+    double cell = 0.25; // lookup grid cell (an index only: results do not depend on it)
     std::unordered_map<uint64_t, std::vector<int>> grid[8];
     uint64_t gathers[8] = {0}, rays = 0;
 
     Rad(const Scene &sc, const std::vector<pvx_material> &m, const pvx_radiosity_settings &set, V3 c)
         : s(sc), mats(m), st(set), cam(c) {
+        // reuse radii grow with the distance to the eye: scale the cells with it
+        cell = 0.25 * std::max(1.0, length(c) / 5.0);
         pool.resize(kPoolSize);
         for (int i = 0; i < kPoolSize; i++) { // HaltonCosWeightedDirectionGenerator::operator[]
             double r = std::sqrt(halton(i, 2));

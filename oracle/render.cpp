@@ -68,7 +68,7 @@ V3 shape_normal(const Scene &s, uint32_t type, uint32_t index, const Isect &h, V
     case PVX_PLANE:
         return s.pl_n[index];
     case PVX_MESH:
-        return mesh_normal(s.meshes[index], h.prim, ip);
+        return mesh_normal_world(s.meshes[index], s.tr(s.mesh_tr[index]), h.prim, ip);
     case PVX_TRIANGLE:
     case PVX_SMOOTH_TRIANGLE:
         return tri_normal(s.tris[index], ip);

@@ -162,7 +162,7 @@ bool Scene::all_intersections(uint32_t type, uint32_t index, uint32_t flags, con
     }
     case PVX_MESH: {
         size_t before = st.size();
-        mesh_all_intersections(meshes[index], r, st, true);
+        mesh_all_intersections(meshes[index], tr(mesh_tr[index]), r, st, true);
         return st.size() > before;
     }
     case PVX_BOX: { // Box::All_Intersections (box.cpp:100-150)
@@ -364,6 +364,13 @@ static Scene *scene_create(const pvx_scene_desc &d) {
         std::memcpy(t.m, d.transforms + 32 * (size_t)i, 16 * sizeof(double));
         std::memcpy(t.inv, d.transforms + 32 * (size_t)i + 16, 16 * sizeof(double));
         s->trs.push_back(t);
+    }
+    s->mesh_tr.assign(d.n_meshes, -1);
+    for (uint32_t i = 0; i < d.n_meshes; i++) { // Mesh::Transform: Recompute_BBox of the mesh box
+        const int32_t t = d.mesh_transform ? d.mesh_transform[i] : -1;
+        if (t < -1 || t >= (int32_t)d.n_transforms) throw std::runtime_error("mesh transform index out of range");
+        s->mesh_tr[i] = t;
+        if (t >= 0) s->meshes[i].bbox = recompute_bbox(s->meshes[i].bbox, &s->trs[t]);
     }
     for (uint32_t i = 0; i < d.n_boxes; i++) {
         s->box_c1.push_back(V3(d.box_corner1[3 * i], d.box_corner1[3 * i + 1], d.box_corner1[3 * i + 2]));

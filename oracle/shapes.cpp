@@ -178,12 +178,21 @@ bool mesh_triangle_intersect(const Mesh &m, const MeshTri &t, const Ray &r, doub
 // Mesh::Intersect (mesh.cpp:179-230) for an untransformed mesh (len = 1):
 // brute force when use_tree is false, else intersect_bbox_tree
 // (mesh.cpp:1452-1560) with its best-first early-out.
-void mesh_all_intersections(const Mesh &m, const Ray &r, std::vector<Isect> &stack, bool use_tree) {
+void mesh_all_intersections(const Mesh &m, const Transform *tr, const Ray &wr, std::vector<Isect> &stack,
+                            bool use_tree) {
     double depth;
+    Ray r = wr;
+    double len = 1.0;
+    if (tr) { // MInvTransRay, then the direction normalised
+        r.o = mtrans_point(tr->inv, wr.o);
+        r.d = mtrans_direction(tr->inv, wr.d);
+        len = length(r.d);
+        r.d = r.d / len;
+    }
     if (!use_tree || m.tree.root < 0) {
         for (size_t i = 0; i < m.tris.size(); i++)
             if (mesh_triangle_intersect(m, m.tris[i], r, &depth))
-                stack.push_back(Isect{depth, (int32_t)i, 0, -1});
+                stack.push_back(Isect{depth / len, (int32_t)i, 0, -1});
         return;
     }
     RayInfo ri(r);
@@ -198,14 +207,31 @@ void mesh_all_intersections(const Mesh &m, const Ray &r, std::vector<Isect> &sta
         if (!n.kids.empty()) {
             for (int32_t k : n.kids) check_and_enqueue(q, m.tree.nodes[k], k, ri);
         } else if (mesh_triangle_intersect(m, m.tris[n.element], r, &depth)) {
-            stack.push_back(Isect{depth, n.element, 0, -1});
+            stack.push_back(Isect{depth / len, n.element, 0, -1});
             best = depth;
         }
     }
 }
 
-// Mesh::Normal (mesh.cpp:380-420) + Smooth_Mesh_Normal (mesh.cpp:2366-2394).
+// Mesh::Normal (mesh.cpp:342-380): a placed mesh interpolates at the hit
+// point in its own space and sends the normal out through MTransNormal.
+static V3 mesh_normal_raw(const Mesh &m, int tri, V3 ip);
+
+V3 mesh_normal_world(const Mesh &m, const Transform *tr, int tri, V3 ip) {
+    if (!tr) return mesh_normal(m, tri, ip);
+    V3 n = mesh_normal_raw(m, tri, mtrans_point(tr->inv, ip));
+    return normalized(minv_trans_normal(tr->inv, n));
+}
+
+// Mesh::Normal without a placement: the face normal, or Smooth_Mesh_Normal
+// normalised.
 V3 mesh_normal(const Mesh &m, int tri, V3 ip) {
+    V3 n = mesh_normal_raw(m, tri, ip);
+    return m.tris[tri].smooth ? normalized(n) : n;
+}
+
+// Smooth_Mesh_Normal (mesh.cpp:410-439) or the face normal, in mesh space.
+static V3 mesh_normal_raw(const Mesh &m, int tri, V3 ip) {
     const MeshTri &t = m.tris[tri];
     auto nrm = [&](int k) {
         return V3(m.normals[3 * k], m.normals[3 * k + 1], m.normals[3 * k + 2]);
@@ -225,7 +251,7 @@ V3 mesh_normal(const Mesh &m, int tri, V3 ip) {
         double v = (pip1[axis] / u + k1 - k2) / (k3 - k2);
         res = n1 + u * (n2 - n1 + v * (n3 - n2));
     }
-    return normalized(res);
+    return res;
 }
 
 bool mesh_inside(const Mesh &, V3) {

@@ -236,8 +236,8 @@ bool torus_inside(V3 p, const Transform *tr, double R, double rr, bool inverted)
     return (r2 <= rr * rr) ? !inverted : inverted;
 }
 
-// Recompute_BBox (boundingbox.cpp:190-240)
-static FBox recompute_bbox(const FBox &b, const Transform *tr) {
+// Recompute_BBox (boundingbox.cpp:168-210)
+FBox recompute_bbox(const FBox &b, const Transform *tr) {
     if (!tr) return b;
     V3 ll(b.ll[0], b.ll[1], b.ll[2]), len(b.size[0], b.size[1], b.size[2]);
     V3 mins(kBoundHuge, kBoundHuge, kBoundHuge), maxs(-kBoundHuge, -kBoundHuge, -kBoundHuge);

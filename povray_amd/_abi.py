@@ -73,6 +73,7 @@ class PvxSceneDesc(C.Structure):
         ("n_quadrics", C.c_uint32), ("quadric_coef", c_dp), ("quadric_transform", c_i32p),
         ("n_polygons", C.c_uint32), ("polygon_first", C.POINTER(C.c_uint32)), ("polygon_points", c_dp),
         ("polygon_transform", c_i32p),
+        ("mesh_transform", c_i32p),
     ]
 
 
@@ -169,6 +170,12 @@ class PvxRadiosityStats(C.Structure):
                 ("final_misses", C.c_uint64), ("surrogate_queries", C.c_uint64),
                 ("fallbacks", C.c_uint64), ("probes", C.c_uint64), ("probe_error", C.c_float),
                 ("train_loss", C.c_float)]
+
+
+class PvxBatchRadiosityStats(C.Structure):
+    _fields_ = [("batch_ms", C.c_double), ("gpu_frame_ms", C.c_double), ("pixels", C.c_uint64),
+                ("gather_rays", C.c_uint64), ("surrogate_queries", C.c_uint64), ("fallbacks", C.c_uint64),
+                ("final_misses", C.c_uint64), ("streams", C.c_uint32), ("pad", C.c_uint32)]
 
 
 def radiosity_settings(scene, mode=PVX_RADIOSITY_PLAIN, **over):

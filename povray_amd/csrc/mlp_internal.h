@@ -1,5 +1,5 @@
 // Irradiance surrogate: Fourier features of (position, normal) -> three
-// ReLU layers of width 64 -> RGB irradiance.  BF16 MFMA (32x32x8) with FP32
+// ReLU layers of width 64 -> RGB irradiance.  BF16 MFMA (32x32x16) with FP32
 // accumulation for the forward pass, the backward pass and the weight
 // gradients; FP32 master weights with Adam.
 #ifndef PVX_MLP_INTERNAL_H
@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 #include "internal.h"
 
@@ -38,21 +39,23 @@ struct Surrogate {
     DevBuf w, m, v;           // FP32 master weights and Adam moments
     DevBuf pack;              // BF16 weight copies laid out for the kernels + FP32 biases
     DevBuf acts, partial, loss_part, grad, feats;
-    size_t acts_batch = 0;
+    DevBuf bar, sched;          // cooperative trainer: grid barrier, per-step lr / bias corrections
+    std::vector<float> hsched;  // host copy of sched (kept until the copy has run)
 
     int init(const float *host_params, uint64_t seed, hipStream_t st);
     // Loss and full gradient over n points (x: n*6 floats, t: n*3 floats, device).
     int grad_batch(const float *x, const float *t, uint32_t n, float *loss, float *grad_host, hipStream_t st);
-    // `steps` Adam steps on mini-batches of `batch` points drawn from n.
+    // `steps` Adam steps on mini-batches of `batch` points drawn from n, in
+    // one cooperative launch; batch is a multiple of 64 up to max_train_batch().
     int train(const float *x, const float *t, uint32_t n, uint32_t steps, uint32_t batch, float lr,
               float *final_loss, hipStream_t st);
     int infer(const float *x, uint32_t n, float *y, hipStream_t st);
     int get_params(float *host, hipStream_t st);
 
+    static int max_train_batch(); // 64 x co-resident trainer blocks on the current device
+
   private:
-    int fwd_bwd(const float *x, const uint16_t *F, const float *t, uint32_t n, uint32_t batch, bool sample,
-                uint32_t sample_seed, hipStream_t st);
-    int ensure_acts(size_t batch);
+    int fwd_bwd(const float *x, const float *t, uint32_t n, uint32_t batch, hipStream_t st);
 };
 
 } // namespace pvx

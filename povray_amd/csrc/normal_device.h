@@ -113,27 +113,37 @@ __device__ inline D3 leaf_normal(const NormalParams &p, uint32_t type, uint32_t 
 __device__ inline D3 hit_normal(const NormalParams &p, const pvx_hit &h, D3 o, D3 d, D3 ip) {
     const SceneView &sv = p.sv;
     uint32_t type = p.obj_type[h.obj], index = p.obj_index[h.obj];
-    if (type == PVX_MESH) { // Mesh::Normal + Smooth_Mesh_Normal (mesh.cpp:380-420, 2366-2394)
+    if (type == PVX_MESH) { // Mesh::Normal + Smooth_Mesh_Normal (mesh.cpp:342-380, 2366-2394)
         const ShadeTri &st = sv.shade_tris[sv.obj_tri_base[h.obj] + (uint32_t)h.prim];
+        const int32_t xf = sv.obj_mesh_xf[h.obj];
         if (!(st.vaxis_smooth >> 4)) {
             const float *fn = sv.mesh_nrm + 3 * (size_t)st.fn;
-            return D3{fn[0], fn[1], fn[2]};
+            const D3 n{fn[0], fn[1], fn[2]};
+            return xf < 0 ? n : normalize3(mtrans_normal(sv.xforms[xf].inv, n));
         }
+        // a placed mesh interpolates at the hit point in its own space
+        const D3 q = xf < 0 ? ip : xf_point(sv.xforms[xf].inv, ip);
         const float *n1 = sv.mesh_nrm + 3 * (size_t)st.n1, *n2 = sv.mesh_nrm + 3 * (size_t)st.n2,
                     *n3 = sv.mesh_nrm + 3 * (size_t)st.n3;
         const float *p1 = sv.mesh_vtx + 3 * (size_t)st.p1, *p2 = sv.mesh_vtx + 3 * (size_t)st.p2,
                     *p3 = sv.mesh_vtx + 3 * (size_t)st.p3;
-        D3 pip1{ip.x - (double)p1[0], ip.y - (double)p1[1], ip.z - (double)p1[2]};
+        D3 pip1{q.x - (double)p1[0], q.y - (double)p1[1], q.z - (double)p1[2]};
         double u = pip1.x * (double)st.perp[0] + pip1.y * (double)st.perp[1] + pip1.z * (double)st.perp[2];
         D3 N1{n1[0], n1[1], n1[2]};
-        if (u < kEpsilon) return normalize3(N1);
-        int axis = (int)(st.vaxis_smooth & 3u);
-        double k1 = p1[axis], k2 = p2[axis], k3 = p3[axis];
-        double v = (comp(pip1, axis) / u + k1 - k2) / (k3 - k2);
-        D3 N2{n2[0], n2[1], n2[2]}, N3{n3[0], n3[1], n3[2]};
-        D3 t1{N2.x - N1.x + v * (N3.x - N2.x), N2.y - N1.y + v * (N3.y - N2.y),
-              N2.z - N1.z + v * (N3.z - N2.z)};
-        return normalize3(D3{N1.x + u * t1.x, N1.y + u * t1.y, N1.z + u * t1.z});
+        D3 r;
+        if (u < kEpsilon) {
+            r = N1;
+        } else {
+            int axis = (int)(st.vaxis_smooth & 3u);
+            double k1 = p1[axis], k2 = p2[axis], k3 = p3[axis];
+            double v = (comp(pip1, axis) / u + k1 - k2) / (k3 - k2);
+            D3 N2{n2[0], n2[1], n2[2]}, N3{n3[0], n3[1], n3[2]};
+            D3 t1{N2.x - N1.x + v * (N3.x - N2.x), N2.y - N1.y + v * (N3.y - N2.y),
+                  N2.z - N1.z + v * (N3.z - N2.z)};
+            r = D3{N1.x + u * t1.x, N1.y + u * t1.y, N1.z + u * t1.z};
+        }
+        if (xf >= 0) r = mtrans_normal(sv.xforms[xf].inv, r);
+        return normalize3(r);
     }
     if (type == PVX_CSG) {
         const CsgChild &c = sv.csg_children[(uint32_t)h.prim];

@@ -298,6 +298,7 @@ extern "C" int pvx_scene_create(const pvx_scene_desc *desc, const pvx_build_opts
     UP(h.csg_leaf_range, csg_leaf_range);
     UP(h.shade_tris, shade_tris);
     UP(h.obj_tri_base, obj_tri_base);
+    UP(h.obj_mesh_xf, obj_mesh_xf);
     UP(h.mesh_vtx, mesh_vtx);
     UP(h.mesh_nrm, mesh_nrm);
     UP(h.triangles, triangles);

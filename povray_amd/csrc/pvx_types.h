@@ -107,7 +107,7 @@ struct alignas(16) PrimRec {
         SphRec sph;
         RefRec ref;
     };
-    uint32_t kind_axis; // kind | dominant_axis << 4
+    uint32_t kind_axis; // kind | dominant_axis << 4 | (mesh placement + 1) << 8 (mesh triangles)
     int32_t obj;        // object id
 };
 static_assert(sizeof(PrimRec) == 64, "prim record is 64 B");
@@ -267,6 +267,7 @@ struct SceneView {
     const uint32_t *csg_leaf_range;  // 2 per CSG node: [begin, end) into csg_leaves
     const ShadeTri *shade_tris;      // per mesh triangle (global index)
     const uint32_t *obj_tri_base;    // per object: first ShadeTri of a mesh object
+    const int32_t *obj_mesh_xf;      // per object: a mesh's placement (xforms index) or -1
     const float *mesh_vtx;           // 3 per vertex (all meshes)
     const float *mesh_nrm;           // 3 per normal (all meshes: vertex + face normals)
     const TriangleRec *triangles;    // stand-alone triangles (by triangle index)

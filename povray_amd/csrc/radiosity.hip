@@ -21,11 +21,15 @@
 #include <rocprim/iterator/counting_iterator.hpp>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
+#include <numeric>
+#include <thread>
 #include <vector>
 
 #include "internal.h"
@@ -1797,3 +1801,82 @@ extern "C" int pvx_radiosity_render(pvx_scene *s, const pvx_camera *cam, const p
     return PVX_OK;
 }
 
+
+// ----------------------------------------------------------------------------
+// many scenes: their radiosity frames on several streams at once
+// ----------------------------------------------------------------------------
+extern "C" int pvx_batch_radiosity_render(pvx_batch *b, const pvx_camera *cams,
+                                          const pvx_radiosity_settings *settings, float *const *rgb,
+                                          uint32_t n_streams, pvx_radiosity_stats *scene_stats,
+                                          double *scene_done_ms, pvx_batch_radiosity_stats *stats) {
+    if (!b || !cams || !settings || !rgb) return fail(PVX_EINVAL, "null argument");
+    const uint32_t n = (uint32_t)b->scenes.size();
+    if (n_streams == 0 || n_streams > 16) return fail(PVX_EINVAL, "n_streams must be in [1, 16]");
+    for (uint32_t k = 0; k < n; k++) {
+        if (!rgb[k]) return fail(PVX_EINVAL, "null image buffer for scene " + std::to_string(k));
+        const uint64_t px = (uint64_t)cams[k].width * cams[k].height;
+        if (px == 0 || px > 0x3FFFFFFFu) return fail(PVX_EINVAL, "bad image size for scene " + std::to_string(k));
+    }
+    // largest frames first, so that the streams finish together
+    std::vector<uint32_t> order(n);
+    std::iota(order.begin(), order.end(), 0u);
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t c) {
+        return (uint64_t)cams[a].width * cams[a].height > (uint64_t)cams[c].width * cams[c].height;
+    });
+    const uint32_t W = std::min(n_streams, n);
+    std::vector<hipStream_t> streams(W, nullptr);
+    for (uint32_t w = 0; w < W; w++) HIP_TRY(hipStreamCreate(&streams[w]));
+    std::vector<pvx_radiosity_stats> st(n);
+    std::vector<double> done(n, 0.0);
+    std::atomic<uint32_t> next{0};
+    std::mutex err_mu;
+    int err_rc = PVX_OK;
+    std::string err_msg;
+    const auto t0 = std::chrono::steady_clock::now();
+    auto worker = [&](uint32_t w) {
+        if (hipSetDevice(b->device) != hipSuccess) {
+            std::lock_guard<std::mutex> lock(err_mu);
+            if (!err_rc) err_rc = PVX_EDEVICE, err_msg = "hipSetDevice failed in a batch worker";
+            return;
+        }
+        for (;;) {
+            const uint32_t i = next.fetch_add(1);
+            if (i >= n) return;
+            {
+                std::lock_guard<std::mutex> lock(err_mu);
+                if (err_rc) return;
+            }
+            const uint32_t k = order[i];
+            const int rc = pvx_radiosity_render(b->scenes[k], &cams[k], &settings[k], rgb[k], &st[k], streams[w]);
+            if (rc) {
+                std::lock_guard<std::mutex> lock(err_mu);
+                if (!err_rc) err_rc = rc, err_msg = "scene " + std::to_string(k) + ": " + pvx_last_error();
+                return;
+            }
+            done[k] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        }
+    };
+    std::vector<std::thread> pool;
+    for (uint32_t w = 1; w < W; w++) pool.emplace_back(worker, w);
+    worker(0);
+    for (auto &t : pool) t.join();
+    const double wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    for (hipStream_t sm : streams) (void)hipStreamDestroy(sm);
+    if (err_rc) return fail(err_rc, err_msg);
+    if (scene_stats) std::copy(st.begin(), st.end(), scene_stats);
+    if (scene_done_ms) std::copy(done.begin(), done.end(), scene_done_ms);
+    if (stats) {
+        *stats = pvx_batch_radiosity_stats{};
+        stats->batch_ms = wall;
+        stats->streams = W;
+        for (uint32_t k = 0; k < n; k++) {
+            stats->pixels += (uint64_t)cams[k].width * cams[k].height;
+            stats->gather_rays += st[k].gather_rays;
+            stats->surrogate_queries += st[k].surrogate_queries;
+            stats->fallbacks += st[k].fallbacks;
+            stats->final_misses += st[k].final_misses;
+            stats->gpu_frame_ms += st[k].frame_ms;
+        }
+    }
+    return PVX_OK;
+}

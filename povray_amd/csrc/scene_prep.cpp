@@ -848,6 +848,7 @@ int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene
     out.obj_flags.resize(d.n_objects);
     out.obj_type.resize(d.n_objects);
     out.obj_tri_base.assign(d.n_objects, 0);
+    out.obj_mesh_xf.assign(d.n_objects, -1);
     for (uint32_t o = 0; o < d.n_objects; o++) {
         const pvx_object &ob = d.objects[o];
         if (!check_ref(ob.type, ob.index)) return PVX_EINVAL;
@@ -943,6 +944,33 @@ int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene
                 err = "mesh without vertices or indices";
                 return PVX_EINVAL;
             }
+            const int32_t mxf = d.mesh_transform ? d.mesh_transform[ob.index] : -1;
+            if (mxf < -1 || mxf >= (int32_t)d.n_transforms) {
+                err = "mesh transform index out of range";
+                return PVX_EINVAL;
+            }
+            out.obj_mesh_xf[o] = mxf;
+            if (mxf >= 0) {
+                // placed meshes are tested by the level-2 walk (the others keep
+                // the lighter triangle walks)
+                out.has_complex = std::max<uint32_t>(out.has_complex, 2u);
+                if ((uint32_t)mxf + 1u >= (1u << 24)) {
+                    err = "mesh transform index above the 24-bit record field";
+                    return PVX_EUNSUPPORTED;
+                }
+            }
+            // world bounds of a placed triangle: its vertices through the
+            // mesh's TRANSFORM (MTransPoint, matrix.cpp:415-427)
+            auto grow_world = [&](DBox &bb, const double *P) {
+                if (mxf < 0) {
+                    bb.grow(P);
+                    return;
+                }
+                const double *M = out.xforms[mxf].m;
+                double q[3];
+                for (int k = 0; k < 3; k++) q[k] = P[0] * M[k] + P[1] * M[4 + k] + P[2] * M[8 + k] + M[12 + k];
+                bb.grow(q);
+            };
             // shading data: vertices, vertex normals then face normals
             // (Normals[] layout of Parse_Mesh2, parser.cpp:4512-4565)
             int32_t vbase = (int32_t)(out.mesh_vtx.size() / 3);
@@ -1005,7 +1033,7 @@ int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene
                     smooth_triangle(Q1, Q2, P3, st);
                     if (smooth) st.vaxis_smooth |= 1u << 4;
                 }
-                pr.kind_axis = kPrimTriangle | (uint32_t)(axis << 4);
+                pr.kind_axis = kPrimTriangle | (uint32_t)(axis << 4) | ((uint32_t)(mxf + 1) << 8);
                 for (int k = 0; k < 3; k++) {
                     pr.tri.n[k] = (float)S[k];
                     pr.tri.p1[k] = q1[k];
@@ -1015,9 +1043,9 @@ int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene
                 pr.tri.dist = dist;
                 pr.tri.prim = (int32_t)t;
                 DBox bb = DBox::empty();
-                bb.grow(P1);
-                bb.grow(P2);
-                bb.grow(P3);
+                grow_world(bb, P1);
+                grow_world(bb, P2);
+                grow_world(bb, P3);
                 to_fbox(bb, lo, hi);
                 add_prim(pr, lo, hi);
                 out.n_triangles++;

@@ -42,6 +42,7 @@ struct HostScene {
     std::vector<uint32_t> csg_leaf_range;
     std::vector<ShadeTri> shade_tris;
     std::vector<uint32_t> obj_tri_base;
+    std::vector<int32_t> obj_mesh_xf; // per object: a mesh's placement (xforms index) or -1
     std::vector<float> mesh_vtx, mesh_nrm;
     uint32_t any_flags = 0;
     uint32_t has_complex = 0; // 0: triangles/spheres, 1: + boxes, 2: + tori or CSG

@@ -27,6 +27,35 @@ __device__ inline float safe_inv(double d) {
     return 1.0f / f;
 }
 
+// Mesh::Intersect for one mesh2 triangle: for a placed mesh the ray moves
+// into the mesh's own space with its direction normalised (mesh.cpp:179-230)
+// and the depth returns to world units divided by that length (test_hit,
+// mesh.cpp:1218-1243).  kind_axis: kind | axis << 4 | (placement + 1) << 8.
+__device__ __forceinline__ bool mesh_tri_hit(const SceneView &sv, const PrimRec &p, uint32_t ka, D3 o, D3 d,
+                                             double &dep) {
+    const uint32_t axis = (ka >> 4) & 15u, xp = ka >> 8;
+    if (xp == 0) return tri_hit64(p.tri, axis, o, d, dep);
+    const Xform &x = sv.xforms[xp - 1];
+    const D3 P = xf_point(x.inv, o);
+    D3 D = xf_dir(x.inv, d);
+    const double len = sqrt(D.x * D.x + D.y * D.y + D.z * D.z);
+    D = D3{D.x / len, D.y / len, D.z / len};
+    double t;
+    if (!tri_hit64(p.tri, axis, P, D, t)) return false;
+    dep = t / len;
+    return true;
+}
+
+// The walks of scenes without placed meshes (kLevel < 2: the scene
+// preparation sends scenes with placed meshes to the level-2 walk) take the
+// unplaced test only, which keeps their register budget.
+template <int kLevel>
+__device__ __forceinline__ bool mesh_tri_hit_l(const SceneView &sv, const PrimRec &p, uint32_t ka, D3 o, D3 d,
+                                               double &dep) {
+    if (kLevel >= 2) return mesh_tri_hit(sv, p, ka, o, d, dep);
+    return tri_hit64(p.tri, (ka >> 4) & 15u, o, d, dep);
+}
+
 __device__ inline bool better(double dep, int32_t obj, int32_t prim, double bt, int32_t bo,
                               int32_t bp) {
     // strictly closer wins; an exact tie between two hits goes to the lower
@@ -290,7 +319,7 @@ __device__ __forceinline__ void trace_closest_body(const SceneView &sv0, const S
                 if (kFlags && !precondition(sv.obj_flags[obj], rk)) continue;
                 if (pk == kPrimTriangle) {
                     double dep;
-                    if (tri_hit64(p.tri, ka >> 4, o, d, dep) && depth_ok(dep, shadow) &&
+                    if (mesh_tri_hit_l<kLevel>(sv, p, ka, o, d, dep) && depth_ok(dep, shadow) &&
                         better(dep, obj, p.tri.prim, bt, bo, bp)) {
                         bt = dep;
                         bo = obj;
@@ -385,7 +414,7 @@ __device__ __forceinline__ void trace_closest_body(const SceneView &sv0, const S
                         if (kFlags && !precondition(sv.obj_flags[obj], rk)) continue;
                         if (pk == kPrimTriangle) {
                             double dep;
-                            if (tri_hit64(p.tri, ka >> 4, o, d, dep) && depth_ok(dep, shadow) &&
+                            if (mesh_tri_hit_l<kLevel>(sv, p, ka, o, d, dep) && depth_ok(dep, shadow) &&
                                 better(dep, obj, p.tri.prim, bt, bo, bp)) {
                                 bt = dep;
                                 bo = obj;
@@ -473,7 +502,7 @@ __device__ __forceinline__ void prim_test_any(const SceneView &sv, const PrimRec
     if (kFlags && !precondition(sv.obj_flags[obj], rk)) return;
     if (pk == kPrimTriangle) {
         double dep;
-        if (tri_hit64(p.tri, ka >> 4, o, d, dep) && depth_ok(dep, shadow) && better(dep, obj, p.tri.prim, bt, bo, bp)) {
+        if (mesh_tri_hit(sv, p, ka, o, d, dep) && depth_ok(dep, shadow) && better(dep, obj, p.tri.prim, bt, bo, bp)) {
             bt = dep;
             bo = obj;
             bp = p.tri.prim;

@@ -257,7 +257,7 @@ __global__ void k_all_intersections(SceneView sv, const pvx_ray *rays, const uin
                     const PrimRec &p = sv.prims[first + k];
                     if ((p.kind_axis & 15u) != kPrimTriangle || p.obj != (int32_t)ob) continue;
                     double dep;
-                    if (tri_hit64(p.tri, p.kind_axis >> 4, o, d, dep)) push(dep, p.tri.prim);
+                    if (mesh_tri_hit(sv, p, p.kind_axis, o, d, dep)) push(dep, p.tri.prim);
                 }
             }
             node = sp == 0 ? kEmpty : stack[--sp];

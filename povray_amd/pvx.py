@@ -67,6 +67,9 @@ def lib():
         L.pvx_radiosity_begin.argtypes = [vp, C.POINTER(A.PvxCamera), C.POINTER(A.PvxRadiositySettings), vp]
         L.pvx_radiosity_compute_ambient.argtypes = [vp, sz, vp, vp, vp, vp, u32, u32, vp,
                                                     C.POINTER(A.PvxRadiosityStats), vp]
+        L.pvx_batch_radiosity_render.argtypes = [vp, C.POINTER(A.PvxCamera), C.POINTER(A.PvxRadiositySettings),
+                                                 C.POINTER(vp), u32, C.POINTER(A.PvxRadiosityStats),
+                                                 C.POINTER(C.c_double), C.POINTER(A.PvxBatchRadiosityStats)]
         L.pvx_radiosity_save.argtypes = [vp, C.c_char_p, C.POINTER(C.c_uint64)]
         L.pvx_radiosity_load.argtypes = [vp, C.c_char_p, C.POINTER(C.c_uint64)]
         L.pvx_render_autotune.argtypes = [vp, C.POINTER(A.PvxCamera), vp, C.POINTER(A.PvxTuneResult),
@@ -94,6 +97,7 @@ EXPORTED_SYMBOLS = (
     "pvx_clock_rate", "pvx_scene_autotune", "pvx_scene_set_traversal",
     "pvx_render_autotune", "pvx_radiosity_save", "pvx_radiosity_load",
     "pvx_radiosity_begin", "pvx_radiosity_compute_ambient", "pvx_trace_closest_noalloc",
+    "pvx_batch_radiosity_render",
 )
 
 
@@ -339,6 +343,37 @@ class GpuBatch:
             self.close()
         except Exception:
             pass
+
+    def radiosity_render(self, mode=A.PVX_RADIOSITY_LEARNED, n_streams: int = 4, images=True, **over):
+        """Radiosity frames of every scene (pvx_batch_radiosity_render) with
+        each scene's camera and radiosity settings plus the overrides.
+        Returns (list of (h, w, 3) images or None, per-scene stats dicts,
+        per-scene completion times in ms, batch stats dict)."""
+        n = len(self.scenes)
+        cams = (A.PvxCamera * n)(*[g.scene.camera_struct() for g in self.scenes])
+        sets = (A.PvxRadiositySettings * n)(*[A.radiosity_settings(g.scene, mode=mode, **over)
+                                               for g in self.scenes])
+        outs = [hip.DeviceBuffer(c.width * c.height * 12) for c in cams]
+        ptrs = (C.c_void_p * n)(*[o.value for o in outs])
+        sst = (A.PvxRadiosityStats * n)()
+        done = (C.c_double * n)()
+        bst = A.PvxBatchRadiosityStats()
+        _check(lib().pvx_batch_radiosity_render(self.h, cams, sets, ptrs, n_streams, sst, done, C.byref(bst)),
+               "pvx_batch_radiosity_render")
+        hip.synchronize()
+
+        def todict(st):
+            d = {}
+            for k, _ in st._fields_:
+                v = getattr(st, k)
+                d[k] = list(v) if hasattr(v, "__len__") else v
+            return d
+
+        imgs = None
+        if images:
+            imgs = [o.download(np.float32, c.width * c.height * 3).reshape(c.height, c.width, 3)
+                    for o, c in zip(outs, cams)]
+        return imgs, [todict(x) for x in sst], list(done), todict(bst)
 
     def _first(self, ray_first):
         first = np.ascontiguousarray(ray_first, np.uint64)

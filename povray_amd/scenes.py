@@ -107,6 +107,7 @@ class Scene:
     polygon_first: np.ndarray = None         # (n + 1,) uint32: offsets into polygon_points (in points)
     polygon_points: np.ndarray = None        # (m, 3) float64: the statements' points
     polygon_transform: np.ndarray = None     # (n,) int32: transform index or -1
+    mesh_transform: np.ndarray = None        # (len(meshes),) int32: transform index or -1
     camera: dict = None
     lights: list = field(default_factory=list)
     materials: dict = field(default_factory=dict)  # per top-level object arrays
@@ -172,6 +173,8 @@ class Scene:
             keep.append(md)
             d.n_meshes = len(self.meshes)
             d.meshes = C.cast(md, C.POINTER(A.PvxMeshDesc))
+            if self.mesh_transform is not None:
+                d.mesh_transform, _ = arr(self.mesh_transform, np.int32, C.c_int32)
         if self.triangle_vertices is not None and len(self.triangle_vertices):
             d.triangle_vertices, _ = arr(np.asarray(self.triangle_vertices).reshape(-1), np.float64, C.c_double)
             d.n_triangles = int(np.asarray(self.triangle_vertices).reshape(-1, 9).shape[0])
@@ -604,6 +607,39 @@ def scale_transform(s):
     return m, inv
 
 
+def placed_mesh_scene(seed: int = 0xC0FFEE + 8, level: int = 4, width=320, height=240) -> Scene:
+    """Four instances of one displaced icosphere mesh2 (level 4, smoothed
+    normals): one unplaced, three with their own placement (rotation, a
+    non-uniform scale, translation), so the rays move into mesh space with a
+    direction of length != 1 (Mesh::Transform / Mesh::Intersect); a point
+    light and a ground plane."""
+    v, f, n = displaced_icosphere(level, seed)
+    r = Rng(seed)
+    xforms = []
+    for k in range(3):
+        sc = np.eye(4)
+        inv = np.eye(4)
+        s3 = r.uniform(3, 0.5, 1.6)
+        sc[0, 0], sc[1, 1], sc[2, 2] = s3
+        inv[0, 0], inv[1, 1], inv[2, 2] = 1.0 / s3
+        t = compose(compose((sc, inv), rotation_transform(r.uniform(3, 0.0, 360.0))),
+                    translation_transform([2.4 * (k - 1), 1.9 if k == 1 else 0.2 * k, 1.2 - 0.3 * k]))
+        xforms.append(_xf_row(t))
+    objects = np.array([[A.PVX_MESH, i, 0, 0] for i in range(4)] + [[A.PVX_PLANE, 0, 0, 0]], np.uint32)
+    nobj = objects.shape[0]
+    return Scene(
+        name=f"placed_meshes_L{level}", objects=objects, meshes=[(v, f, n)] * 4,
+        mesh_transform=np.array([-1, 0, 1, 2], np.int32), transforms=np.array(xforms).reshape(-1, 32),
+        plane_normal=np.array([[0.0, 1.0, 0.0]]), plane_distance=np.array([1.6]),
+        camera=look_at_camera((0.0, 1.2, -4.5), (0.0, 0.0, 1.2), width, height),
+        lights=[dict(kind="point", position=np.array([-5.0, 8.0, -6.0]), colour=np.ones(3))],
+        materials=dict(pigment=r.uniform(3 * nobj, 0.3, 0.9).reshape(nobj, 3), diffuse=np.full(nobj, 0.7),
+                       ambient=np.full(nobj, 0.1), phong=np.full(nobj, 0.4), phong_size=np.full(nobj, 40.0),
+                       reflection=np.where(np.arange(nobj) == 2, 0.3, 0.0)),
+        settings=dict(max_trace_level=5, adc_bailout=1.0 / 255.0),
+    )
+
+
 def torus_scene(seed: int = 0xC0FFEE + 7, n: int = 48, width=320, height=240, sturm: bool = False) -> Scene:
     """`n` top-level tori on an 8-wide grid, randomly rotated.  Two in three
     are ordinary (R U[0.5,1.5], r = U[0.1,0.3]*R); every third is a thin ring
@@ -792,6 +828,40 @@ def batch_plan(seed: int = 0xC0FFEE + 4, n_scenes: int = 64, tris=(1000, 500000)
     seeds = (r.bits(n_scenes) >> np.uint64(1)).astype(np.int64)
     return [dict(seed=int(seeds[k]), n_tris=int(round(np.exp(lt[k]))), n_spheres=int(ns[k]),
                  width=int(w[k]), height=int(h[k])) for k in range(n_scenes)]
+
+
+def batch_radiosity_scene(seed: int, n_tris: int, n_spheres: int, width: int, height: int,
+                          name: str = None) -> Scene:
+    """A batch scene lit by radiosity: the terrain and spheres of
+    `batch_scene` with plain diffuse finishes (diffuse 0.75, no reflection)
+    under an emissive ceiling panel 6 units above the terrain (a 12 x 12 box,
+    emission 3); count 50, recursion_limit 2, error_bound 0.5, learned mode
+    settings of the radiosity room scaled to the frame (train_samples = a
+    twentieth of the pixels, at least 1,024)."""
+    b = batch_scene(seed, n_tris, n_spheres, width, height, name)
+    n0 = b.n_objects
+    objects = np.concatenate([b.objects, np.array([[A.PVX_BOX, 0, 0, 0]], np.uint32)])
+    n = n0 + 1
+    pig = np.concatenate([b.materials["pigment"], np.ones((1, 3))])
+    emission = np.zeros((n, 3))
+    emission[n0] = 3.0
+    diffuse = np.full(n, 0.75)
+    diffuse[n0] = 0.0
+    px = width * height
+    return Scene(
+        name=b.name + "_rad", objects=objects, sphere_center=b.sphere_center, sphere_radius=b.sphere_radius,
+        meshes=b.meshes, box_corner1=np.array([[-6.0, 6.0, -6.0]]), box_corner2=np.array([[6.0, 6.1, 6.0]]),
+        box_transform=np.array([-1], np.int32), camera=b.camera, lights=[],
+        materials=dict(pigment=pig, diffuse=diffuse, ambient=np.zeros(n), phong=np.zeros(n),
+                       phong_size=np.full(n, 40.0), reflection=np.zeros(n), ior=np.ones(n),
+                       transmit=np.zeros(n), emission=emission),
+        settings=dict(max_trace_level=5, adc_bailout=1.0 / 255.0,
+                      radiosity=dict(count=50, recursion_limit=2, error_bound=0.5, minimum_reuse=0.015,
+                                     maximum_reuse=0.2, nearest_count=5, pretrace_start=0.08, pretrace_end=0.04,
+                                     brightness=1.0, adc_bailout=0.01, probe_threshold=0.02,
+                                     train_samples=max(1024, px // 20), train_steps=100, probe_tile=64,
+                                     probes_per_tile=1)),
+    )
 
 
 def batch_scenes(plan=None, **kw):

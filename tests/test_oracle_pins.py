@@ -611,3 +611,73 @@ def test_torus_without_sturm_reaches_every_branch_and_hits_the_surface():
     assert same.mean() > 0.999
     both = same & hit
     np.testing.assert_allclose(h["t"][both], h2["t"][both], rtol=1e-6)
+
+
+# ---------------------------------------------------------------------------
+# Placed mesh2 (Mesh::Transform / Mesh::Intersect / test_hit / Mesh::Normal,
+# mesh.cpp:179-230, 342-380, 568-580, 1218-1243)
+# ---------------------------------------------------------------------------
+def _moller_trumbore(o, d, a, b, c):
+    """Nearest t > 1e-9 of the ray against triangle abc (FP64, world space)."""
+    e1, e2 = b - a, c - a
+    p = np.cross(d, e2)
+    det = e1 @ p
+    if abs(det) < 1e-14:
+        return np.inf
+    s = o - a
+    u = (s @ p) / det
+    q = np.cross(s, e1)
+    v = (d @ q) / det
+    t = (e2 @ q) / det
+    if u < 0 or v < 0 or u + v > 1 or t <= 1e-9:
+        return np.inf
+    return t
+
+
+def test_placed_mesh_hits_and_normals_in_world_space():
+    s = scenes.placed_mesh_scene(level=2, width=64, height=48)
+    o = OracleScene(s)
+    rays = scenes.camera_rays_np(s.camera)
+    h = o.find_intersection(rays, method=1)
+    assert np.array_equal(h["obj"], o.find_intersection(rays, method=0)["obj"])
+    v, f, _ = s.meshes[0]
+    v = v.astype(np.float64)
+    checked = 0
+    for i in np.nonzero((h["obj"] >= 1) & (h["obj"] <= 3))[0][::7]:
+        obj, tri = int(h["obj"][i]), int(h["prim"][i])
+        m = s.transforms[s.mesh_transform[obj]][:16].reshape(4, 4)
+        w = v[f[tri]] @ m[:3, :3] + m[3, :3]  # the triangle in world space
+        t = _moller_trumbore(rays[i, :3], rays[i, 3:], *w)
+        if not np.isfinite(t):  # edge-ambiguous: decided by the FP32 mesh-space test
+            continue
+        # the reference's plane is FP32 (Distance, normal): 1e-6 relative
+        assert h["t"][i] == pytest.approx(t, rel=1e-6), (i, h["t"][i], t)
+        checked += 1
+    assert checked > 100
+    # a placed flat mesh is lit through its face normal sent out by the inverse
+    # transpose (MTransNormal): one triangle scaled non-uniformly and rotated,
+    # a point light, diffuse 1: the pixel is N.L
+    tri = np.array([[0, 1, 2]], np.int32)
+    pv = np.array([[-1.0, -1.0, 0.0], [1.0, -1.0, 0.0], [0.0, 1.0, 0.0]], np.float32)
+    place = scenes.compose((np.diag([1.0, 3.0, 1.0, 1.0]), np.diag([1.0, 1.0 / 3.0, 1.0, 1.0])),
+                           scenes.rotation_transform((35.0, 20.0, 0.0)))
+    light = np.array([3.0, 4.0, -6.0])
+    flat = _scene([(A.PVX_MESH, 0)], dict(diffuse=[1.0]), meshes=[(pv, tri, None)],
+                  lights=[dict(kind="point", position=light, colour=np.ones(3))],
+                  transforms=np.concatenate([place[0].ravel(), place[1].ravel()])[None],
+                  mesh_transform=np.array([0], np.int32))
+    fo = OracleScene(flat)
+    target = np.array([0.1, -0.2, 0.0]) @ place[0][:3, :3] + place[0][3, :3]
+    eye = np.array([0.5, 0.5, -6.0])
+    ray = _rays(eye, target - eye).reshape(1, 6)
+    hh = fo.find_intersection(ray, method=0)
+    assert hh["obj"][0] == 0
+    hp = ray[0, :3] + hh["t"][0] * ray[0, 3:]
+    np.testing.assert_allclose(hp, target, atol=1e-6)
+    n = place[1][:3, :3] @ np.array([0.0, 0.0, 1.0])
+    n /= np.linalg.norm(n)
+    if n @ ray[0, 3:] > 0:
+        n = -n
+    lv = _unit(light - hp)
+    rgb = fo.trace_rays(flat, ray)
+    np.testing.assert_allclose(rgb[0], np.full(3, max(n @ lv, 0.0)), rtol=2e-6, atol=1e-7)
