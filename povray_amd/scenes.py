@@ -117,6 +117,68 @@ class Scene:
     def n_objects(self):
         return int(self.objects.shape[0])
 
+    _ARRAYS = ("objects", "sphere_center", "sphere_radius", "plane_normal", "plane_distance", "box_corner1",
+               "box_corner2", "box_transform", "torus_major", "torus_minor", "torus_transform", "torus_sturm",
+               "transforms", "csg", "csg_children", "triangle_vertices", "triangle_normals", "cone_apex",
+               "cone_base", "cone_radius", "cone_flags", "cone_transform", "disc_center", "disc_normal",
+               "disc_radius", "disc_transform", "quadric_coef", "quadric_transform", "polygon_first",
+               "polygon_points", "polygon_transform", "mesh_transform")
+
+    def save(self, path):
+        """Write the scene to a compressed .npz (arrays, camera, lights,
+        materials, settings; no pickled objects)."""
+        out = {"name": np.array(self.name)}
+        for k in self._ARRAYS:
+            v = getattr(self, k)
+            if v is not None:
+                out["a_" + k] = np.asarray(v)
+        for i, (v, ix, nrm) in enumerate(self.meshes):
+            out[f"mesh{i}_v"], out[f"mesh{i}_f"] = np.asarray(v), np.asarray(ix)
+            if nrm is not None:
+                out[f"mesh{i}_n"] = np.asarray(nrm)
+        for k, v in (self.camera or {}).items():
+            out["cam_" + k] = np.asarray(v)
+        for i, lt in enumerate(self.lights):
+            for k, v in lt.items():
+                out[f"light{i}_{k}"] = np.asarray(v)
+        for k, v in self.materials.items():
+            out["mat_" + k] = np.asarray(v)
+        for k, v in self.settings.items():
+            if k == "radiosity":
+                for k2, v2 in v.items():
+                    out["rad_" + k2] = np.asarray(v2)
+            elif k != "warnings":
+                out["set_" + k] = np.asarray(v)
+        np.savez_compressed(path, **out)
+
+    @classmethod
+    def load(cls, path):
+        z = np.load(path, allow_pickle=False)
+        kw = {k[2:]: z[k] for k in z.files if k.startswith("a_")}
+        meshes, i = [], 0
+        while f"mesh{i}_v" in z.files:
+            meshes.append((z[f"mesh{i}_v"], z[f"mesh{i}_f"], z[f"mesh{i}_n"] if f"mesh{i}_n" in z.files else None))
+            i += 1
+        cam = {k[4:]: z[k] for k in z.files if k.startswith("cam_")}
+        for k in ("width", "height"):
+            if k in cam:
+                cam[k] = int(cam[k])
+        lights, i = [], 0
+        while any(k.startswith(f"light{i}_") for k in z.files):
+            lt = {k[len(f"light{i}_"):]: z[k] for k in z.files if k.startswith(f"light{i}_")}
+            lt["kind"] = str(lt["kind"])
+            if "size" in lt:
+                lt["size"] = tuple(int(x) for x in lt["size"])
+            lights.append(lt)
+            i += 1
+        mats = {k[4:]: z[k] for k in z.files if k.startswith("mat_")}
+        settings = {k[4:]: (z[k].item() if z[k].ndim == 0 else z[k]) for k in z.files if k.startswith("set_")}
+        rad = {k[4:]: z[k].item() for k in z.files if k.startswith("rad_")}
+        if rad:
+            settings["radiosity"] = rad
+        return cls(name=str(z["name"]), meshes=meshes, camera=cam or None, lights=lights, materials=mats,
+                   settings=settings, **kw)
+
     @property
     def n_triangles(self):
         return int(sum(m[1].shape[0] for m in self.meshes))

@@ -9,33 +9,52 @@ It follows the reference parser's reading of each statement it covers
 5539-5590, Parse_Mesh2 4048-4600, Parse_CSG 2197-2245, Parse_Light_Source,
 Parse_Object_Mods; parser_expressions.cpp for expressions):
 
-* values: floats, vectors `<a, b, c>`, `x y z`, `pi`, `+ - * /`, unary
-  minus, parentheses, `sqrt sin cos tan abs pow radians degrees`,
-  identifiers from `#declare` / `#local` (floats, vectors, colours, objects);
-* camera (location, look_at, sky, right/up/direction lengths; `angle` sets
-  the direction length from the right vector's), light_source (point,
-  area_light without jitter), background, global_settings (ambient_light,
-  max_trace_level, adc_bailout, radiosity count / recursion_limit /
-  error_bound / nearest_count);
+* the directive layer (Parse_Directive, parser_tokenizer.cpp:952):
+  #declare / #local (a macro call opens a local scope), #undef, #while,
+  #if / #elseif / #else, #ifdef / #ifndef, #macro with arguments, #default,
+  #version, #debug / #warning / #error, #include (colors.inc's common names
+  are built in; other include files are skipped with a warning);
+* values: floats, vectors `<a, b, c>` (commas optional), `x y z`, `pi`,
+  `+ - * /`, comparisons, `& | !`, `?:`, member access (`.x`, `.red`),
+  `sqrt sin cos tan abs pow exp log ln floor ceil int mod div min max atan2
+  vlength vnormalize vdot vcross select`, `rand` / `seed` as POV_rand
+  (core/math/randomsequence.cpp:113-118), `image_width` / `image_height`,
+  identifiers (floats, vectors, colours, objects, materials, lights);
+* camera (Parse_Camera for v3.5+: location, look_at, sky, right / up /
+  direction, angle, transforms), light_source (point, area_light without
+  jitter), background, global_settings (ambient_light, max_trace_level,
+  adc_bailout, radiosity count / recursion_limit / error_bound /
+  nearest_count);
 * sphere, box, plane, cylinder, cone, disc, torus (sturm), quadric, triangle,
-  smooth_triangle, polygon, mesh2 (vertex_vectors, normal_vectors, face_indices),
+  smooth_triangle, polygon, mesh2 (vertex_vectors, normal_vectors,
+  face_indices; uv and texture lists read and dropped),
   union / intersection / difference / merge, object { IDENT };
 * object modifiers: pigment / color / rgb / rgbf / rgbt, finish (ambient,
   diffuse, brilliance, phong, phong_size, reflection, emission), interior
   (ior), texture { pigment finish }, translate / rotate / scale / matrix,
   inverse, open, sturm, no_shadow / no_image / no_reflection /
-  no_radiosity;
-* `#include "colors.inc"` (its common colour names are known); other
-  includes, `#version` and unknown directives are errors.
+  no_radiosity.
 
-Anything else raises `SdlError` naming the line; nothing is silently
+What pvx does not render is reduced with a warning, collected in the
+scene's `settings["warnings"]`: a pattern pigment becomes the mean of its
+colour map, normals and finish extras (specular, roughness, metallic,
+irid ...) are dropped, as are photons and special camera and light options.
+A top-level union that holds triangles, meshes or polygons is split into its
+operands as Link_To_Frame splits unions (parser.cpp:9425-9440).
+
+Anything else raises `SdlError` naming the line; nothing else is silently
 dropped.  Transforms compose as Compose_Transforms does (row vectors); a
 sphere takes translations, rotations and uniform scales (Sphere::Translate /
-Rotate / Scale without an ellipsoid), meshes and triangles transform their
-vertices (Mesh::Transform, Triangle::Transform).
+Rotate / Scale without an ellipsoid), triangles transform their vertices
+(Triangle::Transform) and a mesh keeps its FP32 vertices in its own space
+with the composed transform as its placement (Mesh::Transform).  A mesh's
+object box is then the mesh box through that one composed transform, where
+the reference recomputes it once per modifier (a looser box that only culls
+differently at grazing rays).
 """
 from __future__ import annotations
 
+import collections
 import math
 import re
 
@@ -57,7 +76,8 @@ _TOKEN = re.compile(r"""
     (?P<str>"[^"\n]*") |
     (?P<dir>\#[A-Za-z_]+) |
     (?P<id>[A-Za-z_][A-Za-z_0-9]*) |
-    (?P<sym>[{}<>,()+\-*/=;])
+    (?P<sym2><=|>=|!=) |
+    (?P<sym>[{}<>,()+\-*/=;.!&|?:\[\]])
 """, re.S | re.X)
 
 # colors.inc (the names scenes use most)
@@ -120,6 +140,17 @@ class _Material(dict):
     """A declared texture / pigment / finish / interior: material fields."""
 
 
+class _Light(dict):
+    """A declared light_source."""
+
+    def copy(self):
+        return _Light({k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in self.items()})
+
+
+def _truth(v):
+    return bool(np.any(np.asarray(v) != 0.0))
+
+
 class _Parser:
     def __init__(self, text: str, width: int, height: int):
         self.toks = []
@@ -131,13 +162,23 @@ class _Parser:
                 raise SdlError(f"line {line}: cannot read {text[pos:pos + 20]!r}")
             kind = m.lastgroup
             val = m.group()
+            if kind == "sym2":
+                kind = "sym"
             if kind not in ("ws", "c1", "c2"):
                 self.toks.append((kind, val, line))
             line += val.count("\n")
             pos = m.end()
         self.i = 0
         self.width, self.height = width, height
-        self.decl = {}
+        # identifiers: #declare writes the global map, #local the innermost
+        # (a macro call opens one)
+        self.decl = collections.ChainMap({})
+        self.ctrl = []          # open #while / #if / macro frames
+        self.no_pump = 0        # > 0 while a directive's own expression is read
+        self.macros = {}        # name -> (parameter names, body start index)
+        self.rand_state = []    # seed() streams (Parser::stream_seed)
+        self.defaults = {}      # #default material fields
+        self.warnings = []      # what the reader reduced (textures, special cameras, light options)
         self.objects = []
         self.camera = None
         self.lights = []
@@ -147,14 +188,198 @@ class _Parser:
         self.radiosity = None
 
     # -- token helpers ----------------------------------------------------------------
-    def peek(self, k=0):
-        j = self.i + k
+    def raw(self):
+        """The token at the cursor without directive processing."""
+        j = self.i
         return self.toks[j] if j < len(self.toks) else ("eof", "", self.toks[-1][2] if self.toks else 1)
+
+    def peek(self):
+        if not self.no_pump:
+            self.pump()
+        return self.raw()
 
     def next(self):
         t = self.peek()
         self.i += 1
         return t
+
+    def warn(self, msg):
+        self.warnings.append(msg)
+
+    # -- the directive layer (Parser::Parse_Directive, parser_tokenizer.cpp:952):
+    # directives and macro calls are handled as the tokens arrive, so they may
+    # appear between statements and inside blocks alike --------------------------------
+    def pump(self):
+        while True:
+            kind, val, ln = self.raw()
+            if kind == "dir":
+                self.i += 1
+                self.directive(val, ln)
+            elif kind == "id" and val in self.macros:
+                self.i += 1
+                self.call_macro(val, ln)
+            else:
+                return
+
+    _OPENERS = ("#while", "#if", "#ifdef", "#ifndef", "#macro", "#switch")
+
+    def skip_block(self, stop_at_else):
+        """Skip tokens to the #end (or, when stop_at_else, an #else / #elseif)
+        closing the current block at this nesting depth; returns the directive
+        that stopped the skip (consumed)."""
+        depth = 0
+        while self.i < len(self.toks):
+            kind, val, ln = self.toks[self.i]
+            self.i += 1
+            if kind != "dir":
+                continue
+            if val in self._OPENERS:
+                depth += 1
+            elif val == "#end":
+                if depth == 0:
+                    return val
+                depth -= 1
+            elif depth == 0 and stop_at_else and val in ("#else", "#elseif"):
+                return val
+        raise SdlError("unterminated #while / #if / #macro block")
+
+    def cond_paren(self):
+        """A directive's condition: read without processing the directives
+        that follow it (its lookahead must not run them early)."""
+        return _truth(self.quiet(self.cond))
+
+    def quiet(self, fn):
+        self.no_pump += 1
+        try:
+            return fn()
+        finally:
+            self.no_pump -= 1
+
+    def directive(self, val, ln):
+        if val in ("#declare", "#local"):
+            kind, name, l2 = self.raw()
+            if kind != "id":
+                raise SdlError(f"line {ln}: expected an identifier after {val}")
+            self.i += 1
+            self.expect("=")
+            value = self.decl_value(ln)
+            if val == "#local":
+                self.decl[name] = value
+            else:  # an existing local is changed, else the global
+                for m in self.decl.maps:
+                    if name in m:
+                        m[name] = value
+                        break
+                else:
+                    self.decl.maps[-1][name] = value
+            self.accept(";")
+        elif val == "#undef":
+            name = self.raw()[1]
+            self.i += 1
+            for m in self.decl.maps:
+                m.pop(name, None)
+        elif val == "#while":
+            start = self.i - 1
+            if self.cond_paren():
+                self.ctrl.append(("while", start))
+            else:
+                self.skip_block(False)
+        elif val in ("#if", "#ifdef", "#ifndef"):
+            if val == "#if":
+                ok = self.cond_paren()
+            else:
+                self.expect("(")
+                name = self.raw()[1]
+                self.i += 1
+                self.expect(")")
+                ok = (name in self.decl) == (val == "#ifdef")
+            while not ok:
+                stop = self.skip_block(True)
+                if stop == "#end":
+                    return
+                ok = True if stop == "#else" else self.cond_paren()
+            self.ctrl.append(("if", None))
+        elif val in ("#else", "#elseif"):  # the branch taken ends here
+            if not self.ctrl or self.ctrl[-1][0] != "if":
+                raise SdlError(f"line {ln}: {val} without #if")
+            self.ctrl.pop()
+            self.skip_block(False)
+        elif val == "#end":
+            if not self.ctrl:
+                raise SdlError(f"line {ln}: #end without an open block")
+            what, data = self.ctrl.pop()
+            if what == "while":
+                self.i = data  # back to the #while: its condition is read again
+            elif what == "macro":
+                self.i = data
+                self.decl = self.decl.parents
+        elif val == "#macro":
+            kind, name, l2 = self.raw()
+            self.i += 1
+            self.expect("(")
+            params = []
+            while not self.accept(")"):
+                params.append(self.raw()[1])
+                self.i += 1
+                self.accept(",")
+            self.macros[name] = (params, self.i)
+            self.skip_block(False)
+        elif val == "#include":
+            f = self.next()[1].strip('"')
+            if f != "colors.inc":
+                self.warn(f"line {ln}: #include {f!r} skipped (colors.inc is built in)")
+        elif val == "#version":
+            self.quiet(self.cond)
+            self.accept(";")
+        elif val == "#default":
+            holder = _Obj("material", {}, ln)
+            self.expect("{")
+            while not self.accept("}"):
+                k = self.next()[1]
+                if k not in ("texture", "pigment", "finish", "normal", "interior"):
+                    raise SdlError(f"line {ln}: unsupported #default block {k!r}")
+                self.material(holder, k)
+            self.defaults.update(holder.mat)
+        elif val in ("#debug", "#warning"):
+            self.next()
+        elif val == "#error":
+            raise SdlError(f"line {ln}: #error {self.next()[1]}")
+        else:
+            raise SdlError(f"line {ln}: unsupported directive {val}")
+
+    def call_macro(self, name, ln):
+        params, body = self.macros[name]
+        self.expect("(")
+        args = []
+        while not self.accept(")"):
+            args.append(self.decl_value(ln))
+            self.accept(",")
+        if len(args) != len(params):
+            raise SdlError(f"line {ln}: macro {name} takes {len(params)} arguments")
+        self.ctrl.append(("macro", self.i))
+        self.decl = self.decl.new_child(dict(zip(params, args)))
+        self.i = body
+
+    def decl_value(self, ln):
+        """The right-hand side of #declare / #local, or a macro argument."""
+        kind, val, _ = self.peek()
+        if val in _SHAPES:
+            return self.shape()
+        if val == "light_source":
+            self.next()
+            return _Light(self.light_block())
+        if val in ("texture", "pigment", "finish", "interior", "normal"):
+            holder = _Obj("material", {}, ln)
+            self.material(holder, self.next()[1])
+            return _Material(holder.mat)
+        if val in ("color", "colour", "rgb", "rgbf", "rgbt", "rgbft"):
+            rgb, f, t = self.quiet(self.colour)
+            return np.concatenate([np.asarray(rgb, np.float64), [f, t]])
+        if kind == "id" and isinstance(self.decl.get(val), (_Obj, _Material, _Light)):
+            self.next()
+            v = self.decl[val]
+            return v.copy() if isinstance(v, _Obj) else v
+        return self.quiet(self.cond)
 
     def line(self):
         return self.peek()[2]
@@ -177,6 +402,40 @@ class _Parser:
         self.accept(",")
 
     # -- expressions --------------------------------------------------------------------
+    def cond(self):
+        """Full expression: ?:, |, &, comparisons, then arithmetic
+        (parser_expressions.cpp Parse_Float precedence)."""
+        v = self.lor()
+        if self.accept("?"):
+            a = self.cond()
+            self.expect(":")
+            b = self.cond()
+            return a if _truth(v) else b
+        return v
+
+    def lor(self):
+        v = self.land()
+        while self.accept("|"):
+            w = self.land()
+            v = float(_truth(v) or _truth(w))
+        return v
+
+    def land(self):
+        v = self.rel()
+        while self.accept("&"):
+            w = self.rel()
+            v = float(_truth(v) and _truth(w))
+        return v
+
+    def rel(self):
+        v = self.expr()
+        while self.peek()[1] in ("<", "<=", "=", "!=", ">=", ">"):
+            op = self.next()[1]
+            w = self.expr()
+            a, b = float(np.max(v)) if np.ndim(v) else float(v), float(np.max(w)) if np.ndim(w) else float(w)
+            v = float({"<": a < b, "<=": a <= b, "=": a == b, "!=": a != b, ">=": a >= b, ">": a > b}[op])
+        return v
+
     def expr(self):
         v = self.term()
         while self.peek()[1] in ("+", "-"):
@@ -194,10 +453,65 @@ class _Parser:
         return v
 
     _FUNCS = {"sqrt": np.sqrt, "sin": np.sin, "cos": np.cos, "tan": np.tan, "abs": np.abs,
-              "radians": np.radians, "degrees": np.degrees}
+              "radians": np.radians, "degrees": np.degrees, "exp": np.exp, "log": np.log10, "ln": np.log,
+              "asin": np.arcsin, "acos": np.arccos, "atan": np.arctan, "floor": np.floor, "ceil": np.ceil,
+              "vlength": np.linalg.norm}
+    _FUNCS2 = ("int", "mod", "min", "max", "rand", "seed", "atan2", "vnormalize", "vdot", "vcross", "select",
+               "div")
+
+    def func2(self, name, a, ln):
+        if name == "int":
+            return float(math.trunc(float(a[0])))
+        if name == "div":
+            return float(math.trunc(float(a[0]) / float(a[1])))
+        if name == "mod":
+            return float(math.fmod(float(a[0]), float(a[1])))
+        if name == "min":
+            return float(min(float(v) for v in a))
+        if name == "max":
+            return float(max(float(v) for v in a))
+        if name == "atan2":
+            return float(math.atan2(float(a[0]), float(a[1])))
+        if name == "vnormalize":
+            v = np.asarray(a[0], np.float64)
+            return v / np.linalg.norm(v)
+        if name == "vdot":
+            return float(np.dot(np.asarray(a[0])[:3], np.asarray(a[1])[:3]))
+        if name == "vcross":
+            return np.cross(np.asarray(a[0])[:3], np.asarray(a[1])[:3])
+        if name == "select":
+            return a[1] if float(a[0]) < 0 else (a[2] if len(a) == 3 or float(a[0]) == 0 else a[3])
+        if name == "seed":  # Parser::stream_seed (parser_expressions.cpp:3344-3351)
+            self.rand_state.append(int(float(a[0])) & 0xFFFFFFFF)
+            return float(len(self.rand_state) - 1)
+        if name == "rand":  # POV_rand (core/math/randomsequence.cpp:113-118)
+            k = int(float(a[0]))
+            if k < 0 or k >= len(self.rand_state):
+                raise SdlError(f"line {ln}: illegal random number generator")
+            self.rand_state[k] = (self.rand_state[k] * 1812433253 + 12345) & 0xFFFFFFFF
+            return self.rand_state[k] / 0xFFFFFFFF
+        raise SdlError(f"line {ln}: unknown function {name}")
 
     def factor(self):
+        v = self.primary()
+        # member access: .x .y .z (.u .v, .red .green .blue .filter .transmit)
+        while self.peek()[1] == ".":
+            self.next()
+            m = self.next()[1]
+            idx = {"x": 0, "u": 0, "red": 0, "y": 1, "v": 1, "green": 1, "z": 2, "blue": 2, "filter": 3,
+                   "transmit": 4, "t": 3}.get(m)
+            if idx is None or np.ndim(v) == 0 or idx >= np.shape(v)[0]:
+                raise SdlError(f"line {self.line()}: bad member .{m}")
+            v = float(np.asarray(v)[idx])
+        return v
+
+    _CONST = {"true": 1.0, "yes": 1.0, "on": 1.0, "false": 0.0, "no": 0.0, "off": 0.0, "clock": 0.0}
+
+    def primary(self):
         kind, val, ln = self.peek()
+        if val == "!":
+            self.next()
+            return float(not _truth(self.factor()))
         if val == "-":
             self.next()
             return -self.factor()
@@ -209,15 +523,17 @@ class _Parser:
             return float(val)
         if val == "(":
             self.next()
-            v = self.expr()
+            v = self.cond()
             self.expect(")")
             return v
-        if val == "<":
+        if val == "<":  # components separated by commas, which may be left out (Parse_Comma)
             self.next()
             comps = [self.expr()]
-            while self.accept(","):
+            while not self.accept(">"):
+                self.accept(",")
+                if self.peek()[0] == "eof":
+                    raise SdlError(f"line {ln}: unterminated vector")
                 comps.append(self.expr())
-            self.expect(">")
             return np.array([float(c) for c in comps])
         if kind == "id":
             self.next()
@@ -225,6 +541,19 @@ class _Parser:
                 return np.eye(3)["xyz".index(val)].copy()
             if val == "pi":
                 return math.pi
+            if val in self._CONST and val not in self.decl:
+                return self._CONST[val]
+            if val == "image_width":
+                return float(self.width)
+            if val == "image_height":
+                return float(self.height)
+            if val in self._FUNCS2:
+                self.expect("(")
+                args = [self.cond()]
+                while self.accept(","):
+                    args.append(self.cond())
+                self.expect(")")
+                return self.func2(val, args, ln)
             if val in self._FUNCS:
                 self.expect("(")
                 a = self.expr()
@@ -240,8 +569,10 @@ class _Parser:
             if val in ("rgb", "color", "colour"):
                 self.i -= 1
                 return self.colour()
-            if val in self.decl and not isinstance(self.decl[val], (_Obj, _Material)):
+            if val in self.decl and not isinstance(self.decl[val], (_Obj, _Material, _Light)):
                 return self.decl[val]
+            if val in _COLOURS:
+                return np.array(list(_COLOURS[val]) + [0.0, 0.0], np.float64)
             raise SdlError(f"line {ln}: unknown identifier {val!r}")
         raise SdlError(f"line {ln}: expected a value, found {val!r}")
 
@@ -261,10 +592,11 @@ class _Parser:
         return float(v)
 
     def colour(self):
-        """color [rgb|rgbf|rgbt] <..> or a colour identifier: (rgb, filter, transmit)."""
+        """color [rgb|rgbf|rgbt|rgbft] <..>, or a colour expression (a named or
+        declared colour, possibly scaled): (rgb, filter, transmit).  Colours
+        are kept as 5-vectors r, g, b, filter, transmit."""
         self.accept("color") or self.accept("colour")
         kind = self.peek()[1]
-        filt = trans = 0.0
         if kind in ("rgb", "rgbf", "rgbt", "rgbft"):
             self.next()
             v = self.expr()
@@ -272,41 +604,53 @@ class _Parser:
             n = {"rgb": 3, "rgbf": 4, "rgbt": 4, "rgbft": 5}[kind]
             if v.shape[0] < n:
                 v = np.concatenate([v, np.zeros(n - v.shape[0])])
+            filt = trans = 0.0
             if kind == "rgbf":
                 filt = v[3]
             elif kind == "rgbt":
                 trans = v[3]
             elif kind == "rgbft":
                 filt, trans = v[3], v[4]
-            return np.asarray(v[:3]), filt, trans
-        t = self.next()
-        if t[1] in _COLOURS and t[1] not in self.decl:
-            return np.asarray(_COLOURS[t[1]], np.float64), 0.0, 0.0
-        if t[1] in self.decl:
-            v = self.decl[t[1]]
-            if isinstance(v, tuple):
-                return v
-            return np.asarray(v, np.float64)[:3], 0.0, 0.0
-        if t[1] == "<":
-            self.i -= 1
-            return self.vector(), 0.0, 0.0
-        raise SdlError(f"line {t[2]}: expected a colour, found {t[1]!r}")
+            return np.asarray(v[:3]), float(filt), float(trans)
+        v = self.expr()
+        v = np.full(3, float(v)) if np.ndim(v) == 0 else np.asarray(v, np.float64)
+        if v.shape[0] < 3:
+            raise SdlError(f"line {self.line()}: expected a colour")
+        f = float(v[3]) if v.shape[0] > 3 else 0.0
+        t = float(v[4]) if v.shape[0] > 4 else 0.0
+        return np.asarray(v[:3]), f, t
+
+    def skip_braces(self):
+        """Skip a { ... } block (the opening brace next)."""
+        self.expect("{")
+        depth = 1
+        while depth:
+            t = self.next()
+            if t[0] == "eof":
+                raise SdlError("unterminated block")
+            depth += (t[1] == "{") - (t[1] == "}")
 
     # -- blocks ---------------------------------------------------------------------------
     def skip_block_to_end(self):
         self.expect("}")
 
     def parse(self):
+        self.run()
+        if self.ctrl:
+            raise SdlError(f"unterminated {self.ctrl[-1][0]} block at the end of the scene")
+        return self
+
+    def run(self):
         while self.peek()[0] != "eof":
             kind, val, ln = self.peek()
-            if kind == "dir":
-                self.directive()
-            elif val == "camera":
+            if kind == "eof":
+                break
+            if val == "camera":
                 self.next()
                 self.camera_block()
             elif val == "light_source":
                 self.next()
-                self.light_block()
+                self.lights.append(self.light_block())
             elif val == "background":
                 self.next()
                 self.expect("{")
@@ -321,43 +665,41 @@ class _Parser:
                 self.next()
             else:
                 self.err(f"unsupported statement {val!r}")
-        return self
 
-    def directive(self):
-        _, val, ln = self.next()
-        if val in ("#declare", "#local"):
-            name = self.next()
-            if name[0] != "id":
-                raise SdlError(f"line {ln}: expected an identifier after {val}")
-            self.expect("=")
-            if self.peek()[1] in _SHAPES:
-                self.decl[name[1]] = self.shape()
-            elif self.peek()[1] in ("texture", "pigment", "finish", "interior"):
-                holder = _Obj("material", {}, ln)
-                self.material(holder, self.next()[1])
-                self.decl[name[1]] = _Material(holder.mat)
-            elif self.peek()[1] in ("color", "colour", "rgb", "rgbf", "rgbt", "rgbft"):
-                self.decl[name[1]] = self.colour()
-            else:
-                self.decl[name[1]] = self.expr()
-            self.accept(";")
-        elif val == "#include":
-            f = self.next()
-            if f[1].strip('"') != "colors.inc":
-                raise SdlError(f"line {ln}: #include {f[1]} is not supported (colors.inc is)")
-        elif val == "#version":
-            self.expr()
-            self.accept(";")
-        else:
-            raise SdlError(f"line {ln}: unsupported directive {val}")
+    def transform_mod(self, val, ln):
+        """translate / rotate / scale / matrix as a (matrix, inverse) pair, or None."""
+        if val == "translate":
+            return scenes.translation_transform(self.vector())
+        if val == "rotate":
+            return scenes.rotation_transform(self.vector())
+        if val == "scale":
+            return _scale(self.vector())
+        if val == "matrix":
+            v = np.asarray(self.expr(), np.float64)
+            if v.shape != (12,):
+                raise SdlError(f"line {ln}: matrix takes 12 values")
+            m = np.eye(4)
+            m[:, :3] = v.reshape(4, 3)
+            return m, np.linalg.inv(m)
+        return None
 
     def camera_block(self):
+        """Parse_Camera for language version 3.5 and later (parser.cpp:1335-1830):
+        the modifiers are collected, then applied in a fixed order — angle
+        (direction length from the right vector's), look_at (with the
+        handedness of the supplied up / direction / right), then the camera's
+        transforms (Camera::Transform)."""
         self.expect("{")
-        loc, look, sky = np.zeros(3), np.array([0.0, 0.0, 1.0]), np.array([0.0, 1.0, 0.0])
-        angle = None
+        loc, dirv, up = np.zeros(3), np.array([0.0, 0.0, 1.0]), np.array([0.0, 1.0, 0.0])
+        right, sky = np.array([1.33, 0.0, 0.0]), np.array([0.0, 1.0, 0.0])
+        look, angle = None, None
+        tr = _identity()
         while not self.accept("}"):
             _, val, ln = self.next()
-            if val == "location":
+            t = self.transform_mod(val, ln)
+            if t is not None:
+                tr = _compose(tr, t)
+            elif val == "location":
                 loc = self.vector()
             elif val == "look_at":
                 look = self.vector()
@@ -365,28 +707,74 @@ class _Parser:
                 sky = self.vector()
             elif val == "angle":
                 angle = self.flt()
-            elif val in ("right", "up", "direction"):
-                self.vector()  # the default aspect / unit lengths are used
+            elif val == "right":
+                right = self.vector()
+            elif val == "up":
+                up = self.vector()
+            elif val == "direction":
+                dirv = self.vector()
             elif val == "perspective":
                 pass
+            elif val in ("orthographic", "fisheye", "ultra_wide_angle", "omnimax", "panoramic", "spherical",
+                         "cylinder"):
+                if val == "cylinder":
+                    self.flt()
+                self.warn(f"line {ln}: {val} camera read as perspective")
+            elif val in ("focal_point", "aperture", "blur_samples", "confidence", "variance"):
+                self.cond()
+                self.warn(f"line {ln}: camera {val} ignored (no focal blur)")
             else:
                 raise SdlError(f"line {ln}: unsupported camera keyword {val!r}")
-        cam = scenes.look_at_camera(loc, look, self.width, self.height, sky)
-        if angle is not None:  # direction length from the right vector (parser.cpp, Parse_Camera)
-            rl = np.linalg.norm(cam["right"])
-            cam["direction"] = cam["direction"] / np.linalg.norm(cam["direction"]) * (
-                0.5 * rl / math.tan(math.radians(angle) * 0.5))
-        self.camera = cam
+        if angle is not None and angle > 0.0:
+            if angle >= 180.0:
+                raise SdlError("viewing angle has to be smaller than 180 degrees")
+            dirv = dirv / np.linalg.norm(dirv) * (np.linalg.norm(right) / math.tan(angle * math.pi / 360.0) / 2.0)
+        if look is not None:
+            dl, ul, rl = np.linalg.norm(dirv), np.linalg.norm(up), np.linalg.norm(right)
+            hand = float(np.dot(np.cross(up, dirv), right))
+            d = look - loc
+            if float(np.dot(d, d)) < 1e-10:
+                raise SdlError("camera location and look_at point must be different")
+            d = d / np.linalg.norm(d)
+            r = np.cross(sky, d)
+            if np.all(np.abs(r) < 1e-10):
+                self.warn("camera look_at direction parallel to sky: the supplied right vector is kept")
+                r = right
+            r = r / np.linalg.norm(r)
+            up = np.cross(d, r) * ul
+            dirv = d * dl
+            right = r * (rl if hand > 0.0 else -rl)
+        m = tr[0]
+        loc = loc @ m[:3, :3] + m[3, :3]
+        dirv, up, right = dirv @ m[:3, :3], up @ m[:3, :3], right @ m[:3, :3]
+        self.camera = dict(location=loc, direction=dirv, right=right, up=up, width=int(self.width),
+                           height=int(self.height))
 
     def light_block(self):
+        """Parse_Light_Source: the light (a declared one copied, or a position
+        and colour), area_light, modifiers; returns the light dict.  The
+        renderer's area lights trace every grid point without jitter: jitter,
+        adaptive, circular and orient are read and reduced with a warning, as
+        are spotlight / cylinder shapes and fading (the light becomes a plain
+        point or area light)."""
         self.expect("{")
-        pos = self.vector()
-        self.comma()
-        col = np.ones(3)
-        light = dict(kind="point", position=pos, colour=col)
+        kind0, val0, _ = self.peek()
+        if kind0 == "id" and isinstance(self.decl.get(val0), _Light):
+            self.next()
+            light = self.decl[val0].copy()
+        else:
+            light = dict(kind="point", position=self.vector(), colour=np.ones(3))
+            self.comma()
         while not self.accept("}"):
             kind, val, ln = self.peek()
-            if val in ("color", "colour", "rgb") or (kind == "id" and val in _COLOURS):
+            if val in ("translate", "rotate", "scale", "matrix"):
+                self.next()
+                m = self.transform_mod(val, ln)[0]
+                light["position"] = light["position"] @ m[:3, :3] + m[3, :3]
+                for k in ("axis1", "axis2"):
+                    if k in light:
+                        light[k] = light[k] @ m[:3, :3]
+            elif val in ("color", "colour", "rgb") or (kind == "id" and val in _COLOURS):
                 light["colour"] = self.colour()[0]
             elif val == "area_light":
                 self.next()
@@ -401,12 +789,27 @@ class _Parser:
             elif val == "adaptive":
                 self.next()
                 if self.flt() < 100:
-                    raise SdlError(f"line {ln}: adaptive below 100 is not supported (every grid point is traced)")
-            elif val in ("jitter", "orient", "circular"):
-                raise SdlError(f"line {ln}: area light {val} is not supported")
+                    self.warn(f"line {ln}: area light adaptive read as full sampling (every grid point is traced)")
+            elif val in ("jitter", "orient", "circular", "spotlight", "cylinder", "parallel", "shadowless",
+                         "media_interaction", "media_attenuation"):
+                self.next()
+                if val in ("media_interaction", "media_attenuation"):
+                    self.cond()
+                self.warn(f"line {ln}: light_source {val} reduced (plain point / area light)")
+            elif val in ("radius", "falloff", "tightness", "fade_power", "fade_distance"):
+                self.next()
+                self.cond()
+                self.warn(f"line {ln}: light_source {val} ignored")
+            elif val == "point_at":
+                self.next()
+                self.vector()
+            elif val == "looks_like":
+                self.next()
+                self.skip_braces()
+                self.warn(f"line {ln}: light_source looks_like ignored")
             else:
                 raise SdlError(f"line {ln}: unsupported light_source keyword {val!r}")
-        self.lights.append(light)
+        return light
 
     def global_settings(self):
         self.expect("{")
@@ -428,10 +831,25 @@ class _Parser:
                     _, k, l2 = self.next()
                     if k in ("count", "recursion_limit", "nearest_count"):
                         self.radiosity[k] = int(self.flt())
-                    elif k in ("error_bound", "minimum_reuse", "maximum_reuse", "pretrace_start", "pretrace_end"):
+                    elif k in ("error_bound", "minimum_reuse", "maximum_reuse", "pretrace_start", "pretrace_end",
+                               "brightness", "adc_bailout"):
                         self.radiosity[k] = self.flt()
+                    elif k in ("low_error_factor", "gray_threshold", "max_sample", "always_sample", "normal",
+                               "media", "subsurface", "nearest_count_min"):
+                        self.cond()
+                        self.warn(f"line {l2}: radiosity {k} ignored")
                     else:
                         raise SdlError(f"line {l2}: unsupported radiosity keyword {k!r}")
+            elif val in ("noise_generator", "charset", "hf_gray_16", "number_of_waves", "irid_wavelength",
+                         "max_intersections", "mm_per_unit"):
+                if val == "charset":
+                    self.next()
+                else:
+                    self.cond()
+                self.warn(f"line {ln}: global_settings {val} ignored")
+            elif val in ("photons", "subsurface"):
+                self.skip_braces()
+                self.warn(f"line {ln}: global_settings {val} ignored")
             else:
                 raise SdlError(f"line {ln}: unsupported global_settings keyword {val!r}")
 
@@ -482,9 +900,8 @@ class _Parser:
             self.comma()
             r = self.flt()
             hole = 0.0
-            if self.accept(","):
-                hole = self.flt()
-            elif self.peek()[0] == "num":
+            self.accept(",")  # Parse_Disc: Parse_Comma, then a hole radius only if a float starts here
+            if self.starts_float():
                 hole = self.flt()
             o = _Obj("disc", dict(c=c, n=n, r=(r, hole)), ln)
         elif kw == "torus":
@@ -535,49 +952,77 @@ class _Parser:
         self.mods(o)
         return o
 
+    def starts_float(self):
+        """Whether the next token can begin a float expression (CASE_FLOAT_UNGET)."""
+        kind, val, _ = self.peek()
+        if kind == "num" or val in ("-", "+", "(", "!"):
+            return True
+        if kind == "id":
+            if val in self._FUNCS or val in self._FUNCS2 or val in self._CONST or val in ("pi", "pow", "image_width",
+                                                                                        "image_height"):
+                return True
+            v = self.decl.get(val)
+            return v is not None and not isinstance(v, (_Obj, _Material, _Light)) and np.ndim(v) == 0
+        return False
+
     def mesh2(self, ln):
-        verts = norms = faces = None
-        while self.peek()[1] in ("vertex_vectors", "normal_vectors", "face_indices"):
+        verts = norms = faces = nidx = None
+        blocks = ("vertex_vectors", "normal_vectors", "uv_vectors", "texture_list", "face_indices",
+                  "normal_indices", "uv_indices", "inside_vector")
+        while self.peek()[1] in blocks:
             _, k, l2 = self.next()
+            if k == "inside_vector":
+                self.vector()
+                self.warn(f"line {l2}: mesh2 inside_vector ignored (the mesh is not a solid)")
+                continue
+            if k == "texture_list":
+                self.skip_braces()
+                self.warn(f"line {l2}: mesh2 texture_list ignored (the object's texture is used)")
+                continue
             self.expect("{")
             cnt = int(self.flt())
             items = []
             for _ in range(cnt):
                 self.comma()
-                items.append(self.vector())
+                items.append(self.expr())
+                if k == "face_indices":  # per-face texture indices: one or three (Parse_Mesh2)
+                    while self.peek()[1] == "," and self.i + 1 < len(self.toks) and \
+                            self.toks[self.i + 1][0] == "num":
+                        self.next()
+                        self.flt()
             self.accept(",")
             self.expect("}")
-            arr = np.array(items).reshape(-1, 3)
+            if k == "uv_vectors":
+                self.warn(f"line {l2}: mesh2 uv_vectors ignored (no uv mapping)")
+                continue
+            if k == "uv_indices":
+                continue
+            arr = np.array([np.asarray(v, np.float64)[:3] if np.ndim(v) else np.full(3, float(v))
+                            for v in items]).reshape(-1, 3)
             if k == "vertex_vectors":
                 verts = arr
             elif k == "normal_vectors":
                 norms = arr
+            elif k == "normal_indices":
+                nidx = arr.astype(np.int64)
             else:
                 faces = arr.astype(np.int64)
         if verts is None or faces is None:
             raise SdlError(f"line {ln}: mesh2 needs vertex_vectors and face_indices")
+        if norms is not None and nidx is not None and not np.array_equal(nidx, faces):
+            self.warn(f"line {ln}: mesh2 normal_indices differ from face_indices: normals dropped (flat faces)")
+            norms = None
         if norms is not None and norms.shape[0] != verts.shape[0]:
-            raise SdlError(f"line {ln}: normal_vectors must match vertex_vectors (one normal per vertex)")
+            self.warn(f"line {ln}: mesh2 normal count differs from the vertex count: normals dropped (flat faces)")
+            norms = None
         return _Obj("mesh", dict(v=verts, n=norms, f=faces), ln)
 
     def mods(self, o: _Obj):
         while not self.accept("}"):
             kind, val, ln = self.next()
-            if val == "translate":
-                v = self.vector()
-                self.transform(o, scenes.translation_transform(v))
-            elif val == "rotate":
-                self.transform(o, scenes.rotation_transform(self.vector()))
-            elif val == "scale":
-                self.transform(o, _scale(self.vector()))
-            elif val == "matrix":
-                v = self.expr()
-                v = np.asarray(v, np.float64)
-                if v.shape != (12,):
-                    raise SdlError(f"line {ln}: matrix takes 12 values")
-                m = np.eye(4)
-                m[:, :3] = v.reshape(4, 3)
-                self.transform(o, (m, np.linalg.inv(m)))
+            t = self.transform_mod(val, ln)
+            if t is not None:
+                self.transform(o, t)
             elif val == "inverse":
                 o.inverted = not o.inverted
             elif val in ("open", "sturm"):
@@ -585,17 +1030,29 @@ class _Parser:
                     o.data["flags"] |= A.PVX_CONE_OPEN
                 elif val == "sturm" and o.kind == "torus":
                     o.data["sturm"] = 1
+                elif val == "sturm" and o.kind in ("csg", "mesh", "sphere", "box", "plane", "triangle"):
+                    pass  # no polynomial to solve there
                 else:
                     raise SdlError(f"line {ln}: {val} does not apply to {o.kind}")
             elif val in ("no_shadow", "no_image", "no_reflection", "no_radiosity"):
                 o.flags |= {"no_shadow": A.PVX_FLAG_NO_SHADOW, "no_image": A.PVX_FLAG_NO_IMAGE,
                             "no_reflection": A.PVX_FLAG_NO_REFLECTION,
                             "no_radiosity": A.PVX_FLAG_NO_RADIOSITY}[val]
-            elif val in ("pigment", "finish", "interior", "texture"):
+            elif val in ("pigment", "finish", "interior", "texture", "normal"):
                 self.material(o, val)
             elif val in ("color", "colour", "rgb"):
                 self.i -= 1
                 self.set_colour(o)
+            elif val in ("hollow", "double_illuminate", "uv_mapping", "split_union", "cutaway_textures"):
+                if self.peek()[1] in ("on", "off", "true", "false", "yes", "no") or self.starts_float():
+                    self.cond()
+                if val not in ("hollow", "split_union"):
+                    self.warn(f"line {ln}: {val} ignored")
+            elif val in ("photons", "interior_texture", "radiosity"):
+                self.skip_braces()
+                self.warn(f"line {ln}: object {val} ignored")
+            elif val in ("bounded_by", "clipped_by"):
+                raise SdlError(f"line {ln}: {val} is not supported (it changes which hits exist)")
             else:
                 raise SdlError(f"line {ln}: unsupported object modifier {val!r}")
 
@@ -605,31 +1062,136 @@ class _Parser:
         o.mat["filter"] = filt
         o.mat["transmit"] = trans
 
+    # pigment patterns: read, reduced to the mean of their colour map (or of
+    # their listed colours) with a warning
+    _PATTERNS = ("checker", "bozo", "spotted", "agate", "granite", "marble", "wood", "gradient", "radial",
+                 "leopard", "onion", "spiral1", "spiral2", "ripples", "waves", "wrinkles", "dents", "bumps",
+                 "crackle", "hexagon", "brick", "mandel", "julia", "spherical", "cylindrical", "boxed",
+                 "planar", "cells", "average", "image_map", "bump_map", "quilted", "facets", "tiling", "pavement",
+                 "function", "pigment_pattern", "object", "slope", "aoi", "brick", "square", "triangular",
+                 "cubic")
+    _PATTERN_MODS = ("turbulence", "octaves", "omega", "lambda", "frequency", "phase", "scallop_wave",
+                     "sine_wave", "ramp_wave", "triangle_wave", "cubic_wave", "poly_wave", "noise_generator",
+                     "uv_mapping", "once", "interpolate", "map_type", "mortar", "brick_size")
+
+    def pattern_pigment(self, o, first, ln):
+        """A pattern pigment: read to its closing brace; the plain colour is the
+        mean of the colour map entries or listed colours."""
+        cols = []
+        if first in ("checker", "hexagon", "brick", "square", "triangular"):
+            while self.peek()[1] in ("color", "colour", "rgb", "rgbf", "rgbt", "rgbft") or \
+                    (self.peek()[0] == "id" and self.peek()[1] in _COLOURS):
+                cols.append(self.colour())
+                self.accept(",")
+        depth = 0
+        while True:
+            kind, val, l2 = self.peek()
+            if val == "}" and depth == 0:
+                break
+            if val in ("color_map", "colour_map", "pigment_map"):
+                self.next()
+                self.expect("{")
+                while not self.accept("}"):
+                    if self.accept("["):
+                        self.flt()
+                        self.accept(",")
+                        if self.peek()[1] in ("color", "colour", "rgb", "rgbf", "rgbt", "rgbft") or \
+                                self.peek()[0] == "id":
+                            cols.append(self.colour())
+                        while not self.accept("]"):
+                            self.next()
+                    else:
+                        self.next()
+                continue
+            self.next()
+            depth += (val == "{") - (val == "}")
+        if cols:
+            o.mat["pigment"] = np.mean([c[0] for c in cols], axis=0)
+            o.mat["filter"] = float(np.mean([c[1] for c in cols]))
+            o.mat["transmit"] = float(np.mean([c[2] for c in cols]))
+        self.warn(f"line {ln}: pigment pattern {first!r} reduced to a plain colour")
+
     def material(self, o, block):
+        if block == "normal":
+            ln = self.line()
+            self.skip_braces()
+            self.warn(f"line {ln}: normal perturbation ignored")
+            return
         self.expect("{")
         while not self.accept("}"):
             kind, val, ln = self.peek()
-            if block == "texture" and val in ("pigment", "finish"):
+            if block == "texture" and val in ("pigment", "finish", "normal"):
                 self.next()
                 self.material(o, val)
             elif kind == "id" and isinstance(self.decl.get(val), _Material):
                 self.next()
                 o.mat.update(self.decl[val])
+            elif block == "pigment" and val in self._PATTERNS:
+                self.next()
+                self.pattern_pigment(o, val, ln)
             elif block == "pigment" and (val in ("color", "colour", "rgb", "rgbf", "rgbt", "rgbft")
-                                         or val in _COLOURS or val in self.decl):
+                                         or val in _COLOURS or val in self.decl or val == "<"):
                 self.set_colour(o)
+            elif block == "pigment" and val in ("translate", "rotate", "scale", "matrix", "warp", "quick_color",
+                                                "quick_colour", "uv_mapping"):
+                self.next()
+                if val == "warp":
+                    self.skip_braces()
+                elif val in ("quick_color", "quick_colour"):
+                    self.colour()
+                elif val != "uv_mapping":
+                    self.cond()
             elif block == "finish" and val in ("ambient", "diffuse", "brilliance", "phong", "phong_size",
                                                "reflection", "emission"):
                 self.next()
                 if val == "reflection" and self.accept("{"):
-                    r = self.flt()
-                    self.expect("}")
+                    r = self.colour()[0]
+                    r2 = r
+                    if self.accept(","):
+                        r2 = self.colour()[0]
+                    while not self.accept("}"):
+                        _, k2, l3 = self.next()
+                        if k2 in ("fresnel", "metallic", "falloff", "exponent"):
+                            if self.starts_float() or self.peek()[1] in ("on", "off"):
+                                self.cond()
+                            self.warn(f"line {l3}: reflection {k2} ignored")
+                    if not np.allclose(r, r2):
+                        self.warn(f"line {ln}: variable reflection read as its maximum")
+                    o.mat[val] = float(np.max(r2))
+                elif val in ("ambient", "emission", "diffuse", "reflection") and \
+                        self.peek()[1] in ("color", "colour", "rgb", "rgbf", "rgbt", "rgbft"):
+                    c = self.colour()[0]
+                    o.mat[val] = c if val in ("ambient", "emission") else float(np.mean(c))
                 else:
+                    if val == "diffuse" and self.peek()[1] == "albedo":
+                        self.next()
                     r = self.expr()
-                o.mat[val] = r
+                    if val in ("ambient", "emission"):
+                        o.mat[val] = r
+                    else:
+                        o.mat[val] = float(np.mean(r)) if np.ndim(r) else float(r)
+            elif block == "finish" and val in ("specular", "roughness", "metallic", "crand", "irid",
+                                               "conserve_energy", "fresnel", "subsurface", "reflection_exponent",
+                                               "phong_albedo", "specular_albedo"):
+                self.next()
+                if val in ("irid", "subsurface"):
+                    self.skip_braces()
+                elif self.starts_float() or self.peek()[1] in ("on", "off"):
+                    self.cond()
+                self.warn(f"line {ln}: finish {val} ignored")
             elif block == "interior" and val == "ior":
                 self.next()
                 o.mat["ior"] = self.flt()
+            elif block == "interior" and val in ("caustics", "dispersion", "dispersion_samples", "fade_distance",
+                                                 "fade_power", "fade_color", "fade_colour", "media"):
+                self.next()
+                if val == "media":
+                    self.skip_braces()
+                elif val in ("fade_color", "fade_colour"):
+                    self.colour()
+                else:
+                    self.cond()
+                self.warn(f"line {ln}: interior {val} ignored")
             else:
                 raise SdlError(f"line {ln}: unsupported {block} keyword {val!r}")
 
@@ -655,13 +1217,11 @@ class _Parser:
             n2 = n2 / np.linalg.norm(n2)
             o.data["n"], o.data["d"] = n2, -float(n2 @ p2)
             return
-        if o.kind in ("mesh", "triangle"):  # vertices (and normals) transformed
-            o.data["p" if o.kind == "triangle" else "v"] = \
-                o.data["p" if o.kind == "triangle" else "v"] @ tr[0][:3, :3] + tr[0][3, :3]
-            key = "n"
-            if o.data.get(key) is not None:
-                n2 = o.data[key] @ tr[1][:3, :3].T
-                o.data[key] = n2 / np.linalg.norm(n2, axis=1, keepdims=True)
+        if o.kind == "triangle":  # Triangle::Transform: vertices (and normals) transformed
+            o.data["p"] = o.data["p"] @ tr[0][:3, :3] + tr[0][3, :3]
+            if o.data.get("n") is not None:
+                n2 = o.data["n"] @ tr[1][:3, :3].T
+                o.data["n"] = n2 / np.linalg.norm(n2, axis=1, keepdims=True)
             return
         o.tr = tr if o.tr is None else _compose(o.tr, tr)
 
@@ -672,7 +1232,7 @@ class _Parser:
         arrays = dict(sph_c=[], sph_r=[], pl_n=[], pl_d=[], box_c1=[], box_c2=[], box_tr=[], tor_R=[], tor_r=[],
                       tor_tr=[], tor_s=[], xf=[], cone_a=[], cone_b=[], cone_r=[], cone_f=[], cone_t=[],
                       disc_c=[], disc_n=[], disc_r=[], disc_t=[], q=[], q_t=[], tri_p=[], tri_n=[], meshes=[],
-                      poly_first=[0], poly_pts=[], poly_t=[],
+                      poly_first=[0], poly_pts=[], poly_t=[], mesh_t=[],
                       csg=[], kids=[])
 
         def xf_index(o):
@@ -732,9 +1292,10 @@ class _Parser:
                 arrays["poly_first"].append(len(arrays["poly_pts"]))
                 arrays["poly_t"].append(xf_index(o))
                 return [A.PVX_POLYGON, len(arrays["poly_t"]) - 1, f, 0]
-            if k == "mesh":
+            if k == "mesh":  # FP32 vertices stay in mesh space; the placement is kept
                 arrays["meshes"].append((dd["v"].astype(np.float32), dd["f"].astype(np.int32),
                                          None if dd["n"] is None else dd["n"].astype(np.float32)))
+                arrays["mesh_t"].append(xf_index(o))
                 return [A.PVX_MESH, len(arrays["meshes"]) - 1, f, 0]
             raise SdlError(f"line {o.line}: unknown object kind {k}")
 
@@ -767,13 +1328,39 @@ class _Parser:
             arrays["csg"][idx] = [code, first, len(kids), 0]
             return idx
 
-        objs, mats = [], []
+        def needs_split(o):
+            return any(c.kind in ("triangle", "mesh", "polygon") or (c.kind == "csg" and needs_split(c))
+                       for c in o.children)
+
+        def split(o, mat, flags, out):
+            """A top-level union whose operands include triangles, meshes or
+            polygons (which are not CSG operands here) becomes its operands as
+            top-level objects: a union's closest hit is the closest of its
+            operands' (CSGUnion::All_Intersections keeps every operand hit);
+            the union's texture and flags pass to them."""
+            for c in o.children:
+                m = {**mat, **c.mat}
+                if c.kind == "csg" and c.data["op"] == "union" and not c.inverted and needs_split(c):
+                    split(c, m, flags | c.flags, out)
+                else:
+                    c = c.copy()
+                    c.mat = m
+                    c.flags |= flags
+                    out.append(c)
+
+        top = []
         for o in self.objects:
+            if o.kind == "csg" and o.data["op"] == "union" and not o.inverted and needs_split(o):
+                split(o, o.mat, o.flags, top)
+            else:
+                top.append(o)
+        objs, mats = [], []
+        for o in top:
             if o.kind == "csg":
                 objs.append([A.PVX_CSG, csg(o, 0), o.flags, 0])
             else:
                 objs.append(leaf(o, o.flags, True))
-            mats.append(o.mat)
+            mats.append({**self.defaults, **o.mat})
         n = len(objs)
 
         def matarr(key, default, shape=()):
@@ -798,6 +1385,7 @@ class _Parser:
         settings["ambient_light"] = np.asarray(self.ambient_light, np.float64)
         if self.radiosity is not None:
             settings["radiosity"] = dict(self.radiosity)
+        settings["warnings"] = list(self.warnings)
         return scenes.Scene(
             name="sdl", objects=np.array(objs, np.uint32).reshape(-1, 4),
             sphere_center=arr("sph_c", (-1, 3)), sphere_radius=arr("sph_r", (-1,)),
@@ -816,6 +1404,7 @@ class _Parser:
             quadric_coef=arr("q", (-1, 10)), quadric_transform=arr("q_t", (-1,), np.int32),
             polygon_first=np.array(arrays["poly_first"], np.uint32) if arrays["poly_t"] else None,
             polygon_points=arr("poly_pts", (-1, 3)), polygon_transform=arr("poly_t", (-1,), np.int32),
+            mesh_transform=arr("mesh_t", (-1,), np.int32),
             camera=self.camera, lights=self.lights, materials=materials, settings=settings)
 
 

@@ -41,6 +41,8 @@ mesh2 {
   vertex_vectors { 4, <0, 2.5, 0>, <1, 2.5, 0>, <1, 3.5, 0>, <0, 3.5, 0> }
   face_indices { 2, <0, 1, 2>, <0, 2, 3> }
   pigment { rgb <0.9, 0.5, 0.1> }
+  rotate <0, 30, 0>
+  scale <1, 1.5, 1>
   translate <-0.5, 0, 0>
 }
 """
@@ -71,9 +73,16 @@ def test_sdl_reads_the_scene():
     assert np.allclose(s.materials["pigment"][1], [1, 0, 0])
     assert s.materials["transmit"][10] == 0.9 and s.materials["ior"][10] == 1.5
     assert s.materials["reflection"][10] == 0.1
-    # mesh2 moved by its translate
+    # mesh2: FP32 vertices stay in mesh space, the modifiers compose into its
+    # placement (Mesh::Transform)
     v = s.meshes[0][0]
-    assert np.allclose(v[0], [-0.5, 2.5, 0]) and s.meshes[0][1].shape == (2, 3)
+    assert np.allclose(v[0], [0, 2.5, 0]) and s.meshes[0][1].shape == (2, 3)
+    m = s.transforms[s.mesh_transform[0]]
+    fwd, inv = m[:16].reshape(4, 4), m[16:].reshape(4, 4)
+    assert np.allclose(fwd @ inv, np.eye(4))
+    p = np.r_[v[1].astype(np.float64), 1.0] @ fwd
+    c, sn = np.cos(np.radians(30)), np.sin(np.radians(30))
+    assert np.allclose(p[:3], [c - 0.5, 2.5 * 1.5, -sn])
     # lights, camera (angle 60: direction length 0.5 * |right| / tan 30)
     assert s.lights[0]["kind"] == "point" and s.lights[1]["kind"] == "area" and s.lights[1]["size"] == (3, 3)
     cam = s.camera
@@ -82,16 +91,65 @@ def test_sdl_reads_the_scene():
 
 
 @pytest.mark.parametrize("bad, what", [
-    ('sphere { <0,0,0>, 1 pigment { checker } }', "checker"),
-    ('#include "shapes.inc"', "shapes.inc"),
     ('blob { }', "blob"),
     ('sphere { <0,0,0>, 1 scale <1, 2, 1> }', "uniform scale"),
-    ('light_source { <0,0,0> color White area_light x, y, 2, 2 jitter }', "jitter"),
-    ('camera { location <0,0,0> fisheye }', "fisheye"),
+    ('sphere { <0,0,0>, 1 clipped_by { plane { y, 0 } } }', "clipped_by"),
+    ('#error "stop"', "stop"),
+    ('#while (1 < 2) sphere { <0,0,0>, 1 }', "unterminated"),
 ])
 def test_sdl_rejects_what_it_does_not_read(bad, what):
     with pytest.raises(sdl.SdlError, match=what):
         sdl.parse("\n\n" + bad, 32, 24)
+
+
+@pytest.mark.parametrize("text, what", [
+    ('sphere { <0,0,0>, 1 pigment { checker color rgb 1 color rgb <1, 0, 0> } }', "checker"),
+    ('#include "shapes.inc"', "shapes.inc"),
+    ('light_source { <0,0,0> color White area_light x, y, 2, 2 jitter }', "jitter"),
+    ('camera { location <0,0,-5> fisheye }', "fisheye"),
+    ('sphere { 0, 1 texture { normal { bumps 0.5 } finish { specular 0.3 } } }', "normal"),
+])
+def test_sdl_reduces_what_it_cannot_render(text, what):
+    """Textures, special cameras and light options are read and reduced to
+    what pvx renders, each with a warning naming the line."""
+    s = sdl.parse("\n\n" + text, 32, 24)
+    assert any(what in w and w.startswith("line 3") for w in s.settings["warnings"]), s.settings["warnings"]
+
+
+def test_sdl_directives_and_expressions():
+    """#declare / #local, #while, #if / #elseif / #else, #macro, seed() and
+    rand() (POV_rand: next = next * 1812433253 + 12345, / 0xFFFFFFFF),
+    comparisons, logic and member access."""
+    text = """
+    #declare N = 0;
+    #declare S = seed(12);
+    #declare R = array_free;
+    """.replace("#declare R = array_free;", "")
+    text += """
+    #macro Ball(P, Rad)
+      sphere { P, Rad }
+    #end
+    #declare V = <1, 2, 3>;
+    #while (N < 5)
+      #if (mod(N, 2) = 0 & V.y = 2)
+        Ball(<N, 0, 0>, 0.25)
+      #elseif (N = 3)
+        box { <N, 0, 0>, <N + 0.5, 1, 1> }
+      #else
+        #local Q = N * 2;
+        sphere { <N, Q, 0>, rand(S) }
+      #end
+      #declare N = N + 1;
+    #end
+    """
+    s = sdl.parse(text, 32, 24)
+    kinds = [int(t) for t in s.objects[:, 0]]
+    assert kinds == [A.PVX_SPHERE, A.PVX_SPHERE, A.PVX_SPHERE, A.PVX_BOX, A.PVX_SPHERE]
+    assert np.allclose(s.sphere_center[:, 0], [0, 1, 2, 4]) and np.allclose(s.sphere_center[1], [1, 2, 0])
+    # the first draw of the stream seeded with 12
+    nxt = (12 * 1812433253 + 12345) & 0xFFFFFFFF
+    assert s.sphere_radius[1] == pytest.approx(nxt / 0xFFFFFFFF)
+    assert s.sphere_radius[0] == 0.25 and s.sphere_radius[2] == 0.25
 
 
 def test_sdl_scene_through_the_oracle():
@@ -117,3 +175,69 @@ def test_gpu_renders_parsed_scene_as_oracle(gpu):
     ref, (n_closest, n_shadow) = OracleScene(s).render(s)
     ok = np.mean(np.abs(img - ref).max(axis=2) <= 1.0 / 255.0)
     assert ok >= 0.999, ok
+
+
+# ---------------------------------------------------------------------------
+# three of the reference's sample scenes (distribution/scenes), parsed into
+# the fixtures tests/golden/sdl_*.npz by tests/golden/make_sdl_scenes.py
+# ---------------------------------------------------------------------------
+import os  # noqa: E402
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+REF_SCENES = "/root/reference/distribution/scenes"
+SAMPLES = {"tango": "bsp/Tango.pov", "mesh2": "objects/mesh2.pov", "cornell": "radiosity/cornell.pov"}
+
+
+def _fixture(name):
+    return scenes.Scene.load(os.path.join(GOLDEN, f"sdl_{name}.npz"))
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_SCENES), reason="the reference's scene files are not on this machine")
+@pytest.mark.parametrize("name", sorted(SAMPLES))
+def test_sample_scene_reads_as_its_fixture(name):
+    """The reader loads the sample scene (directives, macros, rand, textures
+    reduced with warnings) and gives exactly the stored fixture."""
+    fx = _fixture(name)
+    s = sdl.parse_file(os.path.join(REF_SCENES, SAMPLES[name]), fx.camera["width"], fx.camera["height"])
+    assert s.settings["warnings"], "the sample scenes use features the reader reduces"
+    for k in scenes.Scene._ARRAYS:
+        a, b = getattr(s, k), getattr(fx, k)
+        assert (a is None) == (b is None), k
+        if a is not None:
+            assert np.array_equal(np.asarray(a), np.asarray(b)), k
+    for (v, f, n), (v2, f2, n2) in zip(s.meshes, fx.meshes):
+        assert np.array_equal(v, v2) and np.array_equal(f, f2)
+
+
+@pytest.mark.parametrize("name", sorted(SAMPLES))
+def test_sample_scene_fixture_through_the_oracle(name):
+    s = _fixture(name)
+    o = OracleScene(s)
+    rays = scenes.camera_rays_np(s.camera)
+    h1 = o.find_intersection(rays, method=1)
+    h0 = o.find_intersection(rays[::5], method=0)
+    assert np.array_equal(h1["obj"][::5], h0["obj"]) and np.array_equal(h1["t"][::5], h0["t"])
+    assert np.mean(h1["obj"] >= 0) > 0.4
+    if name == "tango":  # 300 strands of boxes and corner pieces from #while loops and rand()
+        assert s.n_objects > 16000
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", sorted(SAMPLES))
+def test_gpu_sample_scene_matches_oracle(gpu, name):
+    s = _fixture(name)
+    gs, o = gpu.GpuScene(s), OracleScene(s)
+    rays = scenes.camera_rays_np(s.camera)
+    g, c = gs.find_intersection(rays), o.find_intersection(rays, method=1)
+    bad = int(np.sum((g["obj"] != c["obj"]) | (g["prim"] != c["prim"])))
+    print(f"[sdl {name}] {s.n_objects} objects, {rays.shape[0]} rays, id mismatches {bad}", flush=True)
+    assert bad == 0
+    hit = c["obj"] >= 0
+    assert np.all(np.abs(g["t"][hit] - c["t"][hit]) <= 1e-5 * np.abs(c["t"][hit]))
+    gs.set_shading()
+    img, _ = gs.render()
+    ref, _ = o.render(s)
+    q = lambda a: np.round(np.clip(a, 0.0, 1.0) * 255.0).astype(np.int32)  # noqa: E731
+    ok = float(np.mean(np.abs(q(img) - q(ref)).max(-1) <= 1))
+    print(f"[sdl {name}] shaded frame: {ok * 100:.3f}% of pixels within 1/255", flush=True)
+    assert ok >= 0.999
