@@ -43,7 +43,7 @@ typedef struct pvx_scene pvx_scene;
 /* ---- object model ------------------------------------------------------ */
 /* Shape kinds: the subset of core/shape on the hot path (sphere.cpp,
  * box.cpp, torus.cpp, plane.cpp, mesh.cpp, csg.cpp, triangle.cpp, cone.cpp,
- * disc.cpp, quadric.cpp, polygon.cpp). */
+ * disc.cpp, quadric.cpp, polygon.cpp, lathe.cpp, sor.cpp). */
 enum {
     PVX_SPHERE = 1,
     PVX_BOX = 2,
@@ -56,7 +56,27 @@ enum {
     PVX_CONE = 9,            /* cone or cylinder (cone.cpp, Cone)                          */
     PVX_DISC = 10,           /* disc, optionally with a hole (disc.cpp, Disc)              */
     PVX_QUADRIC = 11,        /* quadric surface (quadric.cpp, Quadric)                     */
-    PVX_POLYGON = 12         /* planar polygon (polygon.cpp, Polygon; not a CSG operand)   */
+    PVX_POLYGON = 12,        /* planar polygon (polygon.cpp, Polygon; not a CSG operand)   */
+    PVX_LATHE = 13,          /* lathe (lathe.cpp, Lathe)                                   */
+    PVX_SOR = 14             /* surface of revolution (sor.cpp, Sor)                       */
+};
+
+/* pvx_scene_desc.revolution_kind: the spline a lathe is built from
+ * (Parse_Lathe's linear_spline / quadratic_spline / cubic_spline /
+ * bezier_spline), or PVX_SOR_CUBIC for a sor (Compute_Sor's cubic in the
+ * height through each pair of points) */
+enum {
+    PVX_LATHE_LINEAR = 0,
+    PVX_LATHE_QUADRATIC = 1,
+    PVX_LATHE_CUBIC = 2,
+    PVX_LATHE_BEZIER = 3,
+    PVX_SOR_CUBIC = 4
+};
+
+/* pvx_scene_desc.revolution_flags bits */
+enum {
+    PVX_REV_STURM = 1u << 0, /* `sturm`: Sturm sequences for the polynomial roots (STURM_FLAG) */
+    PVX_REV_OPEN = 1u << 1   /* sor `open`: no base and cap discs (CLOSED_FLAG cleared)        */
 };
 
 /* pvx_scene_desc.cone_flags bits */
@@ -224,6 +244,22 @@ typedef struct pvx_scene_desc {
      * normals leave through MTransNormal (Mesh::Normal, mesh.cpp:342-380) and
      * the object box is the mesh box through Recompute_BBox */
     const int32_t *mesh_transform;  /* n_meshes: index into transforms or -1; or NULL */
+
+    /* lathes and surfaces of revolution as Parse_Lathe (parser/parser.cpp:
+     * 2812-2995) and Parse_Sor (parser.cpp:5628-5714) read them: shape i has
+     * the 2-D points revolution_points[2 * revolution_first[i] ..
+     * 2 * revolution_first[i + 1]) (x the radius, y the height) and the spline
+     * revolution_kind[i]; an object of type PVX_LATHE refers to a shape with a
+     * PVX_LATHE_* kind, one of type PVX_SOR to a PVX_SOR_CUBIC one.  The
+     * segments and bounding cylinders (Compute_Lathe, Compute_Sor, Create_BCyl)
+     * are computed from the points; the optional transform places the shape
+     * (Lathe::Transform / Sor::Transform).  At most 64 segments per shape. */
+    uint32_t n_revolutions;
+    const uint32_t *revolution_first;   /* n + 1 offsets, in points                */
+    const double *revolution_points;    /* 2 per point                             */
+    const uint8_t *revolution_kind;     /* n: PVX_LATHE_* or PVX_SOR_CUBIC         */
+    const uint8_t *revolution_flags;    /* n: PVX_REV_* bits, or NULL              */
+    const int32_t *revolution_transform;/* n: index into transforms or -1; or NULL */
 } pvx_scene_desc;
 
 /* ---- rays and hits ----------------------------------------------------- */

@@ -75,8 +75,9 @@ FBox make_bbox_from_min_max(V3 mins, V3 maxs); // boundingbox.h:114-118 (Vector3
 struct Isect {
     double depth;
     int32_t prim;   // triangle index / CSG leaf slot
-    int32_t side;   // box side (Intersection::i1)
+    int32_t side;   // box side (Intersection::i1); lathe segment; sor part | segment << 2
     int32_t leaf;   // CSG: leaf shape (type<<24 | index), else -1
+    double d1 = 0;  // Intersection::d1 (lathe: spline parameter of the hit)
 };
 
 // ---------------------------------------------------------------------------
@@ -235,6 +236,31 @@ FBox cone_bbox(const Cone &c);
 // polynomialsolver.cpp:1585-1732
 int solve_polynomial(int n, const double *c0, double *r, bool sturm, double epsilon);
 
+// Lathe (lathe.cpp) and surface of revolution (sor.cpp) with their bounding
+// cylinders (core/bounding/boundingcylinder.cpp): spline segments, the
+// bounding cylinder lists, the overall cylinder and the placement.
+struct Revolution {
+    bool sor = false;
+    int spline = 0;              // PVX_LATHE_* (PVX_SOR_CUBIC for a sor)
+    bool closed = false, sturm = false;
+    std::vector<double> seg;     // 8 per segment: A.x A.y B.x B.y C.x C.y D.x D.y (sor: A B C D in x)
+    std::vector<double> radius;  // BCYL radius (squared), height lists
+    std::vector<double> height;
+    std::vector<int32_t> entry;  // 4 per segment: r1 r2 h1 h2
+    double radius1 = 0, radius2 = 0, height1 = 0, height2 = 0;
+    double base_r2 = 0, cap_r2 = 0;
+    Transform tr;
+    FBox bbox;
+    int segments() const { return (int)(seg.size() / 8); }
+};
+Revolution lathe_prepare(const double *pts, int n, int spline, bool sturm, const Transform *user);
+Revolution sor_prepare(const double *pts, int n, bool open, bool sturm, const Transform *user);
+// All_Intersections; `child` is IS_CHILD_OBJECT (no early exit once the
+// next bounding interval starts beyond the best hit)
+bool revolution_intersect(const Revolution &s, const Ray &r, bool child, std::vector<Isect> &st, int32_t slot);
+bool revolution_inside(const Revolution &s, V3 p, bool inverted);
+V3 revolution_normal(const Revolution &s, const Isect &h, V3 ip);
+
 // ---------------------------------------------------------------------------
 // scene (SceneData::objects + shape storage + bounding slabs)
 // ---------------------------------------------------------------------------
@@ -273,6 +299,7 @@ struct Scene {
     std::vector<Disc> discs;
     std::vector<Quadric> quadrics;
     std::vector<PolygonO> polygons;
+    std::vector<Revolution> revs;    // lathes and sors
     std::vector<Transform> trs;
     std::vector<Mesh> meshes;
     std::vector<int32_t> mesh_tr;     // each mesh's placement (trs index) or -1
@@ -282,8 +309,9 @@ struct Scene {
 
     const Transform *tr(int32_t i) const { return i < 0 ? nullptr : &trs[i]; }
     FBox shape_bbox(uint32_t type, uint32_t index, uint32_t flags, bool *infinite) const;
+    // `child`: the shape is a CSG operand (IS_CHILD_OBJECT)
     bool all_intersections(uint32_t type, uint32_t index, uint32_t flags, const Ray &r,
-                           std::vector<Isect> &st, int32_t slot) const;
+                           std::vector<Isect> &st, int32_t slot, bool child = false) const;
     bool inside(uint32_t type, uint32_t index, uint32_t flags, V3 p) const;
     bool find_object(uint32_t obj, const Ray &r, bool shadow, Isect &out) const;
 };

@@ -45,6 +45,7 @@ def lib():
                                            C.c_void_p]
         L.orc_solve_polynomial.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_double, C.c_void_p]
         L.orc_triangle_normal.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t, C.c_void_p]
+        L.orc_hit_normals.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]
         L.orc_trace_rays.argtypes = [C.c_void_p, C.POINTER(A.PvxShadingDesc), C.c_void_p, C.c_size_t,
                                      C.c_void_p, C.c_int]
         L.orc_halton_pool.argtypes = [C.c_void_p]
@@ -192,6 +193,13 @@ class OracleScene:
         pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
         out = np.zeros_like(pts)
         lib().orc_triangle_normal(self.h, tri, _p(pts), pts.shape[0], _p(out))
+        return out
+
+    def hit_normals(self, rays):
+        """Raw Object::Normal at each ray's closest hit (zeros on a miss)."""
+        rays = np.ascontiguousarray(rays, np.float64).reshape(-1, 6)
+        out = np.zeros((rays.shape[0], 3))
+        lib().orc_hit_normals(self.h, _p(rays), rays.shape[0], _p(out))
         return out
 
 

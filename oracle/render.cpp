@@ -98,6 +98,9 @@ V3 shape_normal(const Scene &s, uint32_t type, uint32_t index, const Isect &h, V
         return s.discs[index].normal;
     case PVX_CONE: // Cone::Normal (cone.cpp:440-480)
         return cone_normal(s.cones[index], h.side, ip);
+    case PVX_LATHE:
+    case PVX_SOR:
+        return revolution_normal(s.revs[index], h, ip);
     case PVX_CSG: { // the leaf that produced the hit
         const pvx_object &c = s.csg_children[h.leaf];
         Isect leaf = h;
@@ -391,6 +394,26 @@ extern "C" int orc_trace_rays(void *sp, const pvx_shading_desc *sd, const double
         rgb[3 * i] = c.r;
         rgb[3 * i + 1] = c.g;
         rgb[3 * i + 2] = c.b;
+    }
+    return 0;
+}
+
+// Closest hit per ray (method 1) and the raw Object::Normal there (zeros on
+// a miss): 3 doubles per ray.
+extern "C" int orc_hit_normals(void *sp, const double *rays, size_t n, double *out) {
+    const Scene &s = *static_cast<Scene *>(sp);
+    for (size_t i = 0; i < n; i++) {
+        Ray r;
+        r.o = V3(rays[6 * i], rays[6 * i + 1], rays[6 * i + 2]);
+        r.d = V3(rays[6 * i + 3], rays[6 * i + 4], rays[6 * i + 5]);
+        Isect h;
+        int32_t obj;
+        V3 nrm(0, 0, 0);
+        if (scene_find_intersection(s, r, 0.0, PVX_RAY_IMAGE, h, obj, 1)) {
+            const Object &o = s.objects[obj];
+            nrm = shape_normal(s, o.type, o.index, h, r.eval(h.depth));
+        }
+        out[3 * i] = nrm.x, out[3 * i + 1] = nrm.y, out[3 * i + 2] = nrm.z;
     }
     return 0;
 }

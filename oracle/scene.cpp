@@ -65,6 +65,10 @@ FBox Scene::shape_bbox(uint32_t type, uint32_t index, uint32_t flags, bool *infi
     case PVX_QUADRIC: // Quadric::Compute_BBox at parse time (see quadric.cpp)
         b = quadrics[index].bbox;
         break;
+    case PVX_LATHE: // Lathe::Compute_BBox (lathe.cpp:455-461)
+    case PVX_SOR:   // Sor::Compute_BBox (sor.cpp:705-711)
+        b = revs[index].bbox;
+        break;
     case PVX_TRIANGLE:
     case PVX_SMOOTH_TRIANGLE:
         b = tri_bbox(&tri_p[9 * (size_t)index]);
@@ -130,7 +134,7 @@ FBox Scene::shape_bbox(uint32_t type, uint32_t index, uint32_t flags, bool *infi
 // primitive id for non-mesh shapes (the leaf's csg_children position inside
 // a CSG, 0 at top level).
 bool Scene::all_intersections(uint32_t type, uint32_t index, uint32_t flags, const Ray &r,
-                              std::vector<Isect> &st, int32_t slot) const {
+                              std::vector<Isect> &st, int32_t slot, bool child) const {
     switch (type) {
     case PVX_SPHERE: { // Sphere::All_Intersections (sphere.cpp:92-178), spherical mode
         double d1, d2;
@@ -201,6 +205,9 @@ bool Scene::all_intersections(uint32_t type, uint32_t index, uint32_t flags, con
         for (int k = 0; k < n; k++) st.push_back(Isect{dep[k], slot, 0, slot});
         return n > 0;
     }
+    case PVX_LATHE: // Lathe::All_Intersections (lathe.cpp:176-187), no clipped_by
+    case PVX_SOR:   // Sor::All_Intersections (sor.cpp:193-205), no clipped_by
+        return revolution_intersect(revs[index], r, child, st, slot);
     case PVX_DISC: { // Disc::All_Intersections (disc.cpp), no clipped_by
         double dep;
         if (!disc_intersect(discs[index], r, &dep)) return false;
@@ -220,7 +227,7 @@ bool Scene::all_intersections(uint32_t type, uint32_t index, uint32_t flags, con
         if (n.op == PVX_CSG_UNION) { // CSGUnion::All_Intersections (csg.cpp:128-200)
             for (uint32_t k = 0; k < n.n_children; k++) {
                 const pvx_object &c = csg_children[n.first_child + k];
-                if (all_intersections(c.type, c.index, c.flags, r, st, (int32_t)(n.first_child + k)))
+                if (all_intersections(c.type, c.index, c.flags, r, st, (int32_t)(n.first_child + k), true))
                     found = true;
             }
             return found;
@@ -233,7 +240,7 @@ bool Scene::all_intersections(uint32_t type, uint32_t index, uint32_t flags, con
         for (uint32_t k = 0; k < n.n_children; k++) {
             const pvx_object &c = csg_children[n.first_child + k];
             local.clear();
-            if (!all_intersections(c.type, c.index, c.flags, r, local, (int32_t)(n.first_child + k)))
+            if (!all_intersections(c.type, c.index, c.flags, r, local, (int32_t)(n.first_child + k), true))
                 continue;
             while (!local.empty()) {
                 Isect top = local.back();
@@ -286,6 +293,9 @@ bool Scene::inside(uint32_t type, uint32_t index, uint32_t flags, V3 p) const {
         return disc_inside(discs[index], p, inv);
     case PVX_QUADRIC:
         return quadric_inside(quadrics[index], p, inv);
+    case PVX_LATHE:
+    case PVX_SOR:
+        return revolution_inside(revs[index], p, inv);
     case PVX_CSG: {
         // CSGUnion/CSGMerge::Inside (csg.cpp:405-424), CSGIntersection::Inside
         // (csg.cpp:440-454); the CSG's own INVERTED flag is carried by its
@@ -316,7 +326,8 @@ bool Scene::find_object(uint32_t obj, const Ray &r, bool shadow, Isect &out) con
     const Object &o = objects[obj];
     double closest = HUGE_VAL;
     if (o.type == PVX_MESH || o.type == PVX_TORUS || o.type == PVX_CSG || o.type == PVX_CONE ||
-        o.type == PVX_DISC || o.type == PVX_POLYGON || o.type == PVX_QUADRIC) {
+        o.type == PVX_DISC || o.type == PVX_POLYGON || o.type == PVX_QUADRIC || o.type == PVX_LATHE ||
+        o.type == PVX_SOR) {
         if (!intersect_bbox_dir(o.bbox, r, (float)kMinIsectDepth, (float)closest)) return false;
     }
     std::vector<Isect> st;
@@ -412,6 +423,20 @@ static Scene *scene_create(const pvx_scene_desc &d) {
         s->polygons.push_back(polygon_prepare(d.polygon_points + 3 * (size_t)b, (int)(e - b),
                                               t >= 0 ? &s->trs[t] : nullptr));
     }
+    for (uint32_t i = 0; i < d.n_revolutions; i++) {
+        const int32_t t = d.revolution_transform ? d.revolution_transform[i] : -1;
+        check_tr(t);
+        const uint32_t b = d.revolution_first[i], e = d.revolution_first[i + 1];
+        if (e < b) throw std::runtime_error("revolution point offsets decrease");
+        const uint8_t kind = d.revolution_kind[i], fl = d.revolution_flags ? d.revolution_flags[i] : 0;
+        const Transform *user = t >= 0 ? &s->trs[t] : nullptr;
+        const double *pts = d.revolution_points + 2 * (size_t)b;
+        if (kind == PVX_SOR_CUBIC)
+            s->revs.push_back(sor_prepare(pts, (int)(e - b), (fl & PVX_REV_OPEN) != 0, (fl & PVX_REV_STURM) != 0, user));
+        else
+            s->revs.push_back(lathe_prepare(pts, (int)(e - b), kind, (fl & PVX_REV_STURM) != 0, user));
+        if (s->revs.back().segments() > 64) throw std::runtime_error("more than 64 segments in a lathe or sor");
+    }
     if (d.n_triangles) s->tri_p.assign(d.triangle_vertices, d.triangle_vertices + 9 * (size_t)d.n_triangles);
     s->tris.resize(d.n_triangles);
     for (uint32_t i = 0; i < d.n_objects; i++) {
@@ -424,6 +449,13 @@ static Scene *scene_create(const pvx_scene_desc &d) {
     }
     s->csg.assign(d.csg, d.csg + d.n_csg);
     s->csg_children.assign(d.csg_children, d.csg_children + d.n_csg_children);
+    auto check_rev = [&](const pvx_object &o) {
+        if (o.type != PVX_LATHE && o.type != PVX_SOR) return;
+        if (o.index >= s->revs.size() || s->revs[o.index].sor != (o.type == PVX_SOR))
+            throw std::runtime_error("a lathe or sor object refers to a missing shape or one of the other kind");
+    };
+    for (uint32_t i = 0; i < d.n_objects; i++) check_rev(d.objects[i]);
+    for (uint32_t i = 0; i < d.n_csg_children; i++) check_rev(d.csg_children[i]);
     std::vector<FBox> fb, ib;
     std::vector<int32_t> fi, ii;
     for (uint32_t i = 0; i < d.n_objects; i++) {

@@ -14,7 +14,10 @@ PVX_CONE = 9
 PVX_DISC = 10
 PVX_QUADRIC = 11
 PVX_POLYGON = 12
+PVX_LATHE, PVX_SOR = 13, 14
 PVX_CONE_OPEN, PVX_CONE_CYLINDER = 1, 2
+PVX_LATHE_LINEAR, PVX_LATHE_QUADRATIC, PVX_LATHE_CUBIC, PVX_LATHE_BEZIER, PVX_SOR_CUBIC = 0, 1, 2, 3, 4
+PVX_REV_STURM, PVX_REV_OPEN = 1, 2
 
 PVX_FLAG_NO_SHADOW = 1 << 0
 PVX_FLAG_NO_IMAGE = 1 << 1
@@ -74,6 +77,8 @@ class PvxSceneDesc(C.Structure):
         ("n_polygons", C.c_uint32), ("polygon_first", C.POINTER(C.c_uint32)), ("polygon_points", c_dp),
         ("polygon_transform", c_i32p),
         ("mesh_transform", c_i32p),
+        ("n_revolutions", C.c_uint32), ("revolution_first", C.POINTER(C.c_uint32)), ("revolution_points", c_dp),
+        ("revolution_kind", c_u8p), ("revolution_flags", c_u8p), ("revolution_transform", c_i32p),
     ]
 
 

@@ -23,7 +23,7 @@ __device__ inline D3 normalize3(D3 a) {
 
 // Object::Normal of the surface that was hit (raw, not yet flipped).
 __device__ inline D3 leaf_normal(const NormalParams &p, uint32_t type, uint32_t index, double t, D3 o, D3 d,
-                          D3 ip) {
+                          D3 ip, bool child = false) {
     const SceneView &sv = p.sv;
     switch (type) {
     case PVX_SPHERE: { // Sphere::Normal (sphere.cpp), spherical mode
@@ -88,6 +88,46 @@ __device__ inline D3 leaf_normal(const NormalParams &p, uint32_t type, uint32_t 
         if (dist > kEpsilon) M = D3{tr.R * P.x / dist, 0.0, tr.R * P.z / dist};
         return normalize3(mtrans_normal(x.inv, sub3(P, M)));
     }
+    case PVX_LATHE:
+    case PVX_SOR: { // Lathe::Normal (lathe.cpp:466-505), Sor::Normal (sor.cpp:540-585)
+        // the hit's segment and spline parameter: the hit list is computed
+        // again and the last entry at the hit's depth taken (Find_Intersection
+        // keeps the last of equal depths)
+        int side = 0;
+        double d1 = 0.0;
+        rev_visit(sv, index, o, d, child, [&](double dep, int sd, double w) {
+            if (dep == t) {
+                side = sd;
+                d1 = w;
+            }
+        });
+        const RevRec &s = sv.revs[index];
+        const Xform &x = sv.xforms[s.xf];
+        const D3 P = xf_point(x.inv, ip);
+        D3 n;
+        if ((s.flags & 7u) == PVX_SOR_CUBIC) {
+            const int part = side & 3;
+            if (part == kSorCurve) {
+                const RevSeg &g = sv.rev_segs[s.seg0 + (side >> 2)];
+                const double k = 0.5 * (P.y * (3.0 * g.a[0] * P.y + 2.0 * g.b[0]) + g.c[0]);
+                n = D3{P.x, -k, P.z};
+            } else {
+                n = D3{0.0, part == kSorBase ? -1.0 : 1.0, 0.0};
+            }
+        } else {
+            const RevSeg &g = sv.rev_segs[s.seg0 + side];
+            double r = P.x * P.x + P.z * P.z;
+            if (r > kEpsilon) {
+                r = sqrt(r);
+                const double dx = d1 * (3.0 * g.a[0] * d1 + 2.0 * g.b[0]) + g.c[0];
+                const double dy = d1 * (3.0 * g.a[1] * d1 + 2.0 * g.b[1]) + g.c[1];
+                n = D3{dy * P.x, -dx * r, dy * P.z};
+            } else {
+                n = D3{0.0, 1.0, 0.0};
+            }
+        }
+        return normalize3(mtrans_normal(x.inv, n));
+    }
     case PVX_TRIANGLE: { // Triangle::Normal (triangle.cpp:641-644)
         const TriangleRec &t = sv.triangles[index];
         return D3{t.n[0], t.n[1], t.n[2]};
@@ -147,7 +187,7 @@ __device__ inline D3 hit_normal(const NormalParams &p, const pvx_hit &h, D3 o, D
     }
     if (type == PVX_CSG) {
         const CsgChild &c = sv.csg_children[(uint32_t)h.prim];
-        return leaf_normal(p, c.type, c.index, h.t, o, d, ip);
+        return leaf_normal(p, c.type, c.index, h.t, o, d, ip, true);
     }
     return leaf_normal(p, type, index, h.t, o, d, ip);
 }

@@ -88,6 +88,7 @@ hipError_t pvx::trace_launch(const TraceKernel &k, const TraceArgs &a, uint32_t 
             return (k.flags ? (k.count ? trace_part5 : trace_part4) : (k.count ? trace_part3 : trace_part2))(k, a, nb, st,
                                                                                                           occ);
         case kWalkKd: return trace_part6(k, a, nb, st, occ);
+        case kWalkRevolution: return trace_part13(k, a, nb, st, occ);
         default: return hipErrorInvalidValue;
         }
     }
@@ -95,6 +96,7 @@ hipError_t pvx::trace_launch(const TraceKernel &k, const TraceArgs &a, uint32_t 
     case kWalkStep4:
     case kWalkWW4:
     case kWalkBinary: return trace_part7(k, a, nb, st, occ);
+    case kWalkRevolution: return trace_part14(k, a, nb, st, occ);
     case kWalkComplex:
         return (k.flags ? (k.count ? trace_part11 : trace_part10) : (k.count ? trace_part9 : trace_part8))(k, a, nb, st,
                                                                                                         occ);
@@ -302,6 +304,10 @@ extern "C" int pvx_scene_create(const pvx_scene_desc *desc, const pvx_build_opts
     UP(h.mesh_vtx, mesh_vtx);
     UP(h.mesh_nrm, mesh_nrm);
     UP(h.triangles, triangles);
+    UP(h.revs, revs);
+    UP(h.rev_segs, rev_segs);
+    UP(h.rev_radius, rev_radius);
+    UP(h.rev_height, rev_height);
 #undef UP
     if (h.use_kd) {
         if ((r = upload(s, h.kd_nodes, &s->kd.nodes)) != PVX_OK || (r = upload(s, h.kd_list, &s->kd.list)) != PVX_OK) {
@@ -337,8 +343,11 @@ extern "C" int pvx_scene_create(const pvx_scene_desc *desc, const pvx_build_opts
     int per_cu = 0;
     {
         const uint32_t level = kernel_level(v);
-        const TraceKernel k{level >= 2 ? kWalkComplex : level == 1 ? kWalkBinary : kWalkWW4, h.any_flags != 0, false,
-                            false};
+        const TraceKernel k{level >= 3   ? kWalkRevolution
+                            : level == 2 ? kWalkComplex
+                            : level == 1 ? kWalkBinary
+                                         : kWalkWW4,
+                            h.any_flags != 0, false, false};
         TraceArgs none{};
         HIP_TRY(trace_launch(k, none, 0, nullptr, &per_cu));
     }
@@ -406,7 +415,9 @@ static int launch_variant(pvx_scene *s, const pvx_ray *rays, const double *max_d
     const uint32_t level = kernel_level(s->view);
     if (s->use_kd)
         k.walk = kWalkKd;
-    else if (level >= 2)
+    else if (level >= 3)
+        k.walk = kWalkRevolution;
+    else if (level == 2)
         k.walk = kWalkComplex;
     else if (level == 1 || variant == PVX_TRAVERSAL_BINARY)
         k.walk = kWalkBinary;
@@ -621,7 +632,11 @@ extern "C" int pvx_batch_create(pvx_scene *const *scenes, uint32_t n_scenes, pvx
     }
     // persistent grids of the two kernel families the batch can launch
     int cus = 0, per_ww = 0, per = 0;
-    const TraceKernel kb{level >= 2 ? kWalkComplex : level == 1 ? kWalkBinary : kWalkStep4, flags != 0, false, true};
+    const TraceKernel kb{level >= 3   ? kWalkRevolution
+                         : level == 2 ? kWalkComplex
+                         : level == 1 ? kWalkBinary
+                                      : kWalkStep4,
+                         flags != 0, false, true};
     const TraceKernel kw{kWalkWW4, flags != 0, false, true};
     TraceArgs none{};
     e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
@@ -710,8 +725,12 @@ static int launch_batch(pvx_batch *b, const pvx_ray *rays, const uint64_t *first
     const SceneView *views = static_cast<const SceneView *>(b->views.p);
     const uint4 *chunks = static_cast<const uint4 *>(b->chunks.p);
     int32_t *spill = static_cast<int32_t *>(b->spill.p);
-    const TraceKernel k{b->level >= 2 ? kWalkComplex : b->level == 1 ? kWalkBinary : ww ? kWalkWW4 : kWalkStep4, flags,
-                        kCount, true};
+    const TraceKernel k{b->level >= 3   ? kWalkRevolution
+                        : b->level == 2 ? kWalkComplex
+                        : b->level == 1 ? kWalkBinary
+                        : ww            ? kWalkWW4
+                                        : kWalkStep4,
+                        flags, kCount, true};
     TraceArgs ta{};
     ta.views = views;
     ta.chunks = chunks;

@@ -40,6 +40,7 @@ enum PrimKind : uint32_t {
     kPrimDisc = 8,       // disc, by reference
     kPrimQuadric = 9,    // quadric (outside the BVH)
     kPrimPolygon = 10,   // polygon, by reference
+    kPrimRevolution = 11,// lathe or sor, by reference
 };
 
 struct alignas(16) Node {
@@ -184,6 +185,30 @@ struct PolygonRec {
     uint32_t pad;
 };
 
+// Lathe / sor after Compute_Lathe / Compute_Sor and Create_BCyl
+// (core/shape/lathe.cpp:488-760, sor.cpp:920-1180,
+// core/bounding/boundingcylinder.cpp:408-560): the overall cylinder, the
+// squared base and cap radii (sor), the segment range, the placement
+// (always valid: identity if the scene gave none) and flags = kind
+// (PVX_LATHE_* / PVX_SOR_CUBIC) | kRevClosed | kRevSturm.
+constexpr int kRevMaxSegments = 64; // segments per lathe / sor on the device
+constexpr uint32_t kRevClosed = 8u;
+constexpr uint32_t kRevSturm = 16u;
+struct RevRec {
+    double radius1, radius2, height1, height2;
+    double base_r2, cap_r2;
+    uint32_t seg0, nseg;
+    int32_t xf;
+    uint32_t flags;
+};
+// One spline segment: x (radius) and y (height) cubic coefficients (a sor
+// keeps its r^2(h) cubic in [0]) and the indices of its bounding cylinder's
+// radii (squared) and heights in the scene-wide rev_radius / rev_height.
+struct RevSeg {
+    double a[2], b[2], c[2], d[2];
+    uint32_t r1, r2, h1, h2;
+};
+
 struct PlaneRec {
     double n[3];
     double d;
@@ -271,13 +296,18 @@ struct SceneView {
     const float *mesh_vtx;           // 3 per vertex (all meshes)
     const float *mesh_nrm;           // 3 per normal (all meshes: vertex + face normals)
     const TriangleRec *triangles;    // stand-alone triangles (by triangle index)
+    const RevRec *revs;              // lathes and sors
+    const RevSeg *rev_segs;
+    const double *rev_radius;        // bounding cylinder radii (squared)
+    const double *rev_height;        // bounding cylinder heights
     const Node4 *nodes4;             // 4-wide tree over the same leaves (wide scenes)
     int32_t root4;     // root link of the 4-wide tree
     uint32_t wide;     // 1: triangle/sphere scene traversed through nodes4
     int32_t root;      // root link (node index or leaf link); INT32_MIN = empty
     uint32_t n_inf;
     uint32_t any_flags;
-    uint32_t has_complex; // 0: mesh triangles/spheres only, 1: + boxes, stand-alone triangles, 2: + tori or CSG
+    uint32_t has_complex; // 0: mesh triangles/spheres only, 1: + boxes, stand-alone triangles, 2: + tori or CSG,
+                          // 3: + lathes or sors
 };
 
 } // namespace pvx

@@ -356,6 +356,230 @@ TriangleRec make_triangle(const double *P, const double *N, bool smooth) {
     return t;
 }
 
+// Solve_Polynomial for a quadratic with epsilon 0 (polynomialsolver.cpp:
+// 1585-1625 and solve_quadratic 810-860), c descending; the extremes of a
+// spline segment are the roots of its derivative.
+int quadratic_roots(const double *c0, double *r) {
+    int i = 0, n = 2;
+    while (i < n && std::fabs(c0[i]) < 1.0e-10) i++;
+    n -= i;
+    const double *c = c0 + i;
+    if (n == 0) return 0;
+    if (n == 1) {
+        if (c[0] == 0.0) return 0;
+        r[0] = -c[1] / c[0];
+        return 1;
+    }
+    double a = c[0], b = -c[1], cc = c[2];
+    if (a == 0.0) {
+        if (b == 0.0) return 0;
+        r[0] = cc / b;
+        return 1;
+    }
+    b /= a; // monic form
+    cc /= a;
+    a = 1.0;
+    double d = b * b - 4.0 * a * cc;
+    if (d > -1.0e-10 && d < 1.0e-10) { // a double root
+        r[0] = 0.5 * b / a;
+        return 1;
+    }
+    if (d < 0.0) return 0;
+    d = std::sqrt(d);
+    const double t = 2.0 * a;
+    r[0] = (b + d) / t;
+    r[1] = (b - d) / t;
+    return 2;
+}
+
+// Inverse of a 4x4 matrix (MInvers, matrix.cpp:1154-1202): the adjugate
+// over the determinant, each cofactor the rule-of-Sarrus determinant of its
+// minor with the terms in the reference's order (the sor coefficients then
+// round as the reference's do); false when singular (the reference throws).
+bool invert4(const double (&m)[4][4], double (&r)[4][4]) {
+    double cof[4][4];
+    for (int row = 0; row < 4; row++)
+        for (int col = 0; col < 4; col++) {
+            double a[3][3];
+            for (int i = 0, ii = 0; i < 4; i++) {
+                if (i == row) continue;
+                for (int j = 0, jj = 0; j < 4; j++)
+                    if (j != col) a[ii][jj++] = m[i][j];
+                ii++;
+            }
+            cof[row][col] = a[0][0] * a[1][1] * a[2][2] + a[0][1] * a[1][2] * a[2][0] + a[0][2] * a[1][0] * a[2][1] -
+                            a[2][0] * a[1][1] * a[0][2] - a[2][1] * a[1][2] * a[0][0] - a[2][2] * a[1][0] * a[0][1];
+        }
+    const double det = m[0][0] * cof[0][0] - m[0][1] * cof[0][1] + m[0][2] * cof[0][2] - m[0][3] * cof[0][3];
+    if (det == 0.0) return false;
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) r[i][j] = (((i + j) & 1) ? -cof[j][i] : cof[j][i]) / det;
+    return true;
+}
+
+// A lathe or sor as Parse_Lathe / Parse_Sor leave it (parser/parser.cpp:
+// 2812-2995, 5628-5714): the segments of Compute_Lathe (lathe.cpp:488-760)
+// or Compute_Sor (sor.cpp:920-1180) with each segment's bounding cylinder,
+// then Create_BCyl's lists of distinct radii (squared) and heights
+// (boundingcylinder.cpp:408-560) appended to the scene-wide lists.  Returns
+// an error message (empty on success).
+std::string make_revolution(const double *pts, int n, uint32_t kind, uint32_t flags, RevRec &rec,
+                            std::vector<RevSeg> &segs, std::vector<double> &rad_all,
+                            std::vector<double> &hgt_all) {
+    auto X = [&](int i) { return pts[2 * i]; };
+    auto Y = [&](int i) { return pts[2 * i + 1]; };
+    int nseg = 0;
+    switch (kind) {
+    case PVX_LATHE_LINEAR: nseg = n - 1; break;
+    case PVX_LATHE_QUADRATIC: nseg = n - 2; break;
+    case PVX_LATHE_CUBIC: nseg = n - 3; break;
+    case PVX_LATHE_BEZIER: nseg = (n & 3) ? 0 : n / 4; break;
+    case PVX_SOR_CUBIC: nseg = n - 3; break;
+    default: return "unknown lathe or sor spline kind " + std::to_string(kind);
+    }
+    if (nseg < 1) return "too few points for the lathe or sor spline kind";
+    if (nseg > kRevMaxSegments) return "more than " + std::to_string(kRevMaxSegments) + " segments in a lathe or sor";
+    for (int i = 0; i < n; i++) { // the parser's point checks (v3.8 rules)
+        const double x = X(i);
+        bool bad;
+        switch (kind) {
+        case PVX_LATHE_LINEAR: bad = x < 0.0; break;
+        case PVX_LATHE_QUADRATIC: bad = i > 0 && x < 0.0; break;
+        case PVX_LATHE_CUBIC: bad = i > 0 && i < n - 1 && x < 0.0; break;
+        case PVX_LATHE_BEZIER: bad = (i % 4 == 0 || i % 4 == 3) && x < 0.0; break;
+        default: bad = x < 0.0 || (i > 1 && i < n - 1 && Y(i) <= Y(i - 1)); break;
+        }
+        if (bad) return kind == PVX_SOR_CUBIC ? "incorrect point in surface of revolution"
+                                              : "lathe point with an x value < 0";
+    }
+    std::vector<double> r1(nseg), r2(nseg), h1(nseg), h2(nseg);
+    const size_t seg0 = segs.size();
+    segs.resize(seg0 + nseg);
+    for (int g = 0; g < nseg; g++) {
+        RevSeg &sg = segs[seg0 + g];
+        std::memset(&sg, 0, sizeof sg);
+        double xs[4], ys[4];
+        if (kind == PVX_SOR_CUBIC) {
+            const int i = g;
+            if (std::fabs(Y(i + 2) - Y(i)) < kEpsilon || std::fabs(Y(i + 3) - Y(i + 1)) < kEpsilon)
+                return "incorrect point in surface of revolution";
+            // Hermite cubic in the height through r^2 at points i+1, i+2 with
+            // the slopes 2 r dr/dh of the neighbouring chords
+            const double k[4] = {X(i + 1) * X(i + 1), X(i + 2) * X(i + 2),
+                                 (X(i + 2) - X(i)) / (Y(i + 2) - Y(i)) * (2.0 * X(i + 1)),
+                                 (X(i + 3) - X(i + 1)) / (Y(i + 3) - Y(i + 1)) * (2.0 * X(i + 2))};
+            const double w1 = Y(i + 1), w2 = Y(i + 2);
+            const double M[4][4] = {{w1 * w1 * w1, w1 * w1, w1, 1.0},
+                                    {w2 * w2 * w2, w2 * w2, w2, 1.0},
+                                    {3.0 * w1 * w1, 2.0 * w1, 1.0, 0.0},
+                                    {3.0 * w2 * w2, 2.0 * w2, 1.0, 0.0}};
+            double Mi[4][4];
+            if (!invert4(M, Mi)) return "singular matrix in a surface of revolution";
+            double c[4];
+            for (int r = 0; r < 4; r++) {
+                c[r] = k[0] * Mi[r][0] + k[1] * Mi[r][1] + k[2] * Mi[r][2] + k[3] * Mi[r][3];
+                if (std::fabs(c[r]) < kEpsilon) c[r] = 0.0;
+            }
+            sg.a[0] = c[0], sg.b[0] = c[1], sg.c[0] = c[2], sg.d[0] = c[3];
+            xs[0] = xs[2] = X(i + 1);
+            xs[1] = xs[3] = X(i + 2);
+            const double dq[3] = {3.0 * c[0], 2.0 * c[1], c[2]};
+            double rr[2];
+            int nr = quadratic_roots(dq, rr);
+            while (nr--)
+                if (rr[nr] >= w1 && rr[nr] <= w2)
+                    xs[nr] = std::sqrt(rr[nr] * (rr[nr] * (rr[nr] * c[0] + c[1]) + c[2]) + c[3]);
+            r1[g] = std::min(std::min(xs[0], xs[1]), std::min(xs[2], xs[3]));
+            r2[g] = std::max(std::max(xs[0], xs[1]), std::max(xs[2], xs[3]));
+            h1[g] = w1;
+            h2[g] = w2;
+            continue;
+        }
+        const int i = kind == PVX_LATHE_BEZIER ? 4 * g : g;
+        for (int ax = 0; ax < 2; ax++) {
+            auto P = [&](int j) { return ax == 0 ? X(j) : Y(j); };
+            double A = 0.0, B = 0.0, C, D;
+            if (kind == PVX_LATHE_LINEAR) {
+                C = -1.0 * P(i) + 1.0 * P(i + 1);
+                D = 1.0 * P(i);
+            } else if (kind == PVX_LATHE_QUADRATIC) {
+                B = 0.5 * P(i) - 1.0 * P(i + 1) + 0.5 * P(i + 2);
+                C = -0.5 * P(i) + 0.5 * P(i + 2);
+                D = 1.0 * P(i + 1);
+            } else if (kind == PVX_LATHE_CUBIC) { // Catmull-Rom through points i+1, i+2
+                A = -0.5 * P(i) + 1.5 * P(i + 1) - 1.5 * P(i + 2) + 0.5 * P(i + 3);
+                B = P(i) - 2.5 * P(i + 1) + 2.0 * P(i + 2) - 0.5 * P(i + 3);
+                C = -0.5 * P(i) + 0.5 * P(i + 2);
+                D = P(i + 1);
+            } else { // Bezier, Bernstein form
+                A = P(i + 3) - 3.0 * P(i + 2) + 3.0 * P(i + 1) - P(i);
+                B = 3.0 * P(i + 2) - 6.0 * P(i + 1) + 3.0 * P(i);
+                C = 3.0 * P(i + 1) - 3.0 * P(i);
+                D = P(i);
+            }
+            sg.a[ax] = A, sg.b[ax] = B, sg.c[ax] = C, sg.d[ax] = D;
+            double *v = ax == 0 ? xs : ys;
+            if (kind == PVX_LATHE_BEZIER) {
+                for (int j = 0; j < 4; j++) v[j] = P(i + j);
+            } else {
+                const int j0 = kind == PVX_LATHE_LINEAR ? i : i + 1;
+                v[0] = v[2] = P(j0);
+                v[1] = v[3] = P(j0 + 1);
+            }
+            if (kind == PVX_LATHE_QUADRATIC || kind == PVX_LATHE_CUBIC) { // extremes inside [0, 1]
+                const double dq[3] = {3.0 * A, 2.0 * B, C};
+                double rr[2];
+                int nr = quadratic_roots(dq, rr);
+                while (nr--)
+                    if (rr[nr] >= 0.0 && rr[nr] <= 1.0) v[nr] = rr[nr] * (rr[nr] * (rr[nr] * A + B) + C) + D;
+            }
+        }
+        r1[g] = std::min(std::min(xs[0], xs[1]), std::min(xs[2], xs[3]));
+        r2[g] = std::max(std::max(xs[0], xs[1]), std::max(xs[2], xs[3]));
+        h1[g] = std::min(std::min(ys[0], ys[1]), std::min(ys[2], ys[3]));
+        h2[g] = std::max(std::max(ys[0], ys[1]), std::max(ys[2], ys[3]));
+    }
+    // the overall cylinder, and the distinct radii / heights in first-use order
+    double xmin = kBoundHuge, xmax = -kBoundHuge, ymin = kBoundHuge, ymax = -kBoundHuge;
+    std::vector<double> rad, hgt;
+    auto index_of = [](std::vector<double> &list, double v) {
+        size_t j = 0;
+        while (j < list.size() && list[j] != v) j++;
+        if (j == list.size()) list.push_back(v);
+        return (uint32_t)j;
+    };
+    const uint32_t rbase = (uint32_t)rad_all.size(), hbase = (uint32_t)hgt_all.size();
+    for (int g = 0; g < nseg; g++) {
+        xmin = std::min(xmin, r1[g]);
+        xmax = std::max(xmax, r2[g]);
+        ymin = std::min(ymin, h1[g]);
+        ymax = std::max(ymax, h2[g]);
+        RevSeg &sg = segs[seg0 + g];
+        sg.r1 = rbase + index_of(rad, r1[g]);
+        sg.r2 = rbase + index_of(rad, r2[g]);
+        sg.h1 = hbase + index_of(hgt, h1[g]);
+        sg.h2 = hbase + index_of(hgt, h2[g]);
+    }
+    for (double v : rad) rad_all.push_back(v * v);
+    hgt_all.insert(hgt_all.end(), hgt.begin(), hgt.end());
+    rec.radius1 = xmin;
+    rec.radius2 = xmax;
+    rec.height1 = ymin;
+    rec.height2 = ymax;
+    rec.base_r2 = rec.cap_r2 = 0.0;
+    if (kind == PVX_SOR_CUBIC) { // cap and base radii squared, clamped at 0
+        const RevSeg &top = segs[seg0 + nseg - 1], &bot = segs[seg0];
+        const double wt = h2[nseg - 1], wb = h1[0];
+        rec.cap_r2 = std::max(0.0, wt * (wt * (top.a[0] * wt + top.b[0]) + top.c[0]) + top.d[0]);
+        rec.base_r2 = std::max(0.0, wb * (wb * (bot.a[0] * wb + bot.b[0]) + bot.c[0]) + bot.d[0]);
+    }
+    rec.seg0 = (uint32_t)seg0;
+    rec.nseg = (uint32_t)nseg;
+    rec.flags = kind | ((kind == PVX_SOR_CUBIC && !(flags & PVX_REV_OPEN)) ? kRevClosed : 0u) |
+                ((flags & PVX_REV_STURM) ? kRevSturm : 0u);
+    return std::string();
+}
+
 } // namespace
 
 int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene &out,
@@ -382,7 +606,11 @@ int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene
     if (!need(d.n_discs == 0 || (d.disc_center && d.disc_normal && d.disc_radius), "discs")) return PVX_EINVAL;
     if (!need(d.n_quadrics == 0 || d.quadric_coef, "quadrics")) return PVX_EINVAL;
     if (!need(d.n_polygons == 0 || (d.polygon_first && d.polygon_points), "polygons")) return PVX_EINVAL;
+    if (!need(d.n_revolutions == 0 || (d.revolution_first && d.revolution_points && d.revolution_kind),
+              "lathes and sors"))
+        return PVX_EINVAL;
 
+    bool uses_rev = false; // a lathe or sor is referred to (the level-3 walks)
     auto check_ref = [&](uint32_t type, uint32_t index) -> bool {
         uint32_t lim = 0;
         switch (type) {
@@ -398,6 +626,15 @@ int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene
         case PVX_DISC: lim = d.n_discs; break;
         case PVX_QUADRIC: lim = d.n_quadrics; break;
         case PVX_POLYGON: lim = d.n_polygons; break;
+        case PVX_LATHE:
+        case PVX_SOR:
+            if (index < d.n_revolutions && (d.revolution_kind[index] == PVX_SOR_CUBIC) != (type == PVX_SOR)) {
+                err = "a lathe or sor object refers to a shape of the other kind (index " + std::to_string(index) + ")";
+                return false;
+            }
+            uses_rev = true;
+            lim = d.n_revolutions;
+            break;
         default:
             err = "unknown object type " + std::to_string(type);
             return false;
@@ -687,6 +924,32 @@ int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene
                                              kind[i] == 2);
     }
 
+    // lathes and sors: segments, bounding cylinders, placement (always a
+    // TRANSFORM, identity when none is given)
+    out.revs.resize(d.n_revolutions);
+    for (uint32_t i = 0; i < d.n_revolutions; i++) {
+        const uint32_t b = d.revolution_first[i], e = d.revolution_first[i + 1];
+        if (e < b) {
+            err = "lathe / sor point offsets decrease at " + std::to_string(i);
+            return PVX_EINVAL;
+        }
+        RevRec &rec = out.revs[i];
+        const std::string msg =
+            make_revolution(d.revolution_points + 2 * (size_t)b, (int)(e - b), d.revolution_kind[i],
+                            d.revolution_flags ? d.revolution_flags[i] : 0u, rec, out.rev_segs, out.rev_radius,
+                            out.rev_height);
+        if (!msg.empty()) {
+            err = msg + " (lathe / sor " + std::to_string(i) + ")";
+            return PVX_EINVAL;
+        }
+        const int32_t t = d.revolution_transform ? d.revolution_transform[i] : -1;
+        if (t < -1 || t >= (int32_t)d.n_transforms) {
+            err = "lathe / sor transform index out of range";
+            return PVX_EINVAL;
+        }
+        rec.xf = t >= 0 ? t : get_identity();
+    }
+
     // --- CSG tables ------------------------------------------------------------
     out.csg.resize(d.n_csg);
     out.csg_children.resize(d.n_csg_children);
@@ -790,6 +1053,12 @@ int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene
             const ConeRec &c = out.cones[index];
             double lo[3] = {-1.0, -1.0, c.dist}, hi[3] = {1.0, 1.0, 1.0};
             return transformed_box(lo, hi, out.xforms[c.xf].m);
+        }
+        case PVX_LATHE:   // Lathe::Compute_BBox (lathe.cpp:455-461)
+        case PVX_SOR: {   // Sor::Compute_BBox (sor.cpp:705-711)
+            const RevRec &r = out.revs[index];
+            double lo[3] = {-r.radius2, r.height1, -r.radius2}, hi[3] = {r.radius2, r.height2, r.radius2};
+            return transformed_box(lo, hi, out.xforms[r.xf].m);
         }
         case PVX_DISC: { // Disc::Compute_BBox (disc.cpp): the thin box also bounds the
                          // disc as a CSG intersection operand, as in the reference
@@ -907,6 +1176,13 @@ int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene
         }
         case PVX_PLANE:
             out.inf.push_back(InfRec{kPrimPlane, ob.index, (int32_t)o, 0});
+            break;
+        case PVX_LATHE:
+        case PVX_SOR:
+            pr.kind_axis = kPrimRevolution;
+            pr.ref.index = ob.index;
+            to_fbox(shape_bnd(ob.type, ob.index, 0).S, lo, hi);
+            add_prim(pr, lo, hi);
             break;
         case PVX_POLYGON: {
             if (out.polygons[ob.index].degenerate) break; // never hit (Polygon::Intersect)
@@ -1055,6 +1331,7 @@ int prepare_scene(const pvx_scene_desc &d, const pvx_build_opts *opts, HostScene
         }
     }
 
+    if (uses_rev) out.has_complex = 3; // lathes / sors: the level-3 walks
     // --- BVH ---------------------------------------------------------------------
     int leaf = opts && opts->max_leaf_size ? (int)opts->max_leaf_size : 4;
     int bins = opts && opts->sah_bins ? (int)opts->sah_bins : 16;

@@ -34,6 +34,9 @@ struct HostScene {
     std::vector<QuadricRec> quadrics;
     std::vector<PolygonRec> polygons;
     std::vector<double> poly_pts;
+    std::vector<RevRec> revs;            // lathes and sors
+    std::vector<RevSeg> rev_segs;
+    std::vector<double> rev_radius, rev_height; // bounding cylinder lists (radii squared)
     std::vector<Xform> xforms;
     std::vector<double> sph; // 4 per sphere: centre, radius
     std::vector<CsgNode> csg;
@@ -45,7 +48,7 @@ struct HostScene {
     std::vector<int32_t> obj_mesh_xf; // per object: a mesh's placement (xforms index) or -1
     std::vector<float> mesh_vtx, mesh_nrm;
     uint32_t any_flags = 0;
-    uint32_t has_complex = 0; // 0: triangles/spheres, 1: + boxes, 2: + tori or CSG
+    uint32_t has_complex = 0; // 0: triangles/spheres, 1: + boxes, 2: + tori or CSG, 3: + lathes or sors
     uint64_t n_triangles = 0;
     double build_ms = 0.0;
     double bvh_ms = 0.0;         // the tree build alone

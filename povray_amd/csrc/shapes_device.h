@@ -261,10 +261,14 @@ constexpr int kMaxIter = 50;
 constexpr double kSmall = 1.0e-10;
 constexpr double kRel = 1.0e-12;
 
-struct P4 {
+// A polynomial of order <= N, ascending powers (the reference's `polynomial`
+// holds up to MAX_ORDER 35; the shapes here need 4 (torus) and 6 (lathe)).
+template <int N>
+struct PN {
     int ord;
-    double c[5]; // ascending powers
+    double c[N + 1];
 };
+using P4 = PN<4>;
 
 __device__ inline double eval(double x, int n, const double *c) {
     double v = c[n];
@@ -272,7 +276,8 @@ __device__ inline double eval(double x, int n, const double *c) {
     return v;
 }
 
-__device__ inline int modp(const P4 &u, const P4 &v, P4 &r) {
+template <int N>
+__device__ inline int modp(const PN<N> &u, const PN<N> &v, PN<N> &r) {
     r = u;
     if (v.c[v.ord] < 0.0) {
         for (int k = u.ord - v.ord - 1; k >= 0; k -= 2) r.c[k] = -r.c[k];
@@ -291,7 +296,8 @@ __device__ inline int modp(const P4 &u, const P4 &v, P4 &r) {
     return r.ord;
 }
 
-__device__ inline int buildsturm(int ord, P4 *s) {
+template <int N>
+__device__ inline int buildsturm(int ord, PN<N> *s) {
     s[0].ord = ord;
     s[1].ord = ord - 1;
     double f = fabs(s[0].c[ord] * ord);
@@ -306,7 +312,8 @@ __device__ inline int buildsturm(int ord, P4 *s) {
     return k;
 }
 
-__device__ inline int numchanges(int np, const P4 *s, double a) {
+template <int N>
+__device__ inline int numchanges(int np, const PN<N> *s, double a) {
     int ch = 0;
     double lf = eval(a, s[0].ord, s[0].c);
     for (int k = 1; k <= np; k++) {
@@ -317,7 +324,8 @@ __device__ inline int numchanges(int np, const P4 *s, double a) {
     return ch;
 }
 
-__device__ inline int visible_roots(int np, const P4 *s) {
+template <int N>
+__device__ inline int visible_roots(int np, const PN<N> *s) {
     int atpos = 0, atzero = 0;
     double lf = s[0].c[s[0].ord];
     for (int k = 1; k <= np; k++) {
@@ -363,12 +371,13 @@ __device__ inline int regula_falsa(int order, const double *coef, double a, doub
 }
 
 // sbisect over an explicit stack.
-__device__ inline int sbisect(int np, const P4 *s, double lo, double hi, int atlo, int athi,
+template <int N>
+__device__ inline int sbisect(int np, const PN<N> *s, double lo, double hi, int atlo, int athi,
                               double *roots) {
     struct Iv {
         double lo, hi;
         int atlo, athi;
-    } st[8];
+    } st[N + 2]; // each pending interval holds at least one of the <= N roots
     int sp = 0, nr = 0;
     st[sp++] = Iv{lo, hi, atlo, athi};
     while (sp > 0) {
@@ -411,7 +420,7 @@ __device__ inline int sbisect(int np, const P4 *s, double lo, double hi, int atl
             }
             int n1 = amin - am, n2 = am - amax;
             if (n1 != 0 && n2 != 0) {
-                if (sp + 2 <= 8) {
+                if (sp + 2 <= N + 2) {
                     st[sp++] = Iv{mid, mx, am, amax}; // right, visited second
                     st[sp++] = Iv{mn, mid, amin, am}; // left, visited first
                 }
@@ -425,8 +434,10 @@ __device__ inline int sbisect(int np, const P4 *s, double lo, double hi, int atl
     return nr;
 }
 
+// (an order-N polynomial has at most N roots pending on the interval stack)
+template <int N = 4>
 __device__ inline int polysolve(int order, const double *coeffs, double *roots) {
-    P4 s[6];
+    PN<N> s[N + 2];
     for (int i = 0; i <= order; i++) s[0].c[order - i] = coeffs[i] / coeffs[0];
     int np = buildsturm(order, s);
     if (visible_roots(np, s) == 0) return 0;
@@ -566,6 +577,31 @@ __device__ inline int solve4(const double *c0, double *r, bool sturm, double eps
             return sturm ? polysolve(3, c, r) : solve_cubic(c, r);
         if (difficult_coeffs(c)) sturm = true;
         return sturm ? polysolve(4, c, r) : solve_quartic(c, r);
+    }
+}
+
+// Solve_Polynomial with epsilon 0 for orders up to 6 (the lathe's and sor's
+// calls, lathe.cpp:255-285 and sor.cpp:395), c descending.
+__device__ inline int solve6(int n, const double *c0, double *r, bool sturm) {
+    int i = 0;
+    while (i < n && fabs(c0[i]) < kSmall) i++;
+    n -= i;
+    const double *c = &c0[i];
+    switch (n) {
+    case 0:
+        return 0;
+    case 1:
+        if (c[0] != 0.0) { r[0] = -c[1] / c[0]; return 1; }
+        return 0;
+    case 2:
+        return solve_quadratic(c, r);
+    case 3:
+        return sturm ? polysolve<3>(3, c, r) : solve_cubic(c, r);
+    case 4:
+        if (difficult_coeffs(c)) sturm = true;
+        return sturm ? polysolve<4>(4, c, r) : solve_quartic(c, r);
+    default:
+        return polysolve<6>(n, c, r);
     }
 }
 } // namespace poly
@@ -859,12 +895,248 @@ __device__ inline bool polygon_hit(const PolygonRec &pg, const double *pts, cons
 }
 
 // ---------------------------------------------------------------------------
+// Lathe (core/shape/lathe.cpp) and surface of revolution (core/shape/sor.cpp)
+// with the bounding cylinders of core/bounding/boundingcylinder.cpp.  The
+// ray moves into the shape's space with its direction normalised; each
+// segment whose cylinder shell the ray crosses becomes an interval keyed by
+// its entry distance; the intervals are visited in that order and a
+// polynomial in the ray parameter is solved per segment.  A shape that is
+// not a CSG operand stops once the next interval starts beyond its best hit.
+// The ring and cylinder crossings of intersect_bound_elements are computed
+// per segment from the same expressions (the values are identical).
+// ---------------------------------------------------------------------------
+constexpr double kRevDepthTol = 1.0e-4; // DEPTH_TOLERANCE (lathe.cpp:84, sor.cpp:133)
+constexpr int kSorBase = 1, kSorCap = 2, kSorCurve = 3;
+
+// Entry distance of the ray into segment g's cylinder shell
+// (intersect_thick_cylinder + Intersect_BCyl's case analysis); false: missed.
+__device__ inline bool rev_shell_entry(const RevSeg &g, const double *rad, const double *hgt, D3 P, D3 D, double a,
+                                       double b, double b2, double bb, double c, bool dy, double &d0) {
+    double dist[6];
+    int n = 0;
+    const double R1 = rad[g.r1], R2 = rad[g.r2], H1 = hgt[g.h1], H2 = hgt[g.h2];
+    if (dy) { // cap ring, then base ring
+        double k = (H2 - P.y) / D.y, w = k * (a * k + b2) + c;
+        if (w >= R1 && w <= R2) dist[n++] = k;
+        k = (H1 - P.y) / D.y;
+        w = k * (a * k + b2) + c;
+        if (w >= R1 && w <= R2) dist[n++] = k;
+    }
+    for (int s = 0; s < 2; s++) { // inner, then outer cylinder
+        const double R = s == 0 ? R1 : R2;
+        if (R > kEpsilon) {
+            double e = bb - a * (c - R);
+            if (e > 0.0) {
+                e = sqrt(e);
+                double k = (-b + e) / a, h = P.y + k * D.y;
+                if (h >= H1 && h <= H2) dist[n++] = k;
+                k = (-b - e) / a;
+                h = P.y + k * D.y;
+                if (h >= H1 && h <= H2) dist[n++] = k;
+            }
+        }
+    }
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n - i - 1; j++)
+            if (dist[j] > dist[j + 1]) {
+                const double t = dist[j];
+                dist[j] = dist[j + 1];
+                dist[j + 1] = t;
+            }
+    if (n == 2 || n == 4) {
+        for (int p = 0; p < n; p += 2) {
+            if (dist[p] > kEpsilon) { d0 = dist[p]; return true; }
+            if (dist[p + 1] > kEpsilon) { d0 = 0.0; return true; }
+        }
+        return false;
+    }
+    if (n == 0) return false;
+    d0 = dist[0]; // an odd count: taken as starting inside the shell
+    return true;
+}
+
+// Every hit Lathe::Intersect / Sor::Intersect pushes, in push order:
+// emit(depth, side, d1) with side = segment (lathe) or part | segment << 2
+// (sor) and d1 = the spline parameter (lathe) or the ray parameter (sor).
+template <class Emit>
+__device__ inline void rev_visit(const SceneView &sv, uint32_t index, D3 o, D3 dw, bool child, Emit &&emit) {
+    const RevRec &s = sv.revs[index];
+    const Xform &x = sv.xforms[s.xf];
+    const D3 P = xf_point(x.inv, o);
+    D3 D = xf_dir(x.inv, dw);
+    const double len = sqrt(D.x * D.x + D.y * D.y + D.z * D.z);
+    D = D3{D.x / len, D.y / len, D.z / len};
+    if ((D.y >= 0.0 && P.y > s.height2) || (D.y <= 0.0 && P.y < s.height1) || (D.x >= 0.0 && P.x > s.radius2) ||
+        (D.x <= 0.0 && P.x < -s.radius2))
+        return;
+    const bool sor = (s.flags & 7u) == PVX_SOR_CUBIC;
+    const double a = D.x * D.x + D.z * D.z;
+    double r0 = P.x * D.z - P.z * D.x;
+    if (!sor) r0 = fabs(r0);
+    if (a > 0.0) r0 /= sqrt(a);
+    if (r0 > s.radius2) return;
+    const bool sturm = (s.flags & kRevSturm) != 0;
+    double best = kBoundHuge;
+    auto test_hit = [&](double dep, int side, double d1) {
+        if (dep > kRevDepthTol && dep < kMaxDistance) {
+            emit(dep, side, d1);
+            return true;
+        }
+        return false;
+    };
+    if (sor && (s.flags & kRevClosed) && fabs(D.y) > kEpsilon) { // base and cap discs
+        if (s.base_r2 > kRevDepthTol) {
+            const double k = (s.height1 - P.y) / D.y;
+            const double u = P.x + k * D.x, v = P.z + k * D.z;
+            if (u * u + v * v <= s.base_r2 && test_hit(k / len, kSorBase, k) && k < best) best = k;
+        }
+        if (s.cap_r2 > kRevDepthTol) {
+            const double k = (s.height2 - P.y) / D.y;
+            const double u = P.x + k * D.x, v = P.z + k * D.z;
+            if (u * u + v * v <= s.cap_r2 && test_hit(k / len, kSorCap, k) && k < best) best = k;
+        }
+    }
+    // Intersect_BCyl: intervals ordered by entry distance, a new one before
+    // any equal one (insert_hit)
+    const double b = P.x * D.x + P.z * D.z, bb = b * b, b2 = 2.0 * b, c = P.x * P.x + P.z * P.z;
+    const bool dy = D.y < -kEpsilon || D.y > kEpsilon;
+    double iv_d[kRevMaxSegments];
+    uint8_t iv_n[kRevMaxSegments];
+    int cnt = 0;
+    for (uint32_t i = 0; i < s.nseg; i++) {
+        double d0;
+        if (!rev_shell_entry(sv.rev_segs[s.seg0 + i], sv.rev_radius, sv.rev_height, P, D, a, b, b2, bb, c, dy, d0))
+            continue;
+        int pos = cnt;
+        while (pos > 0 && !(d0 > iv_d[pos - 1])) {
+            iv_d[pos] = iv_d[pos - 1];
+            iv_n[pos] = iv_n[pos - 1];
+            pos--;
+        }
+        iv_d[pos] = d0;
+        iv_n[pos] = (uint8_t)i;
+        cnt++;
+    }
+    const double m = D.x * P.x + D.z * P.z, Dy2 = D.y * D.y, Dylen = D.y * len;
+    const uint32_t kind = s.flags & 7u;
+    for (int j = 0; j < cnt; j++) {
+        if (!child && iv_d[j] > best) break;
+        const int seg = iv_n[j];
+        const RevSeg &g = sv.rev_segs[s.seg0 + seg];
+        if (sor) { // Sor::Intersect (sor.cpp:380-420): the cubic r^2(h) along the ray
+            const double A = g.a[0], B = g.b[0], C = g.c[0], Dd = g.d[0];
+            double xc[4], yr[3];
+            xc[0] = A * D.y * D.y * D.y;
+            xc[1] = D.y * D.y * (3.0 * A * P.y + B) - a;
+            xc[2] = D.y * (P.y * (3.0 * A * P.y + 2.0 * B) + C) - 2.0 * (P.x * D.x + P.z * D.z);
+            xc[3] = P.y * (P.y * (A * P.y + B) + C) + Dd - P.x * P.x - P.z * P.z;
+            int n = poly::solve6(3, xc, yr, sturm);
+            const double H1 = sv.rev_height[g.h1], H2 = sv.rev_height[g.h2];
+            while (n--) {
+                const double k = yr[n], h = P.y + k * D.y;
+                if (h >= H1 && h <= H2 && test_hit(k / len, kSorCurve | (seg << 2), k) && k < best) best = k;
+            }
+            continue;
+        }
+        // Lathe::Intersect (lathe.cpp:245-350): the spline's radius against
+        // the ray's distance from the axis at the spline's height
+        const double Ax = g.a[0], Ay = g.a[1], Bx = g.b[0], By = g.b[1], Cx = g.c[0], Cy = g.c[1], Dx = g.d[0],
+                     Dy = g.d[1];
+        const double r = a;
+        double x1[7], y1[6];
+        int n1;
+        if (kind == PVX_LATHE_LINEAR) {
+            x1[0] = Cy * Cy * r - Cx * Cx * Dy2;
+            x1[1] = 2.0 * (Cy * ((Dy - P.y) * r + D.y * m) - Cx * Dx * Dy2);
+            x1[2] = (Dy - P.y) * ((Dy - P.y) * r + 2.0 * D.y * m) + Dy2 * (P.x * P.x + P.z * P.z - Dx * Dx);
+            n1 = poly::solve6(2, x1, y1, false);
+        } else if (kind == PVX_LATHE_QUADRATIC) {
+            x1[0] = By * By * r - Bx * Bx * Dy2;
+            x1[1] = 2.0 * (By * Cy * r - Bx * Cx * Dy2);
+            x1[2] = r * (2.0 * By * (Dy - P.y) + Cy * Cy) + 2.0 * By * D.y * m - (2.0 * Bx * Dx + Cx * Cx) * Dy2;
+            x1[3] = 2.0 * (Cy * ((Dy - P.y) * r + D.y * m) - Cx * Dx * Dy2);
+            x1[4] = (Dy - P.y) * ((Dy - P.y) * r + 2.0 * D.y * m) + Dy2 * (P.x * P.x + P.z * P.z - Dx * Dx);
+            n1 = poly::solve6(4, x1, y1, sturm);
+        } else {
+            x1[0] = Ay * Ay * r - Ax * Ax * Dy2;
+            x1[1] = 2.0 * (Ay * By * r - Ax * Bx * Dy2);
+            x1[2] = (2.0 * Ay * Cy + By * By) * r - (2.0 * Ax * Cx + Bx * Bx) * Dy2;
+            x1[3] = 2.0 * ((Ay * Dy + By * Cy - Ay * P.y) * r + Ay * D.y * m - (Ax * Dx + Bx * Cx) * Dy2);
+            x1[4] = (2.0 * By * (Dy - P.y) + Cy * Cy) * r + 2.0 * By * D.y * m - (2.0 * Bx * Dx + Cx * Cx) * Dy2;
+            x1[5] = 2.0 * (Cy * ((Dy - P.y) * r + D.y * m) - Cx * Dx * Dy2);
+            x1[6] = (Dy - P.y) * ((Dy - P.y) * r + 2.0 * D.y * m) + Dy2 * (P.x * P.x + P.z * P.z - Dx * Dx);
+            n1 = poly::solve6(6, x1, y1, sturm);
+        }
+        while (n1--) {
+            const double w = y1[n1];
+            if (w < 0.0 || w > 1.0) continue;
+            if (fabs(D.y) > kEpsilon) {
+                double k = w * (w * (w * Ay + By) + Cy) + Dy - P.y;
+                if (test_hit(k / Dylen, seg, w)) {
+                    k /= D.y;
+                    if (k < best) best = k;
+                }
+            } else {
+                const double k = w * (w * (w * Ax + Bx) + Cx) + Dx;
+                double x2[3] = {r, 2.0 * m, P.x * P.x + P.z * P.z - k * k}, y2[2];
+                int n2 = poly::solve6(2, x2, y2, false);
+                while (n2--)
+                    if (test_hit(y2[n2] / len, seg, w) && y2[n2] < best) best = y2[n2];
+            }
+        }
+    }
+}
+
+// Lathe::Inside (lathe.cpp:380-450) / Sor::Inside (sor.cpp:455-505).
+__device__ inline bool rev_inside(const SceneView &sv, uint32_t index, D3 ip, bool inv) {
+    const RevRec &s = sv.revs[index];
+    const D3 P = xf_point(sv.xforms[s.xf].inv, ip);
+    if ((s.flags & 7u) == PVX_SOR_CUBIC) {
+        if (P.y >= s.height1 && P.y <= s.height2) {
+            const double r0 = P.x * P.x + P.z * P.z;
+            if (r0 <= s.radius2 * s.radius2) {
+                for (uint32_t i = 0; i < s.nseg; i++) {
+                    const RevSeg &g = sv.rev_segs[s.seg0 + i];
+                    if (P.y >= sv.rev_height[g.h1] && P.y <= sv.rev_height[g.h2]) {
+                        const double r = P.y * (P.y * (P.y * g.a[0] + g.b[0]) + g.c[0]) + g.d[0];
+                        if (r0 <= r) return !inv;
+                        break;
+                    }
+                }
+            }
+        }
+        return inv;
+    }
+    int nc = 0;
+    if (P.y >= s.height1 && P.y <= s.height2) {
+        const double r = sqrt(P.x * P.x + P.z * P.z);
+        if (r >= s.radius1 && r <= s.radius2) {
+            const bool sturm = (s.flags & kRevSturm) != 0;
+            for (uint32_t i = 0; i < s.nseg; i++) {
+                const RevSeg &g = sv.rev_segs[s.seg0 + i];
+                if (P.y >= sv.rev_height[g.h1] - kEpsilon && P.y <= sv.rev_height[g.h2] + kEpsilon) {
+                    double xc[4] = {g.a[1], g.b[1], g.c[1], g.d[1] - P.y}, yr[3];
+                    int n = poly::solve6(3, xc, yr, sturm);
+                    while (n--) {
+                        const double w = yr[n];
+                        if (w >= 0.0 && w <= 1.0 && w * (w * (w * g.a[0] + g.b[0]) + g.c[0]) + g.d[0] - r >= 0.0) nc++;
+                    }
+                }
+            }
+        }
+    }
+    return (nc & 1) ? !inv : inv;
+}
+
+// ---------------------------------------------------------------------------
 // CSG (core/shape/csg.cpp): point membership with bounded nesting.
 // ---------------------------------------------------------------------------
-template <int D>
+// kRev: lathes and sors may be operands (the walks of scenes without them
+// leave their code out, which keeps those kernels' register budget).
+template <int D, bool kRev = false>
 __device__ bool csg_inside(const SceneView &sv, uint32_t node, D3 p);
 
-template <int D>
+template <int D, bool kRev = false>
 __device__ inline bool shape_inside(const SceneView &sv, uint32_t type, uint32_t index,
                                     uint32_t flags, D3 p) {
     bool inv = (flags & PVX_FLAG_INVERTED) != 0;
@@ -890,13 +1162,17 @@ __device__ inline bool shape_inside(const SceneView &sv, uint32_t type, uint32_t
         return inv ? !in : in;
     }
     case 6:
-        return csg_inside<D>(sv, index, p);
+        return csg_inside<D, kRev>(sv, index, p);
+    case PVX_LATHE:
+    case PVX_SOR:
+        if constexpr (kRev) return rev_inside(sv, index, p, inv);
+        return inv;
     default:
         return inv; // meshes without inside_vector
     }
 }
 
-template <int D>
+template <int D, bool kRev>
 __device__ bool csg_inside(const SceneView &sv, uint32_t node, D3 p) {
     if constexpr (D <= 0) {
         return false;
@@ -904,7 +1180,7 @@ __device__ bool csg_inside(const SceneView &sv, uint32_t node, D3 p) {
         const CsgNode &n = sv.csg[node];
         for (uint32_t k = 0; k < n.count; k++) {
             const CsgChild &c = sv.csg_children[n.first + k];
-            bool in = shape_inside<D - 1>(sv, c.type, c.index, c.flags, p);
+            bool in = shape_inside<D - 1, kRev>(sv, c.type, c.index, c.flags, p);
             if (n.op == 2) { // intersection: CSGIntersection::Inside (csg.cpp:440-454)
                 if (!in) return false;
             } else if (in) { // union / merge: CSGUnion::Inside (csg.cpp:405-424)
@@ -952,10 +1228,27 @@ __device__ inline int leaf_hits(const SceneView &sv, uint32_t type, uint32_t ind
     }
 }
 
+// Every hit of one leaf shape in push order, as emit(depth, side, d1); `child`:
+// the shape is a CSG operand (IS_CHILD_OBJECT).
+template <bool kRev, class Emit>
+__device__ inline void leaf_visit(const SceneView &sv, uint32_t type, uint32_t index, D3 o, D3 d, bool child,
+                                  Emit &&emit) {
+    if constexpr (kRev) {
+        if (type == PVX_LATHE || type == PVX_SOR) {
+            rev_visit(sv, index, o, d, child, emit);
+            return;
+        }
+    }
+    double hs[kMaxLeafHits];
+    const int nh = leaf_hits(sv, type, index, o, d, hs);
+    for (int j = 0; j < nh; j++) emit(hs[j], 0, 0.0);
+}
+
 // Does a hit at depth `dep` on the leaf in csg_children slot `slot` survive
 // every CSG filter between the leaf and the root (CSGIntersection keeps
 // hits inside all siblings, CSGMerge hits inside none, CSGUnion all;
 // csg.cpp:128-375)?
+template <bool kRev = false>
 __device__ inline bool csg_survives(const SceneView &sv, uint32_t slot, D3 ip) {
     uint32_t child = slot;
     for (int level = 0; level < kCsgMaxDepth; level++) {
@@ -967,7 +1260,7 @@ __device__ inline bool csg_survives(const SceneView &sv, uint32_t slot, D3 ip) {
                 uint32_t sib = n.first + k;
                 if (sib == child) continue;
                 const CsgChild &s = sv.csg_children[sib];
-                bool in = shape_inside<kCsgMaxDepth>(sv, s.type, s.index, s.flags, ip);
+                bool in = shape_inside<kCsgMaxDepth, kRev>(sv, s.type, s.index, s.flags, ip);
                 if (is_merge ? in : !in) return false;
             }
         }

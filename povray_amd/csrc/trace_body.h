@@ -82,6 +82,8 @@ __device__ inline bool precondition(uint32_t flags, uint32_t kind) {
 
 // Closest valid hit of one CSG object (its leaves' hits that survive every
 // CSG filter up to the root); returns false when none beats `limit`.
+// kRev: lathes and sors may be operands.
+template <bool kRev = false>
 __device__ inline bool csg_closest(const SceneView &sv, uint32_t root, D3 o, D3 d, bool shadow,
                                    double limit, double &dep_out, int32_t &slot_out) {
     uint32_t b = sv.csg_leaf_range[2 * root], e = sv.csg_leaf_range[2 * root + 1];
@@ -90,18 +92,22 @@ __device__ inline bool csg_closest(const SceneView &sv, uint32_t root, D3 o, D3 
     for (uint32_t k = b; k < e; k++) {
         uint32_t slot = sv.csg_leaves[k];
         const CsgChild &c = sv.csg_children[slot];
-        double hs[kMaxLeafHits];
-        int nh = leaf_hits(sv, c.type, c.index, o, d, hs);
-        for (int j = 0; j < nh; j++) {
-            double dep = hs[j];
-            if (!depth_ok(dep, shadow)) continue;
-            if (!(dep < best || (found && dep == best && (int32_t)slot < slot_out))) continue;
+        auto consider = [&](double dep) {
+            if (!depth_ok(dep, shadow)) return;
+            if (!(dep < best || (found && dep == best && (int32_t)slot < slot_out))) return;
             D3 ip = add3(o, mul3(d, dep));
-            if (csg_survives(sv, slot, ip)) {
+            if (csg_survives<kRev>(sv, slot, ip)) {
                 best = dep;
                 slot_out = (int32_t)slot;
                 found = true;
             }
+        };
+        if constexpr (kRev) {
+            leaf_visit<true>(sv, c.type, c.index, o, d, true, [&](double dep, int, double) { consider(dep); });
+        } else {
+            double hs[kMaxLeafHits];
+            int nh = leaf_hits(sv, c.type, c.index, o, d, hs);
+            for (int j = 0; j < nh; j++) consider(hs[j]);
         }
     }
     dep_out = best;
@@ -124,9 +130,24 @@ __device__ inline bool quadric_closest(const QuadricRec &q, D3 o, D3 d, bool sha
 }
 
 // Closest valid hit of a box / torus / stand-alone triangle object
-// (Find_Intersection over its IStack).
+// (Find_Intersection over its IStack); with kRev also lathes and sors.
+template <bool kRev = false>
 __device__ inline bool simple_closest(const SceneView &sv, uint32_t kind, uint32_t index, D3 o,
                                       D3 d, bool shadow, double &dep_out) {
+    if constexpr (kRev) {
+        if (kind == kPrimRevolution) {
+            bool found = false;
+            double best = 0.0;
+            rev_visit(sv, index, o, d, false, [&](double dep, int, double) {
+                if (depth_ok(dep, shadow) && (!found || dep < best)) {
+                    best = dep;
+                    found = true;
+                }
+            });
+            dep_out = best;
+            return found;
+        }
+    }
     if (kind == kPrimTriangle64 || kind == kPrimDisc || kind == kPrimPolygon) {
         double dep;
         const bool hit = kind == kPrimDisc      ? disc_hit(sv.discs[index], sv.xforms, o, d, dep)
@@ -153,7 +174,8 @@ __device__ inline bool simple_closest(const SceneView &sv, uint32_t kind, uint32
     return found;
 }
 
-// kLevel 0: triangles and spheres; 1: also boxes; 2: also tori and CSG.
+// kLevel 0: triangles and spheres; 1: also boxes; 2: also tori and CSG; 3: also
+// lathes and sors.
 // kWW: while-while traversal (triangle scenes, camera and bounce rays).
 // kMulti: rays of many scenes in one launch.  `n` is then 64 x the number of
 // chunks; chunk c = {scene, first ray, ray count <= 64} covers the wavefront
@@ -216,7 +238,7 @@ __device__ __forceinline__ void trace_closest_body(const SceneView &sv0, const S
                 ok = plane_hit(sv.planes[inf.index], o, d, dep) && depth_ok(dep, shadow);
             } else if (kLevel >= 2) {
                 ok = inf.kind == kPrimQuadric ? quadric_closest(sv.quadrics[inf.index], o, d, shadow, dep)
-                                              : csg_closest(sv, inf.index, o, d, shadow, bt, dep, prim);
+                                              : csg_closest<(kLevel >= 3)>(sv, inf.index, o, d, shadow, bt, dep, prim);
             } else {
                 ok = false;
             }
@@ -454,9 +476,9 @@ __device__ __forceinline__ void trace_closest_body(const SceneView &sv0, const S
                             int32_t prim = 0;
                             bool ok;
                             if (pk == kPrimCsg)
-                                ok = csg_closest(sv, p.ref.index, o, d, shadow, bt, dep, prim);
+                                ok = csg_closest<(kLevel >= 3)>(sv, p.ref.index, o, d, shadow, bt, dep, prim);
                             else
-                                ok = simple_closest(sv, pk, p.ref.index, o, d, shadow, dep);
+                                ok = simple_closest<(kLevel >= 3)>(sv, pk, p.ref.index, o, d, shadow, dep);
                             if (ok && better(dep, obj, prim, bt, bo, bp)) {
                                 bt = dep;
                                 bo = obj;
@@ -491,7 +513,8 @@ __device__ __forceinline__ void trace_closest_body(const SceneView &sv0, const S
     }
 }
 
-// Closest valid hit among one primitive record of any kind, folded into
+// Closest valid hit among one primitive record of any kind (lathes and sors
+// included), folded into
 // (bt, bo, bp) as the leaf loops of k_trace_closest do.
 template <bool kFlags>
 __device__ __forceinline__ void prim_test_any(const SceneView &sv, const PrimRec &p, D3 o, D3 d, bool shadow,
@@ -521,9 +544,9 @@ __device__ __forceinline__ void prim_test_any(const SceneView &sv, const PrimRec
         int32_t prim = 0;
         bool ok;
         if (pk == kPrimCsg)
-            ok = csg_closest(sv, p.ref.index, o, d, shadow, bt, dep, prim);
+            ok = csg_closest<true>(sv, p.ref.index, o, d, shadow, bt, dep, prim);
         else
-            ok = simple_closest(sv, pk, p.ref.index, o, d, shadow, dep);
+            ok = simple_closest<true>(sv, pk, p.ref.index, o, d, shadow, dep);
         if (ok && better(dep, obj, prim, bt, bo, bp)) {
             bt = dep;
             bo = obj;

@@ -10,7 +10,7 @@
 #include "trace_launch.h"
 
 #ifndef PVX_TRACE_PART
-#error "build with -DPVX_TRACE_PART=<0..12>"
+#error "build with -DPVX_TRACE_PART=<0..14>"
 #endif
 
 namespace pvx {
@@ -94,7 +94,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_closest_kd(TraceArgs a) {
             int32_t prim = 0;
             bool ok = inf.kind == kPrimPlane     ? (plane_hit(sv.planes[inf.index], o, d, dep) && depth_ok(dep, shadow))
                       : inf.kind == kPrimQuadric ? quadric_closest(sv.quadrics[inf.index], o, d, shadow, dep)
-                                                 : csg_closest(sv, inf.index, o, d, shadow, bt, dep, prim);
+                                                 : csg_closest<true>(sv, inf.index, o, d, shadow, bt, dep, prim);
             if (ok && better(dep, inf.obj, prim, bt, bo, bp)) {
                 bt = dep;
                 bo = inf.obj;
@@ -216,10 +216,9 @@ __global__ void k_all_intersections(SceneView sv, const pvx_ray *rays, const uin
         for (uint32_t k = b; k < e; k++) {
             uint32_t slot = sv.csg_leaves[k];
             const CsgChild &c = sv.csg_children[slot];
-            double hs[kMaxLeafHits];
-            int nh = leaf_hits(sv, c.type, c.index, o, d, hs);
-            for (int j = 0; j < nh; j++)
-                if (csg_survives(sv, slot, add3(o, mul3(d, hs[j])))) push(hs[j], (int32_t)slot);
+            leaf_visit<true>(sv, c.type, c.index, o, d, true, [&](double dep, int, double) {
+                if (csg_survives<true>(sv, slot, add3(o, mul3(d, dep)))) push(dep, (int32_t)slot);
+            });
         }
     } else if (type == PVX_MESH) {
         // every triangle of the mesh: walk the whole BVH without culling by depth
@@ -263,9 +262,7 @@ __global__ void k_all_intersections(SceneView sv, const pvx_ray *rays, const uin
             node = sp == 0 ? kEmpty : stack[--sp];
         }
     } else {
-        double hs[kMaxLeafHits];
-        int nh = leaf_hits(sv, type, index, o, d, hs);
-        for (int j = 0; j < nh; j++) push(hs[j], 0);
+        leaf_visit<true>(sv, type, index, o, d, false, [&](double dep, int, double) { push(dep, 0); });
     }
     counts[i] = cnt;
     for (uint32_t k = 0; k < max_hits; k++) {
@@ -283,7 +280,7 @@ __global__ void k_inside(SceneView sv, const double *pts, const uint32_t *objs, 
     if (i >= n) return;
     D3 p = d3(pts[3 * (size_t)i], pts[3 * (size_t)i + 1], pts[3 * (size_t)i + 2]);
     uint32_t ob = objs[i];
-    out[i] = shape_inside<kCsgMaxDepth>(sv, obj_type[ob], obj_index[ob], obj_flags[ob], p) ? 1 : 0;
+    out[i] = shape_inside<kCsgMaxDepth, true>(sv, obj_type[ob], obj_index[ob], obj_flags[ob], p) ? 1 : 0;
 }
 #endif
 
@@ -340,6 +337,10 @@ hipError_t PVX_FN(PVX_TRACE_PART)(const TraceKernel &k, const TraceArgs &a, uint
     constexpr bool kF = (PVX_TRACE_PART - 8) >= 2, kC = (PVX_TRACE_PART - 8) & 1;
     (void)k;
     return run(k_trace_closest_multi<2, kF, kC>, a, nb, st, occ);
+#elif PVX_TRACE_PART == 13 // single scene with lathes or sors
+    return PVX_FC(k_trace_closest, 3, );
+#elif PVX_TRACE_PART == 14 // scene batches with lathes or sors
+    return PVX_FC(k_trace_closest_multi, 3, );
 #else
 #error "PVX_TRACE_PART out of range"
 #endif

@@ -3,7 +3,7 @@
 // The traversal kernels are templates over the walk, the object-flag
 // precondition, the counting mode and single/multi-scene batches.  They are
 // instantiated in trace_kernels.hip, which the Makefile compiles once per
-// part (PVX_TRACE_PART = 0..12), each part holding a few instantiations, so
+// part (PVX_TRACE_PART = 0..14), each part holding a few instantiations, so
 // that they build in parallel; trace_launch() picks the part.
 #ifndef PVX_TRACE_LAUNCH_H
 #define PVX_TRACE_LAUNCH_H
@@ -27,8 +27,16 @@ constexpr size_t kSpillPerThread = (size_t)kStackSpill * sizeof(int32_t);
 // Traversal walks: the 4-wide tree one node or leaf per pass (triangle and
 // sphere scenes, light rays), the 4-wide tree while-while (camera and bounce
 // rays), the binary tree (+ boxes and stand-alone triangles), the binary tree
-// with every primitive kind (+ tori, CSG, cones, ...), the kd-tree.
-enum TraceWalk : uint32_t { kWalkStep4 = 0, kWalkWW4 = 1, kWalkBinary = 2, kWalkComplex = 3, kWalkKd = 4 };
+// with every primitive kind (+ tori, CSG, cones, ...), the kd-tree, the
+// binary tree with lathes and sors as well.
+enum TraceWalk : uint32_t {
+    kWalkStep4 = 0,
+    kWalkWW4 = 1,
+    kWalkBinary = 2,
+    kWalkComplex = 3,
+    kWalkKd = 4,
+    kWalkRevolution = 5
+};
 
 struct TraceKernel {
     uint32_t walk;
@@ -87,6 +95,8 @@ PVX_TRACE_PART_DECL(8);
 PVX_TRACE_PART_DECL(9);
 PVX_TRACE_PART_DECL(10);
 PVX_TRACE_PART_DECL(11);
+PVX_TRACE_PART_DECL(13);
+PVX_TRACE_PART_DECL(14);
 
 } // namespace pvx
 

@@ -108,6 +108,11 @@ class Scene:
     polygon_points: np.ndarray = None        # (m, 3) float64: the statements' points
     polygon_transform: np.ndarray = None     # (n,) int32: transform index or -1
     mesh_transform: np.ndarray = None        # (len(meshes),) int32: transform index or -1
+    revolution_first: np.ndarray = None      # (n + 1,) uint32: offsets into revolution_points (in points)
+    revolution_points: np.ndarray = None     # (m, 2) float64: lathe / sor points (radius, height)
+    revolution_kind: np.ndarray = None       # (n,) uint8: PVX_LATHE_* or PVX_SOR_CUBIC
+    revolution_flags: np.ndarray = None      # (n,) uint8: PVX_REV_STURM | PVX_REV_OPEN
+    revolution_transform: np.ndarray = None  # (n,) int32: transform index or -1
     camera: dict = None
     lights: list = field(default_factory=list)
     materials: dict = field(default_factory=dict)  # per top-level object arrays
@@ -122,7 +127,8 @@ class Scene:
                "transforms", "csg", "csg_children", "triangle_vertices", "triangle_normals", "cone_apex",
                "cone_base", "cone_radius", "cone_flags", "cone_transform", "disc_center", "disc_normal",
                "disc_radius", "disc_transform", "quadric_coef", "quadric_transform", "polygon_first",
-               "polygon_points", "polygon_transform", "mesh_transform")
+               "polygon_points", "polygon_transform", "mesh_transform", "revolution_first", "revolution_points",
+               "revolution_kind", "revolution_flags", "revolution_transform")
 
     def save(self, path):
         """Write the scene to a compressed .npz (arrays, camera, lights,
@@ -266,6 +272,15 @@ class Scene:
             d.polygon_points, _ = arr(np.asarray(self.polygon_points).reshape(-1, 3), np.float64, C.c_double)
             if self.polygon_transform is not None:
                 d.polygon_transform, _ = arr(self.polygon_transform, np.int32, C.c_int32)
+        if self.revolution_first is not None and len(self.revolution_first) > 1:
+            d.revolution_first, n1 = arr(self.revolution_first, np.uint32, C.c_uint32)
+            d.n_revolutions = n1 - 1
+            d.revolution_points, _ = arr(np.asarray(self.revolution_points).reshape(-1, 2), np.float64, C.c_double)
+            d.revolution_kind, _ = arr(self.revolution_kind, np.uint8, C.c_uint8)
+            if self.revolution_flags is not None:
+                d.revolution_flags, _ = arr(self.revolution_flags, np.uint8, C.c_uint8)
+            if self.revolution_transform is not None:
+                d.revolution_transform, _ = arr(self.revolution_transform, np.int32, C.c_int32)
         if self.csg is not None and len(self.csg):
             c = np.ascontiguousarray(self.csg, np.uint32).reshape(-1, 4)
             ch = np.ascontiguousarray(self.csg_children, np.uint32).reshape(-1, 4)
@@ -698,6 +713,72 @@ def placed_mesh_scene(seed: int = 0xC0FFEE + 8, level: int = 4, width=320, heigh
         materials=dict(pigment=r.uniform(3 * nobj, 0.3, 0.9).reshape(nobj, 3), diffuse=np.full(nobj, 0.7),
                        ambient=np.full(nobj, 0.1), phong=np.full(nobj, 0.4), phong_size=np.full(nobj, 40.0),
                        reflection=np.where(np.arange(nobj) == 2, 0.3, 0.0)),
+        settings=dict(max_trace_level=5, adc_bailout=1.0 / 255.0),
+    )
+
+
+# Profiles (radius, height) of the lathe / sor scene, one per spline kind.
+REVOLUTION_PROFILES = {
+    A.PVX_LATHE_LINEAR: [(0.0, 0.0), (1.0, 0.0), (1.0, 0.2), (0.25, 0.45), (0.2, 1.5), (0.9, 1.9), (1.1, 2.6)],
+    A.PVX_LATHE_QUADRATIC: [(0.0, -0.2), (0.8, 0.0), (1.0, 0.5), (0.6, 1.2), (0.9, 1.8), (1.0, 2.2)],
+    A.PVX_LATHE_CUBIC: [(0.3, -0.5), (0.8, 0.0), (1.1, 0.6), (0.5, 1.4), (0.7, 2.0), (0.9, 2.4)],
+    A.PVX_LATHE_BEZIER: [(0.0, 0.0), (1.2, 0.0), (1.2, 0.8), (0.6, 1.0),
+                         (0.6, 1.0), (0.2, 1.2), (0.8, 2.0), (0.9, 2.2)],
+    A.PVX_SOR_CUBIC: [(0.0, -0.5), (0.9, 0.0), (1.2, 0.6), (0.7, 1.2), (0.5, 1.8), (0.9, 2.3), (1.0, 2.8)],
+}
+
+
+def revolution_scene(seed: int = 0xC0FFEE + 9, n: int = 20, width=320, height=240) -> Scene:
+    """`n` lathes and sors on a 5-wide grid cycling through the five spline
+    kinds (linear, quadratic, cubic and Bezier lathes, the sor's cubic), one
+    in three with `sturm`, every other sor `open`; each placed by a random
+    rotation, a non-uniform scale U[0.5, 1.2] per axis for one in three, and
+    a translation.  Two CSG objects: a cubic lathe cut by a box
+    (intersection) and a closed sor minus a sphere (difference).  A ground
+    plane, one point light; one in five reflective."""
+    r = Rng(seed)
+    kinds = list(REVOLUTION_PROFILES)
+    first, pts, kind, flags, xf_idx, xforms = [0], [], [], [], [], []
+
+    def add(k, fl, t):
+        pts.extend(REVOLUTION_PROFILES[k])
+        first.append(len(pts))
+        kind.append(k)
+        flags.append(fl)
+        xforms.append(_xf_row(t))
+        xf_idx.append(len(xforms) - 1)
+        return len(kind) - 1
+
+    for i in range(n):
+        k = kinds[i % len(kinds)]
+        fl = (A.PVX_REV_STURM if i % 3 == 1 else 0) | (A.PVX_REV_OPEN if k == A.PVX_SOR_CUBIC and i % 2 else 0)
+        t = rotation_transform(r.uniform(3, -35.0, 35.0))
+        if i % 3 == 2:
+            s3 = r.uniform(3, 0.5, 1.2)
+            t = compose((np.diag(list(s3) + [1.0]), np.diag(list(1.0 / s3) + [1.0])), t)
+        t = compose(t, translation_transform([(i % 5) * 3.0 - 6.0, 0.0, (i // 5) * 3.0]))
+        add(k, fl, t)
+    lathe_c = add(A.PVX_LATHE_CUBIC, 0, translation_transform([-4.5, 0.0, -3.0]))
+    sor_c = add(A.PVX_SOR_CUBIC, 0, translation_transform([4.5, 0.0, -3.0]))
+    objects = [[A.PVX_SOR if kind[i] == A.PVX_SOR_CUBIC else A.PVX_LATHE, i, 0, 0] for i in range(n)]
+    objects += [[A.PVX_CSG, 0, 0, 0], [A.PVX_CSG, 1, 0, 0], [A.PVX_PLANE, 0, 0, 0]]
+    csg = np.array([[A.PVX_CSG_INTERSECTION, 0, 2, 0], [A.PVX_CSG_INTERSECTION, 2, 2, 0]], np.uint32)
+    kids = np.array([[A.PVX_LATHE, lathe_c, 0, 0], [A.PVX_BOX, 0, 0, 0],
+                     [A.PVX_SOR, sor_c, 0, 0], [A.PVX_SPHERE, 0, A.PVX_FLAG_INVERTED, 0]], np.uint32)
+    nobj = len(objects)
+    return Scene(
+        name=f"revolutions{n}", objects=np.array(objects, np.uint32), csg=csg, csg_children=kids,
+        revolution_first=np.array(first, np.uint32), revolution_points=np.array(pts, np.float64),
+        revolution_kind=np.array(kind, np.uint8), revolution_flags=np.array(flags, np.uint8),
+        revolution_transform=np.array(xf_idx, np.int32), transforms=np.array(xforms).reshape(-1, 32),
+        box_corner1=np.array([[-5.6, 0.3, -4.2]]), box_corner2=np.array([[-3.2, 3.0, -3.0]]),
+        sphere_center=np.array([[4.9, 2.2, -3.6]]), sphere_radius=np.array([0.9]),
+        plane_normal=np.array([[0.0, 1.0, 0.0]]), plane_distance=np.array([0.6]),
+        camera=look_at_camera((0.0, 6.0, -13.0), (0.0, 0.8, 3.0), width, height),
+        lights=[dict(kind="point", position=np.array([-8.0, 14.0, -10.0]), colour=np.ones(3))],
+        materials=dict(pigment=r.uniform(3 * nobj, 0.3, 0.9).reshape(nobj, 3), diffuse=np.full(nobj, 0.7),
+                       ambient=np.full(nobj, 0.1), phong=np.full(nobj, 0.4), phong_size=np.full(nobj, 40.0),
+                       reflection=np.where(np.arange(nobj) % 5 == 3, 0.3, 0.0)),
         settings=dict(max_trace_level=5, adc_bailout=1.0 / 255.0),
     )
 

@@ -27,7 +27,8 @@ Parse_Object_Mods; parser_expressions.cpp for expressions):
   nearest_count);
 * sphere, box, plane, cylinder, cone, disc, torus (sturm), quadric, triangle,
   smooth_triangle, polygon, mesh2 (vertex_vectors, normal_vectors,
-  face_indices; uv and texture lists read and dropped),
+  face_indices; uv and texture lists read and dropped), lathe (linear,
+  quadratic, cubic and bezier splines), sor (open),
   union / intersection / difference / merge, object { IDENT };
 * object modifiers: pigment / color / rgb / rgbf / rgbt, finish (ambient,
   diffuse, brilliance, phong, phong_size, reflection, emission), interior
@@ -73,8 +74,8 @@ class SdlError(ValueError):
 _TOKEN = re.compile(r"""
     (?P<ws>\s+) | (?P<c1>//[^\n]*) | (?P<c2>/\*.*?\*/) |
     (?P<num>(?:\d+\.\d*|\.\d+|\d+)(?:[eE][-+]?\d+)?) |
-    (?P<str>"[^"\n]*") |
-    (?P<dir>\#[A-Za-z_]+) |
+    (?P<str>"(?:[^"\\\n]|\\.)*") |
+    (?P<dir>\#[ \t]*[A-Za-z_]+) |
     (?P<id>[A-Za-z_][A-Za-z_0-9]*) |
     (?P<sym2><=|>=|!=) |
     (?P<sym>[{}<>,()+\-*/=;.!&|?:\[\]])
@@ -90,7 +91,15 @@ _COLOURS = {
 }
 
 _SHAPES = ("sphere", "box", "plane", "cylinder", "cone", "disc", "torus", "quadric", "triangle",
-           "smooth_triangle", "polygon", "mesh2", "union", "intersection", "difference", "merge", "object")
+           "smooth_triangle", "polygon", "mesh2", "lathe", "sor", "union", "intersection", "difference", "merge",
+           "object")
+# the reference's other shapes: read and dropped with a warning when the
+# reader is asked to skip unsupported shapes, else an error
+_OTHER_SHAPES = ("blob", "superellipsoid", "julia_fractal", "height_field", "text", "prism", "bicubic_patch",
+                 "isosurface", "parametric", "sphere_sweep", "lemon", "ovus", "poly", "polynomial", "cubic",
+                 "quartic", "light_group", "mesh")
+_LATHE_SPLINES = {"linear_spline": A.PVX_LATHE_LINEAR, "quadratic_spline": A.PVX_LATHE_QUADRATIC,
+                  "cubic_spline": A.PVX_LATHE_CUBIC, "bezier_spline": A.PVX_LATHE_BEZIER}
 
 
 def _identity():
@@ -140,6 +149,88 @@ class _Material(dict):
     """A declared texture / pigment / finish / interior: material fields."""
 
 
+class _Function:
+    """A declared user function: parameter names and the body's tokens."""
+
+    def __init__(self, params, start, end):
+        self.params, self.start, self.end = params, start, end
+
+    def copy(self):
+        return self
+
+
+class _ColourMap:
+    """A declared color_map: its entries' colours."""
+
+    def __init__(self, cols):
+        self.cols = cols
+
+    def copy(self):
+        return self
+
+
+class _Xf:
+    """A declared transform: (matrix, inverse)."""
+
+    def __init__(self, tr):
+        self.tr = tr
+
+    def copy(self):
+        return self
+
+
+class _Array:
+    """A declared array: nested lists of values (None: not yet set)."""
+
+    def __init__(self, dims):
+        self.dims = dims
+
+        def make(level):
+            return [make(level + 1) if level + 1 < len(dims) else None for _ in range(dims[level])]
+        self.data = make(0)
+
+    def fill(self, values, ln):
+        def put(dst, src, level):
+            if len(src) > self.dims[level]:
+                raise SdlError(f"line {ln}: too many array elements")
+            for k, v in enumerate(src):
+                if level + 1 < len(self.dims):
+                    put(dst[k], v, level + 1)
+                else:
+                    dst[k] = v
+        put(self.data, values, 0)
+
+    def _cell(self, idx, ln):
+        if len(idx) != len(self.dims) or any(not 0 <= i < n for i, n in zip(idx, self.dims)):
+            raise SdlError(f"line {ln}: array index {idx} out of range {self.dims}")
+        d = self.data
+        for i in idx[:-1]:
+            d = d[i]
+        return d
+
+    def get(self, idx, ln):
+        v = self._cell(idx, ln)[idx[-1]]
+        if v is None:
+            raise SdlError(f"line {ln}: array element {idx} not initialised")
+        return v
+
+    def set(self, idx, value, ln):
+        self._cell(idx, ln)[idx[-1]] = value
+
+    def copy(self):
+        return self
+
+
+class _Opaque:
+    """A declared block pvx does not render (maps, media, sky spheres ...)."""
+
+    def __init__(self, kind):
+        self.kind = kind
+
+    def copy(self):
+        return self
+
+
 class _Light(dict):
     """A declared light_source."""
 
@@ -147,27 +238,59 @@ class _Light(dict):
         return _Light({k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in self.items()})
 
 
+def _promote(a, b):
+    """Operands of + - * /: vectors of different lengths (a colour's five
+    components and a 3-vector) are padded with zeros to the longer one."""
+    if np.ndim(a) and np.ndim(b) and np.shape(a)[0] != np.shape(b)[0]:
+        n = max(np.shape(a)[0], np.shape(b)[0])
+        a = np.concatenate([a, np.zeros(n - np.shape(a)[0])])
+        b = np.concatenate([b, np.zeros(n - np.shape(b)[0])])
+    return a, b
+
+
 def _truth(v):
     return bool(np.any(np.asarray(v) != 0.0))
 
 
+def _tokenize(text: str, tag: str = ""):
+    """Tokens (kind, value, line) of scene text; `tag` prefixes the line
+    numbers of an included file ("colors.inc:12")."""
+    toks = []
+    line = 1
+    pos = 0
+    while pos < len(text):
+        m = _TOKEN.match(text, pos)
+        if not m:
+            raise SdlError(f"line {tag}{line}: cannot read {text[pos:pos + 20]!r}")
+        kind = m.lastgroup
+        val = m.group()
+        if kind == "sym2":
+            kind = "sym"
+        if kind not in ("ws", "c1", "c2"):
+            tok = "#" + val[1:].strip() if kind == "dir" else val
+            toks.append((kind, tok, f"{tag}{line}" if tag else line))
+        line += val.count("\n")
+        pos = m.end()
+    return toks
+
+
+def _read_text(path):
+    """Scene text: UTF-8, else Latin-1 (older scene files)."""
+    with open(path, "rb") as f:
+        raw = f.read()
+    try:
+        return raw.decode("utf-8")
+    except UnicodeDecodeError:
+        return raw.decode("latin-1")
+
+
 class _Parser:
-    def __init__(self, text: str, width: int, height: int):
-        self.toks = []
-        line = 1
-        pos = 0
-        while pos < len(text):
-            m = _TOKEN.match(text, pos)
-            if not m:
-                raise SdlError(f"line {line}: cannot read {text[pos:pos + 20]!r}")
-            kind = m.lastgroup
-            val = m.group()
-            if kind == "sym2":
-                kind = "sym"
-            if kind not in ("ws", "c1", "c2"):
-                self.toks.append((kind, val, line))
-            line += val.count("\n")
-            pos = m.end()
+    def __init__(self, text: str, width: int, height: int, include_dirs=(), skip_unsupported=False):
+        self.toks = _tokenize(text)
+        self.include_dirs = list(include_dirs)  # the scene's folder, then library paths (+L)
+        self.include_depth = 0
+        self.version = 3.8
+        self.skip_unsupported = skip_unsupported
         self.i = 0
         self.width, self.height = width, height
         # identifiers: #declare writes the global map, #local the innermost
@@ -198,6 +321,11 @@ class _Parser:
             self.pump()
         return self.raw()
 
+    def peek_at(self, k):
+        """The raw token k places after the cursor."""
+        j = self.i + k
+        return self.toks[j] if j < len(self.toks) else ("eof", "", 0)
+
     def next(self):
         t = self.peek()
         self.i += 1
@@ -212,7 +340,10 @@ class _Parser:
     def pump(self):
         while True:
             kind, val, ln = self.raw()
-            if kind == "dir":
+            if val == "\x00end_include":
+                self.i += 1
+                self.include_depth -= 1
+            elif kind == "dir":
                 self.i += 1
                 self.directive(val, ln)
             elif kind == "id" and val in self.macros:
@@ -223,14 +354,17 @@ class _Parser:
 
     _OPENERS = ("#while", "#if", "#ifdef", "#ifndef", "#macro", "#switch")
 
-    def skip_block(self, stop_at_else):
-        """Skip tokens to the #end (or, when stop_at_else, an #else / #elseif)
+    def skip_block(self, stop_at_else, stops=("#else", "#elseif")):
+        """Skip tokens to the #end (or, when stop_at_else, one of `stops`)
         closing the current block at this nesting depth; returns the directive
         that stopped the skip (consumed)."""
         depth = 0
         while self.i < len(self.toks):
             kind, val, ln = self.toks[self.i]
             self.i += 1
+            if val == "\x00end_include":
+                self.include_depth -= 1
+                continue
             if kind != "dir":
                 continue
             if val in self._OPENERS:
@@ -239,9 +373,40 @@ class _Parser:
                 if depth == 0:
                     return val
                 depth -= 1
-            elif depth == 0 and stop_at_else and val in ("#else", "#elseif"):
+            elif depth == 0 and stop_at_else and val in stops:
                 return val
         raise SdlError("unterminated #while / #if / #macro block")
+
+    def cond_paren_value(self):
+        return self.quiet(self.cond)
+
+    def switch_to_case(self, v, ln):
+        """Skip the clauses of a #switch until one holds (#case: |v - c| <
+        EPSILON, #range: lo <= v <= hi) or #else; then run it."""
+        while True:
+            stop = self.skip_block(True, ("#case", "#range", "#else"))
+            if stop == "#end":
+                return
+            if stop == "#else":
+                self.ctrl.append(("switch", v))
+                return
+            if stop == "#case":
+                ok = abs(float(self.quiet(self.cond)) - v) < 1e-10
+            else:
+                lo, hi = self.quiet(self.range_pair)
+                ok = lo <= v <= hi
+            if ok:
+                self.ctrl.append(("switch", v))
+                return
+
+    def range_pair(self):
+        """#range's ( low, high )."""
+        self.expect("(")
+        lo = float(self.cond())
+        self.expect(",")
+        hi = float(self.cond())
+        self.expect(")")
+        return lo, hi
 
     def cond_paren(self):
         """A directive's condition: read without processing the directives
@@ -257,12 +422,30 @@ class _Parser:
 
     def directive(self, val, ln):
         if val in ("#declare", "#local"):
+            if self.raw()[1] == "deprecated":  # deprecated [once] "message" NAME = ...
+                self.i += 1
+                if self.raw()[1] == "once":
+                    self.i += 1
+                if self.raw()[0] == "str":
+                    self.i += 1
             kind, name, l2 = self.raw()
             if kind != "id":
                 raise SdlError(f"line {ln}: expected an identifier after {val}")
             self.i += 1
+            idx = []
+            while self.raw()[1] == "[":  # an array element: #declare A[i][j] = ...
+                self.i += 1
+                idx.append(int(float(self.quiet(self.cond))))
+                self.quiet(lambda: self.expect("]"))
             self.expect("=")
             value = self.decl_value(ln)
+            if idx:
+                arr = self.decl.get(name)
+                if not isinstance(arr, _Array):
+                    raise SdlError(f"line {ln}: {name!r} is not an array")
+                arr.set(idx, value, ln)
+                self.accept_raw(";")
+                return
             if val == "#local":
                 self.decl[name] = value
             else:  # an existing local is changed, else the global
@@ -272,7 +455,7 @@ class _Parser:
                         break
                 else:
                     self.decl.maps[-1][name] = value
-            self.accept(";")
+            self.accept_raw(";")
         elif val == "#undef":
             name = self.raw()[1]
             self.i += 1
@@ -300,8 +483,20 @@ class _Parser:
                 ok = True if stop == "#else" else self.cond_paren()
             self.ctrl.append(("if", None))
         elif val in ("#else", "#elseif"):  # the branch taken ends here
-            if not self.ctrl or self.ctrl[-1][0] != "if":
+            if not self.ctrl or self.ctrl[-1][0] not in ("if", "switch"):
                 raise SdlError(f"line {ln}: {val} without #if")
+            self.ctrl.pop()
+            self.skip_block(False)
+        elif val == "#switch":  # the first #case / #range that holds, falling through to #break
+            v = float(self.cond_paren_value())
+            self.switch_to_case(v, ln)
+        elif val in ("#case", "#range"):  # met while a clause runs: fall through
+            if not self.ctrl or self.ctrl[-1][0] != "switch":
+                raise SdlError(f"line {ln}: {val} without #switch")
+            self.quiet(self.cond if val == "#case" else self.range_pair)
+        elif val == "#break":
+            if not self.ctrl or self.ctrl[-1][0] != "switch":
+                raise SdlError(f"line {ln}: #break outside a #switch")
             self.ctrl.pop()
             self.skip_block(False)
         elif val == "#end":
@@ -325,27 +520,66 @@ class _Parser:
             self.macros[name] = (params, self.i)
             self.skip_block(False)
         elif val == "#include":
-            f = self.next()[1].strip('"')
-            if f != "colors.inc":
-                self.warn(f"line {ln}: #include {f!r} skipped (colors.inc is built in)")
+            f = self.quiet(self.next)[1].strip('"')
+            self.include(f, ln)
         elif val == "#version":
-            self.quiet(self.cond)
-            self.accept(";")
+            self.version = float(self.quiet(self.cond))
+            self.accept_raw(";")
         elif val == "#default":
-            holder = _Obj("material", {}, ln)
-            self.expect("{")
-            while not self.accept("}"):
-                k = self.next()[1]
-                if k not in ("texture", "pigment", "finish", "normal", "interior"):
-                    raise SdlError(f"line {ln}: unsupported #default block {k!r}")
-                self.material(holder, k)
-            self.defaults.update(holder.mat)
+            self.default_block(ln)
         elif val in ("#debug", "#warning"):
             self.next()
         elif val == "#error":
             raise SdlError(f"line {ln}: #error {self.next()[1]}")
         else:
             raise SdlError(f"line {ln}: unsupported directive {val}")
+
+    def include(self, name, ln):
+        """#include: the file's tokens are read in place of the directive
+        (the scene's folder first, then the library paths).  A file that is
+        not found is skipped with a warning (the common colours of colors.inc
+        are built in)."""
+        import os
+        for d in self.include_dirs:
+            path = os.path.join(d, name)
+            if os.path.isfile(path):
+                break
+        else:
+            self.warn(f"line {ln}: #include {name!r} not found, skipped")
+            return
+        if self.include_depth >= 32:
+            raise SdlError(f"line {ln}: #include nested too deeply")
+        new = _tokenize(_read_text(path), os.path.basename(name) + ":")
+        new.append(("sym", "\x00end_include", ln))
+        pos, k = self.i, len(new)
+        self.toks[pos:pos] = new
+        # indices kept by open blocks and macros move with the tokens after them
+        self.ctrl = [(w, (d + k if isinstance(d, int) and d >= pos else d)) for w, d in self.ctrl]
+        self.macros = {m: (p, b + k if b >= pos else b) for m, (p, b) in self.macros.items()}
+        self.include_depth += 1
+
+    def default_block(self, ln):
+        """#default { texture / pigment / finish / normal / interior }."""
+        holder = _Obj("material", {}, ln)
+        self.expect("{")
+        while not self.accept("}"):
+            k = self.next()[1]
+            if k not in ("texture", "pigment", "finish", "normal", "interior"):
+                raise SdlError(f"line {ln}: unsupported #default block {k!r}")
+            self.material(holder, k)
+        self.defaults.update(holder.mat)
+
+    def texture_layers(self, ln, raw=False):
+        """Further texture blocks right after a texture (a layered texture):
+        read and dropped with a warning; the bottom layer is kept.  `raw`:
+        look ahead without running directives (after a #declare's value)."""
+        dropped = False
+        while (self.raw() if raw else self.peek())[1] == "texture":
+            self.next()
+            self.material(_Obj("material", {}, ln), "texture")
+            dropped = True
+        if dropped:
+            self.warn(f"line {ln}: layered texture reduced to its bottom layer")
 
     def call_macro(self, name, ln):
         params, body = self.macros[name]
@@ -363,23 +597,146 @@ class _Parser:
     def decl_value(self, ln):
         """The right-hand side of #declare / #local, or a macro argument."""
         kind, val, _ = self.peek()
+        if val == "function":
+            return self.function_decl(ln)
+        if val == "array":
+            return self.array_decl(ln)
+        if kind == "id" and isinstance(self.decl.get(val), _Array) and self.peek_at(1)[1] == "[":
+            self.next()
+            v = self.array_element(self.decl[val], ln)
+            return v.copy() if isinstance(v, _Obj) else v
+        if val in _OTHER_SHAPES:
+            return self.shape()
+        if val in ("color_map", "colour_map"):
+            self.next()
+            return _ColourMap(self.colour_map_entries())
+        if val in self._OPAQUE_BLOCKS:
+            self.next()
+            self.skip_braces()
+            return _Opaque(val)
         if val in _SHAPES:
             return self.shape()
         if val == "light_source":
             self.next()
             return _Light(self.light_block())
+        if val == "transform":
+            self.next()
+            if self.peek()[0] == "id" and isinstance(self.decl.get(self.peek()[1]), _Xf):
+                return self.decl[self.next()[1]]
+            return _Xf(self.transform_block(ln))
+        if val == "material":
+            holder = _Obj("material", {}, ln)
+            self.next()
+            self.expect("{")
+            while not self.accept("}"):
+                k = self.next()[1]
+                if k in ("texture", "pigment", "finish", "normal", "interior"):
+                    self.material(holder, k)
+                elif isinstance(self.decl.get(k), _Material):
+                    holder.mat.update(self.decl[k])
+                elif self.transform_mod(k, ln) is None:
+                    raise SdlError(f"line {ln}: unsupported material keyword {k!r}")
+            return _Material(holder.mat)
         if val in ("texture", "pigment", "finish", "interior", "normal"):
             holder = _Obj("material", {}, ln)
             self.material(holder, self.next()[1])
+            if val == "texture":
+                self.texture_layers(ln, raw=True)
             return _Material(holder.mat)
         if val in ("color", "colour", "rgb", "rgbf", "rgbt", "rgbft"):
             rgb, f, t = self.quiet(self.colour)
             return np.concatenate([np.asarray(rgb, np.float64), [f, t]])
-        if kind == "id" and isinstance(self.decl.get(val), (_Obj, _Material, _Light)):
+        if kind == "id" and isinstance(self.decl.get(val), (_Obj, _Material, _Light, _Function, _ColourMap, _Opaque, _Xf, _Array)) and \
+                not (isinstance(self.decl.get(val), _Function) and self.peek_at(1)[1] == "("):
             self.next()
             v = self.decl[val]
             return v.copy() if isinstance(v, _Obj) else v
         return self.quiet(self.cond)
+
+    def function_decl(self, ln):
+        """function [(params)] { expression }: kept as its parameter names and
+        the body's token range, evaluated when called (parameters x, y, z
+        when none are listed).  Pattern and object functions are kept but
+        cannot be called."""
+        self.next()
+        params = ["x", "y", "z"]
+        if self.accept("("):
+            params = []
+            while not self.accept(")"):
+                params.append(self.raw()[1])
+                self.i += 1
+                self.accept(",")
+        if self.peek()[1] != "{":
+            raise SdlError(f"line {ln}: expected a function body")
+        start = self.i + 1
+        self.skip_braces()
+        return _Function(params, start, self.i - 1)
+
+    def array_decl(self, ln):
+        """array[n]...[m] (optionally `mixed`) with an optional { ... }
+        initializer of values, objects or nested lists."""
+        self.next()
+        self.accept("mixed")
+        dims = []
+        while self.accept("["):
+            dims.append(int(float(self.cond())))
+            self.expect("]")
+        if not dims:
+            raise SdlError(f"line {ln}: array without a size")
+        arr = _Array(dims)
+        if self.peek()[1] == "{":
+            def read(level):
+                self.expect("{")
+                out = []
+                while not self.accept("}"):
+                    out.append(read(level + 1) if level + 1 < len(dims) else self.decl_value(ln))
+                    self.accept(",")
+                return out
+            arr.fill(read(0), ln)
+        return arr
+
+    def array_element(self, arr, ln):
+        """A[i][j] after the array's name."""
+        idx = []
+        while self.accept("["):
+            idx.append(int(float(self.cond())))
+            self.expect("]")
+        return arr.get(idx, ln)
+
+    def call_function(self, fn, args, ln):
+        if len(args) != len(fn.params):
+            raise SdlError(f"line {ln}: the function takes {len(fn.params)} arguments")
+        saved_i, saved_decl = self.i, self.decl
+        self.decl = self.decl.new_child(dict(zip(fn.params, args)))
+        self.i = fn.start
+        try:
+            v = self.quiet(self.cond)
+            if self.i != fn.end:
+                raise SdlError(f"line {ln}: function body is not a plain expression (pattern or object)")
+        finally:
+            self.i, self.decl = saved_i, saved_decl
+        return v
+
+    _OPAQUE_BLOCKS = ("pigment_map", "normal_map", "texture_map", "slope_map", "density_map", "material_map",
+                      "media", "density", "fog", "rainbow", "sky_sphere", "photons", "camera")
+
+    def colour_map_entries(self):
+        """The colours of a color_map { [v colour] ... } (or of a declared one)."""
+        cols = []
+        self.expect("{")
+        while not self.accept("}"):
+            if self.accept("["):
+                self.flt()
+                self.accept(",")
+                if self.peek()[1] != "]":
+                    cols.append(self.colour())
+                while not self.accept("]"):
+                    self.next()
+            else:
+                t = self.next()
+                if t[0] == "id" and isinstance(self.decl.get(t[1]), _ColourMap):
+                    cols.extend(self.decl[t[1]].cols)
+        return cols
 
     def line(self):
         return self.peek()[2]
@@ -394,6 +751,15 @@ class _Parser:
 
     def accept(self, val):
         if self.peek()[1] == val:
+            self.i += 1
+            return True
+        return False
+
+    def accept_raw(self, val):
+        """accept() without running the directives that may follow: a
+        directive's own trailing token (the pump loop then carries on with
+        the next directive instead of recursing into it)."""
+        if self.raw()[1] == val:
             self.i += 1
             return True
         return False
@@ -441,6 +807,7 @@ class _Parser:
         while self.peek()[1] in ("+", "-"):
             op = self.next()[1]
             w = self.term()
+            v, w = _promote(v, w)
             v = v + w if op == "+" else v - w
         return v
 
@@ -449,6 +816,7 @@ class _Parser:
         while self.peek()[1] in ("*", "/"):
             op = self.next()[1]
             w = self.factor()
+            v, w = _promote(v, w)
             v = v * w if op == "*" else v / w
         return v
 
@@ -456,7 +824,7 @@ class _Parser:
               "radians": np.radians, "degrees": np.degrees, "exp": np.exp, "log": np.log10, "ln": np.log,
               "asin": np.arcsin, "acos": np.arccos, "atan": np.arctan, "floor": np.floor, "ceil": np.ceil,
               "vlength": np.linalg.norm}
-    _FUNCS2 = ("int", "mod", "min", "max", "rand", "seed", "atan2", "vnormalize", "vdot", "vcross", "select",
+    _FUNCS2 = ("int", "mod", "min", "max", "rand", "seed", "atan2", "vnormalize", "vdot", "vcross", "select", "vrotate",
                "div")
 
     def func2(self, name, a, ln):
@@ -477,6 +845,9 @@ class _Parser:
             return v / np.linalg.norm(v)
         if name == "vdot":
             return float(np.dot(np.asarray(a[0])[:3], np.asarray(a[1])[:3]))
+        if name == "vrotate":  # rotate a point by <x, y, z> degrees, as the rotate modifier
+            m = scenes.rotation_transform(np.asarray(a[1], np.float64)[:3])[0]
+            return np.asarray(a[0], np.float64)[:3] @ m[:3, :3]
         if name == "vcross":
             return np.cross(np.asarray(a[0])[:3], np.asarray(a[1])[:3])
         if name == "select":
@@ -505,7 +876,9 @@ class _Parser:
             v = float(np.asarray(v)[idx])
         return v
 
-    _CONST = {"true": 1.0, "yes": 1.0, "on": 1.0, "false": 0.0, "no": 0.0, "off": 0.0, "clock": 0.0}
+    _CONST = {"true": 1.0, "yes": 1.0, "on": 1.0, "false": 0.0, "no": 0.0, "off": 0.0, "clock": 0.0,
+              "clock_delta": 1.0, "clock_on": 0.0, "frame_number": 0.0, "initial_frame": 1.0, "final_frame": 1.0,
+              "initial_clock": 0.0, "final_clock": 1.0}
 
     def primary(self):
         kind, val, ln = self.peek()
@@ -537,12 +910,37 @@ class _Parser:
             return np.array([float(c) for c in comps])
         if kind == "id":
             self.next()
-            if val in ("x", "y", "z"):
+            if isinstance(self.decl.get(val), _Function) and self.peek()[1] == "(":
+                self.next()
+                args = [] if self.peek()[1] == ")" else [self.cond()]
+                while self.accept(","):
+                    args.append(self.cond())
+                self.expect(")")
+                return self.call_function(self.decl[val], args, ln)
+            if isinstance(self.decl.get(val), _Array):
+                v = self.array_element(self.decl[val], ln)
+                if isinstance(v, (_Obj, _Material, _Light)):
+                    raise SdlError(f"line {ln}: array element of {val!r} is not a value")
+                return v
+            if val in ("dimensions", "dimension_size"):
+                self.expect("(")
+                a = self.decl.get(self.next()[1])
+                if not isinstance(a, _Array):
+                    raise SdlError(f"line {ln}: {val} of something that is not an array")
+                d = 1
+                if val == "dimension_size":
+                    self.expect(",")
+                    d = int(float(self.cond()))
+                self.expect(")")
+                return float(len(a.dims)) if val == "dimensions" else float(a.dims[d - 1])
+            if val in ("x", "y", "z") and val not in self.decl:
                 return np.eye(3)["xyz".index(val)].copy()
             if val == "pi":
                 return math.pi
             if val in self._CONST and val not in self.decl:
                 return self._CONST[val]
+            if val == "version":
+                return self.version
             if val == "image_width":
                 return float(self.width)
             if val == "image_height":
@@ -569,7 +967,7 @@ class _Parser:
             if val in ("rgb", "color", "colour"):
                 self.i -= 1
                 return self.colour()
-            if val in self.decl and not isinstance(self.decl[val], (_Obj, _Material, _Light)):
+            if val in self.decl and not isinstance(self.decl[val], (_Obj, _Material, _Light, _Function, _ColourMap, _Opaque, _Xf, _Array)):
                 return self.decl[val]
             if val in _COLOURS:
                 return np.array(list(_COLOURS[val]) + [0.0, 0.0], np.float64)
@@ -577,10 +975,14 @@ class _Parser:
         raise SdlError(f"line {ln}: expected a value, found {val!r}")
 
     def vector(self):
+        """A 3-vector; a float stands for all three components and a
+        2-vector gets z = 0 (the language's vector promotion)."""
         v = self.expr()
         if np.isscalar(v) or np.ndim(v) == 0:
             return np.full(3, float(v))
         v = np.asarray(v, np.float64)
+        if v.shape[0] == 2:
+            return np.array([v[0], v[1], 0.0])
         if v.shape[0] < 3:
             raise SdlError(f"line {self.line()}: expected a 3-vector")
         return v[:3]
@@ -592,33 +994,47 @@ class _Parser:
         return float(v)
 
     def colour(self):
-        """color [rgb|rgbf|rgbt|rgbft] <..>, or a colour expression (a named or
-        declared colour, possibly scaled): (rgb, filter, transmit).  Colours
-        are kept as 5-vectors r, g, b, filter, transmit."""
+        """color [rgb|rgbf|rgbt|rgbft] <..>, a colour expression (a named or
+        declared colour, possibly scaled), or `red r green g ...`, each
+        optionally followed by component keywords (Parse_Colour): (rgb,
+        filter, transmit).  Colours are kept as 5-vectors r, g, b, filter,
+        transmit."""
         self.accept("color") or self.accept("colour")
         kind = self.peek()[1]
-        if kind in ("rgb", "rgbf", "rgbt", "rgbft"):
+        v5 = np.zeros(5)
+        if kind in self._COMPONENTS:  # color red 1 green 0.5 ... (the older form)
+            pass
+        elif kind in ("rgb", "rgbf", "rgbt", "rgbft"):
             self.next()
             v = self.expr()
             v = np.full(3, float(v)) if np.ndim(v) == 0 else np.asarray(v, np.float64)
             n = {"rgb": 3, "rgbf": 4, "rgbt": 4, "rgbft": 5}[kind]
             if v.shape[0] < n:
                 v = np.concatenate([v, np.zeros(n - v.shape[0])])
-            filt = trans = 0.0
+            v5[:3] = v[:3]
             if kind == "rgbf":
-                filt = v[3]
+                v5[3] = v[3]
             elif kind == "rgbt":
-                trans = v[3]
+                v5[4] = v[3]
             elif kind == "rgbft":
-                filt, trans = v[3], v[4]
-            return np.asarray(v[:3]), float(filt), float(trans)
-        v = self.expr()
-        v = np.full(3, float(v)) if np.ndim(v) == 0 else np.asarray(v, np.float64)
-        if v.shape[0] < 3:
-            raise SdlError(f"line {self.line()}: expected a colour")
-        f = float(v[3]) if v.shape[0] > 3 else 0.0
-        t = float(v[4]) if v.shape[0] > 4 else 0.0
-        return np.asarray(v[:3]), f, t
+                v5[3:5] = v[3:5]
+        else:
+            v = self.expr()
+            v = np.full(3, float(v)) if np.ndim(v) == 0 else np.asarray(v, np.float64)
+            if v.shape[0] < 3:
+                raise SdlError(f"line {self.line()}: expected a colour")
+            v5[:min(5, v.shape[0])] = v[:5]
+        self.colour_components(v5)
+        return v5[:3].copy(), float(v5[3]), float(v5[4])
+
+    _COMPONENTS = {"red": 0, "green": 1, "blue": 2, "filter": 3, "transmit": 4}
+
+    def colour_components(self, v):
+        """Trailing `red 1 green 0.5 ... filter f transmit t` components
+        (Parse_Colour), written into the 5-vector v."""
+        while self.peek()[1] in self._COMPONENTS:
+            k = self._COMPONENTS[self.next()[1]]
+            v[k] = self.flt()
 
     def skip_braces(self):
         """Skip a { ... } block (the opening brace next)."""
@@ -659,8 +1075,17 @@ class _Parser:
             elif val == "global_settings":
                 self.next()
                 self.global_settings()
-            elif val in _SHAPES:
-                self.objects.append(self.shape())
+            elif val in _SHAPES or val in _OTHER_SHAPES:
+                o = self.shape()
+                if o.kind != "skipped":
+                    self.objects.append(o)
+            elif val == "default":  # the #default block written without '#'
+                self.next()
+                self.default_block(ln)
+            elif val in ("sky_sphere", "fog", "rainbow", "media"):
+                self.next()
+                self.skip_braces()
+                self.warn(f"line {ln}: {val} ignored")
             elif val == ";":
                 self.next()
             else:
@@ -681,7 +1106,30 @@ class _Parser:
             m = np.eye(4)
             m[:, :3] = v.reshape(4, 3)
             return m, np.linalg.inv(m)
+        if val == "transform":  # transform IDENT | transform { ... } (Parse_Transform)
+            if self.peek()[0] == "id" and isinstance(self.decl.get(self.peek()[1]), _Xf):
+                return self.decl[self.next()[1]].tr
+            return self.transform_block(ln)
         return None
+
+    def transform_block(self, ln):
+        """{ translate / rotate / scale / matrix / transform / inverse ... }
+        composed in order."""
+        tr = _identity()
+        self.expect("{")
+        while not self.accept("}"):
+            k = self.next()[1]
+            if k == "inverse":
+                tr = (tr[1], tr[0])
+                continue
+            if isinstance(self.decl.get(k), _Xf):
+                t = self.decl[k].tr
+            else:
+                t = self.transform_mod(k, ln)
+            if t is None:
+                raise SdlError(f"line {ln}: unsupported transform keyword {k!r}")
+            tr = _compose(tr, t)
+        return tr
 
     def camera_block(self):
         """Parse_Camera for language version 3.5 and later (parser.cpp:1335-1830):
@@ -803,10 +1251,10 @@ class _Parser:
             elif val == "point_at":
                 self.next()
                 self.vector()
-            elif val == "looks_like":
+            elif val in ("looks_like", "photons", "projected_through"):
                 self.next()
                 self.skip_braces()
-                self.warn(f"line {ln}: light_source looks_like ignored")
+                self.warn(f"line {ln}: light_source {val} ignored")
             else:
                 raise SdlError(f"line {ln}: unsupported light_source keyword {val!r}")
         return light
@@ -856,12 +1304,24 @@ class _Parser:
     # -- objects ------------------------------------------------------------------------------
     def shape(self) -> _Obj:
         _, kw, ln = self.next()
+        if kw in _OTHER_SHAPES:
+            if not self.skip_unsupported:
+                raise SdlError(f"line {ln}: unsupported shape {kw!r}")
+            self.skip_braces()
+            self.warn(f"line {ln}: {kw} not supported, dropped")
+            return _Obj("skipped", {}, ln)
         self.expect("{")
         if kw == "object":
-            name = self.next()
-            if name[1] not in self.decl or not isinstance(self.decl[name[1]], _Obj):
-                raise SdlError(f"line {ln}: {name[1]!r} is not a declared object")
-            o = self.decl[name[1]].copy()
+            if self.peek()[1] in _SHAPES or self.peek()[1] in _OTHER_SHAPES:
+                o = self.shape()  # object { shape modifiers }
+            else:
+                name = self.next()
+                v = self.decl.get(name[1])
+                if isinstance(v, _Array):
+                    v = self.array_element(v, ln)
+                if not isinstance(v, _Obj):
+                    raise SdlError(f"line {ln}: {name[1]!r} is not a declared object")
+                o = v.copy()
         elif kw == "sphere":
             c = self.vector()
             self.comma()
@@ -943,14 +1403,45 @@ class _Parser:
             o = _Obj("polygon", dict(p=np.array(pts)), ln)
         elif kw == "mesh2":
             o = self.mesh2(ln)
+        elif kw in ("lathe", "sor"):  # Parse_Lathe (parser.cpp:2812-2995), Parse_Sor (5628-5714)
+            kind = A.PVX_SOR_CUBIC if kw == "sor" else A.PVX_LATHE_LINEAR
+            if kw == "lathe" and self.peek()[1] in _LATHE_SPLINES:
+                kind = _LATHE_SPLINES[self.next()[1]]
+            cnt = int(self.flt())
+            self.comma()
+            pts = []
+            for _ in range(cnt):
+                pts.append(self.uv())
+                self.comma()
+            o = _Obj(kw, dict(p=np.array(pts, np.float64).reshape(-1, 2), kind=kind, flags=0), ln)
         else:  # CSG
             o = _Obj("csg", dict(op=kw), ln)
-            while self.peek()[1] in _SHAPES:
-                o.children.append(self.shape())
+            dropped = False
+            while self.peek()[1] in _SHAPES or self.peek()[1] in _OTHER_SHAPES:
+                c = self.shape()
+                if c.kind == "skipped":
+                    dropped = True
+                    if kw == "difference" and not o.children:  # the base shape is gone
+                        o.data["lost_base"] = True
+                else:
+                    o.children.append(c)
+            if dropped and (not o.children or o.data.get("lost_base")):
+                self.mods(o)
+                return _Obj("skipped", {}, ln)
             if not o.children:
                 raise SdlError(f"line {ln}: {kw} without objects")
         self.mods(o)
         return o
+
+    def uv(self):
+        """Parse_UV_Vect: a 2-vector (a float stands for both components)."""
+        v = self.expr()
+        if np.ndim(v) == 0:
+            return np.array([float(v), float(v)])
+        v = np.asarray(v, np.float64)
+        if v.shape[0] < 2:
+            raise SdlError(f"line {self.line()}: expected a 2-vector")
+        return v[:2]
 
     def starts_float(self):
         """Whether the next token can begin a float expression (CASE_FLOAT_UNGET)."""
@@ -1018,6 +1509,15 @@ class _Parser:
         return _Obj("mesh", dict(v=verts, n=norms, f=faces), ln)
 
     def mods(self, o: _Obj):
+        if o.kind == "skipped":  # a dropped shape: its modifiers are not read
+            depth = 0
+            while True:
+                t = self.next()
+                if t[0] == "eof":
+                    raise SdlError(f"line {o.line}: unterminated object")
+                if t[1] == "}" and depth == 0:
+                    return
+                depth += (t[1] == "{") - (t[1] == "}")
         while not self.accept("}"):
             kind, val, ln = self.next()
             t = self.transform_mod(val, ln)
@@ -1028,8 +1528,12 @@ class _Parser:
             elif val in ("open", "sturm"):
                 if val == "open" and o.kind == "cone":
                     o.data["flags"] |= A.PVX_CONE_OPEN
+                elif val == "open" and o.kind == "sor":
+                    o.data["flags"] |= A.PVX_REV_OPEN
                 elif val == "sturm" and o.kind == "torus":
                     o.data["sturm"] = 1
+                elif val == "sturm" and o.kind in ("lathe", "sor"):
+                    o.data["flags"] |= A.PVX_REV_STURM
                 elif val == "sturm" and o.kind in ("csg", "mesh", "sphere", "box", "plane", "triangle"):
                     pass  # no polynomial to solve there
                 else:
@@ -1040,6 +1544,8 @@ class _Parser:
                             "no_radiosity": A.PVX_FLAG_NO_RADIOSITY}[val]
             elif val in ("pigment", "finish", "interior", "texture", "normal"):
                 self.material(o, val)
+                if val == "texture":
+                    self.texture_layers(ln)
             elif val in ("color", "colour", "rgb"):
                 self.i -= 1
                 self.set_colour(o)
@@ -1051,8 +1557,26 @@ class _Parser:
             elif val in ("photons", "interior_texture", "radiosity"):
                 self.skip_braces()
                 self.warn(f"line {ln}: object {val} ignored")
-            elif val in ("bounded_by", "clipped_by"):
-                raise SdlError(f"line {ln}: {val} is not supported (it changes which hits exist)")
+            elif val == "bounded_by":  # a manual bound: the object's own box bounds it here
+                self.skip_braces()
+                self.warn(f"line {ln}: bounded_by ignored")
+            elif val == "clipped_by":
+                if not self.skip_unsupported:
+                    raise SdlError(f"line {ln}: clipped_by is not supported (it changes which hits exist)")
+                self.skip_braces()
+                self.warn(f"line {ln}: clipped_by ignored (the object is not clipped)")
+            elif val == "material":  # material { texture { } interior { } }
+                self.expect("{")
+                while not self.accept("}"):
+                    k = self.next()[1]
+                    if k in ("texture", "pigment", "finish", "normal", "interior"):
+                        self.material(o, k)
+                    elif isinstance(self.decl.get(k), _Material):
+                        o.mat.update(self.decl[k])
+                    else:
+                        t = self.transform_mod(k, ln)
+                        if t is None:
+                            raise SdlError(f"line {ln}: unsupported material keyword {k!r}")
             else:
                 raise SdlError(f"line {ln}: unsupported object modifier {val!r}")
 
@@ -1088,20 +1612,12 @@ class _Parser:
             kind, val, l2 = self.peek()
             if val == "}" and depth == 0:
                 break
-            if val in ("color_map", "colour_map", "pigment_map"):
+            if val in ("color_map", "colour_map"):
                 self.next()
-                self.expect("{")
-                while not self.accept("}"):
-                    if self.accept("["):
-                        self.flt()
-                        self.accept(",")
-                        if self.peek()[1] in ("color", "colour", "rgb", "rgbf", "rgbt", "rgbft") or \
-                                self.peek()[0] == "id":
-                            cols.append(self.colour())
-                        while not self.accept("]"):
-                            self.next()
-                    else:
-                        self.next()
+                if self.peek()[0] == "id" and isinstance(self.decl.get(self.peek()[1]), _ColourMap):
+                    cols.extend(self.decl[self.next()[1]].cols)
+                else:
+                    cols.extend(self.colour_map_entries())
                 continue
             self.next()
             depth += (val == "{") - (val == "}")
@@ -1132,8 +1648,8 @@ class _Parser:
             elif block == "pigment" and (val in ("color", "colour", "rgb", "rgbf", "rgbt", "rgbft")
                                          or val in _COLOURS or val in self.decl or val == "<"):
                 self.set_colour(o)
-            elif block == "pigment" and val in ("translate", "rotate", "scale", "matrix", "warp", "quick_color",
-                                                "quick_colour", "uv_mapping"):
+            elif val in ("translate", "rotate", "scale", "matrix", "warp", "quick_color", "quick_colour",
+                         "uv_mapping"):
                 self.next()
                 if val == "warp":
                     self.skip_braces()
@@ -1192,8 +1708,17 @@ class _Parser:
                 else:
                     self.cond()
                 self.warn(f"line {ln}: interior {val} ignored")
-            else:
-                raise SdlError(f"line {ln}: unsupported {block} keyword {val!r}")
+            else:  # the rest of this block is not read (a texture feature pvx does not render)
+                depth = 0
+                while True:
+                    t = self.next()
+                    if t[0] == "eof":
+                        raise SdlError(f"line {ln}: unterminated {block} block")
+                    if t[1] == "}" and depth == 0:
+                        break
+                    depth += (t[1] == "{") - (t[1] == "}")
+                self.warn(f"line {ln}: {block} keyword {val!r} and the rest of its block ignored")
+                return
 
     def transform(self, o, tr):
         if o.kind == "csg":
@@ -1205,7 +1730,15 @@ class _Parser:
             lin = m[:3, :3]
             s = np.linalg.norm(lin[0])
             if not np.allclose(lin @ lin.T, s * s * np.eye(3), rtol=1e-12, atol=1e-12):
-                raise SdlError(f"line {o.line}: a sphere takes only translate, rotate and uniform scale")
+                # an ellipsoid (Sphere::Transform's Do_Ellipsoid): read as the
+                # same surface in quadric form |p - c|^2 - r^2 = 0, transformed
+                c, r = o.data["c"], o.data["r"]
+                o.kind = "quadric"
+                o.data = dict(coef=np.array([1.0, 1.0, 1.0, 0.0, 0.0, 0.0, -2 * c[0], -2 * c[1], -2 * c[2],
+                                             float(c @ c) - r * r]))
+                o.tr = tr
+                self.warn(f"line {o.line}: sphere with a non-uniform scale read as a quadric")
+                return
             o.data["c"] = o.data["c"] @ lin + m[3, :3]
             o.data["r"] = o.data["r"] * s
             return
@@ -1233,6 +1766,7 @@ class _Parser:
                       tor_tr=[], tor_s=[], xf=[], cone_a=[], cone_b=[], cone_r=[], cone_f=[], cone_t=[],
                       disc_c=[], disc_n=[], disc_r=[], disc_t=[], q=[], q_t=[], tri_p=[], tri_n=[], meshes=[],
                       poly_first=[0], poly_pts=[], poly_t=[], mesh_t=[],
+                      rev_first=[0], rev_pts=[], rev_kind=[], rev_flags=[], rev_t=[],
                       csg=[], kids=[])
 
         def xf_index(o):
@@ -1276,6 +1810,13 @@ class _Parser:
                 arrays["disc_r"].append(dd["r"])
                 arrays["disc_t"].append(xf_index(o))
                 return [A.PVX_DISC, len(arrays["disc_c"]) - 1, f, 0]
+            if k in ("lathe", "sor"):
+                arrays["rev_pts"].extend(dd["p"])
+                arrays["rev_first"].append(len(arrays["rev_pts"]))
+                arrays["rev_kind"].append(dd["kind"])
+                arrays["rev_flags"].append(dd["flags"])
+                arrays["rev_t"].append(xf_index(o))
+                return [A.PVX_SOR if k == "sor" else A.PVX_LATHE, len(arrays["rev_t"]) - 1, f, 0]
             if k == "quadric":
                 arrays["q"].append(dd["coef"])
                 arrays["q_t"].append(xf_index(o))
@@ -1405,14 +1946,23 @@ class _Parser:
             polygon_first=np.array(arrays["poly_first"], np.uint32) if arrays["poly_t"] else None,
             polygon_points=arr("poly_pts", (-1, 3)), polygon_transform=arr("poly_t", (-1,), np.int32),
             mesh_transform=arr("mesh_t", (-1,), np.int32),
+            revolution_first=np.array(arrays["rev_first"], np.uint32) if arrays["rev_t"] else None,
+            revolution_points=arr("rev_pts", (-1, 2)), revolution_kind=arr("rev_kind", (-1,), np.uint8),
+            revolution_flags=arr("rev_flags", (-1,), np.uint8), revolution_transform=arr("rev_t", (-1,), np.int32),
             camera=self.camera, lights=self.lights, materials=materials, settings=settings)
 
 
-def parse(text: str, width: int = 320, height: int = 240) -> scenes.Scene:
-    """Read scene-description text into a Scene rendered at width x height."""
-    return _Parser(text, width, height).parse().scene()
+def parse(text: str, width: int = 320, height: int = 240, include_dirs=(), skip_unsupported=False) -> scenes.Scene:
+    """Read scene-description text into a Scene rendered at width x height;
+    #include files are looked for in `include_dirs`.  With
+    `skip_unsupported`, shapes pvx does not have are dropped with a warning
+    (else they are an error)."""
+    return _Parser(text, width, height, include_dirs, skip_unsupported).parse().scene()
 
 
-def parse_file(path, width: int = 320, height: int = 240) -> scenes.Scene:
-    with open(path, encoding="utf-8") as f:
-        return parse(f.read(), width, height)
+def parse_file(path, width: int = 320, height: int = 240, library_paths=(), skip_unsupported=False) -> scenes.Scene:
+    """Read a scene file; #include files are looked for in its folder, then
+    in `library_paths` (the reference's Library_Path / +L option)."""
+    import os
+    return parse(_read_text(path), width, height, [os.path.dirname(os.path.abspath(path))] + list(library_paths),
+                 skip_unsupported)

@@ -92,7 +92,6 @@ def test_sdl_reads_the_scene():
 
 @pytest.mark.parametrize("bad, what", [
     ('blob { }', "blob"),
-    ('sphere { <0,0,0>, 1 scale <1, 2, 1> }', "uniform scale"),
     ('sphere { <0,0,0>, 1 clipped_by { plane { y, 0 } } }', "clipped_by"),
     ('#error "stop"', "stop"),
     ('#while (1 < 2) sphere { <0,0,0>, 1 }', "unterminated"),
@@ -114,6 +113,41 @@ def test_sdl_reduces_what_it_cannot_render(text, what):
     what pvx renders, each with a warning naming the line."""
     s = sdl.parse("\n\n" + text, 32, 24)
     assert any(what in w and w.startswith("line 3") for w in s.settings["warnings"]), s.settings["warnings"]
+
+
+def test_sdl_language_forms(tmp_path):
+    """#include from the scene's folder and the library paths, user
+    functions, the older `color red r green g` form with trailing
+    components, 2-vectors promoted to 3, layered textures, lathe and sor
+    statements, an ellipsoid read as a quadric, and unsupported shapes
+    dropped with a warning on request."""
+    (tmp_path / "lib").mkdir()
+    (tmp_path / "lib" / "mine.inc").write_text("#declare Rad = 0.5;\n#declare F = function(a, b) { a * 2 + b }\n")
+    (tmp_path / "scene.pov").write_text("""
+        #include "mine.inc"
+        sphere { <F(1, 0.5), 0, 0>, Rad pigment { color red 1 green 0.5 filter 0.25 } }
+        polygon { 4, <0, 0>, <1, 0>, <1, 1>, <0, 0> }
+        lathe { bezier_spline 4, <0, 0>, <1, 0.3>, <1, 0.7>, <0.2, 1> sturm translate x }
+        sor { 4, <0, 0>, <1, 0.5>, <0.6, 1>, <0.6, 1.5> open }
+        sphere { 0, 1 scale <1, 2, 1> translate 3 * z texture { pigment { rgb 1 } } texture { pigment { rgbt 1 } } }
+        union { box { 0, 1 } blob { threshold 0.5 sphere { 0, 1, 1 } } }
+    """)
+    with pytest.raises(sdl.SdlError, match="blob"):
+        sdl.parse_file(str(tmp_path / "scene.pov"), 32, 24, [str(tmp_path / "lib")])
+    s = sdl.parse_file(str(tmp_path / "scene.pov"), 32, 24, [str(tmp_path / "lib")], skip_unsupported=True)
+    assert np.allclose(s.sphere_center[0], [2.5, 0, 0]) and s.sphere_radius[0] == 0.5
+    assert np.allclose(s.materials["pigment"][0], [1, 0.5, 0]) and s.materials["filter"][0] == 0.25
+    assert np.allclose(s.polygon_points[:, 2], 0.0)
+    assert list(s.revolution_kind) == [A.PVX_LATHE_BEZIER, A.PVX_SOR_CUBIC]
+    assert list(s.revolution_flags) == [A.PVX_REV_STURM, A.PVX_REV_OPEN]
+    assert s.revolution_points.shape == (8, 2) and list(s.revolution_first) == [0, 4, 8]
+    kinds = [int(k) for k in s.objects[:, 0]]
+    assert kinds == [A.PVX_SPHERE, A.PVX_POLYGON, A.PVX_LATHE, A.PVX_SOR, A.PVX_QUADRIC, A.PVX_CSG]
+    w = " | ".join(s.settings["warnings"])
+    assert "layered texture" in w and "quadric" in w and "blob" in w
+    # the ellipsoid: x^2 + (y/2)^2 + (z-3)^2 = 1, hit along -z from behind
+    h = OracleScene(s).find_intersection(np.array([[0.0, 1.0, 10.0, 0.0, 0.0, -1.0]]), method=0)
+    assert h["obj"][0] == 4 and h["t"][0] == pytest.approx(7.0 - np.sqrt(0.75), rel=1e-12)
 
 
 def test_sdl_directives_and_expressions():
@@ -178,14 +212,16 @@ def test_gpu_renders_parsed_scene_as_oracle(gpu):
 
 
 # ---------------------------------------------------------------------------
-# three of the reference's sample scenes (distribution/scenes), parsed into
+# five of the reference's sample scenes (distribution/scenes), parsed into
 # the fixtures tests/golden/sdl_*.npz by tests/golden/make_sdl_scenes.py
 # ---------------------------------------------------------------------------
 import os  # noqa: E402
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 REF_SCENES = "/root/reference/distribution/scenes"
-SAMPLES = {"tango": "bsp/Tango.pov", "mesh2": "objects/mesh2.pov", "cornell": "radiosity/cornell.pov"}
+REF_INCLUDE = "/root/reference/distribution/include"
+SAMPLES = {"tango": "bsp/Tango.pov", "mesh2": "objects/mesh2.pov", "cornell": "radiosity/cornell.pov",
+           "sor1": "objects/sor1.pov", "lathe2": "objects/lathe2.pov"}
 
 
 def _fixture(name):
@@ -198,7 +234,7 @@ def test_sample_scene_reads_as_its_fixture(name):
     """The reader loads the sample scene (directives, macros, rand, textures
     reduced with warnings) and gives exactly the stored fixture."""
     fx = _fixture(name)
-    s = sdl.parse_file(os.path.join(REF_SCENES, SAMPLES[name]), fx.camera["width"], fx.camera["height"])
+    s = sdl.parse_file(os.path.join(REF_SCENES, SAMPLES[name]), fx.camera["width"], fx.camera["height"], [REF_INCLUDE])
     assert s.settings["warnings"], "the sample scenes use features the reader reduces"
     for k in scenes.Scene._ARRAYS:
         a, b = getattr(s, k), getattr(fx, k)
@@ -238,6 +274,25 @@ def test_gpu_sample_scene_matches_oracle(gpu, name):
     img, _ = gs.render()
     ref, _ = o.render(s)
     q = lambda a: np.round(np.clip(a, 0.0, 1.0) * 255.0).astype(np.int32)  # noqa: E731
-    ok = float(np.mean(np.abs(q(img) - q(ref)).max(-1) <= 1))
+    close = np.abs(q(img) - q(ref)).max(-1) <= 1
+    if s.revolution_flags is not None and not np.all(s.revolution_flags & A.PVX_REV_STURM):
+        # sors without `sturm` solve their cubic with the reference's closed
+        # form, which loses most digits for rays near a sor's horizontal
+        # (reflections here): pixels whose oracle colour changes when `sturm`
+        # is set are solver-sensitive in the reference itself and left out;
+        # the same frame with `sturm` set is compared in full below
+        st = scenes.Scene.load(os.path.join(GOLDEN, f"sdl_{name}.npz"))
+        st.revolution_flags = st.revolution_flags | A.PVX_REV_STURM
+        ref_st, _ = OracleScene(st).render(st)
+        stable = np.abs(q(ref) - q(ref_st)).max(-1) <= 1
+        gst = gpu.GpuScene(st)
+        gst.set_shading()
+        img_st, _ = gst.render()
+        ok_st = float(np.mean(np.abs(q(img_st) - q(ref_st)).max(-1) <= 1))
+        print(f"[sdl {name}] with sturm: {ok_st * 100:.3f}% of pixels within 1/255; solver-sensitive pixels "
+              f"left out of the default frame: {int((~stable).sum())}", flush=True)
+        assert ok_st >= 0.999
+        close = close[stable]
+    ok = float(np.mean(close))
     print(f"[sdl {name}] shaded frame: {ok * 100:.3f}% of pixels within 1/255", flush=True)
     assert ok >= 0.999
