@@ -379,7 +379,17 @@ enum {
     PVX_TRAVERSAL_DEFAULT = 0,   /* by scene content and ray kind                     */
     PVX_TRAVERSAL_WIDE_WW = 1,   /* 4-wide tree, while-while (triangle/sphere scenes) */
     PVX_TRAVERSAL_WIDE_STEP = 2, /* 4-wide tree, one node or leaf per pass            */
-    PVX_TRAVERSAL_BINARY = 3     /* binary tree (scenes without tori / CSG)           */
+    PVX_TRAVERSAL_BINARY = 3,    /* binary tree (scenes without tori / CSG)           */
+    /* pvx_render's light rays in triangle/sphere scenes whose materials are all
+     * opaque (chosen by pvx_render_autotune, reported in its result): the rays
+     * are made in the kernel from the shaded point and the light's grid and
+     * only tested for occlusion, with the 4-wide tree walked ... */
+    PVX_TRAVERSAL_PACKET = 4,    /* ... by each wavefront as one packet (one shared
+                                    stack, ballots of the lanes' box tests)          */
+    PVX_TRAVERSAL_FUSED = 5,     /* ... while-while per lane (the default)          */
+    PVX_TRAVERSAL_REFILL = 6     /* ... while-while per lane, a wavefront refilling
+                                    its free lanes with the next unstarted rays once
+                                    16 or fewer of its lanes still hold rays         */
 };
 
 typedef struct pvx_tune_result {
@@ -567,9 +577,12 @@ int pvx_render_samples(pvx_scene *scene, const pvx_camera *cam, uint32_t n_sampl
 
 /* pvx_render that first runs pvx_scene_autotune on the frame's own camera
  * rays (tune[0]) and first wavefront of light rays (tune[1], all zero when
- * the frame has none); the image is the one pvx_render gives.  tune may be
- * NULL.  stats->total_ms includes the tuning launches; trace_ms and
- * shadow_ms do not. */
+ * the frame has none); the image is the one pvx_render gives.  Where light
+ * rays are made in the kernel (PVX_TRAVERSAL_PACKET / FUSED / REFILL), the
+ * walk and grid are timed on the light rays of camera-ray hits (reported in
+ * tune[1]) and again on those of the first bounce, each set keeping its own
+ * choice for later frames.  tune may be NULL.  stats->total_ms includes the
+ * tuning launches; trace_ms and shadow_ms do not. */
 int pvx_render_autotune(pvx_scene *scene, const pvx_camera *cam, float *rgb, pvx_tune_result *tune,
                         pvx_render_stats *stats, void *stream);
 

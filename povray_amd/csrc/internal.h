@@ -65,6 +65,7 @@ struct RenderBuffers {
     DevBuf hits, contrib, shade, lslot, lterm;
     DevBuf crays, cstate, nchild, offs, sums; // children of a bounce and their placement
     DevBuf srays[2], sdepth[2], sowner[2], shits, scol; // light rays
+    DevBuf groups, vis; // packet light rays: shaded point per (hit, light), visibility per ray
     DevBuf counters, offsets;
 };
 
@@ -76,6 +77,10 @@ struct pvx_scene {
     pvx::SceneView view{};
     uint32_t tune_variant[2] = {0, 0}; // autotuner's choice: [0] camera/bounce rays, [1] light rays
     uint32_t tune_grid[2] = {0, 0};    // (0: the default grid)
+    // pvx_render's light rays made in the traversal kernel: walk and grid for
+    // those of camera-ray hits [0] and of later bounces [1] (0: the default)
+    uint32_t light_variant[2] = {0, 0};
+    uint32_t light_grid[2] = {0, 0};
     pvx::KdView kd{};     // kd-tree of a PVX_ACCEL_BSP scene
     bool use_kd = false;
     std::vector<void *> allocs;
@@ -118,6 +123,15 @@ namespace pvx {
 int trace_closest_internal(pvx_scene *s, const pvx_ray *rays, const double *max_depth,
                            uint32_t ray_kind, size_t n, pvx_hit *hits, hipStream_t stream,
                            const uint8_t *kinds = nullptr);
+// Light rays made in the traversal kernel from shaded points (LightGroup per
+// lit point and light; S rays per group, the light's grid in the first ones)
+// with the occlusion test; one visibility byte per ray.  Triangle / sphere
+// scenes with a 4-wide tree.  `lights` is the device light array.  With
+// `next` (a device word holding 0) lanes take new rays from that counter as
+// their rays end; without it each lane walks a fixed stride of rays.  `grid`
+// blocks (0: the walk's default).
+int light_rays_internal(pvx_scene *s, const LightGroup *groups, const LightRec *lights, uint32_t S,
+                        uint32_t n_rays, uint8_t *vis, uint32_t *next, uint32_t grid, hipStream_t stream);
 // The autotuner over sample rays with optional per-ray max depths.
 int autotune_internal(pvx_scene *s, const pvx_ray *rays, const double *max_depth, size_t n,
                       uint32_t ray_kind, pvx_tune_result *out, hipStream_t stream);

@@ -262,6 +262,14 @@ struct LightRec {
     uint32_t first;       // first shadow-sample slot of this light within a hit
 };
 
+// A shaded point lit by one light: the light rays to the light's grid points
+// start here (light rays made inside the traversal kernels).
+struct LightGroup {
+    double ip[3];
+    uint32_t hit;
+    uint32_t light; // index into the scene's lights
+};
+
 // An object tested outside the BVH.
 struct InfRec {
     uint32_t kind; // kPrimPlane / kPrimCsg / kPrimQuadric

@@ -24,6 +24,7 @@
 #include "internal.h"
 #include "normal_device.h"
 #include "shapes_device.h"
+#include "trace_body.h"
 
 using namespace pvx;
 
@@ -73,8 +74,17 @@ __global__ void k_fill(float *p, size_t n, float v) {
     if (i < n) p[i] = v;
 }
 
+// counters: [0] light rays, [1] light-ray slots, [2] next queue size,
+// [3] lit (hit, light) groups whose light rays the traversal kernel makes
+constexpr uint32_t kGroupCounter = 3;
 __global__ void k_zero_counters(uint32_t *c) {
-    if (threadIdx.x < 4) c[threadIdx.x] = 0;
+    if (threadIdx.x < 64) c[threadIdx.x] = 0;
+}
+
+// Slot of the render counters that hands out light rays to the refilling walk.
+constexpr uint32_t kRefillCounter = 8;
+__global__ void k_zero_word(uint32_t *c) {
+    if (threadIdx.x == 0) *c = 0;
 }
 
 // Camera rays of sample s of pixel p at queue position p * S + s
@@ -140,12 +150,17 @@ __device__ inline void write_child(pvx_ray *crays, PathState *cst, size_t slot, 
 // Shade the hits of one bounce: ambient now, per-light terms and shadow rays
 // for the finish pass, and up to two rays of the next bounce (refracted or
 // totally reflected first, reflected second) in slots 2i, 2i+1.
+// kPacket: light rays are not written; each lit (hit, light) appends its
+// shaded point and light to the group list and the traversal kernel makes
+// the rays.
+template <bool kPacket>
 __global__ void k_shade(RenderParams p, const pvx_ray *__restrict__ rays, const PathState *__restrict__ st,
                         const pvx_hit *__restrict__ hits, uint32_t n, float *__restrict__ contrib,
                         ShadeOut *__restrict__ shade, uint32_t *__restrict__ lslot, double *__restrict__ lterm,
                         pvx_ray *__restrict__ srays, double *__restrict__ sdepth, uint32_t *__restrict__ sowner,
                         float *__restrict__ scol, pvx_ray *__restrict__ crays, PathState *__restrict__ cst,
-                        uint32_t *__restrict__ nchild, uint32_t *__restrict__ counters) {
+                        uint32_t *__restrict__ nchild, uint32_t *__restrict__ counters,
+                        LightGroup *__restrict__ groups) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const pvx_ray r = rays[i];
@@ -197,25 +212,24 @@ __global__ void k_shade(RenderParams p, const pvx_ray *__restrict__ rays, const 
         }
         lterm[2 * li] = m.diffuse > 0.0f ? inten : 0.0;
         lterm[2 * li + 1] = ph;
+        if constexpr (kPacket) {
+            atomicAdd(&counters[1], lt.n1 * lt.n2); // light rays (statistics)
+            const uint32_t g = atomicAdd(&counters[kGroupCounter], 1u);
+            lslot[li] = g;
+            groups[g] = LightGroup{{ip.x, ip.y, ip.z}, i, L};
+            continue;
+        }
         uint32_t ns = lt.n1 * lt.n2;
         uint32_t base = atomicAdd(&counters[1], ns);
         lslot[li] = base;
         for (uint32_t u = 0; u < lt.n1; u++)
             for (uint32_t v = 0; v < lt.n2; v++) {
-                D3 tgt = C;
-                if (lt.n1 > 1) {
-                    double f = (double)u / (double)(lt.n1 - 1) - 0.5;
-                    tgt = add3(tgt, D3{lt.axis1[0] * f, lt.axis1[1] * f, lt.axis1[2] * f});
-                }
-                if (lt.n2 > 1) {
-                    double f = (double)v / (double)(lt.n2 - 1) - 0.5;
-                    tgt = add3(tgt, D3{lt.axis2[0] * f, lt.axis2[1] * f, lt.axis2[2] * f});
-                }
-                D3 sd = sub3(tgt, ip);
-                double sl = sqrt(dot3(sd, sd));
+                D3 sd;
+                double sl;
+                light_sample_ray(lt, u, v, ip, sd, sl);
                 pvx_ray sr;
                 sr.o[0] = ip.x; sr.o[1] = ip.y; sr.o[2] = ip.z;
-                sr.d[0] = sd.x / sl; sr.d[1] = sd.y / sl; sr.d[2] = sd.z / sl;
+                sr.d[0] = sd.x; sr.d[1] = sd.y; sr.d[2] = sd.z;
                 uint32_t k = base + u * lt.n2 + v;
                 srays[k] = sr;
                 sdepth[k] = sl;
@@ -356,30 +370,188 @@ __global__ void k_shadow_step(RenderParams p, const pvx_ray *__restrict__ srays,
     nowner[slot] = ow;
 }
 
-// TraceAreaLightSubsetShadowRay's quadtree average with adaptive 100 (every
-// region subdivides until it spans <= 2x2 grid points).
-template <int D>
-__device__ float area_average(const float *vis, int n2, int u1, int v1, int u2, int v2) {
-    float c0 = vis[u1 * n2 + v1], c1 = vis[u2 * n2 + v1], c2 = vis[u1 * n2 + v2], c3 = vis[u2 * n2 + v2];
-    if constexpr (D > 0) {
-        if ((u2 - u1 > 1) || (v2 - v1 > 1)) {
-            int mu0 = (u1 + u2) / 2, mu1 = (u1 + u2 + 1) / 2;
-            int mv0 = (v1 + v2) / 2, mv1 = (v1 + v2 + 1) / 2;
-            c0 = area_average<D - 1>(vis, n2, u1, v1, mu0, mv0);
-            c1 = area_average<D - 1>(vis, n2, mu1, v1, u2, mv0);
-            c2 = area_average<D - 1>(vis, n2, u1, mv1, mu0, v2);
-            c3 = area_average<D - 1>(vis, n2, mu1, mv1, u2, v2);
+// Light rays in packets (triangle / sphere scenes, every material opaque):
+// each wavefront takes 64 consecutive rays (ray k = group k / S, grid point
+// k % S of the group's light; consecutive groups are neighbouring hits) and
+// walks the 4-wide tree as one packet: one node sequence for the whole
+// wavefront with one stack in LDS, a child visited when any live lane's ray
+// enters its box within that ray's light distance (ballot), children in the
+// entry order of the first live lane.  A lane's ray is blocked by the first
+// valid hit closer than the light minus SHADOW_TOLERANCE (the occlusion test
+// of TracePointLightShadowRay, trace.cpp:2021-2073, with an opaque blocker);
+// a lane with a blocked ray takes no further part, and the walk ends when no
+// lane is live.  Every hit a lane's own walk could find lies in a box that
+// lane enters, so the visibility is the per-ray walk's: vis[k] = 1 when
+// nothing blocks.  The stack holds at most 3 entries per tree level (the
+// scene's 4-wide tree qualifies only when 3 * depth + 1 < kStackMax).
+constexpr int kPacketWaves = 4; // wavefronts per block of 256
+template <bool kFlags>
+__global__ __launch_bounds__(256) void k_light_packet(SceneView sv, const LightRec *__restrict__ lights,
+                                                      const LightGroup *__restrict__ groups, uint32_t S,
+                                                      uint32_t n_rays, uint8_t *__restrict__ vis) {
+    __shared__ int32_t wstack[kPacketWaves][kStackMax];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    int32_t *stk = wstack[wave];
+    const uint32_t stride = gridDim.x * kPacketWaves * 64u;
+    for (uint32_t base = (blockIdx.x * kPacketWaves + wave) * 64u; base < n_rays; base += stride) {
+        const uint32_t k = base + lane;
+        bool valid = k < n_rays;
+        D3 o{0.0, 0.0, 0.0}, d{0.0, 0.0, 1.0};
+        double sl = 1.0;
+        if (valid) {
+            const uint32_t g = k / S, sidx = k - g * S;
+            const LightGroup grp = groups[g];
+            const LightRec &lt = lights[grp.light];
+            if (sidx < lt.n1 * lt.n2) {
+                o = D3{grp.ip[0], grp.ip[1], grp.ip[2]};
+                light_sample_ray(lt, sidx / lt.n2, sidx % lt.n2, o, d, sl);
+            } else {
+                valid = false; // past this light's grid
+            }
         }
+        const double occl_lim = sl - kShadowTolerance;
+        bool blocked = false;
+        // objects outside the tree (planes)
+        if (valid)
+            for (uint32_t q = 0; q < sv.n_inf; q++) {
+                const InfRec inf = sv.inf[q];
+                if (kFlags && !precondition(sv.obj_flags[inf.obj], PVX_RAY_SHADOW)) continue;
+                double dep;
+                if (inf.kind == kPrimPlane && plane_hit(sv.planes[inf.index], o, d, dep) && depth_ok(dep, true) &&
+                    dep < occl_lim)
+                    blocked = true;
+            }
+        RayF rf;
+        rf.ix = safe_inv(d.x);
+        rf.iy = safe_inv(d.y);
+        rf.iz = safe_inv(d.z);
+        rf.oix = (float)o.x * rf.ix;
+        rf.oiy = (float)o.y * rf.iy;
+        rf.oiz = (float)o.z * rf.iz;
+        const float tb = (float)sl * 1.000001f;
+        uint64_t live = __ballot(valid && !blocked);
+        int32_t node = live ? sv.root4 : kEmpty;
+        int sp = 0;
+        while (node != kEmpty) {
+            node = __builtin_amdgcn_readfirstlane(node);
+            if (node >= 0) {
+                const float4 *np = reinterpret_cast<const float4 *>(sv.nodes4 + node);
+                const float4 xl = np[0], xh = np[1], yl = np[2], yh = np[3], zl = np[4], zh = np[5];
+                const int4 cl = reinterpret_cast<const int4 *>(np)[6];
+                const bool me = valid && !blocked;
+                auto slab = [&](float lx, float hx, float ly, float hy, float lz, float hz, int32_t link) {
+                    const float x0 = fmaf(lx, rf.ix, -rf.oix), x1 = fmaf(hx, rf.ix, -rf.oix);
+                    const float y0 = fmaf(ly, rf.iy, -rf.oiy), y1 = fmaf(hy, rf.iy, -rf.oiy);
+                    const float z0 = fmaf(lz, rf.iz, -rf.oiz), z1 = fmaf(hz, rf.iz, -rf.oiz);
+                    const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
+                    const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tb));
+                    return (me && link != kEmpty && tn <= tf * 1.000001f) ? tn : INFINITY;
+                };
+                const float e[4] = {slab(xl.x, xh.x, yl.x, yh.x, zl.x, zh.x, cl.x),
+                                    slab(xl.y, xh.y, yl.y, yh.y, zl.y, zh.y, cl.y),
+                                    slab(xl.z, xh.z, yl.z, yh.z, zl.z, zh.z, cl.z),
+                                    slab(xl.w, xh.w, yl.w, yh.w, zl.w, zh.w, cl.w)};
+                const int32_t links[4] = {cl.x, cl.y, cl.z, cl.w};
+                const int leader = __builtin_amdgcn_readfirstlane(__ffsll((unsigned long long)live) - 1);
+                // wave-uniform order keys: the leader's entry distance, after
+                // it the children only other lanes enter (in slot order)
+                float key[4];
+                int32_t lk[4];
+                int cnt = 0;
+                for (int c = 0; c < 4; c++) {
+                    const uint64_t m = __ballot(e[c] != INFINITY);
+                    if (!m) continue;
+                    const float kl = __builtin_bit_cast(
+                        float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, e[c]), leader));
+                    key[cnt] = kl != INFINITY ? kl : 3.0e38f; // the sort is stable: slot order
+                    lk[cnt] = links[c];
+                    cnt++;
+                }
+                if (cnt == 0) {
+                    node = sp == 0 ? kEmpty : stk[--sp];
+                    continue;
+                }
+                for (int a = 1; a < cnt; a++) // insertion sort of <= 4 uniform entries
+                    for (int b = a; b > 0 && key[b] < key[b - 1]; b--) {
+                        const float tk = key[b];
+                        key[b] = key[b - 1];
+                        key[b - 1] = tk;
+                        const int32_t tl = lk[b];
+                        lk[b] = lk[b - 1];
+                        lk[b - 1] = tl;
+                    }
+                for (int a = cnt - 1; a >= 1; a--) stk[sp++] = lk[a]; // farthest first
+                node = lk[0];
+            } else {
+                const uint32_t leaf = (uint32_t)(~node);
+                const uint32_t first = leaf >> 3, count = (leaf & 7u) + 1u;
+                for (uint32_t q = 0; q < count; q++) {
+                    const PrimRec &pr = sv.prims[first + q];
+                    const uint32_t ka = pr.kind_axis;
+                    const uint32_t pk = ka & 15u;
+                    if (!(valid && !blocked)) continue;
+                    if (kFlags && !precondition(sv.obj_flags[pr.obj], PVX_RAY_SHADOW)) continue;
+                    if (pk == kPrimTriangle) {
+                        double dep;
+                        if (mesh_tri_hit_l<0>(sv, pr, ka, o, d, dep) && depth_ok(dep, true) && dep < occl_lim)
+                            blocked = true;
+                    } else if (pk == kPrimSphere) {
+                        double hs[2];
+                        const int nh = sphere_hits(d3(pr.sph.c[0], pr.sph.c[1], pr.sph.c[2]), pr.sph.r, o, d, hs);
+                        for (int j = 0; j < nh; j++)
+                            if (depth_ok(hs[j], true) && hs[j] < occl_lim) blocked = true;
+                    }
+                }
+                live = __ballot(valid && !blocked);
+                if (!live) break;
+                node = sp == 0 ? kEmpty : stk[--sp];
+            }
+        }
+        if (valid) vis[k] = blocked ? 0 : 1;
     }
-    return (c0 + c1 + c2 + c3) * 0.25f;
+}
+
+// TraceAreaLightSubsetShadowRay's quadtree average with adaptive 100 (every
+// region subdivides until it spans <= 2x2 grid points), the three colour
+// channels together; get(k) is the light colour that reached sample k.
+struct F3 {
+    float x, y, z;
+};
+template <int D, class Get>
+__device__ F3 area_average(const Get &get, int n2, int u1, int v1, int u2, int v2) {
+    F3 c0, c1, c2, c3;
+    if (D > 0 && ((u2 - u1 > 1) || (v2 - v1 > 1))) {
+        if constexpr (D > 0) {
+            const int mu0 = (u1 + u2) / 2, mu1 = (u1 + u2 + 1) / 2;
+            const int mv0 = (v1 + v2) / 2, mv1 = (v1 + v2 + 1) / 2;
+            c0 = area_average<D - 1>(get, n2, u1, v1, mu0, mv0);
+            c1 = area_average<D - 1>(get, n2, mu1, v1, u2, mv0);
+            c2 = area_average<D - 1>(get, n2, u1, mv1, mu0, v2);
+            c3 = area_average<D - 1>(get, n2, mu1, mv1, u2, v2);
+        }
+    } else {
+        c0 = get(u1 * n2 + v1);
+        c1 = get(u2 * n2 + v1);
+        c2 = get(u1 * n2 + v2);
+        c3 = get(u2 * n2 + v2);
+    }
+    return F3{(c0.x + c1.x + c2.x + c3.x) * 0.25f, (c0.y + c1.y + c2.y + c3.y) * 0.25f,
+              (c0.z + c1.z + c2.z + c3.z) * 0.25f};
 }
 
 constexpr int kMaxAreaGrid = 16; // per axis
+// A region spanning w grid steps splits into two of floor(w / 2) steps until
+// w <= 1: 15 -> 7 -> 3 -> 1, so three levels of splits cover 16 points.
+constexpr int kAreaSplits = 3;
 
 // Light colour -> surface colour for each shaded hit, added to its contribution.
+// kPacket: the light colour of sample k of group g is the light's colour
+// times the visibility byte vis[g * S + k].
+template <bool kPacket>
 __global__ void k_finish(RenderParams p, const ShadeOut *__restrict__ shade, uint32_t n,
                          const uint32_t *__restrict__ lslot, const double *__restrict__ lterm,
-                         const float *__restrict__ scol, float *__restrict__ contrib) {
+                         const float *__restrict__ scol, float *__restrict__ contrib,
+                         const uint8_t *__restrict__ vis, uint32_t S) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const ShadeOut so = shade[i];
@@ -392,16 +564,36 @@ __global__ void k_finish(RenderParams p, const ShadeOut *__restrict__ shade, uin
         if (base == kNone) continue;
         const LightRec &lt = p.lights[L];
         float lc[3];
-        uint32_t ns = lt.n1 * lt.n2;
-        if (ns == 1) {
-            for (int c = 0; c < 3; c++) lc[c] = lt.colour[c] * scol[3 * (size_t)base + c];
-        } else {
-            float vis[kMaxAreaGrid * kMaxAreaGrid];
-            for (int c = 0; c < 3; c++) {
-                for (uint32_t k = 0; k < ns; k++) vis[k] = lt.colour[c] * scol[3 * (size_t)(base + k) + c];
-                lc[c] = area_average<5>(vis, (int)lt.n2, 0, 0, (int)lt.n1 - 1, (int)lt.n2 - 1);
+        const uint32_t ns = lt.n1 * lt.n2;
+        F3 avg;
+        if constexpr (kPacket) {
+            // visibility bits of the group's samples (at most 16 x 16)
+            uint64_t m0 = 0, m1 = 0, m2 = 0, m3 = 0;
+            const uint8_t *vb = vis + (size_t)base * S;
+            for (uint32_t k = 0; k < ns; k++) {
+                const uint64_t b = (uint64_t)(vb[k] != 0) << (k & 63u);
+                if (k < 64) m0 |= b;
+                else if (k < 128) m1 |= b;
+                else if (k < 192) m2 |= b;
+                else m3 |= b;
             }
+            auto get = [&](int k) -> F3 {
+                const uint64_t w = k < 128 ? (k < 64 ? m0 : m1) : (k < 192 ? m2 : m3);
+                const float v = ((w >> (k & 63)) & 1u) ? 1.0f : 0.0f;
+                return F3{lt.colour[0] * v, lt.colour[1] * v, lt.colour[2] * v};
+            };
+            avg = ns == 1 ? get(0) : area_average<kAreaSplits>(get, (int)lt.n2, 0, 0, (int)lt.n1 - 1, (int)lt.n2 - 1);
+        } else {
+            // the filter colour that reached the light from sample k
+            auto get = [&](int k) -> F3 {
+                const float *c = scol + 3 * ((size_t)base + (size_t)k);
+                return F3{lt.colour[0] * c[0], lt.colour[1] * c[1], lt.colour[2] * c[2]};
+            };
+            avg = ns == 1 ? get(0) : area_average<kAreaSplits>(get, (int)lt.n2, 0, 0, (int)lt.n1 - 1, (int)lt.n2 - 1);
         }
+        lc[0] = avg.x;
+        lc[1] = avg.y;
+        lc[2] = avg.z;
         if (fabsf(lc[0]) < 1e-10f && fabsf(lc[1]) < 1e-10f && fabsf(lc[2]) < 1e-10f) continue;
         double inten = lterm[2 * li], ph = lterm[2 * li + 1];
         for (int k = 0; k < 3; k++) {
@@ -640,7 +832,20 @@ static int render_frame(pvx_scene *s, const pvx_camera *cam, uint32_t S, const d
         if ((rr = b.shits.reserve(ns * sizeof(pvx_hit))) || (rr = b.scol.reserve(ns * 12))) return rr;
         return PVX_OK;
     };
-    if ((r = ensure_queue(nray)) || (r = ensure_shadow(nray * spp)) || (r = b.offsets.reserve(16 * S))) return r;
+    // light rays in packets: triangle / sphere scenes walked through the
+    // 4-wide tree, every material opaque, unless the caller picked another
+    // walk for light rays
+    const bool packet = sh.all_opaque && s->view.has_complex == 0 && s->view.wide && !s->use_kd &&
+                        (s->tune_variant[1] == PVX_TRAVERSAL_DEFAULT || s->tune_variant[1] >= PVX_TRAVERSAL_PACKET);
+    uint32_t smax = 1; // rays per group: the largest light grid
+    for (const LightRec &lt : sh.lights) smax = std::max(smax, lt.n1 * lt.n2);
+    auto ensure_packet = [&](size_t nh) -> int {
+        int rr;
+        if ((rr = b.groups.reserve(nh * nl * sizeof(LightGroup))) || (rr = b.vis.reserve(nh * nl * smax))) return rr;
+        return PVX_OK;
+    };
+    if ((r = ensure_queue(nray)) || (r = b.offsets.reserve(16 * S)) || (r = b.counters.reserve(256))) return r;
+    if ((r = packet ? ensure_packet(nray) : ensure_shadow(nray * spp))) return r;
     HIP_TRY(hipMemcpyAsync(b.offsets.p, offsets, 16 * S, hipMemcpyHostToDevice, st));
 
     RenderParams p;
@@ -693,15 +898,15 @@ static int render_frame(pvx_scene *s, const pvx_camera *cam, uint32_t S, const d
         HIP_TRY(hipEventRecord(t1, st));
         n_closest += n;
         uint32_t *counters = static_cast<uint32_t *>(b.counters.p);
-        if ((r = ensure_shadow((size_t)n * spp))) return r; // light rays of this bounce
+        if ((r = packet ? ensure_packet(n) : ensure_shadow((size_t)n * spp))) return r; // light rays of this bounce
         k_zero_counters<<<1, 64, 0, st>>>(counters);
-        k_shade<<<(n + 127) / 128, 128, 0, st>>>(
+        (packet ? k_shade<true> : k_shade<false>)<<<(n + 127) / 128, 128, 0, st>>>(
             p, rays, pst, static_cast<const pvx_hit *>(b.hits.p), n, static_cast<float *>(b.contrib.p),
             static_cast<ShadeOut *>(b.shade.p), static_cast<uint32_t *>(b.lslot.p), static_cast<double *>(b.lterm.p),
             static_cast<pvx_ray *>(b.srays[0].p), static_cast<double *>(b.sdepth[0].p),
             static_cast<uint32_t *>(b.sowner[0].p), static_cast<float *>(b.scol.p),
             static_cast<pvx_ray *>(b.crays.p), static_cast<PathState *>(b.cstate.p),
-            static_cast<uint32_t *>(b.nchild.p), counters);
+            static_cast<uint32_t *>(b.nchild.p), counters, static_cast<LightGroup *>(b.groups.p));
         HIP_TRY(hipGetLastError());
         // next queue size: exclusive scan of the child counts
         const uint32_t nb = (n + kScanBlock - 1) / kScanBlock;
@@ -710,16 +915,83 @@ static int render_frame(pvx_scene *s, const pvx_camera *cam, uint32_t S, const d
                                                 static_cast<uint32_t *>(b.offs.p), static_cast<uint32_t *>(b.sums.p));
         k_scan_sums<<<1, kScanBlock, 0, st>>>(static_cast<uint32_t *>(b.sums.p), nb, counters + 2);
         HIP_TRY(hipGetLastError());
-        uint32_t cnt[3];
+        uint32_t cnt[kGroupCounter + 1];
         HIP_TRY(hipMemcpyAsync(cnt, counters, sizeof cnt, hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
         float ms = 0.0f;
         HIP_TRY(hipEventElapsedTime(&ms, t0, t1));
         trace_ms += ms;
+        if (packet) { // light rays made in the kernel and tested for occlusion, one launch per light
+            const uint64_t nr = (uint64_t)cnt[kGroupCounter] * smax;
+            if (nr > 0xFFFFFFFFull) return fail(PVX_EUNSUPPORTED, "more than 2^32 light rays in one bounce");
+            // walk and grid (0: the walk's default) of this bounce's light rays
+            auto light_pass = [&](uint32_t variant, uint32_t grid) -> int {
+                if (nr == 0) return PVX_OK;
+                const LightGroup *gl = static_cast<const LightGroup *>(b.groups.p);
+                uint8_t *vl = static_cast<uint8_t *>(b.vis.p);
+                if (variant == PVX_TRAVERSAL_PACKET) {
+                    const uint32_t nb = (uint32_t)std::min<uint64_t>(grid ? grid : s->grid, (nr + 255) / 256);
+                    if (s->view.any_flags)
+                        k_light_packet<true><<<nb, 256, 0, st>>>(s->view, sh.d_lights, gl, smax, (uint32_t)nr, vl);
+                    else
+                        k_light_packet<false><<<nb, 256, 0, st>>>(s->view, sh.d_lights, gl, smax, (uint32_t)nr, vl);
+                    HIP_TRY(hipGetLastError());
+                    return PVX_OK;
+                }
+                uint32_t *next = nullptr;
+                if (variant == PVX_TRAVERSAL_REFILL) {
+                    next = counters + kRefillCounter;
+                    k_zero_word<<<1, 64, 0, st>>>(next);
+                }
+                return light_rays_internal(s, gl, sh.d_lights, smax, (uint32_t)nr, vl, next, grid, st);
+            };
+            auto timed_pass = [&](uint32_t variant, uint32_t grid, float &out_ms) -> int {
+                HIP_TRY(hipEventRecord(t0, st));
+                int rr = light_pass(variant, grid);
+                if (rr) return rr;
+                HIP_TRY(hipEventRecord(t1, st));
+                HIP_TRY(hipEventSynchronize(t1));
+                HIP_TRY(hipEventElapsedTime(&out_ms, t0, t1));
+                return PVX_OK;
+            };
+            // The light rays of camera-ray hits come in pixel order and walk
+            // alike in runs; those of later bounces less so, and the walk that
+            // suits them may differ: each keeps its own choice.
+            const int cls = bounces == 0 ? 0 : 1;
+            uint32_t variant = s->light_variant[cls], grid = s->light_grid[cls];
+            if (variant < PVX_TRAVERSAL_PACKET) variant = PVX_TRAVERSAL_FUSED;
+            if (tune && bounces < 2 && nr > 0) { // the walks and grids on this frame's light rays; the fastest is kept
+                struct Cand {
+                    uint32_t variant, grid;
+                    float ms;
+                } cands[4] = {{PVX_TRAVERSAL_FUSED, 2 * s->grid, 0.0f},
+                              {PVX_TRAVERSAL_FUSED, s->grid, 0.0f},
+                              {PVX_TRAVERSAL_REFILL, s->grid, 0.0f},
+                              {PVX_TRAVERSAL_PACKET, s->grid, 0.0f}};
+                if ((rc = light_pass(PVX_TRAVERSAL_FUSED, 0))) return rc; // warm-up: the first launch is slower
+                for (Cand &c : cands)
+                    if ((rc = timed_pass(c.variant, c.grid, c.ms))) return rc;
+                const Cand *best = &cands[0];
+                for (const Cand &c : cands)
+                    if (c.ms < best->ms) best = &c;
+                variant = best->variant;
+                grid = best->grid;
+                s->light_variant[cls] = variant;
+                s->light_grid[cls] = grid;
+                if (bounces == 0) {
+                    s->tune_variant[1] = variant;
+                    s->tune_grid[1] = grid;
+                    tune[1] = pvx_tune_result{variant, grid, best->ms, cands[0].ms};
+                }
+            }
+            if ((rc = timed_pass(variant, grid, ms))) return rc;
+            shadow_ms += ms;
+            n_shadow += cnt[1]; // the light rays made (k_shade counts them as it did when writing them)
+        }
         // light rays, followed through transparent objects
-        uint32_t ns = cnt[1];
+        uint32_t ns = packet ? 0u : cnt[1];
         int sq = 0;
-        if (tune && bounces == 0) {
+        if (tune && bounces == 0 && !packet) {
             if (ns > 0) {
                 if ((rc = autotune_internal(s, static_cast<const pvx_ray *>(b.srays[0].p),
                                             static_cast<const double *>(b.sdepth[0].p), ns, light_kind, &tune[1],
@@ -752,11 +1024,10 @@ static int render_frame(pvx_scene *s, const pvx_camera *cam, uint32_t S, const d
             ns = c2;
             sq = 1 - sq;
         }
-        k_finish<<<(n + 127) / 128, 128, 0, st>>>(p, static_cast<const ShadeOut *>(b.shade.p), n,
-                                                  static_cast<const uint32_t *>(b.lslot.p),
-                                                  static_cast<const double *>(b.lterm.p),
-                                                  static_cast<const float *>(b.scol.p),
-                                                  static_cast<float *>(b.contrib.p));
+        (packet ? k_finish<true> : k_finish<false>)<<<(n + 127) / 128, 128, 0, st>>>(
+            p, static_cast<const ShadeOut *>(b.shade.p), n, static_cast<const uint32_t *>(b.lslot.p),
+            static_cast<const double *>(b.lterm.p), static_cast<const float *>(b.scol.p),
+            static_cast<float *>(b.contrib.p), static_cast<const uint8_t *>(b.vis.p), smax);
         k_accumulate<<<(n + 255) / 256, 256, 0, st>>>(pst, static_cast<const float *>(b.contrib.p), n, rgb);
         HIP_TRY(hipGetLastError());
         const uint32_t next = cnt[2];
@@ -784,7 +1055,8 @@ static int render_frame(pvx_scene *s, const pvx_camera *cam, uint32_t S, const d
                 std::swap(cur_s.cap, b.state[cur].cap);
                 std::swap(cur_k.p, b.kinds[cur].p);
                 std::swap(cur_k.cap, b.kinds[cur].cap);
-                if ((r = ensure_queue((size_t)next + next / 2)) || (r = ensure_shadow(((size_t)next + next / 2) * spp)))
+                if ((r = ensure_queue((size_t)next + next / 2)) ||
+                    (r = packet ? ensure_packet((size_t)next + next / 2) : ensure_shadow(((size_t)next + next / 2) * spp)))
                     return r;
                 k_emit<<<(n + 255) / 256, 256, 0, st>>>(
                     static_cast<const uint32_t *>(keep_n.p), static_cast<const uint32_t *>(keep_o.p),

@@ -894,6 +894,24 @@ __device__ inline bool polygon_hit(const PolygonRec &pg, const double *pts, cons
     return true;
 }
 
+// Light ray to grid point (u, v) of light `lt` from ip
+// (TraceAreaLightSubsetShadowRay's grid points, trace.cpp:2131-2272): unit
+// direction and distance.
+__device__ inline void light_sample_ray(const LightRec &lt, uint32_t u, uint32_t v, D3 ip, D3 &dir, double &dist) {
+    D3 tgt{lt.pos[0], lt.pos[1], lt.pos[2]};
+    if (lt.n1 > 1) {
+        double f = (double)u / (double)(lt.n1 - 1) - 0.5;
+        tgt = add3(tgt, D3{lt.axis1[0] * f, lt.axis1[1] * f, lt.axis1[2] * f});
+    }
+    if (lt.n2 > 1) {
+        double f = (double)v / (double)(lt.n2 - 1) - 0.5;
+        tgt = add3(tgt, D3{lt.axis2[0] * f, lt.axis2[1] * f, lt.axis2[2] * f});
+    }
+    D3 sd = sub3(tgt, ip);
+    dist = sqrt(dot3(sd, sd));
+    dir = D3{sd.x / dist, sd.y / dist, sd.z / dist};
+}
+
 // ---------------------------------------------------------------------------
 // Lathe (core/shape/lathe.cpp) and surface of revolution (core/shape/sor.cpp)
 // with the bounding cylinders of core/bounding/boundingcylinder.cpp.  The

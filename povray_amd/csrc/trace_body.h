@@ -183,7 +183,12 @@ __device__ inline bool simple_closest(const SceneView &sv, uint32_t kind, uint32
 // `views`) is uniform across the wavefront.  With `clocks`, lane 0 of each
 // wavefront records the wall clock at its start (minimum into clocks[0]) and
 // at the end of each chunk (maximum into clocks[1 + scene]).
-template <int kLevel, bool kFlags, bool kCount, bool kWW, bool kMulti = false>
+// kGen: light rays made here (ray k from gen_groups[k / gen_S].ip to grid
+// point k mod gen_S of its group's light, rays past that light's grid
+// skipped; max depth the distance) whose visibility byte goes to gen_vis[k]
+// (1: nothing closer than the light minus SHADOW_TOLERANCE blocks it)
+// instead of a hit record.
+template <int kLevel, bool kFlags, bool kCount, bool kWW, bool kMulti = false, bool kGen = false>
 __device__ __forceinline__ void trace_closest_body(const SceneView &sv0, const SceneView *__restrict__ views,
                                                    const uint4 *__restrict__ chunks,
                                                    unsigned long long *__restrict__ clocks,
@@ -192,7 +197,10 @@ __device__ __forceinline__ void trace_closest_body(const SceneView &sv0, const S
                                                    const uint8_t *__restrict__ kinds, uint32_t n,
                                                    pvx_hit *__restrict__ hits, int32_t *__restrict__ spill,
                                                    uint32_t *__restrict__ node_visits,
-                                                   uint32_t *__restrict__ prim_tests, int32_t *s_stack) {
+                                                   uint32_t *__restrict__ prim_tests, int32_t *s_stack,
+                                                   const LightGroup *__restrict__ gen_groups = nullptr,
+                                                   const LightRec *__restrict__ gen_lights = nullptr,
+                                                   uint32_t gen_S = 1, uint8_t *__restrict__ gen_vis = nullptr) {
     const uint32_t lane = threadIdx.x;
     const uint32_t gtid = blockIdx.x * kBlock + lane;
     int32_t *my_spill = spill + (size_t)gtid * kStackSpill;
@@ -214,15 +222,27 @@ __device__ __forceinline__ void trace_closest_body(const SceneView &sv0, const S
             ri = ch.y + (i & 63u);
         }
         const SceneView sv = kMulti ? views[scene] : sv0;
-        const pvx_ray r = rays[ri];
         // per-ray kind (precondition only; shadow rays come in launches of their own)
         const uint32_t rk = (kFlags && kinds) ? (uint32_t)kinds[ri] : kind;
-        D3 o = d3(r.o[0], r.o[1], r.o[2]);
-        D3 d = d3(r.d[0], r.d[1], r.d[2]);
+        D3 o, d;
         double bt = kBoundHuge;
-        if (max_depth) {
-            double md = max_depth[ri];
+        if constexpr (kGen) {
+            const uint32_t g = ri / gen_S, sidx = ri - g * gen_S;
+            const LightGroup grp = gen_groups[g];
+            const LightRec &lt = gen_lights[grp.light];
+            if (sidx >= lt.n1 * lt.n2) continue; // past this light's grid
+            o = D3{grp.ip[0], grp.ip[1], grp.ip[2]};
+            double md;
+            light_sample_ray(lt, sidx / lt.n2, sidx % lt.n2, o, d, md);
             if (md >= kEpsilon) bt = md; // trace.cpp:157-158
+        } else {
+            const pvx_ray r = rays[ri];
+            o = d3(r.o[0], r.o[1], r.o[2]);
+            d = d3(r.d[0], r.d[1], r.d[2]);
+            if (max_depth) {
+                double md = max_depth[ri];
+                if (md >= kEpsilon) bt = md; // trace.cpp:157-158
+            }
         }
         const double occl_lim = bt - kShadowTolerance;
         int32_t bo = -1, bp = 0;
@@ -306,6 +326,13 @@ __device__ __forceinline__ void trace_closest_body(const SceneView &sv0, const S
                     lb = tl;
                 }
             };
+            auto push = [&](int32_t x) {
+                if (sp < kStackLds)
+                    s_stack[sp * kBlock + lane] = x;
+                else
+                    my_spill[sp - kStackLds] = x;
+                sp++;
+            };
             cx(d0, l0, d1, l1);
             cx(d2, l2, d3, l3);
             cx(d0, l0, d2, l2);
@@ -315,13 +342,6 @@ __device__ __forceinline__ void trace_closest_body(const SceneView &sv0, const S
                 PVX_POP();
                 return;
             }
-            auto push = [&](int32_t x) {
-                if (sp < kStackLds)
-                    s_stack[sp * kBlock + lane] = x;
-                else
-                    my_spill[sp - kStackLds] = x;
-                sp++;
-            };
             if (d3 != INFINITY) push(l3);
             if (d2 != INFINITY) push(l2);
             if (d1 != INFINITY) push(l1);
@@ -502,7 +522,10 @@ __device__ __forceinline__ void trace_closest_body(const SceneView &sv0, const S
             }
         }
 #undef PVX_POP
-        hits[ri] = pvx_hit{bt, bo, bp};
+        if constexpr (kGen) // TracePointLightShadowRay with an opaque blocker (trace.cpp:2021-2073)
+            gen_vis[ri] = (bo >= 0 && bt < occl_lim && bt > kShadowTolerance) ? 0 : 1;
+        else
+            hits[ri] = pvx_hit{bt, bo, bp};
         if (kCount) {
             node_visits[ri] = cnt_nodes;
             prim_tests[ri] = cnt_prims;

@@ -10,7 +10,7 @@
 #include "trace_launch.h"
 
 #ifndef PVX_TRACE_PART
-#error "build with -DPVX_TRACE_PART=<0..14>"
+#error "build with -DPVX_TRACE_PART=<0..15>"
 #endif
 
 namespace pvx {
@@ -32,6 +32,198 @@ __global__ __launch_bounds__(kBlock) void k_trace_closest_ww(TraceArgs a) {
     __shared__ int32_t s_stack[kStackLds * kBlock];
     trace_closest_body<0, kFlags, kCount, true>(a.sv, nullptr, nullptr, nullptr, a.rays, a.max_depth, a.kind,
                                                 a.kinds, a.n, a.hits, a.spill, a.node_visits, a.prim_tests, s_stack);
+}
+
+// Light rays of pvx_render made in the kernel (triangle / sphere scenes,
+// every material opaque): occlusion test, while-while walk, a visibility
+// byte per ray.
+template <bool kFlags>
+__global__ __launch_bounds__(kBlock, 6) void k_light_ww(TraceArgs a) {
+    __shared__ int32_t s_stack[kStackLds * kBlock];
+    trace_closest_body<0, kFlags, false, true, false, true>(a.sv, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                                            PVX_RAY_SHADOW | kRayOcclusion, nullptr, a.n, nullptr,
+                                                            a.spill, nullptr, nullptr, s_stack, a.gen_groups,
+                                                            a.gen_lights, a.gen_S, a.gen_vis);
+}
+
+// Light rays made in the kernel with refill: once no more than
+// a.gen_refill_below lanes of a wavefront still hold rays, its free lanes
+// take the next unstarted rays (one atomic per wavefront on a launch-wide
+// counter), instead of waiting for the wavefront's slowest ray (Aila and
+// Laine, "Understanding the efficiency of ray traversal on GPUs", HPG 2009:
+// persistent threads with fetching and ray replacement).  Same rays, same
+// tests and same visibility bytes as k_light_ww.
+template <bool kFlags>
+__global__ __launch_bounds__(kBlock, 6) void k_light_dyn(TraceArgs a) {
+    __shared__ int32_t s_stack[kStackLds * kBlock];
+    const SceneView &sv = a.sv;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t gtid = blockIdx.x * kBlock + lane;
+    int32_t *my_spill = a.spill + (size_t)gtid * kStackSpill;
+    const uint32_t wl = lane & 63u;
+    bool have = false, done = false;
+    uint32_t ri = 0;
+    D3 o{0, 0, 0}, d{0, 0, 1};
+    double bt = 0.0, occl_lim = 0.0;
+    int32_t bo = -1, bp = 0;
+    RayF rf{};
+    float tb = 0.0f;
+    int32_t node = kEmpty;
+    int sp = 0;
+    auto push = [&](int32_t x) {
+        if (sp < kStackLds)
+            s_stack[sp * kBlock + lane] = x;
+        else
+            my_spill[sp - kStackLds] = x;
+        sp++;
+    };
+    auto pop = [&]() {
+        if (sp == 0) {
+            node = kEmpty;
+        } else {
+            sp--;
+            node = sp < kStackLds ? s_stack[sp * kBlock + lane] : my_spill[sp - kStackLds];
+        }
+    };
+    while (true) {
+        // hand out new rays to the lanes that have none
+        const uint64_t need = __ballot(!have && !done);
+        if (need && (uint32_t)__popcll(__ballot(have)) <= a.gen_refill_below) {
+            const int leader = __ffsll((unsigned long long)need) - 1;
+            uint32_t base = 0;
+            if ((int)wl == leader) base = atomicAdd(a.gen_next, (uint32_t)__popcll(need));
+            base = (uint32_t)__shfl((int)base, leader);
+            if (!have && !done) {
+                // rank among the lanes that asked: those below this one in `need`
+                ri = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(need >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
+                if (ri >= a.n) {
+                    done = true;
+                } else {
+                    const uint32_t g = ri / a.gen_S, sidx = ri - g * a.gen_S;
+                    const LightGroup grp = a.gen_groups[g];
+                    const LightRec &lt = a.gen_lights[grp.light];
+                    if (sidx < lt.n1 * lt.n2) { // else: past this light's grid, take another
+                        o = D3{grp.ip[0], grp.ip[1], grp.ip[2]};
+                        double md;
+                        light_sample_ray(lt, sidx / lt.n2, sidx % lt.n2, o, d, md);
+                        bt = md >= kEpsilon ? md : kBoundHuge; // trace.cpp:157-158
+                        occl_lim = bt - kShadowTolerance;
+                        bo = -1;
+                        bp = 0;
+                        for (uint32_t q = 0; q < sv.n_inf; q++) { // planes
+                            const InfRec inf = sv.inf[q];
+                            if (kFlags && !precondition(sv.obj_flags[inf.obj], PVX_RAY_SHADOW)) continue;
+                            double dep;
+                            if (inf.kind == kPrimPlane && plane_hit(sv.planes[inf.index], o, d, dep) &&
+                                depth_ok(dep, true) && better(dep, inf.obj, 0, bt, bo, bp)) {
+                                bt = dep;
+                                bo = inf.obj;
+                                bp = 0;
+                            }
+                        }
+                        rf.ix = safe_inv(d.x);
+                        rf.iy = safe_inv(d.y);
+                        rf.iz = safe_inv(d.z);
+                        rf.oix = (float)o.x * rf.ix;
+                        rf.oiy = (float)o.y * rf.iy;
+                        rf.oiz = (float)o.z * rf.iz;
+                        tb = (float)bt * 1.000001f;
+                        node = (bo >= 0 && bt < occl_lim) ? kEmpty : sv.root4;
+                        sp = 0;
+                        have = true;
+                    }
+                }
+            }
+        }
+        if (!__ballot(have)) {
+            if (!__ballot(!done)) break; // every lane out of rays
+            continue;
+        }
+        if (have) {
+            // one while-while round: inner nodes, then leaves
+            while (node >= 0) {
+                const float4 *np = reinterpret_cast<const float4 *>(sv.nodes4 + node);
+                const float4 xl = np[0], xh = np[1], yl = np[2], yh = np[3], zl = np[4], zh = np[5];
+                const int4 cl = reinterpret_cast<const int4 *>(np)[6];
+                auto slab = [&](float lx, float hx, float ly, float hy, float lz, float hz, int32_t link) {
+                    const float x0 = fmaf(lx, rf.ix, -rf.oix), x1 = fmaf(hx, rf.ix, -rf.oix);
+                    const float y0 = fmaf(ly, rf.iy, -rf.oiy), y1 = fmaf(hy, rf.iy, -rf.oiy);
+                    const float z0 = fmaf(lz, rf.iz, -rf.oiz), z1 = fmaf(hz, rf.iz, -rf.oiz);
+                    const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
+                    const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tb));
+                    return (link != kEmpty && tn <= tf * 1.000001f) ? tn : INFINITY;
+                };
+                float d0 = slab(xl.x, xh.x, yl.x, yh.x, zl.x, zh.x, cl.x);
+                float d1 = slab(xl.y, xh.y, yl.y, yh.y, zl.y, zh.y, cl.y);
+                float d2 = slab(xl.z, xh.z, yl.z, yh.z, zl.z, zh.z, cl.z);
+                float d3 = slab(xl.w, xh.w, yl.w, yh.w, zl.w, zh.w, cl.w);
+                int32_t l0 = cl.x, l1 = cl.y, l2 = cl.z, l3 = cl.w;
+                auto cx = [](float &da, int32_t &la, float &db, int32_t &lb) {
+                    if (db < da) {
+                        const float td = da;
+                        da = db;
+                        db = td;
+                        const int32_t tl = la;
+                        la = lb;
+                        lb = tl;
+                    }
+                };
+                cx(d0, l0, d1, l1);
+                cx(d2, l2, d3, l3);
+                cx(d0, l0, d2, l2);
+                cx(d1, l1, d3, l3);
+                cx(d1, l1, d2, l2);
+                if (d0 == INFINITY) {
+                    pop();
+                    continue;
+                }
+                if (d3 != INFINITY) push(l3);
+                if (d2 != INFINITY) push(l2);
+                if (d1 != INFINITY) push(l1);
+                node = l0;
+            }
+            while (node < 0 && node != kEmpty) {
+                const uint32_t leaf = (uint32_t)(~node);
+                const uint32_t first = leaf >> 3, count = (leaf & 7u) + 1u;
+                for (uint32_t k = 0; k < count; k++) {
+                    const PrimRec &p = sv.prims[first + k];
+                    const uint32_t ka = p.kind_axis;
+                    const uint32_t pk = ka & 15u;
+                    const int32_t obj = p.obj;
+                    if (kFlags && !precondition(sv.obj_flags[obj], PVX_RAY_SHADOW)) continue;
+                    if (pk == kPrimTriangle) {
+                        double dep;
+                        if (mesh_tri_hit_l<0>(sv, p, ka, o, d, dep) && depth_ok(dep, true) &&
+                            better(dep, obj, p.tri.prim, bt, bo, bp)) {
+                            bt = dep;
+                            bo = obj;
+                            bp = p.tri.prim;
+                        }
+                    } else if (pk == kPrimSphere) {
+                        double hs[2];
+                        const int nh = sphere_hits(d3(p.sph.c[0], p.sph.c[1], p.sph.c[2]), p.sph.r, o, d, hs);
+#pragma unroll
+                        for (int j = 0; j < 2; j++) // unrolled: hs stays in registers
+                            if (j < nh && depth_ok(hs[j], true) && better(hs[j], obj, 0, bt, bo, bp)) {
+                                bt = hs[j];
+                                bo = obj;
+                                bp = 0;
+                            }
+                    }
+                }
+                tb = (float)bt * 1.000001f;
+                if (bo >= 0 && bt < occl_lim) {
+                    node = kEmpty; // blocked: the occlusion test is decided
+                    break;
+                }
+                pop();
+            }
+            if (node == kEmpty) { // TracePointLightShadowRay with an opaque blocker (trace.cpp:2021-2073)
+                a.gen_vis[ri] = (bo >= 0 && bt < occl_lim && bt > kShadowTolerance) ? 0 : 1;
+                have = false;
+            }
+        }
+    }
 }
 
 // The same kernels over a batch of scenes (chunk table and SceneView table).
@@ -341,6 +533,10 @@ hipError_t PVX_FN(PVX_TRACE_PART)(const TraceKernel &k, const TraceArgs &a, uint
     return PVX_FC(k_trace_closest, 3, );
 #elif PVX_TRACE_PART == 14 // scene batches with lathes or sors
     return PVX_FC(k_trace_closest_multi, 3, );
+#elif PVX_TRACE_PART == 15 // light rays made in the kernel
+    if (k.walk == kWalkLightDyn)
+        return k.flags ? run(k_light_dyn<true>, a, nb, st, occ) : run(k_light_dyn<false>, a, nb, st, occ);
+    return k.flags ? run(k_light_ww<true>, a, nb, st, occ) : run(k_light_ww<false>, a, nb, st, occ);
 #else
 #error "PVX_TRACE_PART out of range"
 #endif

@@ -3,7 +3,7 @@
 // The traversal kernels are templates over the walk, the object-flag
 // precondition, the counting mode and single/multi-scene batches.  They are
 // instantiated in trace_kernels.hip, which the Makefile compiles once per
-// part (PVX_TRACE_PART = 0..14), each part holding a few instantiations, so
+// part (PVX_TRACE_PART = 0..15), each part holding a few instantiations, so
 // that they build in parallel; trace_launch() picks the part.
 #ifndef PVX_TRACE_LAUNCH_H
 #define PVX_TRACE_LAUNCH_H
@@ -35,7 +35,9 @@ enum TraceWalk : uint32_t {
     kWalkBinary = 2,
     kWalkComplex = 3,
     kWalkKd = 4,
-    kWalkRevolution = 5
+    kWalkRevolution = 5,
+    kWalkLightWW = 6, // light rays made in the kernel, 4-wide tree while-while (occlusion)
+    kWalkLightDyn = 7 // the same, each lane taking a new ray as soon as its ray ends
 };
 
 struct TraceKernel {
@@ -63,6 +65,16 @@ struct TraceArgs {
     int32_t *spill; // kSpillPerThread bytes for each thread of the grid
     uint32_t *node_visits;
     uint32_t *prim_tests;
+    // kWalkLightWW: ray k goes from gen_groups[k / gen_S].ip to grid point
+    // k mod gen_S of light gen_lights[gen_groups[k / gen_S].light] (rays past
+    // that light's grid are skipped) with that distance as its max depth; its
+    // visibility (1: nothing blocks) is written to gen_vis[k]
+    const LightGroup *gen_groups;
+    const LightRec *gen_lights;
+    uint32_t gen_S;
+    uint8_t *gen_vis;
+    uint32_t *gen_next; // kWalkLightDyn: next ray to hand out (zero at launch)
+    uint32_t gen_refill_below; // kWalkLightDyn: refill once no more than this many lanes hold rays
 };
 
 // Launches kernel `k` as `nblocks` blocks of kBlock threads on `st`; with
@@ -97,6 +109,7 @@ PVX_TRACE_PART_DECL(10);
 PVX_TRACE_PART_DECL(11);
 PVX_TRACE_PART_DECL(13);
 PVX_TRACE_PART_DECL(14);
+PVX_TRACE_PART_DECL(15);
 
 } // namespace pvx
 
