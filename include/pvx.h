@@ -487,6 +487,16 @@ typedef struct pvx_camera {
 /* Writes width*height rays (row-major, y down) into device buffer `rays`. */
 int pvx_camera_rays(const pvx_camera *cam, pvx_ray *rays, void *stream);
 
+/* The same rays in tile order: tiles of tile_w x tile_h pixels, the tiles
+ * row-major and the pixels row-major inside each (the last tile column and
+ * row hold only the pixels inside the image), and, when `pixel` is not
+ * NULL, each ray's pixel y * width + x.  The reference traces pixels in
+ * blocks of 32 x 32 (backend/scene/view.cpp:258-261, 810); 8 x 8 tiles keep
+ * the 64 rays of a wavefront together on the screen (closest hit on the
+ * 1.3M-triangle frame: 0.672 ms against 0.754 ms in row-major order). */
+int pvx_camera_rays_tiled(const pvx_camera *cam, uint32_t tile_w, uint32_t tile_h, pvx_ray *rays,
+                          uint32_t *pixel, void *stream);
+
 /* ---- shading (plain finishes) ------------------------------------------ */
 /* Surface description of one top-level object: pigment { rgb filter
  * transmit }, finish { ambient diffuse brilliance phong phong_size

@@ -69,6 +69,29 @@ __global__ void k_camera_rays(pvx_camera cam, pvx_ray *rays) {
     rays[(size_t)y * cam.width + x] = r;
 }
 
+// The same rays in tile order (tile_pixel), with each ray's pixel y * W + x.
+__global__ void k_camera_rays_tiled(pvx_camera cam, uint32_t tw, uint32_t th, pvx_ray *rays, uint32_t *pixel) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cam.width * cam.height) return;
+    uint32_t x, y;
+    tile_pixel(cam.width, cam.height, tw, th, t, x, y);
+    double x0 = ((double)x + cam.jitter_x) / (double)cam.width - 0.5;
+    double y0 = 0.5 - ((double)y + cam.jitter_y) / (double)cam.height;
+    double dx = cam.direction[0] + x0 * cam.right[0] + y0 * cam.up[0];
+    double dy = cam.direction[1] + x0 * cam.right[1] + y0 * cam.up[1];
+    double dz = cam.direction[2] + x0 * cam.right[2] + y0 * cam.up[2];
+    double len = sqrt(dx * dx + dy * dy + dz * dz);
+    pvx_ray r;
+    r.o[0] = cam.location[0];
+    r.o[1] = cam.location[1];
+    r.o[2] = cam.location[2];
+    r.d[0] = dx / len;
+    r.d[1] = dy / len;
+    r.d[2] = dz / len;
+    rays[t] = r;
+    if (pixel) pixel[t] = y * cam.width + x;
+}
+
 __global__ void k_copy16(uint4 *__restrict__ dst, const uint4 *__restrict__ src, size_t n16) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16;
          i += (size_t)gridDim.x * blockDim.x)
@@ -856,6 +879,20 @@ extern "C" int pvx_camera_rays(const pvx_camera *cam, pvx_ray *rays, void *strea
     hipStream_t st = static_cast<hipStream_t>(stream);
     dim3 grid((cam->width + 255) / 256, cam->height);
     k_camera_rays<<<grid, 256, 0, st>>>(*cam, rays);
+    HIP_TRY(hipGetLastError());
+    return PVX_OK;
+}
+
+extern "C" int pvx_camera_rays_tiled(const pvx_camera *cam, uint32_t tile_w, uint32_t tile_h, pvx_ray *rays,
+                                     uint32_t *pixel, void *stream) {
+    if (!cam || !rays) return fail(PVX_EINVAL, "null argument");
+    if (cam->width == 0 || cam->height == 0 || (uint64_t)cam->width * cam->height > 0xFFFFFFFFull)
+        return fail(PVX_EINVAL, "bad image size");
+    if (tile_w == 0 || tile_h == 0 || (uint64_t)tile_h * cam->width > 0xFFFFFFFFull)
+        return fail(PVX_EINVAL, "bad tile size");
+    const uint32_t n = cam->width * cam->height;
+    k_camera_rays_tiled<<<(n + 255) / 256, 256, 0, static_cast<hipStream_t>(stream)>>>(*cam, tile_w, tile_h, rays,
+                                                                                        pixel);
     HIP_TRY(hipGetLastError());
     return PVX_OK;
 }

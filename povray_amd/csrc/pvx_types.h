@@ -51,6 +51,32 @@ struct alignas(16) Node {
 };
 static_assert(sizeof(Node) == 64, "node is one 64 B record");
 
+// Pixel of position t in tile order: tiles of tw x th pixels in row-major
+// tile order, row-major inside each tile; the last tile column and row hold
+// only the pixels inside the W x H image.  (The reference renders blocks of
+// 32 x 32 pixels, backend/scene/view.cpp:258-261, 810; smaller tiles here
+// keep the 64 rays of a wavefront close together on the screen.)
+#ifdef __HIPCC__
+__host__ __device__ inline void tile_pixel(uint32_t W, uint32_t H, uint32_t tw, uint32_t th, uint32_t t,
+                                           uint32_t &x, uint32_t &y) {
+    const uint32_t row = W * th;   // pixels in a full row of tiles
+    const uint32_t tr = t / row;   // tile row (all before it are full)
+    const uint32_t k = t - tr * row;
+    const uint32_t thr = H - tr * th < th ? H - tr * th : th; // rows in this tile row
+    const uint32_t full = W / tw, per = tw * thr;
+    const uint32_t tx = k / per;
+    if (tx < full) {
+        const uint32_t o = k - tx * per;
+        x = tx * tw + o % tw;
+        y = tr * th + o / tw;
+    } else { // the narrower last tile of the row
+        const uint32_t o = k - full * per, wr = W - full * tw;
+        x = full * tw + o % wr;
+        y = tr * th + o / wr;
+    }
+}
+#endif
+
 // 4-wide node, 128 B (triangle / sphere scenes): the boxes of up to four
 // children as six float4 rows (x lo, x hi, y lo, y hi, z lo, z hi), their
 // links (same encoding as Node; INT32_MIN = no child in this slot).

@@ -87,14 +87,21 @@ __global__ void k_zero_word(uint32_t *c) {
     if (threadIdx.x == 0) *c = 0;
 }
 
-// Camera rays of sample s of pixel p at queue position p * S + s
+// The queue holds the pixels in tile order (tile_pixel with 8 x 8 tiles:
+// a wavefront's rays stay together on the screen; 8 x 8 tiles measured the
+// fastest closest-hit order on the 1.3M-triangle frame), a pixel's samples
+// in a run: camera ray of sample s of the pixel at tile position p sits at
+// queue position p * S + s.
+constexpr uint32_t kRenderTile = 8;
+
 // (TracePixel::CreateCameraRay perspective branch, tracepixel.cpp:357-372).
 __global__ void k_sample_rays(pvx_camera cam, const double *__restrict__ off, uint32_t S, uint32_t n,
                               pvx_ray *__restrict__ rays) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const uint32_t px = i / S, s = i % S;
-    const uint32_t x = px % cam.width, y = px / cam.width;
+    const uint32_t s = i % S;
+    uint32_t x, y;
+    tile_pixel(cam.width, cam.height, kRenderTile, kRenderTile, i / S, x, y);
     double x0 = ((double)x + off[2 * s]) / (double)cam.width - 0.5;
     double y0 = 0.5 - ((double)y + off[2 * s + 1]) / (double)cam.height;
     double dx = cam.direction[0] + x0 * cam.right[0] + y0 * cam.up[0];
@@ -116,11 +123,13 @@ __global__ void k_scale3(float *rgb, size_t n3, float k) {
     if (i < n3) rgb[i] *= k;
 }
 
-__global__ void k_init_paths(uint32_t n, uint32_t S, PathState *st, uint8_t *kinds) {
+__global__ void k_init_paths(uint32_t n, uint32_t S, uint32_t W, uint32_t H, PathState *st, uint8_t *kinds) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     PathState ps;
-    ps.pixel = i / S;
+    uint32_t x, y;
+    tile_pixel(W, H, kRenderTile, kRenderTile, i / S, x, y);
+    ps.pixel = y * W + x;
     ps.level = 0;
     ps.flags = kFPrimary;
     ps.n_in = 0;
@@ -875,7 +884,7 @@ static int render_frame(pvx_scene *s, const pvx_camera *cam, uint32_t S, const d
     uint32_t n = (uint32_t)nray;
     k_sample_rays<<<(n + 255) / 256, 256, 0, st>>>(*cam, static_cast<const double *>(b.offsets.p), S, n,
                                                    static_cast<pvx_ray *>(b.rays[0].p));
-    k_init_paths<<<(n + 255) / 256, 256, 0, st>>>(n, S, static_cast<PathState *>(b.state[0].p),
+    k_init_paths<<<(n + 255) / 256, 256, 0, st>>>(n, S, cam->width, cam->height, static_cast<PathState *>(b.state[0].p),
                                                    static_cast<uint8_t *>(b.kinds[0].p));
     int rc = PVX_OK;
     int cur = 0;

@@ -50,6 +50,7 @@ def lib():
         L.pvx_all_intersections.argtypes = [vp, vp, vp, sz, u32, vp, vp, vp, vp]
         L.pvx_inside.argtypes = [vp, vp, vp, sz, vp, vp]
         L.pvx_camera_rays.argtypes = [C.POINTER(A.PvxCamera), vp, vp]
+        L.pvx_camera_rays_tiled.argtypes = [C.POINTER(A.PvxCamera), C.c_uint32, C.c_uint32, vp, vp, vp]
         L.pvx_copy.argtypes = [vp, vp, sz, vp]
         L.pvx_scene_set_shading.argtypes = [vp, C.POINTER(A.PvxShadingDesc)]
         L.pvx_render.argtypes = [vp, C.POINTER(A.PvxCamera), vp, C.POINTER(A.PvxRenderStats), vp]
@@ -90,7 +91,7 @@ EXPORTED_SYMBOLS = (
     "pvx_last_error", "pvx_init", "pvx_scene_create", "pvx_scene_destroy",
     "pvx_scene_get_stats", "pvx_trace_closest", "pvx_trace_closest_counted",
     "pvx_all_intersections", "pvx_inside",
-    "pvx_camera_rays", "pvx_copy", "pvx_scene_set_shading", "pvx_render", "pvx_render_samples",
+    "pvx_camera_rays", "pvx_camera_rays_tiled", "pvx_copy", "pvx_scene_set_shading", "pvx_render", "pvx_render_samples",
     "pvx_radiosity_render", "pvx_surrogate_create", "pvx_surrogate_destroy",
     "pvx_surrogate_get_params", "pvx_surrogate_grad", "pvx_surrogate_train", "pvx_surrogate_infer",
     "pvx_batch_create", "pvx_batch_destroy", "pvx_batch_trace_closest", "pvx_batch_trace_closest_counted",
@@ -507,14 +508,22 @@ def m1_offsets(aa_depth: int = 2):
     return np.array([[0.5, 0.5]] + [[0.5 + xx, 0.5 + yy] for yy in grid for xx in grid])
 
 
-def camera_rays(scene, width=None, height=None, jitter=(0.5, 0.5), out=None, stream=None):
-    """Primary rays for the scene's camera, generated on the GPU."""
+def camera_rays(scene, width=None, height=None, jitter=(0.5, 0.5), out=None, stream=None, tile=None,
+                pixel=None):
+    """Primary rays for the scene's camera, generated on the GPU: row-major,
+    or with tile=(w, h) in tile order (pvx_camera_rays_tiled; `pixel`, a
+    device buffer of n uint32, receives each ray's pixel index)."""
     cam = scene.camera_struct(jitter=jitter, width=width, height=height)
     n = cam.width * cam.height
     if out is None:
         out = hip.DeviceBuffer(n * 48)
-    _check(lib().pvx_camera_rays(C.byref(cam), out.value, stream.h if stream else None),
-           "pvx_camera_rays")
+    if tile is None:
+        _check(lib().pvx_camera_rays(C.byref(cam), out.value, stream.h if stream else None),
+               "pvx_camera_rays")
+    else:
+        _check(lib().pvx_camera_rays_tiled(C.byref(cam), int(tile[0]), int(tile[1]), out.value,
+                                           pixel.value if pixel is not None else None,
+                                           stream.h if stream else None), "pvx_camera_rays_tiled")
     return out, n
 
 
