@@ -620,6 +620,9 @@ typedef struct pvx_radiosity_settings {
     uint32_t train_steps;      /* learned: optimiser steps per network          */
     uint32_t probe_tile;       /* learned: probe tile edge in pixels (32)       */
     uint32_t probes_per_tile;  /* learned: plain gathers probed per tile (2)    */
+    uint32_t train_batch;      /* learned: samples per optimiser step, a
+                                * multiple of 256 (0: 2048); the learning
+                                * rate grows with its square root            */
 } pvx_radiosity_settings;
 
 typedef struct pvx_radiosity_stats {

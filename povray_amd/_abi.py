@@ -166,7 +166,8 @@ class PvxRadiositySettings(C.Structure):
                 ("pretrace_end", C.c_float), ("brightness", C.c_float),
                 ("adc_bailout", C.c_float), ("probe_threshold", C.c_float),
                 ("train_samples", C.c_uint32), ("train_steps", C.c_uint32),
-                ("probe_tile", C.c_uint32), ("probes_per_tile", C.c_uint32)]
+                ("probe_tile", C.c_uint32), ("probes_per_tile", C.c_uint32),
+                ("train_batch", C.c_uint32)]
 
 
 class PvxRadiosityStats(C.Structure):
@@ -190,7 +191,7 @@ def radiosity_settings(scene, mode=PVX_RADIOSITY_PLAIN, **over):
     cfg = dict(count=35, recursion_limit=2, nearest_count=5, error_bound=1.8,
                minimum_reuse=0.015, maximum_reuse=0.2, pretrace_start=0.08, pretrace_end=0.04,
                brightness=1.0, adc_bailout=0.01, probe_threshold=0.02, train_samples=8192,
-               train_steps=1500, probe_tile=32, probes_per_tile=2)
+               train_steps=1500, probe_tile=32, probes_per_tile=2, train_batch=0)
     cfg.update(scene.settings.get("radiosity", {}))
     cfg.update(over)
     st = PvxRadiositySettings()

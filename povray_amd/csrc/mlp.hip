@@ -152,7 +152,27 @@ __device__ inline uint32_t mix32(uint32_t x) {
 __device__ inline void load_pack(const uint8_t *__restrict__ pack, uint16_t *sw, int welems, float *sb) {
     const uint4 *src = reinterpret_cast<const uint4 *>(pack);
     uint4 *dst = reinterpret_cast<uint4 *>(sw);
-    for (int i = threadIdx.x; i < welems * 2 / 16; i += blockDim.x) dst[i] = src[i];
+    // eight 16-byte loads in flight per lane before the LDS stores: the pack
+    // was just rewritten by other workgroups, so each load waits the full
+    // memory latency, once per batch instead of once per 16 bytes
+    constexpr int kInFlight = 8;
+    const int n16 = welems * 2 / 16;
+    const int step = kInFlight * (int)blockDim.x;
+    int i0 = threadIdx.x;
+    for (; i0 + step - (int)blockDim.x < n16; i0 += step) { // whole batches
+        uint4 v0 = src[i0], v1 = src[i0 + blockDim.x], v2 = src[i0 + 2 * blockDim.x], v3 = src[i0 + 3 * blockDim.x];
+        uint4 v4 = src[i0 + 4 * blockDim.x], v5 = src[i0 + 5 * blockDim.x], v6 = src[i0 + 6 * blockDim.x],
+              v7 = src[i0 + 7 * blockDim.x];
+        dst[i0] = v0;
+        dst[i0 + blockDim.x] = v1;
+        dst[i0 + 2 * blockDim.x] = v2;
+        dst[i0 + 3 * blockDim.x] = v3;
+        dst[i0 + 4 * blockDim.x] = v4;
+        dst[i0 + 5 * blockDim.x] = v5;
+        dst[i0 + 6 * blockDim.x] = v6;
+        dst[i0 + 7 * blockDim.x] = v7;
+    }
+    for (; i0 < n16; i0 += blockDim.x) dst[i0] = src[i0]; // the rest
     const float *bsrc = reinterpret_cast<const float *>(pack + (size_t)kAllElems * 2);
     for (int i = threadIdx.x; i < kBiasFloats; i += blockDim.x) sb[i] = bsrc[i];
 }

@@ -1206,10 +1206,13 @@ int RadiosityEngine::train_on(Surrogate &nt, const RadRec *recs, uint32_t n, uin
     HIP_TRY(hipEventCreate(&t0));
     HIP_TRY(hipEventCreate(&t1));
     HIP_TRY(hipEventRecord(t0, stream));
-    const uint32_t batch = 2048; // samples drawn with replacement from the n records
+    // samples drawn with replacement from the n records per step; the rate
+    // grows with the square root of the batch
+    const uint32_t batch = rs.train_batch ? rs.train_batch : 2048;
+    const float lr = 5e-3f * std::sqrt((float)batch / 2048.0f);
     if ((r = nt.init(nullptr, seed, stream)) ||
         (r = nt.train(static_cast<const float *>(lx.p), static_cast<const float *>(lt.p), n, rs.train_steps, batch,
-                      5e-3f, loss, stream))) {
+                      lr, loss, stream))) {
         (void)hipEventDestroy(t0);
         (void)hipEventDestroy(t1);
         return r;
@@ -1673,6 +1676,8 @@ static int check_settings(pvx_scene *s, const pvx_radiosity_settings *set) {
     if (set->mode == PVX_RADIOSITY_LEARNED &&
         (set->probe_tile == 0 || set->train_steps == 0 || set->train_samples == 0 || !(set->probe_threshold >= 0.0f)))
         return fail(PVX_EINVAL, "learned radiosity needs probe_tile, train_steps, train_samples > 0");
+    if (set->mode == PVX_RADIOSITY_LEARNED && set->train_batch % 256)
+        return fail(PVX_EINVAL, "train_batch must be a multiple of 256");
     for (const MatRec &m : s->shading.mats)
         if (m.reflective)
             return fail(PVX_EUNSUPPORTED, "radiosity frames take plain diffuse / emissive finishes only");

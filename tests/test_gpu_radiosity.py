@@ -143,3 +143,45 @@ def test_learned_is_deterministic_and_probes_guard_a_bad_network(gpu):
     assert sbad["probe_error"] > sa["probe_error"]
     assert sbad["fallbacks"] > sa["fallbacks"]
     assert _rmse(bad, plain) < 2.0 * _rmse(a, plain) + 2.0 / 255
+
+
+def test_learned_frame_three_times_faster_at_full_size(gpu):
+    """The full radiosity room (5,000 boxes, 1280x720, count 200,
+    recursion_limit 2, error_bound 0.5): learned frames at least 3x faster
+    than plain frames (best of three each, after a warm frame), with the
+    learned frame no further from a converged one than 1.15x the plain's
+    distance."""
+    s = scenes.radiosity_scene()
+    g = _gpu_scene(gpu, s)
+    g.radiosity_render()
+    plains = [g.radiosity_render() for _ in range(3)]
+    plain, ps = min(plains, key=lambda x: x[1]["frame_ms"])
+    _learned(g)
+    learns = [_learned(g) for _ in range(3)]
+    img, ls = min(learns, key=lambda x: x[1]["frame_ms"])
+    conv, _ = g.radiosity_render(count=1600, error_bound=0.2, nearest_count=10)
+    e_plain, e_learned = _rmse(plain, conv), _rmse(img, conv)
+    print(f"[rad-full] plain {ps['frame_ms']:.1f} ms, learned {ls['frame_ms']:.1f} ms "
+          f"(x{ps['frame_ms'] / ls['frame_ms']:.2f}, train {ls['train_ms']:.2f} ms); rmse vs converged "
+          f"{e_plain * 255:.2f} / {e_learned * 255:.2f} per 255", flush=True)
+    assert ps["frame_ms"] >= 3.0 * ls["frame_ms"]
+    assert e_learned <= 1.15 * e_plain
+
+
+def test_train_batch_setting(gpu):
+    """train_batch: samples per optimiser step (a multiple of 256; 0 keeps
+    2048); other values are refused, and a larger batch with fewer steps
+    still trains a network the probes accept somewhere."""
+    from povray_amd import _abi as A
+
+    s = scenes.radiosity_scene(n_boxes=500, width=320, height=180)
+    g = _gpu_scene(gpu, s)
+    with pytest.raises(gpu.PvxError):
+        _learned(g, train_batch=1000)
+    img, st = _learned(g, train_batch=4096, train_steps=25)
+    plain, _ = g.radiosity_render()
+    print(f"[rad-batch] 4096 x 25: train {st['train_ms']:.2f} ms loss {st['train_loss']:.4f} "
+          f"network {st['surrogate_queries']} fallback {st['fallbacks']} rmse vs plain "
+          f"{_rmse(img, plain) * 255:.2f}/255", flush=True)
+    assert st["surrogate_queries"] > 0 and np.isfinite(st["train_loss"])
+    assert A.radiosity_settings(s, mode=A.PVX_RADIOSITY_LEARNED).train_batch == 0
