@@ -397,6 +397,9 @@ typedef struct pvx_tune_result {
     uint32_t grid_blocks;  /* chosen persistent grid (blocks of 256 lanes)  */
     double best_ms;        /* its time per launch on the sample              */
     double default_ms;     /* the untuned choice's time on the same sample   */
+    uint32_t tile_w;       /* pvx_render_autotune's camera rays: the chosen  */
+    uint32_t tile_h;       /* pixel tile of the ray queue (0 x 0 otherwise;
+                              tile_w 0 with tile_h 1: row-major)             */
 } pvx_tune_result;
 
 /* Autotuner: times the traversal variants this scene admits on persistent
@@ -415,6 +418,14 @@ int pvx_scene_autotune(pvx_scene *scene, const pvx_ray *rays, size_t n, uint32_t
  * twice the resident blocks.  Not to be called while launches on the scene
  * are being issued from another thread. */
 int pvx_scene_set_traversal(pvx_scene *scene, uint32_t ray_kind, uint32_t variant, uint32_t grid_blocks);
+
+/* Pixel order of pvx_render's ray queue: tiles of tile_w x tile_h pixels
+ * (tile order as pvx_camera_rays_tiled), or row-major with tile_w 0 and
+ * tile_h 1.  8 x 8 by default; pvx_render_autotune picks among 8 x 8,
+ * 16 x 4, 32 x 2 and row-major (reported in its tune[0]).  The image does
+ * not depend on the order.  No counterpart in the reference, whose render
+ * blocks are 32 x 32 pixels (backend/scene/view.cpp:258-261, 810). */
+int pvx_scene_set_render_tile(pvx_scene *scene, uint32_t tile_w, uint32_t tile_h);
 
 /* Per-object intersection list for n (ray, object) pairs.
  * Replaces virtual bool ObjectBase::All_Intersections(const Ray&, IStack&,
@@ -586,8 +597,11 @@ int pvx_render_samples(pvx_scene *scene, const pvx_camera *cam, uint32_t n_sampl
                        float *rgb, pvx_render_stats *stats, void *stream);
 
 /* pvx_render that first runs pvx_scene_autotune on the frame's own camera
- * rays (tune[0]) and first wavefront of light rays (tune[1], all zero when
- * the frame has none); the image is the one pvx_render gives.  Where light
+ * rays, generated in each candidate pixel order of the ray queue (8 x 8,
+ * 16 x 4, 32 x 2 tiles and row-major; the fastest {tile, walk, grid} is
+ * kept, tune[0] with its tile), and on its first wavefront of light rays
+ * (tune[1], all zero when the frame has none); the image is the one
+ * pvx_render gives.  Where light
  * rays are made in the kernel (PVX_TRAVERSAL_PACKET / FUSED / REFILL), the
  * walk and grid are timed on the light rays of camera-ray hits (reported in
  * tune[1]) and again on those of the first bounce, each set keeping its own

@@ -106,7 +106,8 @@ PVX_TRAVERSAL_PACKET, PVX_TRAVERSAL_FUSED, PVX_TRAVERSAL_REFILL = 4, 5, 6
 
 class PvxTuneResult(C.Structure):
     _fields_ = [("variant", C.c_uint32), ("grid_blocks", C.c_uint32),
-                ("best_ms", C.c_double), ("default_ms", C.c_double)]
+                ("best_ms", C.c_double), ("default_ms", C.c_double),
+                ("tile_w", C.c_uint32), ("tile_h", C.c_uint32)]
 
 
 class PvxSceneStats(C.Structure):

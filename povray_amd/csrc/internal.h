@@ -81,6 +81,8 @@ struct pvx_scene {
     // those of camera-ray hits [0] and of later bounces [1] (0: the default)
     uint32_t light_variant[2] = {0, 0};
     uint32_t light_grid[2] = {0, 0};
+    // pixel tile of pvx_render's ray queue (tile_w 0: row-major)
+    uint32_t render_tile[2] = {8, 8};
     pvx::KdView kd{};     // kd-tree of a PVX_ACCEL_BSP scene
     bool use_kd = false;
     std::vector<void *> allocs;

@@ -87,21 +87,28 @@ __global__ void k_zero_word(uint32_t *c) {
     if (threadIdx.x == 0) *c = 0;
 }
 
-// The queue holds the pixels in tile order (tile_pixel with 8 x 8 tiles:
-// a wavefront's rays stay together on the screen; 8 x 8 tiles measured the
-// fastest closest-hit order on the 1.3M-triangle frame), a pixel's samples
-// in a run: camera ray of sample s of the pixel at tile position p sits at
-// queue position p * S + s.
-constexpr uint32_t kRenderTile = 8;
+// The queue holds the pixels in tile order (tile_pixel with tw x th tiles,
+// 8 x 8 unless pvx_render_autotune or pvx_scene_set_render_tile chose
+// others; tw 0 is row-major), a pixel's samples in a run: camera ray of
+// sample s of the pixel at tile position p sits at queue position p * S + s.
+__device__ inline void queue_pixel(uint32_t W, uint32_t H, uint32_t tw, uint32_t th, uint32_t p, uint32_t &x,
+                                   uint32_t &y) {
+    if (tw == 0) {
+        x = p % W;
+        y = p / W;
+    } else {
+        tile_pixel(W, H, tw, th, p, x, y);
+    }
+}
 
 // (TracePixel::CreateCameraRay perspective branch, tracepixel.cpp:357-372).
-__global__ void k_sample_rays(pvx_camera cam, const double *__restrict__ off, uint32_t S, uint32_t n,
-                              pvx_ray *__restrict__ rays) {
+__global__ void k_sample_rays(pvx_camera cam, const double *__restrict__ off, uint32_t S, uint32_t n, uint32_t tw,
+                              uint32_t th, pvx_ray *__restrict__ rays) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t s = i % S;
     uint32_t x, y;
-    tile_pixel(cam.width, cam.height, kRenderTile, kRenderTile, i / S, x, y);
+    queue_pixel(cam.width, cam.height, tw, th, i / S, x, y);
     double x0 = ((double)x + off[2 * s]) / (double)cam.width - 0.5;
     double y0 = 0.5 - ((double)y + off[2 * s + 1]) / (double)cam.height;
     double dx = cam.direction[0] + x0 * cam.right[0] + y0 * cam.up[0];
@@ -123,12 +130,13 @@ __global__ void k_scale3(float *rgb, size_t n3, float k) {
     if (i < n3) rgb[i] *= k;
 }
 
-__global__ void k_init_paths(uint32_t n, uint32_t S, uint32_t W, uint32_t H, PathState *st, uint8_t *kinds) {
+__global__ void k_init_paths(uint32_t n, uint32_t S, uint32_t W, uint32_t H, uint32_t tw, uint32_t th, PathState *st,
+                             uint8_t *kinds) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     PathState ps;
     uint32_t x, y;
-    tile_pixel(W, H, kRenderTile, kRenderTile, i / S, x, y);
+    queue_pixel(W, H, tw, th, i / S, x, y);
     ps.pixel = y * W + x;
     ps.level = 0;
     ps.flags = kFPrimary;
@@ -791,8 +799,17 @@ extern "C" int pvx_render_samples(pvx_scene *s, const pvx_camera *cam, uint32_t 
     return render_frame(s, cam, S, offsets, rgb, stats, stream, nullptr);
 }
 
-// One frame that first tunes the closest-hit walk on its own camera rays
-// and on its first wavefront of light rays.
+extern "C" int pvx_scene_set_render_tile(pvx_scene *s, uint32_t tile_w, uint32_t tile_h) {
+    if (!s) return fail(PVX_EINVAL, "null scene");
+    if (tile_w == 0 ? tile_h != 1 : (tile_h == 0 || tile_w > 4096 || tile_h > 4096))
+        return fail(PVX_EINVAL, "tile must be w x h in [1, 4096], or 0 x 1 (row-major)");
+    s->render_tile[0] = tile_w;
+    s->render_tile[1] = tile_h;
+    return PVX_OK;
+}
+
+// One frame that first tunes the queue's pixel tile and the closest-hit walk
+// on its own camera rays, and the light walks on its light rays.
 extern "C" int pvx_render_autotune(pvx_scene *s, const pvx_camera *cam, float *rgb, pvx_tune_result *tune,
                                    pvx_render_stats *stats, void *stream) {
     if (!cam) return fail(PVX_EINVAL, "null argument");
@@ -882,10 +899,40 @@ static int render_frame(pvx_scene *s, const pvx_camera *cam, uint32_t S, const d
         k_fill<<<(unsigned)((n3 + 255) / 256), 256, 0, st>>>(rgb, n3, 0.0f);
     }
     uint32_t n = (uint32_t)nray;
-    k_sample_rays<<<(n + 255) / 256, 256, 0, st>>>(*cam, static_cast<const double *>(b.offsets.p), S, n,
-                                                   static_cast<pvx_ray *>(b.rays[0].p));
-    k_init_paths<<<(n + 255) / 256, 256, 0, st>>>(n, S, cam->width, cam->height, static_cast<PathState *>(b.state[0].p),
-                                                   static_cast<uint8_t *>(b.kinds[0].p));
+    auto sample_rays = [&](uint32_t tw, uint32_t th) {
+        k_sample_rays<<<(n + 255) / 256, 256, 0, st>>>(*cam, static_cast<const double *>(b.offsets.p), S, n, tw, th,
+                                                       static_cast<pvx_ray *>(b.rays[0].p));
+    };
+    if (tune) {
+        // the queue's pixel order: each candidate tile's camera rays are
+        // timed with every walk and grid the scene admits; the fastest
+        // {tile, walk, grid} is kept for this and later frames
+        const uint32_t cands[4][2] = {{8, 8}, {16, 4}, {32, 2}, {0, 1}};
+        pvx_tune_result best{};
+        for (const auto &c : cands) {
+            sample_rays(c[0], c[1]);
+            HIP_TRY(hipGetLastError());
+            pvx_tune_result tr{};
+            if ((r = autotune_internal(s, static_cast<const pvx_ray *>(b.rays[0].p), nullptr, n, PVX_RAY_IMAGE, &tr,
+                                       st)))
+                return r;
+            if (best.best_ms == 0.0 || tr.best_ms < best.best_ms) {
+                best = tr;
+                best.tile_w = c[0];
+                best.tile_h = c[1];
+            }
+        }
+        s->render_tile[0] = best.tile_w;
+        s->render_tile[1] = best.tile_h;
+        s->tune_variant[0] = best.variant;
+        s->tune_grid[0] = best.grid_blocks;
+        tune[0] = best;
+    }
+    const uint32_t tw = s->render_tile[0], th = s->render_tile[1];
+    sample_rays(tw, th);
+    k_init_paths<<<(n + 255) / 256, 256, 0, st>>>(n, S, cam->width, cam->height, tw, th,
+                                                  static_cast<PathState *>(b.state[0].p),
+                                                  static_cast<uint8_t *>(b.kinds[0].p));
     int rc = PVX_OK;
     int cur = 0;
     // with every material opaque a light ray only needs to know whether
@@ -897,8 +944,6 @@ static int render_frame(pvx_scene *s, const pvx_camera *cam, uint32_t S, const d
     while (n > 0 && bounces < max_bounces) {
         const pvx_ray *rays = static_cast<const pvx_ray *>(b.rays[cur].p);
         const PathState *pst = static_cast<const PathState *>(b.state[cur].p);
-        if (tune && bounces == 0 && (rc = autotune_internal(s, rays, nullptr, n, PVX_RAY_IMAGE, &tune[0], st)))
-            return rc;
         HIP_TRY(hipEventRecord(t0, st));
         rc = trace_closest_internal(s, rays, nullptr, bounces == 0 ? (uint32_t)PVX_RAY_IMAGE : (uint32_t)PVX_RAY_REFLECTION,
                                     n, static_cast<pvx_hit *>(b.hits.p), st,

@@ -51,6 +51,7 @@ def lib():
         L.pvx_inside.argtypes = [vp, vp, vp, sz, vp, vp]
         L.pvx_camera_rays.argtypes = [C.POINTER(A.PvxCamera), vp, vp]
         L.pvx_camera_rays_tiled.argtypes = [C.POINTER(A.PvxCamera), C.c_uint32, C.c_uint32, vp, vp, vp]
+        L.pvx_scene_set_render_tile.argtypes = [vp, C.c_uint32, C.c_uint32]
         L.pvx_copy.argtypes = [vp, vp, sz, vp]
         L.pvx_scene_set_shading.argtypes = [vp, C.POINTER(A.PvxShadingDesc)]
         L.pvx_render.argtypes = [vp, C.POINTER(A.PvxCamera), vp, C.POINTER(A.PvxRenderStats), vp]
@@ -91,7 +92,7 @@ EXPORTED_SYMBOLS = (
     "pvx_last_error", "pvx_init", "pvx_scene_create", "pvx_scene_destroy",
     "pvx_scene_get_stats", "pvx_trace_closest", "pvx_trace_closest_counted",
     "pvx_all_intersections", "pvx_inside",
-    "pvx_camera_rays", "pvx_camera_rays_tiled", "pvx_copy", "pvx_scene_set_shading", "pvx_render", "pvx_render_samples",
+    "pvx_camera_rays", "pvx_camera_rays_tiled", "pvx_scene_set_render_tile", "pvx_copy", "pvx_scene_set_shading", "pvx_render", "pvx_render_samples",
     "pvx_radiosity_render", "pvx_surrogate_create", "pvx_surrogate_destroy",
     "pvx_surrogate_get_params", "pvx_surrogate_grad", "pvx_surrogate_train", "pvx_surrogate_infer",
     "pvx_batch_create", "pvx_batch_destroy", "pvx_batch_trace_closest", "pvx_batch_trace_closest_counted",
@@ -245,9 +246,15 @@ class GpuScene:
         return (img.reshape(cam.height, cam.width, 3),
                 {k: getattr(st, k) for k, _ in A.PvxRenderStats._fields_ if k != "pad"})
 
+    def set_render_tile(self, tile_w=8, tile_h=8):
+        """Pixel order of pvx_render's ray queue (pvx_scene_set_render_tile;
+        tile_w 0 with tile_h 1 is row-major)."""
+        _check(lib().pvx_scene_set_render_tile(self.h, tile_w, tile_h), "pvx_scene_set_render_tile")
+
     def render_autotune(self, width=None, height=None):
-        """pvx_render_autotune: one frame that tunes the closest-hit walk on its
-        camera rays and first light rays.  Returns (image, stats, [camera, light] choices)."""
+        """pvx_render_autotune: one frame that tunes the queue's pixel tile and
+        the closest-hit walk on its camera rays, and the light walks on its
+        light rays.  Returns (image, stats, [camera, light] choices)."""
         cam = self.scene.camera_struct(width=width, height=height)
         out = hip.DeviceBuffer(cam.width * cam.height * 12)
         st = A.PvxRenderStats()

@@ -45,3 +45,28 @@ def test_shaded_frame_in_tile_order_matches_oracle(gpu):
     ok = float(np.mean(np.abs(q(img) - q(orc)).max(-1) <= 1))
     print(f"[tiles] frame: {ok * 100:.4f}% of pixels within 1/255 of the oracle", flush=True)
     assert ok >= 0.999
+
+
+def test_render_tile_choice_leaves_the_frame_unchanged(gpu):
+    """pvx_scene_set_render_tile: every pixel order of the ray queue gives
+    the same frame, bit for bit; pvx_render_autotune reports the tile it
+    keeps (one of its candidates) and renders that same frame."""
+    s = scenes.mesh_scene(level=4, width=203, height=117)
+    gs = gpu.GpuScene(s)
+    gs.set_shading()
+    ref, st = gs.render()
+    for tw, th in ((16, 4), (0, 1), (3, 5), (203, 117), (8, 8)):
+        gs.set_render_tile(tw, th)
+        img, st2 = gs.render()
+        print(f"[tiles] queue tile {tw}x{th}: closest {st2['closest_rays']} light {st2['shadow_rays']}", flush=True)
+        assert np.array_equal(img, ref), (tw, th)
+        assert st2["closest_rays"] == st["closest_rays"] and st2["shadow_rays"] == st["shadow_rays"]
+    for bad in ((0, 0), (0, 2), (4, 0), (5000, 1)):
+        with pytest.raises(gpu.PvxError):
+            gs.set_render_tile(*bad)
+    img, _, tune = gs.render_autotune()
+    print(f"[tiles] autotune kept tile {tune[0]['tile_w']}x{tune[0]['tile_h']}, walk {tune[0]['variant']} "
+          f"grid {tune[0]['grid_blocks']}", flush=True)
+    assert (tune[0]["tile_w"], tune[0]["tile_h"]) in ((8, 8), (16, 4), (32, 2), (0, 1))
+    assert np.array_equal(img, ref)
+    assert np.array_equal(gs.render()[0], ref)
