@@ -562,7 +562,10 @@ typedef struct pvx_render_stats {
     uint32_t bounces;        /* wavefront iterations run                      */
     uint32_t pad;
     double trace_ms;         /* GPU time in closest-hit launches              */
-    double shadow_ms;        /* GPU time in shadow-ray launches               */
+    double shadow_ms;        /* GPU time in shadow-ray launches (light rays
+                                made in the kernel run on a second stream,
+                                beside the next bounces: the two then overlap
+                                and may add up to more than total_ms)        */
     double total_ms;         /* GPU time of the whole frame                   */
 } pvx_render_stats;
 

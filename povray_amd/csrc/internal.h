@@ -59,6 +59,37 @@ struct ShadingState {
     LightRec *d_lights = nullptr;
 };
 
+// A stream created on first use (default priority) and destroyed with its owner.
+struct StreamHolder {
+    hipStream_t s = nullptr;
+    StreamHolder() = default;
+    StreamHolder(const StreamHolder &) = delete;
+    StreamHolder &operator=(const StreamHolder &) = delete;
+    ~StreamHolder() {
+        if (s) (void)hipStreamDestroy(s);
+    }
+};
+
+// Events kept between frames (created on demand, destroyed with the owner).
+struct EventPool {
+    std::vector<hipEvent_t> ev;
+    EventPool() = default;
+    EventPool(const EventPool &) = delete;
+    EventPool &operator=(const EventPool &) = delete;
+    ~EventPool() {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    }
+    hipError_t reserve(size_t n) {
+        while (ev.size() < n) {
+            hipEvent_t e;
+            hipError_t r = hipEventCreate(&e);
+            if (r != hipSuccess) return r;
+            ev.push_back(e);
+        }
+        return hipSuccess;
+    }
+};
+
 // Render work buffers, grown on demand and kept between frames.
 struct RenderBuffers {
     DevBuf rays[2], state[2], kinds[2]; // ray queues (ping-pong)
@@ -67,6 +98,13 @@ struct RenderBuffers {
     DevBuf srays[2], sdepth[2], sowner[2], shits, scol; // light rays
     DevBuf groups, vis; // packet light rays: shaded point per (hit, light), visibility per ray
     DevBuf counters, offsets;
+    DevBuf pix[2]; // each hit's pixel (per bounce parity)
+    // light passes overlapped with the next bounces (light rays made in the
+    // kernel): the per-hit buffers of odd bounces, the refill counter, the
+    // light stream and its events
+    DevBuf contrib_b, shade_b, lslot_b, lterm_b, groups_b, refill;
+    StreamHolder light_stream;
+    EventPool events;
 };
 
 } // namespace pvx

@@ -81,8 +81,6 @@ __global__ void k_zero_counters(uint32_t *c) {
     if (threadIdx.x < 64) c[threadIdx.x] = 0;
 }
 
-// Slot of the render counters that hands out light rays to the refilling walk.
-constexpr uint32_t kRefillCounter = 8;
 __global__ void k_zero_word(uint32_t *c) {
     if (threadIdx.x == 0) *c = 0;
 }
@@ -177,11 +175,12 @@ __global__ void k_shade(RenderParams p, const pvx_ray *__restrict__ rays, const 
                         pvx_ray *__restrict__ srays, double *__restrict__ sdepth, uint32_t *__restrict__ sowner,
                         float *__restrict__ scol, pvx_ray *__restrict__ crays, PathState *__restrict__ cst,
                         uint32_t *__restrict__ nchild, uint32_t *__restrict__ counters,
-                        LightGroup *__restrict__ groups) {
+                        LightGroup *__restrict__ groups, uint32_t *__restrict__ pix) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const pvx_ray r = rays[i];
     const PathState ps = st[i];
+    pix[i] = ps.pixel;
     const pvx_hit h = hits[i];
     float *ct = contrib + 3 * (size_t)i;
     ShadeOut so;
@@ -627,14 +626,14 @@ __global__ void k_finish(RenderParams p, const ShadeOut *__restrict__ shade, uin
 
 // Adds each pixel's contributions in queue order (the queue is ordered by
 // pixel, so a pixel's rays form one run).
-__global__ void k_accumulate(const PathState *__restrict__ st, const float *__restrict__ contrib, uint32_t n,
+__global__ void k_accumulate(const uint32_t *__restrict__ pix, const float *__restrict__ contrib, uint32_t n,
                              float *__restrict__ rgb) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const uint32_t px = st[i].pixel;
-    if (i > 0 && st[i - 1].pixel == px) return;
+    const uint32_t px = pix[i];
+    if (i > 0 && pix[i - 1] == px) return;
     float c[3] = {0.0f, 0.0f, 0.0f};
-    for (uint32_t j = i; j < n && st[j].pixel == px; j++)
+    for (uint32_t j = i; j < n && pix[j] == px; j++)
         for (int a = 0; a < 3; a++) c[a] += contrib[3 * (size_t)j + a];
     float *o = rgb + 3 * (size_t)px;
     o[0] += c[0];
@@ -845,7 +844,9 @@ static int render_frame(pvx_scene *s, const pvx_camera *cam, uint32_t S, const d
             (rr = b.lterm.reserve(cap * nl * 16)) || (rr = b.crays.reserve(2 * cap * sizeof(pvx_ray))) ||
             (rr = b.cstate.reserve(2 * cap * sizeof(PathState))) || (rr = b.nchild.reserve(cap * 4)) ||
             (rr = b.offs.reserve(cap * 4)) || (rr = b.sums.reserve((cap / kScanBlock + 2) * 4)) ||
-            (rr = b.counters.reserve(64)))
+            (rr = b.counters.reserve(64)) || (rr = b.pix[0].reserve(cap * 4)) || (rr = b.pix[1].reserve(cap * 4)) ||
+            (rr = b.contrib_b.reserve(cap * 12)) || (rr = b.shade_b.reserve(cap * sizeof(ShadeOut))) ||
+            (rr = b.lslot_b.reserve(cap * nl * 4)) || (rr = b.lterm_b.reserve(cap * nl * 16)))
             return rr;
         return PVX_OK;
     };
@@ -867,12 +868,46 @@ static int render_frame(pvx_scene *s, const pvx_camera *cam, uint32_t S, const d
     for (const LightRec &lt : sh.lights) smax = std::max(smax, lt.n1 * lt.n2);
     auto ensure_packet = [&](size_t nh) -> int {
         int rr;
-        if ((rr = b.groups.reserve(nh * nl * sizeof(LightGroup))) || (rr = b.vis.reserve(nh * nl * smax))) return rr;
+        if ((rr = b.groups.reserve(nh * nl * sizeof(LightGroup))) ||
+            (rr = b.groups_b.reserve(nh * nl * sizeof(LightGroup))) || (rr = b.vis.reserve(nh * nl * smax)))
+            return rr;
         return PVX_OK;
     };
-    if ((r = ensure_queue(nray)) || (r = b.offsets.reserve(16 * S)) || (r = b.counters.reserve(256))) return r;
+    // With light rays made in the kernel, a bounce's light pass, k_finish and
+    // k_accumulate run on a second stream while the next bounces trace and
+    // shade: odd bounces keep their per-hit buffers in a second set, the
+    // light stream keeps the bounce order of the accumulation into rgb
+    // (so the image is the sequential one, bit for bit), and bounce j waits
+    // for the light pass of bounce j - 2 before reusing its buffers.
+    const bool overlap = packet;
+    if ((r = ensure_queue(nray)) || (r = b.offsets.reserve(16 * S)) || (r = b.counters.reserve(256)) ||
+        (r = b.refill.reserve(256)))
+        return r;
     if ((r = packet ? ensure_packet(nray) : ensure_shadow(nray * spp))) return r;
     HIP_TRY(hipMemcpyAsync(b.offsets.p, offsets, 16 * S, hipMemcpyHostToDevice, st));
+    hipStream_t ls = st;
+    if (overlap) {
+        if (!b.light_stream.s) HIP_TRY(hipStreamCreateWithFlags(&b.light_stream.s, hipStreamNonBlocking));
+        ls = b.light_stream.s;
+    }
+    // per-hit buffers of bounce parity `par` (the second set only when overlapping)
+    struct HitBufs {
+        float *contrib;
+        ShadeOut *shade;
+        uint32_t *lslot;
+        double *lterm;
+        LightGroup *groups;
+        uint32_t *pix;
+    };
+    auto hit_bufs = [&](int par) -> HitBufs {
+        const bool second = overlap && par == 1;
+        return HitBufs{static_cast<float *>(second ? b.contrib_b.p : b.contrib.p),
+                       static_cast<ShadeOut *>(second ? b.shade_b.p : b.shade.p),
+                       static_cast<uint32_t *>(second ? b.lslot_b.p : b.lslot.p),
+                       static_cast<double *>(second ? b.lterm_b.p : b.lterm.p),
+                       static_cast<LightGroup *>(second ? b.groups_b.p : b.groups.p),
+                       static_cast<uint32_t *>(b.pix[overlap ? par : 0].p)};
+    };
 
     RenderParams p;
     p.sv = s->view;
@@ -893,6 +928,10 @@ static int render_frame(pvx_scene *s, const pvx_camera *cam, uint32_t S, const d
     double trace_ms = 0.0, shadow_ms = 0.0;
     uint64_t n_closest = 0, n_shadow = 0;
     uint32_t bounces = 0;
+    // events per bounce k: [4k] after k_shade (st), [4k+1], [4k+2] around the
+    // light pass (ls), [4k+3] after k_accumulate (ls)
+    std::vector<int> timed_light; // bounces whose light pass is timed at the end
+    auto ev = [&](uint32_t k, int j) { return b.events.ev[4 * (size_t)k + j]; };
     HIP_TRY(hipEventRecord(e0, st));
     {
         size_t n3 = 3 * npix;
@@ -941,7 +980,19 @@ static int render_frame(pvx_scene *s, const pvx_camera *cam, uint32_t S, const d
     // every bounce raises the trace level of all but continued rays; the cap
     // only guards against pathological chains of continued rays
     const uint32_t max_bounces = 4 * std::max<uint32_t>(1, sh.max_level) + 16;
+    // buffers the light stream may still be reading are reallocated only
+    // after it has drained
+    auto drain_light = [&]() -> int {
+        if (overlap) HIP_TRY(hipStreamSynchronize(ls));
+        return PVX_OK;
+    };
+    auto packet_fits = [&](size_t nh) {
+        return nh * nl * sizeof(LightGroup) <= b.groups.cap && nh * nl * sizeof(LightGroup) <= b.groups_b.cap &&
+               nh * nl * smax <= b.vis.cap;
+    };
     while (n > 0 && bounces < max_bounces) {
+        const int par = (int)(bounces & 1u);
+        if (overlap) HIP_TRY(b.events.reserve(4 * ((size_t)bounces + 1)));
         const pvx_ray *rays = static_cast<const pvx_ray *>(b.rays[cur].p);
         const PathState *pst = static_cast<const PathState *>(b.state[cur].p);
         HIP_TRY(hipEventRecord(t0, st));
@@ -952,16 +1003,24 @@ static int render_frame(pvx_scene *s, const pvx_camera *cam, uint32_t S, const d
         HIP_TRY(hipEventRecord(t1, st));
         n_closest += n;
         uint32_t *counters = static_cast<uint32_t *>(b.counters.p);
-        if ((r = packet ? ensure_packet(n) : ensure_shadow((size_t)n * spp))) return r; // light rays of this bounce
+        if (packet) { // light groups of this bounce
+            if (!packet_fits(n) && (r = drain_light())) return r;
+            if ((r = ensure_packet(n))) return r;
+        } else if ((r = ensure_shadow((size_t)n * spp))) {
+            return r;
+        }
+        // the light pass of bounce - 2 used this parity's buffers
+        if (overlap && bounces >= 2) HIP_TRY(hipStreamWaitEvent(st, ev(bounces - 2, 3), 0));
+        const HitBufs hb = hit_bufs(par);
         k_zero_counters<<<1, 64, 0, st>>>(counters);
         (packet ? k_shade<true> : k_shade<false>)<<<(n + 127) / 128, 128, 0, st>>>(
-            p, rays, pst, static_cast<const pvx_hit *>(b.hits.p), n, static_cast<float *>(b.contrib.p),
-            static_cast<ShadeOut *>(b.shade.p), static_cast<uint32_t *>(b.lslot.p), static_cast<double *>(b.lterm.p),
+            p, rays, pst, static_cast<const pvx_hit *>(b.hits.p), n, hb.contrib, hb.shade, hb.lslot, hb.lterm,
             static_cast<pvx_ray *>(b.srays[0].p), static_cast<double *>(b.sdepth[0].p),
             static_cast<uint32_t *>(b.sowner[0].p), static_cast<float *>(b.scol.p),
             static_cast<pvx_ray *>(b.crays.p), static_cast<PathState *>(b.cstate.p),
-            static_cast<uint32_t *>(b.nchild.p), counters, static_cast<LightGroup *>(b.groups.p));
+            static_cast<uint32_t *>(b.nchild.p), counters, hb.groups, hb.pix);
         HIP_TRY(hipGetLastError());
+        if (overlap) HIP_TRY(hipEventRecord(ev(bounces, 0), st));
         // next queue size: exclusive scan of the child counts
         const uint32_t nb = (n + kScanBlock - 1) / kScanBlock;
         if (nb > (uint32_t)kScanBlock * 8) return fail(PVX_EUNSUPPORTED, "ray queue above 8M rays");
@@ -975,35 +1034,35 @@ static int render_frame(pvx_scene *s, const pvx_camera *cam, uint32_t S, const d
         float ms = 0.0f;
         HIP_TRY(hipEventElapsedTime(&ms, t0, t1));
         trace_ms += ms;
-        if (packet) { // light rays made in the kernel and tested for occlusion, one launch per light
+        if (packet) { // light rays made in the kernel and tested for occlusion, on the light stream
             const uint64_t nr = (uint64_t)cnt[kGroupCounter] * smax;
             if (nr > 0xFFFFFFFFull) return fail(PVX_EUNSUPPORTED, "more than 2^32 light rays in one bounce");
+            HIP_TRY(hipStreamWaitEvent(ls, ev(bounces, 0), 0));
             // walk and grid (0: the walk's default) of this bounce's light rays
             auto light_pass = [&](uint32_t variant, uint32_t grid) -> int {
                 if (nr == 0) return PVX_OK;
-                const LightGroup *gl = static_cast<const LightGroup *>(b.groups.p);
                 uint8_t *vl = static_cast<uint8_t *>(b.vis.p);
                 if (variant == PVX_TRAVERSAL_PACKET) {
                     const uint32_t nb = (uint32_t)std::min<uint64_t>(grid ? grid : s->grid, (nr + 255) / 256);
                     if (s->view.any_flags)
-                        k_light_packet<true><<<nb, 256, 0, st>>>(s->view, sh.d_lights, gl, smax, (uint32_t)nr, vl);
+                        k_light_packet<true><<<nb, 256, 0, ls>>>(s->view, sh.d_lights, hb.groups, smax, (uint32_t)nr, vl);
                     else
-                        k_light_packet<false><<<nb, 256, 0, st>>>(s->view, sh.d_lights, gl, smax, (uint32_t)nr, vl);
+                        k_light_packet<false><<<nb, 256, 0, ls>>>(s->view, sh.d_lights, hb.groups, smax, (uint32_t)nr, vl);
                     HIP_TRY(hipGetLastError());
                     return PVX_OK;
                 }
                 uint32_t *next = nullptr;
                 if (variant == PVX_TRAVERSAL_REFILL) {
-                    next = counters + kRefillCounter;
-                    k_zero_word<<<1, 64, 0, st>>>(next);
+                    next = static_cast<uint32_t *>(b.refill.p);
+                    k_zero_word<<<1, 64, 0, ls>>>(next);
                 }
-                return light_rays_internal(s, gl, sh.d_lights, smax, (uint32_t)nr, vl, next, grid, st);
+                return light_rays_internal(s, hb.groups, sh.d_lights, smax, (uint32_t)nr, vl, next, grid, ls);
             };
             auto timed_pass = [&](uint32_t variant, uint32_t grid, float &out_ms) -> int {
-                HIP_TRY(hipEventRecord(t0, st));
+                HIP_TRY(hipEventRecord(t0, ls));
                 int rr = light_pass(variant, grid);
                 if (rr) return rr;
-                HIP_TRY(hipEventRecord(t1, st));
+                HIP_TRY(hipEventRecord(t1, ls));
                 HIP_TRY(hipEventSynchronize(t1));
                 HIP_TRY(hipEventElapsedTime(&out_ms, t0, t1));
                 return PVX_OK;
@@ -1035,11 +1094,14 @@ static int render_frame(pvx_scene *s, const pvx_camera *cam, uint32_t S, const d
                 if (bounces == 0) {
                     s->tune_variant[1] = variant;
                     s->tune_grid[1] = grid;
-                    tune[1] = pvx_tune_result{variant, grid, best->ms, cands[0].ms};
+                    tune[1] = pvx_tune_result{variant, grid, best->ms, cands[0].ms, 0, 0};
                 }
             }
-            if ((rc = timed_pass(variant, grid, ms))) return rc;
-            shadow_ms += ms;
+            // the pass itself, timed by events read at the end of the frame
+            HIP_TRY(hipEventRecord(ev(bounces, 1), ls));
+            if ((rc = light_pass(variant, grid))) return rc;
+            HIP_TRY(hipEventRecord(ev(bounces, 2), ls));
+            timed_light.push_back((int)bounces);
             n_shadow += cnt[1]; // the light rays made (k_shade counts them as it did when writing them)
         }
         // light rays, followed through transparent objects
@@ -1052,7 +1114,7 @@ static int render_frame(pvx_scene *s, const pvx_camera *cam, uint32_t S, const d
                                             st)))
                     return rc;
             } else {
-                tune[1] = pvx_tune_result{0, 0, 0.0, 0.0}; // no light rays: nothing to tune
+                tune[1] = pvx_tune_result{0, 0, 0.0, 0.0, 0, 0}; // no light rays: nothing to tune
             }
         }
         for (int step = 0; ns > 0 && step < kMaxShadowSteps; step++) {
@@ -1078,17 +1140,18 @@ static int render_frame(pvx_scene *s, const pvx_camera *cam, uint32_t S, const d
             ns = c2;
             sq = 1 - sq;
         }
-        (packet ? k_finish<true> : k_finish<false>)<<<(n + 127) / 128, 128, 0, st>>>(
-            p, static_cast<const ShadeOut *>(b.shade.p), n, static_cast<const uint32_t *>(b.lslot.p),
-            static_cast<const double *>(b.lterm.p), static_cast<const float *>(b.scol.p),
-            static_cast<float *>(b.contrib.p), static_cast<const uint8_t *>(b.vis.p), smax);
-        k_accumulate<<<(n + 255) / 256, 256, 0, st>>>(pst, static_cast<const float *>(b.contrib.p), n, rgb);
+        (packet ? k_finish<true> : k_finish<false>)<<<(n + 127) / 128, 128, 0, ls>>>(
+            p, hb.shade, n, hb.lslot, hb.lterm, static_cast<const float *>(b.scol.p), hb.contrib,
+            static_cast<const uint8_t *>(b.vis.p), smax);
+        k_accumulate<<<(n + 255) / 256, 256, 0, ls>>>(hb.pix, hb.contrib, n, rgb);
         HIP_TRY(hipGetLastError());
+        if (overlap) HIP_TRY(hipEventRecord(ev(bounces, 3), ls));
         const uint32_t next = cnt[2];
         if (next > 0) {
             if (next > b.rays[1 - cur].cap / sizeof(pvx_ray) || next > b.nchild.cap / 4) {
                 // grow the queue (keeping this bounce's children, which live in crays/cstate)
                 HIP_TRY(hipStreamSynchronize(st));
+                if ((r = drain_light())) return r;
                 DevBuf keep_r, keep_s;
                 std::swap(keep_r.p, b.crays.p);
                 std::swap(keep_r.cap, b.crays.cap);
@@ -1132,12 +1195,19 @@ static int render_frame(pvx_scene *s, const pvx_camera *cam, uint32_t S, const d
         cur = 1 - cur;
         bounces++;
     }
+    // the light stream's last accumulation, then the caller's stream goes on
+    if (overlap && bounces > 0) HIP_TRY(hipStreamWaitEvent(st, ev(bounces - 1, 3), 0));
     if (S > 1) // the pixel is the mean of its samples (tracetask.cpp:889)
         k_scale3<<<(unsigned)((3 * npix + 255) / 256), 256, 0, st>>>(rgb, 3 * npix, 1.0f / (float)S);
     HIP_TRY(hipEventRecord(e1, st));
     HIP_TRY(hipEventSynchronize(e1));
     float total = 0.0f;
     HIP_TRY(hipEventElapsedTime(&total, e0, e1));
+    for (int k : timed_light) {
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, ev((uint32_t)k, 1), ev((uint32_t)k, 2)));
+        shadow_ms += ms;
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     (void)hipEventDestroy(t0);
@@ -1153,3 +1223,4 @@ static int render_frame(pvx_scene *s, const pvx_camera *cam, uint32_t S, const d
     }
     return PVX_OK;
 }
+
