@@ -103,6 +103,7 @@ struct RenderBuffers {
     // kernel): the per-hit buffers of odd bounces, the refill counter, the
     // light stream and its events
     DevBuf contrib_b, shade_b, lslot_b, lterm_b, groups_b, refill;
+    DevBuf gcount; // light groups of the bounces of each parity (two words)
     StreamHolder light_stream;
     EventPool events;
 };
@@ -169,9 +170,11 @@ int trace_closest_internal(pvx_scene *s, const pvx_ray *rays, const double *max_
 // scenes with a 4-wide tree.  `lights` is the device light array.  With
 // `next` (a device word holding 0) lanes take new rays from that counter as
 // their rays end; without it each lane walks a fixed stride of rays.  `grid`
-// blocks (0: the walk's default).
+// blocks (0: the walk's default).  With `count` (device), the kernel reads
+// the number of groups there and n_rays only bounds the grid.
 int light_rays_internal(pvx_scene *s, const LightGroup *groups, const LightRec *lights, uint32_t S,
-                        uint32_t n_rays, uint8_t *vis, uint32_t *next, uint32_t grid, hipStream_t stream);
+                        uint32_t n_rays, uint8_t *vis, uint32_t *next, uint32_t grid, const uint32_t *count,
+                        hipStream_t stream);
 // The autotuner over sample rays with optional per-ray max depths.
 int autotune_internal(pvx_scene *s, const pvx_ray *rays, const double *max_depth, size_t n,
                       uint32_t ray_kind, pvx_tune_result *out, hipStream_t stream);

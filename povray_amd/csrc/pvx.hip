@@ -107,6 +107,7 @@ hipError_t pvx::trace_launch(const TraceKernel &k, const TraceArgs &a, uint32_t 
         case kWalkStep4:
         case kWalkWW4: return trace_part0(k, a, nb, st, occ);
         case kWalkBinary: return trace_part1(k, a, nb, st, occ);
+        case kWalkBinary0: return trace_part16(k, a, nb, st, occ);
         case kWalkComplex:
             return (k.flags ? (k.count ? trace_part5 : trace_part4) : (k.count ? trace_part3 : trace_part2))(k, a, nb, st,
                                                                                                           occ);
@@ -445,7 +446,7 @@ static int launch_variant(pvx_scene *s, const pvx_ray *rays, const double *max_d
     else if (level == 2)
         k.walk = kWalkComplex;
     else if (level == 1 || variant == PVX_TRAVERSAL_BINARY)
-        k.walk = kWalkBinary;
+        k.walk = level == 0 ? kWalkBinary0 : kWalkBinary;
     else if (variant == PVX_TRAVERSAL_WIDE_WW ||
              (variant == PVX_TRAVERSAL_DEFAULT && (ray_kind & 0xFFu) != PVX_RAY_SHADOW))
         k.walk = kWalkWW4;
@@ -588,7 +589,7 @@ extern "C" int pvx_scene_autotune(pvx_scene *s, const pvx_ray *rays, size_t n, u
 
 int pvx::light_rays_internal(pvx_scene *s, const LightGroup *groups, const LightRec *lights, uint32_t S,
                              uint32_t n_rays, uint8_t *vis, uint32_t *next, uint32_t grid_blocks,
-                             hipStream_t st) {
+                             const uint32_t *count, hipStream_t st) {
     if (n_rays == 0) return PVX_OK;
     int r;
     int32_t *spill = nullptr;
@@ -609,6 +610,7 @@ int pvx::light_rays_internal(pvx_scene *s, const LightGroup *groups, const Light
     ta.gen_S = S;
     ta.gen_vis = vis;
     ta.gen_next = next;
+    ta.gen_count = count;
     // refill a wavefront once no more than 16 of its lanes hold rays: new
     // rays then come in runs of at least 48 consecutive ones (rays of a few
     // shaded points), which keeps the lanes' walks alike; refilling every

@@ -175,7 +175,7 @@ __global__ void k_shade(RenderParams p, const pvx_ray *__restrict__ rays, const 
                         pvx_ray *__restrict__ srays, double *__restrict__ sdepth, uint32_t *__restrict__ sowner,
                         float *__restrict__ scol, pvx_ray *__restrict__ crays, PathState *__restrict__ cst,
                         uint32_t *__restrict__ nchild, uint32_t *__restrict__ counters,
-                        LightGroup *__restrict__ groups, uint32_t *__restrict__ pix) {
+                        LightGroup *__restrict__ groups, uint32_t *__restrict__ pix, uint32_t *__restrict__ gcount) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const pvx_ray r = rays[i];
@@ -230,7 +230,7 @@ __global__ void k_shade(RenderParams p, const pvx_ray *__restrict__ rays, const 
         lterm[2 * li + 1] = ph;
         if constexpr (kPacket) {
             atomicAdd(&counters[1], lt.n1 * lt.n2); // light rays (statistics)
-            const uint32_t g = atomicAdd(&counters[kGroupCounter], 1u);
+            const uint32_t g = atomicAdd(gcount, 1u);
             lslot[li] = g;
             groups[g] = LightGroup{{ip.x, ip.y, ip.z}, i, L};
             continue;
@@ -404,7 +404,10 @@ constexpr int kPacketWaves = 4; // wavefronts per block of 256
 template <bool kFlags>
 __global__ __launch_bounds__(256) void k_light_packet(SceneView sv, const LightRec *__restrict__ lights,
                                                       const LightGroup *__restrict__ groups, uint32_t S,
-                                                      uint32_t n_rays, uint8_t *__restrict__ vis) {
+                                                      uint32_t n_bound, const uint32_t *__restrict__ count,
+                                                      uint8_t *__restrict__ vis) {
+    // the group count on the device when given (n_bound then only sized the grid)
+    const uint32_t n_rays = count ? count[0] * S : n_bound;
     __shared__ int32_t wstack[kPacketWaves][kStackMax];
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     int32_t *stk = wstack[wave];
@@ -881,7 +884,7 @@ static int render_frame(pvx_scene *s, const pvx_camera *cam, uint32_t S, const d
     // for the light pass of bounce j - 2 before reusing its buffers.
     const bool overlap = packet;
     if ((r = ensure_queue(nray)) || (r = b.offsets.reserve(16 * S)) || (r = b.counters.reserve(256)) ||
-        (r = b.refill.reserve(256)))
+        (r = b.refill.reserve(256)) || (r = b.gcount.reserve(256)))
         return r;
     if ((r = packet ? ensure_packet(nray) : ensure_shadow(nray * spp))) return r;
     HIP_TRY(hipMemcpyAsync(b.offsets.p, offsets, 16 * S, hipMemcpyHostToDevice, st));
@@ -990,6 +993,31 @@ static int render_frame(pvx_scene *s, const pvx_camera *cam, uint32_t S, const d
         return nh * nl * sizeof(LightGroup) <= b.groups.cap && nh * nl * sizeof(LightGroup) <= b.groups_b.cap &&
                nh * nl * smax <= b.vis.cap;
     };
+    // a light pass on the light stream: walk, grid (0: the walk's default),
+    // rays to launch for (with `count`, the kernel reads the number of groups
+    // there and n_launch only sizes the grid)
+    auto light_pass = [&](const LightGroup *groups, uint32_t variant, uint32_t grid, uint64_t n_launch,
+                          const uint32_t *count) -> int {
+        if (n_launch == 0) return PVX_OK;
+        uint8_t *vl = static_cast<uint8_t *>(b.vis.p);
+        if (variant == PVX_TRAVERSAL_PACKET) {
+            const uint32_t nb = (uint32_t)std::min<uint64_t>(grid ? grid : s->grid, (n_launch + 255) / 256);
+            if (s->view.any_flags)
+                k_light_packet<true><<<nb, 256, 0, ls>>>(s->view, sh.d_lights, groups, smax, (uint32_t)n_launch, count,
+                                                         vl);
+            else
+                k_light_packet<false><<<nb, 256, 0, ls>>>(s->view, sh.d_lights, groups, smax, (uint32_t)n_launch, count,
+                                                          vl);
+            HIP_TRY(hipGetLastError());
+            return PVX_OK;
+        }
+        uint32_t *next = nullptr;
+        if (variant == PVX_TRAVERSAL_REFILL) {
+            next = static_cast<uint32_t *>(b.refill.p);
+            k_zero_word<<<1, 64, 0, ls>>>(next);
+        }
+        return light_rays_internal(s, groups, sh.d_lights, smax, (uint32_t)n_launch, vl, next, grid, count, ls);
+    };
     while (n > 0 && bounces < max_bounces) {
         const int par = (int)(bounces & 1u);
         if (overlap) HIP_TRY(b.events.reserve(4 * ((size_t)bounces + 1)));
@@ -1012,15 +1040,34 @@ static int render_frame(pvx_scene *s, const pvx_camera *cam, uint32_t S, const d
         // the light pass of bounce - 2 used this parity's buffers
         if (overlap && bounces >= 2) HIP_TRY(hipStreamWaitEvent(st, ev(bounces - 2, 3), 0));
         const HitBufs hb = hit_bufs(par);
+        uint32_t *gcount = static_cast<uint32_t *>(b.gcount.p) + par;
         k_zero_counters<<<1, 64, 0, st>>>(counters);
+        k_zero_word<<<1, 64, 0, st>>>(gcount);
         (packet ? k_shade<true> : k_shade<false>)<<<(n + 127) / 128, 128, 0, st>>>(
             p, rays, pst, static_cast<const pvx_hit *>(b.hits.p), n, hb.contrib, hb.shade, hb.lslot, hb.lterm,
             static_cast<pvx_ray *>(b.srays[0].p), static_cast<double *>(b.sdepth[0].p),
             static_cast<uint32_t *>(b.sowner[0].p), static_cast<float *>(b.scol.p),
             static_cast<pvx_ray *>(b.crays.p), static_cast<PathState *>(b.cstate.p),
-            static_cast<uint32_t *>(b.nchild.p), counters, hb.groups, hb.pix);
+            static_cast<uint32_t *>(b.nchild.p), counters, hb.groups, hb.pix, gcount);
         HIP_TRY(hipGetLastError());
         if (overlap) HIP_TRY(hipEventRecord(ev(bounces, 0), st));
+        // The light rays of camera-ray hits come in pixel order and walk
+        // alike in runs; those of later bounces less so, and the walk that
+        // suits them may differ: each keeps its own choice.
+        const int lcls = bounces == 0 ? 0 : 1;
+        uint32_t lvariant = s->light_variant[lcls], lgrid = s->light_grid[lcls];
+        if (lvariant < PVX_TRAVERSAL_PACKET) lvariant = PVX_TRAVERSAL_FUSED;
+        // outside the tuning bounces the light pass starts at once, with the
+        // group count read on the device (no wait for the host's read-back)
+        const uint64_t nr_bound = (uint64_t)n * nl * smax;
+        const bool early = packet && !(tune && bounces < 2) && nr_bound <= 0xFFFFFFFFull;
+        if (early) {
+            HIP_TRY(hipStreamWaitEvent(ls, ev(bounces, 0), 0));
+            HIP_TRY(hipEventRecord(ev(bounces, 1), ls));
+            if ((r = light_pass(hb.groups, lvariant, lgrid, nr_bound, gcount))) return r;
+            HIP_TRY(hipEventRecord(ev(bounces, 2), ls));
+            timed_light.push_back((int)bounces);
+        }
         // next queue size: exclusive scan of the child counts
         const uint32_t nb = (n + kScanBlock - 1) / kScanBlock;
         if (nb > (uint32_t)kScanBlock * 8) return fail(PVX_EUNSUPPORTED, "ray queue above 8M rays");
@@ -1030,6 +1077,7 @@ static int render_frame(pvx_scene *s, const pvx_camera *cam, uint32_t S, const d
         HIP_TRY(hipGetLastError());
         uint32_t cnt[kGroupCounter + 1];
         HIP_TRY(hipMemcpyAsync(cnt, counters, sizeof cnt, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(&cnt[kGroupCounter], gcount, 4, hipMemcpyDeviceToHost, st)); // light groups
         HIP_TRY(hipStreamSynchronize(st));
         float ms = 0.0f;
         HIP_TRY(hipEventElapsedTime(&ms, t0, t1));
@@ -1037,71 +1085,48 @@ static int render_frame(pvx_scene *s, const pvx_camera *cam, uint32_t S, const d
         if (packet) { // light rays made in the kernel and tested for occlusion, on the light stream
             const uint64_t nr = (uint64_t)cnt[kGroupCounter] * smax;
             if (nr > 0xFFFFFFFFull) return fail(PVX_EUNSUPPORTED, "more than 2^32 light rays in one bounce");
-            HIP_TRY(hipStreamWaitEvent(ls, ev(bounces, 0), 0));
-            // walk and grid (0: the walk's default) of this bounce's light rays
-            auto light_pass = [&](uint32_t variant, uint32_t grid) -> int {
-                if (nr == 0) return PVX_OK;
-                uint8_t *vl = static_cast<uint8_t *>(b.vis.p);
-                if (variant == PVX_TRAVERSAL_PACKET) {
-                    const uint32_t nb = (uint32_t)std::min<uint64_t>(grid ? grid : s->grid, (nr + 255) / 256);
-                    if (s->view.any_flags)
-                        k_light_packet<true><<<nb, 256, 0, ls>>>(s->view, sh.d_lights, hb.groups, smax, (uint32_t)nr, vl);
-                    else
-                        k_light_packet<false><<<nb, 256, 0, ls>>>(s->view, sh.d_lights, hb.groups, smax, (uint32_t)nr, vl);
-                    HIP_TRY(hipGetLastError());
+            if (!early) {
+                HIP_TRY(hipStreamWaitEvent(ls, ev(bounces, 0), 0));
+                auto timed_pass = [&](uint32_t variant, uint32_t grid, float &out_ms) -> int {
+                    HIP_TRY(hipEventRecord(t0, ls));
+                    int rr = light_pass(hb.groups, variant, grid, nr, nullptr);
+                    if (rr) return rr;
+                    HIP_TRY(hipEventRecord(t1, ls));
+                    HIP_TRY(hipEventSynchronize(t1));
+                    HIP_TRY(hipEventElapsedTime(&out_ms, t0, t1));
                     return PVX_OK;
+                };
+                if (tune && bounces < 2 && nr > 0) { // the walks and grids on this frame's light rays; the fastest is kept
+                    struct Cand {
+                        uint32_t variant, grid;
+                        float ms;
+                    } cands[4] = {{PVX_TRAVERSAL_FUSED, 2 * s->grid, 0.0f},
+                                  {PVX_TRAVERSAL_FUSED, s->grid, 0.0f},
+                                  {PVX_TRAVERSAL_REFILL, s->grid, 0.0f},
+                                  {PVX_TRAVERSAL_PACKET, s->grid, 0.0f}};
+                    // warm-up: the first launch is slower
+                    if ((rc = light_pass(hb.groups, PVX_TRAVERSAL_FUSED, 0, nr, nullptr))) return rc;
+                    for (Cand &c : cands)
+                        if ((rc = timed_pass(c.variant, c.grid, c.ms))) return rc;
+                    const Cand *best = &cands[0];
+                    for (const Cand &c : cands)
+                        if (c.ms < best->ms) best = &c;
+                    lvariant = best->variant;
+                    lgrid = best->grid;
+                    s->light_variant[lcls] = lvariant;
+                    s->light_grid[lcls] = lgrid;
+                    if (bounces == 0) {
+                        s->tune_variant[1] = lvariant;
+                        s->tune_grid[1] = lgrid;
+                        tune[1] = pvx_tune_result{lvariant, lgrid, best->ms, cands[0].ms, 0, 0};
+                    }
                 }
-                uint32_t *next = nullptr;
-                if (variant == PVX_TRAVERSAL_REFILL) {
-                    next = static_cast<uint32_t *>(b.refill.p);
-                    k_zero_word<<<1, 64, 0, ls>>>(next);
-                }
-                return light_rays_internal(s, hb.groups, sh.d_lights, smax, (uint32_t)nr, vl, next, grid, ls);
-            };
-            auto timed_pass = [&](uint32_t variant, uint32_t grid, float &out_ms) -> int {
-                HIP_TRY(hipEventRecord(t0, ls));
-                int rr = light_pass(variant, grid);
-                if (rr) return rr;
-                HIP_TRY(hipEventRecord(t1, ls));
-                HIP_TRY(hipEventSynchronize(t1));
-                HIP_TRY(hipEventElapsedTime(&out_ms, t0, t1));
-                return PVX_OK;
-            };
-            // The light rays of camera-ray hits come in pixel order and walk
-            // alike in runs; those of later bounces less so, and the walk that
-            // suits them may differ: each keeps its own choice.
-            const int cls = bounces == 0 ? 0 : 1;
-            uint32_t variant = s->light_variant[cls], grid = s->light_grid[cls];
-            if (variant < PVX_TRAVERSAL_PACKET) variant = PVX_TRAVERSAL_FUSED;
-            if (tune && bounces < 2 && nr > 0) { // the walks and grids on this frame's light rays; the fastest is kept
-                struct Cand {
-                    uint32_t variant, grid;
-                    float ms;
-                } cands[4] = {{PVX_TRAVERSAL_FUSED, 2 * s->grid, 0.0f},
-                              {PVX_TRAVERSAL_FUSED, s->grid, 0.0f},
-                              {PVX_TRAVERSAL_REFILL, s->grid, 0.0f},
-                              {PVX_TRAVERSAL_PACKET, s->grid, 0.0f}};
-                if ((rc = light_pass(PVX_TRAVERSAL_FUSED, 0))) return rc; // warm-up: the first launch is slower
-                for (Cand &c : cands)
-                    if ((rc = timed_pass(c.variant, c.grid, c.ms))) return rc;
-                const Cand *best = &cands[0];
-                for (const Cand &c : cands)
-                    if (c.ms < best->ms) best = &c;
-                variant = best->variant;
-                grid = best->grid;
-                s->light_variant[cls] = variant;
-                s->light_grid[cls] = grid;
-                if (bounces == 0) {
-                    s->tune_variant[1] = variant;
-                    s->tune_grid[1] = grid;
-                    tune[1] = pvx_tune_result{variant, grid, best->ms, cands[0].ms, 0, 0};
-                }
+                // the pass itself, timed by events read at the end of the frame
+                HIP_TRY(hipEventRecord(ev(bounces, 1), ls));
+                if ((rc = light_pass(hb.groups, lvariant, lgrid, nr, nullptr))) return rc;
+                HIP_TRY(hipEventRecord(ev(bounces, 2), ls));
+                timed_light.push_back((int)bounces);
             }
-            // the pass itself, timed by events read at the end of the frame
-            HIP_TRY(hipEventRecord(ev(bounces, 1), ls));
-            if ((rc = light_pass(variant, grid))) return rc;
-            HIP_TRY(hipEventRecord(ev(bounces, 2), ls));
-            timed_light.push_back((int)bounces);
             n_shadow += cnt[1]; // the light rays made (k_shade counts them as it did when writing them)
         }
         // light rays, followed through transparent objects

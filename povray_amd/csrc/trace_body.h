@@ -188,7 +188,9 @@ __device__ inline bool simple_closest(const SceneView &sv, uint32_t kind, uint32
 // skipped; max depth the distance) whose visibility byte goes to gen_vis[k]
 // (1: nothing closer than the light minus SHADOW_TOLERANCE blocks it)
 // instead of a hit record.
-template <int kLevel, bool kFlags, bool kCount, bool kWW, bool kMulti = false, bool kGen = false>
+// kBin: a level-0 scene (triangles, spheres, planes) walked through the
+// binary tree instead of the 4-wide one, with only level-0 leaf tests.
+template <int kLevel, bool kFlags, bool kCount, bool kWW, bool kMulti = false, bool kGen = false, bool kBin = false>
 __device__ __forceinline__ void trace_closest_body(const SceneView &sv0, const SceneView *__restrict__ views,
                                                    const uint4 *__restrict__ chunks,
                                                    unsigned long long *__restrict__ clocks,
@@ -278,7 +280,7 @@ __device__ __forceinline__ void trace_closest_body(const SceneView &sv0, const S
         rf.oiz = (float)o.z * rf.iz;
         float tb = (float)bt * 1.000001f;
 
-        int32_t node = (occl && bo >= 0 && bt < occl_lim) ? kEmpty : kLevel == 0 ? sv.root4 : sv.root;
+        int32_t node = (occl && bo >= 0 && bt < occl_lim) ? kEmpty : (kLevel == 0 && !kBin) ? sv.root4 : sv.root;
         int sp = 0;
         uint32_t cnt_nodes = 0, cnt_prims = 0;
         // Triangle / sphere scenes traverse the 4-wide tree; camera and
@@ -385,7 +387,7 @@ __device__ __forceinline__ void trace_closest_body(const SceneView &sv0, const S
             }
             PVX_POP();
         };
-        if constexpr (kLevel == 0) {
+        if constexpr (kLevel == 0 && !kBin) {
             if constexpr (kWW) {
                 while (node != kEmpty) {
                     while (node >= 0) step_node4();

@@ -10,7 +10,7 @@
 #include "trace_launch.h"
 
 #ifndef PVX_TRACE_PART
-#error "build with -DPVX_TRACE_PART=<0..15>"
+#error "build with -DPVX_TRACE_PART=<0..16>"
 #endif
 
 namespace pvx {
@@ -22,6 +22,17 @@ __global__ __launch_bounds__(kBlock) void k_trace_closest(TraceArgs a) {
     trace_closest_body<kLevel, kFlags, kCount, false>(a.sv, nullptr, nullptr, nullptr, a.rays, a.max_depth, a.kind,
                                                       a.kinds, a.n, a.hits, a.spill, a.node_visits, a.prim_tests,
                                                       s_stack);
+}
+
+// Level-0 scenes (triangles, spheres, planes) through the binary tree.
+// 92 VGPRs, 5 wavefronts per SIMD (the level-1 binary kernel, with box
+// leaves, needs 108 and spills; forcing 6 here spills and is slower).
+template <bool kFlags, bool kCount>
+__global__ __launch_bounds__(kBlock) void k_trace_closest_bin0(TraceArgs a) {
+    __shared__ int32_t s_stack[kStackLds * kBlock];
+    trace_closest_body<0, kFlags, kCount, false, false, false, true>(a.sv, nullptr, nullptr, nullptr, a.rays,
+                                                                      a.max_depth, a.kind, a.kinds, a.n, a.hits,
+                                                                      a.spill, a.node_visits, a.prim_tests, s_stack);
 }
 
 // Triangle scenes, camera and bounce rays: while-while traversal (78 VGPRs
@@ -40,8 +51,9 @@ __global__ __launch_bounds__(kBlock) void k_trace_closest_ww(TraceArgs a) {
 template <bool kFlags>
 __global__ __launch_bounds__(kBlock, 6) void k_light_ww(TraceArgs a) {
     __shared__ int32_t s_stack[kStackLds * kBlock];
+    const uint32_t n = a.gen_count ? a.gen_count[0] * a.gen_S : a.n;
     trace_closest_body<0, kFlags, false, true, false, true>(a.sv, nullptr, nullptr, nullptr, nullptr, nullptr,
-                                                            PVX_RAY_SHADOW | kRayOcclusion, nullptr, a.n, nullptr,
+                                                            PVX_RAY_SHADOW | kRayOcclusion, nullptr, n, nullptr,
                                                             a.spill, nullptr, nullptr, s_stack, a.gen_groups,
                                                             a.gen_lights, a.gen_S, a.gen_vis);
 }
@@ -57,6 +69,7 @@ template <bool kFlags>
 __global__ __launch_bounds__(kBlock, 6) void k_light_dyn(TraceArgs a) {
     __shared__ int32_t s_stack[kStackLds * kBlock];
     const SceneView &sv = a.sv;
+    const uint32_t n = a.gen_count ? a.gen_count[0] * a.gen_S : a.n;
     const uint32_t lane = threadIdx.x;
     const uint32_t gtid = blockIdx.x * kBlock + lane;
     int32_t *my_spill = a.spill + (size_t)gtid * kStackSpill;
@@ -96,7 +109,7 @@ __global__ __launch_bounds__(kBlock, 6) void k_light_dyn(TraceArgs a) {
             if (!have && !done) {
                 // rank among the lanes that asked: those below this one in `need`
                 ri = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(need >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
-                if (ri >= a.n) {
+                if (ri >= n) {
                     done = true;
                 } else {
                     const uint32_t g = ri / a.gen_S, sidx = ri - g * a.gen_S;
@@ -533,6 +546,8 @@ hipError_t PVX_FN(PVX_TRACE_PART)(const TraceKernel &k, const TraceArgs &a, uint
     return PVX_FC(k_trace_closest, 3, );
 #elif PVX_TRACE_PART == 14 // scene batches with lathes or sors
     return PVX_FC(k_trace_closest_multi, 3, );
+#elif PVX_TRACE_PART == 16 // single scene, level-0 primitives on the binary tree
+    return PVX_FC(k_trace_closest_bin0, );
 #elif PVX_TRACE_PART == 15 // light rays made in the kernel
     if (k.walk == kWalkLightDyn)
         return k.flags ? run(k_light_dyn<true>, a, nb, st, occ) : run(k_light_dyn<false>, a, nb, st, occ);

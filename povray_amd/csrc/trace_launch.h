@@ -3,7 +3,7 @@
 // The traversal kernels are templates over the walk, the object-flag
 // precondition, the counting mode and single/multi-scene batches.  They are
 // instantiated in trace_kernels.hip, which the Makefile compiles once per
-// part (PVX_TRACE_PART = 0..15), each part holding a few instantiations, so
+// part (PVX_TRACE_PART = 0..16), each part holding a few instantiations, so
 // that they build in parallel; trace_launch() picks the part.
 #ifndef PVX_TRACE_LAUNCH_H
 #define PVX_TRACE_LAUNCH_H
@@ -37,7 +37,8 @@ enum TraceWalk : uint32_t {
     kWalkKd = 4,
     kWalkRevolution = 5,
     kWalkLightWW = 6, // light rays made in the kernel, 4-wide tree while-while (occlusion)
-    kWalkLightDyn = 7 // the same, each lane taking a new ray as soon as its ray ends
+    kWalkLightDyn = 7, // the same, each lane taking a new ray as soon as its ray ends
+    kWalkBinary0 = 8   // level-0 scene (triangles, spheres) through the binary tree
 };
 
 struct TraceKernel {
@@ -74,6 +75,7 @@ struct TraceArgs {
     uint32_t gen_S;
     uint8_t *gen_vis;
     uint32_t *gen_next; // kWalkLightDyn: next ray to hand out (zero at launch)
+    const uint32_t *gen_count; // when set: n = gen_count[0] * gen_S, read on the device
     uint32_t gen_refill_below; // kWalkLightDyn: refill once no more than this many lanes hold rays
 };
 
@@ -110,6 +112,7 @@ PVX_TRACE_PART_DECL(11);
 PVX_TRACE_PART_DECL(13);
 PVX_TRACE_PART_DECL(14);
 PVX_TRACE_PART_DECL(15);
+PVX_TRACE_PART_DECL(16);
 
 } // namespace pvx
 
