@@ -387,9 +387,10 @@ enum {
     PVX_TRAVERSAL_PACKET = 4,    /* ... by each wavefront as one packet (one shared
                                     stack, ballots of the lanes' box tests)          */
     PVX_TRAVERSAL_FUSED = 5,     /* ... while-while per lane (the default)          */
-    PVX_TRAVERSAL_REFILL = 6     /* ... while-while per lane, a wavefront refilling
+    PVX_TRAVERSAL_REFILL = 6,    /* ... while-while per lane, a wavefront refilling
                                     its free lanes with the next unstarted rays once
                                     16 or fewer of its lanes still hold rays         */
+    PVX_TRAVERSAL_LIGHT_BINARY = 7 /* ... through the binary tree, per lane       */
 };
 
 typedef struct pvx_tune_result {

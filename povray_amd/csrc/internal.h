@@ -171,10 +171,11 @@ int trace_closest_internal(pvx_scene *s, const pvx_ray *rays, const double *max_
 // `next` (a device word holding 0) lanes take new rays from that counter as
 // their rays end; without it each lane walks a fixed stride of rays.  `grid`
 // blocks (0: the walk's default).  With `count` (device), the kernel reads
-// the number of groups there and n_rays only bounds the grid.
+// the number of groups there and n_rays only bounds the grid.  `binary`:
+// through the binary tree (one node or leaf per pass per lane).
 int light_rays_internal(pvx_scene *s, const LightGroup *groups, const LightRec *lights, uint32_t S,
                         uint32_t n_rays, uint8_t *vis, uint32_t *next, uint32_t grid, const uint32_t *count,
-                        hipStream_t stream);
+                        bool binary, hipStream_t stream);
 // The autotuner over sample rays with optional per-ray max depths.
 int autotune_internal(pvx_scene *s, const pvx_ray *rays, const double *max_depth, size_t n,
                       uint32_t ray_kind, pvx_tune_result *out, hipStream_t stream);

@@ -114,7 +114,8 @@ hipError_t pvx::trace_launch(const TraceKernel &k, const TraceArgs &a, uint32_t 
         case kWalkKd: return trace_part6(k, a, nb, st, occ);
         case kWalkRevolution: return trace_part13(k, a, nb, st, occ);
         case kWalkLightWW:
-        case kWalkLightDyn: return trace_part15(k, a, nb, st, occ);
+        case kWalkLightDyn:
+        case kWalkLightBin: return trace_part15(k, a, nb, st, occ);
         default: return hipErrorInvalidValue;
         }
     }
@@ -496,11 +497,12 @@ extern "C" int pvx_scene_set_traversal(pvx_scene *s, uint32_t ray_kind, uint32_t
     if (grid_blocks > 2 * s->grid)
         return fail(PVX_EINVAL, "grid_blocks above twice the resident blocks (" + std::to_string(2 * s->grid) + ")");
     const int cls = (ray_kind & 0xFFu) == PVX_RAY_SHADOW ? 1 : 0;
-    if (variant == PVX_TRAVERSAL_PACKET || variant == PVX_TRAVERSAL_FUSED || variant == PVX_TRAVERSAL_REFILL) {
+    if (variant == PVX_TRAVERSAL_PACKET || variant == PVX_TRAVERSAL_FUSED || variant == PVX_TRAVERSAL_REFILL ||
+        variant == PVX_TRAVERSAL_LIGHT_BINARY) {
         // pvx_render's light rays made in the kernel: triangle / sphere scenes
         if (cls != 1 || s->use_kd || kernel_level(s->view) != 0 || !s->view.wide)
             return fail(PVX_EINVAL,
-                        "light-ray walks PACKET / FUSED / REFILL need light rays of a triangle / sphere scene");
+                        "light-ray walks PACKET / FUSED / REFILL / LIGHT_BINARY need light rays of a triangle / sphere scene");
     } else if (variant != PVX_TRAVERSAL_DEFAULT) {
         auto v = scene_variants(s);
         if (std::find(v.begin(), v.end(), variant) == v.end())
@@ -589,7 +591,7 @@ extern "C" int pvx_scene_autotune(pvx_scene *s, const pvx_ray *rays, size_t n, u
 
 int pvx::light_rays_internal(pvx_scene *s, const LightGroup *groups, const LightRec *lights, uint32_t S,
                              uint32_t n_rays, uint8_t *vis, uint32_t *next, uint32_t grid_blocks,
-                             const uint32_t *count, hipStream_t st) {
+                             const uint32_t *count, bool binary, hipStream_t st) {
     if (n_rays == 0) return PVX_OK;
     int r;
     int32_t *spill = nullptr;
@@ -617,8 +619,9 @@ int pvx::light_rays_internal(pvx_scene *s, const LightGroup *groups, const Light
     // free lane at once mixes rays of distant points and is slower (the
     // 1.3M-triangle frame: 27.5 ms against 18.0 ms at 16 and 18.5 ms at 0)
     ta.gen_refill_below = 16;
-    HIP_TRY(trace_launch(TraceKernel{next ? kWalkLightDyn : kWalkLightWW, s->view.any_flags != 0, false, false}, ta,
-                         nb, st));
+    HIP_TRY(trace_launch(TraceKernel{binary ? kWalkLightBin : next ? kWalkLightDyn : kWalkLightWW,
+                                     s->view.any_flags != 0, false, false},
+                         ta, nb, st));
     return PVX_OK;
 }
 

@@ -1016,7 +1016,8 @@ static int render_frame(pvx_scene *s, const pvx_camera *cam, uint32_t S, const d
             next = static_cast<uint32_t *>(b.refill.p);
             k_zero_word<<<1, 64, 0, ls>>>(next);
         }
-        return light_rays_internal(s, groups, sh.d_lights, smax, (uint32_t)n_launch, vl, next, grid, count, ls);
+        return light_rays_internal(s, groups, sh.d_lights, smax, (uint32_t)n_launch, vl, next, grid, count,
+                                   variant == PVX_TRAVERSAL_LIGHT_BINARY, ls);
     };
     while (n > 0 && bounces < max_bounces) {
         const int par = (int)(bounces & 1u);
@@ -1100,9 +1101,11 @@ static int render_frame(pvx_scene *s, const pvx_camera *cam, uint32_t S, const d
                     struct Cand {
                         uint32_t variant, grid;
                         float ms;
-                    } cands[4] = {{PVX_TRAVERSAL_FUSED, 2 * s->grid, 0.0f},
+                    } cands[6] = {{PVX_TRAVERSAL_FUSED, 2 * s->grid, 0.0f},
                                   {PVX_TRAVERSAL_FUSED, s->grid, 0.0f},
                                   {PVX_TRAVERSAL_REFILL, s->grid, 0.0f},
+                                  {PVX_TRAVERSAL_LIGHT_BINARY, 2 * s->grid, 0.0f},
+                                  {PVX_TRAVERSAL_LIGHT_BINARY, s->grid, 0.0f},
                                   {PVX_TRAVERSAL_PACKET, s->grid, 0.0f}};
                     // warm-up: the first launch is slower
                     if ((rc = light_pass(hb.groups, PVX_TRAVERSAL_FUSED, 0, nr, nullptr))) return rc;

@@ -58,6 +58,16 @@ __global__ __launch_bounds__(kBlock, 6) void k_light_ww(TraceArgs a) {
                                                             a.gen_lights, a.gen_S, a.gen_vis);
 }
 
+// The same light rays through the binary tree (level-0 leaf tests only).
+template <bool kFlags>
+__global__ __launch_bounds__(kBlock) void k_light_bin(TraceArgs a) {
+    __shared__ int32_t s_stack[kStackLds * kBlock];
+    const uint32_t n = a.gen_count ? a.gen_count[0] * a.gen_S : a.n;
+    trace_closest_body<0, kFlags, false, false, false, true, true>(
+        a.sv, nullptr, nullptr, nullptr, nullptr, nullptr, PVX_RAY_SHADOW | kRayOcclusion, nullptr, n, nullptr, a.spill,
+        nullptr, nullptr, s_stack, a.gen_groups, a.gen_lights, a.gen_S, a.gen_vis);
+}
+
 // Light rays made in the kernel with refill: once no more than
 // a.gen_refill_below lanes of a wavefront still hold rays, its free lanes
 // take the next unstarted rays (one atomic per wavefront on a launch-wide
@@ -551,6 +561,8 @@ hipError_t PVX_FN(PVX_TRACE_PART)(const TraceKernel &k, const TraceArgs &a, uint
 #elif PVX_TRACE_PART == 15 // light rays made in the kernel
     if (k.walk == kWalkLightDyn)
         return k.flags ? run(k_light_dyn<true>, a, nb, st, occ) : run(k_light_dyn<false>, a, nb, st, occ);
+    if (k.walk == kWalkLightBin)
+        return k.flags ? run(k_light_bin<true>, a, nb, st, occ) : run(k_light_bin<false>, a, nb, st, occ);
     return k.flags ? run(k_light_ww<true>, a, nb, st, occ) : run(k_light_ww<false>, a, nb, st, occ);
 #else
 #error "PVX_TRACE_PART out of range"

@@ -101,6 +101,7 @@ def test_render_autotune_same_image(gpu, name):
     assert st["closest_rays"] == st0["closest_rays"] and st["shadow_rays"] == st0["shadow_rays"]
     assert tune[0]["variant"] in V_ALL and tune[0]["best_ms"] > 0
     # light rays of these opaque triangle / sphere scenes go in packets
-    assert tune[1]["variant"] in (A.PVX_TRAVERSAL_PACKET, A.PVX_TRAVERSAL_FUSED, A.PVX_TRAVERSAL_REFILL) and tune[1]["best_ms"] > 0
+    assert tune[1]["variant"] in (A.PVX_TRAVERSAL_PACKET, A.PVX_TRAVERSAL_FUSED, A.PVX_TRAVERSAL_REFILL,
+                                  A.PVX_TRAVERSAL_LIGHT_BINARY) and tune[1]["best_ms"] > 0
     again, _ = gs.render()
     assert np.array_equal(again, ref)
