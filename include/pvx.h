@@ -390,7 +390,8 @@ enum {
     PVX_TRAVERSAL_REFILL = 6,    /* ... while-while per lane, a wavefront refilling
                                     its free lanes with the next unstarted rays once
                                     16 or fewer of its lanes still hold rays         */
-    PVX_TRAVERSAL_LIGHT_BINARY = 7 /* ... through the binary tree, per lane       */
+    PVX_TRAVERSAL_LIGHT_BINARY = 7, /* ... through the binary tree, per lane      */
+    PVX_TRAVERSAL_BINARY_WW = 8  /* binary tree, while-while (triangle/sphere scenes) */
 };
 
 typedef struct pvx_tune_result {

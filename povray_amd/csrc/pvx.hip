@@ -107,7 +107,8 @@ hipError_t pvx::trace_launch(const TraceKernel &k, const TraceArgs &a, uint32_t 
         case kWalkStep4:
         case kWalkWW4: return trace_part0(k, a, nb, st, occ);
         case kWalkBinary: return trace_part1(k, a, nb, st, occ);
-        case kWalkBinary0: return trace_part16(k, a, nb, st, occ);
+        case kWalkBinary0:
+        case kWalkBinary0WW: return trace_part16(k, a, nb, st, occ);
         case kWalkComplex:
             return (k.flags ? (k.count ? trace_part5 : trace_part4) : (k.count ? trace_part3 : trace_part2))(k, a, nb, st,
                                                                                                           occ);
@@ -446,6 +447,8 @@ static int launch_variant(pvx_scene *s, const pvx_ray *rays, const double *max_d
         k.walk = kWalkRevolution;
     else if (level == 2)
         k.walk = kWalkComplex;
+    else if (level == 0 && variant == PVX_TRAVERSAL_BINARY_WW)
+        k.walk = kWalkBinary0WW;
     else if (level == 1 || variant == PVX_TRAVERSAL_BINARY)
         k.walk = level == 0 ? kWalkBinary0 : kWalkBinary;
     else if (variant == PVX_TRAVERSAL_WIDE_WW ||
@@ -478,7 +481,8 @@ static int launch_closest(pvx_scene *s, const pvx_ray *rays, const double *max_d
     const int cls = (ray_kind & 0xFFu) == PVX_RAY_SHADOW ? 1 : 0; // the autotuner's choice for this ray class
     const uint32_t grid = s->tune_grid[cls] ? s->tune_grid[cls] : s->grid;
     uint32_t variant = s->tune_variant[cls];
-    if (variant >= PVX_TRAVERSAL_PACKET) variant = PVX_TRAVERSAL_DEFAULT; // render-only
+    if (variant >= PVX_TRAVERSAL_PACKET && variant <= PVX_TRAVERSAL_LIGHT_BINARY)
+        variant = PVX_TRAVERSAL_DEFAULT; // render-only light walks
     return launch_variant<kCount>(s, rays, max_depth, ray_kind, kinds, n, hits, nv, pt,
                                   static_cast<hipStream_t>(stream), variant, grid);
 }
@@ -489,7 +493,7 @@ static std::vector<uint32_t> scene_variants(const pvx_scene *s) {
     const uint32_t level = kernel_level(s->view);
     if (s->use_kd || level >= 2) return {PVX_TRAVERSAL_DEFAULT};
     if (level == 1) return {PVX_TRAVERSAL_BINARY};
-    return {PVX_TRAVERSAL_WIDE_WW, PVX_TRAVERSAL_WIDE_STEP, PVX_TRAVERSAL_BINARY};
+    return {PVX_TRAVERSAL_WIDE_WW, PVX_TRAVERSAL_WIDE_STEP, PVX_TRAVERSAL_BINARY, PVX_TRAVERSAL_BINARY_WW};
 }
 
 extern "C" int pvx_scene_set_traversal(pvx_scene *s, uint32_t ray_kind, uint32_t variant, uint32_t grid_blocks) {

@@ -866,7 +866,8 @@ static int render_frame(pvx_scene *s, const pvx_camera *cam, uint32_t S, const d
     // 4-wide tree, every material opaque, unless the caller picked another
     // walk for light rays
     const bool packet = sh.all_opaque && s->view.has_complex == 0 && s->view.wide && !s->use_kd &&
-                        (s->tune_variant[1] == PVX_TRAVERSAL_DEFAULT || s->tune_variant[1] >= PVX_TRAVERSAL_PACKET);
+                        (s->tune_variant[1] == PVX_TRAVERSAL_DEFAULT ||
+                         (s->tune_variant[1] >= PVX_TRAVERSAL_PACKET && s->tune_variant[1] <= PVX_TRAVERSAL_LIGHT_BINARY));
     uint32_t smax = 1; // rays per group: the largest light grid
     for (const LightRec &lt : sh.lights) smax = std::max(smax, lt.n1 * lt.n2);
     auto ensure_packet = [&](size_t nh) -> int {
@@ -1057,7 +1058,7 @@ static int render_frame(pvx_scene *s, const pvx_camera *cam, uint32_t S, const d
         // suits them may differ: each keeps its own choice.
         const int lcls = bounces == 0 ? 0 : 1;
         uint32_t lvariant = s->light_variant[lcls], lgrid = s->light_grid[lcls];
-        if (lvariant < PVX_TRAVERSAL_PACKET) lvariant = PVX_TRAVERSAL_FUSED;
+        if (lvariant < PVX_TRAVERSAL_PACKET || lvariant > PVX_TRAVERSAL_LIGHT_BINARY) lvariant = PVX_TRAVERSAL_FUSED;
         // outside the tuning bounces the light pass starts at once, with the
         // group count read on the device (no wait for the host's read-back)
         const uint64_t nr_bound = (uint64_t)n * nl * smax;

@@ -401,6 +401,90 @@ __device__ __forceinline__ void trace_closest_body(const SceneView &sv0, const S
                         step_leaf();
                 }
             }
+        } else if constexpr (kBin && kWW) {
+            // triangle / sphere scene, binary tree, while-while: the lanes of
+            // a wavefront step through inner nodes together, then leaves
+            auto bin_node = [&]() {
+                if (kCount) cnt_nodes++;
+                const float4 *np = reinterpret_cast<const float4 *>(sv.nodes + node);
+                const float4 a = np[0], b = np[1], c = np[2];
+                const int4 l = reinterpret_cast<const int4 *>(np)[3];
+                float x0 = fmaf(a.x, rf.ix, -rf.oix), x1 = fmaf(a.y, rf.ix, -rf.oix);
+                float y0 = fmaf(a.z, rf.iy, -rf.oiy), y1 = fmaf(a.w, rf.iy, -rf.oiy);
+                float z0 = fmaf(c.x, rf.iz, -rf.oiz), z1 = fmaf(c.y, rf.iz, -rf.oiz);
+                const float tn0 = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
+                const float tf0 = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tb));
+                x0 = fmaf(b.x, rf.ix, -rf.oix);
+                x1 = fmaf(b.y, rf.ix, -rf.oix);
+                y0 = fmaf(b.z, rf.iy, -rf.oiy);
+                y1 = fmaf(b.w, rf.iy, -rf.oiy);
+                z0 = fmaf(c.z, rf.iz, -rf.oiz);
+                z1 = fmaf(c.w, rf.iz, -rf.oiz);
+                const float tn1 = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
+                const float tf1 = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tb));
+                const bool h0 = tn0 <= tf0 * 1.000001f;
+                const bool h1 = tn1 <= tf1 * 1.000001f;
+                if (h0 && h1) {
+                    int32_t nearc = l.x, farc = l.y;
+                    if (tn1 < tn0) {
+                        nearc = l.y;
+                        farc = l.x;
+                    }
+                    if (sp < kStackLds)
+                        s_stack[sp * kBlock + lane] = farc;
+                    else
+                        my_spill[sp - kStackLds] = farc;
+                    sp++;
+                    node = nearc;
+                } else if (h0) {
+                    node = l.x;
+                } else if (h1) {
+                    node = l.y;
+                } else {
+                    PVX_POP();
+                }
+            };
+            auto bin_leaf = [&]() {
+                // leaf: ~node = first << 3 | (count - 1)
+                const uint32_t leaf = (uint32_t)(~node);
+                const uint32_t first = leaf >> 3, count = (leaf & 7u) + 1u;
+                if (kCount) cnt_prims += count;
+                for (uint32_t k = 0; k < count; k++) {
+                    const PrimRec &p = sv.prims[first + k];
+                    const uint32_t ka = p.kind_axis;
+                    const uint32_t pk = ka & 15u;
+                    const int32_t obj = p.obj;
+                    if (kFlags && !precondition(sv.obj_flags[obj], rk)) continue;
+                    if (pk == kPrimTriangle) {
+                        double dep;
+                        if (mesh_tri_hit_l<0>(sv, p, ka, o, d, dep) && depth_ok(dep, shadow) &&
+                            better(dep, obj, p.tri.prim, bt, bo, bp)) {
+                            bt = dep;
+                            bo = obj;
+                            bp = p.tri.prim;
+                        }
+                    } else if (pk == kPrimSphere) {
+                        double hs[2];
+                        const int nh = sphere_hits(d3(p.sph.c[0], p.sph.c[1], p.sph.c[2]), p.sph.r, o, d, hs);
+                        for (int j = 0; j < nh; j++)
+                            if (depth_ok(hs[j], shadow) && better(hs[j], obj, 0, bt, bo, bp)) {
+                                bt = hs[j];
+                                bo = obj;
+                                bp = 0;
+                            }
+                    }
+                }
+                tb = (float)bt * 1.000001f;
+                if (occl && bo >= 0 && bt < occl_lim) {
+                    node = kEmpty;
+                    return;
+                }
+                PVX_POP();
+            };
+            while (node != kEmpty) {
+                while (node >= 0) bin_node();
+                while (node < 0 && node != kEmpty) bin_leaf();
+            }
         } else {
             while (node != kEmpty) {
                 if (node >= 0) {

@@ -27,12 +27,13 @@ __global__ __launch_bounds__(kBlock) void k_trace_closest(TraceArgs a) {
 // Level-0 scenes (triangles, spheres, planes) through the binary tree.
 // 92 VGPRs, 5 wavefronts per SIMD (the level-1 binary kernel, with box
 // leaves, needs 108 and spills; forcing 6 here spills and is slower).
-template <bool kFlags, bool kCount>
+// kWW: while-while passes (inner nodes, then leaves, lane by lane).
+template <bool kWW, bool kFlags, bool kCount>
 __global__ __launch_bounds__(kBlock) void k_trace_closest_bin0(TraceArgs a) {
     __shared__ int32_t s_stack[kStackLds * kBlock];
-    trace_closest_body<0, kFlags, kCount, false, false, false, true>(a.sv, nullptr, nullptr, nullptr, a.rays,
-                                                                      a.max_depth, a.kind, a.kinds, a.n, a.hits,
-                                                                      a.spill, a.node_visits, a.prim_tests, s_stack);
+    trace_closest_body<0, kFlags, kCount, kWW, false, false, true>(a.sv, nullptr, nullptr, nullptr, a.rays,
+                                                                    a.max_depth, a.kind, a.kinds, a.n, a.hits,
+                                                                    a.spill, a.node_visits, a.prim_tests, s_stack);
 }
 
 // Triangle scenes, camera and bounce rays: while-while traversal (78 VGPRs
@@ -557,7 +558,7 @@ hipError_t PVX_FN(PVX_TRACE_PART)(const TraceKernel &k, const TraceArgs &a, uint
 #elif PVX_TRACE_PART == 14 // scene batches with lathes or sors
     return PVX_FC(k_trace_closest_multi, 3, );
 #elif PVX_TRACE_PART == 16 // single scene, level-0 primitives on the binary tree
-    return PVX_FC(k_trace_closest_bin0, );
+    return k.walk == kWalkBinary0WW ? PVX_FC(k_trace_closest_bin0, true, ) : PVX_FC(k_trace_closest_bin0, false, );
 #elif PVX_TRACE_PART == 15 // light rays made in the kernel
     if (k.walk == kWalkLightDyn)
         return k.flags ? run(k_light_dyn<true>, a, nb, st, occ) : run(k_light_dyn<false>, a, nb, st, occ);

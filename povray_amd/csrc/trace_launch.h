@@ -39,7 +39,8 @@ enum TraceWalk : uint32_t {
     kWalkLightWW = 6, // light rays made in the kernel, 4-wide tree while-while (occlusion)
     kWalkLightDyn = 7, // the same, each lane taking a new ray as soon as its ray ends
     kWalkBinary0 = 8,  // level-0 scene (triangles, spheres) through the binary tree
-    kWalkLightBin = 9  // light rays made in the kernel, binary tree (occlusion)
+    kWalkLightBin = 9,  // light rays made in the kernel, binary tree (occlusion)
+    kWalkBinary0WW = 10 // kWalkBinary0 in while-while passes
 };
 
 struct TraceKernel {

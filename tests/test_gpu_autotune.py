@@ -9,7 +9,7 @@ from povray_amd import pvx, scenes
 
 pytestmark = pytest.mark.gpu
 
-V_ALL = (A.PVX_TRAVERSAL_WIDE_WW, A.PVX_TRAVERSAL_WIDE_STEP, A.PVX_TRAVERSAL_BINARY)
+V_ALL = (A.PVX_TRAVERSAL_WIDE_WW, A.PVX_TRAVERSAL_WIDE_STEP, A.PVX_TRAVERSAL_BINARY, A.PVX_TRAVERSAL_BINARY_WW)
 
 
 def _same(a, b, what):
