@@ -75,9 +75,11 @@ __global__ __launch_bounds__(kBlock) void k_light_bin(TraceArgs a) {
 // counter), instead of waiting for the wavefront's slowest ray (Aila and
 // Laine, "Understanding the efficiency of ray traversal on GPUs", HPG 2009:
 // persistent threads with fetching and ray replacement).  Same rays, same
-// tests and same visibility bytes as k_light_ww.
+// tests and same visibility bytes as k_light_ww.  5 wavefronts per SIMD
+// (92-96 VGPRs, no spills; held to 80 for 6 it spills and the camera-hit
+// light rays of the 1.3M-triangle frame take 12.2 instead of 11.3 ms).
 template <bool kFlags>
-__global__ __launch_bounds__(kBlock, 6) void k_light_dyn(TraceArgs a) {
+__global__ __launch_bounds__(kBlock, 5) void k_light_dyn(TraceArgs a) {
     __shared__ int32_t s_stack[kStackLds * kBlock];
     const SceneView &sv = a.sv;
     const uint32_t n = a.gen_count ? a.gen_count[0] * a.gen_S : a.n;
