@@ -391,7 +391,9 @@ enum {
                                     its free lanes with the next unstarted rays once
                                     16 or fewer of its lanes still hold rays         */
     PVX_TRAVERSAL_LIGHT_BINARY = 7, /* ... through the binary tree, per lane      */
-    PVX_TRAVERSAL_BINARY_WW = 8  /* binary tree, while-while (triangle/sphere scenes) */
+    PVX_TRAVERSAL_BINARY_WW = 8, /* binary tree, while-while (triangle/sphere scenes) */
+    PVX_TRAVERSAL_REFILL_BINARY = 9 /* pvx_render's light rays as REFILL, through the
+                                       binary tree                                  */
 };
 
 typedef struct pvx_tune_result {

@@ -102,7 +102,7 @@ PVX_BUILDER_HOST_SAH, PVX_BUILDER_GPU = 0, 1
 # traversal variants (pvx_scene_autotune)
 PVX_TRAVERSAL_DEFAULT, PVX_TRAVERSAL_WIDE_WW, PVX_TRAVERSAL_WIDE_STEP, PVX_TRAVERSAL_BINARY = 0, 1, 2, 3
 PVX_TRAVERSAL_PACKET, PVX_TRAVERSAL_FUSED, PVX_TRAVERSAL_REFILL, PVX_TRAVERSAL_LIGHT_BINARY = 4, 5, 6, 7
-PVX_TRAVERSAL_BINARY_WW = 8
+PVX_TRAVERSAL_BINARY_WW, PVX_TRAVERSAL_REFILL_BINARY = 8, 9
 
 
 class PvxTuneResult(C.Structure):

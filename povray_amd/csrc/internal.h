@@ -16,6 +16,12 @@ namespace pvx {
 
 int fail(int code, const std::string &msg);
 
+// pvx_render's light walks (light rays made in the traversal kernel).
+inline bool is_light_walk(uint32_t v) {
+    return v == PVX_TRAVERSAL_PACKET || v == PVX_TRAVERSAL_FUSED || v == PVX_TRAVERSAL_REFILL ||
+           v == PVX_TRAVERSAL_LIGHT_BINARY || v == PVX_TRAVERSAL_REFILL_BINARY;
+}
+
 #define HIP_TRY(expr)                                                                        \
     do {                                                                                     \
         hipError_t e_ = (expr);                                                              \

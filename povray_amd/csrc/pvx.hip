@@ -116,7 +116,8 @@ hipError_t pvx::trace_launch(const TraceKernel &k, const TraceArgs &a, uint32_t 
         case kWalkRevolution: return trace_part13(k, a, nb, st, occ);
         case kWalkLightWW:
         case kWalkLightDyn:
-        case kWalkLightBin: return trace_part15(k, a, nb, st, occ);
+        case kWalkLightBin:
+        case kWalkLightDynBin: return trace_part15(k, a, nb, st, occ);
         default: return hipErrorInvalidValue;
         }
     }
@@ -481,8 +482,7 @@ static int launch_closest(pvx_scene *s, const pvx_ray *rays, const double *max_d
     const int cls = (ray_kind & 0xFFu) == PVX_RAY_SHADOW ? 1 : 0; // the autotuner's choice for this ray class
     const uint32_t grid = s->tune_grid[cls] ? s->tune_grid[cls] : s->grid;
     uint32_t variant = s->tune_variant[cls];
-    if (variant >= PVX_TRAVERSAL_PACKET && variant <= PVX_TRAVERSAL_LIGHT_BINARY)
-        variant = PVX_TRAVERSAL_DEFAULT; // render-only light walks
+    if (is_light_walk(variant)) variant = PVX_TRAVERSAL_DEFAULT; // render-only light walks
     return launch_variant<kCount>(s, rays, max_depth, ray_kind, kinds, n, hits, nv, pt,
                                   static_cast<hipStream_t>(stream), variant, grid);
 }
@@ -501,12 +501,12 @@ extern "C" int pvx_scene_set_traversal(pvx_scene *s, uint32_t ray_kind, uint32_t
     if (grid_blocks > 2 * s->grid)
         return fail(PVX_EINVAL, "grid_blocks above twice the resident blocks (" + std::to_string(2 * s->grid) + ")");
     const int cls = (ray_kind & 0xFFu) == PVX_RAY_SHADOW ? 1 : 0;
-    if (variant == PVX_TRAVERSAL_PACKET || variant == PVX_TRAVERSAL_FUSED || variant == PVX_TRAVERSAL_REFILL ||
-        variant == PVX_TRAVERSAL_LIGHT_BINARY) {
+    if (is_light_walk(variant)) {
         // pvx_render's light rays made in the kernel: triangle / sphere scenes
         if (cls != 1 || s->use_kd || kernel_level(s->view) != 0 || !s->view.wide)
             return fail(PVX_EINVAL,
-                        "light-ray walks PACKET / FUSED / REFILL / LIGHT_BINARY need light rays of a triangle / sphere scene");
+                        "light-ray walks PACKET / FUSED / REFILL / LIGHT_BINARY / REFILL_BINARY need light rays of a "
+                        "triangle / sphere scene");
     } else if (variant != PVX_TRAVERSAL_DEFAULT) {
         auto v = scene_variants(s);
         if (std::find(v.begin(), v.end(), variant) == v.end())
@@ -623,7 +623,7 @@ int pvx::light_rays_internal(pvx_scene *s, const LightGroup *groups, const Light
     // free lane at once mixes rays of distant points and is slower (the
     // 1.3M-triangle frame: 27.5 ms against 18.0 ms at 16 and 18.5 ms at 0)
     ta.gen_refill_below = 16;
-    HIP_TRY(trace_launch(TraceKernel{binary ? kWalkLightBin : next ? kWalkLightDyn : kWalkLightWW,
+    HIP_TRY(trace_launch(TraceKernel{next ? (binary ? kWalkLightDynBin : kWalkLightDyn) : binary ? kWalkLightBin : kWalkLightWW,
                                      s->view.any_flags != 0, false, false},
                          ta, nb, st));
     return PVX_OK;

@@ -867,7 +867,7 @@ static int render_frame(pvx_scene *s, const pvx_camera *cam, uint32_t S, const d
     // walk for light rays
     const bool packet = sh.all_opaque && s->view.has_complex == 0 && s->view.wide && !s->use_kd &&
                         (s->tune_variant[1] == PVX_TRAVERSAL_DEFAULT ||
-                         (s->tune_variant[1] >= PVX_TRAVERSAL_PACKET && s->tune_variant[1] <= PVX_TRAVERSAL_LIGHT_BINARY));
+                         is_light_walk(s->tune_variant[1]));
     uint32_t smax = 1; // rays per group: the largest light grid
     for (const LightRec &lt : sh.lights) smax = std::max(smax, lt.n1 * lt.n2);
     auto ensure_packet = [&](size_t nh) -> int {
@@ -1013,12 +1013,12 @@ static int render_frame(pvx_scene *s, const pvx_camera *cam, uint32_t S, const d
             return PVX_OK;
         }
         uint32_t *next = nullptr;
-        if (variant == PVX_TRAVERSAL_REFILL) {
+        if (variant == PVX_TRAVERSAL_REFILL || variant == PVX_TRAVERSAL_REFILL_BINARY) {
             next = static_cast<uint32_t *>(b.refill.p);
             k_zero_word<<<1, 64, 0, ls>>>(next);
         }
         return light_rays_internal(s, groups, sh.d_lights, smax, (uint32_t)n_launch, vl, next, grid, count,
-                                   variant == PVX_TRAVERSAL_LIGHT_BINARY, ls);
+                                   variant == PVX_TRAVERSAL_LIGHT_BINARY || variant == PVX_TRAVERSAL_REFILL_BINARY, ls);
     };
     while (n > 0 && bounces < max_bounces) {
         const int par = (int)(bounces & 1u);
@@ -1058,7 +1058,7 @@ static int render_frame(pvx_scene *s, const pvx_camera *cam, uint32_t S, const d
         // suits them may differ: each keeps its own choice.
         const int lcls = bounces == 0 ? 0 : 1;
         uint32_t lvariant = s->light_variant[lcls], lgrid = s->light_grid[lcls];
-        if (lvariant < PVX_TRAVERSAL_PACKET || lvariant > PVX_TRAVERSAL_LIGHT_BINARY) lvariant = PVX_TRAVERSAL_FUSED;
+        if (!is_light_walk(lvariant)) lvariant = PVX_TRAVERSAL_FUSED;
         // outside the tuning bounces the light pass starts at once, with the
         // group count read on the device (no wait for the host's read-back)
         const uint64_t nr_bound = (uint64_t)n * nl * smax;
@@ -1102,9 +1102,10 @@ static int render_frame(pvx_scene *s, const pvx_camera *cam, uint32_t S, const d
                     struct Cand {
                         uint32_t variant, grid;
                         float ms;
-                    } cands[6] = {{PVX_TRAVERSAL_FUSED, 2 * s->grid, 0.0f},
+                    } cands[7] = {{PVX_TRAVERSAL_FUSED, 2 * s->grid, 0.0f},
                                   {PVX_TRAVERSAL_FUSED, s->grid, 0.0f},
                                   {PVX_TRAVERSAL_REFILL, s->grid, 0.0f},
+                                  {PVX_TRAVERSAL_REFILL_BINARY, s->grid, 0.0f},
                                   {PVX_TRAVERSAL_LIGHT_BINARY, 2 * s->grid, 0.0f},
                                   {PVX_TRAVERSAL_LIGHT_BINARY, s->grid, 0.0f},
                                   {PVX_TRAVERSAL_PACKET, s->grid, 0.0f}};

@@ -78,7 +78,7 @@ __global__ __launch_bounds__(kBlock) void k_light_bin(TraceArgs a) {
 // tests and same visibility bytes as k_light_ww.  5 wavefronts per SIMD
 // (92-96 VGPRs, no spills; held to 80 for 6 it spills and the camera-hit
 // light rays of the 1.3M-triangle frame take 12.2 instead of 11.3 ms).
-template <bool kFlags>
+template <bool kFlags, bool kBin>
 __global__ __launch_bounds__(kBlock, 5) void k_light_dyn(TraceArgs a) {
     __shared__ int32_t s_stack[kStackLds * kBlock];
     const SceneView &sv = a.sv;
@@ -154,7 +154,7 @@ __global__ __launch_bounds__(kBlock, 5) void k_light_dyn(TraceArgs a) {
                         rf.oiy = (float)o.y * rf.iy;
                         rf.oiz = (float)o.z * rf.iz;
                         tb = (float)bt * 1.000001f;
-                        node = (bo >= 0 && bt < occl_lim) ? kEmpty : sv.root4;
+                        node = (bo >= 0 && bt < occl_lim) ? kEmpty : kBin ? sv.root : sv.root4;
                         sp = 0;
                         have = true;
                     }
@@ -167,7 +167,37 @@ __global__ __launch_bounds__(kBlock, 5) void k_light_dyn(TraceArgs a) {
         }
         if (have) {
             // one while-while round: inner nodes, then leaves
-            while (node >= 0) {
+            while (kBin && node >= 0) { // binary tree: near child first, the far one pushed
+                const float4 *np = reinterpret_cast<const float4 *>(sv.nodes + node);
+                const float4 na = np[0], nb = np[1], nc = np[2];
+                const int4 nl = reinterpret_cast<const int4 *>(np)[3];
+                float x0 = fmaf(na.x, rf.ix, -rf.oix), x1 = fmaf(na.y, rf.ix, -rf.oix);
+                float y0 = fmaf(na.z, rf.iy, -rf.oiy), y1 = fmaf(na.w, rf.iy, -rf.oiy);
+                float z0 = fmaf(nc.x, rf.iz, -rf.oiz), z1 = fmaf(nc.y, rf.iz, -rf.oiz);
+                const float tn0 = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
+                const float tf0 = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tb));
+                x0 = fmaf(nb.x, rf.ix, -rf.oix);
+                x1 = fmaf(nb.y, rf.ix, -rf.oix);
+                y0 = fmaf(nb.z, rf.iy, -rf.oiy);
+                y1 = fmaf(nb.w, rf.iy, -rf.oiy);
+                z0 = fmaf(nc.z, rf.iz, -rf.oiz);
+                z1 = fmaf(nc.w, rf.iz, -rf.oiz);
+                const float tn1 = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
+                const float tf1 = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tb));
+                const bool h0 = tn0 <= tf0 * 1.000001f, h1 = tn1 <= tf1 * 1.000001f;
+                if (h0 && h1) {
+                    const bool swap = tn1 < tn0;
+                    push(swap ? nl.x : nl.y);
+                    node = swap ? nl.y : nl.x;
+                } else if (h0) {
+                    node = nl.x;
+                } else if (h1) {
+                    node = nl.y;
+                } else {
+                    pop();
+                }
+            }
+            while (!kBin && node >= 0) {
                 const float4 *np = reinterpret_cast<const float4 *>(sv.nodes4 + node);
                 const float4 xl = np[0], xh = np[1], yl = np[2], yh = np[3], zl = np[4], zh = np[5];
                 const int4 cl = reinterpret_cast<const int4 *>(np)[6];
@@ -563,7 +593,9 @@ hipError_t PVX_FN(PVX_TRACE_PART)(const TraceKernel &k, const TraceArgs &a, uint
     return k.walk == kWalkBinary0WW ? PVX_FC(k_trace_closest_bin0, true, ) : PVX_FC(k_trace_closest_bin0, false, );
 #elif PVX_TRACE_PART == 15 // light rays made in the kernel
     if (k.walk == kWalkLightDyn)
-        return k.flags ? run(k_light_dyn<true>, a, nb, st, occ) : run(k_light_dyn<false>, a, nb, st, occ);
+        return k.flags ? run(k_light_dyn<true, false>, a, nb, st, occ) : run(k_light_dyn<false, false>, a, nb, st, occ);
+    if (k.walk == kWalkLightDynBin)
+        return k.flags ? run(k_light_dyn<true, true>, a, nb, st, occ) : run(k_light_dyn<false, true>, a, nb, st, occ);
     if (k.walk == kWalkLightBin)
         return k.flags ? run(k_light_bin<true>, a, nb, st, occ) : run(k_light_bin<false>, a, nb, st, occ);
     return k.flags ? run(k_light_ww<true>, a, nb, st, occ) : run(k_light_ww<false>, a, nb, st, occ);

@@ -40,7 +40,8 @@ enum TraceWalk : uint32_t {
     kWalkLightDyn = 7, // the same, each lane taking a new ray as soon as its ray ends
     kWalkBinary0 = 8,  // level-0 scene (triangles, spheres) through the binary tree
     kWalkLightBin = 9,  // light rays made in the kernel, binary tree (occlusion)
-    kWalkBinary0WW = 10 // kWalkBinary0 in while-while passes
+    kWalkBinary0WW = 10, // kWalkBinary0 in while-while passes
+    kWalkLightDynBin = 11 // kWalkLightDyn through the binary tree
 };
 
 struct TraceKernel {

@@ -102,6 +102,6 @@ def test_render_autotune_same_image(gpu, name):
     assert tune[0]["variant"] in V_ALL and tune[0]["best_ms"] > 0
     # light rays of these opaque triangle / sphere scenes go in packets
     assert tune[1]["variant"] in (A.PVX_TRAVERSAL_PACKET, A.PVX_TRAVERSAL_FUSED, A.PVX_TRAVERSAL_REFILL,
-                                  A.PVX_TRAVERSAL_LIGHT_BINARY) and tune[1]["best_ms"] > 0
+                                  A.PVX_TRAVERSAL_LIGHT_BINARY, A.PVX_TRAVERSAL_REFILL_BINARY) and tune[1]["best_ms"] > 0
     again, _ = gs.render()
     assert np.array_equal(again, ref)

@@ -95,7 +95,8 @@ def test_gpu_build_all_intersections_and_render(gpu):
     assert np.array_equal(idv, ih)
     # the autotuner works over the GPU-built trees too
     t = dev.autotune(scenes.camera_rays_np(s.camera))
-    assert t["variant"] in (A.PVX_TRAVERSAL_WIDE_WW, A.PVX_TRAVERSAL_WIDE_STEP, A.PVX_TRAVERSAL_BINARY)
+    assert t["variant"] in (A.PVX_TRAVERSAL_WIDE_WW, A.PVX_TRAVERSAL_WIDE_STEP, A.PVX_TRAVERSAL_BINARY,
+                            A.PVX_TRAVERSAL_BINARY_WW)
     _same(dev.find_intersection(rays), host.find_intersection(rays), "tuned GPU-built tree")
 
 

@@ -37,10 +37,10 @@ def test_packet_light_rays_equal_per_ray_walk(gpu, name):
     print(f"[packet {name}] light tune {tune[1]}; shadow {st['shadow_rays']} rays in {st['shadow_ms']:.3f} ms",
           flush=True)
     assert tune[1]["variant"] in (A.PVX_TRAVERSAL_PACKET, A.PVX_TRAVERSAL_FUSED, A.PVX_TRAVERSAL_REFILL,
-                                  A.PVX_TRAVERSAL_LIGHT_BINARY)
+                                  A.PVX_TRAVERSAL_LIGHT_BINARY, A.PVX_TRAVERSAL_REFILL_BINARY)
     frames = {}
     for v in (A.PVX_TRAVERSAL_PACKET, A.PVX_TRAVERSAL_FUSED, A.PVX_TRAVERSAL_REFILL, A.PVX_TRAVERSAL_LIGHT_BINARY,
-              A.PVX_TRAVERSAL_WIDE_WW):
+              A.PVX_TRAVERSAL_REFILL_BINARY, A.PVX_TRAVERSAL_WIDE_WW):
         gs.set_traversal(v, 0, A.PVX_RAY_SHADOW)
         frames[v] = gs.render()
         print(f"[packet {name}] walk {v}: light rays {frames[v][1]['shadow_ms']:.3f} ms", flush=True)
