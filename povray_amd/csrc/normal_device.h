@@ -22,6 +22,9 @@ __device__ inline D3 normalize3(D3 a) {
 }
 
 // Object::Normal of the surface that was hit (raw, not yet flipped).
+// kRev false: the scene has no lathe / sor (their case, which retraces
+// the hit list, is left out, and with it a large per-lane stack).
+template <bool kRev = true>
 __device__ inline D3 leaf_normal(const NormalParams &p, uint32_t type, uint32_t index, double t, D3 o, D3 d,
                           D3 ip, bool child = false) {
     const SceneView &sv = p.sv;
@@ -90,43 +93,47 @@ __device__ inline D3 leaf_normal(const NormalParams &p, uint32_t type, uint32_t 
     }
     case PVX_LATHE:
     case PVX_SOR: { // Lathe::Normal (lathe.cpp:466-505), Sor::Normal (sor.cpp:540-585)
-        // the hit's segment and spline parameter: the hit list is computed
-        // again and the last entry at the hit's depth taken (Find_Intersection
-        // keeps the last of equal depths)
-        int side = 0;
-        double d1 = 0.0;
-        rev_visit(sv, index, o, d, child, [&](double dep, int sd, double w) {
-            if (dep == t) {
-                side = sd;
-                d1 = w;
-            }
-        });
-        const RevRec &s = sv.revs[index];
-        const Xform &x = sv.xforms[s.xf];
-        const D3 P = xf_point(x.inv, ip);
-        D3 n;
-        if ((s.flags & 7u) == PVX_SOR_CUBIC) {
-            const int part = side & 3;
-            if (part == kSorCurve) {
-                const RevSeg &g = sv.rev_segs[s.seg0 + (side >> 2)];
-                const double k = 0.5 * (P.y * (3.0 * g.a[0] * P.y + 2.0 * g.b[0]) + g.c[0]);
-                n = D3{P.x, -k, P.z};
-            } else {
-                n = D3{0.0, part == kSorBase ? -1.0 : 1.0, 0.0};
-            }
+        if constexpr (!kRev) {
+            return D3{0.0, 1.0, 0.0}; // not in such scenes
         } else {
-            const RevSeg &g = sv.rev_segs[s.seg0 + side];
-            double r = P.x * P.x + P.z * P.z;
-            if (r > kEpsilon) {
-                r = sqrt(r);
-                const double dx = d1 * (3.0 * g.a[0] * d1 + 2.0 * g.b[0]) + g.c[0];
-                const double dy = d1 * (3.0 * g.a[1] * d1 + 2.0 * g.b[1]) + g.c[1];
-                n = D3{dy * P.x, -dx * r, dy * P.z};
+            // the hit's segment and spline parameter: the hit list is computed
+            // again and the last entry at the hit's depth taken (Find_Intersection
+            // keeps the last of equal depths)
+            int side = 0;
+            double d1 = 0.0;
+            rev_visit(sv, index, o, d, child, [&](double dep, int sd, double w) {
+                if (dep == t) {
+                    side = sd;
+                    d1 = w;
+                }
+            });
+            const RevRec &s = sv.revs[index];
+            const Xform &x = sv.xforms[s.xf];
+            const D3 P = xf_point(x.inv, ip);
+            D3 n;
+            if ((s.flags & 7u) == PVX_SOR_CUBIC) {
+                const int part = side & 3;
+                if (part == kSorCurve) {
+                    const RevSeg &g = sv.rev_segs[s.seg0 + (side >> 2)];
+                    const double k = 0.5 * (P.y * (3.0 * g.a[0] * P.y + 2.0 * g.b[0]) + g.c[0]);
+                    n = D3{P.x, -k, P.z};
+                } else {
+                    n = D3{0.0, part == kSorBase ? -1.0 : 1.0, 0.0};
+                }
             } else {
-                n = D3{0.0, 1.0, 0.0};
+                const RevSeg &g = sv.rev_segs[s.seg0 + side];
+                double r = P.x * P.x + P.z * P.z;
+                if (r > kEpsilon) {
+                    r = sqrt(r);
+                    const double dx = d1 * (3.0 * g.a[0] * d1 + 2.0 * g.b[0]) + g.c[0];
+                    const double dy = d1 * (3.0 * g.a[1] * d1 + 2.0 * g.b[1]) + g.c[1];
+                    n = D3{dy * P.x, -dx * r, dy * P.z};
+                } else {
+                    n = D3{0.0, 1.0, 0.0};
+                }
             }
+            return normalize3(mtrans_normal(x.inv, n));
         }
-        return normalize3(mtrans_normal(x.inv, n));
     }
     case PVX_TRIANGLE: { // Triangle::Normal (triangle.cpp:641-644)
         const TriangleRec &t = sv.triangles[index];
@@ -150,6 +157,7 @@ __device__ inline D3 leaf_normal(const NormalParams &p, uint32_t type, uint32_t 
     }
 }
 
+template <bool kRev = true>
 __device__ inline D3 hit_normal(const NormalParams &p, const pvx_hit &h, D3 o, D3 d, D3 ip) {
     const SceneView &sv = p.sv;
     uint32_t type = p.obj_type[h.obj], index = p.obj_index[h.obj];
@@ -187,9 +195,9 @@ __device__ inline D3 hit_normal(const NormalParams &p, const pvx_hit &h, D3 o, D
     }
     if (type == PVX_CSG) {
         const CsgChild &c = sv.csg_children[(uint32_t)h.prim];
-        return leaf_normal(p, c.type, c.index, h.t, o, d, ip, true);
+        return leaf_normal<kRev>(p, c.type, c.index, h.t, o, d, ip, true);
     }
-    return leaf_normal(p, type, index, h.t, o, d, ip);
+    return leaf_normal<kRev>(p, type, index, h.t, o, d, ip);
 }
 
 

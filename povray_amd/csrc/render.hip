@@ -168,7 +168,8 @@ __device__ inline void write_child(pvx_ray *crays, PathState *cst, size_t slot, 
 // kPacket: light rays are not written; each lit (hit, light) appends its
 // shaded point and light to the group list and the traversal kernel makes
 // the rays.
-template <bool kPacket>
+// kRev: the scene has lathes / sors (their normal retraces the hit list).
+template <bool kPacket, bool kRev>
 __global__ void k_shade(RenderParams p, const pvx_ray *__restrict__ rays, const PathState *__restrict__ st,
                         const pvx_hit *__restrict__ hits, uint32_t n, float *__restrict__ contrib,
                         ShadeOut *__restrict__ shade, uint32_t *__restrict__ lslot, double *__restrict__ lterm,
@@ -197,7 +198,7 @@ __global__ void k_shade(RenderParams p, const pvx_ray *__restrict__ rays, const 
     const uint16_t lev = (uint16_t)(ps.level + ((ps.flags & kFContinued) ? 0 : 1));
     D3 o{r.o[0], r.o[1], r.o[2]}, d{r.d[0], r.d[1], r.d[2]};
     D3 ip = add3(o, mul3(d, h.t));
-    D3 nrm = hit_normal(p, h, o, d, ip);
+    D3 nrm = hit_normal<kRev>(p, h, o, d, ip);
     if (dot3(nrm, d) > 0.0) nrm = D3{-nrm.x, -nrm.y, -nrm.z};
     const MatRec m = p.mats[h.obj];
     // classic ambient: pigment * (ambient * ambient_light) * opacity
@@ -1045,7 +1046,9 @@ static int render_frame(pvx_scene *s, const pvx_camera *cam, uint32_t S, const d
         uint32_t *gcount = static_cast<uint32_t *>(b.gcount.p) + par;
         k_zero_counters<<<1, 64, 0, st>>>(counters);
         k_zero_word<<<1, 64, 0, st>>>(gcount);
-        (packet ? k_shade<true> : k_shade<false>)<<<(n + 127) / 128, 128, 0, st>>>(
+        const bool rev = s->view.has_complex >= 3; // lathes / sors in the scene
+        (packet ? (rev ? k_shade<true, true> : k_shade<true, false>)
+                : (rev ? k_shade<false, true> : k_shade<false, false>))<<<(n + 127) / 128, 128, 0, st>>>(
             p, rays, pst, static_cast<const pvx_hit *>(b.hits.p), n, hb.contrib, hb.shade, hb.lslot, hb.lterm,
             static_cast<pvx_ray *>(b.srays[0].p), static_cast<double *>(b.sdepth[0].p),
             static_cast<uint32_t *>(b.sowner[0].p), static_cast<float *>(b.scol.p),
