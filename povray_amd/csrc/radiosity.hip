@@ -204,6 +204,8 @@ struct ShadeParams : NormalParams {
 // Surface response at each hit: base = pigment*emission*opacity and
 // coef = pigment*opacity*diffuse, plus a depth+1 irradiance query when
 // radiosity is needed there (ComputeLightedTexture, trace.cpp:826-939).
+// kRev: the scene has lathes / sors (their normal retraces the hit list).
+template <bool kRev>
 __global__ void k_hit_queries(ShadeParams p, const pvx_ray *__restrict__ rays,
                               const pvx_hit *__restrict__ hits, uint32_t n, float wnext,
                               int want_ambient, float *__restrict__ base, float *__restrict__ coef,
@@ -222,7 +224,7 @@ __global__ void k_hit_queries(ShadeParams p, const pvx_ray *__restrict__ rays,
     const pvx_ray r = rays[i];
     D3 o{r.o[0], r.o[1], r.o[2]}, d{r.d[0], r.d[1], r.d[2]};
     D3 ip = add3(o, mul3(d, h.t));
-    D3 nrm = hit_normal(p, h, o, d, ip);
+    D3 nrm = hit_normal<kRev>(p, h, o, d, ip);
     if (dot3(nrm, d) > 0.0) nrm = D3{-nrm.x, -nrm.y, -nrm.z};
     const MatRec &m = p.mats[h.obj];
     float cf[3];
@@ -925,7 +927,7 @@ int RadiosityEngine::gather_trace(int depth, const RadQuery *q, const uint32_t *
     sp.obj_index = scene->d_obj_index;
     sp.mats = scene->shading.d_mats;
     sp.adc = rs.adc_bailout;
-    k_hit_queries<<<blocks(nr), 256, 0, stream>>>(sp, static_cast<const pvx_ray *>(S.rays.p),
+    (scene->view.has_complex >= 3 ? k_hit_queries<true> : k_hit_queries<false>)<<<blocks(nr), 256, 0, stream>>>(sp, static_cast<const pvx_ray *>(S.rays.p),
                                                   static_cast<const pvx_hit *>(S.hits.p), (uint32_t)nr,
                                                   wnext, deeper ? 1 : 0, static_cast<float *>(S.base.p),
                                                   static_cast<float *>(S.coef.p),
@@ -1525,7 +1527,7 @@ int RadiosityEngine::pixel_pass(const std::vector<uint32_t> &px, bool final, flo
     sp.obj_index = scene->d_obj_index;
     sp.mats = scene->shading.d_mats;
     sp.adc = rs.adc_bailout;
-    k_hit_queries<<<blocks(n), 256, 0, stream>>>(sp, static_cast<const pvx_ray *>(S.rays.p),
+    (scene->view.has_complex >= 3 ? k_hit_queries<true> : k_hit_queries<false>)<<<blocks(n), 256, 0, stream>>>(sp, static_cast<const pvx_ray *>(S.rays.p),
                                                  static_cast<const pvx_hit *>(S.hits.p), n, 1.0f,
                                                  rs.recursion_limit > 0 ? 1 : 0, static_cast<float *>(S.base.p),
                                                  static_cast<float *>(S.coef.p), static_cast<RadQuery *>(S.q.p),
