@@ -262,8 +262,9 @@ __global__ void k_shade(RenderParams p, const pvx_ray *__restrict__ rays, const 
 
     uint32_t nc = 0;
     bool tir = false;
-    // transmitted component (ComputeLightedTexture, trace.cpp:1075-1131)
-    if (m.trans > p.bailout) {
+    // transmitted component (ComputeLightedTexture, trace.cpp:1075-1131);
+    // none when every material is opaque (kPacket)
+    if (!kPacket && m.trans > p.bailout) {
         const float nw = (float)((double)ps.weight * (double)m.fmax);
         // ComputeRefraction (trace.cpp:1323-1452): ior ratio from the interiors
         PathState c = ps;
