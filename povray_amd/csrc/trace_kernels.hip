@@ -48,9 +48,10 @@ __global__ __launch_bounds__(kBlock) void k_trace_closest_ww(TraceArgs a) {
 
 // Light rays of pvx_render made in the kernel (triangle / sphere scenes,
 // every material opaque): occlusion test, while-while walk, a visibility
-// byte per ray.
+// byte per ray.  5 wavefronts per SIMD: held to 80 VGPRs for 6 it was
+// slower (18.7 against 18.3 ms of light passes on the 1.3M-triangle frame).
 template <bool kFlags>
-__global__ __launch_bounds__(kBlock, 6) void k_light_ww(TraceArgs a) {
+__global__ __launch_bounds__(kBlock, 5) void k_light_ww(TraceArgs a) {
     __shared__ int32_t s_stack[kStackLds * kBlock];
     const uint32_t n = a.gen_count ? a.gen_count[0] * a.gen_S : a.n;
     trace_closest_body<0, kFlags, false, true, false, true>(a.sv, nullptr, nullptr, nullptr, nullptr, nullptr,
